@@ -1,0 +1,257 @@
+// Driver for the reference engine's own hot-path components, built from the
+// reference sources in place into oracle/_ref/libsfref.so (see Makefile).
+//
+// TEST INFRASTRUCTURE ONLY: used in the build container to certify the
+// oracle restatement (sfo.cpp) and to write golden vectors.  It never ships
+// to the GPU machine and is never loaded by the product.
+//
+// engine/simulation.cpp cannot be compiled here: it includes the CMake-
+// generated <softflow/build_info.h>.  This driver therefore composes the
+// reference's LBMSolver, ImmersedBoundary, CapsuleSystem, RepulsionForce and
+// AdvectionDiffusion in the order of Simulation::step (simulation.cpp:470-787)
+// and restates Simulation::addCapsuleRandom (simulation.cpp:220-274).
+#include "sfo.h"
+
+#include "core/parameters.h"
+#include "coupling/immersed_boundary.h"
+#include "lbm/advection_diffusion.h"
+#include "lbm/lbm_solver.h"
+#include "membrane/capsule_system.h"
+#include "membrane/repulsion.h"
+
+#include <cstring>
+#include <memory>
+#include <random>
+
+using namespace softflow;
+
+struct sfr_sim {
+    SimulationParams params;
+    std::unique_ptr<LBMSolver> lbm;
+    CapsuleSystem caps;
+    ImmersedBoundary ibm;
+    std::unique_ptr<RepulsionForce> rep;
+    std::unique_ptr<AdvectionDiffusion> adr;
+    int capped = 0;
+};
+
+static SimulationParams to_params(const sfo_params* p) {
+    SimulationParams s;
+    s.nx = p->nx; s.ny = p->ny; s.dt = p->dt; s.ibm_iterations = p->ibm_iterations;
+    s.fluid.tau = p->tau; s.fluid.rho0 = p->rho0;
+    s.fluid.boundary_type = static_cast<BoundaryType>(p->boundary_type);
+    s.fluid.body_force_x = p->body_force_x; s.fluid.body_force_y = p->body_force_y;
+    s.fluid.gravity_x = p->gravity_x; s.fluid.gravity_y = p->gravity_y;
+    s.fluid.apply_gravity_to_fluid = p->apply_gravity_to_fluid != 0;
+    s.fluid.apply_gravity_to_capsules = p->apply_gravity_to_capsules != 0;
+    s.fluid.collision_model = static_cast<CollisionModel>(p->collision_model);
+    s.fluid.use_mrt = p->use_mrt != 0;
+    s.fluid.mrt_se = p->mrt_se; s.fluid.mrt_s_eps = p->mrt_s_eps; s.fluid.mrt_sq = p->mrt_sq;
+    s.fluid.periodic_y = p->periodic_y != 0;
+    s.max_lattice_force = p->max_lattice_force;
+    s.repulsion.epsilon = p->rep_epsilon; s.repulsion.sigma = p->rep_sigma;
+    s.repulsion.r_cut = p->rep_r_cut; s.repulsion.power = p->rep_power;
+    s.repulsion.damping_normal = p->rep_damping_normal;
+    s.repulsion.friction_coeff = p->rep_friction_coeff;
+    s.scalar.enabled = p->scalar_enabled != 0;
+    s.scalar.n_species = p->n_species;
+    s.scalar.diffusivity.assign(p->diffusivity, p->diffusivity + p->n_species);
+    s.scalar.inlet_concentration.assign(p->inlet_concentration,
+                                        p->inlet_concentration + p->n_species);
+    s.scalar.periodic_y = p->scalar_periodic_y != 0;
+    return s;
+}
+
+static MembraneParams to_mem(const sfo_membrane* m) {
+    MembraneParams r;
+    r.model = static_cast<MembraneModel>(m->model);
+    r.k_stretch = m->k_stretch; r.G_s = m->G_s; r.C_skalak = m->C_skalak;
+    r.wlc_L_max_ratio = m->wlc_L_max_ratio; r.wlc_kBT_p = m->wlc_kBT_p; r.wlc_k_pow = m->wlc_k_pow;
+    r.k_bend = m->k_bend; r.use_helfrich_bending = m->use_helfrich != 0; r.kappa_0 = m->kappa_0;
+    r.k_area = m->k_area; r.k_perimeter = m->k_perimeter; r.gamma_visc = m->gamma_visc;
+    r.eta_membrane = m->eta_membrane; r.viscosity_ratio = m->viscosity_ratio;
+    r.density = m->density; r.shape = static_cast<CapsuleShape>(m->shape);
+    r.aspect_ratio = m->aspect_ratio; r.indent_depth = m->indent_depth;
+    r.is_rigid = m->is_rigid != 0;
+    return r;
+}
+
+extern "C" {
+
+sfr_sim* sfr_create(const sfo_params* p) {
+    auto* s = new sfr_sim();
+    s->params = to_params(p);
+    s->lbm = std::make_unique<LBMSolver>(s->params);
+    s->rep = std::make_unique<RepulsionForce>(s->params.repulsion);
+    if (s->params.fluid.boundary_type == BoundaryType::PERIODIC) {
+        s->rep->setPeriodicX(static_cast<Real>(s->params.nx));
+        s->caps.setPeriodicX(s->params.nx);
+    }
+    if (s->params.scalar.enabled)
+        s->adr = std::make_unique<AdvectionDiffusion>(s->params.nx, s->params.ny, s->params.scalar);
+    return s;
+}
+
+void sfr_destroy(sfr_sim* s) { delete s; }
+
+void sfr_add_capsule(sfr_sim* s, double cx, double cy, double radius, int num_nodes,
+                     const sfo_membrane* m, int type) {
+    s->caps.addCapsule(Vec2d{cx, cy}, radius, num_nodes, to_mem(m), type);
+}
+
+int sfr_add_capsule_random(sfr_sim* s, int count, double x0, double y0, double x1, double y1,
+                           double rmin, double rmax, int num_nodes, const sfo_membrane* m,
+                           int type, int seed, double min_gap, int max_attempts) {
+    MembraneParams mp = to_mem(m);
+    std::mt19937 rng(static_cast<unsigned>(seed));
+    std::uniform_real_distribution<Real> dx(x0, x1), dy(y0, y1), dr(rmin, rmax);
+    int placed = 0, attempts = 0;
+    int total = (max_attempts > 0) ? max_attempts : count * 500;
+    const Real wb = 0.0, wt = static_cast<Real>(s->params.ny - 1);   // ChannelBuilder walls
+    while (placed < count && attempts < total) {
+        Real cx = dx(rng);
+        Real cy = dy(rng);
+        Real r = dr(rng);
+        bool overlap = false;
+        for (int c = 0; c < s->caps.numCapsules(); ++c) {
+            Vec2d oc = s->caps[c].centroid();
+            Real orr = s->caps[c].effectiveRadius();
+            if ((Vec2d{cx, cy} - oc).norm() < r + orr + min_gap) { overlap = true; break; }
+        }
+        if (cy - r < wb + min_gap || cy + r > wt - min_gap) overlap = true;
+        if (!overlap) {
+            int nn = (num_nodes > 0) ? num_nodes : Capsule::computeOptimalNodes(r);
+            s->caps.addCapsule({cx, cy}, r, nn, mp, type);
+            placed++;
+        }
+        attempts++;
+    }
+    return placed;
+}
+
+void sfr_initialize(sfr_sim* s) {
+    s->lbm->initialize();
+    if (s->adr) s->adr->initialize(0.0);
+    if (s->params.fluid.boundary_type == BoundaryType::PERIODIC) s->caps.setPeriodicX(s->params.nx);
+}
+
+void sfr_scalar_initialize(sfr_sim* s, double c0) { if (s->adr) s->adr->initialize(c0); }
+void sfr_set_scalar_point(sfr_sim* s, int x, int y, double c, int sp) {
+    if (s->adr) s->adr->setPoint(x, y, c, sp);
+}
+
+void sfr_step(sfr_sim* s, int nsteps) {
+    const SimulationParams& P = s->params;
+    for (int it = 0; it < nsteps; ++it) {
+        LatticeField& field = s->lbm->field();
+        field.clearForces();
+        s->caps.clearAllForces();
+        s->caps.computeAllMembraneForces();
+        s->rep->computeAll(s->caps, 0.5, static_cast<Real>(P.ny) - 1.5);
+        if (P.fluid.apply_gravity_to_capsules &&
+            (P.fluid.gravity_x != 0.0 || P.fluid.gravity_y != 0.0))
+            for (int c = 0; c < s->caps.numCapsules(); ++c)
+                s->caps[c].applyBuoyancyForce(P.fluid.gravity_x, P.fluid.gravity_y, P.fluid.rho0);
+        if (P.ibm_iterations > 1) s->ibm.multiDirectForcing(s->caps, field, P.ibm_iterations);
+        else s->ibm.spreadForces(s->caps, field);
+        const Real bfx = P.fluid.body_force_x, bfy = P.fluid.body_force_y;
+        const Real gx = P.fluid.gravity_x, gy = P.fluid.gravity_y;
+        const bool hb = (bfx != 0.0 || bfy != 0.0);
+        const bool hg = P.fluid.apply_gravity_to_fluid && (gx != 0.0 || gy != 0.0);
+        const int N = field.size();
+        if (hb || hg)
+            for (int n = 0; n < N; ++n)
+                if (field.cellTypeData()[n] == CellType::FLUID) {
+                    if (hb) { field.FxData()[n] += bfx; field.FyData()[n] += bfy; }
+                    if (hg) {
+                        Real rho = field.rhoData()[n];
+                        field.FxData()[n] += rho * gx; field.FyData()[n] += rho * gy;
+                    }
+                }
+        {
+            const Real Fm = P.max_lattice_force, Fm2 = Fm * Fm;
+            int capped = 0;
+            for (int n = 0; n < N; ++n) {
+                Real fx = field.FxData()[n], fy = field.FyData()[n];
+                Real m2 = fx * fx + fy * fy;
+                if (m2 > Fm2) {
+                    Real sc = Fm / std::sqrt(m2);
+                    field.FxData()[n] *= sc; field.FyData()[n] *= sc;
+                    ++capped;
+                }
+            }
+            s->capped = capped;
+        }
+        s->lbm->step();
+        if (s->adr) s->adr->step(field);
+        s->ibm.interpolateVelocity(field, s->caps);
+        s->caps.moveAllNodes(P.dt);
+        if (!P.fluid.periodic_y) {
+            Real ymin = 1.5, ymax = static_cast<Real>(P.ny) - 1.5;
+            for (int c = 0; c < s->caps.numCapsules(); ++c)
+                for (auto& p : s->caps[c].positions()) {
+                    if (p.y < ymin) p.y = ymin;
+                    if (p.y > ymax) p.y = ymax;
+                }
+        }
+    }
+}
+
+int sfr_num_capsules(const sfr_sim* s) { return s->caps.numCapsules(); }
+int sfr_total_nodes(const sfr_sim* s) { return s->caps.totalNodes(); }
+void sfr_capsule_offsets(const sfr_sim* s, int* off) {
+    off[0] = 0;
+    for (int c = 0; c < s->caps.numCapsules(); ++c) off[c + 1] = off[c] + s->caps[c].numNodes();
+}
+void sfr_get_nodes(const sfr_sim* s, double* x, double* y, double* vx, double* vy, double* fx,
+                   double* fy) {
+    int k = 0;
+    for (int c = 0; c < s->caps.numCapsules(); ++c) {
+        const Capsule& cp = s->caps[c];
+        for (int i = 0; i < cp.numNodes(); ++i, ++k) {
+            if (x) x[k] = cp.nodePosition(i).x;
+            if (y) y[k] = cp.nodePosition(i).y;
+            if (vx) vx[k] = cp.nodeVelocity(i).x;
+            if (vy) vy[k] = cp.nodeVelocity(i).y;
+            if (fx) fx[k] = cp.nodeForce(i).x;
+            if (fy) fy[k] = cp.nodeForce(i).y;
+        }
+    }
+}
+void sfr_get_field(const sfr_sim* s, double* rho, double* ux, double* uy, double* fx,
+                   double* fy) {
+    const LatticeField& f = s->lbm->field();
+    size_t N = f.size();
+    if (rho) std::memcpy(rho, f.rhoData(), N * 8);
+    if (ux) std::memcpy(ux, f.uxData(), N * 8);
+    if (uy) std::memcpy(uy, f.uyData(), N * 8);
+    if (fx) std::memcpy(fx, f.FxData(), N * 8);
+    if (fy) std::memcpy(fy, f.FyData(), N * 8);
+}
+void sfr_get_f(const sfr_sim* s, double* out) {
+    const LatticeField& f = s->lbm->field();
+    std::memcpy(out, f.fData(), static_cast<size_t>(f.size()) * 9 * 8);
+}
+void sfr_get_scalar(const sfr_sim* s, double* C, int sp) {
+    if (s->adr) std::memcpy(C, s->adr->concentrationData(sp), static_cast<size_t>(s->params.nx) * s->params.ny * 8);
+}
+int sfr_last_capped(const sfr_sim* s) { return s->capped; }
+double sfr_capsule_geometry(const sfr_sim* s, int c, int what) {
+    const Capsule& cp = s->caps[c];
+    switch (what) {
+    case 0: return cp.area();
+    case 1: return cp.perimeter();
+    case 2: return cp.centroid().x;
+    case 3: return cp.centroid().y;
+    case 4: return cp.deformationIndex();
+    case 7: return cp.inclinationAngle();
+    }
+    return 0.0;
+}
+double sfr_phi(double r) { return PeskinDelta::phi(r); }
+void sfr_membrane_forces_only(sfr_sim* s) {
+    s->caps.clearAllForces();
+    s->caps.computeAllMembraneForces();
+}
+
+}  // extern "C"

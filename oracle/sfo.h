@@ -1,0 +1,107 @@
+/* Oracle C-ABI: the CPU restatement of SoftFlow's per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it may be loaded
+ * by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg,
+ * never by the product path (softflow_amd/).
+ *
+ * The arithmetic follows the reference engine (ramc77/SoftFlow) operation by
+ * operation, in IEEE double with no FMA contraction, single-threaded, so that
+ * it reproduces the reference's bits when both are built with
+ * -O2 -ffp-contract=off (the reference's OpenMP paths at 1 thread).
+ */
+#ifndef SFO_H
+#define SFO_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int    nx, ny;
+    double dt;
+    int    ibm_iterations;
+    /* fluid */
+    double tau, rho0;
+    int    boundary_type;      /* 0 inlet/outlet, 1 periodic, 2 closed */
+    double body_force_x, body_force_y;
+    double gravity_x, gravity_y;
+    int    apply_gravity_to_fluid, apply_gravity_to_capsules;
+    int    collision_model;    /* 0 BGK, 1 MRT, 2 regularized */
+    int    use_mrt;
+    double mrt_se, mrt_s_eps, mrt_sq;
+    int    periodic_y;
+    double max_lattice_force;
+    /* repulsion */
+    double rep_epsilon, rep_sigma, rep_r_cut;
+    int    rep_power;
+    double rep_damping_normal, rep_friction_coeff;
+    /* scalar transport */
+    int    scalar_enabled, n_species;
+    double diffusivity[4];
+    double inlet_concentration[4];
+    int    scalar_periodic_y;
+} sfo_params;
+
+typedef struct {
+    int    model;              /* 0 Hookean, 1 neo-Hookean, 2 Skalak, 3 WLC */
+    double k_stretch, G_s, C_skalak;
+    double wlc_L_max_ratio, wlc_kBT_p, wlc_k_pow;
+    double k_bend;
+    int    use_helfrich;
+    double kappa_0;
+    double k_area, k_perimeter, gamma_visc, eta_membrane;
+    double viscosity_ratio, density;
+    int    shape;              /* 0 circle, 1 ellipse, 2 biconcave, 3 fiber */
+    double aspect_ratio, indent_depth;
+    int    is_rigid;
+} sfo_membrane;
+
+typedef struct sfo_sim sfo_sim;
+
+void   sfo_default_params(sfo_params* p);
+void   sfo_default_membrane(sfo_membrane* m);
+
+sfo_sim* sfo_create(const sfo_params* p);
+void   sfo_destroy(sfo_sim* s);
+void   sfo_add_capsule(sfo_sim* s, double cx, double cy, double radius,
+                       int num_nodes, const sfo_membrane* m, int type);
+int    sfo_add_capsule_random(sfo_sim* s, int count, double x0, double y0,
+                              double x1, double y1, double rmin, double rmax,
+                              int num_nodes, const sfo_membrane* m, int type,
+                              int seed, double min_gap, int max_attempts);
+void   sfo_initialize(sfo_sim* s);
+void   sfo_scalar_initialize(sfo_sim* s, double c0);
+void   sfo_set_scalar_point(sfo_sim* s, int x, int y, double c, int species);
+void   sfo_step(sfo_sim* s, int nsteps);
+
+int    sfo_num_capsules(const sfo_sim* s);
+int    sfo_total_nodes(const sfo_sim* s);
+void   sfo_capsule_offsets(const sfo_sim* s, int* off /* ncaps+1 */);
+void   sfo_get_nodes(const sfo_sim* s, double* x, double* y, double* vx,
+                     double* vy, double* fx, double* fy);
+void   sfo_set_nodes(sfo_sim* s, const double* x, const double* y,
+                     const double* vx, const double* vy);
+void   sfo_get_field(const sfo_sim* s, double* rho, double* ux, double* uy,
+                     double* fx, double* fy);
+void   sfo_get_f(const sfo_sim* s, double* f /* 9*N, q-major */);
+void   sfo_set_f(sfo_sim* s, const double* f);
+void   sfo_set_macro(sfo_sim* s, const double* rho, const double* ux,
+                     const double* uy);
+void   sfo_get_flags(const sfo_sim* s, int* flags);
+void   sfo_get_scalar(const sfo_sim* s, double* C, int species);
+int    sfo_last_capped(const sfo_sim* s);
+double sfo_max_speed(const sfo_sim* s);
+double sfo_capsule_geometry(const sfo_sim* s, int c, int what);
+void   sfo_rest_lengths(const sfo_sim* s, double* out);
+
+/* single-operation probes used by unit tests */
+double sfo_phi(double r);
+double sfo_phi_exact(double r);
+void   sfo_collide_cell(int model, double tau, double se, double seps,
+                        double sq, double* f9, double rho, double ux,
+                        double uy, double Fx, double Fy);
+void   sfo_membrane_forces_only(sfo_sim* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
