@@ -1,0 +1,155 @@
+/* softflow.h -- C-ABI of the MI355X (gfx950) per-timestep engine.
+ *
+ * Every entry point takes plain pointers and sizes, returns 0 on success or a
+ * negative SF_E* code, and leaves a message for sf_last_error().  Host buffers
+ * are borrowed only for the duration of the call; device state is owned by the
+ * context.  Layouts at this boundary are the reference engine's:
+ *   populations  f[q*N + y*nx + x], q in D2Q9 order (lattice.h:22-27), as
+ *                left by LBMSolver::step (post-stream, post-bounce-back);
+ *   fields       rho/ux/uy/Fx/Fy[y*nx + x];
+ *   scalar       g[n*9 + q] (advection_diffusion.h:149) and C[n];
+ *   nodes        capsule-major flat arrays with CSR offsets off[ncaps+1].
+ *
+ * Citations name the reference interface each entry point replaces
+ * (ramc77/SoftFlow, path:line).
+ */
+#ifndef SOFTFLOW_H
+#define SOFTFLOW_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SF_OK = 0,
+    SF_E_ARG = -1,        /* bad argument / unsupported configuration */
+    SF_E_HIP = -2,        /* HIP runtime error */
+    SF_E_STATE = -3,      /* call out of order (e.g. step before set_lattice) */
+    SF_E_CAPACITY = -4    /* a fixed-capacity device list overflowed */
+};
+
+enum { SF_FP64 = 0, SF_FP32 = 1 };            /* lattice storage / collide precision */
+enum { SF_BGK = 0, SF_MRT = 1, SF_REGULARIZED = 2 };
+enum { SF_INLET_OUTLET = 0, SF_PERIODIC = 1, SF_CLOSED = 2 };
+enum { SF_FLUID = 0, SF_SOLID = 1, SF_INLET = 2, SF_OUTLET = 3, SF_EMPTY = 4 };
+
+/* Subset of SimulationParams (core/parameters.h:279-342) that the step reads. */
+typedef struct {
+    int    nx, ny;
+    double dt;
+    int    ibm_iterations;          /* parameters.h:286 */
+    double tau, rho0;
+    int    boundary_type;           /* SF_PERIODIC or SF_CLOSED */
+    double body_force_x, body_force_y;
+    int    collision_model;         /* SF_BGK / SF_MRT / SF_REGULARIZED */
+    double mrt_se, mrt_s_eps, mrt_sq;
+    int    periodic_y;
+    double max_lattice_force;       /* parameters.h:337 */
+    double rep_epsilon, rep_sigma, rep_r_cut;
+    int    rep_power;
+    double rep_damping_normal, rep_friction_coeff;
+    int    scalar_enabled, n_species;
+    double diffusivity[4];
+    double inlet_concentration[4];
+    int    scalar_periodic_y;
+    int    precision;               /* SF_FP64 or SF_FP32 */
+} sf_params;
+
+/* Per-capsule constants resolved on the host (rest geometry and WLC constants
+ * as computed by Capsule::Capsule, membrane/capsule.cpp:7-141). */
+typedef struct {
+    int    model;                   /* 0 Hookean, 1 neo-Hookean, 2 Skalak, 3 WLC */
+    int    use_helfrich;
+    int    is_rigid;
+    int    type;
+    double k_stretch, G_s, C_skalak, k_bend;
+    double k_area, k_perimeter, gamma_visc, eta_membrane;
+    double wlc_L_max_ratio, wlc_kBT_p, wlc_k_pow;
+    double rest_area, rest_perimeter;
+} sf_capsule_desc;
+
+typedef struct sf_ctx sf_ctx;
+
+const char* sf_last_error(void);
+int  sf_device_count(int* n);
+
+/* Context = one Simulation's device state (simulation.h:37, Simulation ctor). */
+int  sf_create(sf_ctx** ctx, int device, const sf_params* p);
+int  sf_destroy(sf_ctx* ctx);
+
+/* Cell types (LatticeField::cellTypeData, lattice_field.h:87).  flags[N]. */
+int  sf_set_flags(sf_ctx* ctx, const int32_t* flags);
+
+/* Populations in the reference layout (LatticeField::fData, lattice_field.h:41). */
+int  sf_set_populations(sf_ctx* ctx, const double* f);
+int  sf_get_populations(sf_ctx* ctx, double* f);
+
+/* Macroscopic arrays as LatticeField holds them after a step
+ * (computeMacroscopic, lattice_field.cpp:28-73) and the force field of the
+ * last step (Fx/Fy after body force and cap, simulation.cpp:541-609).
+ * Any pointer may be NULL. */
+int  sf_get_macroscopic(sf_ctx* ctx, double* rho, double* ux, double* uy,
+                        double* Fx, double* Fy);
+/* Overwrite the stored rho/ux/uy that the next collision reads (the
+ * reference's setRho/setUx/setUy, lattice_field.h:94-96). */
+int  sf_set_macroscopic(sf_ctx* ctx, const double* rho, const double* ux,
+                        const double* uy);
+
+/* Capsules (CapsuleSystem::addCapsule, capsule_system.cpp:5-9): node arrays
+ * of K = off[ncaps] entries; prev_len/has_prev carry the Kelvin-Voigt state
+ * (capsule.h:121-122). rest_angle/rest_ds may be NULL without Helfrich. */
+int  sf_set_capsules(sf_ctx* ctx, int ncaps, const int32_t* off,
+                     const sf_capsule_desc* desc, const double* x,
+                     const double* y, const double* vx, const double* vy,
+                     const double* rest_len, const double* rest_angle,
+                     const double* rest_ds, const double* prev_len,
+                     const int32_t* has_prev);
+int  sf_get_nodes(sf_ctx* ctx, double* x, double* y, double* vx, double* vy,
+                  double* fx, double* fy);
+int  sf_set_nodes(sf_ctx* ctx, const double* x, const double* y,
+                  const double* vx, const double* vy);
+int  sf_get_membrane_state(sf_ctx* ctx, double* prev_len, int32_t* has_prev);
+
+/* Scalar transport (AdvectionDiffusion, advection_diffusion.h:17-99). */
+int  sf_set_scalar(sf_ctx* ctx, int species, const double* g, const double* C);
+int  sf_get_scalar(sf_ctx* ctx, int species, double* C, double* g);
+
+/* Simulation::step (simulation.cpp:470-787) nsteps times, in-scope stages. */
+int  sf_step(sf_ctx* ctx, int nsteps);
+/* Replay the step as a captured HIP graph (1) or launch kernel by kernel (0). */
+int  sf_set_graph_mode(sf_ctx* ctx, int on);
+int  sf_synchronize(sf_ctx* ctx);
+
+/* Stage-level entry points of one step, in simulation.cpp:470-787 order.
+ * SF_STAGE_FORCES  = clearAllForces + computeAllMembraneForces + repulsion
+ * SF_STAGE_IBM     = spreadForces or multiDirectForcing (+ body force, cap)
+ * SF_STAGE_LBM     = LBMSolver::step
+ * SF_STAGE_SCALAR  = AdvectionDiffusion::step
+ * SF_STAGE_MOVE    = interpolateVelocity + moveAllNodes + y-clamp */
+enum { SF_STAGE_FORCES = 0, SF_STAGE_IBM = 1, SF_STAGE_LBM = 2,
+       SF_STAGE_SCALAR = 3, SF_STAGE_MOVE = 4 };
+int  sf_stage(sf_ctx* ctx, int stage);
+
+/* last_capped_nodes_ (simulation.cpp:588-609). */
+int  sf_last_capped(sf_ctx* ctx, int32_t* capped);
+/* FP64 device reductions over FLUID cells of the current fields:
+ * out[0] = sum rho, out[1] = sum rho*ux, out[2] = sum rho*uy,
+ * out[3] = max |u| (getMaxSpeed, simulation.cpp:892-907),
+ * out[4] = number of non-finite cells (checkStability, :820-890). */
+int  sf_field_stats(sf_ctx* ctx, double* out5);
+int  sf_step_count(sf_ctx* ctx, int64_t* n);
+
+/* Per-kernel timing of the last sf_step call, measured with HIP events on the
+ * context's stream: names[i] (static strings) and mean ms per launch. */
+int  sf_kernel_times(sf_ctx* ctx, int max, const char** names, double* ms,
+                     int* count);
+int  sf_set_timing(sf_ctx* ctx, int on);
+
+/* Device math probe used by the tests: op 0 sqrt, 1 divide (a/b), 2 atan2(a,b),
+ * 3 pow(a, b) with integer b as used by the contact force. */
+int  sf_probe_math(int op, int n, const double* a, const double* b, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -19,6 +19,8 @@
 #include "membrane/capsule_system.h"
 #include "membrane/repulsion.h"
 
+#include <omp.h>
+
 #include <cstring>
 #include <memory>
 #include <random>
@@ -79,6 +81,7 @@ static MembraneParams to_mem(const sfo_membrane* m) {
 extern "C" {
 
 sfr_sim* sfr_create(const sfo_params* p) {
+    omp_set_num_threads(1);   // the oracle's summation order is the one-thread order
     auto* s = new sfr_sim();
     s->params = to_params(p);
     s->lbm = std::make_unique<LBMSolver>(s->params);

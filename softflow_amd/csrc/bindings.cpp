@@ -1,0 +1,347 @@
+// pybind11 module softflow_core: the reference's Python surface
+// (python/bindings.cpp) over the MI355X engine.
+//
+// Field views are zero-copy numpy arrays over the host mirror, refreshed from
+// the device when the view is requested; they are read-only because a write
+// into them would not reach the device (use the setters).
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "engine.h"
+
+namespace py = pybind11;
+using namespace softflow;
+
+namespace {
+
+py::array view2d(Real* data, int ny, int nx, py::handle owner) {
+    py::array a(py::dtype::of<Real>(), {ny, nx},
+                {static_cast<py::ssize_t>(nx) * static_cast<py::ssize_t>(sizeof(Real)),
+                 static_cast<py::ssize_t>(sizeof(Real))},
+                data, owner);
+    py::detail::array_proxy(a.ptr())->flags &= ~py::detail::npy_api::NPY_ARRAY_WRITEABLE_;
+    return a;
+}
+
+// LBMSolver handle exposing field() as the reference does (bindings.cpp:456-475)
+struct LBMSolverView {
+    Simulation* sim;
+    LatticeField& field() { return sim->field(); }
+    int getNx() const { return sim->params().nx; }
+    int getNy() const { return sim->params().ny; }
+    int getStep() const { return sim->currentStep(); }
+    bool isPeriodicX() const { return sim->params().fluid.boundary_type == BoundaryType::PERIODIC; }
+    bool isPeriodicY() const { return sim->params().fluid.periodic_y; }
+};
+
+}  // namespace
+
+PYBIND11_MODULE(softflow_core, m) {
+    m.doc() = "SoftFlow per-timestep engine on MI355X (gfx950) HIP kernels";
+
+    py::class_<Vec2d>(m, "Vec2d")
+        .def(py::init<>())
+        .def(py::init<Real, Real>(), py::arg("x"), py::arg("y"))
+        .def_readwrite("x", &Vec2d::x)
+        .def_readwrite("y", &Vec2d::y)
+        .def("norm", &Vec2d::norm)
+        .def("__repr__", [](const Vec2d& v) {
+            return "Vec2d(" + std::to_string(v.x) + ", " + std::to_string(v.y) + ")";
+        });
+
+    py::enum_<CellType>(m, "CellType")
+        .value("FLUID", CellType::FLUID).value("SOLID", CellType::SOLID)
+        .value("INLET", CellType::INLET).value("OUTLET", CellType::OUTLET)
+        .value("EMPTY", CellType::EMPTY);
+    py::enum_<BoundaryType>(m, "BoundaryType")
+        .value("INLET_OUTLET", BoundaryType::INLET_OUTLET)
+        .value("PERIODIC", BoundaryType::PERIODIC)
+        .value("CLOSED", BoundaryType::CLOSED);
+    py::enum_<CollisionModel>(m, "CollisionModel")
+        .value("BGK", CollisionModel::BGK).value("MRT", CollisionModel::MRT)
+        .value("REGULARIZED", CollisionModel::REGULARIZED);
+    py::enum_<MembraneModel>(m, "MembraneModel")
+        .value("HOOKEAN", MembraneModel::HOOKEAN).value("NEO_HOOKEAN", MembraneModel::NEO_HOOKEAN)
+        .value("SKALAK", MembraneModel::SKALAK).value("WLC", MembraneModel::WLC);
+    py::enum_<CapsuleShape>(m, "CapsuleShape")
+        .value("CIRCLE", CapsuleShape::CIRCLE).value("ELLIPSE", CapsuleShape::ELLIPSE)
+        .value("BICONCAVE", CapsuleShape::BICONCAVE).value("FIBER", CapsuleShape::FIBER);
+    py::enum_<Precision>(m, "Precision")
+        .value("FP64", Precision::FP64).value("FP32", Precision::FP32);
+
+    py::class_<FluidParams>(m, "FluidParams")
+        .def(py::init<>())
+        .def_readwrite("tau", &FluidParams::tau)
+        .def_readwrite("rho0", &FluidParams::rho0)
+        .def_readwrite("inlet_velocity", &FluidParams::inlet_velocity)
+        .def_readwrite("outlet_density", &FluidParams::outlet_density)
+        .def_readwrite("boundary_type", &FluidParams::boundary_type)
+        .def_readwrite("body_force_x", &FluidParams::body_force_x)
+        .def_readwrite("body_force_y", &FluidParams::body_force_y)
+        .def_readwrite("gravity_x", &FluidParams::gravity_x)
+        .def_readwrite("gravity_y", &FluidParams::gravity_y)
+        .def_readwrite("apply_gravity_to_fluid", &FluidParams::apply_gravity_to_fluid)
+        .def_readwrite("apply_gravity_to_capsules", &FluidParams::apply_gravity_to_capsules)
+        .def_readwrite("collision_model", &FluidParams::collision_model)
+        .def_readwrite("use_mrt", &FluidParams::use_mrt)
+        .def_readwrite("mrt_se", &FluidParams::mrt_se)
+        .def_readwrite("mrt_s_eps", &FluidParams::mrt_s_eps)
+        .def_readwrite("mrt_sq", &FluidParams::mrt_sq)
+        .def_readwrite("periodic_y", &FluidParams::periodic_y)
+        .def_readwrite("capsule_periodic_x", &FluidParams::capsule_periodic_x)
+        .def_readwrite("top_wall_velocity", &FluidParams::top_wall_velocity)
+        .def_readwrite("bottom_wall_velocity", &FluidParams::bottom_wall_velocity)
+        .def_readwrite("use_interpolated_bb", &FluidParams::use_interpolated_bb)
+        .def_readwrite("viscosity_contrast", &FluidParams::viscosity_contrast)
+        .def_readwrite("viscosity_update_interval", &FluidParams::viscosity_update_interval);
+
+    py::class_<MembraneParams>(m, "MembraneParams")
+        .def(py::init<>())
+        .def_readwrite("model", &MembraneParams::model)
+        .def_readwrite("k_stretch", &MembraneParams::k_stretch)
+        .def_readwrite("G_s", &MembraneParams::G_s)
+        .def_readwrite("C_skalak", &MembraneParams::C_skalak)
+        .def_readwrite("wlc_L_max_ratio", &MembraneParams::wlc_L_max_ratio)
+        .def_readwrite("wlc_kBT_p", &MembraneParams::wlc_kBT_p)
+        .def_readwrite("wlc_k_pow", &MembraneParams::wlc_k_pow)
+        .def_readwrite("k_bend", &MembraneParams::k_bend)
+        .def_readwrite("use_helfrich_bending", &MembraneParams::use_helfrich_bending)
+        .def_readwrite("kappa_0", &MembraneParams::kappa_0)
+        .def_readwrite("k_area", &MembraneParams::k_area)
+        .def_readwrite("k_perimeter", &MembraneParams::k_perimeter)
+        .def_readwrite("gamma_visc", &MembraneParams::gamma_visc)
+        .def_readwrite("eta_membrane", &MembraneParams::eta_membrane)
+        .def_readwrite("viscosity_ratio", &MembraneParams::viscosity_ratio)
+        .def_readwrite("density", &MembraneParams::density)
+        .def_readwrite("shape", &MembraneParams::shape)
+        .def_readwrite("aspect_ratio", &MembraneParams::aspect_ratio)
+        .def_readwrite("indent_depth", &MembraneParams::indent_depth)
+        .def_readwrite("is_rigid", &MembraneParams::is_rigid);
+
+    py::class_<RepulsionParams>(m, "RepulsionParams")
+        .def(py::init<>())
+        .def_readwrite("epsilon", &RepulsionParams::epsilon)
+        .def_readwrite("sigma", &RepulsionParams::sigma)
+        .def_readwrite("r_cut", &RepulsionParams::r_cut)
+        .def_readwrite("power", &RepulsionParams::power)
+        .def_readwrite("damping_normal", &RepulsionParams::damping_normal)
+        .def_readwrite("friction_coeff", &RepulsionParams::friction_coeff);
+
+    py::class_<LubricationParams>(m, "LubricationParams")
+        .def(py::init<>())
+        .def_readwrite("enabled", &LubricationParams::enabled)
+        .def_readwrite("h_threshold", &LubricationParams::h_threshold)
+        .def_readwrite("h_min", &LubricationParams::h_min);
+    py::class_<AdhesionParams>(m, "AdhesionParams")
+        .def(py::init<>())
+        .def_readwrite("enabled", &AdhesionParams::enabled);
+
+    py::class_<ScalarParams>(m, "ScalarParams")
+        .def(py::init<>())
+        .def_readwrite("enabled", &ScalarParams::enabled)
+        .def_readwrite("n_species", &ScalarParams::n_species)
+        .def_readwrite("diffusivity", &ScalarParams::diffusivity)
+        .def_readwrite("inlet_concentration", &ScalarParams::inlet_concentration)
+        .def_readwrite("periodic_y", &ScalarParams::periodic_y);
+
+    py::class_<SimulationParams>(m, "SimulationParams")
+        .def(py::init<>())
+        .def_readwrite("nx", &SimulationParams::nx)
+        .def_readwrite("ny", &SimulationParams::ny)
+        .def_readwrite("dt", &SimulationParams::dt)
+        .def_readwrite("num_steps", &SimulationParams::num_steps)
+        .def_readwrite("output_interval", &SimulationParams::output_interval)
+        .def_readwrite("ibm_iterations", &SimulationParams::ibm_iterations)
+        .def_readwrite("fluid", &SimulationParams::fluid)
+        .def_readwrite("membrane", &SimulationParams::membrane)
+        .def_readwrite("repulsion", &SimulationParams::repulsion)
+        .def_readwrite("lubrication", &SimulationParams::lubrication)
+        .def_readwrite("adhesion", &SimulationParams::adhesion)
+        .def_readwrite("scalar", &SimulationParams::scalar)
+        .def_readwrite("output_dir", &SimulationParams::output_dir)
+        .def_readwrite("checkpoint_interval", &SimulationParams::checkpoint_interval)
+        .def_readwrite("metrics_interval", &SimulationParams::metrics_interval)
+        .def_readwrite("vtk_dump_every", &SimulationParams::vtk_dump_every)
+        .def_readwrite("csv_dump_every", &SimulationParams::csv_dump_every)
+        .def_readwrite("probe_dump_every", &SimulationParams::probe_dump_every)
+        .def_readwrite("stats_dump_every", &SimulationParams::stats_dump_every)
+        .def_readwrite("enable_stability_checks", &SimulationParams::enable_stability_checks)
+        .def_readwrite("max_density_deviation", &SimulationParams::max_density_deviation)
+        .def_readwrite("max_velocity", &SimulationParams::max_velocity)
+        .def_readwrite("stability_check_interval", &SimulationParams::stability_check_interval)
+        .def_readwrite("enable_profiling", &SimulationParams::enable_profiling)
+        .def_readwrite("rng_seed", &SimulationParams::rng_seed)
+        .def_readwrite("max_lattice_force", &SimulationParams::max_lattice_force)
+        .def_readwrite("precision", &SimulationParams::precision)
+        .def_readwrite("use_graph", &SimulationParams::use_graph)
+        .def_readwrite("device", &SimulationParams::device)
+        .def("kinematicViscosity", &SimulationParams::kinematicViscosity);
+
+    py::class_<LatticeField>(m, "LatticeField")
+        .def("getNx", &LatticeField::getNx)
+        .def("getNy", &LatticeField::getNy)
+        .def("size", &LatticeField::size)
+        .def("density", [](py::object self) {
+            auto& f = self.cast<LatticeField&>();
+            return view2d(f.rhoData(), f.getNy(), f.getNx(), self);
+        })
+        .def("velocity_x", [](py::object self) {
+            auto& f = self.cast<LatticeField&>();
+            return view2d(f.uxData(), f.getNy(), f.getNx(), self);
+        })
+        .def("velocity_y", [](py::object self) {
+            auto& f = self.cast<LatticeField&>();
+            return view2d(f.uyData(), f.getNy(), f.getNx(), self);
+        })
+        .def("force_x", [](py::object self) {
+            auto& f = self.cast<LatticeField&>();
+            return view2d(f.FxData(), f.getNy(), f.getNx(), self);
+        })
+        .def("force_y", [](py::object self) {
+            auto& f = self.cast<LatticeField&>();
+            return view2d(f.FyData(), f.getNy(), f.getNx(), self);
+        })
+        .def("distributions", [](py::object self) {
+            auto& f = self.cast<LatticeField&>();
+            const py::ssize_t N = f.size(), nx = f.getNx(), s = sizeof(Real);
+            py::array a(py::dtype::of<Real>(), {static_cast<py::ssize_t>(9), static_cast<py::ssize_t>(f.getNy()), nx},
+                        {N * s, nx * s, s}, f.fData(), self);
+            py::detail::array_proxy(a.ptr())->flags &= ~py::detail::npy_api::NPY_ARRAY_WRITEABLE_;
+            return a;
+        })
+        .def("cell_types", [](LatticeField& f) {
+            py::array_t<int> a({f.getNy(), f.getNx()});
+            std::memcpy(a.mutable_data(), f.cellTypeData(), sizeof(int) * f.size());
+            return a;
+        })
+        .def("getRho", &LatticeField::getRho, py::arg("x"), py::arg("y"))
+        .def("getUx", &LatticeField::getUx, py::arg("x"), py::arg("y"))
+        .def("getUy", &LatticeField::getUy, py::arg("x"), py::arg("y"))
+        .def("getCellType", &LatticeField::getCellType, py::arg("x"), py::arg("y"))
+        .def("setCellType", &LatticeField::setCellType, py::arg("x"), py::arg("y"), py::arg("ct"))
+        .def("setRho", &LatticeField::setRho, py::arg("x"), py::arg("y"), py::arg("rho"))
+        .def("setUx", &LatticeField::setUx, py::arg("x"), py::arg("y"), py::arg("ux"))
+        .def("setUy", &LatticeField::setUy, py::arg("x"), py::arg("y"), py::arg("uy"))
+        .def("setEquilibrium", &LatticeField::setEquilibrium, py::arg("rho0"), py::arg("ux0"), py::arg("uy0"))
+        .def("initializeEquilibriumAt", &LatticeField::initializeEquilibriumAt, py::arg("x"), py::arg("y"))
+        .def("initializeAllEquilibrium", &LatticeField::initializeAllEquilibrium)
+        .def("computeMacroscopic", &LatticeField::computeMacroscopic)
+        .def("f", &LatticeField::f, py::arg("x"), py::arg("y"), py::arg("q"))
+        .def("set_f", &LatticeField::set_f, py::arg("x"), py::arg("y"), py::arg("q"), py::arg("value"));
+
+    py::class_<Capsule>(m, "Capsule")
+        .def("nodePosition", &Capsule::nodePosition, py::arg("k"))
+        .def("nodeVelocity", &Capsule::nodeVelocity, py::arg("k"))
+        .def("nodeForce", &Capsule::nodeForce, py::arg("k"))
+        .def("setNodeVelocity", &Capsule::setNodeVelocity, py::arg("k"), py::arg("v"))
+        .def("centroid", &Capsule::centroid)
+        .def("area", &Capsule::area)
+        .def("perimeter", &Capsule::perimeter)
+        .def("deformationIndex", &Capsule::deformationIndex)
+        .def("inclinationAngle", &Capsule::inclinationAngle)
+        .def("effectiveRadius", &Capsule::effectiveRadius)
+        .def("numNodes", &Capsule::numNodes)
+        .def("getId", &Capsule::getId)
+        .def("getType", &Capsule::getType)
+        .def("getTypeId", &Capsule::getType)
+        .def("setPeriodicX", &Capsule::setPeriodicX, py::arg("nx"))
+        .def("getPeriodicX", &Capsule::getPeriodicX)
+        .def("node_positions_array", [](const Capsule& c) {
+            const int n = c.numNodes();
+            py::array_t<Real> a({n, 2});
+            auto b = a.mutable_unchecked<2>();
+            for (int i = 0; i < n; ++i) { Vec2d p = c.nodePosition(i); b(i, 0) = p.x; b(i, 1) = p.y; }
+            return a;
+        });
+
+    py::class_<CapsuleSystem>(m, "CapsuleSystem")
+        .def("addCapsule", &CapsuleSystem::addCapsule, py::arg("center"), py::arg("radius"),
+             py::arg("num_nodes") = 0, py::arg("params"), py::arg("type") = 0)
+        .def("numCapsules", &CapsuleSystem::numCapsules)
+        .def("totalNodes", &CapsuleSystem::totalNodes)
+        .def("__getitem__", [](CapsuleSystem& cs, int i) -> Capsule& {
+            if (i < 0 || i >= cs.numCapsules()) throw py::index_error("Capsule index out of range");
+            return cs[i];
+        }, py::return_value_policy::reference_internal)
+        .def("__len__", &CapsuleSystem::numCapsules);
+
+    py::class_<AdvectionDiffusion>(m, "AdvectionDiffusion")
+        .def("initialize", &AdvectionDiffusion::initialize, py::arg("initial_concentration") = 0.0)
+        .def("setRegion", &AdvectionDiffusion::setRegion, py::arg("x0"), py::arg("y0"), py::arg("x1"),
+             py::arg("y1"), py::arg("concentration"), py::arg("species") = 0)
+        .def("setPoint", &AdvectionDiffusion::setPoint, py::arg("x"), py::arg("y"),
+             py::arg("concentration"), py::arg("species") = 0)
+        .def("getConcentration", &AdvectionDiffusion::getConcentration, py::arg("x"), py::arg("y"),
+             py::arg("species") = 0)
+        .def("concentration", [](py::object self, int species) {
+            auto& a = self.cast<AdvectionDiffusion&>();
+            Real* d = a.concentrationData(species);
+            if (!d) throw py::index_error("no such species");
+            return view2d(d, a.getNy(), a.getNx(), self);
+        }, py::arg("species") = 0)
+        .def("getNumSpecies", &AdvectionDiffusion::getNumSpecies);
+
+    py::class_<ChannelBuilder>(m, "ChannelBuilder")
+        .def(py::init<int, int>(), py::arg("nx"), py::arg("ny"))
+        .def("addWalls", &ChannelBuilder::addWalls, py::return_value_policy::reference_internal)
+        .def("setBoundaryType", &ChannelBuilder::setBoundaryType, py::arg("bt"),
+             py::return_value_policy::reference_internal)
+        .def("wallBottom", &ChannelBuilder::wallBottom)
+        .def("wallTop", &ChannelBuilder::wallTop);
+
+    py::class_<LBMSolverView>(m, "LBMSolver")
+        .def("field", &LBMSolverView::field, py::return_value_policy::reference_internal)
+        .def("getNx", &LBMSolverView::getNx)
+        .def("getNy", &LBMSolverView::getNy)
+        .def("getStep", &LBMSolverView::getStep)
+        .def("isPeriodicX", &LBMSolverView::isPeriodicX)
+        .def("isPeriodicY", &LBMSolverView::isPeriodicY);
+
+    py::class_<Simulation>(m, "Simulation")
+        .def(py::init<const SimulationParams&>(), py::arg("params"))
+        .def("setChannelBuilder", &Simulation::setChannelBuilder, py::arg("builder"))
+        .def("addCapsule", &Simulation::addCapsule, py::arg("center"), py::arg("radius"),
+             py::arg("num_nodes") = 0, py::arg("mparams"), py::arg("type") = 0)
+        .def("addCapsuleRandom", &Simulation::addCapsuleRandom, py::arg("count"), py::arg("x0"),
+             py::arg("y0"), py::arg("x1"), py::arg("y1"), py::arg("radius_min"), py::arg("radius_max"),
+             py::arg("num_nodes") = 0, py::arg("mparams"), py::arg("type") = 0,
+             py::arg("seed") = 12345, py::arg("min_gap") = 1.0, py::arg("max_attempts") = 0)
+        .def("initialize", &Simulation::initialize)
+        .def("step", &Simulation::step)
+        .def("finalize", &Simulation::finalize)
+        .def("run", &Simulation::run, py::arg("num_steps"), py::call_guard<py::gil_scoped_release>())
+        .def("currentStep", &Simulation::currentStep)
+        .def("getMaxSpeed", &Simulation::getMaxSpeed)
+        .def("lastCappedNodes", &Simulation::lastCappedNodes)
+        .def("checkStability", &Simulation::checkStability)
+        .def("fieldStats", &Simulation::fieldStats)
+        .def("lbmSolver", [](Simulation& s) { return LBMSolverView{&s}; }, py::keep_alive<0, 1>())
+        .def("field", [](Simulation& s) -> LatticeField& { return s.field(); },
+             py::return_value_policy::reference_internal)
+        .def("capsules", [](Simulation& s) -> CapsuleSystem& { return s.capsules(); },
+             py::return_value_policy::reference_internal)
+        .def("params", [](const Simulation& s) -> const SimulationParams& { return s.params(); },
+             py::return_value_policy::reference_internal)
+        .def("advectionDiffusion", [](Simulation& s) { return s.advectionDiffusion(); },
+             py::return_value_policy::reference_internal)
+        .def("setFluidRegion", &Simulation::setFluidRegion, py::arg("x0"), py::arg("y0"), py::arg("x1"),
+             py::arg("y1"), py::arg("rho"), py::arg("ux"), py::arg("uy"))
+        .def("setScalarRegion", &Simulation::setScalarRegion, py::arg("x0"), py::arg("y0"),
+             py::arg("x1"), py::arg("y1"), py::arg("concentration"), py::arg("species") = 0)
+        .def("setStepCallback", [](Simulation& sim, py::function cb) {
+            sim.setStepCallback([cb](Simulation& s, int step) {
+                py::gil_scoped_acquire acquire;
+                cb(py::cast(s, py::return_value_policy::reference), step);
+            });
+        }, py::arg("callback"))
+        .def("setTiming", &Simulation::setTiming, py::arg("on"))
+        .def("kernelTimes", &Simulation::kernelTimes);
+
+    m.def("device_count", []() {
+        int n = 0;
+        if (sf_device_count(&n) != SF_OK) return 0;
+        return n;
+    });
+}

@@ -1,0 +1,674 @@
+// Host engine: reference stepping API over libsoftflow_hip's C-ABI.
+#include "engine.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <random>
+
+namespace softflow {
+
+namespace {
+constexpr int Q = 9;
+constexpr Real W[Q] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
+                       1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+constexpr int CX[Q] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int CY[Q] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+constexpr Real CS2 = 1.0 / 3.0;
+constexpr Real CS4 = CS2 * CS2;
+
+// lbm/lattice.h:35-39
+inline Real feq(int q, Real rho, Real ux, Real uy) {
+    Real eu = static_cast<Real>(CX[q]) * ux + static_cast<Real>(CY[q]) * uy;
+    Real uu = ux * ux + uy * uy;
+    return W[q] * rho * (1.0 + eu / CS2 + (eu * eu) / (2.0 * CS4) - uu / (2.0 * CS2));
+}
+}  // namespace
+
+// ============================================================================
+LatticeField::LatticeField(int nx, int ny)
+    : nx_(nx), ny_(ny), N_(nx * ny),
+      f_(static_cast<size_t>(nx) * ny * Q, 0.0), rho_(static_cast<size_t>(nx) * ny, 0.0),
+      ux_(rho_.size(), 0.0), uy_(rho_.size(), 0.0), Fx_(rho_.size(), 0.0), Fy_(rho_.size(), 0.0),
+      flags_(rho_.size(), static_cast<int>(CellType::FLUID)) {}
+
+void LatticeField::pull() {
+    if (stale_ && sim_) sim_->pullField(*this);
+}
+void LatticeField::set_f(int x, int y, int q, Real v) {
+    touch_f();
+    f_[static_cast<size_t>(q) * N_ + idx(x, y)] = v;
+}
+void LatticeField::setCellType(int x, int y, CellType ct) {
+    flags_[idx(x, y)] = static_cast<int>(ct);
+    flags_dirty_ = true;
+}
+void LatticeField::setRho(int x, int y, Real v) { touch_macro(); rho_[idx(x, y)] = v; }
+void LatticeField::setUx(int x, int y, Real v) { touch_macro(); ux_[idx(x, y)] = v; }
+void LatticeField::setUy(int x, int y, Real v) { touch_macro(); uy_[idx(x, y)] = v; }
+
+// lattice_field.cpp:75-90
+void LatticeField::setEquilibrium(Real rho0, Real ux0, Real uy0) {
+    touch_f();
+    touch_macro();
+    for (int n = 0; n < N_; ++n) {
+        rho_[n] = rho0; ux_[n] = ux0; uy_[n] = uy0;
+        for (int q = 0; q < Q; ++q) f_[static_cast<size_t>(q) * N_ + n] = feq(q, rho0, ux0, uy0);
+    }
+}
+// lattice_field.cpp:92-113
+void LatticeField::initializeEquilibriumAt(int x, int y) {
+    touch_f();
+    const int n = idx(x, y);
+    for (int q = 0; q < Q; ++q) f_[static_cast<size_t>(q) * N_ + n] = feq(q, rho_[n], ux_[n], uy_[n]);
+}
+void LatticeField::initializeAllEquilibrium() {
+    touch_f();
+    for (int n = 0; n < N_; ++n)
+        for (int q = 0; q < Q; ++q) f_[static_cast<size_t>(q) * N_ + n] = feq(q, rho_[n], ux_[n], uy_[n]);
+}
+// lattice_field.cpp:28-73, on the host mirror
+void LatticeField::computeMacroscopic() {
+    touch_macro();
+    for (int n = 0; n < N_; ++n) {
+        if (flags_[n] == static_cast<int>(CellType::SOLID) || flags_[n] == static_cast<int>(CellType::EMPTY)) {
+            rho_[n] = ux_[n] = uy_[n] = 0.0;
+            continue;
+        }
+        Real r = 0.0, vx = 0.0, vy = 0.0;
+        for (int q = 0; q < Q; ++q) {
+            Real fi = f_[static_cast<size_t>(q) * N_ + n];
+            r += fi;
+            vx += fi * static_cast<Real>(CX[q]);
+            vy += fi * static_cast<Real>(CY[q]);
+        }
+        rho_[n] = r;
+        if (r > 1e-15) { ux_[n] = (vx + 0.5 * Fx_[n]) / r; uy_[n] = (vy + 0.5 * Fy_[n]) / r; }
+        else { ux_[n] = 0.0; uy_[n] = 0.0; }
+    }
+}
+
+// ============================================================================
+// Capsule rest geometry: membrane/capsule.cpp:7-141
+Capsule::Capsule(int id, Vec2d center, Real radius, int num_nodes, const MembraneParams& p, int type)
+    : id_(id), type_(type), n_(num_nodes > 0 ? num_nodes : computeOptimalNodes(radius)), params_(p) {
+    pos_.resize(n_);
+    vel_.assign(n_, Vec2d{0.0, 0.0});
+    frc_.assign(n_, Vec2d{0.0, 0.0});
+    rest_len_.resize(n_);
+    auto shape_pt = [&](Real th) -> Vec2d {
+        if (p.shape == CapsuleShape::ELLIPSE || p.shape == CapsuleShape::FIBER)
+            return {radius * std::cos(th), radius * p.aspect_ratio * std::sin(th)};
+        if (p.shape == CapsuleShape::BICONCAVE) {
+            Real r = radius * (1.0 - p.indent_depth * std::sin(th) * std::sin(th));
+            return {r * std::cos(th), r * std::sin(th)};
+        }
+        return {radius * std::cos(th), radius * std::sin(th)};
+    };
+    if (p.shape == CapsuleShape::CIRCLE) {
+        for (int i = 0; i < n_; ++i) {
+            Vec2d r = shape_pt(2.0 * PI * i / n_);
+            pos_[i] = Vec2d{center.x + r.x, center.y + r.y};
+        }
+    } else {
+        const int M = 256 * n_;
+        std::vector<Vec2d> pts(M + 1);
+        std::vector<Real> cum(M + 1, 0.0);
+        for (int m = 0; m <= M; ++m) {
+            pts[m] = shape_pt(2.0 * PI * m / M);
+            if (m > 0) cum[m] = cum[m - 1] + (pts[m] - pts[m - 1]).norm();
+        }
+        const Real total = cum[M];
+        int m = 0;
+        for (int i = 0; i < n_; ++i) {
+            const Real target = total * i / n_;
+            while (m < M - 1 && cum[m + 1] < target) ++m;
+            const Real seg = cum[m + 1] - cum[m];
+            const Real w = (seg > 1e-15) ? (target - cum[m]) / seg : 0.0;
+            Vec2d r = pts[m] + (pts[m + 1] - pts[m]) * w;
+            pos_[i] = Vec2d{center.x + r.x, center.y + r.y};
+        }
+    }
+    for (int i = 0; i < n_; ++i) rest_len_[i] = (pos_[(i + 1) % n_] - pos_[i]).norm();
+    prev_len_ = rest_len_;
+    has_prev_ = false;
+    rest_area_ = area();
+    rest_perimeter_ = perimeter();
+    wlc_kBT_p_ = p.wlc_kBT_p;
+    wlc_k_pow_ = p.wlc_k_pow;
+    if (p.model == MembraneModel::WLC && n_ > 0) {
+        Real L0 = rest_len_[0];
+        Real Lmax = p.wlc_L_max_ratio * L0;
+        Real x0 = L0 / Lmax;
+        if (wlc_kBT_p_ <= 0.0) {
+            Real denom = 1.0 / (2.0 * (1.0 - x0) * (1.0 - x0) * (1.0 - x0)) + 1.0;
+            wlc_kBT_p_ = p.G_s * Lmax / (L0 * denom);
+        }
+        if (wlc_k_pow_ <= 0.0) {
+            Real Fr = wlc_kBT_p_ * (1.0 / (4.0 * (1.0 - x0) * (1.0 - x0)) - 0.25 + x0);
+            wlc_k_pow_ = Fr * L0;
+        }
+    }
+    if (p.use_helfrich_bending) {
+        rest_angle_.resize(n_);
+        rest_ds_.resize(n_);
+        for (int i = 0; i < n_; ++i) {
+            Vec2d ep = minImageDiff(pos_[i], pos_[(i - 1 + n_) % n_]);
+            Vec2d en = minImageDiff(pos_[(i + 1) % n_], pos_[i]);
+            rest_ds_[i] = 0.5 * (ep.norm() + en.norm());
+            rest_angle_[i] = (p.kappa_0 != 0.0) ? p.kappa_0 * rest_ds_[i] : std::atan2(ep.cross(en), ep.dot(en));
+        }
+    }
+}
+
+Vec2d Capsule::minImageDiff(const Vec2d& a, const Vec2d& b) const {
+    Vec2d d = a - b;
+    if (periodic_nx_ > 0) {
+        Real Lx = static_cast<Real>(periodic_nx_);
+        if (d.x > 0.5 * Lx) d.x -= Lx;
+        if (d.x < -0.5 * Lx) d.x += Lx;
+    }
+    return d;
+}
+
+void Capsule::setNodeVelocity(int k, const Vec2d& v) {
+    vel_.at(k) = v;
+    if (sys_) sys_->dirty_ = true;
+}
+
+Vec2d Capsule::centroid() const {                    // capsule.cpp:159-173
+    if (periodic_nx_ > 0 && n_ > 0) {
+        Vec2d ref = pos_[0], s = ref;
+        for (int i = 1; i < n_; ++i) s = s + (ref + minImageDiff(pos_[i], ref));
+        return s / static_cast<Real>(n_);
+    }
+    Vec2d c{0.0, 0.0};
+    for (int i = 0; i < n_; ++i) c = c + pos_[i];
+    return c / static_cast<Real>(n_);
+}
+
+Real Capsule::area() const {                         // capsule.cpp:175-195
+    Real A = 0.0;
+    if (periodic_nx_ > 0 && n_ > 0) {
+        std::vector<Vec2d> u(n_);
+        u[0] = pos_[0];
+        for (int i = 1; i < n_; ++i) u[i] = u[i - 1] + minImageDiff(pos_[i], pos_[i - 1]);
+        for (int i = 0; i < n_; ++i) A += u[i].cross(u[(i + 1) % n_]);
+        return 0.5 * std::abs(A);
+    }
+    for (int i = 0; i < n_; ++i) A += pos_[i].cross(pos_[(i + 1) % n_]);
+    return 0.5 * std::abs(A);
+}
+
+Real Capsule::perimeter() const {                    // capsule.cpp:197-204
+    Real P = 0.0;
+    for (int i = 0; i < n_; ++i) P += minImageDiff(pos_[(i + 1) % n_], pos_[i]).norm();
+    return P;
+}
+
+void Capsule::secondMoments(Real& mxx, Real& myy, Real& mxy) const {   // capsule.cpp:210-246
+    std::vector<Vec2d> p(n_);
+    p[0] = pos_[0];
+    for (int i = 1; i < n_; ++i) p[i] = p[i - 1] + minImageDiff(pos_[i], pos_[i - 1]);
+    Real A = 0.0, cx = 0.0, cy = 0.0;
+    for (int i = 0; i < n_; ++i) {
+        const Vec2d& a = p[i];
+        const Vec2d& b = p[(i + 1) % n_];
+        Real cr = a.x * b.y - b.x * a.y;
+        A += cr; cx += (a.x + b.x) * cr; cy += (a.y + b.y) * cr;
+    }
+    A *= 0.5;
+    mxx = myy = mxy = 0.0;
+    if (std::abs(A) < 1e-15) return;
+    cx /= (6.0 * A);
+    cy /= (6.0 * A);
+    for (int i = 0; i < n_; ++i) {
+        Real x0 = p[i].x - cx, y0 = p[i].y - cy;
+        const Vec2d& q = p[(i + 1) % n_];
+        Real x1 = q.x - cx, y1 = q.y - cy;
+        Real cr = x0 * y1 - x1 * y0;
+        mxx += (x0 * x0 + x0 * x1 + x1 * x1) * cr;
+        myy += (y0 * y0 + y0 * y1 + y1 * y1) * cr;
+        mxy += (x0 * y1 + 2.0 * x0 * y0 + 2.0 * x1 * y1 + x1 * y0) * cr;
+    }
+    const Real sg = (A > 0.0) ? 1.0 : -1.0;
+    mxx *= sg / 12.0; myy *= sg / 12.0; mxy *= sg / 24.0;
+}
+
+Real Capsule::deformationIndex() const {             // capsule.cpp:248-257
+    if (n_ < 3) return 0.0;
+    Real mxx, myy, mxy;
+    secondMoments(mxx, myy, mxy);
+    Real tr = 0.5 * (mxx + myy);
+    Real dd = std::sqrt(0.25 * (mxx - myy) * (mxx - myy) + mxy * mxy);
+    Real J1 = tr + dd, J2 = std::max(tr - dd, 0.0);
+    Real s1 = std::sqrt(J1), s2 = std::sqrt(J2);
+    return (s1 + s2 > 1e-15) ? (s1 - s2) / (s1 + s2) : 0.0;
+}
+
+Real Capsule::inclinationAngle() const {             // capsule.cpp:259-265
+    if (n_ < 3) return 0.0;
+    Real mxx, myy, mxy;
+    secondMoments(mxx, myy, mxy);
+    return 0.5 * std::atan2(2.0 * mxy, mxx - myy);
+}
+
+sf_capsule_desc Capsule::desc() const {
+    sf_capsule_desc d{};
+    d.model = static_cast<int>(params_.model);
+    d.use_helfrich = params_.use_helfrich_bending ? 1 : 0;
+    d.is_rigid = params_.is_rigid ? 1 : 0;
+    d.type = type_;
+    d.k_stretch = params_.k_stretch; d.G_s = params_.G_s; d.C_skalak = params_.C_skalak;
+    d.k_bend = params_.k_bend; d.k_area = params_.k_area; d.k_perimeter = params_.k_perimeter;
+    d.gamma_visc = params_.gamma_visc; d.eta_membrane = params_.eta_membrane;
+    d.wlc_L_max_ratio = params_.wlc_L_max_ratio; d.wlc_kBT_p = wlc_kBT_p_; d.wlc_k_pow = wlc_k_pow_;
+    d.rest_area = rest_area_; d.rest_perimeter = rest_perimeter_;
+    return d;
+}
+
+// ============================================================================
+void CapsuleSystem::addCapsule(Vec2d center, Real radius, int num_nodes, const MembraneParams& p,
+                               int type) {
+    pull();
+    caps_.emplace_back(next_id_++, center, radius, num_nodes, p, type);
+    if (periodic_nx_ > 0) caps_.back().setPeriodicX(periodic_nx_);
+    for (auto& c : caps_) c.sys_ = this;
+    dirty_ = true;
+}
+void CapsuleSystem::setPeriodicX(int nx) {
+    periodic_nx_ = nx;
+    for (auto& c : caps_) c.setPeriodicX(nx);
+}
+int CapsuleSystem::totalNodes() const {
+    int t = 0;
+    for (const auto& c : caps_) t += c.numNodes();
+    return t;
+}
+void CapsuleSystem::pull() {
+    if (stale_ && sim_) sim_->pullNodes(*this);
+}
+
+// ============================================================================
+AdvectionDiffusion::AdvectionDiffusion(int nx, int ny, const ScalarParams& p)
+    : nx_(nx), ny_(ny), nsp_(p.n_species) {
+    g_.assign(nsp_, std::vector<Real>(static_cast<size_t>(nx) * ny * Q, 0.0));
+    C_.assign(nsp_, std::vector<Real>(static_cast<size_t>(nx) * ny, 0.0));
+}
+void AdvectionDiffusion::pull() {
+    if (stale_ && sim_) sim_->pullScalar(*this);
+}
+void AdvectionDiffusion::initialize(Real c0) {        // advection_diffusion.cpp:46-57
+    stale_ = false;
+    for (int s = 0; s < nsp_; ++s)
+        for (size_t n = 0; n < C_[s].size(); ++n) {
+            C_[s][n] = c0;
+            for (int q = 0; q < Q; ++q) g_[s][n * Q + q] = W[q] * c0;
+        }
+    dirty_ = true;
+}
+void AdvectionDiffusion::setRegion(int x0, int y0, int x1, int y1, Real c, int s) {   // :59-70
+    if (s < 0 || s >= nsp_) return;
+    pull();
+    for (int y = std::max(0, y0); y < std::min(ny_, y1); ++y)
+        for (int x = std::max(0, x0); x < std::min(nx_, x1); ++x) {
+            const size_t n = static_cast<size_t>(y) * nx_ + x;
+            C_[s][n] = c;
+            for (int q = 0; q < Q; ++q) g_[s][n * Q + q] = W[q] * c;
+        }
+    dirty_ = true;
+}
+void AdvectionDiffusion::setPoint(int x, int y, Real c, int s) {                      // :72-80
+    if (x < 0 || x >= nx_ || y < 0 || y >= ny_ || s < 0 || s >= nsp_) return;
+    pull();
+    const size_t n = static_cast<size_t>(y) * nx_ + x;
+    C_[s][n] = c;
+    for (int q = 0; q < Q; ++q) g_[s][n * Q + q] = W[q] * c;
+    dirty_ = true;
+}
+Real AdvectionDiffusion::getConcentration(int x, int y, int s) {
+    if (x < 0 || x >= nx_ || y < 0 || y >= ny_ || s < 0 || s >= nsp_) return 0.0;
+    pull();
+    return C_[s][static_cast<size_t>(y) * nx_ + x];
+}
+Real* AdvectionDiffusion::concentrationData(int s) {
+    if (s < 0 || s >= nsp_) return nullptr;
+    pull();
+    return C_[s].data();
+}
+
+// ============================================================================
+Simulation::Simulation(const SimulationParams& p) : params_(p) {
+    if (p.nx < 3 || p.ny < 3) throw std::invalid_argument("Simulation: domain must be at least 3x3");
+    field_ = std::make_unique<LatticeField>(p.nx, p.ny);
+    field_->sim_ = this;
+    caps_.sim_ = this;
+    if (p.fluid.boundary_type == BoundaryType::PERIODIC) caps_.setPeriodicX(p.nx);   // simulation.cpp:129-133
+    if (p.scalar.enabled) {
+        adr_ = std::make_unique<AdvectionDiffusion>(p.nx, p.ny, p.scalar);
+        adr_->sim_ = this;
+    }
+}
+
+Simulation::~Simulation() {
+    if (ctx_) sf_destroy(ctx_);
+}
+
+void Simulation::check(int rc, const char* what) const {
+    if (rc != SF_OK) throw std::runtime_error(std::string(what) + ": " + sf_last_error());
+}
+
+void Simulation::setChannelBuilder(const ChannelBuilder& b) { builder_ = std::make_unique<ChannelBuilder>(b); }
+
+void Simulation::addCapsule(Vec2d center, Real radius, int num_nodes, const MembraneParams& mp, int type) {
+    caps_.addCapsule(center, radius, num_nodes, mp, type);
+}
+
+// Simulation::addCapsuleRandom: engine/simulation.cpp:220-274
+int Simulation::addCapsuleRandom(int count, Real x0, Real y0, Real x1, Real y1, Real rmin, Real rmax,
+                                 int num_nodes, const MembraneParams& mp, int type, int seed,
+                                 Real min_gap, int max_attempts) {
+    std::mt19937 rng(static_cast<unsigned>(seed));
+    std::uniform_real_distribution<Real> dx(x0, x1), dy(y0, y1), dr(rmin, rmax);
+    int placed = 0, attempts = 0;
+    const int total = (max_attempts > 0) ? max_attempts : count * 500;
+    while (placed < count && attempts < total) {
+        Real cx = dx(rng);
+        Real cy = dy(rng);
+        Real r = dr(rng);
+        bool overlap = false;
+        for (int c = 0; c < caps_.numCapsules(); ++c) {
+            Vec2d oc = caps_[c].centroid();
+            Real orr = caps_[c].effectiveRadius();
+            if ((Vec2d{cx, cy} - oc).norm() < r + orr + min_gap) { overlap = true; break; }
+        }
+        if (builder_) {
+            if (cy - r < builder_->wallBottom() + min_gap || cy + r > builder_->wallTop() - min_gap) overlap = true;
+        }
+        if (!overlap) {
+            int nn = (num_nodes > 0) ? num_nodes : Capsule::computeOptimalNodes(r);
+            caps_.addCapsule({cx, cy}, r, nn, mp, type);
+            placed++;
+        }
+        attempts++;
+    }
+    std::cout << "Placed " << placed << "/" << count << " capsules (" << attempts << " attempts)" << std::endl;
+    return placed;
+}
+
+void Simulation::initialize() {
+    const SimulationParams& P = params_;
+    // features of SimulationParams outside the device path
+    if (P.fluid.boundary_type == BoundaryType::INLET_OUTLET)
+        throw std::invalid_argument("Simulation: Zou-He inlet/outlet boundaries are not implemented on the MI355X path; use PERIODIC or CLOSED");
+    if ((P.fluid.gravity_x != 0.0 || P.fluid.gravity_y != 0.0))
+        throw std::invalid_argument("Simulation: gravity is not implemented on the MI355X path");
+    if (P.fluid.top_wall_velocity != 0.0 || P.fluid.bottom_wall_velocity != 0.0)
+        throw std::invalid_argument("Simulation: moving walls are not implemented on the MI355X path");
+    if (P.lubrication.enabled || P.adhesion.enabled)
+        throw std::invalid_argument("Simulation: lubrication/adhesion are not implemented on the MI355X path");
+    if (P.fluid.use_interpolated_bb)
+        throw std::invalid_argument("Simulation: interpolated bounce-back is not implemented on the MI355X path");
+
+    sf_params sp{};
+    sp.nx = P.nx; sp.ny = P.ny; sp.dt = P.dt; sp.ibm_iterations = P.ibm_iterations;
+    sp.tau = P.fluid.tau; sp.rho0 = P.fluid.rho0;
+    sp.boundary_type = static_cast<int>(P.fluid.boundary_type);
+    sp.body_force_x = P.fluid.body_force_x; sp.body_force_y = P.fluid.body_force_y;
+    // lbm_solver.cpp:22-24: regularized wins over MRT, MRT over BGK
+    int model = 0;
+    if (P.fluid.collision_model == CollisionModel::REGULARIZED) model = SF_REGULARIZED;
+    else if (P.fluid.use_mrt || P.fluid.collision_model == CollisionModel::MRT) model = SF_MRT;
+    sp.collision_model = model;
+    sp.mrt_se = P.fluid.mrt_se; sp.mrt_s_eps = P.fluid.mrt_s_eps; sp.mrt_sq = P.fluid.mrt_sq;
+    sp.periodic_y = P.fluid.periodic_y ? 1 : 0;
+    sp.max_lattice_force = P.max_lattice_force;
+    sp.rep_epsilon = P.repulsion.epsilon; sp.rep_sigma = P.repulsion.sigma; sp.rep_r_cut = P.repulsion.r_cut;
+    sp.rep_power = P.repulsion.power; sp.rep_damping_normal = P.repulsion.damping_normal;
+    sp.rep_friction_coeff = P.repulsion.friction_coeff;
+    sp.scalar_enabled = P.scalar.enabled ? 1 : 0;
+    sp.n_species = P.scalar.n_species;
+    for (int s = 0; s < std::min(4, P.scalar.n_species); ++s) {
+        sp.diffusivity[s] = s < static_cast<int>(P.scalar.diffusivity.size()) ? P.scalar.diffusivity[s] : 0.01;
+        sp.inlet_concentration[s] = s < static_cast<int>(P.scalar.inlet_concentration.size()) ? P.scalar.inlet_concentration[s] : 0.0;
+    }
+    sp.scalar_periodic_y = P.scalar.periodic_y ? 1 : 0;
+    sp.precision = static_cast<int>(P.precision);
+
+    if (ctx_) { sf_destroy(ctx_); ctx_ = nullptr; }
+    check(sf_create(&ctx_, P.device, &sp), "sf_create");
+    check(sf_set_graph_mode(ctx_, P.use_graph ? 1 : 0), "sf_set_graph_mode");
+
+    // cell types: lbm_solver.cpp:130-158 (with the channel builder's walls)
+    LatticeField& F = *field_;
+    const int nx = P.nx, ny = P.ny;
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            int ct = static_cast<int>(CellType::FLUID);
+            if (y == 0 || y == ny - 1) ct = static_cast<int>(CellType::SOLID);
+            if (P.fluid.boundary_type == BoundaryType::CLOSED && (x == 0 || x == nx - 1)) ct = static_cast<int>(CellType::SOLID);
+            if (P.fluid.periodic_y && (y == 0 || y == ny - 1) &&
+                !(P.fluid.boundary_type == BoundaryType::CLOSED && (x == 0 || x == nx - 1)))
+                ct = static_cast<int>(CellType::FLUID);
+            F.flags_[static_cast<size_t>(y) * nx + x] = ct;
+        }
+    check(sf_set_flags(ctx_, F.flags_.data()), "sf_set_flags");
+    // equilibrium at rest (lattice_field.cpp:75-90)
+    F.stale_ = false;
+    for (int n = 0; n < F.N_; ++n) {
+        F.rho_[n] = P.fluid.rho0; F.ux_[n] = 0.0; F.uy_[n] = 0.0;
+        F.Fx_[n] = 0.0; F.Fy_[n] = 0.0;
+        for (int q = 0; q < Q; ++q) F.f_[static_cast<size_t>(q) * F.N_ + n] = feq(q, P.fluid.rho0, 0.0, 0.0);
+    }
+    check(sf_set_populations(ctx_, F.f_.data()), "sf_set_populations");
+    F.f_dirty_ = F.macro_dirty_ = F.flags_dirty_ = false;
+    F.stale_ = true;   // rho/u as the device computes them (computeMacroscopic with F = 0)
+    if (adr_) adr_->initialize(0.0);
+    if (P.fluid.boundary_type == BoundaryType::PERIODIC) caps_.setPeriodicX(P.nx);
+    caps_.dirty_ = true;
+    step_count_ = 0;
+    pushDirty();
+    std::cout << "Simulation initialized: " << nx << "x" << ny << " lattice, " << caps_.caps_.size()
+              << " capsules (" << caps_.totalNodes() << " membrane nodes)" << std::endl;
+}
+
+void Simulation::pushDirty() {
+    if (!ctx_) throw std::runtime_error("Simulation: initialize() has not been called");
+    LatticeField& F = *field_;
+    if (F.flags_dirty_) { check(sf_set_flags(ctx_, F.flags_.data()), "sf_set_flags"); F.flags_dirty_ = false; }
+    if (F.f_dirty_) { check(sf_set_populations(ctx_, F.f_.data()), "sf_set_populations"); }
+    if (F.f_dirty_ || F.macro_dirty_) {
+        // the next collision reads the stored rho/u, as the reference's does
+        check(sf_set_macroscopic(ctx_, F.rho_.data(), F.ux_.data(), F.uy_.data()), "sf_set_macroscopic");
+    }
+    F.f_dirty_ = F.macro_dirty_ = false;
+    if (caps_.dirty_) {
+        auto& cs = caps_.caps_;
+        const int nc = static_cast<int>(cs.size());
+        std::vector<int32_t> off(nc + 1, 0), hp(nc, 0);
+        std::vector<sf_capsule_desc> desc(nc);
+        for (int c = 0; c < nc; ++c) {
+            off[c + 1] = off[c] + cs[c].n_;
+            desc[c] = cs[c].desc();
+            hp[c] = cs[c].has_prev_ ? 1 : 0;
+        }
+        const int K = off[nc];
+        std::vector<double> x(K), y(K), vx(K), vy(K), L0(K), th(K, 0.0), ds(K, 0.0), Lp(K);
+        for (int c = 0; c < nc; ++c)
+            for (int i = 0; i < cs[c].n_; ++i) {
+                const int k = off[c] + i;
+                x[k] = cs[c].pos_[i].x; y[k] = cs[c].pos_[i].y;
+                vx[k] = cs[c].vel_[i].x; vy[k] = cs[c].vel_[i].y;
+                L0[k] = cs[c].rest_len_[i]; Lp[k] = cs[c].prev_len_[i];
+                if (!cs[c].rest_angle_.empty()) { th[k] = cs[c].rest_angle_[i]; ds[k] = cs[c].rest_ds_[i]; }
+            }
+        check(sf_set_capsules(ctx_, nc, off.data(), desc.data(), x.data(), y.data(), vx.data(), vy.data(),
+                              L0.data(), th.data(), ds.data(), Lp.data(), hp.data()),
+              "sf_set_capsules");
+        caps_.dirty_ = false;
+    }
+    if (adr_ && adr_->dirty_) {
+        for (int s = 0; s < adr_->nsp_; ++s)
+            check(sf_set_scalar(ctx_, s, adr_->g_[s].data(), adr_->C_[s].data()), "sf_set_scalar");
+        adr_->dirty_ = false;
+    }
+}
+
+void Simulation::pullField(LatticeField& F) {
+    if (!ctx_) return;
+    F.stale_ = false;
+    check(sf_get_populations(ctx_, F.f_.data()), "sf_get_populations");
+    check(sf_get_macroscopic(ctx_, F.rho_.data(), F.ux_.data(), F.uy_.data(), F.Fx_.data(), F.Fy_.data()),
+          "sf_get_macroscopic");
+}
+
+void Simulation::pullNodes(CapsuleSystem& cs) {
+    cs.stale_ = false;
+    if (!ctx_ || cs.caps_.empty()) return;
+    const int K = cs.totalNodes();
+    const int nc = static_cast<int>(cs.caps_.size());
+    std::vector<double> x(K), y(K), vx(K), vy(K), fx(K), fy(K), Lp(K);
+    std::vector<int32_t> hp(nc);
+    check(sf_get_nodes(ctx_, x.data(), y.data(), vx.data(), vy.data(), fx.data(), fy.data()), "sf_get_nodes");
+    check(sf_get_membrane_state(ctx_, Lp.data(), hp.data()), "sf_get_membrane_state");
+    int k = 0;
+    for (int c = 0; c < nc; ++c) {
+        Capsule& cp = cs.caps_[c];
+        for (int i = 0; i < cp.n_; ++i, ++k) {
+            cp.pos_[i] = {x[k], y[k]};
+            cp.vel_[i] = {vx[k], vy[k]};
+            cp.frc_[i] = {fx[k], fy[k]};
+            cp.prev_len_[i] = Lp[k];
+        }
+        cp.has_prev_ = hp[c] != 0;
+    }
+}
+
+void Simulation::pullScalar(AdvectionDiffusion& a) {
+    a.stale_ = false;
+    if (!ctx_) return;
+    for (int s = 0; s < a.nsp_; ++s) check(sf_get_scalar(ctx_, s, a.C_[s].data(), a.g_[s].data()), "sf_get_scalar");
+}
+
+void Simulation::step() {
+    pushDirty();
+    check(sf_step(ctx_, 1), "sf_step");
+    field_->stale_ = true;
+    caps_.stale_ = true;
+    if (adr_) adr_->stale_ = true;
+    if (params_.enable_stability_checks && params_.stability_check_interval > 0 &&
+        step_count_ % params_.stability_check_interval == 0)
+        checkStability();
+    if (cb_) cb_(*this, static_cast<int>(step_count_));
+    step_count_++;
+}
+
+void Simulation::run(int num_steps) {
+    if (cb_) {
+        for (int s = 0; s < num_steps; ++s) step();
+        return;
+    }
+    int done = 0;
+    while (done < num_steps) {
+        // advance in chunks that end where the reference checks stability
+        int chunk = num_steps - done;
+        const int iv = params_.stability_check_interval;
+        const bool checks = params_.enable_stability_checks && iv > 0;
+        if (checks) {
+            const int64_t r = step_count_ % iv;
+            const int to_check = static_cast<int>(r == 0 ? 1 : iv - r + 1);
+            chunk = std::min(chunk, to_check);
+        }
+        pushDirty();
+        check(sf_step(ctx_, chunk), "sf_step");
+        field_->stale_ = true;
+        caps_.stale_ = true;
+        if (adr_) adr_->stale_ = true;
+        step_count_ += chunk;
+        if (checks && (step_count_ - 1) % iv == 0) checkStability();
+        done += chunk;
+    }
+}
+
+Real Simulation::getMaxSpeed() {
+    pushDirty();
+    double s[5];
+    check(sf_field_stats(ctx_, s), "sf_field_stats");
+    return s[3];
+}
+
+std::vector<double> Simulation::fieldStats() {
+    pushDirty();
+    std::vector<double> s(5);
+    check(sf_field_stats(ctx_, s.data()), "sf_field_stats");
+    return s;
+}
+
+int Simulation::lastCappedNodes() {
+    int32_t c = 0;
+    check(sf_last_capped(ctx_, &c), "sf_last_capped");
+    return c;
+}
+
+// simulation.cpp:820-890 on the host mirror
+bool Simulation::checkStability() {
+    LatticeField& F = *field_;
+    F.pull();
+    const Real rho0 = params_.fluid.rho0;
+    bool nan = false, dwarn = false, vwarn = false;
+    Real worst_rho = rho0, worst_vel = 0.0;
+    for (int n = 0; n < F.N_ && !nan; ++n) {
+        if (F.flags_[n] == static_cast<int>(CellType::SOLID) || F.flags_[n] == static_cast<int>(CellType::EMPTY)) continue;
+        const Real r = F.rho_[n], u = F.ux_[n], v = F.uy_[n];
+        if (!std::isfinite(r) || !std::isfinite(u) || !std::isfinite(v)) {
+            std::cerr << "*** STABILITY WARNING: NaN/Inf detected at step " << step_count_ << std::endl;
+            nan = true;
+            break;
+        }
+        if (std::abs(r - rho0) / rho0 > params_.max_density_deviation) {
+            if (!dwarn || std::abs(r - rho0) > std::abs(worst_rho - rho0)) worst_rho = r;
+            dwarn = true;
+        }
+        const Real vel = std::sqrt(u * u + v * v);
+        if (vel > params_.max_velocity) { if (vel > worst_vel) worst_vel = vel; vwarn = true; }
+    }
+    if (dwarn) std::cerr << "*** STABILITY WARNING: density deviation at step " << step_count_ << " - rho = " << worst_rho << std::endl;
+    if (vwarn) std::cerr << "*** STABILITY WARNING: high velocity at step " << step_count_ << " - |u| = " << worst_vel << std::endl;
+    const int capped = lastCappedNodes();
+    if (capped > 0)
+        std::cerr << "*** STABILITY NOTICE: max_lattice_force cap rescaled " << capped << " fluid nodes at step " << step_count_ << std::endl;
+    return !(nan || dwarn || vwarn);
+}
+
+// LBMSolver::setFluidRegion: lbm_solver.cpp:339-350
+void Simulation::setFluidRegion(int x0, int y0, int x1, int y1, Real rho, Real ux, Real uy) {
+    LatticeField& F = *field_;
+    F.touch_f();
+    F.touch_macro();
+    for (int y = std::max(0, y0); y < std::min(params_.ny, y1); ++y)
+        for (int x = std::max(0, x0); x < std::min(params_.nx, x1); ++x) {
+            if (F.getCellType(x, y) == CellType::SOLID) continue;
+            F.setRho(x, y, rho); F.setUx(x, y, ux); F.setUy(x, y, uy);
+            F.initializeEquilibriumAt(x, y);
+        }
+}
+
+void Simulation::setScalarRegion(int x0, int y0, int x1, int y1, Real c, int species) {
+    if (adr_) adr_->setRegion(x0, y0, x1, y1, c, species);
+}
+
+void Simulation::setTiming(bool on) { if (ctx_) check(sf_set_timing(ctx_, on ? 1 : 0), "sf_set_timing"); }
+
+std::vector<std::pair<std::string, double>> Simulation::kernelTimes() {
+    std::vector<std::pair<std::string, double>> out;
+    if (!ctx_) return out;
+    const char* names[32];
+    double ms[32];
+    int n = 0;
+    check(sf_kernel_times(ctx_, 32, names, ms, &n), "sf_kernel_times");
+    for (int i = 0; i < std::min(n, 32); ++i) out.emplace_back(names[i], ms[i]);
+    return out;
+}
+
+}  // namespace softflow

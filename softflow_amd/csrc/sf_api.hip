@@ -1,0 +1,751 @@
+// C-ABI entry points, device state management and the per-step schedule.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sf_internal.h"
+#include "sf_math.h"
+
+struct sf_ctx {
+    sf::Ctx c;
+};
+
+namespace sf {
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define SF_HIP(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(SF_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+__global__ void k_fill_int(int* p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_zero(uint32_t* p, size_t n4) {
+    const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (i < n4) p[i] = 0u;
+}
+
+void launch_fill_int(Ctx& c, int* p, int n, int v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, c.stream, p, n, v);
+}
+void launch_fill_zero(Ctx& c, void* p, size_t bytes) {
+    const size_t n4 = bytes / 4;     // buffers cleared this way hold 4-byte words
+    if (n4 == 0) return;
+    hipLaunchKernelGGL(k_fill_zero, dim3(static_cast<unsigned>((n4 + 255) / 256)), dim3(256), 0,
+                       c.stream, static_cast<uint32_t*>(p), n4);
+}
+
+// Scalar concentration from the distributions (computeConcentration,
+// advection_diffusion.cpp:212-223), SoA planes in this build.
+template <typename T>
+__global__ void k_csum(const T* g, int N, double* C) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) s += static_cast<double>(g[static_cast<size_t>(q) * N + n]);
+    C[n] = s;
+}
+
+__global__ void k_probe(int op, int n, const double* a, const double* b, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r = 0.0;
+    if (op == 0) r = sqrt(a[i]);
+    else if (op == 1) r = a[i] / b[i];
+    else if (op == 2) r = atan2(a[i], b[i]);
+    else if (op == 3) r = pow_int_cr(a[i], static_cast<int>(b[i]));
+    out[i] = r;
+}
+
+// ---------------------------------------------------------------------------
+// event timing of named stages
+struct TimingPool {
+    struct Rec { const char* name; hipEvent_t a, b; };
+    std::vector<Rec> live;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> freelist;
+    std::map<std::string, std::pair<double, long>> acc;
+};
+static std::map<const Ctx*, TimingPool>& pools() {
+    static std::map<const Ctx*, TimingPool> p;
+    return p;
+}
+
+struct Scope {
+    Ctx& c;
+    TimingPool::Rec rec{};
+    bool on;
+    Scope(Ctx& cc, const char* name) : c(cc), on(cc.timing && !cc.graph_mode) {
+        if (!on) return;
+        TimingPool& P = pools()[&c];
+        if (P.freelist.empty()) {
+            hipEvent_t a, b;
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            P.freelist.push_back({a, b});
+        }
+        auto pr = P.freelist.back();
+        P.freelist.pop_back();
+        rec = {name, pr.first, pr.second};
+        hipEventRecord(rec.a, c.stream);
+    }
+    ~Scope() {
+        if (!on) return;
+        hipEventRecord(rec.b, c.stream);
+        pools()[&c].live.push_back(rec);
+    }
+};
+
+static void harvest_timing(Ctx& c) {
+    auto it = pools().find(&c);
+    if (it == pools().end()) return;
+    TimingPool& P = it->second;
+    for (auto& r : P.live) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        auto& e = P.acc[r.name];
+        e.first += ms;
+        e.second += 1;
+        P.freelist.push_back({r.a, r.b});
+    }
+    P.live.clear();
+}
+
+// ---------------------------------------------------------------------------
+// One step in simulation.cpp:470-787 order.
+static void step_forces(Ctx& c) {
+    if (c.ncaps == 0) return;
+    { Scope s(c, "membrane"); launch_membrane(c); }
+    { Scope s(c, "repulsion"); launch_repulsion(c); }
+}
+
+static int step_ibm(Ctx& c) {
+    int shift;
+    { Scope s(c, "spread"); launch_spread(c, 0); }
+    if (c.p.ibm_iterations > 1) {
+        if (c.K > 0) {
+            for (int it = 1; it < c.p.ibm_iterations; ++it) {
+                { Scope s(c, "mdf_interp"); launch_mdf_interp(c); }
+                { Scope s(c, "mdf_spread"); launch_spread(c, 1); }
+            }
+            shift = SHIFT_PARTIAL;
+        } else {
+            shift = SHIFT_ZERO;   // computeMacroscopic with F = 0
+        }
+    } else {
+        shift = c.shift_next;
+    }
+    return shift;
+}
+
+static void one_step(Ctx& c) {
+    // rotate the raw-force buffers: this step's force goes into the spare one
+    const int spare = 3 - c.Fcur - c.Fprev;
+    c.Fprev = c.Fcur;
+    c.Fcur = spare;
+    launch_fill_int(c, c.counters, 1, 0);
+    step_forces(c);
+    const int shift = step_ibm(c);
+    { Scope s(c, "lbm"); launch_lbm(c, shift); }
+    if (c.nsp > 0) { Scope s(c, "adr"); launch_adr(c); }
+    if (c.ncaps > 0) { Scope s(c, "interp_move"); launch_interp_move(c); }
+    c.shift_next = SHIFT_PREV;
+    c.step_count++;
+}
+
+// Graph replay: the step's buffer rotation has period <= 6 (two population
+// buffers, a permutation of three force buffers), so six steps are captured
+// as one graph whose end state maps buffers back onto the start state.
+static int run_graph(Ctx& c, int nsteps) {
+    const int period = 6;
+    while (nsteps > 0) {
+        if (c.shift_next != SHIFT_PREV || nsteps < period || (c.nsp > 0 && c.c_from_store)) {
+            one_step(c);
+            --nsteps;
+            continue;
+        }
+        const int f0 = c.fcur, g0 = c.gcur, a0 = c.Fcur, b0 = c.Fprev, p0 = c.Fpart;
+        if (!c.graph_valid) {
+            hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+            for (int s = 0; s < period; ++s) one_step(c);
+            hipError_t e = hipStreamEndCapture(c.stream, &c.graph);
+            c.step_count -= period;   // counted again below
+            if (e != hipSuccess) return fail(SF_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            if (c.fcur != f0 || c.gcur != g0 || c.Fcur != a0 || c.Fprev != b0)
+                return fail(SF_E_STATE, "step buffer rotation did not close over the graph period");
+            SF_HIP(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
+            c.graph_valid = true;
+        }
+        SF_HIP(hipGraphLaunch(c.gexec, c.stream));
+        c.step_count += period;
+        c.Fpart = p0;
+        nsteps -= period;
+        c.c_from_store = false;
+    }
+    return SF_OK;
+}
+
+static void invalidate_graph(Ctx& c) {
+    if (c.gexec) hipGraphExecDestroy(c.gexec);
+    if (c.graph) hipGraphDestroy(c.graph);
+    c.gexec = nullptr;
+    c.graph = nullptr;
+    c.graph_valid = false;
+}
+
+static int check_counters(Ctx& c) {
+    SF_HIP(hipMemcpyAsync(c.host_counters, c.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    if (c.host_counters[1]) return fail(SF_E_CAPACITY, "IBM tile node list overflowed (more than 1024 nodes near one 32x8 tile)");
+    if (c.host_counters[2] & 1) return fail(SF_E_CAPACITY, "contact cell list overflowed (more than 48 nodes in one cell)");
+    if (c.host_counters[2] & 2) return fail(SF_E_CAPACITY, "contact cell grid exceeded its capacity");
+    return SF_OK;
+}
+
+static double phi_exact_host(double r) {       // coupling/delta_function.h:18-26
+    double ar = std::abs(r);
+    if (ar <= 1.0) return 0.125 * (3.0 - 2.0 * ar + std::sqrt(1.0 + 4.0 * ar - 4.0 * ar * ar));
+    if (ar <= 2.0) return 0.125 * (5.0 - 2.0 * ar - std::sqrt(-7.0 + 12.0 * ar - 4.0 * ar * ar));
+    return 0.0;
+}
+
+template <typename T>
+static int dmalloc(T** p, size_t bytes) {
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(p), bytes));
+    return SF_OK;
+}
+
+static void free_caps(Ctx& c) {
+    void* ptrs[] = {c.off, c.owner, c.desc, c.has_prev, c.x, c.y, c.vx, c.vy, c.fx, c.fy,
+                    c.L0, c.th0, c.ds0, c.Lprev, c.spread};
+    for (void* p : ptrs) if (p) hipFree(p);
+    c.off = c.owner = c.has_prev = nullptr;
+    c.desc = nullptr;
+    c.x = c.y = c.vx = c.vy = c.fx = c.fy = c.L0 = c.th0 = c.ds0 = c.Lprev = nullptr;
+    c.spread = nullptr;
+    c.ncaps = c.K = 0;
+}
+
+}  // namespace sf
+
+using namespace sf;
+
+extern "C" {
+
+const char* sf_last_error(void) { return g_err.c_str(); }
+
+int sf_device_count(int* n) {
+    if (!n) return fail(SF_E_ARG, "null pointer");
+    SF_HIP(hipGetDeviceCount(n));
+    return SF_OK;
+}
+
+int sf_create(sf_ctx** out, int device, const sf_params* p) {
+    if (!out || !p) return fail(SF_E_ARG, "null pointer");
+    *out = nullptr;
+    if (p->nx < 3 || p->ny < 3) return fail(SF_E_ARG, "domain must be at least 3x3");
+    if (p->boundary_type != SF_PERIODIC && p->boundary_type != SF_CLOSED)
+        return fail(SF_E_ARG, "device path supports periodic or closed x boundaries (Zou-He inlet/outlet is not implemented)");
+    if (p->tau <= 0.5) return fail(SF_E_ARG, "tau must exceed 0.5");
+    if (p->collision_model < 0 || p->collision_model > 2) return fail(SF_E_ARG, "unknown collision model");
+    if (p->precision != SF_FP64 && p->precision != SF_FP32) return fail(SF_E_ARG, "precision must be SF_FP64 or SF_FP32");
+    if (p->scalar_enabled && (p->n_species < 1 || p->n_species > 4)) return fail(SF_E_ARG, "1..4 scalar species supported");
+    if (p->ibm_iterations < 1) return fail(SF_E_ARG, "ibm_iterations must be >= 1");
+    int ndev = 0;
+    SF_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(SF_E_ARG, "no such HIP device");
+    SF_HIP(hipSetDevice(device));
+    auto* h = new sf_ctx();
+    Ctx& c = h->c;
+    c.p = *p;
+    c.device = device;
+    c.nx = p->nx; c.ny = p->ny; c.N = p->nx * p->ny;
+    c.periodic_x = p->boundary_type == SF_PERIODIC;
+    c.periodic_y = p->periodic_y != 0;
+    c.walls_y = !c.periodic_y;
+    c.walls_x = p->boundary_type == SF_CLOSED;
+    c.fp32 = p->precision == SF_FP32;
+    const size_t N = c.N;
+    const size_t es = c.fp32 ? 4 : 8;
+    int rc = SF_OK;
+    auto chk = [&](int r) { if (r != SF_OK && rc == SF_OK) rc = r; };
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(SF_E_HIP, "stream creation failed");
+    }
+    chk(dmalloc(&c.flags, N));
+    for (int b = 0; b < 2; ++b) chk(dmalloc(&c.f[b], 9 * N * es));
+    for (int b = 0; b < 3; ++b) chk(dmalloc(&c.F[b], 2 * N * es));
+    chk(dmalloc(&c.stored_rho, N * 8));
+    chk(dmalloc(&c.stored_u, 2 * N * 8));
+    chk(dmalloc(&c.counters, 64 * sizeof(int)));
+    chk(dmalloc(&c.phi, 1025 * 8));
+    if (rc != SF_OK) { sf_destroy(h); return rc; }
+    SF_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host_counters), 64 * sizeof(int)));
+    // flags: the plain channel of LBMSolver::initialize (lbm_solver.cpp:130-158)
+    std::vector<uint8_t> fl(N, SF_FLUID);
+    for (int y = 0; y < c.ny; ++y)
+        for (int x = 0; x < c.nx; ++x)
+            if ((c.walls_y && (y == 0 || y == c.ny - 1)) || (c.walls_x && (x == 0 || x == c.nx - 1)))
+                fl[static_cast<size_t>(y) * c.nx + x] = SF_SOLID;
+    SF_HIP(hipMemcpy(c.flags, fl.data(), N, hipMemcpyHostToDevice));
+    for (int b = 0; b < 3; ++b) SF_HIP(hipMemset(c.F[b], 0, 2 * N * es));
+    SF_HIP(hipMemset(c.counters, 0, 64 * sizeof(int)));
+    c.Fcur = 0; c.Fprev = 1; c.Fpart = 2;
+    double tab[1025];
+    for (int i = 0; i <= 1024; ++i) tab[i] = phi_exact_host(2.0 * static_cast<double>(i) / 1024);
+    SF_HIP(hipMemcpy(c.phi, tab, sizeof(tab), hipMemcpyHostToDevice));
+    c.ntx = (c.nx + TILE_X - 1) / TILE_X;
+    c.nty = (c.ny + TILE_Y - 1) / TILE_Y;
+    c.ntiles = c.ntx * c.nty;
+    // contact cell grid (cell_list.h:47-70)
+    const double rcut = p->rep_r_cut;
+    const double ybot = 0.5, ytop = static_cast<double>(c.ny) - 1.5;
+    c.ncy = std::max(1, static_cast<int>(std::floor((ytop - ybot) / rcut)));
+    c.cdy = (ytop - ybot) / static_cast<double>(c.ncy);
+    if (c.periodic_x) {
+        c.ncx = std::max(1, static_cast<int>(std::floor(static_cast<double>(c.nx) / rcut)));
+        c.cdx = static_cast<double>(c.nx) / static_cast<double>(c.ncx);
+    } else {
+        c.ncx = static_cast<int>(std::floor((static_cast<double>(c.nx) + 4.0 * rcut) / rcut)) + 2;
+        c.cdx = rcut;
+    }
+    if (p->scalar_enabled) {
+        c.nsp = p->n_species;
+        for (int s = 0; s < c.nsp; ++s) {
+            for (int b = 0; b < 2; ++b) {
+                if (dmalloc(&c.g[s][b], 9 * N * es) != SF_OK) { sf_destroy(h); return SF_E_HIP; }
+                SF_HIP(hipMemset(c.g[s][b], 0, 9 * N * es));
+            }
+            if (dmalloc(&c.Cstore[s], N * 8) != SF_OK) { sf_destroy(h); return SF_E_HIP; }
+            SF_HIP(hipMemset(c.Cstore[s], 0, N * 8));
+        }
+        c.c_from_store = true;
+    }
+    c.shift_next = SHIFT_ZERO;
+    *out = h;
+    return SF_OK;
+}
+
+int sf_destroy(sf_ctx* h) {
+    if (!h) return SF_OK;
+    Ctx& c = h->c;
+    hipSetDevice(c.device);
+    if (c.stream) hipStreamSynchronize(c.stream);
+    invalidate_graph(c);
+    free_caps(c);
+    void* ptrs[] = {c.flags, c.f[0], c.f[1], c.F[0], c.F[1], c.F[2], c.stored_rho, c.stored_u,
+                    c.counters, c.phi, c.tile_cnt, c.tile_list, c.cell_cnt, c.cell_list};
+    for (void* p : ptrs) if (p) hipFree(p);
+    for (int s = 0; s < 4; ++s) {
+        for (int b = 0; b < 2; ++b) if (c.g[s][b]) hipFree(c.g[s][b]);
+        if (c.Cstore[s]) hipFree(c.Cstore[s]);
+    }
+    if (c.host_counters) hipHostFree(c.host_counters);
+    auto it = pools().find(&c);
+    if (it != pools().end()) {
+        for (auto& pr : it->second.freelist) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+        for (auto& r : it->second.live) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+        pools().erase(it);
+    }
+    if (c.stream) hipStreamDestroy(c.stream);
+    delete h;
+    return SF_OK;
+}
+
+int sf_set_flags(sf_ctx* h, const int32_t* flags) {
+    if (!h || !flags) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    std::vector<uint8_t> fl(c.N);
+    bool channel = true;
+    for (int y = 0; y < c.ny; ++y)
+        for (int x = 0; x < c.nx; ++x) {
+            const size_t n = static_cast<size_t>(y) * c.nx + x;
+            const int v = flags[n];
+            if (v != SF_FLUID && v != SF_SOLID)
+                return fail(SF_E_ARG, "device path supports FLUID and SOLID cells only");
+            fl[n] = static_cast<uint8_t>(v);
+            const bool wall = (c.walls_y && (y == 0 || y == c.ny - 1)) || (c.walls_x && (x == 0 || x == c.nx - 1));
+            if ((v == SF_SOLID) != wall) channel = false;
+        }
+    SF_HIP(hipMemcpy(c.flags, fl.data(), c.N, hipMemcpyHostToDevice));
+    c.generic_flags = !channel;
+    invalidate_graph(c);
+    return SF_OK;
+}
+
+int sf_set_populations(sf_ctx* h, const double* f) {
+    if (!h || !f) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    const size_t N = c.N;
+    if (!c.fp32) {
+        SF_HIP(hipMemcpy(c.f[c.fcur], f, 9 * N * 8, hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(c.f[1 - c.fcur], f, 9 * N * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> d(9 * N);
+        for (int q = 0; q < 9; ++q)
+            for (size_t n = 0; n < N; ++n) d[q * N + n] = static_cast<float>(f[q * N + n] - dw(q));
+        SF_HIP(hipMemcpy(c.f[c.fcur], d.data(), 9 * N * 4, hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(c.f[1 - c.fcur], d.data(), 9 * N * 4, hipMemcpyHostToDevice));
+    }
+    c.lattice_ready = true;
+    return SF_OK;
+}
+
+int sf_get_populations(sf_ctx* h, double* f) {
+    if (!h || !f) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    const size_t N = c.N;
+    if (!c.fp32) {
+        SF_HIP(hipMemcpy(f, c.f[c.fcur], 9 * N * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> d(9 * N);
+        SF_HIP(hipMemcpy(d.data(), c.f[c.fcur], 9 * N * 4, hipMemcpyDeviceToHost));
+        for (int q = 0; q < 9; ++q)
+            for (size_t n = 0; n < N; ++n) f[q * N + n] = static_cast<double>(d[q * N + n]) + dw(q);
+    }
+    return SF_OK;
+}
+
+int sf_get_macroscopic(sf_ctx* h, double* rho, double* ux, double* uy, double* Fx, double* Fy) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    const size_t N = c.N;
+    double* buf = nullptr;
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&buf), 5 * N * 8));
+    launch_macro(c, buf, buf + N, buf + 2 * N, buf + 3 * N, buf + 4 * N);
+    hipError_t e = hipStreamSynchronize(c.stream);
+    double* outs[5] = {rho, ux, uy, Fx, Fy};
+    for (int k = 0; k < 5 && e == hipSuccess; ++k)
+        if (outs[k]) e = hipMemcpy(outs[k], buf + k * N, N * 8, hipMemcpyDeviceToHost);
+    hipFree(buf);
+    SF_HIP(e);
+    return SF_OK;
+}
+
+int sf_set_macroscopic(sf_ctx* h, const double* rho, const double* ux, const double* uy) {
+    if (!h || !rho || !ux || !uy) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    std::vector<double> u(2 * static_cast<size_t>(c.N));
+    for (int n = 0; n < c.N; ++n) { u[2 * n] = ux[n]; u[2 * n + 1] = uy[n]; }
+    SF_HIP(hipMemcpy(c.stored_rho, rho, c.N * 8, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(c.stored_u, u.data(), 2 * c.N * 8, hipMemcpyHostToDevice));
+    c.shift_next = SHIFT_STORED;
+    return SF_OK;
+}
+
+int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_desc* desc,
+                    const double* x, const double* y, const double* vx, const double* vy,
+                    const double* rest_len, const double* rest_angle, const double* rest_ds,
+                    const double* prev_len, const int32_t* has_prev) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    invalidate_graph(c);
+    free_caps(c);
+    if (ncaps == 0) return SF_OK;
+    if (ncaps < 0 || !off || !desc || !x || !y || !vx || !vy || !rest_len || !prev_len || !has_prev)
+        return fail(SF_E_ARG, "null capsule array");
+    if (off[0] != 0) return fail(SF_E_ARG, "offsets must start at 0");
+    int maxn = 0;
+    for (int i = 0; i < ncaps; ++i) {
+        const int n = off[i + 1] - off[i];
+        if (n < 1) return fail(SF_E_ARG, "capsule with no nodes");
+        if (n > 256) return fail(SF_E_ARG, "capsules with more than 256 nodes are not supported");
+        if (desc[i].use_helfrich && (!rest_angle || !rest_ds)) return fail(SF_E_ARG, "Helfrich capsule without rest angles");
+        maxn = std::max(maxn, n);
+    }
+    if (c.nx < 4) return fail(SF_E_ARG, "capsules need nx >= 4");
+    const int K = off[ncaps];
+    c.ncaps = ncaps; c.K = K; c.max_nodes = maxn;
+    std::vector<int> owner(K);
+    for (int i = 0; i < ncaps; ++i) for (int k = off[i]; k < off[i + 1]; ++k) owner[k] = i;
+    std::vector<double> zeros(K, 0.0);
+    int rc = SF_OK;
+    auto up = [&](double** d, const double* src) {
+        if (rc != SF_OK) return;
+        rc = dmalloc(d, K * 8);
+        if (rc == SF_OK && hipMemcpy(*d, src ? src : zeros.data(), K * 8, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(SF_E_HIP, "capsule upload failed");
+    };
+    up(&c.x, x); up(&c.y, y); up(&c.vx, vx); up(&c.vy, vy);
+    up(&c.fx, nullptr); up(&c.fy, nullptr);
+    up(&c.L0, rest_len); up(&c.th0, rest_angle); up(&c.ds0, rest_ds); up(&c.Lprev, prev_len);
+    if (rc != SF_OK) return rc;
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.off), (ncaps + 1) * sizeof(int)));
+    SF_HIP(hipMemcpy(c.off, off, (ncaps + 1) * sizeof(int), hipMemcpyHostToDevice));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.owner), K * sizeof(int)));
+    SF_HIP(hipMemcpy(c.owner, owner.data(), K * sizeof(int), hipMemcpyHostToDevice));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.desc), ncaps * sizeof(sf_capsule_desc)));
+    SF_HIP(hipMemcpy(c.desc, desc, ncaps * sizeof(sf_capsule_desc), hipMemcpyHostToDevice));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.has_prev), ncaps * sizeof(int)));
+    SF_HIP(hipMemcpy(c.has_prev, has_prev, ncaps * sizeof(int), hipMemcpyHostToDevice));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.spread), K * sizeof(NodeSpread)));
+    if (!c.tile_cnt) {
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_cnt), c.ntiles * sizeof(int)));
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_list), static_cast<size_t>(c.ntiles) * TILE_MAXL * sizeof(int)));
+    }
+    if (!c.cell_cnt && K > 100) {
+        const size_t ncells = static_cast<size_t>(c.ncx) * c.ncy;
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_cnt), ncells * sizeof(int)));
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_list), ncells * CELL_MAXC * sizeof(int)));
+    }
+    return SF_OK;
+}
+
+int sf_get_nodes(sf_ctx* h, double* x, double* y, double* vx, double* vy, double* fx, double* fy) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    const size_t b = static_cast<size_t>(c.K) * 8;
+    if (b == 0) return SF_OK;
+    if (x) SF_HIP(hipMemcpy(x, c.x, b, hipMemcpyDeviceToHost));
+    if (y) SF_HIP(hipMemcpy(y, c.y, b, hipMemcpyDeviceToHost));
+    if (vx) SF_HIP(hipMemcpy(vx, c.vx, b, hipMemcpyDeviceToHost));
+    if (vy) SF_HIP(hipMemcpy(vy, c.vy, b, hipMemcpyDeviceToHost));
+    if (fx) SF_HIP(hipMemcpy(fx, c.fx, b, hipMemcpyDeviceToHost));
+    if (fy) SF_HIP(hipMemcpy(fy, c.fy, b, hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+int sf_set_nodes(sf_ctx* h, const double* x, const double* y, const double* vx, const double* vy) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    const size_t b = static_cast<size_t>(c.K) * 8;
+    if (b == 0) return SF_OK;
+    if (x) SF_HIP(hipMemcpy(c.x, x, b, hipMemcpyHostToDevice));
+    if (y) SF_HIP(hipMemcpy(c.y, y, b, hipMemcpyHostToDevice));
+    if (vx) SF_HIP(hipMemcpy(c.vx, vx, b, hipMemcpyHostToDevice));
+    if (vy) SF_HIP(hipMemcpy(c.vy, vy, b, hipMemcpyHostToDevice));
+    return SF_OK;
+}
+
+int sf_get_membrane_state(sf_ctx* h, double* prev_len, int32_t* has_prev) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    if (c.K == 0) return SF_OK;
+    if (prev_len) SF_HIP(hipMemcpy(prev_len, c.Lprev, c.K * 8, hipMemcpyDeviceToHost));
+    if (has_prev) SF_HIP(hipMemcpy(has_prev, c.has_prev, c.ncaps * sizeof(int), hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+int sf_set_scalar(sf_ctx* h, int sp, const double* g, const double* C) {
+    if (!h || !g || !C) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    if (sp < 0 || sp >= c.nsp) return fail(SF_E_ARG, "no such scalar species");
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    const size_t N = c.N;
+    if (!c.fp32) {
+        std::vector<double> t(9 * N);
+        for (size_t n = 0; n < N; ++n) for (int q = 0; q < 9; ++q) t[q * N + n] = g[n * 9 + q];
+        for (int b = 0; b < 2; ++b) SF_HIP(hipMemcpy(c.g[sp][b], t.data(), 9 * N * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> t(9 * N);
+        for (size_t n = 0; n < N; ++n) for (int q = 0; q < 9; ++q) t[q * N + n] = static_cast<float>(g[n * 9 + q]);
+        for (int b = 0; b < 2; ++b) SF_HIP(hipMemcpy(c.g[sp][b], t.data(), 9 * N * 4, hipMemcpyHostToDevice));
+    }
+    SF_HIP(hipMemcpy(c.Cstore[sp], C, N * 8, hipMemcpyHostToDevice));
+    c.c_from_store = true;
+    invalidate_graph(c);
+    return SF_OK;
+}
+
+int sf_get_scalar(sf_ctx* h, int sp, double* C, double* g) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    if (sp < 0 || sp >= c.nsp) return fail(SF_E_ARG, "no such scalar species");
+    SF_HIP(hipSetDevice(c.device));
+    const size_t N = c.N;
+    if (C) {
+        if (c.c_from_store) {
+            SF_HIP(hipStreamSynchronize(c.stream));
+            SF_HIP(hipMemcpy(C, c.Cstore[sp], N * 8, hipMemcpyDeviceToHost));
+        } else {
+            double* buf = nullptr;
+            SF_HIP(hipMalloc(reinterpret_cast<void**>(&buf), N * 8));
+            if (c.fp32)
+                hipLaunchKernelGGL(k_csum<float>, dim3((c.N + 255) / 256), dim3(256), 0, c.stream,
+                                   static_cast<const float*>(c.g[sp][c.gcur]), c.N, buf);
+            else
+                hipLaunchKernelGGL(k_csum<double>, dim3((c.N + 255) / 256), dim3(256), 0, c.stream,
+                                   static_cast<const double*>(c.g[sp][c.gcur]), c.N, buf);
+            hipError_t e = hipStreamSynchronize(c.stream);
+            if (e == hipSuccess) e = hipMemcpy(C, buf, N * 8, hipMemcpyDeviceToHost);
+            hipFree(buf);
+            SF_HIP(e);
+        }
+    }
+    if (g) {
+        SF_HIP(hipStreamSynchronize(c.stream));
+        if (!c.fp32) {
+            std::vector<double> t(9 * N);
+            SF_HIP(hipMemcpy(t.data(), c.g[sp][c.gcur], 9 * N * 8, hipMemcpyDeviceToHost));
+            for (size_t n = 0; n < N; ++n) for (int q = 0; q < 9; ++q) g[n * 9 + q] = t[q * N + n];
+        } else {
+            std::vector<float> t(9 * N);
+            SF_HIP(hipMemcpy(t.data(), c.g[sp][c.gcur], 9 * N * 4, hipMemcpyDeviceToHost));
+            for (size_t n = 0; n < N; ++n) for (int q = 0; q < 9; ++q) g[n * 9 + q] = t[q * N + n];
+        }
+    }
+    return SF_OK;
+}
+
+int sf_step(sf_ctx* h, int nsteps) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    if (!c.lattice_ready) return fail(SF_E_STATE, "populations not set (call sf_set_populations first)");
+    if (nsteps < 0) return fail(SF_E_ARG, "negative step count");
+    SF_HIP(hipSetDevice(c.device));
+    const bool use_graph = c.graph_mode && !c.timing && (c.ncaps == 0 || c.periodic_x);
+    if (use_graph) {
+        int rc = run_graph(c, nsteps);
+        if (rc != SF_OK) return rc;
+    } else {
+        for (int s = 0; s < nsteps; ++s) one_step(c);
+    }
+    SF_HIP(hipGetLastError());
+    int rc = check_counters(c);
+    if (c.timing) harvest_timing(c);
+    return rc;
+}
+
+int sf_set_graph_mode(sf_ctx* h, int on) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    h->c.graph_mode = on != 0;
+    if (!on) invalidate_graph(h->c);
+    return SF_OK;
+}
+
+int sf_synchronize(sf_ctx* h) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    SF_HIP(hipStreamSynchronize(h->c.stream));
+    return SF_OK;
+}
+
+int sf_stage(sf_ctx* h, int stage) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    switch (stage) {
+    case SF_STAGE_FORCES: {
+        const int spare = 3 - c.Fcur - c.Fprev;
+        c.Fprev = c.Fcur;
+        c.Fcur = spare;
+        launch_fill_int(c, c.counters, 1, 0);
+        step_forces(c);
+        break;
+    }
+    case SF_STAGE_IBM: c.pending_shift = step_ibm(c); break;
+    case SF_STAGE_LBM: launch_lbm(c, c.pending_shift); break;
+    case SF_STAGE_SCALAR: if (c.nsp) launch_adr(c); break;
+    case SF_STAGE_MOVE:
+        if (c.ncaps) launch_interp_move(c);
+        c.shift_next = SHIFT_PREV;
+        c.step_count++;
+        break;
+    default: return fail(SF_E_ARG, "unknown stage");
+    }
+    SF_HIP(hipGetLastError());
+    return check_counters(c);
+}
+
+int sf_last_capped(sf_ctx* h, int32_t* capped) {
+    if (!h || !capped) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipMemcpyAsync(c.host_counters, c.counters, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    *capped = c.host_counters[0];
+    return SF_OK;
+}
+
+int sf_field_stats(sf_ctx* h, double* out5) {
+    if (!h || !out5) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    double* buf = nullptr;
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&buf), (8 + 3 * static_cast<size_t>(c.N) + 5 * 256) * 8));
+    launch_field_stats(c, buf);
+    hipError_t e = hipStreamSynchronize(c.stream);
+    if (e == hipSuccess) e = hipMemcpy(out5, buf, 5 * 8, hipMemcpyDeviceToHost);
+    hipFree(buf);
+    SF_HIP(e);
+    return SF_OK;
+}
+
+int sf_step_count(sf_ctx* h, int64_t* n) {
+    if (!h || !n) return fail(SF_E_ARG, "null pointer");
+    *n = h->c.step_count;
+    return SF_OK;
+}
+
+int sf_set_timing(sf_ctx* h, int on) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    h->c.timing = on != 0;
+    pools()[&h->c].acc.clear();
+    return SF_OK;
+}
+
+int sf_kernel_times(sf_ctx* h, int max, const char** names, double* ms, int* count) {
+    if (!h || !count) return fail(SF_E_ARG, "null pointer");
+    auto& P = pools()[&h->c];
+    int i = 0;
+    static std::vector<std::string> keep;
+    keep.clear();
+    for (auto& kv : P.acc) keep.push_back(kv.first);
+    for (auto& kv : P.acc) {
+        if (i < max) {
+            names[i] = keep[i].c_str();
+            ms[i] = kv.second.second ? kv.second.first / kv.second.second : 0.0;
+        }
+        ++i;
+    }
+    *count = i;
+    return SF_OK;
+}
+
+int sf_probe_math(int op, int n, const double* a, const double* b, double* out) {
+    if (n <= 0 || !a || !out || (op != 0 && !b)) return fail(SF_E_ARG, "bad probe arguments");
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&da), n * 8));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&db), n * 8));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&dout), n * 8));
+    SF_HIP(hipMemcpy(da, a, n * 8, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(db, b ? b : a, n * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_probe, dim3((n + 255) / 256), dim3(256), 0, nullptr, op, n, da, db, dout);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, dout, n * 8, hipMemcpyDeviceToHost);
+    hipFree(da); hipFree(db); hipFree(dout);
+    SF_HIP(e);
+    return SF_OK;
+}
+
+}  // extern "C"

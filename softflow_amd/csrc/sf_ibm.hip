@@ -1,0 +1,455 @@
+// Immersed-boundary kernels: Peskin 4-point spreading as a deterministic
+// per-tile gather, interpolation, multi-direct forcing and node motion.
+//
+// Spreading without float atomics: every node is binned into the lattice tiles
+// its 4x4 stencil touches; one workgroup per tile sorts its node list by
+// global node index (capsule-major, the reference's loop order) and every
+// cell-thread sums the contributions of the listed nodes in that order.  A
+// cell's force is therefore the same sequential sum the reference forms
+// (coupling/immersed_boundary.cpp:44-80 at one thread), independent of the
+// launch and of how nodes were appended to the lists.
+#include "sf_internal.h"
+
+namespace sf {
+
+// Peskin phi from the 1025-entry table (coupling/delta_function.h:45-54).
+__device__ __forceinline__ double phi_lut(const double* t, double r) {
+    double ar = fabs(r);
+    if (ar >= 2.0) return 0.0;
+    double idx = ar * (1024 * 0.5);
+    int i = static_cast<int>(idx);
+    if (i >= 1024) return 0.0;
+    double frac = idx - static_cast<double>(i);
+    return t[i] * (1.0 - frac) + t[i + 1] * frac;
+}
+
+__device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, double by, double Lx) {
+    double dx = ax - bx, dy = ay - by;   // immersed_boundary.cpp:16-23
+    if (Lx > 0.0) {
+        if (dx > 0.5 * Lx) dx -= Lx;
+        if (dx < -0.5 * Lx) dx += Lx;
+    }
+    return sqrt(dx * dx + dy * dy);
+}
+
+// ---------------------------------------------------------------------------
+// Node records + tile binning.  mode 0: the node's own force; mode 1: keep the
+// force already written by the MDF interpolation (only re-validate).
+__global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,
+                            const int* off, const int* owner, NodeSpread* rec, int K, int nx, int ny,
+                            int ntx, int nty, int* tile_cnt, int* tile_list, int* counters,
+                            int mode) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const double px = x[k], py = y[k];
+    NodeSpread s;
+    s.px = px;
+    s.py = py;
+    if (mode == 0) {
+        s.fx = fx[k];
+        s.fy = fy[k];
+        const int cap = owner[k];
+        const int o = off[cap], n = off[cap + 1] - o, i = k - o;
+        const int kp = o + (i - 1 + n) % n, kn = o + (i + 1) % n;
+        const double Lx = static_cast<double>(nx);
+        const double dp = min_image_ds(px, py, x[kp], y[kp], Lx);
+        const double dn = min_image_ds(x[kn], y[kn], px, py, Lx);
+        s.ds = 0.5 * (dp + dn);
+    } else {
+        s.fx = rec[k].fx;
+        s.fy = rec[k].fy;
+        s.ds = rec[k].ds;
+    }
+    const bool pos_ok = !(isnan(px) || isnan(py) || isinf(px) || isinf(py));
+    int i0 = 0, j0 = 0;
+    bool ok = pos_ok && !(isnan(s.fx) || isnan(s.fy));
+    if (pos_ok) {
+        i0 = static_cast<int>(floor(px));
+        j0 = static_cast<int>(floor(py));
+        if (j0 < -2 || j0 >= ny + 2) ok = false;
+    }
+    s.i0 = i0;
+    s.j0 = j0;
+    s.valid = ok ? 1 : 0;
+    s.pad = 0;
+    rec[k] = s;
+    if (mode != 0 || !pos_ok || j0 < -2 || j0 >= ny + 2) return;
+    // tiles touched by the stencil columns i0-1..i0+2 (single wrap) and rows j0-1..j0+2
+    int ylo = j0 - 1, yhi = j0 + 2;
+    if (ylo < 0) ylo = 0;
+    if (yhi > ny - 1) yhi = ny - 1;
+    if (ylo > yhi) return;
+    int txs[4], nt = 0;
+    for (int ix = i0 - 1; ix <= i0 + 2; ++ix) {
+        int jx = ix;
+        if (jx < 0) jx += nx;
+        if (jx >= nx) jx -= nx;
+        if (jx < 0 || jx >= nx) continue;
+        const int tx = jx / TILE_X;
+        bool seen = false;
+        for (int t = 0; t < nt; ++t) seen |= (txs[t] == tx);
+        if (!seen) txs[nt++] = tx;
+    }
+    for (int ty = ylo / TILE_Y; ty <= yhi / TILE_Y; ++ty)
+        for (int t = 0; t < nt; ++t) {
+            const int tile = ty * ntx + txs[t];
+            const int slot = atomicAdd(&tile_cnt[tile], 1);
+            if (slot < TILE_MAXL) tile_list[static_cast<size_t>(tile) * TILE_MAXL + slot] = k;
+            else atomicOr(&counters[1], 1);
+        }
+}
+
+// Gather: one 256-thread workgroup per TILE_X x TILE_Y tile.
+// mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction).
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const int* tile_cnt,
+                                                const int* tile_list, const double* lut,
+                                                const void* Fin, void* Fout, int nx, int ny,
+                                                int ntx, int mode) {
+    __shared__ int list[TILE_MAXL];
+    __shared__ double tab[1025];
+    const int tile = blockIdx.x;
+    const int t = threadIdx.x;
+    int L = tile_cnt[tile];
+    if (L > TILE_MAXL) L = TILE_MAXL;
+    int P = 1;
+    while (P < L) P <<= 1;
+    for (int i = t; i < P; i += 256)
+        list[i] = i < L ? tile_list[static_cast<size_t>(tile) * TILE_MAXL + i] : 0x7fffffff;
+    if (L > 0)
+        for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
+    __syncthreads();
+    // bitonic sort of the node list (ascending global node index)
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < P; i += 256) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int a = list[i], b = list[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) { list[i] = b; list[l] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int cx = (tile % ntx) * TILE_X + (t % TILE_X);
+    const int cy = (tile / ntx) * TILE_Y + (t / TILE_X);
+    if (cx >= nx || cy >= ny) return;
+    const int n = cy * nx + cx;
+    double ax = 0.0, ay = 0.0;
+    if (mode == 1) {
+        if (FP32) {
+            const float* p = static_cast<const float*>(Fin) + 2 * static_cast<size_t>(n);
+            ax = p[0]; ay = p[1];
+        } else {
+            const double* p = static_cast<const double*>(Fin) + 2 * static_cast<size_t>(n);
+            ax = p[0]; ay = p[1];
+        }
+    }
+    for (int i = 0; i < L; ++i) {
+        const NodeSpread s = rec[list[i]];
+        if (!s.valid) continue;
+        const int dyc = cy - (s.j0 - 1);
+        if (dyc < 0 || dyc > 3) continue;
+        // unwrapped stencil column whose single wrap lands on cx
+        int ix;
+        const int d0 = cx - (s.i0 - 1);
+        if (d0 >= 0 && d0 <= 3) ix = cx;
+        else if (cx - nx - (s.i0 - 1) >= 0 && cx - nx - (s.i0 - 1) <= 3 && cx - nx < 0) ix = cx - nx;
+        else if (cx + nx - (s.i0 - 1) >= 0 && cx + nx - (s.i0 - 1) <= 3 && cx + nx >= nx) ix = cx + nx;
+        else continue;
+        const double w = phi_lut(tab, s.px - static_cast<double>(ix)) *
+                         phi_lut(tab, s.py - static_cast<double>(cy));
+        if (w > 1e-15) {
+            ax += s.fx * w * s.ds;
+            ay += s.fy * w * s.ds;
+        }
+    }
+    if (FP32) {
+        float* p = static_cast<float*>(Fout) + 2 * static_cast<size_t>(n);
+        p[0] = static_cast<float>(ax); p[1] = static_cast<float>(ay);
+    } else {
+        double* p = static_cast<double*>(Fout) + 2 * static_cast<size_t>(n);
+        p[0] = ax; p[1] = ay;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Velocity at a stencil cell from the populations and a force field.
+// force_mode 0: raw force (the MDF's computeMacroscopic on F so far);
+// force_mode 1: the collision force (raw + body force, capped).
+struct CellVelArgs {
+    const void* f;
+    const void* F;
+    int nx, ny, N;
+    int walls_x, walls_y;
+    const uint8_t* flags;
+    int generic_flags;
+    int has_body;
+    double bfx, bfy, fmax, fmax2;
+};
+
+__device__ __forceinline__ bool cell_solid(const CellVelArgs& a, int x, int y) {
+    if (a.generic_flags) return a.flags[y * a.nx + x] == SF_SOLID;
+    return (a.walls_y && (y == 0 || y == a.ny - 1)) || (a.walls_x && (x == 0 || x == a.nx - 1));
+}
+
+template <bool FP32>
+__device__ __forceinline__ void cell_velocity(const CellVelArgs& a, int x, int y, int force_mode,
+                                              double& rho, double& ux, double& uy) {
+    if (cell_solid(a, x, y)) { rho = 0.0; ux = 0.0; uy = 0.0; return; }
+    const int n = y * a.nx + x;
+    const size_t N = static_cast<size_t>(a.N);
+    if constexpr (!FP32) {
+        const double* f = static_cast<const double*>(a.f);
+        double r = 0.0, vx = 0.0, vy = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            double fi = f[q * N + n];
+            r += fi;
+            vx += fi * static_cast<double>(dcx(q));
+            vy += fi * static_cast<double>(dcy(q));
+        }
+        const double* Fp = static_cast<const double*>(a.F) + 2 * static_cast<size_t>(n);
+        double fx = Fp[0], fy = Fp[1];
+        if (force_mode == 1) {
+            if (a.has_body) { fx += a.bfx; fy += a.bfy; }
+            double m2 = fx * fx + fy * fy;
+            if (m2 > a.fmax2) { double sc = a.fmax / sqrt(m2); fx *= sc; fy *= sc; }
+        }
+        rho = r;
+        if (r > 1e-15) { ux = (vx + 0.5 * fx) / r; uy = (vy + 0.5 * fy) / r; }
+        else { ux = 0.0; uy = 0.0; }
+    } else {
+        const float* f = static_cast<const float*>(a.f);
+        float d[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) d[q] = f[q * N + n];
+        const float drho = ((d[0] + (d[1] + d[3])) + (d[2] + d[4])) + ((d[5] + d[7]) + (d[6] + d[8]));
+        const float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
+        const float jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
+        const float* Fp = static_cast<const float*>(a.F) + 2 * static_cast<size_t>(n);
+        float fx = Fp[0], fy = Fp[1];
+        if (force_mode == 1) {
+            if (a.has_body) { fx += static_cast<float>(a.bfx); fy += static_cast<float>(a.bfy); }
+            float m2 = fx * fx + fy * fy;
+            if (m2 > static_cast<float>(a.fmax2)) { float sc = static_cast<float>(a.fmax) / sqrtf(m2); fx *= sc; fy *= sc; }
+        }
+        const float r = 1.0f + drho;
+        rho = r;
+        ux = (jx + 0.5f * fx) / r;
+        uy = (jy + 0.5f * fy) / r;
+    }
+}
+
+// Interpolation at a point (immersed_boundary.cpp:191-229).
+template <bool FP32>
+__device__ __forceinline__ void interp_point(const CellVelArgs& a, const double* lut, double px,
+                                             double py, int force_mode, double& Ux, double& Uy) {
+    Ux = 0.0; Uy = 0.0;
+    if (isnan(px) || isnan(py) || isinf(px) || isinf(py)) return;
+    const int ix0 = static_cast<int>(floor(px));
+    const int iy0 = static_cast<int>(floor(py));
+    if (iy0 < -2 || iy0 >= a.ny + 2) return;
+    for (int iy = iy0 - 1; iy <= iy0 + 2; ++iy) {
+        for (int ix = ix0 - 1; ix <= ix0 + 2; ++ix) {
+            int jx = ix;
+            if (jx < 0) jx += a.nx;
+            if (jx >= a.nx) jx -= a.nx;
+            const int jy = iy;
+            if (jy < 0 || jy >= a.ny) continue;
+            const double w = phi_lut(lut, px - static_cast<double>(ix)) *
+                             phi_lut(lut, py - static_cast<double>(iy));
+            if (w > 1e-15) {
+                double r, ux, uy;
+                cell_velocity<FP32>(a, jx, jy, force_mode, r, ux, uy);
+                Ux += ux * w;
+                Uy += uy * w;
+            }
+        }
+    }
+}
+
+// MDF correction forces (immersed_boundary.cpp:245-284): for every node,
+// dF = 2 rho_l (V_prev - U(X)) with U from the populations and the raw force
+// spread so far; written into the node's spread record.
+template <bool FP32>
+__global__ void k_mdf_interp(CellVelArgs a, const double* lut, const double* x, const double* y,
+                             const double* vx, const double* vy, NodeSpread* rec, int K) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const double px = x[k], py = y[k];
+    double Ux, Uy;
+    interp_point<FP32>(a, lut, px, py, 0, Ux, Uy);
+    const double dux = vx[k] - Ux, duy = vy[k] - Uy;
+    double rl = 1.0;
+    int ix = static_cast<int>(px + 0.5);
+    const int iy = static_cast<int>(py + 0.5);
+    if (ix < 0) ix += a.nx;
+    if (ix >= a.nx) ix -= a.nx;
+    if (iy >= 0 && iy < a.ny) {
+        double r, u, v;
+        cell_velocity<FP32>(a, ix, iy, 0, r, u, v);
+        rl = r;
+    }
+    const double two = 2.0 * rl;
+    rec[k].fx = dux * two;
+    rec[k].fy = duy * two;
+}
+
+// interpolateVelocity (immersed_boundary.cpp:174-189) with the post-step
+// velocity u = macro(f_{n+1}, F_n).
+template <bool FP32>
+__global__ void k_interp(CellVelArgs a, const double* lut, const double* x, const double* y,
+                         double* vx, double* vy, int K) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    double Ux, Uy;
+    interp_point<FP32>(a, lut, x[k], y[k], 1, Ux, Uy);
+    vx[k] = Ux;
+    vy[k] = Uy;
+}
+
+// moveAllNodes + y-clamp (capsule.cpp:267-297, simulation.cpp:665-675); one
+// wavefront per capsule so rigid capsules can project their velocities.
+__global__ void k_move(double* x, double* y, double* vx, double* vy, const int* off,
+                       const sf_capsule_desc* desc, int ncaps, double dt, int periodic_nx,
+                       int clamp_y, double ymin, double ymax) {
+    const int cap = blockIdx.x;
+    if (cap >= ncaps) return;
+    const int o = off[cap], n = off[cap + 1] - o;
+    const int lane = threadIdx.x;
+    if (desc[cap].is_rigid) {
+        // centroid with periodic minimum image about node 0 (capsule.cpp:159-173)
+        __shared__ double ctr[2], vcm[2], om;
+        if (lane == 0) {
+            double sx, sy;
+            if (periodic_nx > 0) {
+                const double Lx = static_cast<double>(periodic_nx);
+                const double rx = x[o], ry = y[o];
+                sx = rx; sy = ry;
+                for (int i = 1; i < n; ++i) {
+                    double dx = x[o + i] - rx, dy = y[o + i] - ry;
+                    if (dx > 0.5 * Lx) dx -= Lx;
+                    if (dx < -0.5 * Lx) dx += Lx;
+                    sx += rx + dx; sy += ry + dy;
+                }
+            } else {
+                sx = 0.0; sy = 0.0;
+                for (int i = 0; i < n; ++i) { sx += x[o + i]; sy += y[o + i]; }
+            }
+            ctr[0] = sx / static_cast<double>(n); ctr[1] = sy / static_cast<double>(n);
+            double cx = 0.0, cy = 0.0;
+            for (int i = 0; i < n; ++i) { cx += vx[o + i]; cy += vy[o + i]; }
+            const double inv = 1.0 / n;
+            cx *= inv; cy *= inv;
+            double num = 0.0, den = 0.0;
+            for (int i = 0; i < n; ++i) {
+                const double rx = x[o + i] - ctr[0], ry = y[o + i] - ctr[1];
+                num += rx * vy[o + i] - ry * vx[o + i];
+                den += rx * rx + ry * ry;
+            }
+            vcm[0] = cx; vcm[1] = cy;
+            om = (den > 1e-10) ? num / den : 0.0;
+        }
+        __syncthreads();
+        for (int i = lane; i < n; i += blockDim.x) {
+            const double rx = x[o + i] - ctr[0], ry = y[o + i] - ctr[1];
+            vx[o + i] = vcm[0] - om * ry;
+            vy[o + i] = vcm[1] + om * rx;
+        }
+        __syncthreads();
+    }
+    for (int i = lane; i < n; i += blockDim.x) {
+        double px = x[o + i] + vx[o + i] * dt;
+        double py = y[o + i] + vy[o + i] * dt;
+        if (periodic_nx > 0 && isfinite(px)) {
+            // repeated shifts as the reference does; a bounded count keeps a
+            // runaway coordinate from looping (it is then folded with fmod)
+            const double Lx = static_cast<double>(periodic_nx);
+            if (fabs(px) > 64.0 * Lx) px = fmod(px, Lx);
+            while (px < 0.0) px += Lx;
+            while (px >= Lx) px -= Lx;
+        }
+        if (clamp_y) {
+            if (py < ymin) py = ymin;
+            if (py > ymax) py = ymax;
+        }
+        x[o + i] = px;
+        y[o + i] = py;
+    }
+}
+
+static CellVelArgs vel_args(Ctx& c, const void* f, const void* F) {
+    CellVelArgs a{};
+    a.f = f; a.F = F;
+    a.nx = c.nx; a.ny = c.ny; a.N = c.N;
+    a.walls_x = c.walls_x; a.walls_y = c.walls_y;
+    a.flags = c.flags; a.generic_flags = c.generic_flags;
+    a.has_body = (c.p.body_force_x != 0.0 || c.p.body_force_y != 0.0);
+    a.bfx = c.p.body_force_x; a.bfy = c.p.body_force_y;
+    a.fmax = c.p.max_lattice_force; a.fmax2 = a.fmax * a.fmax;
+    return a;
+}
+
+// mode 0: bin nodes and spread node forces into F[Fcur] (overwrite).
+// mode 1: spread the MDF corrections on top of F[Fcur] into the spare buffer
+//         and make it current; the previous contents become F[Fpart].
+void launch_spread(Ctx& c, int mode) {
+    if (c.K == 0) {
+        if (mode == 0) launch_fill_zero(c, c.F[c.Fcur], (c.fp32 ? 8 : 16) * static_cast<size_t>(c.N));
+        return;
+    }
+    if (mode == 0) {
+        launch_fill_int(c, c.tile_cnt, c.ntiles, 0);
+        hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
+                           c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
+                           c.tile_cnt, c.tile_list, c.counters, 0);
+    } else {
+        hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
+                           c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
+                           c.tile_cnt, c.tile_list, c.counters, 1);
+    }
+    const void* Fin = c.F[c.Fcur];
+    // mode 1 writes into the buffer that is neither current nor previous
+    const int out = mode == 1 ? 3 - c.Fcur - c.Fprev : c.Fcur;
+    void* Fout = c.F[out];
+    if (c.fp32)
+        hipLaunchKernelGGL((k_spread<true>), dim3(c.ntiles), dim3(256), 0, c.stream, c.spread,
+                           c.tile_cnt, c.tile_list, c.phi, Fin, Fout, c.nx, c.ny, c.ntx, mode);
+    else
+        hipLaunchKernelGGL((k_spread<false>), dim3(c.ntiles), dim3(256), 0, c.stream, c.spread,
+                           c.tile_cnt, c.tile_list, c.phi, Fin, Fout, c.nx, c.ny, c.ntx, mode);
+    if (mode == 1) {
+        c.Fpart = c.Fcur;
+        c.Fcur = out;
+    }
+}
+
+void launch_mdf_interp(Ctx& c) {
+    if (c.K == 0) return;
+    CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
+    if (c.fp32)
+        hipLaunchKernelGGL((k_mdf_interp<true>), dim3((c.K + 255) / 256), dim3(256), 0, c.stream, a,
+                           c.phi, c.x, c.y, c.vx, c.vy, c.spread, c.K);
+    else
+        hipLaunchKernelGGL((k_mdf_interp<false>), dim3((c.K + 255) / 256), dim3(256), 0, c.stream, a,
+                           c.phi, c.x, c.y, c.vx, c.vy, c.spread, c.K);
+}
+
+void launch_interp_move(Ctx& c) {
+    if (c.K == 0) return;
+    CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
+    if (c.fp32)
+        hipLaunchKernelGGL((k_interp<true>), dim3((c.K + 255) / 256), dim3(256), 0, c.stream, a,
+                           c.phi, c.x, c.y, c.vx, c.vy, c.K);
+    else
+        hipLaunchKernelGGL((k_interp<false>), dim3((c.K + 255) / 256), dim3(256), 0, c.stream, a,
+                           c.phi, c.x, c.y, c.vx, c.vy, c.K);
+    hipLaunchKernelGGL(k_move, dim3(c.ncaps), dim3(64), 0, c.stream, c.x, c.y, c.vx, c.vy, c.off,
+                       c.desc, c.ncaps, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
+                       1.5, static_cast<double>(c.ny) - 1.5);
+}
+
+}  // namespace sf

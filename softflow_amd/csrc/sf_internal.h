@@ -1,0 +1,134 @@
+// Internal declarations shared by the HIP translation units of libsoftflow_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/softflow.h"
+
+namespace sf {
+
+// D2Q9 direction tables in the reference's order (lbm/lattice.h:9-31).
+__host__ __device__ constexpr int dcx(int q) {
+    return q == 1 || q == 5 || q == 8 ? 1 : (q == 3 || q == 6 || q == 7 ? -1 : 0);
+}
+__host__ __device__ constexpr int dcy(int q) {
+    return q == 2 || q == 5 || q == 6 ? 1 : (q == 4 || q == 7 || q == 8 ? -1 : 0);
+}
+__host__ __device__ constexpr int dopp(int q) {
+    return q == 0 ? 0 : (q <= 4 ? ((q + 1) % 4) + 1 : ((q - 3) % 4) + 5);
+}
+__host__ __device__ constexpr double dw(int q) {
+    return q == 0 ? 4.0 / 9.0 : (q <= 4 ? 1.0 / 9.0 : 1.0 / 36.0);
+}
+
+// Tile of lattice cells owned by one workgroup of the spread gather.
+constexpr int TILE_X = 32;
+constexpr int TILE_Y = 8;
+constexpr int TILE_MAXL = 1024;    // node references per tile
+constexpr int CELL_MAXC = 48;      // nodes per repulsion cell
+
+// Shift-force source of the collision velocity (see Ctx::shift_next).
+enum Shift : int { SHIFT_ZERO = 0, SHIFT_PREV = 1, SHIFT_PARTIAL = 2, SHIFT_STORED = 3 };
+
+// Per-node contribution to a spread: position, force, arc length and the
+// stencil origin.  One record per node, capsule-major.
+struct NodeSpread {
+    double px, py, fx, fy, ds;
+    int i0, j0, valid, pad;
+};
+
+struct Timer {
+    const char* name;
+    hipEvent_t a, b;
+    double total_ms;
+    int launches;
+};
+
+struct Ctx {
+    sf_params p{};
+    int device = 0;
+    int nx = 0, ny = 0, N = 0;
+    bool periodic_x = false, periodic_y = false;
+    bool generic_flags = false;          // flags differ from the plain channel
+    bool walls_y = true, walls_x = false;
+    bool fp32 = false;
+    hipStream_t stream = nullptr;
+
+    // lattice
+    uint8_t* flags = nullptr;            // N
+    void* f[2] = {nullptr, nullptr};     // 9*N populations (double, or float f-w)
+    int fcur = 0;
+    void* F[3] = {nullptr, nullptr, nullptr};   // raw spread force, (x,y) pairs
+    int Fcur = 0, Fprev = 1, Fpart = 2;
+    double* stored_rho = nullptr;        // macro override for one collision
+    double* stored_u = nullptr;          // 2N
+    int shift_next = SHIFT_ZERO;
+    int pending_shift = SHIFT_ZERO;      // shift chosen by the IBM stage (sf_stage)
+
+    // scalar
+    int nsp = 0;
+    void* g[4][2] = {};                  // 9*N, SoA planes
+    int gcur = 0;
+    double* Cstore[4] = {};
+    bool c_from_store = true;
+
+    // capsules
+    int ncaps = 0, K = 0, max_nodes = 0;
+    int* off = nullptr;
+    int* owner = nullptr;
+    sf_capsule_desc* desc = nullptr;
+    int* has_prev = nullptr;
+    double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr;
+    double *fx = nullptr, *fy = nullptr;
+    double *L0 = nullptr, *th0 = nullptr, *ds0 = nullptr, *Lprev = nullptr;
+    NodeSpread* spread = nullptr;        // K
+
+    // IBM tiles
+    double* phi = nullptr;               // 1025-entry table
+    int ntx = 0, nty = 0, ntiles = 0;
+    int* tile_cnt = nullptr;
+    int* tile_list = nullptr;
+
+    // repulsion cells
+    int ncx = 0, ncy = 0;
+    double cdx = 1.0, cdy = 1.0, xorg = 0.0, Lx = 0.0;
+    int* cell_cnt = nullptr;
+    int* cell_list = nullptr;
+
+    // counters: [0] capped cells, [1] tile overflow, [2] cell overflow
+    int* counters = nullptr;
+    int* host_counters = nullptr;        // pinned
+
+    // graph
+    bool graph_mode = false;
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_shift = -1;
+
+    // timing
+    bool timing = false;
+    std::vector<Timer> timers;
+
+    int64_t step_count = 0;
+    bool lattice_ready = false;
+};
+
+// -- launchers (defined in the .hip files) --------------------------------
+void launch_lbm(Ctx& c, int shift_mode);
+void launch_adr(Ctx& c);
+void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy);
+void launch_field_stats(Ctx& c, double* dev_out);
+void launch_membrane(Ctx& c);
+void launch_repulsion(Ctx& c);
+void launch_spread(Ctx& c, int mode);        // 0 node forces into F[Fcur]; 1 MDF correction
+void launch_mdf_interp(Ctx& c);
+void launch_interp_move(Ctx& c);
+void launch_fill_int(Ctx& c, int* p, int n, int v);
+void launch_fill_zero(Ctx& c, void* p, size_t bytes);
+void probe_math(int op, int n, const double* a, const double* b, double* out);
+
+}  // namespace sf

@@ -1,0 +1,670 @@
+// Lattice kernels: fused D2Q9 collide + push-stream with in-kernel halfway
+// bounce-back and periodic wrap, the advection-diffusion step, and the
+// on-demand macroscopic / reduction kernels.
+//
+// Layout in HBM: populations in SoA planes f[q][y*nx+x] (q-major, x fastest),
+// double-buffered.  FP64 mode stores f itself and evaluates every expression
+// in the reference's order (bit parity); FP32 mode stores f - w_q and
+// collides in FP32.  The raw spread force is an interleaved (Fx,Fy) pair per
+// cell; body force and the |F| cap are applied in-kernel.
+#include <type_traits>
+
+#include "sf_internal.h"
+
+namespace sf {
+
+// ---------------------------------------------------------------------------
+// FP64 collision operators in the reference's operation order.
+// BGK + Guo: lbm/bgk_collision.cpp:56-85
+__device__ __forceinline__ void col_bgk(double* f, double om, double r, double vx, double vy,
+                                        double fx, double fy) {
+    const double om1 = 1.0 - om;
+    const double guo = 1.0 - 0.5 * om;
+    const double uu = vx * vx + vy * vy;
+    const double uu_term = -0.5 * 3.0 * uu;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const double ex = static_cast<double>(dcx(q));
+        const double ey = static_cast<double>(dcy(q));
+        const double eu = ex * vx + ey * vy;
+        double fe = dw(q) * r * (1.0 + eu * 3.0 + 0.5 * eu * eu * 9.0 + uu_term);
+        double tx = (ex - vx) * 3.0 + (eu * 9.0) * ex;
+        double ty = (ey - vy) * 3.0 + (eu * 9.0) * ey;
+        double Fi = guo * dw(q) * (tx * fx + ty * fy);
+        f[q] = om1 * f[q] + om * fe + Fi;
+    }
+}
+
+// MRT + Guo in moment space: lbm/mrt_collision.cpp:65-138
+__device__ __forceinline__ void col_mrt(double* f, double sn, double s1, double s2, double s4,
+                                        double r, double vx, double vy, double fx, double fy) {
+    const double s6 = s4;
+    double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+    double f5 = f[5], f6 = f[6], f7 = f[7], f8 = f[8];
+    double m0 = f0 + f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8;
+    double m1 = -4.0 * f0 - f1 - f2 - f3 - f4 + 2.0 * f5 + 2.0 * f6 + 2.0 * f7 + 2.0 * f8;
+    double m2 = 4.0 * f0 - 2.0 * f1 - 2.0 * f2 - 2.0 * f3 - 2.0 * f4 + f5 + f6 + f7 + f8;
+    double m3 = f1 - f3 + f5 - f6 - f7 + f8;
+    double m4 = -2.0 * f1 + 2.0 * f3 + f5 - f6 - f7 + f8;
+    double m5 = f2 - f4 + f5 + f6 - f7 - f8;
+    double m6 = -2.0 * f2 + 2.0 * f4 + f5 + f6 - f7 - f8;
+    double m7 = f1 - f2 + f3 - f4;
+    double m8 = f5 - f6 + f7 - f8;
+    double uu = vx * vx + vy * vy;
+    double e1 = -2.0 * r + 3.0 * r * uu;
+    double e2 = r - 3.0 * r * uu;
+    double e4 = -r * vx;
+    double e6 = -r * vy;
+    double e7 = r * (vx * vx - vy * vy);
+    double e8 = r * vx * vy;
+    double uF = vx * fx + vy * fy;
+    double S1 = 6.0 * uF, S2 = -6.0 * uF, S3 = fx, S4 = -fx, S5 = fy, S6 = -fy;
+    double S7 = 2.0 * (vx * fx - vy * fy);
+    double S8 = vx * fy + vy * fx;
+    m1 = m1 - s1 * (m1 - e1) + (1.0 - s1 * 0.5) * S1;
+    m2 = m2 - s2 * (m2 - e2) + (1.0 - s2 * 0.5) * S2;
+    m3 = m3 + S3;
+    m4 = m4 - s4 * (m4 - e4) + (1.0 - s4 * 0.5) * S4;
+    m5 = m5 + S5;
+    m6 = m6 - s6 * (m6 - e6) + (1.0 - s6 * 0.5) * S6;
+    m7 = m7 - sn * (m7 - e7) + (1.0 - sn * 0.5) * S7;
+    m8 = m8 - sn * (m8 - e8) + (1.0 - sn * 0.5) * S8;
+    f[0] = m0 / 9.0 - m1 / 9.0 + m2 / 9.0;
+    f[1] = m0 / 9.0 - m1 / 36.0 - m2 / 18.0 + m3 / 6.0 - m4 / 6.0 + m7 / 4.0;
+    f[2] = m0 / 9.0 - m1 / 36.0 - m2 / 18.0 + m5 / 6.0 - m6 / 6.0 - m7 / 4.0;
+    f[3] = m0 / 9.0 - m1 / 36.0 - m2 / 18.0 - m3 / 6.0 + m4 / 6.0 + m7 / 4.0;
+    f[4] = m0 / 9.0 - m1 / 36.0 - m2 / 18.0 - m5 / 6.0 + m6 / 6.0 - m7 / 4.0;
+    f[5] = m0 / 9.0 + m1 / 18.0 + m2 / 36.0 + m3 / 6.0 + m4 / 12.0 + m5 / 6.0 + m6 / 12.0 + m8 / 4.0;
+    f[6] = m0 / 9.0 + m1 / 18.0 + m2 / 36.0 - m3 / 6.0 - m4 / 12.0 + m5 / 6.0 + m6 / 12.0 - m8 / 4.0;
+    f[7] = m0 / 9.0 + m1 / 18.0 + m2 / 36.0 - m3 / 6.0 - m4 / 12.0 - m5 / 6.0 - m6 / 12.0 + m8 / 4.0;
+    f[8] = m0 / 9.0 + m1 / 18.0 + m2 / 36.0 + m3 / 6.0 + m4 / 12.0 - m5 / 6.0 - m6 / 12.0 - m8 / 4.0;
+}
+
+// Regularized BGK keeping the first-order non-equilibrium moment:
+// lbm/regularized_bgk.cpp:35-91; equilibrium as lbm/lattice.h:35-39.
+__device__ __forceinline__ double feq_ref(int q, double rho, double ux, double uy) {
+    constexpr double cs2 = 1.0 / 3.0;
+    constexpr double cs4 = cs2 * cs2;
+    double eu = static_cast<double>(dcx(q)) * ux + static_cast<double>(dcy(q)) * uy;
+    double uu = ux * ux + uy * uy;
+    return dw(q) * rho * (1.0 + eu / cs2 + (eu * eu) / (2.0 * cs4) - uu / (2.0 * cs2));
+}
+__device__ __forceinline__ void col_reg(double* f, double omega, double rho, double ux, double uy,
+                                        double Fx, double Fy) {
+    double fe[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) fe[q] = feq_ref(q, rho, ux, uy);
+    double jx = 0.0, jy = 0.0, pxx = 0.0, pxy = 0.0, pyy = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        double fn = f[q] - fe[q];
+        jx += fn * dcx(q);
+        jy += fn * dcy(q);
+        pxx += fn * dcx(q) * dcx(q);
+        pxy += fn * dcx(q) * dcy(q);
+        pyy += fn * dcy(q) * dcy(q);
+    }
+    constexpr double cs2 = 1.0 / 3.0;
+    constexpr double cs4_2 = 2.0 * cs2 * cs2;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int cx = dcx(q), cy = dcy(q);
+        double Qxx = cx * cx - cs2;
+        double Qxy = cx * cy;
+        double Qyy = cy * cy - cs2;
+        double reg = dw(q) * ((cx * jx + cy * jy) / cs2 + (Qxx * pxx + 2.0 * Qxy * pxy + Qyy * pyy) / cs4_2);
+        double eu = cx * ux + cy * uy;
+        double Fi = (1.0 - 0.5 * omega) * dw(q) *
+                    ((cx - ux + eu * cx / cs2) * Fx / cs2 + (cy - uy + eu * cy / cs2) * Fy / cs2);
+        f[q] = fe[q] + (1.0 - omega) * reg + Fi;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 collision operators on deviations d_q = f_q - w_q.  The moments are
+// formed from the deviations (rho = 1 + sum d), so the small Guo increments
+// are not lost against w_q ~ 0.1.
+__device__ __forceinline__ void dcol_bgk(float* d, float om, float drho, float rho, float ux,
+                                         float uy, float fx, float fy) {
+    const float guo = 1.0f - 0.5f * om;
+    const float uu15 = 1.5f * (ux * ux + uy * uy);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const float ex = static_cast<float>(dcx(q)), ey = static_cast<float>(dcy(q));
+        const float w = static_cast<float>(dw(q));
+        const float eu = ex * ux + ey * uy;
+        const float deq = w * (drho + rho * (3.0f * eu + 4.5f * eu * eu - uu15));
+        const float src = guo * w * ((3.0f * (ex - ux) + 9.0f * eu * ex) * fx +
+                                     (3.0f * (ey - uy) + 9.0f * eu * ey) * fy);
+        d[q] = d[q] - om * (d[q] - deq) + src;
+    }
+}
+
+__device__ __forceinline__ void dcol_mrt(float* d, float sn, float s1, float s2, float s4,
+                                         float drho, float rho, float ux, float uy, float fx,
+                                         float fy) {
+    // moments of the deviations; the weights contribute m1 = -2, m2 = 1, which
+    // cancel in m - m_eq and in M^-1 (their image is w itself).
+    const float d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4];
+    const float d5 = d[5], d6 = d[6], d7 = d[7], d8 = d[8];
+    const float a14 = d1 + d2 + d3 + d4, a58 = d5 + d6 + d7 + d8;
+    float m1 = -4.0f * d0 - a14 + 2.0f * a58;
+    float m2 = 4.0f * d0 - 2.0f * a14 + a58;
+    float m3 = (d1 - d3) + (d5 - d6) + (d8 - d7);
+    float m4 = -2.0f * (d1 - d3) + (d5 - d6) + (d8 - d7);
+    float m5 = (d2 - d4) + (d5 + d6) - (d7 + d8);
+    float m6 = -2.0f * (d2 - d4) + (d5 + d6) - (d7 + d8);
+    float m7 = (d1 + d3) - (d2 + d4);
+    float m8 = (d5 + d7) - (d6 + d8);
+    const float uu = ux * ux + uy * uy;
+    const float e1 = -2.0f * drho + 3.0f * rho * uu;
+    const float e2 = drho - 3.0f * rho * uu;
+    const float e4 = -rho * ux, e6 = -rho * uy;
+    const float e7 = rho * (ux * ux - uy * uy), e8 = rho * ux * uy;
+    const float uF = ux * fx + uy * fy;
+    const float s6 = s4;
+    m1 = m1 - s1 * (m1 - e1) + (1.0f - 0.5f * s1) * (6.0f * uF);
+    m2 = m2 - s2 * (m2 - e2) - (1.0f - 0.5f * s2) * (6.0f * uF);
+    m3 = m3 + fx;
+    m4 = m4 - s4 * (m4 - e4) - (1.0f - 0.5f * s4) * fx;
+    m5 = m5 + fy;
+    m6 = m6 - s6 * (m6 - e6) - (1.0f - 0.5f * s6) * fy;
+    m7 = m7 - sn * (m7 - e7) + (1.0f - 0.5f * sn) * (2.0f * (ux * fx - uy * fy));
+    m8 = m8 - sn * (m8 - e8) + (1.0f - 0.5f * sn) * (ux * fy + uy * fx);
+    const float m0 = drho;
+    const float b = m0 * (1.0f / 9.0f);
+    const float c14 = b - m1 * (1.0f / 36.0f) - m2 * (1.0f / 18.0f);
+    const float c58 = b + m1 * (1.0f / 18.0f) + m2 * (1.0f / 36.0f);
+    d[0] = b - m1 * (1.0f / 9.0f) + m2 * (1.0f / 9.0f);
+    const float hx = m3 * (1.0f / 6.0f) - m4 * (1.0f / 6.0f);
+    const float hy = m5 * (1.0f / 6.0f) - m6 * (1.0f / 6.0f);
+    const float q7 = m7 * 0.25f;
+    d[1] = c14 + hx + q7;
+    d[2] = c14 + hy - q7;
+    d[3] = c14 - hx + q7;
+    d[4] = c14 - hy - q7;
+    const float gx = m3 * (1.0f / 6.0f) + m4 * (1.0f / 12.0f);
+    const float gy = m5 * (1.0f / 6.0f) + m6 * (1.0f / 12.0f);
+    const float q8 = m8 * 0.25f;
+    d[5] = c58 + gx + gy + q8;
+    d[6] = c58 - gx + gy - q8;
+    d[7] = c58 - gx - gy + q8;
+    d[8] = c58 + gx - gy - q8;
+}
+
+__device__ __forceinline__ void dcol_reg(float* d, float om, float drho, float rho, float ux,
+                                         float uy, float fx, float fy) {
+    float deq[9];
+    const float uu15 = 1.5f * (ux * ux + uy * uy);
+    float jx = 0.f, jy = 0.f, pxx = 0.f, pxy = 0.f, pyy = 0.f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const float ex = static_cast<float>(dcx(q)), ey = static_cast<float>(dcy(q));
+        const float eu = ex * ux + ey * uy;
+        deq[q] = static_cast<float>(dw(q)) * (drho + rho * (3.0f * eu + 4.5f * eu * eu - uu15));
+        const float fn = d[q] - deq[q];
+        jx += fn * ex; jy += fn * ey;
+        pxx += fn * ex * ex; pxy += fn * ex * ey; pyy += fn * ey * ey;
+    }
+    const float guo = 1.0f - 0.5f * om;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const float ex = static_cast<float>(dcx(q)), ey = static_cast<float>(dcy(q));
+        const float w = static_cast<float>(dw(q));
+        const float Qxx = ex * ex - (1.0f / 3.0f), Qyy = ey * ey - (1.0f / 3.0f), Qxy = ex * ey;
+        const float reg = w * (3.0f * (ex * jx + ey * jy) + 4.5f * (Qxx * pxx + 2.0f * Qxy * pxy + Qyy * pyy));
+        const float eu = ex * ux + ey * uy;
+        const float src = guo * w * ((3.0f * (ex - ux) + 9.0f * eu * ex) * fx +
+                                     (3.0f * (ey - uy) + 9.0f * eu * ey) * fy);
+        d[q] = deq[q] + (1.0f - om) * reg + src;
+    }
+}
+
+// ---------------------------------------------------------------------------
+struct LbmArgs {
+    const void* fsrc;
+    void* fdst;
+    const void* Fcur;        // raw spread force of this step
+    const void* Fshift;      // raw force feeding the collision velocity
+    const uint8_t* flags;
+    const double* srho;      // stored macro (SHIFT_STORED)
+    const double* su;
+    int* counters;
+    int nx, ny, N;
+    int periodic_x, periodic_y, walls_x, walls_y;
+    int shift;
+    int has_body;
+    double bfx, bfy, fmax, fmax2;
+    double om, se, seps, sq;
+};
+
+__device__ __forceinline__ bool is_solid(const LbmArgs& a, int x, int y, bool use_flags) {
+    if (use_flags) return a.flags[y * a.nx + x] == SF_SOLID;
+    return (a.walls_y && (y == 0 || y == a.ny - 1)) || (a.walls_x && (x == 0 || x == a.nx - 1));
+}
+
+template <typename T>
+__device__ __forceinline__ void load_force(const void* F, int n, T& fx, T& fy) {
+    const T* p = static_cast<const T*>(F) + 2 * static_cast<size_t>(n);
+    fx = p[0];
+    fy = p[1];
+}
+
+// Force of the collision: raw spread + body force (FLUID cells) then the |F|
+// cap (simulation.cpp:541-609).  Returns 1 if the cap fired.
+template <typename T>
+__device__ __forceinline__ int collision_force(const LbmArgs& a, const void* F, int n, bool fluid,
+                                               T& fx, T& fy) {
+    load_force<T>(F, n, fx, fy);
+    if (a.has_body && fluid) {
+        fx += static_cast<T>(a.bfx);
+        fy += static_cast<T>(a.bfy);
+    }
+    T m2 = fx * fx + fy * fy;
+    if (m2 > static_cast<T>(a.fmax2)) {
+        T sc = static_cast<T>(a.fmax) / sqrt(m2);
+        fx *= sc;
+        fy *= sc;
+        return 1;
+    }
+    return 0;
+}
+
+template <int OP, bool FP32, bool FLAGS>
+__global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    int capped = 0;
+    if (x < a.nx && y < a.ny) {
+        const int n = y * a.nx + x;
+        const bool solid = is_solid(a, x, y, FLAGS);
+        const bool fluid = !solid;
+        using T = typename std::conditional<FP32, float, double>::type;
+        T Fx, Fy;
+        capped = collision_force<T>(a, a.Fcur, n, fluid, Fx, Fy);
+        if (fluid) {
+            const T* src = static_cast<const T*>(a.fsrc);
+            T* dst = static_cast<T*>(a.fdst);
+            const size_t N = static_cast<size_t>(a.N);
+            T f[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) f[q] = src[q * N + n];
+            if constexpr (!FP32) {
+                double r = 0.0, vx = 0.0, vy = 0.0;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    double fi = f[q];
+                    r += fi;
+                    vx += fi * static_cast<double>(dcx(q));
+                    vy += fi * static_cast<double>(dcy(q));
+                }
+                double sx = 0.0, sy = 0.0;
+                if (a.shift == SHIFT_PREV) collision_force<double>(a, a.Fshift, n, true, sx, sy);
+                else if (a.shift == SHIFT_PARTIAL) load_force<double>(a.Fshift, n, sx, sy);
+                double ux, uy;
+                if (a.shift == SHIFT_STORED) {
+                    r = a.srho[n];
+                    ux = a.su[2 * n];
+                    uy = a.su[2 * n + 1];
+                } else if (r > 1e-15) {
+                    ux = (vx + 0.5 * sx) / r;
+                    uy = (vy + 0.5 * sy) / r;
+                } else {
+                    ux = 0.0;
+                    uy = 0.0;
+                }
+                if (OP == SF_REGULARIZED) col_reg(f, a.om, r, ux, uy, Fx, Fy);
+                else if (OP == SF_MRT) col_mrt(f, a.om, a.se, a.seps, a.sq, r, ux, uy, Fx, Fy);
+                else col_bgk(f, a.om, r, ux, uy, Fx, Fy);
+            } else {
+                float drho = ((f[0] + (f[1] + f[3])) + (f[2] + f[4])) + ((f[5] + f[7]) + (f[6] + f[8]));
+                float jx = (f[1] - f[3]) + (f[5] - f[6]) + (f[8] - f[7]);
+                float jy = (f[2] - f[4]) + (f[5] + f[6]) - (f[7] + f[8]);
+                float sx = 0.f, sy = 0.f;
+                if (a.shift == SHIFT_PREV) collision_force<float>(a, a.Fshift, n, true, sx, sy);
+                else if (a.shift == SHIFT_PARTIAL) load_force<float>(a.Fshift, n, sx, sy);
+                float rho = 1.0f + drho, ux, uy;
+                if (a.shift == SHIFT_STORED) {
+                    rho = static_cast<float>(a.srho[n]);
+                    drho = static_cast<float>(a.srho[n] - 1.0);
+                    ux = static_cast<float>(a.su[2 * n]);
+                    uy = static_cast<float>(a.su[2 * n + 1]);
+                } else {
+                    const float inv = 1.0f / rho;
+                    ux = (jx + 0.5f * sx) * inv;
+                    uy = (jy + 0.5f * sy) * inv;
+                }
+                const float om = static_cast<float>(a.om);
+                if (OP == SF_REGULARIZED) dcol_reg(f, om, drho, rho, ux, uy, Fx, Fy);
+                else if (OP == SF_MRT)
+                    dcol_mrt(f, om, static_cast<float>(a.se), static_cast<float>(a.seps),
+                             static_cast<float>(a.sq), drho, rho, ux, uy, Fx, Fy);
+                else dcol_bgk(f, om, drho, rho, ux, uy, Fx, Fy);
+            }
+            // push with halfway bounce-back (lbm_solver.cpp:212-235, bounce_back.cpp:12-40)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                int xd = x + dcx(q), yd = y + dcy(q);
+                bool oob = false;
+                if (a.periodic_x) xd = xd < 0 ? xd + a.nx : (xd >= a.nx ? xd - a.nx : xd);
+                else oob |= (xd < 0 || xd >= a.nx);
+                if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
+                else oob |= (yd < 0 || yd >= a.ny);
+                if (!oob) {
+                    if (is_solid(a, xd, yd, FLAGS)) dst[dopp(q) * N + n] = f[q];
+                    else dst[q * N + (yd * a.nx + xd)] = f[q];
+                }
+                if (q != 0) {
+                    const int xs = x - dcx(q), ys = y - dcy(q);
+                    const bool soob = (!a.periodic_x && (xs < 0 || xs >= a.nx)) ||
+                                      (!a.periodic_y && (ys < 0 || ys >= a.ny));
+                    if (soob) dst[q * N + n] = f[q];
+                }
+            }
+        }
+    }
+    // count capped cells: one atomic per wavefront
+    unsigned long long b = __ballot(capped);
+    if ((threadIdx.x + threadIdx.y * blockDim.x) % warpSize == 0 && b)
+        atomicAdd(&a.counters[0], __popcll(b));
+}
+
+template <int OP, bool FP32>
+static void lbm_dispatch_flags(Ctx& c, const LbmArgs& a, dim3 g, dim3 b) {
+    if (c.generic_flags) hipLaunchKernelGGL((k_lbm<OP, FP32, true>), g, b, 0, c.stream, a);
+    else hipLaunchKernelGGL((k_lbm<OP, FP32, false>), g, b, 0, c.stream, a);
+}
+
+void launch_lbm(Ctx& c, int shift) {
+    LbmArgs a{};
+    a.fsrc = c.f[c.fcur];
+    a.fdst = c.f[1 - c.fcur];
+    a.Fcur = c.F[c.Fcur];
+    a.Fshift = shift == SHIFT_PARTIAL ? c.F[c.Fpart] : c.F[c.Fprev];
+    a.flags = c.flags;
+    a.srho = c.stored_rho;
+    a.su = c.stored_u;
+    a.counters = c.counters;
+    a.nx = c.nx; a.ny = c.ny; a.N = c.N;
+    a.periodic_x = c.periodic_x; a.periodic_y = c.periodic_y;
+    a.walls_x = c.walls_x; a.walls_y = c.walls_y;
+    a.shift = shift;
+    a.has_body = (c.p.body_force_x != 0.0 || c.p.body_force_y != 0.0);
+    a.bfx = c.p.body_force_x; a.bfy = c.p.body_force_y;
+    a.fmax = c.p.max_lattice_force; a.fmax2 = a.fmax * a.fmax;
+    a.om = 1.0 / c.p.tau;
+    a.se = c.p.mrt_se; a.seps = c.p.mrt_s_eps; a.sq = c.p.mrt_sq;
+    dim3 b(64, 4), g((c.nx + 63) / 64, (c.ny + 3) / 4);
+    const int op = c.p.collision_model;
+    if (c.fp32) {
+        if (op == SF_REGULARIZED) lbm_dispatch_flags<SF_REGULARIZED, true>(c, a, g, b);
+        else if (op == SF_MRT) lbm_dispatch_flags<SF_MRT, true>(c, a, g, b);
+        else lbm_dispatch_flags<SF_BGK, true>(c, a, g, b);
+    } else {
+        if (op == SF_REGULARIZED) lbm_dispatch_flags<SF_REGULARIZED, false>(c, a, g, b);
+        else if (op == SF_MRT) lbm_dispatch_flags<SF_MRT, false>(c, a, g, b);
+        else lbm_dispatch_flags<SF_BGK, false>(c, a, g, b);
+    }
+    c.fcur = 1 - c.fcur;
+}
+
+// ---------------------------------------------------------------------------
+// Velocity of a cell after the LBM step (computeMacroscopic with the step's
+// collision force): used by the scalar step and the macroscopic views.
+template <bool FP32>
+__device__ __forceinline__ void cell_macro(const void* fv, size_t N, int n, const void* F,
+                                           const LbmArgs& a, bool zero_force, double& r,
+                                           double& ux, double& uy) {
+    if constexpr (!FP32) {
+        const double* f = static_cast<const double*>(fv);
+        double rr = 0.0, vx = 0.0, vy = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            double fi = f[q * N + n];
+            rr += fi;
+            vx += fi * static_cast<double>(dcx(q));
+            vy += fi * static_cast<double>(dcy(q));
+        }
+        double sx = 0.0, sy = 0.0;
+        if (!zero_force) collision_force<double>(a, F, n, true, sx, sy);
+        r = rr;
+        if (rr > 1e-15) {
+            ux = (vx + 0.5 * sx) / rr;
+            uy = (vy + 0.5 * sy) / rr;
+        } else {
+            ux = 0.0;
+            uy = 0.0;
+        }
+    } else {
+        const float* f = static_cast<const float*>(fv);
+        float d[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) d[q] = f[q * N + n];
+        float drho = ((d[0] + (d[1] + d[3])) + (d[2] + d[4])) + ((d[5] + d[7]) + (d[6] + d[8]));
+        float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
+        float jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
+        float sx = 0.f, sy = 0.f;
+        if (!zero_force) collision_force<float>(a, F, n, true, sx, sy);
+        float rho = 1.0f + drho;
+        r = rho;
+        ux = (jx + 0.5f * sx) / rho;
+        uy = (jy + 0.5f * sy) / rho;
+    }
+}
+
+// Advection-diffusion step: lbm/advection_diffusion.cpp:84-223.  The scalar
+// distributions are SoA planes g[q][n]; SOLID cells are never written (their
+// g stays at its initial value in both buffers, as the reference's copy does).
+struct AdrArgs {
+    LbmArgs L;
+    const void* f;           // populations after the LBM step
+    const void* gsrc[4];
+    void* gdst[4];
+    const double* Cst[4];
+    int nsp, from_store, scalar_periodic_y;
+    double oms[4];
+    int zero_force;
+};
+
+template <bool FP32, bool FLAGS>
+__global__ __launch_bounds__(256) void k_adr(AdrArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= a.L.nx || y >= a.L.ny) return;
+    const int n = y * a.L.nx + x;
+    if (is_solid(a.L, x, y, FLAGS)) return;
+    const size_t N = static_cast<size_t>(a.L.N);
+    double r, ux, uy;
+    cell_macro<FP32>(a.f, N, n, a.L.Fcur, a.L, a.zero_force != 0, r, ux, uy);
+    using T = typename std::conditional<FP32, float, double>::type;
+    for (int s = 0; s < a.nsp; ++s) {
+        const T* gs = static_cast<const T*>(a.gsrc[s]);
+        T* gd = static_cast<T*>(a.gdst[s]);
+        T g[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = gs[q * N + n];
+        double C;
+        if (a.from_store) {
+            C = a.Cst[s][n];
+        } else {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) acc += static_cast<double>(g[q]);
+            C = acc;
+        }
+        const double oms = a.oms[s];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            if constexpr (!FP32) {
+                const double eu = static_cast<double>(dcx(q)) * ux + static_cast<double>(dcy(q)) * uy;
+                const double ge = dw(q) * C * (1.0 + eu * 3.0);
+                g[q] = g[q] - oms * (g[q] - ge);
+            } else {
+                const float eu = static_cast<float>(dcx(q) * ux + dcy(q) * uy);
+                const float ge = static_cast<float>(dw(q)) * static_cast<float>(C) * (1.0f + 3.0f * eu);
+                g[q] = g[q] - static_cast<float>(oms) * (g[q] - ge);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            int xd = x + dcx(q);
+            xd = xd < 0 ? xd + a.L.nx : (xd >= a.L.nx ? xd - a.L.nx : xd);
+            int yd = y + dcy(q);
+            bool bounce;
+            if (a.scalar_periodic_y) {
+                yd = yd < 0 ? yd + a.L.ny : (yd >= a.L.ny ? yd - a.L.ny : yd);
+                bounce = is_solid(a.L, xd, yd, FLAGS);
+            } else {
+                bounce = (yd < 0 || yd >= a.L.ny) || is_solid(a.L, xd, yd, FLAGS);
+            }
+            if (bounce) gd[dopp(q) * N + n] = g[q];
+            else gd[q * N + (yd * a.L.nx + xd)] = g[q];
+        }
+    }
+}
+
+static LbmArgs view_args(Ctx& c) {
+    LbmArgs a{};
+    a.flags = c.flags;
+    a.nx = c.nx; a.ny = c.ny; a.N = c.N;
+    a.periodic_x = c.periodic_x; a.periodic_y = c.periodic_y;
+    a.walls_x = c.walls_x; a.walls_y = c.walls_y;
+    a.has_body = (c.p.body_force_x != 0.0 || c.p.body_force_y != 0.0);
+    a.bfx = c.p.body_force_x; a.bfy = c.p.body_force_y;
+    a.fmax = c.p.max_lattice_force; a.fmax2 = a.fmax * a.fmax;
+    a.Fcur = c.F[c.Fcur];
+    return a;
+}
+
+void launch_adr(Ctx& c) {
+    AdrArgs a{};
+    a.L = view_args(c);
+    a.f = c.f[c.fcur];
+    a.nsp = c.nsp;
+    a.from_store = c.c_from_store ? 1 : 0;
+    a.scalar_periodic_y = c.p.scalar_periodic_y;
+    for (int s = 0; s < c.nsp; ++s) {
+        a.gsrc[s] = c.g[s][c.gcur];
+        a.gdst[s] = c.g[s][1 - c.gcur];
+        a.Cst[s] = c.Cstore[s];
+        a.oms[s] = 1.0 / (3.0 * c.p.diffusivity[s] + 0.5);
+    }
+    dim3 b(64, 4), g((c.nx + 63) / 64, (c.ny + 3) / 4);
+    if (c.fp32) {
+        if (c.generic_flags) hipLaunchKernelGGL((k_adr<true, true>), g, b, 0, c.stream, a);
+        else hipLaunchKernelGGL((k_adr<true, false>), g, b, 0, c.stream, a);
+    } else {
+        if (c.generic_flags) hipLaunchKernelGGL((k_adr<false, true>), g, b, 0, c.stream, a);
+        else hipLaunchKernelGGL((k_adr<false, false>), g, b, 0, c.stream, a);
+    }
+    c.gcur = 1 - c.gcur;
+    c.c_from_store = false;
+}
+
+// ---------------------------------------------------------------------------
+// Macroscopic view: rho/ux/uy/Fx/Fy exactly as LatticeField holds them after
+// the last step.  mode: 0 = force of the last collision (after a step),
+// 1 = zero force (fresh initialisation), 2 = stored macro arrays.
+template <bool FP32, bool FLAGS>
+__global__ void k_macro(LbmArgs a, const void* f, int mode, double* rho, double* ux, double* uy,
+                        double* Fx, double* Fy) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.N) return;
+    const int x = n % a.nx, y = n / a.nx;
+    const bool solid = is_solid(a, x, y, FLAGS);
+    if (Fx) {
+        if (mode == 1) {
+            Fx[n] = 0.0; Fy[n] = 0.0;
+        } else if (FP32) {
+            float fx, fy;
+            collision_force<float>(a, a.Fcur, n, !solid, fx, fy);
+            Fx[n] = fx; Fy[n] = fy;
+        } else {
+            double fx, fy;
+            collision_force<double>(a, a.Fcur, n, !solid, fx, fy);
+            Fx[n] = fx; Fy[n] = fy;
+        }
+    }
+    if (!rho) return;
+    if (mode == 2) {
+        rho[n] = a.srho[n]; ux[n] = a.su[2 * n]; uy[n] = a.su[2 * n + 1];
+        return;
+    }
+    if (solid) {
+        rho[n] = 0.0; ux[n] = 0.0; uy[n] = 0.0;
+        return;
+    }
+    double r, u, v;
+    cell_macro<FP32>(f, static_cast<size_t>(a.N), n, a.Fcur, a, mode == 1, r, u, v);
+    rho[n] = r; ux[n] = u; uy[n] = v;
+}
+
+void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy) {
+    LbmArgs a = view_args(c);
+    a.srho = c.stored_rho;
+    a.su = c.stored_u;
+    const int mode = c.shift_next == SHIFT_STORED ? 2 : (c.step_count == 0 ? 1 : 0);
+    dim3 b(256), g((c.N + 255) / 256);
+    const void* f = c.f[c.fcur];
+    if (c.fp32) {
+        if (c.generic_flags) hipLaunchKernelGGL((k_macro<true, true>), g, b, 0, c.stream, a, f, mode, rho, ux, uy, Fx, Fy);
+        else hipLaunchKernelGGL((k_macro<true, false>), g, b, 0, c.stream, a, f, mode, rho, ux, uy, Fx, Fy);
+    } else {
+        if (c.generic_flags) hipLaunchKernelGGL((k_macro<false, true>), g, b, 0, c.stream, a, f, mode, rho, ux, uy, Fx, Fy);
+        else hipLaunchKernelGGL((k_macro<false, false>), g, b, 0, c.stream, a, f, mode, rho, ux, uy, Fx, Fy);
+    }
+}
+
+// FP64 reductions over FLUID cells: per-block partials in a fixed order, then
+// one block folds the partials (deterministic for a given launch shape).
+__global__ void k_stats_partial(const double* rho, const double* ux, const double* uy,
+                                const uint8_t* flags, LbmArgs a, double* part) {
+    __shared__ double s[5][256];
+    const int t = threadIdx.x;
+    double m = 0, px = 0, py = 0, um = 0, bad = 0;
+    for (int n = blockIdx.x * blockDim.x + t; n < a.N; n += gridDim.x * blockDim.x) {
+        const int x = n % a.nx, y = n / a.nx;
+        if (is_solid(a, x, y, flags != nullptr)) continue;
+        const double r = rho[n], u = ux[n], v = uy[n];
+        if (!isfinite(r) || !isfinite(u) || !isfinite(v)) { bad += 1.0; continue; }
+        m += r; px += r * u; py += r * v;
+        um = fmax(um, u * u + v * v);
+    }
+    s[0][t] = m; s[1][t] = px; s[2][t] = py; s[3][t] = um; s[4][t] = bad;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s[0][t] += s[0][t + w]; s[1][t] += s[1][t + w]; s[2][t] += s[2][t + w];
+            s[3][t] = fmax(s[3][t], s[3][t + w]); s[4][t] += s[4][t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < 5; ++k) part[blockIdx.x * 5 + k] = s[k][0];
+}
+
+__global__ void k_stats_final(const double* part, int nb, double* out) {
+    if (threadIdx.x != 0) return;
+    double m = 0, px = 0, py = 0, um = 0, bad = 0;
+    for (int b = 0; b < nb; ++b) {
+        m += part[b * 5]; px += part[b * 5 + 1]; py += part[b * 5 + 2];
+        um = fmax(um, part[b * 5 + 3]); bad += part[b * 5 + 4];
+    }
+    out[0] = m; out[1] = px; out[2] = py; out[3] = sqrt(um); out[4] = bad;
+}
+
+void launch_field_stats(Ctx& c, double* dev_out) {
+    // dev_out holds 5 results followed by scratch: 3N + 5*nb doubles
+    double* rho = dev_out + 8;
+    double* ux = rho + c.N;
+    double* uy = ux + c.N;
+    double* part = uy + c.N;
+    launch_macro(c, rho, ux, uy, nullptr, nullptr);
+    LbmArgs a = view_args(c);
+    const int nb = 256;
+    hipLaunchKernelGGL(k_stats_partial, dim3(nb), dim3(256), 0, c.stream, rho, ux, uy,
+                       c.generic_flags ? c.flags : nullptr, a, part);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, c.stream, part, nb, dev_out);
+}
+
+}  // namespace sf

@@ -1,0 +1,483 @@
+// Capsule membrane forces (one wavefront per capsule) and capsule contact
+// forces (cell list built by atomic binning + per-cell sort).
+//
+// Each node's force is the same ordered sequence of += / -= the reference
+// applies in its per-capsule loops (membrane/capsule.cpp:149-157, 320-671):
+// lanes compute per-edge and per-angle terms into LDS, then every node applies
+// the terms that touch it in the reference's loop order.  Ring reductions
+// (area, perimeter, unwrapped coordinates) are sequential FP64 sums.
+#include "sf_internal.h"
+#include "sf_math.h"
+
+namespace sf {
+
+constexpr int MAXN = 256;   // nodes per capsule held in LDS
+
+struct MemArgs {
+    double *x, *y, *vx, *vy, *fx, *fy;
+    const double *L0, *th0, *ds0;
+    double* Lprev;
+    int* has_prev;
+    const int* off;
+    const sf_capsule_desc* desc;
+    int periodic_nx;
+};
+
+__device__ __forceinline__ void mimg(double ax, double ay, double bx, double by, double Lx,
+                                     double& dx, double& dy) {
+    dx = ax - bx;
+    dy = ay - by;
+    if (Lx > 0.0) {
+        if (dx > 0.5 * Lx) dx -= Lx;
+        if (dx < -0.5 * Lx) dx += Lx;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_membrane(MemArgs a) {
+    __shared__ double X[MAXN], Y[MAXN], FX[MAXN], FY[MAXN];
+    __shared__ double Ex[MAXN], Ey[MAXN], Gx[MAXN], Gy[MAXN], Cc[MAXN], Ln[MAXN];
+    __shared__ int skip[MAXN];
+    __shared__ double red[2];
+    const int cap = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int o = a.off[cap], n = a.off[cap + 1] - o;
+    const sf_capsule_desc d = a.desc[cap];
+    const double Lx = static_cast<double>(a.periodic_nx);
+    for (int i = lane; i < n; i += 64) {
+        X[i] = a.x[o + i];
+        Y[i] = a.y[o + i];
+        FX[i] = 0.0;
+        FY[i] = 0.0;
+    }
+    __syncthreads();
+    if (d.is_rigid) {
+        for (int i = lane; i < n; i += 64) { a.fx[o + i] = 0.0; a.fy[o + i] = 0.0; }
+        return;
+    }
+    auto nxt = [n](int i) { return (i + 1) % n; };
+    auto prv = [n](int i) { return (i - 1 + n) % n; };
+    // edge forces applied as +F to node i and -F to node i+1, in edge order
+    auto apply_edges = [&](bool plus_first) {
+        for (int j = lane; j < n; j += 64) {
+            if (j >= 1) {
+                const int e = j - 1;
+                if (!skip[e]) { if (plus_first) { FX[j] -= Ex[e]; FY[j] -= Ey[e]; } else { FX[j] += Ex[e]; FY[j] += Ey[e]; } }
+                if (!skip[j]) { if (plus_first) { FX[j] += Ex[j]; FY[j] += Ey[j]; } else { FX[j] -= Ex[j]; FY[j] -= Ey[j]; } }
+            } else {
+                if (!skip[0]) { if (plus_first) { FX[0] += Ex[0]; FY[0] += Ey[0]; } else { FX[0] -= Ex[0]; FY[0] -= Ey[0]; } }
+                const int e = n - 1;
+                if (!skip[e]) { if (plus_first) { FX[0] -= Ex[e]; FY[0] -= Ey[e]; } else { FX[0] += Ex[e]; FY[0] += Ey[e]; } }
+            }
+        }
+    };
+
+    // -- stretching: capsule.cpp:320-470 --------------------------------
+    for (int i = lane; i < n; i += 64) {
+        const int j = nxt(i);
+        double dx, dy;
+        mimg(X[j], Y[j], X[i], Y[i], Lx, dx, dy);
+        const double L = sqrt(dx * dx + dy * dy);
+        skip[i] = L < 1e-15;
+        if (skip[i]) continue;
+        const double L0 = a.L0[o + i];
+        double Fm;
+        if (d.model == 1) {
+            double lam = L / L0;
+            double li3 = 1.0 / (lam * lam * lam);
+            Fm = d.G_s * (lam - li3);
+        } else if (d.model == 2) {
+            double lam = L / L0;
+            double lam2 = lam * lam;
+            Fm = d.G_s * lam * (lam2 - 1.0) * (1.0 + d.C_skalak);
+        } else if (d.model == 3) {
+            double Lmax = d.wlc_L_max_ratio * L0;
+            double xx = L / Lmax;
+            if (xx > 0.99) xx = 0.99;
+            double Fw = d.wlc_kBT_p * (1.0 / (4.0 * (1.0 - xx) * (1.0 - xx)) - 0.25 + xx);
+            double Fp = -d.wlc_k_pow / L;
+            Fm = Fw + Fp;
+        } else {
+            Fm = d.k_stretch * (L / L0 - 1.0);
+        }
+        const double ux = dx / L, uy = dy / L;
+        Ex[i] = ux * Fm;
+        Ey[i] = uy * Fm;
+    }
+    __syncthreads();
+    apply_edges(true);
+    __syncthreads();
+
+    // -- bending: capsule.cpp:476-541 ------------------------------------
+    if (d.use_helfrich) {
+        for (int i = lane; i < n; i += 64) {
+            const int ip = prv(i), in = nxt(i);
+            double epx, epy, enx, eny;
+            mimg(X[i], Y[i], X[ip], Y[ip], Lx, epx, epy);
+            mimg(X[in], Y[in], X[i], Y[i], Lx, enx, eny);
+            const double lp2 = epx * epx + epy * epy, ln2 = enx * enx + eny * eny;
+            skip[i] = (lp2 < 1e-30 || ln2 < 1e-30);
+            if (skip[i]) continue;
+            const double dth = atan2(epx * eny - epy * enx, epx * enx + epy * eny) - a.th0[o + i];
+            Cc[i] = -d.k_bend * dth / a.ds0[o + i];
+            Ex[i] = -eny / ln2;   // d theta / d x_{i+1}
+            Ey[i] = enx / ln2;
+            Gx[i] = -epy / lp2;   // d theta / d x_{i-1}
+            Gy[i] = epx / lp2;
+        }
+        __syncthreads();
+        for (int j = lane; j < n; j += 64) {
+            int cand[3] = {prv(j), j, nxt(j)};
+            // ascending, de-duplicated angle indices touching node j
+            for (int u = 0; u < 3; ++u)
+                for (int v = u + 1; v < 3; ++v)
+                    if (cand[v] < cand[u]) { int t = cand[u]; cand[u] = cand[v]; cand[v] = t; }
+            for (int u = 0; u < 3; ++u) {
+                const int i = cand[u];
+                if (u > 0 && cand[u - 1] == i) continue;
+                if (skip[i]) continue;
+                const double c = Cc[i];
+                if (nxt(i) == j) { FX[j] += Ex[i] * c; FY[j] += Ey[i] * c; }
+                if (prv(i) == j) { FX[j] += Gx[i] * c; FY[j] += Gy[i] * c; }
+                if (i == j) {
+                    const double sx = Ex[i] + Gx[i], sy = Ey[i] + Gy[i];
+                    FX[j] -= sx * c; FY[j] -= sy * c;
+                }
+            }
+        }
+    } else {
+        for (int i = lane; i < n; i += 64) {
+            double px, py, qx, qy;
+            mimg(X[prv(i)], Y[prv(i)], X[i], Y[i], Lx, px, py);
+            mimg(X[nxt(i)], Y[nxt(i)], X[i], Y[i], Lx, qx, qy);
+            const double lx = px + qx, ly = py + qy;
+            FX[i] += lx * d.k_bend;
+            FY[i] += ly * d.k_bend;
+        }
+    }
+    __syncthreads();
+
+    // -- area penalty: capsule.cpp:547-586 (area() and the signed sum) ----
+    if (lane == 0) {
+        double A = 0.0;
+        if (Lx > 0.0 && n > 0) {
+            double ux = X[0], uy = Y[0];
+            // unwrapped ring (capsule.cpp:177-186); Gx/Gy hold the coordinates
+            Gx[0] = ux; Gy[0] = uy;
+            for (int i = 1; i < n; ++i) {
+                double dx, dy;
+                mimg(X[i], Y[i], X[i - 1], Y[i - 1], Lx, dx, dy);
+                ux = Gx[i - 1] + dx; uy = Gy[i - 1] + dy;
+                Gx[i] = ux; Gy[i] = uy;
+            }
+            for (int i = 0; i < n; ++i) {
+                const int j = (i + 1) % n;
+                A += Gx[i] * Gy[j] - Gy[i] * Gx[j];
+            }
+        } else {
+            for (int i = 0; i < n; ++i) {
+                const int j = nxt(i);
+                A += X[i] * Y[j] - Y[i] * X[j];
+            }
+        }
+        red[0] = A;
+    }
+    __syncthreads();
+    {
+        const double A0 = d.rest_area;
+        if (!(A0 < 1e-15)) {
+            const double sa = red[0];
+            const double area = 0.5 * fabs(sa);
+            const double coeff = -d.k_area * (area - A0) / A0;
+            const double sign = (sa >= 0.0) ? 1.0 : -1.0;
+            for (int i = lane; i < n; i += 64) {
+                double epx, epy, enx, eny;
+                mimg(X[i], Y[i], X[prv(i)], Y[prv(i)], Lx, epx, epy);
+                mimg(X[nxt(i)], Y[nxt(i)], X[i], Y[i], Lx, enx, eny);
+                double ox = (epy + eny) * 0.5, oy = (-epx + -enx) * 0.5;
+                const double len = sqrt(ox * ox + oy * oy);
+                if (len < 1e-15) continue;
+                ox = ox / len * sign;
+                oy = oy / len * sign;
+                FX[i] += ox * coeff;
+                FY[i] += oy * coeff;
+            }
+        }
+    }
+    __syncthreads();
+
+    // -- perimeter penalty: capsule.cpp:588-604 --------------------------
+    for (int i = lane; i < n; i += 64) {
+        const int j = nxt(i);
+        double dx, dy;
+        mimg(X[j], Y[j], X[i], Y[i], Lx, dx, dy);
+        const double L = sqrt(dx * dx + dy * dy);
+        Ln[i] = L;
+        skip[i] = L < 1e-15;
+        if (!skip[i]) { Ex[i] = dx / L; Ey[i] = dy / L; }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double P = 0.0;
+        for (int i = 0; i < n; ++i) P += Ln[i];
+        red[1] = P;
+    }
+    __syncthreads();
+    {
+        const double P0 = d.rest_perimeter;
+        if (!(P0 < 1e-15)) {
+            const double coeff = -d.k_perimeter * (red[1] - P0) / P0;
+            for (int i = lane; i < n; i += 64)
+                if (!skip[i]) { Ex[i] = Ex[i] * coeff; Ey[i] = Ey[i] * coeff; }
+            __syncthreads();
+            apply_edges(false);
+        }
+    }
+    __syncthreads();
+
+    // -- nodal damping: capsule.cpp:606-610 ------------------------------
+    for (int i = lane; i < n; i += 64) {
+        FX[i] -= a.vx[o + i] * d.gamma_visc;
+        FY[i] -= a.vy[o + i] * d.gamma_visc;
+    }
+    __syncthreads();
+
+    // -- Kelvin-Voigt membrane viscosity: capsule.cpp:612-671 ------------
+    if (d.eta_membrane > 0.0) {
+        const bool first = a.has_prev[cap] == 0;
+        for (int i = lane; i < n; i += 64) {
+            const int j = nxt(i);
+            double dx, dy;
+            mimg(X[j], Y[j], X[i], Y[i], Lx, dx, dy);
+            const double L = sqrt(dx * dx + dy * dy);
+            if (first) { a.Lprev[o + i] = L; skip[i] = 1; continue; }
+            skip[i] = L < 1e-15;
+            if (skip[i]) continue;
+            const double sr = (L - a.Lprev[o + i]) / a.L0[o + i];
+            const double Fv = d.eta_membrane * sr;
+            const double ux = dx / L, uy = dy / L;
+            Ex[i] = ux * Fv;
+            Ey[i] = uy * Fv;
+            a.Lprev[o + i] = L;
+        }
+        __syncthreads();
+        if (!first) apply_edges(true);
+        __syncthreads();
+        if (first && lane == 0) a.has_prev[cap] = 1;
+    }
+    for (int i = lane; i < n; i += 64) {
+        a.fx[o + i] = FX[i];
+        a.fy[o + i] = FY[i];
+    }
+}
+
+void launch_membrane(Ctx& c) {
+    if (c.ncaps == 0) return;
+    MemArgs a{};
+    a.x = c.x; a.y = c.y; a.vx = c.vx; a.vy = c.vy; a.fx = c.fx; a.fy = c.fy;
+    a.L0 = c.L0; a.th0 = c.th0; a.ds0 = c.ds0; a.Lprev = c.Lprev; a.has_prev = c.has_prev;
+    a.off = c.off; a.desc = c.desc;
+    a.periodic_nx = c.periodic_x ? c.nx : 0;
+    hipLaunchKernelGGL(k_membrane, dim3(c.ncaps), dim3(64), 0, c.stream, a);
+}
+
+// ---------------------------------------------------------------------------
+// Contact forces: membrane/repulsion.cpp:6-178, membrane/cell_list.h:36-146.
+struct RepArgs {
+    const double *x, *y, *vx, *vy;
+    double *fx, *fy;
+    const int *off, *owner;
+    int K, ncaps;
+    double eps, sig, rcut, gn, mu, Lx;
+    int pw;
+    double ybot, ytop;
+    // cell list
+    const int* cnt;
+    const int* list;
+    int ncx, ncy;
+    double xorg, cdx, cdy;
+};
+
+__device__ __forceinline__ void pair_force(const RepArgs& a, double pax, double pay, double vax,
+                                           double vay, double pbx, double pby, double vbx,
+                                           double vby, double& Fx, double& Fy) {
+    double dx = pax - pbx, dy = pay - pby;
+    if (a.Lx > 0.0) {
+        if (dx > 0.5 * a.Lx) dx -= a.Lx;
+        if (dx < -0.5 * a.Lx) dx += a.Lx;
+    }
+    const double r = sqrt(dx * dx + dy * dy);
+    if (r < 1e-15 || r >= a.rcut) { Fx = 0.0; Fy = 0.0; return; }
+    const double nx = dx / r, ny = dy / r;
+    const double Fr = a.eps * pow_int_cr(a.sig / r, a.pw);
+    double Fn = Fr, tx = 0.0, ty = 0.0;
+    if (a.gn > 0.0 || a.mu > 0.0) {
+        const double rx = vax - vbx, ry = vay - vby;
+        const double vn = rx * nx + ry * ny;
+        Fn = Fr - a.gn * vn;
+        if (Fn < 0.0) Fn = 0.0;
+        if (a.mu > 0.0) {
+            const double vtx = rx - nx * vn, vty = ry - ny * vn;
+            const double vtm = sqrt(vtx * vtx + vty * vty);
+            if (vtm > 1e-12) {
+                const double s = -a.mu * Fn;
+                tx = vtx / vtm * s;
+                ty = vty / vtm * s;
+            }
+        }
+    }
+    Fx = nx * Fn + tx;
+    Fy = ny * Fn + ty;
+}
+
+__device__ __forceinline__ int cell_index(const RepArgs& a, double px0, double py) {
+    if (isnan(px0) || isnan(py) || isinf(px0) || isinf(py)) return 0;
+    double px = px0 - a.xorg;
+    if (a.Lx > 0.0) {
+        px = fmod(px, a.Lx);
+        if (px < 0.0) px += a.Lx;
+    }
+    int ix = static_cast<int>(floor(px / a.cdx));
+    int iy = static_cast<int>(floor((py - a.ybot) / a.cdy));
+    if (ix < 0) ix = 0;
+    if (ix >= a.ncx) ix = a.ncx - 1;
+    if (iy < 0) iy = 0;
+    if (iy >= a.ncy) iy = a.ncy - 1;
+    return iy * a.ncx + ix;
+}
+
+__global__ void k_cells_fill(RepArgs a, int* cnt, int* list, int* counters) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.K) return;
+    const int ci = cell_index(a, a.x[k], a.y[k]);
+    const int slot = atomicAdd(&cnt[ci], 1);
+    if (slot < CELL_MAXC) list[static_cast<size_t>(ci) * CELL_MAXC + slot] = k;
+    else atomicOr(&counters[2], 1);
+}
+
+__global__ void k_cells_sort(int ncells, const int* cnt, int* list) {
+    const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= ncells) return;
+    int m = cnt[ci];
+    if (m > CELL_MAXC) m = CELL_MAXC;
+    int* l = list + static_cast<size_t>(ci) * CELL_MAXC;
+    for (int i = 1; i < m; ++i) {
+        const int v = l[i];
+        int j = i - 1;
+        while (j >= 0 && l[j] > v) { l[j + 1] = l[j]; --j; }
+        l[j + 1] = v;
+    }
+}
+
+template <bool CELLS>
+__global__ void k_repulsion(RepArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.K) return;
+    const int cap = a.owner[k];
+    const double px = a.x[k], py = a.y[k], vx = a.vx[k], vy = a.vy[k];
+    double fx = a.fx[k], fy = a.fy[k];
+    double Fx, Fy;
+    if (!CELLS) {
+        for (int j = 0; j < a.ncaps; ++j) {
+            if (j == cap) continue;
+            for (int b = a.off[j]; b < a.off[j + 1]; ++b) {
+                pair_force(a, px, py, vx, vy, a.x[b], a.y[b], a.vx[b], a.vy[b], Fx, Fy);
+                fx += Fx; fy += Fy;
+            }
+        }
+    } else {
+        const int ci = cell_index(a, px, py);
+        const int cx0 = ci % a.ncx, cy0 = ci / a.ncx;
+        for (int dyc = -1; dyc <= 1; ++dyc) {
+            const int yy = cy0 + dyc;
+            if (yy < 0 || yy >= a.ncy) continue;
+            for (int dxc = -1; dxc <= 1; ++dxc) {
+                int xx = cx0 + dxc;
+                if (a.Lx > 0.0) { if (xx < 0) xx += a.ncx; if (xx >= a.ncx) xx -= a.ncx; }
+                else if (xx < 0 || xx >= a.ncx) continue;
+                const int cell = yy * a.ncx + xx;
+                int m = a.cnt[cell];
+                if (m > CELL_MAXC) m = CELL_MAXC;
+                const int* l = a.list + static_cast<size_t>(cell) * CELL_MAXC;
+                for (int t = 0; t < m; ++t) {
+                    const int b = l[t];
+                    if (a.owner[b] == cap) continue;
+                    pair_force(a, px, py, vx, vy, a.x[b], a.y[b], a.vx[b], a.vy[b], Fx, Fy);
+                    fx += Fx; fy += Fy;
+                }
+            }
+        }
+    }
+    pair_force(a, px, py, vx, vy, px, a.ybot, 0.0, 0.0, Fx, Fy);
+    fx += Fx; fy += Fy;
+    pair_force(a, px, py, vx, vy, px, a.ytop, 0.0, 0.0, Fx, Fy);
+    fx += Fx; fy += Fy;
+    a.fx[k] = fx;
+    a.fy[k] = fy;
+}
+
+// x extent of the cell grid when x is not periodic (cell_list.h:50-66):
+// bounding box of the nodes, computed on the device by one workgroup.
+struct CellGeom { double xorg, cdx; int ncx, pad; };
+__global__ void k_cell_geom(const double* x, int K, double rcut, int max_ncx, CellGeom* out,
+                            int* counters) {
+    __shared__ double mn[256], mx[256];
+    const int t = threadIdx.x;
+    double a = 1e30, b = -1e30;
+    for (int k = t; k < K; k += 256) { a = fmin(a, x[k]); b = fmax(b, x[k]); }
+    mn[t] = a; mx[t] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) { mn[t] = fmin(mn[t], mn[t + w]); mx[t] = fmax(mx[t], mx[t + w]); }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double xorg = mn[0] - rcut;
+        const double ext = (mx[0] - mn[0]) + 2.0 * rcut;
+        int ncx = static_cast<int>(floor(ext / rcut));
+        if (ncx < 1) ncx = 1;
+        if (ncx > max_ncx) { atomicOr(&counters[2], 2); ncx = max_ncx; }
+        out->xorg = xorg;
+        out->cdx = ext / static_cast<double>(ncx);
+        out->ncx = ncx;
+    }
+}
+
+void launch_repulsion(Ctx& c) {
+    if (c.ncaps == 0) return;
+    RepArgs a{};
+    a.x = c.x; a.y = c.y; a.vx = c.vx; a.vy = c.vy; a.fx = c.fx; a.fy = c.fy;
+    a.off = c.off; a.owner = c.owner; a.K = c.K; a.ncaps = c.ncaps;
+    a.eps = c.p.rep_epsilon; a.sig = c.p.rep_sigma; a.rcut = c.p.rep_r_cut;
+    a.gn = c.p.rep_damping_normal; a.mu = c.p.rep_friction_coeff; a.pw = c.p.rep_power;
+    a.Lx = c.periodic_x ? static_cast<double>(c.nx) : 0.0;
+    a.ybot = 0.5; a.ytop = static_cast<double>(c.ny) - 1.5;
+    const int bs = 128, nb = (c.K + bs - 1) / bs;
+    if (c.K <= 100) {
+        hipLaunchKernelGGL((k_repulsion<false>), dim3(nb), dim3(bs), 0, c.stream, a);
+        return;
+    }
+    a.ncy = c.ncy;
+    a.cdy = c.cdy;
+    if (c.periodic_x) {
+        a.ncx = c.ncx; a.xorg = 0.0; a.cdx = c.cdx;
+    } else {
+        // the grid changes with the node bounding box; the kernels read it
+        // from a device record written by k_cell_geom
+        CellGeom* g = reinterpret_cast<CellGeom*>(c.counters + 8);
+        hipLaunchKernelGGL(k_cell_geom, dim3(1), dim3(256), 0, c.stream, c.x, c.K, a.rcut, c.ncx, g, c.counters);
+        // non-periodic grids are resolved on the host after a sync
+        CellGeom h;
+        hipMemcpyAsync(&h, g, sizeof(h), hipMemcpyDeviceToHost, c.stream);
+        hipStreamSynchronize(c.stream);
+        a.ncx = h.ncx; a.xorg = h.xorg; a.cdx = h.cdx;
+    }
+    const int ncells = a.ncx * a.ncy;
+    launch_fill_int(c, c.cell_cnt, ncells, 0);
+    hipLaunchKernelGGL(k_cells_fill, dim3(nb), dim3(bs), 0, c.stream, a, c.cell_cnt, c.cell_list, c.counters);
+    hipLaunchKernelGGL(k_cells_sort, dim3((ncells + 255) / 256), dim3(256), 0, c.stream, ncells, c.cell_cnt, c.cell_list);
+    a.cnt = c.cell_cnt;
+    a.list = c.cell_list;
+    hipLaunchKernelGGL((k_repulsion<true>), dim3(nb), dim3(bs), 0, c.stream, a);
+}
+
+}  // namespace sf

@@ -1,0 +1,160 @@
+"""Shared case construction: the same configuration built in the product
+(``softflow_core`` over the HIP kernels) and in the oracle (``oracle/libsfo.so``).
+
+A case is a dict:
+    nx, ny, tau, coll (0 BGK, 1 MRT, 2 regularized), bfx, mdf (ibm_iterations),
+    boundary ('periodic' | 'closed'), periodic_y, precision ('fp64' | 'fp32'),
+    capsules: list of {count, region:(x0,y0,x1,y1), rmin, rmax, nodes, seed, min_gap, mem:{...}}
+    scalar: {D, blob:(cx, cy, sigma)}      (optional)
+    rep: {damping_normal, friction_coeff}  (optional)
+Membrane keys follow the oracle's struct (model, G_s, C_skalak, k_bend, use_helfrich, ...).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_MEM_RENAME = {"use_helfrich": "use_helfrich_bending"}
+
+
+def _blob(case):
+    s = case.get("scalar")
+    if not s:
+        return None
+    cx, cy, sg = s["blob"]
+    nx, ny = case["nx"], case["ny"]
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    return np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / (2.0 * sg * sg))
+
+
+def oracle_sim(case, which="oracle"):
+    from oracle import pyoracle as po
+
+    s = case.get("scalar")
+    rep = case.get("rep", {})
+    p = po.default_params(
+        nx=case["nx"], ny=case["ny"], tau=case.get("tau", 0.8),
+        boundary_type=2 if case.get("boundary") == "closed" else 1,
+        periodic_y=1 if case.get("periodic_y") else 0,
+        body_force_x=case.get("bfx", 0.0), collision_model=case.get("coll", 0),
+        ibm_iterations=case.get("mdf", 1), scalar_enabled=1 if s else 0,
+        diffusivity=[s["D"]] if s else [0.01],
+        rep_damping_normal=rep.get("damping_normal", 0.0),
+        rep_friction_coeff=rep.get("friction_coeff", 0.0),
+    )
+    sim = po.Sim(p, which)
+    for spec in case.get("capsules", []):
+        m = po.default_membrane(**spec.get("mem", {}))
+        if "centers" in spec:
+            for (cx, cy) in spec["centers"]:
+                sim.add_capsule(cx, cy, spec["rmin"], spec.get("nodes", 0), m, spec.get("type", 0))
+        else:
+            x0, y0, x1, y1 = spec["region"]
+            sim.add_capsule_random(spec["count"], x0, y0, x1, y1, spec["rmin"], spec["rmax"],
+                                   spec.get("nodes", 0), m, spec.get("type", 0), spec.get("seed", 42),
+                                   spec.get("min_gap", 1.0), 0)
+    sim.initialize()
+    blob = _blob(case)
+    if blob is not None:
+        sim.scalar_initialize(0.0)
+        for y in range(case["ny"]):
+            for x in range(case["nx"]):
+                sim.set_scalar_point(x, y, float(blob[y, x]), 0)
+    return sim
+
+
+def product_sim(case, graph=True):
+    from softflow_amd import softflow_core as sc
+
+    s = case.get("scalar")
+    rep = case.get("rep", {})
+    p = sc.SimulationParams()
+    p.nx, p.ny = case["nx"], case["ny"]
+    p.fluid.tau = case.get("tau", 0.8)
+    p.fluid.boundary_type = sc.BoundaryType.CLOSED if case.get("boundary") == "closed" else sc.BoundaryType.PERIODIC
+    p.fluid.periodic_y = bool(case.get("periodic_y", False))
+    p.fluid.body_force_x = case.get("bfx", 0.0)
+    p.fluid.collision_model = [sc.CollisionModel.BGK, sc.CollisionModel.MRT,
+                               sc.CollisionModel.REGULARIZED][case.get("coll", 0)]
+    p.ibm_iterations = case.get("mdf", 1)
+    p.repulsion.damping_normal = rep.get("damping_normal", 0.0)
+    p.repulsion.friction_coeff = rep.get("friction_coeff", 0.0)
+    if s:
+        p.scalar.enabled = True
+        p.scalar.diffusivity = [s["D"]]
+    p.enable_stability_checks = False
+    p.precision = sc.Precision.FP32 if case.get("precision") == "fp32" else sc.Precision.FP64
+    p.use_graph = graph
+    sim = sc.Simulation(p)
+    b = sc.ChannelBuilder(p.nx, p.ny)
+    b.addWalls()
+    sim.setChannelBuilder(b)
+    models = [sc.MembraneModel.HOOKEAN, sc.MembraneModel.NEO_HOOKEAN, sc.MembraneModel.SKALAK,
+              sc.MembraneModel.WLC]
+    shapes = [sc.CapsuleShape.CIRCLE, sc.CapsuleShape.ELLIPSE, sc.CapsuleShape.BICONCAVE,
+              sc.CapsuleShape.FIBER]
+    for spec in case.get("capsules", []):
+        mp = sc.MembraneParams()
+        for k, v in spec.get("mem", {}).items():
+            if k == "model":
+                v = models[v]
+            elif k == "shape":
+                v = shapes[v]
+            setattr(mp, _MEM_RENAME.get(k, k), v)
+        if "centers" in spec:
+            for (cx, cy) in spec["centers"]:
+                sim.addCapsule(sc.Vec2d(cx, cy), spec["rmin"], spec.get("nodes", 0), mp, spec.get("type", 0))
+        else:
+            x0, y0, x1, y1 = spec["region"]
+            sim.addCapsuleRandom(spec["count"], x0, y0, x1, y1, spec["rmin"], spec["rmax"],
+                                 spec.get("nodes", 0), mp, spec.get("type", 0), spec.get("seed", 42),
+                                 spec.get("min_gap", 1.0))
+    sim.initialize()
+    blob = _blob(case)
+    if blob is not None:
+        ad = sim.advectionDiffusion()
+        ad.initialize(0.0)
+        for y in range(case["ny"]):
+            for x in range(case["nx"]):
+                ad.setPoint(x, y, float(blob[y, x]), 0)
+    return sim
+
+
+def oracle_state(sim):
+    out = dict(sim.field())
+    n = sim.nodes()
+    out.update(x=n["x"], y=n["y"], vx=n["vx"], vy=n["vy"], fx_node=n["fx"], fy_node=n["fy"])
+    if sim.params.scalar_enabled:
+        out["C"] = sim.scalar()
+    out["capped"] = sim.last_capped()
+    return out
+
+
+def product_state(sim):
+    f = sim.field()
+    out = {"rho": np.array(f.density()), "ux": np.array(f.velocity_x()),
+           "uy": np.array(f.velocity_y()), "fx": np.array(f.force_x()), "fy": np.array(f.force_y())}
+    caps = sim.capsules()
+    xs, ys, vxs, vys, fxs, fys = [], [], [], [], [], []
+    for i in range(len(caps)):
+        c = caps[i]
+        for k in range(c.numNodes()):
+            p, v, fo = c.nodePosition(k), c.nodeVelocity(k), c.nodeForce(k)
+            xs.append(p.x); ys.append(p.y); vxs.append(v.x); vys.append(v.y)
+            fxs.append(fo.x); fys.append(fo.y)
+    out.update(x=np.array(xs), y=np.array(ys), vx=np.array(vxs), vy=np.array(vys),
+               fx_node=np.array(fxs), fy_node=np.array(fys))
+    ad = sim.advectionDiffusion()
+    if ad is not None:
+        out["C"] = np.array(ad.concentration(0))
+    out["capped"] = sim.lastCappedNodes()
+    return out
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    if a.size == 0:
+        return 0.0
+    den = np.linalg.norm(b)
+    num = np.linalg.norm(a - b)
+    return float(num / den) if den > 0 else float(num)
