@@ -1,0 +1,208 @@
+"""Benchmark of the per-timestep hot path on one MI355X.
+
+Workload (the large single-domain configuration): 4096 x 1024 D2Q9 channel,
+MRT collision (tau = 0.7, s_e = s_eps = s_q = 1.4), body force 1e-5, periodic
+x with half-way bounce-back walls in y; 2000 bidisperse Skalak capsules
+(800 of radius 6 with 38 nodes and G_s = 0.03 placed first with seed 101,
+then 1200 of radius 3 with 19 nodes and G_s = 0.06 with seed 202; C = 10,
+Helfrich bending k_b = 1e-3); an advection-diffusion scalar (D = 0.05)
+starting from a unit Gaussian blob of sigma = 20 at the centre.  Lattice
+storage and collide in FP32 (populations stored as f - w_q); capsules FP64.
+
+A step is one Simulation::step: membrane + contact forces, IBM spread, LBM
+collide-stream, scalar step, interpolation + node move.  ``value`` is whole
+step MLUPS over the timed region (HIP-graph replay, inputs resident in HBM).
+``roofline`` is for the dominant kernel (the fused collide-stream): its mean
+duration comes from HIP events around each launch in a second timed region
+of the same length, run kernel by kernel.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NX, NY = 4096, 1024
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per lattice update of the fused collide-stream in FP32:
+# 9 populations read + 9 written (72 B), the step's raw force (8 B) and the
+# previous step's raw force that sets the collision velocity (8 B)
+LBM_BYTES_PER_LUP = 88
+# the scalar step carried by the same kernel: 9 g read + 9 g written (FP32)
+ADR_BYTES_PER_LUP = 72
+
+
+def c3_case():
+    big = dict(count=800, region=(0, 0, NX, NY), rmin=6.0, rmax=6.0, nodes=38, seed=101,
+               mem=dict(model=2, G_s=0.03, C_skalak=10.0, use_helfrich=1, k_bend=1e-3))
+    small = dict(count=1200, region=(0, 0, NX, NY), rmin=3.0, rmax=3.0, nodes=19, seed=202,
+                 mem=dict(model=2, G_s=0.06, C_skalak=10.0, use_helfrich=1, k_bend=1e-3))
+    return dict(nx=NX, ny=NY, tau=0.7, bfx=1e-5, coll=1, capsules=[big, small],
+                scalar=dict(D=0.05, blob=(NX / 2, NY / 2, 20.0)))
+
+
+def build_product(case, precision="fp32", graph=True):
+    from softflow_amd import softflow_core as sc
+
+    p = sc.SimulationParams()
+    p.nx, p.ny = case["nx"], case["ny"]
+    p.fluid.tau = case["tau"]
+    p.fluid.boundary_type = sc.BoundaryType.PERIODIC
+    p.fluid.body_force_x = case["bfx"]
+    p.fluid.collision_model = [sc.CollisionModel.BGK, sc.CollisionModel.MRT,
+                               sc.CollisionModel.REGULARIZED][case["coll"]]
+    p.scalar.enabled = True
+    p.scalar.diffusivity = [case["scalar"]["D"]]
+    p.enable_stability_checks = False
+    p.precision = sc.Precision.FP32 if precision == "fp32" else sc.Precision.FP64
+    p.use_graph = graph
+    sim = sc.Simulation(p)
+    b = sc.ChannelBuilder(p.nx, p.ny)
+    b.addWalls()
+    sim.setChannelBuilder(b)
+    for spec in case["capsules"]:
+        mp = sc.MembraneParams()
+        m = spec["mem"]
+        mp.model = sc.MembraneModel.SKALAK
+        mp.G_s, mp.C_skalak = m["G_s"], m["C_skalak"]
+        mp.use_helfrich_bending = bool(m["use_helfrich"])
+        mp.k_bend = m["k_bend"]
+        x0, y0, x1, y1 = spec["region"]
+        sim.addCapsuleRandom(spec["count"], x0, y0, x1, y1, spec["rmin"], spec["rmax"],
+                             spec["nodes"], mp, 0, spec["seed"], 1.0)
+    sim.initialize()
+    cx, cy, sg = case["scalar"]["blob"]
+    yy, xx = np.mgrid[0:p.ny, 0:p.nx]
+    blob = np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / (2.0 * sg * sg))
+    ad = sim.advectionDiffusion()
+    ad.initialize(0.0)
+    ad.setField(blob.ravel(), 0)
+    return sim
+
+
+def cpu_baseline(case, steps):
+    """The oracle (single-threaded FP64 restatement) on this host: ``steps``
+    full steps of the same workload."""
+    from oracle import pyoracle as po
+
+    p = po.default_params(nx=case["nx"], ny=case["ny"], tau=case["tau"], boundary_type=1,
+                          body_force_x=case["bfx"], collision_model=case["coll"],
+                          scalar_enabled=1, diffusivity=[case["scalar"]["D"]])
+    s = po.Sim(p, "oracle")
+    for spec in case["capsules"]:
+        m = po.default_membrane(**spec["mem"])
+        x0, y0, x1, y1 = spec["region"]
+        s.add_capsule_random(spec["count"], x0, y0, x1, y1, spec["rmin"], spec["rmax"],
+                             spec["nodes"], m, 0, spec["seed"], 1.0, 0)
+    s.initialize()
+    s.step(1)   # first step outside the timing (page-in of the arrays)
+    t = time.perf_counter()
+    s.step(steps)
+    dt = time.perf_counter() - t
+    return case["nx"] * case["ny"] * steps / dt / 1e6, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing-pass", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    case = c3_case()
+    t0 = time.perf_counter()
+    sim = build_product(case, "fp32", graph=True)
+    print(f"# setup {time.perf_counter() - t0:.1f}s, {sum(c['count'] for c in case['capsules'])} capsules",
+          file=sys.stderr, flush=True)
+    sim.run(min(args.warmup, 10))
+    tune = sim.autotune(10)
+    print("# autotune:\n" + "".join("#   " + l + "\n" for l in tune.strip().splitlines()), file=sys.stderr, flush=True)
+    sim.run(args.warmup)
+    sim.getMaxSpeed()   # synchronises
+    t = time.perf_counter()
+    sim.run(args.steps)
+    sim.getMaxSpeed()
+    dt = time.perf_counter() - t
+    ms_per_step = dt * 1e3 / args.steps
+    mlups = NX * NY * args.steps / dt / 1e6
+    print(f"# graph replay: {ms_per_step:.4f} ms/step, {mlups:.0f} MLUPS", file=sys.stderr, flush=True)
+
+    roof = None
+    kt = {}
+    if not args.no_timing_pass:
+        sim.setTiming(True)
+        t = time.perf_counter()
+        sim.run(args.steps)
+        sim.getMaxSpeed()
+        dt_eager = time.perf_counter() - t
+        kt = dict(sim.kernelTimes())
+        sim.setTiming(False)
+        # with the scalar on, the collide-stream kernel also carries the scalar
+        # step: 9 scalar distributions read + 9 written per cell on top
+        fused = "lbm+adr" in kt
+        lbm_ms = kt.get("lbm+adr", kt.get("lbm"))
+        bpl = LBM_BYTES_PER_LUP + (ADR_BYTES_PER_LUP if fused else 0)
+        if lbm_ms:
+            achieved = bpl * NX * NY / (lbm_ms * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "k_lbm<MRT,fp32,channel,scalar> (fused collide + push-stream "
+                              "+ lagged scalar step)" if fused else "k_lbm<MRT,fp32> (fused collide + push-stream)",
+                    "kernel_ms": round(lbm_ms, 5), "bytes_per_lup": bpl}
+        print(f"# event-timed pass: {dt_eager * 1e3 / args.steps:.4f} ms/step; per-kernel ms: "
+              + ", ".join(f"{k}={v:.4f}" for k, v in sorted(kt.items())), file=sys.stderr, flush=True)
+
+    stats = sim.fieldStats()
+    cpu = None
+    if not args.no_cpu:
+        try:
+            v, cdt = cpu_baseline(case, args.cpu_steps)
+            cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+                   "sample": f"{args.cpu_steps} full steps of the same 4096x1024 + 2000-capsule "
+                             f"workload in the FP64 oracle ({cdt:.1f} s)"}
+        except Exception as e:  # the oracle library is test infrastructure; report its absence
+            cpu = {"value": None, "unit": "MLUPS", "cores": 1, "kind": "port", "sample": f"unavailable: {e}"}
+
+    out = {
+        "metric": "MLUPS (whole step), 4096x1024 + 2000 capsules",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "C3 large single domain: 4096x1024 MRT tau=0.7 fx=1e-5, 2000 bidisperse "
+                               "Skalak+Helfrich capsules, ADR D=0.05; FP32 lattice storage",
+                   "nx": NX, "ny": NY, "capsules": 2000, "membrane_nodes": 800 * 38 + 1200 * 19},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernel_ms": {k: round(v, 5) for k, v in sorted(kt.items())},
+        "lbm_launch": tune.strip().splitlines()[-1],
+        "mass": stats[0],
+        "max_speed": stats[3],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

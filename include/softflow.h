@@ -145,6 +145,13 @@ int  sf_kernel_times(sf_ctx* ctx, int max, const char** names, double* ms,
                      int* count);
 int  sf_set_timing(sf_ctx* ctx, int on);
 
+/* Autotuner (no counterpart in the reference): times the collide-stream
+ * kernel in place over launch shapes (block width x height, XCD-banded or
+ * linear workgroup order) on the current state -- the kernel only reads its
+ * source buffers, so repeated launches leave the state unchanged -- and keeps
+ * the fastest.  report (may be NULL) receives one line per candidate. */
+int  sf_autotune(sf_ctx* ctx, int reps, char* report, int report_len);
+
 /* Device math probe used by the tests: op 0 sqrt, 1 divide (a/b), 2 atan2(a,b),
  * 3 pow(a, b) with integer b as used by the contact force. */
 int  sf_probe_math(int op, int n, const double* a, const double* b, double* out);

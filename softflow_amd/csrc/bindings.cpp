@@ -281,7 +281,13 @@ PYBIND11_MODULE(softflow_core, m) {
             if (!d) throw py::index_error("no such species");
             return view2d(d, a.getNy(), a.getNx(), self);
         }, py::arg("species") = 0)
-        .def("getNumSpecies", &AdvectionDiffusion::getNumSpecies);
+        .def("getNumSpecies", &AdvectionDiffusion::getNumSpecies)
+        .def("setField", [](AdvectionDiffusion& a, py::array_t<Real, py::array::c_style | py::array::forcecast> C,
+                            int species) {
+            if (C.size() != static_cast<py::ssize_t>(a.getNx()) * a.getNy())
+                throw py::value_error("concentration array must have ny*nx entries");
+            a.setField(C.data(), species);
+        }, py::arg("C"), py::arg("species") = 0);
 
     py::class_<ChannelBuilder>(m, "ChannelBuilder")
         .def(py::init<int, int>(), py::arg("nx"), py::arg("ny"))
@@ -337,6 +343,7 @@ PYBIND11_MODULE(softflow_core, m) {
             });
         }, py::arg("callback"))
         .def("setTiming", &Simulation::setTiming, py::arg("on"))
+        .def("autotune", &Simulation::autotune, py::arg("reps") = 10)
         .def("kernelTimes", &Simulation::kernelTimes);
 
     m.def("device_count", []() {

@@ -327,6 +327,15 @@ void AdvectionDiffusion::setPoint(int x, int y, Real c, int s) {                
     for (int q = 0; q < Q; ++q) g_[s][n * Q + q] = W[q] * c;
     dirty_ = true;
 }
+void AdvectionDiffusion::setField(const Real* C, int s) {
+    if (s < 0 || s >= nsp_) return;
+    pull();
+    for (size_t n = 0; n < C_[s].size(); ++n) {
+        C_[s][n] = C[n];
+        for (int q = 0; q < Q; ++q) g_[s][n * Q + q] = W[q] * C[n];
+    }
+    dirty_ = true;
+}
 Real AdvectionDiffusion::getConcentration(int x, int y, int s) {
     if (x < 0 || x >= nx_ || y < 0 || y >= ny_ || s < 0 || s >= nsp_) return 0.0;
     pull();
@@ -656,6 +665,13 @@ void Simulation::setFluidRegion(int x0, int y0, int x1, int y1, Real rho, Real u
 
 void Simulation::setScalarRegion(int x0, int y0, int x1, int y1, Real c, int species) {
     if (adr_) adr_->setRegion(x0, y0, x1, y1, c, species);
+}
+
+std::string Simulation::autotune(int reps) {
+    pushDirty();
+    std::vector<char> buf(4096, 0);
+    check(sf_autotune(ctx_, reps, buf.data(), static_cast<int>(buf.size())), "sf_autotune");
+    return std::string(buf.data());
 }
 
 void Simulation::setTiming(bool on) { if (ctx_) check(sf_set_timing(ctx_, on ? 1 : 0), "sf_set_timing"); }

@@ -240,6 +240,8 @@ public:
     void setPoint(int x, int y, Real c, int species = 0);
     Real getConcentration(int x, int y, int species = 0);
     Real* concentrationData(int species = 0);
+    // bulk form of setPoint over the whole lattice (C[y*nx+x], g = w_q C)
+    void setField(const Real* C, int species = 0);
     int getNx() const { return nx_; }
     int getNy() const { return ny_; }
     int getNumSpecies() const { return nsp_; }
@@ -304,6 +306,7 @@ public:
     bool initialized() const { return ctx_ != nullptr; }
     std::vector<std::pair<std::string, double>> kernelTimes();
     void setTiming(bool on);
+    std::string autotune(int reps = 10);
 
     // device <-> host synchronisation (used by the mirrors)
     void pullField(LatticeField& f);

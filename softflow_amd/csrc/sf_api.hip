@@ -90,7 +90,7 @@ struct Scope {
     Ctx& c;
     TimingPool::Rec rec{};
     bool on;
-    Scope(Ctx& cc, const char* name) : c(cc), on(cc.timing && !cc.graph_mode) {
+    Scope(Ctx& cc, const char* name) : c(cc), on(cc.timing) {
         if (!on) return;
         TimingPool& P = pools()[&c];
         if (P.freelist.empty()) {
@@ -136,12 +136,13 @@ static void step_forces(Ctx& c) {
 
 static int step_ibm(Ctx& c) {
     int shift;
-    { Scope s(c, "spread"); launch_spread(c, 0); }
+    const bool mdf = c.p.ibm_iterations > 1 && c.K > 0;
+    { Scope s(c, "spread"); launch_spread(c, 0, !mdf); }
     if (c.p.ibm_iterations > 1) {
         if (c.K > 0) {
             for (int it = 1; it < c.p.ibm_iterations; ++it) {
                 { Scope s(c, "mdf_interp"); launch_mdf_interp(c); }
-                { Scope s(c, "mdf_spread"); launch_spread(c, 1); }
+                { Scope s(c, "mdf_spread"); launch_spread(c, 1, it == c.p.ibm_iterations - 1); }
             }
             shift = SHIFT_PARTIAL;
         } else {
@@ -153,16 +154,40 @@ static int step_ibm(Ctx& c) {
     return shift;
 }
 
+// Run a scalar step that is still pending (it needs this step's populations
+// and the collision force of the step that produced them, F[Fcur]).
+static void flush_adr(Ctx& c) {
+    if (c.nsp > 0 && c.adr_pending) {
+        Scope s(c, "adr");
+        launch_adr(c, c.Fcur);
+        c.adr_pending = false;
+    }
+}
+
 static void one_step(Ctx& c) {
+    // The scalar step n-1 runs inside LBM step n when the collision velocity
+    // of step n is the one that scalar step needs: macro(f_n, F_{n-1}), i.e.
+    // the shift force is the previous step's collision force.
+    const bool fuse_ok = c.nsp > 0 && c.p.ibm_iterations <= 1 && c.shift_next == SHIFT_PREV;
+    if (!fuse_ok) flush_adr(c);
     // rotate the raw-force buffers: this step's force goes into the spare one
     const int spare = 3 - c.Fcur - c.Fprev;
     c.Fprev = c.Fcur;
     c.Fcur = spare;
-    launch_fill_int(c, c.counters, 1, 0);
+    if (c.ncaps == 0) launch_fill_int(c, c.counters, 1, 0);   // else the membrane kernel clears it
     step_forces(c);
     const int shift = step_ibm(c);
-    { Scope s(c, "lbm"); launch_lbm(c, shift); }
-    if (c.nsp > 0) { Scope s(c, "adr"); launch_adr(c); }
+    const bool fuse = fuse_ok && c.adr_pending;
+    { Scope s(c, fuse ? "lbm+adr" : "lbm"); launch_lbm(c, shift, fuse); }
+    if (c.nsp > 0) {
+        if (c.p.ibm_iterations <= 1) {
+            c.adr_pending = true;
+        } else {
+            Scope s(c, "adr");
+            launch_adr(c, c.Fcur);
+            c.adr_pending = false;
+        }
+    }
     if (c.ncaps > 0) { Scope s(c, "interp_move"); launch_interp_move(c); }
     c.shift_next = SHIFT_PREV;
     c.step_count++;
@@ -174,7 +199,8 @@ static void one_step(Ctx& c) {
 static int run_graph(Ctx& c, int nsteps) {
     const int period = 6;
     while (nsteps > 0) {
-        if (c.shift_next != SHIFT_PREV || nsteps < period || (c.nsp > 0 && c.c_from_store)) {
+        if (c.shift_next != SHIFT_PREV || nsteps < period ||
+            (c.nsp > 0 && (c.c_from_store || (c.p.ibm_iterations <= 1 && !c.adr_pending)))) {
             one_step(c);
             --nsteps;
             continue;
@@ -350,7 +376,7 @@ int sf_destroy(sf_ctx* h) {
     invalidate_graph(c);
     free_caps(c);
     void* ptrs[] = {c.flags, c.f[0], c.f[1], c.F[0], c.F[1], c.F[2], c.stored_rho, c.stored_u,
-                    c.counters, c.phi, c.tile_cnt, c.tile_list, c.cell_cnt, c.cell_list};
+                    c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_dirty, c.cell_cnt, c.cell_list};
     for (void* p : ptrs) if (p) hipFree(p);
     for (int s = 0; s < 4; ++s) {
         for (int b = 0; b < 2; ++b) if (c.g[s][b]) hipFree(c.g[s][b]);
@@ -506,11 +532,17 @@ int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_d
     if (!c.tile_cnt) {
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_cnt), c.ntiles * sizeof(int)));
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_list), static_cast<size_t>(c.ntiles) * TILE_MAXL * sizeof(int)));
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_dirty), 3 * static_cast<size_t>(c.ntiles)));
+        SF_HIP(hipMemset(c.tile_cnt, 0, c.ntiles * sizeof(int)));
+        // the force buffers may hold anything from before capsules existed
+        SF_HIP(hipMemset(c.tile_dirty, 1, 3 * static_cast<size_t>(c.ntiles)));
     }
     if (!c.cell_cnt && K > 100) {
         const size_t ncells = static_cast<size_t>(c.ncx) * c.ncy;
-        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_cnt), ncells * sizeof(int)));
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_cnt), 2 * ncells * sizeof(int)));
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_list), ncells * CELL_MAXC * sizeof(int)));
+        SF_HIP(hipMemset(c.cell_cnt, 0, 2 * ncells * sizeof(int)));
+        c.cell_cur = 0;
     }
     return SF_OK;
 }
@@ -561,6 +593,7 @@ int sf_set_scalar(sf_ctx* h, int sp, const double* g, const double* C) {
     Ctx& c = h->c;
     if (sp < 0 || sp >= c.nsp) return fail(SF_E_ARG, "no such scalar species");
     SF_HIP(hipSetDevice(c.device));
+    flush_adr(c);
     SF_HIP(hipStreamSynchronize(c.stream));
     const size_t N = c.N;
     if (!c.fp32) {
@@ -583,6 +616,7 @@ int sf_get_scalar(sf_ctx* h, int sp, double* C, double* g) {
     Ctx& c = h->c;
     if (sp < 0 || sp >= c.nsp) return fail(SF_E_ARG, "no such scalar species");
     SF_HIP(hipSetDevice(c.device));
+    flush_adr(c);
     const size_t N = c.N;
     if (C) {
         if (c.c_from_store) {
@@ -659,13 +693,13 @@ int sf_stage(sf_ctx* h, int stage) {
         const int spare = 3 - c.Fcur - c.Fprev;
         c.Fprev = c.Fcur;
         c.Fcur = spare;
-        launch_fill_int(c, c.counters, 1, 0);
+        if (c.ncaps == 0) launch_fill_int(c, c.counters, 1, 0);
         step_forces(c);
         break;
     }
     case SF_STAGE_IBM: c.pending_shift = step_ibm(c); break;
-    case SF_STAGE_LBM: launch_lbm(c, c.pending_shift); break;
-    case SF_STAGE_SCALAR: if (c.nsp) launch_adr(c); break;
+    case SF_STAGE_LBM: flush_adr(c); launch_lbm(c, c.pending_shift, false); break;
+    case SF_STAGE_SCALAR: if (c.nsp) launch_adr(c, c.Fcur); c.adr_pending = false; break;
     case SF_STAGE_MOVE:
         if (c.ncaps) launch_interp_move(c);
         c.shift_next = SHIFT_PREV;
@@ -729,6 +763,60 @@ int sf_kernel_times(sf_ctx* h, int max, const char** names, double* ms, int* cou
         ++i;
     }
     *count = i;
+    return SF_OK;
+}
+
+int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    if (!c.lattice_ready) return fail(SF_E_STATE, "populations not set");
+    SF_HIP(hipSetDevice(c.device));
+    if (reps < 1) reps = 10;
+    // the steady-state launch: previous-step shift force, scalar riding along
+    const bool fuse = c.nsp > 0 && c.p.ibm_iterations <= 1;
+    const int shift = c.p.ibm_iterations > 1 ? SHIFT_PARTIAL : SHIFT_PREV;
+    // every candidate must fit the kernel's __launch_bounds__(256)
+    static const int shapes[][2] = {{64, 4}, {64, 2}, {64, 1}, {128, 2}, {128, 1},
+                                    {256, 1}, {32, 8}, {32, 4}};
+    hipEvent_t e0, e1;
+    SF_HIP(hipEventCreate(&e0));
+    SF_HIP(hipEventCreate(&e1));
+    std::string rep;
+    double best = 1e30;
+    int bbx = c.lbm_bx, bby = c.lbm_by, brm = c.lbm_remap;
+    SF_HIP(hipGetLastError());
+    for (const auto& sh : shapes) {
+        if (sh[0] * sh[1] > 256) continue;
+        for (int remap = 0; remap <= 1; ++remap) {
+            launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap);   // warm
+            SF_HIP(hipEventRecord(e0, c.stream));
+            for (int r = 0; r < reps; ++r) launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap);
+            SF_HIP(hipEventRecord(e1, c.stream));
+            SF_HIP(hipEventSynchronize(e1));
+            SF_HIP(hipGetLastError());
+            float ms = 0.f;
+            SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const double per = ms / reps;
+            char line[96];
+            std::snprintf(line, sizeof(line), "%dx%d %s %.4f ms\n", sh[0], sh[1], remap ? "xcd-banded" : "linear", per);
+            rep += line;
+            if (per < best) { best = per; bbx = sh[0]; bby = sh[1]; brm = remap; }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    SF_HIP(hipGetLastError());
+    launch_fill_int(c, c.counters, 1, 0);
+    c.lbm_bx = bbx; c.lbm_by = bby; c.lbm_remap = brm;
+    invalidate_graph(c);
+    char line[96];
+    std::snprintf(line, sizeof(line), "chosen %dx%d %s\n", bbx, bby, brm ? "xcd-banded" : "linear");
+    rep += line;
+    if (report && report_len > 0) {
+        std::strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    SF_HIP(hipStreamSynchronize(c.stream));
     return SF_OK;
 }
 

@@ -8,6 +8,8 @@
 // cell's force is therefore the same sequential sum the reference forms
 // (coupling/immersed_boundary.cpp:44-80 at one thread), independent of the
 // launch and of how nodes were appended to the lists.
+#include <type_traits>
+
 #include "sf_internal.h"
 
 namespace sf {
@@ -101,24 +103,53 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
 
 // Gather: one 256-thread workgroup per TILE_X x TILE_Y tile.
 // mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction).
+// Tiles no node touches are skipped when the output buffer already holds zeros
+// there (dirty flags per buffer), so the dense write covers touched tiles only.
+constexpr int SPREAD_CHUNK = 256;   // node records staged in LDS at a time
+
 template <bool FP32>
-__global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const int* tile_cnt,
+__global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile_cnt,
                                                 const int* tile_list, const double* lut,
-                                                const void* Fin, void* Fout, int nx, int ny,
-                                                int ntx, int mode) {
+                                                const void* Fin, void* Fout,
+                                                const uint8_t* dirty_in, uint8_t* dirty_out,
+                                                int nx, int ny, int ntx, int mode, int last) {
     __shared__ int list[TILE_MAXL];
     __shared__ double tab[1025];
+    __shared__ double spx[SPREAD_CHUNK], spy[SPREAD_CHUNK], sfx[SPREAD_CHUNK], sfy[SPREAD_CHUNK],
+        sds[SPREAD_CHUNK];
+    __shared__ int si0[SPREAD_CHUNK], sj0[SPREAD_CHUNK];
+    using T = typename std::conditional<FP32, float, double>::type;
     const int tile = blockIdx.x;
     const int t = threadIdx.x;
     int L = tile_cnt[tile];
     if (L > TILE_MAXL) L = TILE_MAXL;
+    const int tx0 = (tile % ntx) * TILE_X, ty0 = (tile / ntx) * TILE_Y;
+    const int cx = tx0 + (t % TILE_X);
+    const int cy = ty0 + (t / TILE_X);
+    const bool inside = cx < nx && cy < ny;
+    const size_t n = static_cast<size_t>(cy) * nx + cx;
+    if (L == 0) {
+        const bool copy = mode == 1 && dirty_in[tile];
+        const bool clear = !copy && dirty_out[tile];
+        if (inside && (copy || clear)) {
+            T vx = 0, vy = 0;
+            if (copy) { vx = static_cast<const T*>(Fin)[2 * n]; vy = static_cast<const T*>(Fin)[2 * n + 1]; }
+            static_cast<T*>(Fout)[2 * n] = vx;
+            static_cast<T*>(Fout)[2 * n + 1] = vy;
+        }
+        if (t == 0) dirty_out[tile] = copy ? 1 : 0;
+        return;
+    }
     int P = 1;
     while (P < L) P <<= 1;
     for (int i = t; i < P; i += 256)
         list[i] = i < L ? tile_list[static_cast<size_t>(tile) * TILE_MAXL + i] : 0x7fffffff;
-    if (L > 0)
-        for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
+    for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
     __syncthreads();
+    if (t == 0) {
+        if (last) tile_cnt[tile] = 0;
+        dirty_out[tile] = 1;
+    }
     // bitonic sort of the node list (ascending global node index)
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -133,45 +164,45 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const int
             __syncthreads();
         }
     }
-    const int cx = (tile % ntx) * TILE_X + (t % TILE_X);
-    const int cy = (tile / ntx) * TILE_Y + (t / TILE_X);
-    if (cx >= nx || cy >= ny) return;
-    const int n = cy * nx + cx;
     double ax = 0.0, ay = 0.0;
-    if (mode == 1) {
-        if (FP32) {
-            const float* p = static_cast<const float*>(Fin) + 2 * static_cast<size_t>(n);
-            ax = p[0]; ay = p[1];
-        } else {
-            const double* p = static_cast<const double*>(Fin) + 2 * static_cast<size_t>(n);
-            ax = p[0]; ay = p[1];
-        }
+    if (mode == 1 && inside) {
+        ax = static_cast<const T*>(Fin)[2 * n];
+        ay = static_cast<const T*>(Fin)[2 * n + 1];
     }
-    for (int i = 0; i < L; ++i) {
-        const NodeSpread s = rec[list[i]];
-        if (!s.valid) continue;
-        const int dyc = cy - (s.j0 - 1);
-        if (dyc < 0 || dyc > 3) continue;
-        // unwrapped stencil column whose single wrap lands on cx
-        int ix;
-        const int d0 = cx - (s.i0 - 1);
-        if (d0 >= 0 && d0 <= 3) ix = cx;
-        else if (cx - nx - (s.i0 - 1) >= 0 && cx - nx - (s.i0 - 1) <= 3 && cx - nx < 0) ix = cx - nx;
-        else if (cx + nx - (s.i0 - 1) >= 0 && cx + nx - (s.i0 - 1) <= 3 && cx + nx >= nx) ix = cx + nx;
-        else continue;
-        const double w = phi_lut(tab, s.px - static_cast<double>(ix)) *
-                         phi_lut(tab, s.py - static_cast<double>(cy));
-        if (w > 1e-15) {
-            ax += s.fx * w * s.ds;
-            ay += s.fy * w * s.ds;
+    for (int base = 0; base < L; base += SPREAD_CHUNK) {
+        const int m = min(SPREAD_CHUNK, L - base);
+        if (t < m) {
+            const NodeSpread s = rec[list[base + t]];
+            spx[t] = s.px; spy[t] = s.py; sfx[t] = s.fx; sfy[t] = s.fy; sds[t] = s.ds;
+            // invalid records never touch a row
+            si0[t] = s.i0;
+            sj0[t] = s.valid ? s.j0 : -0x40000000;
         }
+        __syncthreads();
+        if (inside) {
+            for (int i = 0; i < m; ++i) {
+                const int dyc = cy - (sj0[i] - 1);
+                if (dyc < 0 || dyc > 3) continue;
+                // unwrapped stencil column whose single wrap lands on cx
+                const int lo = si0[i] - 1;
+                int ix;
+                if (cx - lo >= 0 && cx - lo <= 3) ix = cx;
+                else if (cx - nx - lo >= 0 && cx - nx - lo <= 3) ix = cx - nx;
+                else if (cx + nx - lo >= 0 && cx + nx - lo <= 3) ix = cx + nx;
+                else continue;
+                const double w = phi_lut(tab, spx[i] - static_cast<double>(ix)) *
+                                 phi_lut(tab, spy[i] - static_cast<double>(cy));
+                if (w > 1e-15) {
+                    ax += sfx[i] * w * sds[i];
+                    ay += sfy[i] * w * sds[i];
+                }
+            }
+        }
+        __syncthreads();
     }
-    if (FP32) {
-        float* p = static_cast<float*>(Fout) + 2 * static_cast<size_t>(n);
-        p[0] = static_cast<float>(ax); p[1] = static_cast<float>(ay);
-    } else {
-        double* p = static_cast<double*>(Fout) + 2 * static_cast<size_t>(n);
-        p[0] = ax; p[1] = ay;
+    if (inside) {
+        static_cast<T*>(Fout)[2 * n] = static_cast<T>(ax);
+        static_cast<T*>(Fout)[2 * n + 1] = static_cast<T>(ay);
     }
 }
 
@@ -243,31 +274,49 @@ __device__ __forceinline__ void cell_velocity(const CellVelArgs& a, int x, int y
     }
 }
 
-// Interpolation at a point (immersed_boundary.cpp:191-229).
+// Interpolation at a point (immersed_boundary.cpp:191-229) with 16 lanes per
+// node: lane l handles stencil cell (iy, ix) = (l / 4, l % 4); the terms are
+// summed in the reference's loop order (iy outer, ix inner) by the group's
+// first lane, so the result is the reference's sequential sum.
+constexpr int INTERP_NODES = 16;   // nodes per 256-thread workgroup
+
 template <bool FP32>
-__device__ __forceinline__ void interp_point(const CellVelArgs& a, const double* lut, double px,
-                                             double py, int force_mode, double& Ux, double& Uy) {
-    Ux = 0.0; Uy = 0.0;
-    if (isnan(px) || isnan(py) || isinf(px) || isinf(py)) return;
-    const int ix0 = static_cast<int>(floor(px));
-    const int iy0 = static_cast<int>(floor(py));
-    if (iy0 < -2 || iy0 >= a.ny + 2) return;
-    for (int iy = iy0 - 1; iy <= iy0 + 2; ++iy) {
-        for (int ix = ix0 - 1; ix <= ix0 + 2; ++ix) {
-            int jx = ix;
-            if (jx < 0) jx += a.nx;
-            if (jx >= a.nx) jx -= a.nx;
-            const int jy = iy;
-            if (jy < 0 || jy >= a.ny) continue;
+__device__ __forceinline__ void interp_group(const CellVelArgs& a, const double* lut, double px,
+                                             double py, int force_mode, int l, double* tx,
+                                             double* ty, int* use, double& Ux, double& Uy) {
+    // tx/ty/use: this node's 16-entry slots in LDS
+    bool ok = !(isnan(px) || isnan(py) || isinf(px) || isinf(py));
+    int ix0 = 0, iy0 = 0;
+    if (ok) {
+        ix0 = static_cast<int>(floor(px));
+        iy0 = static_cast<int>(floor(py));
+        if (iy0 < -2 || iy0 >= a.ny + 2) ok = false;
+    }
+    int flag = 0;
+    double vx = 0.0, vy = 0.0;
+    if (ok) {
+        const int ix = ix0 - 1 + (l & 3), iy = iy0 - 1 + (l >> 2);
+        int jx = ix;
+        if (jx < 0) jx += a.nx;
+        if (jx >= a.nx) jx -= a.nx;
+        if (iy >= 0 && iy < a.ny) {
             const double w = phi_lut(lut, px - static_cast<double>(ix)) *
                              phi_lut(lut, py - static_cast<double>(iy));
             if (w > 1e-15) {
                 double r, ux, uy;
-                cell_velocity<FP32>(a, jx, jy, force_mode, r, ux, uy);
-                Ux += ux * w;
-                Uy += uy * w;
+                cell_velocity<FP32>(a, jx, iy, force_mode, r, ux, uy);
+                vx = ux * w;
+                vy = uy * w;
+                flag = 1;
             }
         }
+    }
+    tx[l] = vx; ty[l] = vy; use[l] = flag;
+    __syncthreads();
+    Ux = 0.0; Uy = 0.0;
+    if (l == 0) {
+        for (int m = 0; m < 16; ++m)
+            if (use[m]) { Ux += tx[m]; Uy += ty[m]; }
     }
 }
 
@@ -275,13 +324,18 @@ __device__ __forceinline__ void interp_point(const CellVelArgs& a, const double*
 // dF = 2 rho_l (V_prev - U(X)) with U from the populations and the raw force
 // spread so far; written into the node's spread record.
 template <bool FP32>
-__global__ void k_mdf_interp(CellVelArgs a, const double* lut, const double* x, const double* y,
-                             const double* vx, const double* vy, NodeSpread* rec, int K) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    const double px = x[k], py = y[k];
+__global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double* lut, const double* x,
+                                                    const double* y, const double* vx, const double* vy,
+                                                    NodeSpread* rec, int K) {
+    __shared__ double tx[256], ty[256];
+    __shared__ int use[256];
+    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int k = blockIdx.x * INTERP_NODES + g;
+    const bool live = k < K;
+    const double px = live ? x[k] : 0.0, py = live ? y[k] : 0.0;
     double Ux, Uy;
-    interp_point<FP32>(a, lut, px, py, 0, Ux, Uy);
+    interp_group<FP32>(a, lut, px, py, 0, l, tx + 16 * g, ty + 16 * g, use + 16 * g, Ux, Uy);
+    if (!live || l != 0) return;
     const double dux = vx[k] - Ux, duy = vy[k] - Uy;
     double rl = 1.0;
     int ix = static_cast<int>(px + 0.5);
@@ -301,14 +355,20 @@ __global__ void k_mdf_interp(CellVelArgs a, const double* lut, const double* x, 
 // interpolateVelocity (immersed_boundary.cpp:174-189) with the post-step
 // velocity u = macro(f_{n+1}, F_n).
 template <bool FP32>
-__global__ void k_interp(CellVelArgs a, const double* lut, const double* x, const double* y,
-                         double* vx, double* vy, int K) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
+__global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut, const double* x,
+                                                const double* y, double* vx, double* vy, int K) {
+    __shared__ double tx[256], ty[256];
+    __shared__ int use[256];
+    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int k = blockIdx.x * INTERP_NODES + g;
+    const bool live = k < K;
     double Ux, Uy;
-    interp_point<FP32>(a, lut, x[k], y[k], 1, Ux, Uy);
-    vx[k] = Ux;
-    vy[k] = Uy;
+    interp_group<FP32>(a, lut, live ? x[k] : 0.0, live ? y[k] : 0.0, 1, l, tx + 16 * g, ty + 16 * g,
+                       use + 16 * g, Ux, Uy);
+    if (live && l == 0) {
+        vx[k] = Ux;
+        vy[k] = Uy;
+    }
 }
 
 // moveAllNodes + y-clamp (capsule.cpp:267-297, simulation.cpp:665-675); one
@@ -396,13 +456,12 @@ static CellVelArgs vel_args(Ctx& c, const void* f, const void* F) {
 // mode 0: bin nodes and spread node forces into F[Fcur] (overwrite).
 // mode 1: spread the MDF corrections on top of F[Fcur] into the spare buffer
 //         and make it current; the previous contents become F[Fpart].
-void launch_spread(Ctx& c, int mode) {
+void launch_spread(Ctx& c, int mode, bool last) {
     if (c.K == 0) {
         if (mode == 0) launch_fill_zero(c, c.F[c.Fcur], (c.fp32 ? 8 : 16) * static_cast<size_t>(c.N));
         return;
     }
     if (mode == 0) {
-        launch_fill_int(c, c.tile_cnt, c.ntiles, 0);
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                            c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
                            c.tile_cnt, c.tile_list, c.counters, 0);
@@ -415,12 +474,16 @@ void launch_spread(Ctx& c, int mode) {
     // mode 1 writes into the buffer that is neither current nor previous
     const int out = mode == 1 ? 3 - c.Fcur - c.Fprev : c.Fcur;
     void* Fout = c.F[out];
+    const uint8_t* din = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
+    uint8_t* dout = c.tile_dirty + static_cast<size_t>(out) * c.ntiles;
     if (c.fp32)
         hipLaunchKernelGGL((k_spread<true>), dim3(c.ntiles), dim3(256), 0, c.stream, c.spread,
-                           c.tile_cnt, c.tile_list, c.phi, Fin, Fout, c.nx, c.ny, c.ntx, mode);
+                           c.tile_cnt, c.tile_list, c.phi, Fin, Fout, din, dout, c.nx, c.ny, c.ntx,
+                           mode, last ? 1 : 0);
     else
         hipLaunchKernelGGL((k_spread<false>), dim3(c.ntiles), dim3(256), 0, c.stream, c.spread,
-                           c.tile_cnt, c.tile_list, c.phi, Fin, Fout, c.nx, c.ny, c.ntx, mode);
+                           c.tile_cnt, c.tile_list, c.phi, Fin, Fout, din, dout, c.nx, c.ny, c.ntx,
+                           mode, last ? 1 : 0);
     if (mode == 1) {
         c.Fpart = c.Fcur;
         c.Fcur = out;
@@ -430,23 +493,25 @@ void launch_spread(Ctx& c, int mode) {
 void launch_mdf_interp(Ctx& c) {
     if (c.K == 0) return;
     CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
+    const dim3 grid((c.K + INTERP_NODES - 1) / INTERP_NODES);
     if (c.fp32)
-        hipLaunchKernelGGL((k_mdf_interp<true>), dim3((c.K + 255) / 256), dim3(256), 0, c.stream, a,
-                           c.phi, c.x, c.y, c.vx, c.vy, c.spread, c.K);
+        hipLaunchKernelGGL((k_mdf_interp<true>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y,
+                           c.vx, c.vy, c.spread, c.K);
     else
-        hipLaunchKernelGGL((k_mdf_interp<false>), dim3((c.K + 255) / 256), dim3(256), 0, c.stream, a,
-                           c.phi, c.x, c.y, c.vx, c.vy, c.spread, c.K);
+        hipLaunchKernelGGL((k_mdf_interp<false>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y,
+                           c.vx, c.vy, c.spread, c.K);
 }
 
 void launch_interp_move(Ctx& c) {
     if (c.K == 0) return;
     CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
+    const dim3 grid((c.K + INTERP_NODES - 1) / INTERP_NODES);
     if (c.fp32)
-        hipLaunchKernelGGL((k_interp<true>), dim3((c.K + 255) / 256), dim3(256), 0, c.stream, a,
-                           c.phi, c.x, c.y, c.vx, c.vy, c.K);
+        hipLaunchKernelGGL((k_interp<true>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, c.vx,
+                           c.vy, c.K);
     else
-        hipLaunchKernelGGL((k_interp<false>), dim3((c.K + 255) / 256), dim3(256), 0, c.stream, a,
-                           c.phi, c.x, c.y, c.vx, c.vy, c.K);
+        hipLaunchKernelGGL((k_interp<false>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, c.vx,
+                           c.vy, c.K);
     hipLaunchKernelGGL(k_move, dim3(c.ncaps), dim3(64), 0, c.stream, c.x, c.y, c.vx, c.vy, c.off,
                        c.desc, c.ncaps, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
                        1.5, static_cast<double>(c.ny) - 1.5);

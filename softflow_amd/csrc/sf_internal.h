@@ -74,6 +74,9 @@ struct Ctx {
     int gcur = 0;
     double* Cstore[4] = {};
     bool c_from_store = true;
+    // the scalar step of the last LBM step has not run yet: it rides along with
+    // the next LBM launch, or is flushed before anyone reads the scalar
+    bool adr_pending = false;
 
     // capsules
     int ncaps = 0, K = 0, max_nodes = 0;
@@ -89,14 +92,16 @@ struct Ctx {
     // IBM tiles
     double* phi = nullptr;               // 1025-entry table
     int ntx = 0, nty = 0, ntiles = 0;
-    int* tile_cnt = nullptr;
+    int* tile_cnt = nullptr;             // zero between steps (the last spread resets it)
     int* tile_list = nullptr;
+    uint8_t* tile_dirty = nullptr;       // [3][ntiles]: tile of force buffer b holds non-zeros
 
     // repulsion cells
     int ncx = 0, ncy = 0;
     double cdx = 1.0, cdy = 1.0, xorg = 0.0, Lx = 0.0;
-    int* cell_cnt = nullptr;
+    int* cell_cnt = nullptr;             // two count arrays, alternating by step
     int* cell_list = nullptr;
+    int cell_cur = 0;
 
     // counters: [0] capped cells, [1] tile overflow, [2] cell overflow
     int* counters = nullptr;
@@ -115,16 +120,21 @@ struct Ctx {
 
     int64_t step_count = 0;
     bool lattice_ready = false;
+
+    // collide-stream launch shape (chosen by sf_autotune)
+    int lbm_bx = 64, lbm_by = 4, lbm_remap = 1;
 };
 
 // -- launchers (defined in the .hip files) --------------------------------
-void launch_lbm(Ctx& c, int shift_mode);
-void launch_adr(Ctx& c);
+void launch_lbm(Ctx& c, int shift_mode, bool fuse_adr);
+// the same launch with an explicit shape and no change of buffer roles
+void launch_lbm_cfg(Ctx& c, int shift_mode, bool fuse_adr, int bx, int by, int remap);
+void launch_adr(Ctx& c, int Fidx);
 void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy);
 void launch_field_stats(Ctx& c, double* dev_out);
 void launch_membrane(Ctx& c);
 void launch_repulsion(Ctx& c);
-void launch_spread(Ctx& c, int mode);        // 0 node forces into F[Fcur]; 1 MDF correction
+void launch_spread(Ctx& c, int mode, bool last);   // 0 node forces into F[Fcur]; 1 MDF correction
 void launch_mdf_interp(Ctx& c);
 void launch_interp_move(Ctx& c);
 void launch_fill_int(Ctx& c, int* p, int n, int v);

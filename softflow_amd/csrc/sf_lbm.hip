@@ -236,7 +236,26 @@ struct LbmArgs {
     int has_body;
     double bfx, bfy, fmax, fmax2;
     double om, se, seps, sq;
+    // scalar transport riding along (the step before this one's)
+    const void* gsrc[4];
+    void* gdst[4];
+    const double* Cst[4];
+    int nsp, from_store, scalar_periodic_y;
+    double oms[4];
+    // launch geometry: 1-D grid over nbx x nby blocks of blockDim cells; with
+    // remap, workgroup b (dispatched to XCD b % 8) takes tile (b % 8) * per + b / 8,
+    // so each XCD streams one contiguous band of rows
+    int nbx, nblk, per_xcd, remap;
 };
+
+__device__ __forceinline__ bool block_origin(const LbmArgs& a, int& x, int& y) {
+    const int b = blockIdx.x;
+    const int t = a.remap ? (b % 8) * a.per_xcd + b / 8 : b;
+    if (t >= a.nblk) return false;
+    x = (t % a.nbx) * blockDim.x + threadIdx.x;
+    y = (t / a.nbx) * blockDim.y + threadIdx.y;
+    return true;
+}
 
 __device__ __forceinline__ bool is_solid(const LbmArgs& a, int x, int y, bool use_flags) {
     if (use_flags) return a.flags[y * a.nx + x] == SF_SOLID;
@@ -270,10 +289,63 @@ __device__ __forceinline__ int collision_force(const LbmArgs& a, const void* F, 
     return 0;
 }
 
-template <int OP, bool FP32, bool FLAGS>
+// One cell of the advection-diffusion step (advection_diffusion.cpp:111-223):
+// BGK with first-order equilibrium from C and the post-LBM velocity, then a
+// push with half-way bounce-back; x is always periodic for the scalar.
+template <bool FP32, bool FLAGS>
+__device__ __forceinline__ void adr_cell(const LbmArgs& a, int x, int y, int n, double ux, double uy) {
+    using T = typename std::conditional<FP32, float, double>::type;
+    const size_t N = static_cast<size_t>(a.N);
+    for (int s = 0; s < a.nsp; ++s) {
+        const T* gs = static_cast<const T*>(a.gsrc[s]);
+        T* gd = static_cast<T*>(a.gdst[s]);
+        T g[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = gs[q * N + n];
+        double C;
+        if (a.from_store) {
+            C = a.Cst[s][n];
+        } else {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) acc += static_cast<double>(g[q]);
+            C = acc;
+        }
+        const double oms = a.oms[s];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            if constexpr (!FP32) {
+                const double eu = static_cast<double>(dcx(q)) * ux + static_cast<double>(dcy(q)) * uy;
+                const double ge = dw(q) * C * (1.0 + eu * 3.0);
+                g[q] = g[q] - oms * (g[q] - ge);
+            } else {
+                const float eu = static_cast<float>(dcx(q) * ux + dcy(q) * uy);
+                const float ge = static_cast<float>(dw(q)) * static_cast<float>(C) * (1.0f + 3.0f * eu);
+                g[q] = g[q] - static_cast<float>(oms) * (g[q] - ge);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            int xd = x + dcx(q);
+            xd = xd < 0 ? xd + a.nx : (xd >= a.nx ? xd - a.nx : xd);
+            int yd = y + dcy(q);
+            bool bounce;
+            if (a.scalar_periodic_y) {
+                yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
+                bounce = is_solid(a, xd, yd, FLAGS);
+            } else {
+                bounce = (yd < 0 || yd >= a.ny) || is_solid(a, xd, yd, FLAGS);
+            }
+            if (bounce) gd[dopp(q) * N + n] = g[q];
+            else gd[q * N + (yd * a.nx + xd)] = g[q];
+        }
+    }
+}
+
+template <int OP, bool FP32, bool FLAGS, bool ADRF>
 __global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    int x, y;
+    if (!block_origin(a, x, y)) return;
     int capped = 0;
     if (x < a.nx && y < a.ny) {
         const int n = y * a.nx + x;
@@ -313,6 +385,7 @@ __global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
                     ux = 0.0;
                     uy = 0.0;
                 }
+                if constexpr (ADRF) adr_cell<false, FLAGS>(a, x, y, n, ux, uy);
                 if (OP == SF_REGULARIZED) col_reg(f, a.om, r, ux, uy, Fx, Fy);
                 else if (OP == SF_MRT) col_mrt(f, a.om, a.se, a.seps, a.sq, r, ux, uy, Fx, Fy);
                 else col_bgk(f, a.om, r, ux, uy, Fx, Fy);
@@ -334,6 +407,7 @@ __global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
                     ux = (jx + 0.5f * sx) * inv;
                     uy = (jy + 0.5f * sy) * inv;
                 }
+                if constexpr (ADRF) adr_cell<true, FLAGS>(a, x, y, n, static_cast<double>(ux), static_cast<double>(uy));
                 const float om = static_cast<float>(a.om);
                 if (OP == SF_REGULARIZED) dcol_reg(f, om, drho, rho, ux, uy, Fx, Fy);
                 else if (OP == SF_MRT)
@@ -369,13 +443,40 @@ __global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
         atomicAdd(&a.counters[0], __popcll(b));
 }
 
-template <int OP, bool FP32>
-static void lbm_dispatch_flags(Ctx& c, const LbmArgs& a, dim3 g, dim3 b) {
-    if (c.generic_flags) hipLaunchKernelGGL((k_lbm<OP, FP32, true>), g, b, 0, c.stream, a);
-    else hipLaunchKernelGGL((k_lbm<OP, FP32, false>), g, b, 0, c.stream, a);
+static void set_geometry(const Ctx& c, LbmArgs& a, int bx, int by, int remap, dim3& g, dim3& b) {
+    b = dim3(bx, by);
+    a.nbx = (c.nx + bx - 1) / bx;
+    const int nby = (c.ny + by - 1) / by;
+    a.nblk = a.nbx * nby;
+    a.per_xcd = (a.nblk + 7) / 8;
+    a.remap = remap;
+    g = dim3(remap ? 8 * a.per_xcd : a.nblk);
 }
 
-void launch_lbm(Ctx& c, int shift) {
+template <int OP, bool FP32>
+static void lbm_dispatch_flags(Ctx& c, const LbmArgs& a, dim3 g, dim3 b, bool adr) {
+    if (adr) {
+        if (c.generic_flags) hipLaunchKernelGGL((k_lbm<OP, FP32, true, true>), g, b, 0, c.stream, a);
+        else hipLaunchKernelGGL((k_lbm<OP, FP32, false, true>), g, b, 0, c.stream, a);
+    } else {
+        if (c.generic_flags) hipLaunchKernelGGL((k_lbm<OP, FP32, true, false>), g, b, 0, c.stream, a);
+        else hipLaunchKernelGGL((k_lbm<OP, FP32, false, false>), g, b, 0, c.stream, a);
+    }
+}
+
+static void scalar_args(Ctx& c, LbmArgs& a) {
+    a.nsp = c.nsp;
+    a.from_store = c.c_from_store ? 1 : 0;
+    a.scalar_periodic_y = c.p.scalar_periodic_y;
+    for (int s = 0; s < c.nsp; ++s) {
+        a.gsrc[s] = c.g[s][c.gcur];
+        a.gdst[s] = c.g[s][1 - c.gcur];
+        a.Cst[s] = c.Cstore[s];
+        a.oms[s] = 1.0 / (3.0 * c.p.diffusivity[s] + 0.5);
+    }
+}
+
+void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap) {
     LbmArgs a{};
     a.fsrc = c.f[c.fcur];
     a.fdst = c.f[1 - c.fcur];
@@ -394,18 +495,28 @@ void launch_lbm(Ctx& c, int shift) {
     a.fmax = c.p.max_lattice_force; a.fmax2 = a.fmax * a.fmax;
     a.om = 1.0 / c.p.tau;
     a.se = c.p.mrt_se; a.seps = c.p.mrt_s_eps; a.sq = c.p.mrt_sq;
-    dim3 b(64, 4), g((c.nx + 63) / 64, (c.ny + 3) / 4);
+    if (fuse_adr) scalar_args(c, a);
+    dim3 b, g;
+    set_geometry(c, a, bx, by, remap, g, b);
     const int op = c.p.collision_model;
     if (c.fp32) {
-        if (op == SF_REGULARIZED) lbm_dispatch_flags<SF_REGULARIZED, true>(c, a, g, b);
-        else if (op == SF_MRT) lbm_dispatch_flags<SF_MRT, true>(c, a, g, b);
-        else lbm_dispatch_flags<SF_BGK, true>(c, a, g, b);
+        if (op == SF_REGULARIZED) lbm_dispatch_flags<SF_REGULARIZED, true>(c, a, g, b, fuse_adr);
+        else if (op == SF_MRT) lbm_dispatch_flags<SF_MRT, true>(c, a, g, b, fuse_adr);
+        else lbm_dispatch_flags<SF_BGK, true>(c, a, g, b, fuse_adr);
     } else {
-        if (op == SF_REGULARIZED) lbm_dispatch_flags<SF_REGULARIZED, false>(c, a, g, b);
-        else if (op == SF_MRT) lbm_dispatch_flags<SF_MRT, false>(c, a, g, b);
-        else lbm_dispatch_flags<SF_BGK, false>(c, a, g, b);
+        if (op == SF_REGULARIZED) lbm_dispatch_flags<SF_REGULARIZED, false>(c, a, g, b, fuse_adr);
+        else if (op == SF_MRT) lbm_dispatch_flags<SF_MRT, false>(c, a, g, b, fuse_adr);
+        else lbm_dispatch_flags<SF_BGK, false>(c, a, g, b, fuse_adr);
     }
+}
+
+void launch_lbm(Ctx& c, int shift, bool fuse_adr) {
+    launch_lbm_cfg(c, shift, fuse_adr, c.lbm_bx, c.lbm_by, c.lbm_remap);
     c.fcur = 1 - c.fcur;
+    if (fuse_adr) {
+        c.gcur = 1 - c.gcur;
+        c.c_from_store = false;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -452,75 +563,20 @@ __device__ __forceinline__ void cell_macro(const void* fv, size_t N, int n, cons
     }
 }
 
-// Advection-diffusion step: lbm/advection_diffusion.cpp:84-223.  The scalar
-// distributions are SoA planes g[q][n]; SOLID cells are never written (their
-// g stays at its initial value in both buffers, as the reference's copy does).
-struct AdrArgs {
-    LbmArgs L;
-    const void* f;           // populations after the LBM step
-    const void* gsrc[4];
-    void* gdst[4];
-    const double* Cst[4];
-    int nsp, from_store, scalar_periodic_y;
-    double oms[4];
-    int zero_force;
-};
-
+// Stand-alone advection-diffusion step (advection_diffusion.cpp:84-223): the
+// velocity is recomputed from the populations after the LBM step and that
+// step's collision force.  SOLID cells are never written: their g stays at its
+// initial value in both buffers, as the reference's copy does.
 template <bool FP32, bool FLAGS>
-__global__ __launch_bounds__(256) void k_adr(AdrArgs a) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= a.L.nx || y >= a.L.ny) return;
-    const int n = y * a.L.nx + x;
-    if (is_solid(a.L, x, y, FLAGS)) return;
-    const size_t N = static_cast<size_t>(a.L.N);
+__global__ __launch_bounds__(256) void k_adr(LbmArgs a, const void* f) {
+    int x, y;
+    if (!block_origin(a, x, y)) return;
+    if (x >= a.nx || y >= a.ny) return;
+    const int n = y * a.nx + x;
+    if (is_solid(a, x, y, FLAGS)) return;
     double r, ux, uy;
-    cell_macro<FP32>(a.f, N, n, a.L.Fcur, a.L, a.zero_force != 0, r, ux, uy);
-    using T = typename std::conditional<FP32, float, double>::type;
-    for (int s = 0; s < a.nsp; ++s) {
-        const T* gs = static_cast<const T*>(a.gsrc[s]);
-        T* gd = static_cast<T*>(a.gdst[s]);
-        T g[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) g[q] = gs[q * N + n];
-        double C;
-        if (a.from_store) {
-            C = a.Cst[s][n];
-        } else {
-            double acc = 0.0;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) acc += static_cast<double>(g[q]);
-            C = acc;
-        }
-        const double oms = a.oms[s];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            if constexpr (!FP32) {
-                const double eu = static_cast<double>(dcx(q)) * ux + static_cast<double>(dcy(q)) * uy;
-                const double ge = dw(q) * C * (1.0 + eu * 3.0);
-                g[q] = g[q] - oms * (g[q] - ge);
-            } else {
-                const float eu = static_cast<float>(dcx(q) * ux + dcy(q) * uy);
-                const float ge = static_cast<float>(dw(q)) * static_cast<float>(C) * (1.0f + 3.0f * eu);
-                g[q] = g[q] - static_cast<float>(oms) * (g[q] - ge);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            int xd = x + dcx(q);
-            xd = xd < 0 ? xd + a.L.nx : (xd >= a.L.nx ? xd - a.L.nx : xd);
-            int yd = y + dcy(q);
-            bool bounce;
-            if (a.scalar_periodic_y) {
-                yd = yd < 0 ? yd + a.L.ny : (yd >= a.L.ny ? yd - a.L.ny : yd);
-                bounce = is_solid(a.L, xd, yd, FLAGS);
-            } else {
-                bounce = (yd < 0 || yd >= a.L.ny) || is_solid(a.L, xd, yd, FLAGS);
-            }
-            if (bounce) gd[dopp(q) * N + n] = g[q];
-            else gd[q * N + (yd * a.L.nx + xd)] = g[q];
-        }
-    }
+    cell_macro<FP32>(f, static_cast<size_t>(a.N), n, a.Fcur, a, false, r, ux, uy);
+    adr_cell<FP32, FLAGS>(a, x, y, n, ux, uy);
 }
 
 static LbmArgs view_args(Ctx& c) {
@@ -536,26 +592,19 @@ static LbmArgs view_args(Ctx& c) {
     return a;
 }
 
-void launch_adr(Ctx& c) {
-    AdrArgs a{};
-    a.L = view_args(c);
-    a.f = c.f[c.fcur];
-    a.nsp = c.nsp;
-    a.from_store = c.c_from_store ? 1 : 0;
-    a.scalar_periodic_y = c.p.scalar_periodic_y;
-    for (int s = 0; s < c.nsp; ++s) {
-        a.gsrc[s] = c.g[s][c.gcur];
-        a.gdst[s] = c.g[s][1 - c.gcur];
-        a.Cst[s] = c.Cstore[s];
-        a.oms[s] = 1.0 / (3.0 * c.p.diffusivity[s] + 0.5);
-    }
-    dim3 b(64, 4), g((c.nx + 63) / 64, (c.ny + 3) / 4);
+void launch_adr(Ctx& c, int Fidx) {
+    LbmArgs a = view_args(c);
+    a.Fcur = c.F[Fidx];
+    scalar_args(c, a);
+    const void* f = c.f[c.fcur];
+    dim3 b, g;
+    set_geometry(c, a, c.lbm_bx, c.lbm_by, c.lbm_remap, g, b);
     if (c.fp32) {
-        if (c.generic_flags) hipLaunchKernelGGL((k_adr<true, true>), g, b, 0, c.stream, a);
-        else hipLaunchKernelGGL((k_adr<true, false>), g, b, 0, c.stream, a);
+        if (c.generic_flags) hipLaunchKernelGGL((k_adr<true, true>), g, b, 0, c.stream, a, f);
+        else hipLaunchKernelGGL((k_adr<true, false>), g, b, 0, c.stream, a, f);
     } else {
-        if (c.generic_flags) hipLaunchKernelGGL((k_adr<false, true>), g, b, 0, c.stream, a);
-        else hipLaunchKernelGGL((k_adr<false, false>), g, b, 0, c.stream, a);
+        if (c.generic_flags) hipLaunchKernelGGL((k_adr<false, true>), g, b, 0, c.stream, a, f);
+        else hipLaunchKernelGGL((k_adr<false, false>), g, b, 0, c.stream, a, f);
     }
     c.gcur = 1 - c.gcur;
     c.c_from_store = false;

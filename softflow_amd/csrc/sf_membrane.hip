@@ -21,6 +21,7 @@ struct MemArgs {
     const int* off;
     const sf_capsule_desc* desc;
     int periodic_nx;
+    int* counters;    // counters[0] (capped cells of this step) is cleared here
 };
 
 __device__ __forceinline__ void mimg(double ax, double ay, double bx, double by, double Lx,
@@ -40,6 +41,7 @@ __global__ __launch_bounds__(64) void k_membrane(MemArgs a) {
     __shared__ double red[2];
     const int cap = blockIdx.x;
     const int lane = threadIdx.x;
+    if (cap == 0 && lane == 0) a.counters[0] = 0;
     const int o = a.off[cap], n = a.off[cap + 1] - o;
     const sf_capsule_desc d = a.desc[cap];
     const double Lx = static_cast<double>(a.periodic_nx);
@@ -277,6 +279,7 @@ void launch_membrane(Ctx& c) {
     a.L0 = c.L0; a.th0 = c.th0; a.ds0 = c.ds0; a.Lprev = c.Lprev; a.has_prev = c.has_prev;
     a.off = c.off; a.desc = c.desc;
     a.periodic_nx = c.periodic_x ? c.nx : 0;
+    a.counters = c.counters;
     hipLaunchKernelGGL(k_membrane, dim3(c.ncaps), dim3(64), 0, c.stream, a);
 }
 
@@ -354,8 +357,11 @@ __global__ void k_cells_fill(RepArgs a, int* cnt, int* list, int* counters) {
     else atomicOr(&counters[2], 1);
 }
 
-__global__ void k_cells_sort(int ncells, const int* cnt, int* list) {
+// Sort each cell's list by node index (the reference's insertion order) and
+// clear the other step's count array for the next step.
+__global__ void k_cells_sort(int ncells, int nclear, const int* cnt, int* list, int* other) {
     const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci < nclear) other[ci] = 0;
     if (ci >= ncells) return;
     int m = cnt[ci];
     if (m > CELL_MAXC) m = CELL_MAXC;
@@ -472,10 +478,14 @@ void launch_repulsion(Ctx& c) {
         a.ncx = h.ncx; a.xorg = h.xorg; a.cdx = h.cdx;
     }
     const int ncells = a.ncx * a.ncy;
-    launch_fill_int(c, c.cell_cnt, ncells, 0);
-    hipLaunchKernelGGL(k_cells_fill, dim3(nb), dim3(bs), 0, c.stream, a, c.cell_cnt, c.cell_list, c.counters);
-    hipLaunchKernelGGL(k_cells_sort, dim3((ncells + 255) / 256), dim3(256), 0, c.stream, ncells, c.cell_cnt, c.cell_list);
-    a.cnt = c.cell_cnt;
+    const int nmax = c.ncx * c.ncy;
+    int* cnt = c.cell_cnt + static_cast<size_t>(c.cell_cur) * nmax;
+    int* other = c.cell_cnt + static_cast<size_t>(1 - c.cell_cur) * nmax;
+    hipLaunchKernelGGL(k_cells_fill, dim3(nb), dim3(bs), 0, c.stream, a, cnt, c.cell_list, c.counters);
+    hipLaunchKernelGGL(k_cells_sort, dim3((nmax + 255) / 256), dim3(256), 0, c.stream, ncells, nmax, cnt,
+                       c.cell_list, other);
+    c.cell_cur = 1 - c.cell_cur;
+    a.cnt = cnt;
     a.list = c.cell_list;
     hipLaunchKernelGGL((k_repulsion<true>), dim3(nb), dim3(bs), 0, c.stream, a);
 }
