@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing-pass", action="store_true")
+    ap.add_argument("--lbm-launch", default="auto",
+                    help="'auto' (autotune) or BXxBY[xl] for a fixed collide-stream launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -130,8 +132,16 @@ def main():
     print(f"# setup {time.perf_counter() - t0:.1f}s, {sum(c['count'] for c in case['capsules'])} capsules",
           file=sys.stderr, flush=True)
     sim.run(min(args.warmup, 10))
-    tune = sim.autotune(10)
-    print("# autotune:\n" + "".join("#   " + l + "\n" for l in tune.strip().splitlines()), file=sys.stderr, flush=True)
+    if args.lbm_launch == "auto":
+        tune = sim.autotune(10)
+        print("# autotune:\n" + "".join("#   " + l + "\n" for l in tune.strip().splitlines()),
+              file=sys.stderr, flush=True)
+    else:
+        parts = args.lbm_launch.split("x")
+        bx, by = int(parts[0]), int(parts[1])
+        banded = not (len(parts) > 2 and parts[2] == "l")
+        sim.setLbmLaunch(bx, by, banded)
+        tune = "fixed %dx%d %s" % (bx, by, "xcd-banded" if banded else "linear")
     sim.run(args.warmup)
     sim.getMaxSpeed()   # synchronises
     t = time.perf_counter()

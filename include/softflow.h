@@ -151,6 +151,8 @@ int  sf_set_timing(sf_ctx* ctx, int on);
  * source buffers, so repeated launches leave the state unchanged -- and keeps
  * the fastest.  report (may be NULL) receives one line per candidate. */
 int  sf_autotune(sf_ctx* ctx, int reps, char* report, int report_len);
+/* Fix the collide-stream launch shape instead (bx*by a multiple of 64, <= 256). */
+int  sf_set_lbm_launch(sf_ctx* ctx, int bx, int by, int xcd_banded);
 
 /* Device math probe used by the tests: op 0 sqrt, 1 divide (a/b), 2 atan2(a,b),
  * 3 pow(a, b) with integer b as used by the contact force. */

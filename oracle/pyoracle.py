@@ -93,11 +93,11 @@ def _lib(path: str, prefix: str):
     g("capsule_geometry").argtypes = [C.c_void_p, C.c_int, C.c_int]
     g("phi").restype = C.c_double
     g("phi").argtypes = [C.c_double]
+    g("set_nodes").argtypes = [C.c_void_p] + [_DP] * 4
     if prefix == "sfo_":
         for n in ("default_params",):
             g(n).argtypes = [C.POINTER(Params)]
         g("default_membrane").argtypes = [C.POINTER(Membrane)]
-        g("set_nodes").argtypes = [C.c_void_p] + [_DP] * 4
         g("set_f").argtypes = [C.c_void_p, _DP]
         g("set_macro").argtypes = [C.c_void_p, _DP, _DP, _DP]
         g("get_flags").argtypes = [C.c_void_p, C.POINTER(C.c_int)]
@@ -106,6 +106,7 @@ def _lib(path: str, prefix: str):
         g("rest_lengths").argtypes = [C.c_void_p, _DP]
         g("phi_exact").restype = C.c_double
         g("phi_exact").argtypes = [C.c_double]
+        g("pair_force").argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP]
         g("collide_cell").argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, _DP, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_double]
@@ -256,6 +257,12 @@ class Sim:
 
     def max_speed(self):
         return self._f("max_speed")(self.h)
+
+    def pair_force(self, pa, va, pb, vb):
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (pa, va, pb, vb)]
+        out = np.zeros(2)
+        self._f("pair_force")(self.h, *[_ptr(a) for a in arrs], _ptr(out))
+        return out
 
     def rest_lengths(self):
         a = np.zeros(self.total_nodes)

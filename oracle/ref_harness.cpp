@@ -221,6 +221,19 @@ void sfr_get_nodes(const sfr_sim* s, double* x, double* y, double* vx, double* v
         }
     }
 }
+void sfr_set_nodes(sfr_sim* s, const double* x, const double* y, const double* vx,
+                   const double* vy) {
+    int k = 0;
+    for (int c = 0; c < s->caps.numCapsules(); ++c) {
+        Capsule& cp = s->caps[c];
+        for (int i = 0; i < cp.numNodes(); ++i, ++k) {
+            if (x) cp.positions()[i].x = x[k];
+            if (y) cp.positions()[i].y = y[k];
+            if (vx) cp.velocities()[i].x = vx[k];
+            if (vy) cp.velocities()[i].y = vy[k];
+        }
+    }
+}
 void sfr_get_field(const sfr_sim* s, double* rho, double* ux, double* uy, double* fx,
                    double* fy) {
     const LatticeField& f = s->lbm->field();

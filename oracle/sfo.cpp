@@ -1229,6 +1229,17 @@ void sfo_collide_cell(int model, double tau, double se, double seps, double sq, 
     else collide_bgk(f9, om, rho, ux, uy, Fx, Fy);
 }
 
+void sfo_pair_force(const sfo_sim* s, const double* pa, const double* va, const double* pb,
+                    const double* vb, double* out) {
+    const sfo_params& p = s->p;
+    Repulsion R{p.rep_epsilon, p.rep_sigma, p.rep_r_cut, p.rep_damping_normal,
+                p.rep_friction_coeff, p.rep_power,
+                p.boundary_type == 1 ? static_cast<double>(p.nx) : 0.0};
+    V2 F = R.pair(V2{pa[0], pa[1]}, V2{va[0], va[1]}, V2{pb[0], pb[1]}, V2{vb[0], vb[1]});
+    out[0] = F.x;
+    out[1] = F.y;
+}
+
 void sfo_membrane_forces_only(sfo_sim* s) {
     Caps& c = s->caps;
     std::fill(c.fx.begin(), c.fx.end(), 0.0);

@@ -100,6 +100,10 @@ void   sfo_collide_cell(int model, double tau, double se, double seps,
                         double sq, double* f9, double rho, double ux,
                         double uy, double Fx, double Fy);
 void   sfo_membrane_forces_only(sfo_sim* s);
+/* RepulsionForce::pairForce (repulsion.cpp:6-38) for the sim's contact
+ * parameters; out[2] = force on a. */
+void   sfo_pair_force(const sfo_sim* s, const double* pa, const double* va,
+                      const double* pb, const double* vb, double* out);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,7 @@ EXPORTS = (
     "sf_set_capsules", "sf_get_nodes", "sf_set_nodes", "sf_get_membrane_state",
     "sf_set_scalar", "sf_get_scalar", "sf_step", "sf_set_graph_mode", "sf_synchronize",
     "sf_stage", "sf_last_capped", "sf_field_stats", "sf_step_count", "sf_kernel_times",
-    "sf_set_timing", "sf_autotune", "sf_probe_math",
+    "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_probe_math",
 )
 
 
@@ -99,6 +99,7 @@ def load():
         "sf_kernel_times": [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), _DP, C.POINTER(C.c_int)],
         "sf_set_timing": [C.c_void_p, C.c_int],
         "sf_autotune": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],
+        "sf_set_lbm_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int],
         "sf_probe_math": [C.c_int, C.c_int, _DP, _DP, _DP],
     }
     for name, args in proto.items():

@@ -344,6 +344,7 @@ PYBIND11_MODULE(softflow_core, m) {
         }, py::arg("callback"))
         .def("setTiming", &Simulation::setTiming, py::arg("on"))
         .def("autotune", &Simulation::autotune, py::arg("reps") = 10)
+        .def("setLbmLaunch", &Simulation::setLbmLaunch, py::arg("bx"), py::arg("by"), py::arg("xcd_banded") = true)
         .def("kernelTimes", &Simulation::kernelTimes);
 
     m.def("device_count", []() {

@@ -674,6 +674,11 @@ std::string Simulation::autotune(int reps) {
     return std::string(buf.data());
 }
 
+void Simulation::setLbmLaunch(int bx, int by, bool xcd_banded) {
+    if (!ctx_) throw std::runtime_error("Simulation: initialize() has not been called");
+    check(sf_set_lbm_launch(ctx_, bx, by, xcd_banded ? 1 : 0), "sf_set_lbm_launch");
+}
+
 void Simulation::setTiming(bool on) { if (ctx_) check(sf_set_timing(ctx_, on ? 1 : 0), "sf_set_timing"); }
 
 std::vector<std::pair<std::string, double>> Simulation::kernelTimes() {

@@ -307,6 +307,7 @@ public:
     std::vector<std::pair<std::string, double>> kernelTimes();
     void setTiming(bool on);
     std::string autotune(int reps = 10);
+    void setLbmLaunch(int bx, int by, bool xcd_banded);
 
     // device <-> host synchronisation (used by the mirrors)
     void pullField(LatticeField& f);

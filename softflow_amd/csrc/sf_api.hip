@@ -238,7 +238,7 @@ static int check_counters(Ctx& c) {
     SF_HIP(hipMemcpyAsync(c.host_counters, c.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     SF_HIP(hipStreamSynchronize(c.stream));
     if (c.host_counters[1]) return fail(SF_E_CAPACITY, "IBM tile node list overflowed (more than 1024 nodes near one 32x8 tile)");
-    if (c.host_counters[2] & 1) return fail(SF_E_CAPACITY, "contact cell list overflowed (more than 48 nodes in one cell)");
+    if (c.host_counters[2] & 1) return fail(SF_E_CAPACITY, "contact cell list overflowed (more than 32 nodes in one cell)");
     if (c.host_counters[2] & 2) return fail(SF_E_CAPACITY, "contact cell grid exceeded its capacity");
     return SF_OK;
 }
@@ -376,7 +376,8 @@ int sf_destroy(sf_ctx* h) {
     invalidate_graph(c);
     free_caps(c);
     void* ptrs[] = {c.flags, c.f[0], c.f[1], c.F[0], c.F[1], c.F[2], c.stored_rho, c.stored_u,
-                    c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_dirty, c.cell_cnt, c.cell_list};
+                    c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_dirty, c.cell_cnt, c.cell_list,
+                    c.cell_pay, c.cell_own};
     for (void* p : ptrs) if (p) hipFree(p);
     for (int s = 0; s < 4; ++s) {
         for (int b = 0; b < 2; ++b) if (c.g[s][b]) hipFree(c.g[s][b]);
@@ -503,6 +504,8 @@ int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_d
         if (desc[i].use_helfrich && (!rest_angle || !rest_ds)) return fail(SF_E_ARG, "Helfrich capsule without rest angles");
         maxn = std::max(maxn, n);
     }
+    c.any_rigid = false;
+    for (int i = 0; i < ncaps; ++i) c.any_rigid |= desc[i].is_rigid != 0;
     if (c.nx < 4) return fail(SF_E_ARG, "capsules need nx >= 4");
     const int K = off[ncaps];
     c.ncaps = ncaps; c.K = K; c.max_nodes = maxn;
@@ -541,6 +544,8 @@ int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_d
         const size_t ncells = static_cast<size_t>(c.ncx) * c.ncy;
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_cnt), 2 * ncells * sizeof(int)));
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_list), ncells * CELL_MAXC * sizeof(int)));
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_pay), ncells * CELL_MAXC * sizeof(double4)));
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.cell_own), ncells * CELL_MAXC * sizeof(int)));
         SF_HIP(hipMemset(c.cell_cnt, 0, 2 * ncells * sizeof(int)));
         c.cell_cur = 0;
     }
@@ -817,6 +822,16 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
         report[report_len - 1] = 0;
     }
     SF_HIP(hipStreamSynchronize(c.stream));
+    return SF_OK;
+}
+
+int sf_set_lbm_launch(sf_ctx* h, int bx, int by, int xcd_banded) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    if (bx < 1 || by < 1 || bx * by > 256 || (bx * by) % 64 != 0)
+        return fail(SF_E_ARG, "launch shape must hold 64..256 threads in whole wavefronts");
+    Ctx& c = h->c;
+    c.lbm_bx = bx; c.lbm_by = by; c.lbm_remap = xcd_banded ? 1 : 0;
+    invalidate_graph(c);
     return SF_OK;
 }
 

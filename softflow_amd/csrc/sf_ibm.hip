@@ -105,7 +105,18 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
 // mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction).
 // Tiles no node touches are skipped when the output buffer already holds zeros
 // there (dirty flags per buffer), so the dense write covers touched tiles only.
-constexpr int SPREAD_CHUNK = 256;   // node records staged in LDS at a time
+constexpr int STAGE_MAX = 256;     // node records staged in LDS (lists up to this length)
+constexpr int CELL_MAXPC = 24;     // contributions one cell collects in LDS before falling back
+constexpr int NO_ROW = -0x40000000;
+
+// Unwrapped stencil column (lo .. lo+3) whose single periodic wrap lands on cx,
+// or NO_ROW if none does.
+__device__ __forceinline__ int stencil_column(int cx, int lo, int nx) {
+    if (cx - lo >= 0 && cx - lo <= 3) return cx;
+    if (cx - nx - lo >= 0 && cx - nx - lo <= 3) return cx - nx;
+    if (cx + nx - lo >= 0 && cx + nx - lo <= 3) return cx + nx;
+    return NO_ROW;
+}
 
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile_cnt,
@@ -113,11 +124,13 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile
                                                 const void* Fin, void* Fout,
                                                 const uint8_t* dirty_in, uint8_t* dirty_out,
                                                 int nx, int ny, int ntx, int mode, int last) {
-    __shared__ int list[TILE_MAXL];
     __shared__ double tab[1025];
-    __shared__ double spx[SPREAD_CHUNK], spy[SPREAD_CHUNK], sfx[SPREAD_CHUNK], sfy[SPREAD_CHUNK],
-        sds[SPREAD_CHUNK];
-    __shared__ int si0[SPREAD_CHUNK], sj0[SPREAD_CHUNK];
+    __shared__ double spx[STAGE_MAX], spy[STAGE_MAX], sfx[STAGE_MAX], sfy[STAGE_MAX], sds[STAGE_MAX];
+    __shared__ int si0[STAGE_MAX], sj0[STAGE_MAX], skey[STAGE_MAX];
+    __shared__ int ccnt[TILE_X * TILE_Y];
+    __shared__ int cslot_raw[TILE_X * TILE_Y * CELL_MAXPC / 2];
+    unsigned short* cslot = reinterpret_cast<unsigned short*>(cslot_raw);
+    int* list = cslot_raw;   // the long-list path keeps its sorted list here
     using T = typename std::conditional<FP32, float, double>::type;
     const int tile = blockIdx.x;
     const int t = threadIdx.x;
@@ -140,70 +153,145 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile
         if (t == 0) dirty_out[tile] = copy ? 1 : 0;
         return;
     }
-    int P = 1;
-    while (P < L) P <<= 1;
-    for (int i = t; i < P; i += 256)
-        list[i] = i < L ? tile_list[static_cast<size_t>(tile) * TILE_MAXL + i] : 0x7fffffff;
+    const int* tl = tile_list + static_cast<size_t>(tile) * TILE_MAXL;
     for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
-    __syncthreads();
-    if (t == 0) {
-        if (last) tile_cnt[tile] = 0;
-        dirty_out[tile] = 1;
-    }
-    // bitonic sort of the node list (ascending global node index)
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = t; i < P; i += 256) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const int a = list[i], b = list[l];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) { list[i] = b; list[l] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    }
     double ax = 0.0, ay = 0.0;
     if (mode == 1 && inside) {
         ax = static_cast<const T*>(Fin)[2 * n];
         ay = static_cast<const T*>(Fin)[2 * n + 1];
     }
-    for (int base = 0; base < L; base += SPREAD_CHUNK) {
-        const int m = min(SPREAD_CHUNK, L - base);
-        if (t < m) {
-            const NodeSpread s = rec[list[base + t]];
-            spx[t] = s.px; spy[t] = s.py; sfx[t] = s.fx; sfy[t] = s.fy; sds[t] = s.ds;
-            // invalid records never touch a row
-            si0[t] = s.i0;
-            sj0[t] = s.valid ? s.j0 : -0x40000000;
+    auto stage = [&](int i, int k) {
+        const NodeSpread s = rec[k];
+        spx[i] = s.px; spy[i] = s.py; sfx[i] = s.fx; sfy[i] = s.fy; sds[i] = s.ds;
+        si0[i] = s.i0;
+        sj0[i] = s.valid ? s.j0 : NO_ROW;   // invalid records touch no row
+        skey[i] = k;
+    };
+    auto contribution = [&](int i, double& w, int& ok) {
+        ok = 0;
+        w = 0.0;
+        const int dyc = cy - (sj0[i] - 1);
+        if (dyc < 0 || dyc > 3) return;
+        const int ix = stencil_column(cx, si0[i] - 1, nx);
+        if (ix == NO_ROW) return;
+        w = phi_lut(tab, spx[i] - static_cast<double>(ix)) * phi_lut(tab, spy[i] - static_cast<double>(cy));
+        ok = w > 1e-15;
+    };
+
+    if (L <= STAGE_MAX) {
+        // stage the records, then enumerate node x stencil pairs in parallel
+        // into per-cell lists
+        for (int i = t; i < L; i += 256) stage(i, tl[i]);
+        ccnt[t] = 0;
+        __syncthreads();
+        if (t == 0) {
+            if (last) tile_cnt[tile] = 0;
+            dirty_out[tile] = 1;
+        }
+        for (int i = t; i < 16 * L; i += 256) {
+            const int p = i >> 4, s = i & 15;
+            const int j0 = sj0[p];
+            if (j0 == NO_ROW) continue;
+            const int iy = j0 - 1 + (s >> 2);
+            if (iy < 0 || iy >= ny) continue;
+            const int ly = iy - ty0;
+            if (ly < 0 || ly >= TILE_Y) continue;
+            const int ix = si0[p] - 1 + (s & 3);
+            int jx = ix;
+            if (jx < 0) jx += nx;
+            if (jx >= nx) jx -= nx;
+            const int lx = jx - tx0;
+            if (lx < 0 || lx >= TILE_X) continue;
+            const double w = phi_lut(tab, spx[p] - static_cast<double>(ix)) *
+                             phi_lut(tab, spy[p] - static_cast<double>(iy));
+            if (!(w > 1e-15)) continue;
+            const int cell = ly * TILE_X + lx;
+            const int slot = atomicAdd(&ccnt[cell], 1);
+            if (slot < CELL_MAXPC) cslot[cell * CELL_MAXPC + slot] = static_cast<unsigned short>(p);
         }
         __syncthreads();
-        if (inside) {
-            for (int i = 0; i < m; ++i) {
-                const int dyc = cy - (sj0[i] - 1);
-                if (dyc < 0 || dyc > 3) continue;
-                // unwrapped stencil column whose single wrap lands on cx
-                const int lo = si0[i] - 1;
-                int ix;
-                if (cx - lo >= 0 && cx - lo <= 3) ix = cx;
-                else if (cx - nx - lo >= 0 && cx - nx - lo <= 3) ix = cx - nx;
-                else if (cx + nx - lo >= 0 && cx + nx - lo <= 3) ix = cx + nx;
-                else continue;
-                const double w = phi_lut(tab, spx[i] - static_cast<double>(ix)) *
-                                 phi_lut(tab, spy[i] - static_cast<double>(cy));
-                if (w > 1e-15) {
-                    ax += sfx[i] * w * sds[i];
-                    ay += sfy[i] * w * sds[i];
+        if (!inside) return;
+        const int m = ccnt[t];
+        if (m <= CELL_MAXPC) {
+            // this cell's contributions in global node order (insertion sort)
+            unsigned short* my = cslot + t * CELL_MAXPC;
+            for (int a = 1; a < m; ++a) {
+                const unsigned short v = my[a];
+                const int kv = skey[v];
+                int b = a - 1;
+                while (b >= 0 && skey[my[b]] > kv) { my[b + 1] = my[b]; --b; }
+                my[b + 1] = v;
+            }
+            for (int j = 0; j < m; ++j) {
+                const int p = my[j];
+                double w; int ok;
+                contribution(p, w, ok);
+                ax += sfx[p] * w * sds[p];
+                ay += sfy[p] * w * sds[p];
+            }
+        } else {
+            // more contributions than slots: walk the staged records in node order
+            int lastk = -1;
+            for (;;) {
+                int best = -1, bestk = 0x7fffffff;
+                for (int i = 0; i < L; ++i) {
+                    const int k = skey[i];
+                    if (k <= lastk || k >= bestk) continue;
+                    double w; int ok;
+                    contribution(i, w, ok);
+                    if (ok) { best = i; bestk = k; }
                 }
+                if (best < 0) break;
+                double w; int ok;
+                contribution(best, w, ok);
+                ax += sfx[best] * w * sds[best];
+                ay += sfy[best] * w * sds[best];
+                lastk = bestk;
             }
         }
+    } else {
+        // long list: sort it, then every cell scans it in chunks
+        int P = 1;
+        while (P < L) P <<= 1;
+        for (int i = t; i < P; i += 256) list[i] = i < L ? tl[i] : 0x7fffffff;
         __syncthreads();
+        if (t == 0) {
+            if (last) tile_cnt[tile] = 0;
+            dirty_out[tile] = 1;
+        }
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = t; i < P; i += 256) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const int a = list[i], b = list[l];
+                        const bool up = (i & k) == 0;
+                        if ((a > b) == up) { list[i] = b; list[l] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int base = 0; base < L; base += STAGE_MAX) {
+            const int m = min(STAGE_MAX, L - base);
+            for (int i = t; i < m; i += 256) stage(i, list[base + i]);
+            __syncthreads();
+            if (inside) {
+                for (int i = 0; i < m; ++i) {
+                    double w; int ok;
+                    contribution(i, w, ok);
+                    if (ok) {
+                        ax += sfx[i] * w * sds[i];
+                        ay += sfy[i] * w * sds[i];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (!inside) return;
     }
-    if (inside) {
-        static_cast<T*>(Fout)[2 * n] = static_cast<T>(ax);
-        static_cast<T*>(Fout)[2 * n + 1] = static_cast<T>(ay);
-    }
+    static_cast<T*>(Fout)[2 * n] = static_cast<T>(ax);
+    static_cast<T*>(Fout)[2 * n + 1] = static_cast<T>(ay);
 }
 
 // ---------------------------------------------------------------------------
@@ -354,20 +442,52 @@ __global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double*
 
 // interpolateVelocity (immersed_boundary.cpp:174-189) with the post-step
 // velocity u = macro(f_{n+1}, F_n).
+// With move set (no rigid capsule present) the node is also advanced here:
+// moveNodes (capsule.cpp:287-296) and the y-clamp (simulation.cpp:665-675).
+struct MoveArgs {
+    int move;
+    double dt;
+    int periodic_nx, clamp_y;
+    double ymin, ymax;
+};
+
+__device__ __forceinline__ void move_node(const MoveArgs& m, double& px, double& py, double ux,
+                                          double uy) {
+    px = px + ux * m.dt;
+    py = py + uy * m.dt;
+    if (m.periodic_nx > 0 && isfinite(px)) {
+        // repeated shifts as the reference does; a bounded count keeps a
+        // runaway coordinate from looping (it is then folded with fmod)
+        const double Lx = static_cast<double>(m.periodic_nx);
+        if (fabs(px) > 64.0 * Lx) px = fmod(px, Lx);
+        while (px < 0.0) px += Lx;
+        while (px >= Lx) px -= Lx;
+    }
+    if (m.clamp_y) {
+        if (py < m.ymin) py = m.ymin;
+        if (py > m.ymax) py = m.ymax;
+    }
+}
+
 template <bool FP32>
-__global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut, const double* x,
-                                                const double* y, double* vx, double* vy, int K) {
+__global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut, double* x,
+                                                double* y, double* vx, double* vy, int K, MoveArgs mv) {
     __shared__ double tx[256], ty[256];
     __shared__ int use[256];
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     const int k = blockIdx.x * INTERP_NODES + g;
     const bool live = k < K;
+    double px = live ? x[k] : 0.0, py = live ? y[k] : 0.0;
     double Ux, Uy;
-    interp_group<FP32>(a, lut, live ? x[k] : 0.0, live ? y[k] : 0.0, 1, l, tx + 16 * g, ty + 16 * g,
-                       use + 16 * g, Ux, Uy);
+    interp_group<FP32>(a, lut, px, py, 1, l, tx + 16 * g, ty + 16 * g, use + 16 * g, Ux, Uy);
     if (live && l == 0) {
         vx[k] = Ux;
         vy[k] = Uy;
+        if (mv.move) {
+            move_node(mv, px, py, Ux, Uy);
+            x[k] = px;
+            y[k] = py;
+        }
     }
 }
 
@@ -421,21 +541,10 @@ __global__ void k_move(double* x, double* y, double* vx, double* vy, const int* 
         }
         __syncthreads();
     }
+    const MoveArgs mv{1, dt, periodic_nx, clamp_y, ymin, ymax};
     for (int i = lane; i < n; i += blockDim.x) {
-        double px = x[o + i] + vx[o + i] * dt;
-        double py = y[o + i] + vy[o + i] * dt;
-        if (periodic_nx > 0 && isfinite(px)) {
-            // repeated shifts as the reference does; a bounded count keeps a
-            // runaway coordinate from looping (it is then folded with fmod)
-            const double Lx = static_cast<double>(periodic_nx);
-            if (fabs(px) > 64.0 * Lx) px = fmod(px, Lx);
-            while (px < 0.0) px += Lx;
-            while (px >= Lx) px -= Lx;
-        }
-        if (clamp_y) {
-            if (py < ymin) py = ymin;
-            if (py > ymax) py = ymax;
-        }
+        double px = x[o + i], py = y[o + i];
+        move_node(mv, px, py, vx[o + i], vy[o + i]);
         x[o + i] = px;
         y[o + i] = py;
     }
@@ -506,15 +615,18 @@ void launch_interp_move(Ctx& c) {
     if (c.K == 0) return;
     CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
     const dim3 grid((c.K + INTERP_NODES - 1) / INTERP_NODES);
+    const MoveArgs mv{c.any_rigid ? 0 : 1, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
+                      1.5, static_cast<double>(c.ny) - 1.5};
     if (c.fp32)
         hipLaunchKernelGGL((k_interp<true>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, c.vx,
-                           c.vy, c.K);
+                           c.vy, c.K, mv);
     else
         hipLaunchKernelGGL((k_interp<false>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, c.vx,
-                           c.vy, c.K);
-    hipLaunchKernelGGL(k_move, dim3(c.ncaps), dim3(64), 0, c.stream, c.x, c.y, c.vx, c.vy, c.off,
-                       c.desc, c.ncaps, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
-                       1.5, static_cast<double>(c.ny) - 1.5);
+                           c.vy, c.K, mv);
+    if (c.any_rigid)
+        hipLaunchKernelGGL(k_move, dim3(c.ncaps), dim3(64), 0, c.stream, c.x, c.y, c.vx, c.vy, c.off,
+                           c.desc, c.ncaps, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
+                           1.5, static_cast<double>(c.ny) - 1.5);
 }
 
 }  // namespace sf

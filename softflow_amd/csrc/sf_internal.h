@@ -28,7 +28,7 @@ __host__ __device__ constexpr double dw(int q) {
 constexpr int TILE_X = 32;
 constexpr int TILE_Y = 8;
 constexpr int TILE_MAXL = 1024;    // node references per tile
-constexpr int CELL_MAXC = 48;      // nodes per repulsion cell
+constexpr int CELL_MAXC = 32;      // nodes per contact cell
 
 // Shift-force source of the collision velocity (see Ctx::shift_next).
 enum Shift : int { SHIFT_ZERO = 0, SHIFT_PREV = 1, SHIFT_PARTIAL = 2, SHIFT_STORED = 3 };
@@ -80,6 +80,7 @@ struct Ctx {
 
     // capsules
     int ncaps = 0, K = 0, max_nodes = 0;
+    bool any_rigid = false;
     int* off = nullptr;
     int* owner = nullptr;
     sf_capsule_desc* desc = nullptr;
@@ -101,6 +102,8 @@ struct Ctx {
     double cdx = 1.0, cdy = 1.0, xorg = 0.0, Lx = 0.0;
     int* cell_cnt = nullptr;             // two count arrays, alternating by step
     int* cell_list = nullptr;
+    double4* cell_pay = nullptr;         // (x, y, vx, vy) of each listed node, in list order
+    int* cell_own = nullptr;             // capsule of each listed node
     int cell_cur = 0;
 
     // counters: [0] capped cells, [1] tile overflow, [2] cell overflow

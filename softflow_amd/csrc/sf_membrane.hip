@@ -293,9 +293,11 @@ struct RepArgs {
     double eps, sig, rcut, gn, mu, Lx;
     int pw;
     double ybot, ytop;
-    // cell list
+    // cell list: counts and per-entry node data in node-index order
     const int* cnt;
-    const int* list;
+    const double4* pay;
+    const int* own;
+    double rcut2_hi;   // > rcut^2 by a margin: beyond it no pair can be within rcut
     int ncx, ncy;
     double xorg, cdx, cdy;
 };
@@ -308,7 +310,9 @@ __device__ __forceinline__ void pair_force(const RepArgs& a, double pax, double 
         if (dx > 0.5 * a.Lx) dx -= a.Lx;
         if (dx < -0.5 * a.Lx) dx += a.Lx;
     }
-    const double r = sqrt(dx * dx + dy * dy);
+    const double d2 = dx * dx + dy * dy;
+    if (d2 > a.rcut2_hi) { Fx = 0.0; Fy = 0.0; return; }   // r >= rcut for certain
+    const double r = sqrt(d2);
     if (r < 1e-15 || r >= a.rcut) { Fx = 0.0; Fy = 0.0; return; }
     const double nx = dx / r, ny = dy / r;
     const double Fr = a.eps * pow_int_cr(a.sig / r, a.pw);
@@ -357,20 +361,29 @@ __global__ void k_cells_fill(RepArgs a, int* cnt, int* list, int* counters) {
     else atomicOr(&counters[2], 1);
 }
 
-// Sort each cell's list by node index (the reference's insertion order) and
-// clear the other step's count array for the next step.
-__global__ void k_cells_sort(int ncells, int nclear, const int* cnt, int* list, int* other) {
+// Sort each cell's list by node index (the reference's insertion order), lay
+// the listed nodes' data out in that order, and clear the other step's count
+// array for the next step.
+__global__ void k_cells_sort(int ncells, int nclear, const int* cnt, int* list, int* other,
+                             const double* x, const double* y, const double* vx, const double* vy,
+                             const int* owner, double4* pay, int* own) {
     const int ci = blockIdx.x * blockDim.x + threadIdx.x;
     if (ci < nclear) other[ci] = 0;
     if (ci >= ncells) return;
     int m = cnt[ci];
     if (m > CELL_MAXC) m = CELL_MAXC;
-    int* l = list + static_cast<size_t>(ci) * CELL_MAXC;
+    const size_t base = static_cast<size_t>(ci) * CELL_MAXC;
+    int* l = list + base;
     for (int i = 1; i < m; ++i) {
         const int v = l[i];
         int j = i - 1;
         while (j >= 0 && l[j] > v) { l[j + 1] = l[j]; --j; }
         l[j + 1] = v;
+    }
+    for (int i = 0; i < m; ++i) {
+        const int k = l[i];
+        pay[base + i] = make_double4(x[k], y[k], vx[k], vy[k]);
+        own[base + i] = owner[k];
     }
 }
 
@@ -403,11 +416,11 @@ __global__ void k_repulsion(RepArgs a) {
                 const int cell = yy * a.ncx + xx;
                 int m = a.cnt[cell];
                 if (m > CELL_MAXC) m = CELL_MAXC;
-                const int* l = a.list + static_cast<size_t>(cell) * CELL_MAXC;
+                const size_t base = static_cast<size_t>(cell) * CELL_MAXC;
                 for (int t = 0; t < m; ++t) {
-                    const int b = l[t];
-                    if (a.owner[b] == cap) continue;
-                    pair_force(a, px, py, vx, vy, a.x[b], a.y[b], a.vx[b], a.vy[b], Fx, Fy);
+                    if (a.own[base + t] == cap) continue;
+                    const double4 b = a.pay[base + t];
+                    pair_force(a, px, py, vx, vy, b.x, b.y, b.z, b.w, Fx, Fy);
                     fx += Fx; fy += Fy;
                 }
             }
@@ -457,6 +470,7 @@ void launch_repulsion(Ctx& c) {
     a.gn = c.p.rep_damping_normal; a.mu = c.p.rep_friction_coeff; a.pw = c.p.rep_power;
     a.Lx = c.periodic_x ? static_cast<double>(c.nx) : 0.0;
     a.ybot = 0.5; a.ytop = static_cast<double>(c.ny) - 1.5;
+    a.rcut2_hi = a.rcut * a.rcut * (1.0 + 1e-12);
     const int bs = 128, nb = (c.K + bs - 1) / bs;
     if (c.K <= 100) {
         hipLaunchKernelGGL((k_repulsion<false>), dim3(nb), dim3(bs), 0, c.stream, a);
@@ -483,10 +497,11 @@ void launch_repulsion(Ctx& c) {
     int* other = c.cell_cnt + static_cast<size_t>(1 - c.cell_cur) * nmax;
     hipLaunchKernelGGL(k_cells_fill, dim3(nb), dim3(bs), 0, c.stream, a, cnt, c.cell_list, c.counters);
     hipLaunchKernelGGL(k_cells_sort, dim3((nmax + 255) / 256), dim3(256), 0, c.stream, ncells, nmax, cnt,
-                       c.cell_list, other);
+                       c.cell_list, other, c.x, c.y, c.vx, c.vy, c.owner, c.cell_pay, c.cell_own);
     c.cell_cur = 1 - c.cell_cur;
     a.cnt = cnt;
-    a.list = c.cell_list;
+    a.pay = c.cell_pay;
+    a.own = c.cell_own;
     hipLaunchKernelGGL((k_repulsion<true>), dim3(nb), dim3(bs), 0, c.stream, a);
 }
 

@@ -1,0 +1,73 @@
+"""Write the golden vectors under tests/golden/ from the reference build.
+
+Runs only where the reference tree exists: it loads oracle/_ref/libsfref.so,
+the reference's own hot-path components (compiled from its sources by
+``make -C oracle ref``) driven in Simulation::step order at one OpenMP thread,
+and stores inputs (the case dict) and outputs (fields, node state) as .npz.
+
+    python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from tests.cases import oracle_sim, oracle_state  # noqa: E402
+
+SKALAK = dict(model=2, G_s=0.06, C_skalak=10.0)
+
+CASES = {
+    "channel_bgk": (dict(nx=40, ny=18, tau=0.8, bfx=3e-5, coll=0), 800),
+    "channel_mrt": (dict(nx=40, ny=18, tau=0.7, bfx=1e-5, coll=1), 800),
+    "channel_reg": (dict(nx=40, ny=18, tau=1.2, bfx=2e-5, coll=2), 800),
+    "closed_box_bgk": (dict(nx=24, ny=20, tau=0.9, bfx=2e-5, coll=0, boundary="closed"), 400),
+    "capsules_reg_mdf3": (dict(nx=80, ny=40, tau=0.8, bfx=3e-5, coll=2, mdf=3,
+                               capsules=[dict(count=4, region=(10, 8, 70, 32), rmin=3.0, rmax=3.0,
+                                              nodes=20, seed=42, mem=SKALAK)]), 400),
+    "capsules_bgk_celllist": (dict(nx=80, ny=40, tau=0.8, bfx=3e-5, coll=0,
+                                   capsules=[dict(count=7, region=(8, 8, 72, 32), rmin=3.0, rmax=4.0,
+                                                  nodes=0, seed=7, mem=SKALAK)]), 400),
+    "capsules_helfrich_mrt_scalar": (dict(nx=64, ny=32, tau=0.7, bfx=1e-5, coll=1,
+                                          scalar=dict(D=0.05, blob=(32, 16, 5.0)),
+                                          capsules=[dict(count=3, region=(8, 8, 56, 24), rmin=3.0,
+                                                         rmax=4.0, nodes=0, seed=5,
+                                                         mem=dict(model=2, G_s=0.06, C_skalak=10.0,
+                                                                  use_helfrich=1, k_bend=1e-3))]), 300),
+    "capsules_wlc_neo_hooke": (dict(nx=64, ny=32, tau=0.8, bfx=3e-5, coll=0,
+                                    capsules=[dict(count=2, region=(8, 8, 56, 24), rmin=3.5, rmax=3.5,
+                                                   nodes=0, seed=11, mem=dict(model=3, G_s=0.05)),
+                                              dict(count=2, region=(8, 8, 56, 24), rmin=3.0, rmax=3.0,
+                                                   nodes=0, seed=12, mem=dict(model=1, G_s=0.05,
+                                                                              shape=1, aspect_ratio=0.6)),
+                                              dict(count=1, region=(8, 8, 56, 24), rmin=3.0, rmax=3.0,
+                                                   nodes=16, seed=13, mem=dict(model=0, k_stretch=0.05,
+                                                                               shape=2))]), 300),
+}
+
+
+def main():
+    from oracle import pyoracle
+
+    if not pyoracle.ref_available():
+        sys.exit("oracle/_ref/libsfref.so is not built (make -C oracle ref)")
+    index = {}
+    for name, (case, steps) in CASES.items():
+        sim = oracle_sim(case, which="ref")
+        sim.step(steps)
+        st = oracle_state(sim)
+        arrays = {k: np.asarray(v) for k, v in st.items() if k != "capped"}
+        arrays["capped"] = np.array(st["capped"])
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **arrays)
+        index[name] = {"case": case, "steps": steps}
+        print(name, "saved", flush=True)
+    with open(os.path.join(HERE, "cases.json"), "w") as f:
+        json.dump(index, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

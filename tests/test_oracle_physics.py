@@ -1,0 +1,507 @@
+"""Pins of the oracle that do not depend on running the reference.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign or index, a transposed operand) fails it: closed-form
+identities, analytic flow solutions (the reference's own verification
+problems, tests/verification/*.cpp), conservation laws, special cases that
+reduce one operator to another, and brute force on small inputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+
+
+def feq_np(rho, ux, uy):
+    eu = CX * ux + CY * uy
+    return W * rho * (1 + 3 * eu + 4.5 * eu ** 2 - 1.5 * (ux * ux + uy * uy))
+
+
+# -- lattice / collision operators ---------------------------------------
+def test_equilibrium_moments():
+    # BGK with tau = 1 and no force returns f_eq(rho, u) (lattice.h:35-39)
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        rho, ux, uy = 1 + 0.1 * rng.standard_normal(), 0.05 * rng.standard_normal(), 0.05 * rng.standard_normal()
+        fe = po.collide_cell(0, 1.0, rng.random(9), rho, ux, uy, 0.0, 0.0)
+        assert abs(fe.sum() - rho) < 1e-14
+        assert abs(fe @ CX - rho * ux) < 1e-15
+        assert abs(fe @ CY - rho * uy) < 1e-15
+        assert abs(fe @ (CX * CX) - rho * (1 / 3 + ux * ux)) < 1e-14
+        assert abs(fe @ (CX * CY) - rho * ux * uy) < 1e-15
+        assert np.allclose(fe, feq_np(rho, ux, uy), rtol=0, atol=1e-15)
+
+
+def test_bgk_guo_moments():
+    # Guo source: mass unchanged, momentum gains (1 - omega/2) F ... plus the
+    # relaxation of j toward rho u, closed form for the first moments
+    rng = np.random.default_rng(1)
+    tau = 0.8
+    om = 1 / tau
+    f = feq_np(1.0, 0.02, -0.01) + 1e-4 * rng.standard_normal(9)
+    rho, ux, uy, Fx, Fy = f.sum(), 0.021, -0.009, 3e-5, -1e-5
+    out = po.collide_cell(0, tau, f, rho, ux, uy, Fx, Fy)
+    assert abs(out.sum() - ((1 - om) * f.sum() + om * rho)) < 1e-15
+    jx = (1 - om) * (f @ CX) + om * rho * ux + (1 - om / 2) * Fx
+    assert abs(out @ CX - jx) < 1e-15
+
+
+def test_mrt_equals_bgk_when_all_rates_equal():
+    # MRT with s_e = s_eps = s_q = 1/tau is BGK with Guo forcing
+    # (mrt_collision.cpp:97-105 moment-space source) when the macroscopic
+    # fields are the ones the solver passes: rho = sum f, rho u = sum f c + F/2.
+    # MRT leaves the conserved moments unrelaxed, BGK relaxes them to rho, rho u.
+    rng = np.random.default_rng(2)
+    for tau in (0.6, 0.8, 1.3):
+        s = 1 / tau
+        for _ in range(10):
+            f = feq_np(1.0, 0.03, 0.01) + 1e-3 * rng.standard_normal(9)
+            Fx, Fy = 2e-4, -1e-4
+            rho = f.sum()
+            ux, uy = (f @ CX + 0.5 * Fx) / rho, (f @ CY + 0.5 * Fy) / rho
+            args = (rho, ux, uy, Fx, Fy)
+            a = po.collide_cell(1, tau, f, *args, se=s, seps=s, sq=s)
+            b = po.collide_cell(0, tau, f, *args)
+            assert np.max(np.abs(a - b)) < 2e-15
+            # an inconsistent momentum is relaxed by BGK, kept by MRT
+            c = po.collide_cell(1, tau, f, rho, ux + 1e-3, uy, Fx, Fy, se=s, seps=s, sq=s)
+            assert abs(c @ CX - (f @ CX + Fx)) < 1e-15
+
+
+def test_mrt_differs_when_rates_differ():
+    rng = np.random.default_rng(3)
+    f = feq_np(1.0, 0.03, 0.01) + 1e-3 * rng.standard_normal(9)
+    a = po.collide_cell(1, 0.7, f, f.sum(), 0.03, 0.01, 0.0, 0.0, se=1.4, seps=1.4, sq=1.4)
+    b = po.collide_cell(0, 0.7, f, f.sum(), 0.03, 0.01, 0.0, 0.0)
+    assert np.max(np.abs(a - b)) > 1e-6
+
+
+def test_regularized_equals_bgk_in_hermite_span():
+    # When f - f_eq lies in the span of the first and second Hermite
+    # polynomials, regularisation is the identity and the operator is BGK
+    # (regularized_bgk.cpp:41-57 keeps the first-order moment).
+    rng = np.random.default_rng(4)
+    Qxx, Qxy, Qyy = CX * CX - 1 / 3, CX * CY, CY * CY - 1 / 3
+    for tau in (0.55, 0.8, 1.5):
+        for _ in range(10):
+            rho, ux, uy = 1.0 + 0.01 * rng.standard_normal(), 0.03, -0.02
+            jx, jy = 1e-3 * rng.standard_normal(2)
+            pxx, pxy, pyy = 1e-3 * rng.standard_normal(3)
+            fneq = W * (3 * (CX * jx + CY * jy) + 4.5 * (Qxx * pxx + 2 * Qxy * pxy + Qyy * pyy))
+            f = feq_np(rho, ux, uy) + fneq
+            a = po.collide_cell(2, tau, f, rho, ux, uy, 1e-4, 2e-5)
+            b = po.collide_cell(0, tau, f, rho, ux, uy, 1e-4, 2e-5)
+            assert np.max(np.abs(a - b)) < 2e-15
+
+
+def test_regularized_filters_ghost_moments():
+    # a perturbation orthogonal to the Hermite span (the third/fourth-order
+    # ghost moments) is removed entirely by regularisation
+    ghost = np.array([4, -2, -2, -2, -2, 1, 1, 1, 1], float)   # M row 2 (epsilon)
+    f = feq_np(1.0, 0.0, 0.0) + 1e-3 * ghost
+    a = po.collide_cell(2, 0.8, f, 1.0, 0.0, 0.0, 0.0, 0.0)
+    assert np.allclose(a, feq_np(1.0, 0.0, 0.0), atol=1e-16)
+
+
+# -- Peskin delta table -----------------------------------------------------
+def test_phi_table_properties():
+    assert po.phi_exact(0.0) == 0.5
+    assert po.phi_exact(1.0) == 0.25
+    assert po.phi_exact(2.0) == 0.0
+    # the table reproduces the exact kernel at its nodes
+    for i in range(0, 1025, 7):
+        r = 2.0 * i / 1024
+        assert po.phi(r) == po.phi_exact(r) or abs(po.phi(r) - po.phi_exact(r)) < 1e-16
+    rng = np.random.default_rng(5)
+    for r in rng.uniform(-1, 1, 400):
+        # partition of unity and vanishing first moment over the integer
+        # shifts (Peskin 2002), for the exact kernel and for the table
+        for ph in (po.phi, po.phi_exact):
+            assert abs(sum(ph(r + i) for i in range(-3, 4)) - 1.0) < 1e-13
+            assert abs(sum((r + i) * ph(r + i) for i in range(-3, 4))) < 1e-13
+        assert po.phi(r) == po.phi(-r)
+        # linear interpolation error bound h^2/8 max|phi''| with h = 2/1024
+        assert abs(po.phi(r) - po.phi_exact(r)) < (2 / 1024) ** 2 / 8 * 1.0
+    assert po.phi(2.5) == 0.0 and po.phi(-2.0) == 0.0
+
+
+# -- analytic flows (tests/verification) ----------------------------------
+def channel(coll, tau, nx, ny, G, steps, periodic_y=False):
+    p = po.default_params(nx=nx, ny=ny, tau=tau, boundary_type=1, body_force_x=G,
+                          collision_model=coll, periodic_y=1 if periodic_y else 0)
+    s = po.Sim(p)
+    s.initialize()
+    s.step(steps)
+    return s
+
+
+@pytest.mark.parametrize("coll", [0, 1, 2])
+@pytest.mark.parametrize("tau", [0.8, 1.2])
+def test_poiseuille_profile_and_flow_rate(coll, tau):
+    # test_poiseuille_collision_operators.cpp:86-104: 8 x 34 channel, H = 32
+    nx, ny, G = 8, 34, 1e-6
+    nu = (tau - 0.5) / 3
+    H = ny - 2
+    s = channel(coll, tau, nx, ny, G, int(5.0 * H * H / nu))
+    ux = s.field()["ux"][1:H + 1, nx // 2]
+    yp = np.arange(H) + 0.5
+    ua = G / (2 * nu) * yp * (H - yp)
+    l2 = math.sqrt(np.sum((ux - ua) ** 2) / np.sum(ua ** 2))
+    q = ux.sum() / (G * H ** 3 / (12 * nu))
+    assert l2 < 0.02 and abs(q - 1) < 0.02, (l2, q)
+
+
+def test_periodic_channel_translation_invariant():
+    # test_poiseuille_collision_operators.cpp:115-169
+    def run(nx):
+        s = channel(0, 0.8, nx, 34, 1e-6, 20000)
+        ux = s.field()["ux"]
+        spread = np.max(np.ptp(ux[1:33, :], axis=1))
+        return spread, ux[1:33, nx // 2]
+    s4, u4 = run(4)
+    s32, u32 = run(32)
+    assert s4 < 1e-10 and s32 < 1e-10
+    assert np.max(np.abs(u4 - u32)) < 1e-8
+
+
+def test_startup_poiseuille_series():
+    # the channel of the Poiseuille configuration (H = 78, G = 3e-5, tau 0.8)
+    # is translation invariant in x; 10^4 steps against the start-up series
+    # u(y,t) = u_ss(y) - sum_n 4GH^2/(nu pi^3 (2n+1)^3) sin((2n+1) pi y/H) exp(-(2n+1)^2 pi^2 nu t/H^2)
+    ny, G, tau, T = 80, 3e-5, 0.8, 10000
+    nu = (tau - 0.5) / 3
+    H = ny - 2
+    s = channel(0, tau, 4, ny, G, T)
+    ux = s.field()["ux"][1:H + 1, 1]
+    y = np.arange(H) + 0.5
+    u = G / (2 * nu) * y * (H - y)
+    for n in range(200):
+        k = 2 * n + 1
+        u -= 4 * G * H * H / (nu * math.pi ** 3 * k ** 3) * np.sin(k * math.pi * y / H) * \
+            math.exp(-k * k * math.pi ** 2 * nu * T / (H * H))
+    assert np.max(np.abs(ux - u)) / np.max(u) < 0.01
+
+
+def test_taylor_green_decay():
+    # test_taylor_green.cpp:55-117: doubly periodic 64^2, tau 0.8, U0 = 0.04;
+    # kinetic energy decays by e^-2 after t = 1 / (2 nu k^2)
+    n, tau, U0 = 64, 0.8, 0.04
+    nu = (tau - 0.5) / 3
+    k = 2 * math.pi / n
+    p = po.default_params(nx=n, ny=n, tau=tau, boundary_type=1, periodic_y=1)
+    s = po.Sim(p)
+    s.initialize()
+    yy, xx = np.mgrid[0:n, 0:n].astype(float)
+    ux = -U0 * np.cos(k * xx) * np.sin(k * yy)
+    uy = U0 * np.sin(k * xx) * np.cos(k * yy)
+    rho = 1.0 - 3 * U0 * U0 / 4 * (np.cos(2 * k * xx) + np.cos(2 * k * yy))
+    f = np.stack([feq_np(r, a, b) for r, a, b in zip(rho.ravel(), ux.ravel(), uy.ravel())], axis=1)
+    s.set_f(f.reshape(9, n, n))
+    s.set_macro(rho, ux, uy)
+    E0 = np.sum(ux ** 2 + uy ** 2)
+    T = int(round(1 / (2 * nu * k * k)))
+    s.step(T)
+    fd = s.field()
+    E = np.sum(fd["ux"] ** 2 + fd["uy"] ** 2)
+    expect = math.exp(-4 * nu * k * k * T)
+    assert abs(E / E0 / expect - 1) < 0.02
+
+
+def test_pure_diffusion_variance():
+    # test_pure_diffusion.cpp:28-115: 256 x 8 doubly periodic, D = 0.05,
+    # sigma0 = 8, 5000 steps: sigma^2 = sigma0^2 + 2 D t
+    nx, ny, D, s0, T = 256, 8, 0.05, 8.0, 5000
+    p = po.default_params(nx=nx, ny=ny, tau=0.8, boundary_type=1, periodic_y=1, scalar_enabled=1,
+                          diffusivity=[D], scalar_periodic_y=1)
+    s = po.Sim(p)
+    s.initialize()
+    s.scalar_initialize(0.0)
+    x = np.arange(nx)
+    c = np.exp(-((x - nx / 2) ** 2) / (2 * s0 * s0))
+    for yy in range(ny):
+        for xx in range(nx):
+            s.set_scalar_point(xx, yy, float(c[xx]), 0)
+    s.step(T)
+    C = s.scalar().mean(axis=0)
+    m = C.sum()
+    mu = (x * C).sum() / m
+    var = ((x - mu) ** 2 * C).sum() / m
+    assert abs(var / (s0 * s0 + 2 * D * T) - 1) < 0.02
+    assert abs(m - c.sum()) < 1e-9 * m   # exact scalar mass conservation
+
+
+def test_mass_conservation_channel_with_walls():
+    s = channel(2, 0.8, 20, 16, 3e-5, 2000)
+    rho = s.field()["rho"]
+    assert abs(rho[1:-1].sum() - 20 * 14) < 1e-10
+
+
+# -- IBM ----------------------------------------------------------------------
+def _ring(cx, cy, r, n, model=0):
+    p = po.default_params(nx=64, ny=48, tau=0.8, boundary_type=1)
+    s = po.Sim(p)
+    s.add_capsule(cx, cy, r, n, po.default_membrane(model=model, k_stretch=0.1))
+    s.initialize()
+    return s
+
+
+@pytest.mark.parametrize("cx", [30.3, 0.4])
+def test_ibm_spread_conserves_momentum(cx):
+    # test_ibm_momentum_conservation.cpp: sum over cells of the spread force
+    # equals sum_k f_k ds_k, also for a ring straddling the periodic seam
+    s = _ring(cx, 20.7, 5.0, 24)
+    nd = s.nodes()
+    x, y = nd["x"].copy(), nd["y"].copy()
+    # deform so the membrane forces are non-zero, then take one step
+    x = x + 0.05 * np.cos(3 * np.arange(24))
+    s.set_nodes(x=x)
+    s.step(1)
+    assert s.last_capped() == 0       # the force cap would break conservation
+    nd = s.nodes()
+    Lx = 64.0
+    def mi(d):
+        return d - Lx * np.round(d / Lx)
+    dprev = np.hypot(mi(x - np.roll(x, 1)), y - np.roll(y, 1))
+    dnext = np.hypot(mi(np.roll(x, -1) - x), np.roll(y, -1) - y)
+    ds = 0.5 * (dprev + dnext)
+    Fsum = s.field()
+    assert abs(Fsum["fx"].sum() - np.sum(nd["fx"] * ds)) < 1e-12
+    assert abs(Fsum["fy"].sum() - np.sum(nd["fy"] * ds)) < 1e-12
+
+
+# -- membrane laws --------------------------------------------------------
+@pytest.mark.parametrize("model", [0, 1, 2, 3])
+def test_membrane_tension_laws(model):
+    # test_membrane_constitutive.cpp:52-86: uniform stretch lambda of a regular
+    # ring; the node force of a tension T on both edges is 2 T sin(pi/n)
+    n, r, G, k, C = 24, 5.0, 0.07, 0.09, 10.0
+    mem = po.default_membrane(model=model, G_s=G, k_stretch=k, C_skalak=C, k_bend=0.0, k_area=0.0,
+                              k_perimeter=0.0, gamma_visc=0.0, eta_membrane=0.0)
+    for lam in (0.8, 1.0, 1.2, 1.6):
+        p = po.default_params(nx=64, ny=48, boundary_type=1)
+        s = po.Sim(p)
+        s.add_capsule(32.0, 24.0, r, n, mem)
+        s.initialize()
+        nd = s.nodes()
+        s.set_nodes(x=32.0 + lam * (nd["x"] - 32.0), y=24.0 + lam * (nd["y"] - 24.0))
+        s.membrane_forces_only()
+        nd = s.nodes()
+        Fn = np.hypot(nd["fx"], nd["fy"])
+        L0 = 2 * r * math.sin(math.pi / n)
+        L = lam * L0
+        if model == 0:
+            T = k * (lam - 1)
+        elif model == 1:
+            T = G * (lam - lam ** -3)
+        elif model == 2:
+            T = G * lam * (lam * lam - 1) * (1 + C)
+        else:
+            x0 = 1 / 2.2
+            kbt = G * 2.2 * L0 / (L0 * (1 / (2 * (1 - x0) ** 3) + 1))
+            kpow = kbt * (1 / (4 * (1 - x0) ** 2) - 0.25 + x0) * L0
+            xx = min(lam / 2.2, 0.99)
+            T = kbt * (1 / (4 * (1 - xx) ** 2) - 0.25 + xx) - kpow / L
+        assert np.max(np.abs(Fn - 2 * abs(T) * math.sin(math.pi / n))) < 1e-10
+        # the force points inward under tension, outward under compression
+        if abs(T) > 1e-12:
+            rad = (nd["x"] - 32.0) * nd["fx"] + (nd["y"] - 24.0) * nd["fy"]
+            assert np.all(np.sign(rad) == -np.sign(T))
+
+
+def _helfrich_energy(x, y, th0, ds0, kb):
+    ep = np.stack([x - np.roll(x, 1), y - np.roll(y, 1)])
+    en = np.stack([np.roll(x, -1) - x, np.roll(y, -1) - y])
+    th = np.arctan2(ep[0] * en[1] - ep[1] * en[0], ep[0] * en[0] + ep[1] * en[1])
+    return 0.5 * kb * np.sum((th - th0) ** 2 / ds0)
+
+
+def test_helfrich_bending_is_gradient_and_force_free_at_rest():
+    # test_capsule_bending.cpp:49-94
+    n, r, kb = 20, 4.0, 1e-2
+    mem = po.default_membrane(model=0, k_stretch=0.0, k_bend=kb, use_helfrich=1, k_area=0.0,
+                              k_perimeter=0.0, gamma_visc=0.0, eta_membrane=0.0, shape=1,
+                              aspect_ratio=0.7)
+    p = po.default_params(nx=64, ny=48, boundary_type=1)
+    s = po.Sim(p)
+    s.add_capsule(30.0, 22.0, r, n, mem)
+    s.initialize()
+    s.membrane_forces_only()
+    nd = s.nodes()
+    assert np.max(np.hypot(nd["fx"], nd["fy"])) < 1e-12
+    x0, y0 = nd["x"].copy(), nd["y"].copy()
+    ep = np.stack([x0 - np.roll(x0, 1), y0 - np.roll(y0, 1)])
+    en = np.stack([np.roll(x0, -1) - x0, np.roll(y0, -1) - y0])
+    th0 = np.arctan2(ep[0] * en[1] - ep[1] * en[0], ep[0] * en[0] + ep[1] * en[1])
+    ds0 = 0.5 * (np.hypot(*ep) + np.hypot(*en))
+    rng = np.random.default_rng(6)
+    x = x0 + 0.2 * rng.standard_normal(n)
+    y = y0 + 0.2 * rng.standard_normal(n)
+    s.set_nodes(x=x, y=y)
+    s.membrane_forces_only()
+    nd = s.nodes()
+    h = 1e-6
+    gx, gy = np.zeros(n), np.zeros(n)
+    for i in range(n):
+        for arr, g in ((x, gx), (y, gy)):
+            a = arr.copy()
+            a[i] += h
+            ep_ = _helfrich_energy(a if arr is x else x, a if arr is y else y, th0, ds0, kb)
+            a[i] -= 2 * h
+            em_ = _helfrich_energy(a if arr is x else x, a if arr is y else y, th0, ds0, kb)
+            g[i] = (ep_ - em_) / (2 * h)
+    Fmax = np.max(np.hypot(nd["fx"], nd["fy"]))
+    assert np.max(np.abs(nd["fx"] + gx)) < 1e-6 * Fmax
+    assert np.max(np.abs(nd["fy"] + gy)) < 1e-6 * Fmax
+    assert abs(nd["fx"].sum()) < 1e-12 and abs(nd["fy"].sum()) < 1e-12
+    torque = np.sum(x * nd["fy"] - y * nd["fx"])
+    assert abs(torque) < 1e-11
+
+
+def test_capsule_geometry_closed_forms():
+    n, r = 24, 5.0
+    s = _ring(32.0, 24.0, r, n)
+    assert abs(s.geometry(0, 0) - 0.5 * n * r * r * math.sin(2 * math.pi / n)) < 1e-12
+    assert abs(s.geometry(0, 1) - 2 * n * r * math.sin(math.pi / n)) < 1e-12
+    assert abs(s.geometry(0, 2) - 32.0) < 1e-12 and abs(s.geometry(0, 3) - 24.0) < 1e-12
+    assert abs(s.geometry(0, 4)) < 1e-12
+    # Taylor deformation of an ellipse b/a = 0.6 is (a - b)/(a + b) = 0.25
+    # (test_capsule_deformation.cpp:34-60)
+    p = po.default_params(nx=64, ny=48, boundary_type=1)
+    e = po.Sim(p)
+    e.add_capsule(32.0, 24.0, 8.0, 64, po.default_membrane(shape=1, aspect_ratio=0.6))
+    e.initialize()
+    assert abs(e.geometry(0, 4) - 0.25) < 2e-3
+    assert abs(e.geometry(0, 7)) < 1e-9
+
+
+# -- contact force --------------------------------------------------------
+def test_pair_force_closed_form_and_antisymmetry():
+    # test_contact_friction_damping.py: F = eps (sigma/r)^p n, and
+    # pairForce(a, b) = -pairForce(b, a) with damping and friction on
+    p = po.default_params(nx=64, ny=48, boundary_type=1, rep_damping_normal=0.02,
+                          rep_friction_coeff=0.3)
+    s = po.Sim(p)
+    f = s.pair_force([10.0, 10.0], [0, 0], [8.5, 10.0], [0, 0])
+    assert abs(f[0] - 0.05 * (1 / 1.5) ** 4) < 1e-16 and f[1] == 0.0
+    assert np.all(s.pair_force([10.0, 10.0], [0, 0], [6.0, 10.0], [0, 0]) == 0.0)   # beyond r_cut
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        pa, pb = rng.uniform(0, 3, 2), rng.uniform(0, 3, 2)
+        va, vb = 0.01 * rng.standard_normal(2), 0.01 * rng.standard_normal(2)
+        a = s.pair_force(pa, va, pb, vb)
+        b = s.pair_force(pb, vb, pa, va)
+        assert np.max(np.abs(a + b)) < 1e-16
+    # periodic minimum image across the seam (Lx = 64)
+    g = s.pair_force([0.5, 10.0], [0, 0], [63.5, 10.0], [0, 0])
+    assert g[0] > 0
+
+
+def test_cell_list_matches_brute_force():
+    # > 100 nodes takes the cell-list path (repulsion.cpp:95); the force on
+    # every node must equal the O(K^2) sum over other capsules' nodes plus walls
+    case_p = po.default_params(nx=60, ny=30, boundary_type=1)
+    s = po.Sim(case_p)
+    mem = po.default_membrane(model=2, G_s=0.06, C_skalak=10.0, k_bend=0, k_area=0, k_perimeter=0,
+                              gamma_visc=0, eta_membrane=0)
+    s.add_capsule_random(8, 4, 4, 56, 26, 3.0, 3.0, 20, mem, 0, 3, 0.2, 0)
+    s.initialize()
+    nd0 = s.nodes()
+    s.membrane_forces_only()
+    mf = s.nodes()
+    off = s.offsets()
+    assert off[-1] > 100
+    # run the force stage through a step with no fluid coupling effect on forces
+    s.step(1)
+    got = s.nodes()
+    owner = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    x, y = nd0["x"], nd0["y"]
+    fx, fy = mf["fx"].copy(), mf["fy"].copy()
+    for a in range(len(x)):
+        for b in range(len(x)):
+            if owner[a] == owner[b]:
+                continue
+            F = s.pair_force([x[a], y[a]], [0, 0], [x[b], y[b]], [0, 0])
+            fx[a] += F[0]
+            fy[a] += F[1]
+        for yw in (0.5, 30 - 1.5):
+            F = s.pair_force([x[a], y[a]], [0, 0], [x[a], yw], [0, 0])
+            fx[a] += F[0]
+            fy[a] += F[1]
+    assert np.max(np.abs(got["fx"] - fx)) < 1e-13
+    assert np.max(np.abs(got["fy"] - fy)) < 1e-13
+
+
+# -- placement ------------------------------------------------------------
+class MT19937:
+    """std::mt19937 (32-bit Mersenne twister, C++ [rand.eng.mers])."""
+
+    def __init__(self, seed):
+        self.mt = [0] * 624
+        self.mt[0] = seed & 0xFFFFFFFF
+        for i in range(1, 624):
+            self.mt[i] = (1812433253 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 30)) + i) & 0xFFFFFFFF
+        self.i = 624
+
+    def __call__(self):
+        if self.i >= 624:
+            for k in range(624):
+                y = (self.mt[k] & 0x80000000) | (self.mt[(k + 1) % 624] & 0x7FFFFFFF)
+                self.mt[k] = self.mt[(k + 397) % 624] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+            self.i = 0
+        y = self.mt[self.i]
+        self.i += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        return y
+
+
+def test_mt19937_known_value():
+    g = MT19937(5489)
+    for _ in range(9999):
+        g()
+    assert g() == 4123659995      # the C++ standard's check value
+
+
+def test_random_placement_matches_independent_implementation():
+    # Simulation::addCapsuleRandom (simulation.cpp:220-274): uniform_real_distribution
+    # on mt19937 is generate_canonical<double, 53> from two 32-bit draws.
+    nx, ny, count, r, seed, gap = 200, 60, 15, 3.0, 42, 1.0
+    x0, y0, x1, y1 = 10.0, 5.0, 190.0, 55.0
+    p = po.default_params(nx=nx, ny=ny, boundary_type=1)
+    s = po.Sim(p)
+    placed = s.add_capsule_random(count, x0, y0, x1, y1, r, r, 12, po.default_membrane(), 0, seed, gap, 0)
+    s.initialize()
+    g = MT19937(seed)
+
+    def canon():
+        lo = g()
+        hi = g()
+        v = (lo + hi * 4294967296.0) / 18446744073709551616.0
+        return v if v < 1.0 else math.nextafter(1.0, 0.0)
+
+    centers = []
+    attempts = 0
+    # the overlap test uses the effective radius sqrt(A/pi) of the placed
+    # regular 12-gon, A = n r^2 sin(2 pi / n) / 2
+    reff = math.sqrt(0.5 * 12 * r * r * math.sin(2 * math.pi / 12) / math.pi)
+    while len(centers) < count and attempts < count * 500:
+        cx = x0 + (x1 - x0) * canon()
+        cy = y0 + (y1 - y0) * canon()
+        rr = r + (r - r) * canon()
+        ok = all(math.hypot(cx - ox, cy - oy) >= rr + reff + gap for ox, oy, _ in centers)
+        ok = ok and not (cy - rr < 0 + gap or cy + rr > ny - 1 - gap)
+        if ok:
+            centers.append((cx, cy, rr))
+        attempts += 1
+    assert placed == len(centers)
+    for c, (cx, cy, _) in enumerate(centers):
+        # centroid of a regular 12-gon about its centre, to roundoff
+        assert abs(s.geometry(c, 2) - cx) < 1e-12 and abs(s.geometry(c, 3) - cy) < 1e-12

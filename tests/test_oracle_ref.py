@@ -1,0 +1,101 @@
+"""The oracle against the reference's own components, run side by side.
+
+oracle/_ref/libsfref.so is compiled from the reference's sources where they
+lie (``make -C oracle ref``: the lbm, coupling and membrane translation units at
+-O2 -ffp-contract=off, OpenMP at one thread) and driven in Simulation::step
+order.  It exists only where the reference tree is present; elsewhere these
+tests skip and tests/test_oracle_golden.py carries the same check through the
+committed vectors.  Every comparison is bitwise.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+
+
+def run(which, coll=0, mdf=1, model=2, helf=0, ncap=8, steps=150, scalar=0, nx=100, ny=40,
+        seed=42, gvisc=0.01, damp=0.0, fric=0.0, shape=0, boundary=1, periodic_y=0, nodes=0,
+        eta=0.0, k_area=None, k_per=None):
+    p = po.default_params(nx=nx, ny=ny, tau=0.8, boundary_type=boundary, body_force_x=3e-5,
+                          collision_model=coll, ibm_iterations=mdf, scalar_enabled=scalar,
+                          diffusivity=[0.05], rep_damping_normal=damp, rep_friction_coeff=fric,
+                          periodic_y=periodic_y)
+    s = po.Sim(p, which)
+    kw = dict(model=model, G_s=0.06, C_skalak=10.0, use_helfrich=helf,
+              k_bend=1e-3 if helf else 0.005, gamma_visc=gvisc, shape=shape, aspect_ratio=0.6,
+              eta_membrane=eta)
+    if k_area is not None:
+        kw["k_area"] = k_area
+    if k_per is not None:
+        kw["k_perimeter"] = k_per
+    m = po.default_membrane(**kw)
+    placed = s.add_capsule_random(ncap, 10, 8, nx - 10, ny - 8, 3.0, 4.5, nodes, m, 0, seed, 1.0, 0)
+    s.initialize()
+    if scalar:
+        s.scalar_initialize(0.0)
+        for y in range(ny):
+            for x in range(nx):
+                s.set_scalar_point(x, y, float(np.exp(-((x - nx / 2) ** 2 + (y - ny / 2) ** 2) / 128.0)), 0)
+    s.step(steps)
+    out = dict(s.field())
+    out.update(s.nodes())
+    out["f"] = s.f()
+    if scalar:
+        out["C"] = s.scalar()
+    return out, placed, s.last_capped()
+
+
+CASES = dict(
+    bgk=dict(coll=0, ncap=0),
+    mrt=dict(coll=1, ncap=0),
+    closed_box=dict(coll=2, ncap=0, boundary=2),
+    periodic_y=dict(coll=0, ncap=0, periodic_y=1),
+    reg_caps_mdf3=dict(coll=2, mdf=3),
+    bgk_mdf5=dict(coll=0, mdf=5, ncap=4),
+    bgk_caps=dict(coll=0, ncap=12),
+    mrt_helfrich_adr=dict(coll=1, helf=1, scalar=1, ncap=10),
+    wlc=dict(model=3, ncap=6),
+    neo_hookean=dict(model=1, ncap=6),
+    hookean_ellipse=dict(model=0, ncap=6, shape=1),
+    biconcave=dict(model=2, ncap=4, shape=2),
+    kelvin_voigt=dict(model=2, ncap=6, eta=0.05),
+    area_perimeter_penalty=dict(model=2, ncap=6, k_area=0.2, k_per=0.1),
+    allpairs_few_nodes=dict(ncap=2, nodes=12),
+    dem_contact=dict(ncap=12, damp=0.01, fric=0.1),
+)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_oracle_equals_reference_components(name):
+    a, pa, ca = run("oracle", **CASES[name])
+    b, pb, cb = run("ref", **CASES[name])
+    assert pa == pb and ca == cb
+    for k in a:
+        x, y = np.asarray(a[k]), np.asarray(b[k])
+        assert x.shape == y.shape, k
+        assert np.array_equal(x, y), (k, float(np.max(np.abs(x - y))) if x.size else 0.0)
+
+
+@pytest.mark.parametrize("model", [0, 1, 2, 3])
+@pytest.mark.parametrize("helf", [0, 1])
+def test_membrane_forces_equal_reference(model, helf):
+    # random deformations of a placed capsule; every force law in isolation
+    outs = []
+    for which in ("oracle", "ref"):
+        p = po.default_params(nx=64, ny=48, boundary_type=1)
+        s = po.Sim(p, which)
+        m = po.default_membrane(model=model, G_s=0.05, k_stretch=0.05, use_helfrich=helf,
+                                k_bend=2e-3, k_area=0.1, k_perimeter=0.05, gamma_visc=0.02,
+                                shape=1, aspect_ratio=0.7)
+        s.add_capsule(30.2, 21.7, 5.0, 26, m, 0)
+        s.initialize()
+        rng = np.random.default_rng(model * 10 + helf)
+        nd = s.nodes()
+        s.set_nodes(x=nd["x"] + 0.3 * rng.standard_normal(26), y=nd["y"] + 0.3 * rng.standard_normal(26),
+                    vx=0.01 * rng.standard_normal(26), vy=0.01 * rng.standard_normal(26))
+        s.membrane_forces_only()
+        outs.append(s.nodes())
+    for k in ("fx", "fy"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
