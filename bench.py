@@ -111,6 +111,33 @@ def cpu_baseline(case, steps):
     return case["nx"] * case["ny"] * steps / dt / 1e6, dt
 
 
+def calibrate_copy(mib):
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}))
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the collide-stream kernel, as measured with the
+    FETCH_SIZE and WRITE_SIZE counters and scaled by the copy-kernel calibration
+    (profiles/k_lbm_traffic_4096x1024.txt); None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_lbm_traffic_4096x1024.txt")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,10 +148,16 @@ def main():
     ap.add_argument("--no-timing-pass", action="store_true")
     ap.add_argument("--lbm-launch", default="auto",
                     help="'auto' (autotune) or BXxBY[xl] for a fixed collide-stream launch shape")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (the known-size "
+                         "reference for the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     case = c3_case()
     t0 = time.perf_counter()
@@ -170,7 +203,8 @@ def main():
         if lbm_ms:
             achieved = bpl * NX * NY / (lbm_ms * 1e-3) / 1e9
             roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_from_profile(),
                     "kernel": "k_lbm<MRT,fp32,channel,scalar> (fused collide + push-stream "
                               "+ lagged scalar step)" if fused else "k_lbm<MRT,fp32> (fused collide + push-stream)",
                     "kernel_ms": round(lbm_ms, 5), "bytes_per_lup": bpl}

@@ -3,7 +3,14 @@
 The vectors were written from the reference's own components (see
 tests/golden/make_golden.py); this test needs neither the reference nor the
 oracle at run time.  Bar: relative L2 <= 1e-12 for rho, u, node positions and
-velocities, node forces and the scalar, and the same force-cap count.
+velocities and the scalar, and the same force-cap count.  Force fields (the
+Eulerian force and the node forces) are held to 1e-12 too, except under
+Helfrich bending.  There the device atan2 differs from glibc's (itself not
+correctly rounded) by an ulp in some turning angles, which leaves the state
+~1e-13 away; a node force is a difference of nearly equal quantities
+(k (lambda - 1), k_b (theta - theta_0) with lambda - 1 and theta - theta_0 of
+order 1e-3 .. 1e-5), so it amplifies that state difference by 1e3 .. 1e5 and
+the force bar is 1e-8 for those cases.
 """
 import json
 import os
@@ -21,6 +28,11 @@ with open(os.path.join(GOLDEN, "cases.json")) as fh:
 
 GROUPS = {"rho": ("rho",), "u": ("ux", "uy"), "force": ("fx", "fy"), "pos": ("x", "y"),
           "vel": ("vx", "vy"), "node_force": ("fx_node", "fy_node"), "C": ("C",)}
+FORCES = ("force", "node_force")
+
+
+def uses_helfrich(case):
+    return any(c.get("mem", {}).get("use_helfrich") for c in case.get("capsules", []))
 
 
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
@@ -40,5 +52,6 @@ def test_product_matches_golden(name, graph):
         b = np.concatenate([np.ravel(ref[p]) for p in parts])
         assert a.shape == b.shape, g
         err = rel_l2(a, b)
-        assert err <= 1e-12, f"{name}: {g} rel L2 {err:.3e}"
+        tol = 1e-8 if g in FORCES and uses_helfrich(spec["case"]) else 1e-12
+        assert err <= tol, f"{name}: {g} rel L2 {err:.3e}"
     assert int(got["capped"]) == int(ref["capped"])
