@@ -94,13 +94,14 @@ def _lib(path: str, prefix: str):
     g("phi").restype = C.c_double
     g("phi").argtypes = [C.c_double]
     g("set_nodes").argtypes = [C.c_void_p] + [_DP] * 4
+    g("set_flags").argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    g("get_flags").argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     if prefix == "sfo_":
         for n in ("default_params",):
             g(n).argtypes = [C.POINTER(Params)]
         g("default_membrane").argtypes = [C.POINTER(Membrane)]
         g("set_f").argtypes = [C.c_void_p, _DP]
         g("set_macro").argtypes = [C.c_void_p, _DP, _DP, _DP]
-        g("get_flags").argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         g("max_speed").restype = C.c_double
         g("max_speed").argtypes = [C.c_void_p]
         g("rest_lengths").argtypes = [C.c_void_p, _DP]
@@ -243,6 +244,12 @@ class Sim:
         a = np.zeros(self.nx * self.ny, dtype=np.int32)
         self._f("get_flags")(self.h, a.ctypes.data_as(C.POINTER(C.c_int)))
         return a.reshape(self.ny, self.nx)
+
+    def set_flags(self, flags):
+        a = np.ascontiguousarray(flags, dtype=np.int32).ravel()
+        assert a.size == self.nx * self.ny
+        self._keep_flags = a
+        self._f("set_flags")(self.h, a.ctypes.data_as(C.POINTER(C.c_int)))
 
     def scalar(self, sp=0):
         a = np.zeros(self.nx * self.ny)

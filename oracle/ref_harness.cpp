@@ -234,6 +234,19 @@ void sfr_set_nodes(sfr_sim* s, const double* x, const double* y, const double* v
         }
     }
 }
+void sfr_get_flags(const sfr_sim* s, int* flags) {
+    const LatticeField& f = s->lbm->field();
+    const int nx = s->params.nx, ny = s->params.ny;
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) flags[y * nx + x] = static_cast<int>(f.getCellType(x, y));
+}
+void sfr_set_flags(sfr_sim* s, const int* flags) {
+    LatticeField& f = s->lbm->field();
+    const int nx = s->params.nx, ny = s->params.ny;
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) f.setCellType(x, y, static_cast<CellType>(flags[y * nx + x]));
+    s->lbm->rebuildBoundaryNodeLists();
+}
 void sfr_get_field(const sfr_sim* s, double* rho, double* ux, double* uy, double* fx,
                    double* fy) {
     const LatticeField& f = s->lbm->field();

@@ -1177,6 +1177,13 @@ void sfo_set_macro(sfo_sim* s, const double* rho, const double* ux, const double
     if (uy) std::memcpy(s->uy.data(), uy, N * 8);
 }
 void sfo_get_flags(const sfo_sim* s, int* flags) { std::memcpy(flags, s->flags.data(), s->N * 4); }
+// LatticeField::setCellType for every cell, then LBMSolver::rebuildBoundaryNodeLists
+// (lbm_solver.h:47, lbm_solver.cpp:109-128)
+void sfo_set_flags(sfo_sim* s, const int* flags) {
+    std::memcpy(s->flags.data(), flags, s->N * 4);
+    s->solid.clear();
+    for (int n = 0; n < s->N; ++n) if (s->flags[n] == SOLID) s->solid.push_back(n);
+}
 void sfo_get_scalar(const sfo_sim* s, double* C, int sp) {
     if (sp >= 0 && sp < s->nsp) std::memcpy(C, s->C[sp].data(), s->N * 8);
 }

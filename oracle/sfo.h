@@ -87,6 +87,7 @@ void   sfo_set_f(sfo_sim* s, const double* f);
 void   sfo_set_macro(sfo_sim* s, const double* rho, const double* ux,
                      const double* uy);
 void   sfo_get_flags(const sfo_sim* s, int* flags);
+void   sfo_set_flags(sfo_sim* s, const int* flags);
 void   sfo_get_scalar(const sfo_sim* s, double* C, int species);
 int    sfo_last_capped(const sfo_sim* s);
 double sfo_max_speed(const sfo_sim* s);

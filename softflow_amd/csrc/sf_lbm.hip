@@ -312,17 +312,28 @@ __device__ __forceinline__ void adr_cell(const LbmArgs& a, int x, int y, int n, 
             C = acc;
         }
         const double oms = a.oms[s];
+        if constexpr (!FP32) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            if constexpr (!FP32) {
+            for (int q = 0; q < 9; ++q) {
                 const double eu = static_cast<double>(dcx(q)) * ux + static_cast<double>(dcy(q)) * uy;
                 const double ge = dw(q) * C * (1.0 + eu * 3.0);
                 g[q] = g[q] - oms * (g[q] - ge);
-            } else {
-                const float eu = static_cast<float>(dcx(q) * ux + dcy(q) * uy);
-                const float ge = static_cast<float>(dw(q)) * static_cast<float>(C) * (1.0f + 3.0f * eu);
-                g[q] = g[q] - static_cast<float>(oms) * (g[q] - ge);
             }
+        } else {
+            // FP32: the eight moving populations relax in FP32; the rest
+            // population takes C minus their sum (FP64), which is its exact-
+            // arithmetic value.  Evaluated directly, the FP32 weights (which do
+            // not sum to one) would drift the scalar mass by ~1e-8 per step.
+            const float Cf = static_cast<float>(C), omf = static_cast<float>(oms);
+            double moving = 0.0;
+#pragma unroll
+            for (int q = 1; q < 9; ++q) {
+                const float eu = static_cast<float>(dcx(q) * ux + dcy(q) * uy);
+                const float ge = static_cast<float>(dw(q)) * Cf * (1.0f + 3.0f * eu);
+                g[q] = g[q] - omf * (g[q] - ge);
+                moving += static_cast<double>(g[q]);
+            }
+            g[0] = static_cast<float>(C - moving);
         }
 #pragma unroll
         for (int q = 0; q < 9; ++q) {

@@ -7,6 +7,8 @@ A case is a dict:
     capsules: list of {count, region:(x0,y0,x1,y1), rmin, rmax, nodes, seed, min_gap, mem:{...}}
     scalar: {D, blob:(cx, cy, sigma)}      (optional)
     rep: {damping_normal, friction_coeff}  (optional)
+    solids: list of ('rect', x0, y0, x1, y1) | ('circle', cx, cy, r): extra SOLID
+            cells set after initialize (LatticeField::setCellType)   (optional)
 Membrane keys follow the oracle's struct (model, G_s, C_skalak, k_bend, use_helfrich, ...).
 """
 from __future__ import annotations
@@ -24,6 +26,21 @@ def _blob(case):
     nx, ny = case["nx"], case["ny"]
     yy, xx = np.mgrid[0:ny, 0:nx]
     return np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / (2.0 * sg * sg))
+
+
+def solid_mask(case):
+    """Cells inside the case's extra solids (channel_builder.cpp:26-31 tests)."""
+    nx, ny = case["nx"], case["ny"]
+    yy, xx = np.mgrid[0:ny, 0:nx].astype(float)
+    m = np.zeros((ny, nx), dtype=bool)
+    for s in case.get("solids", []):
+        if s[0] == "rect":
+            x0, y0, x1, y1 = s[1:]
+            m |= (xx >= x0) & (xx <= x1) & (yy >= y0) & (yy <= y1)
+        else:
+            cx, cy, r = s[1:]
+            m |= (xx - cx) ** 2 + (yy - cy) ** 2 <= r * r
+    return m
 
 
 def oracle_sim(case, which="oracle"):
@@ -53,6 +70,10 @@ def oracle_sim(case, which="oracle"):
                                    spec.get("nodes", 0), m, spec.get("type", 0), spec.get("seed", 42),
                                    spec.get("min_gap", 1.0), 0)
     sim.initialize()
+    if case.get("solids"):
+        fl = sim.flags().copy()
+        fl[solid_mask(case)] = 1
+        sim.set_flags(fl)
     blob = _blob(case)
     if blob is not None:
         sim.scalar_initialize(0.0)
@@ -109,6 +130,11 @@ def product_sim(case, graph=True):
                                  spec.get("nodes", 0), mp, spec.get("type", 0), spec.get("seed", 42),
                                  spec.get("min_gap", 1.0))
     sim.initialize()
+    if case.get("solids"):
+        fld = sim.field()
+        ys, xs = np.nonzero(solid_mask(case))
+        for y, x in zip(ys, xs):
+            fld.setCellType(int(x), int(y), sc.CellType.SOLID)
     blob = _blob(case)
     if blob is not None:
         ad = sim.advectionDiffusion()

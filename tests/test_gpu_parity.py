@@ -5,6 +5,8 @@ bar is relative L2 <= 1e-12 on rho, u, node positions and velocities (in
 practice the fields agree bit for bit).  FP32 mode (f - w_q storage, FP32
 collide) is held to the tolerances written in each test.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -163,3 +165,39 @@ def test_c2_capsule_suspension_10k_steps():
                 capsules=[dict(count=30, region=(20, 10, 380, 70), rmin=3.0, rmax=3.0, nodes=20,
                                seed=42, mem=SKALAK)])
     _compare(case, 10000, 1e-12)
+
+
+@pytest.mark.parametrize("coll", [0, 1, 2])
+def test_solid_obstacles_fp64(coll):
+    # SOLID cells beyond the channel walls (LatticeField::setCellType): the
+    # generic-flags variant of the collide-stream kernel and the scalar's
+    # bounce-back at solids
+    case = dict(name=f"solids_coll{coll}", nx=60, ny=30, tau=0.8, bfx=3e-5, coll=coll,
+                scalar=dict(D=0.05, blob=(20, 15, 4.0)),
+                solids=[("rect", 20, 10, 24, 16), ("circle", 42.3, 18.6, 4.2)])
+    got, ref = _compare(case, 1500, 1e-12)
+    assert np.array_equal(got["ux"], ref["ux"])
+
+
+def test_fp32_error_report():
+    """FP32 deviation storage against FP64 on a capsule suspension: velocity
+    field, capsule centroid trajectories and Taylor deformation."""
+    case = dict(name="fp32_report", nx=400, ny=80, tau=0.8, bfx=3e-5, coll=2, mdf=3,
+                capsules=[dict(count=30, region=(20, 10, 380, 70), rmin=3.0, rmax=3.0, nodes=20,
+                               seed=42, mem=SKALAK)])
+    a = product_sim(case)                            # FP64 (equal to the oracle, see above)
+    b = product_sim(dict(case, precision="fp32"))
+    ecen, eD = 0.0, 0.0
+    for _ in range(10):
+        a.run(200)
+        b.run(200)
+        ca, cb = a.capsules(), b.capsules()
+        for i in range(len(ca)):
+            pa, pb = ca[i].centroid(), cb[i].centroid()
+            ecen = max(ecen, math.hypot(pa.x - pb.x, pa.y - pb.y))
+            eD = max(eD, abs(ca[i].deformationIndex() - cb[i].deformationIndex()))
+    sa, sb = product_state(a), product_state(b)
+    eu = metrics(sb, sa, ("u",))["u"]
+    print(f"fp32 vs fp64 after 2000 steps: velocity rel L2 {eu:.3e}, centroid max |dx| {ecen:.3e}, "
+          f"Taylor D max |dD| {eD:.3e}", flush=True)
+    assert eu < 1e-3 and ecen < 1e-2 and eD < 1e-3

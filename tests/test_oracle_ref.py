@@ -99,3 +99,21 @@ def test_membrane_forces_equal_reference(model, helf):
         outs.append(s.nodes())
     for k in ("fx", "fy"):
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+@pytest.mark.parametrize("coll", [0, 1, 2])
+def test_solid_obstacles_equal_reference(coll):
+    # extra SOLID cells set after initialize (LatticeField::setCellType +
+    # rebuildBoundaryNodeLists): a rectangular block and a disc in a channel
+    from tests.cases import oracle_sim, oracle_state
+    case = dict(nx=60, ny=30, tau=0.8, bfx=3e-5, coll=coll, scalar=dict(D=0.05, blob=(20, 15, 4.0)),
+                solids=[("rect", 20, 10, 24, 16), ("circle", 42.3, 18.6, 4.2)])
+    a = oracle_sim(case)
+    b = oracle_sim(case, which="ref")
+    a.step(300)
+    b.step(300)
+    sa, sb = oracle_state(a), oracle_state(b)
+    assert int(np.sum(a.flags() == 1)) > 60 * 2 + 20
+    for k in ("rho", "ux", "uy", "fx", "fy", "C"):
+        assert np.array_equal(sa[k], sb[k]), k
+    assert np.array_equal(a.f(), b.f())
