@@ -154,6 +154,53 @@ int  sf_autotune(sf_ctx* ctx, int reps, char* report, int report_len);
 /* Fix the collide-stream launch shape instead (bx*by a multiple of 64, <= 256). */
 int  sf_set_lbm_launch(sf_ctx* ctx, int bx, int by, int xcd_banded);
 
+/* ---- Multi-direct forcing control and the learned warm start ------------
+ * (no counterpart in the reference; its MDF is a fixed iteration count,
+ * immersed_boundary.cpp:231-286, which stays the default).
+ *
+ * Residual mode (tol > 0): after each MDF interpolation the no-slip residual
+ * r = max_k |V_k - U(X_k)| is taken on the device; corrections stop once
+ * r <= tol, after at most max_corrections.  tol <= 0 keeps the reference's
+ * ibm_iterations - 1 corrections.  probe = 1 measures r once more after the
+ * final correction (for statistics).  Residual mode runs MDF even when
+ * ibm_iterations is 1. */
+int  sf_mdf_set_control(sf_ctx* ctx, double tol, int max_corrections, int probe);
+/* out4: mean corrections per step, mean and max final residual, steps; since
+ * the last call (which resets them). */
+int  sf_mdf_stats(sf_ctx* ctx, double* out4);
+/* Record (inputs, converged correction) rows of every node at every step into
+ * a device sample set of up to capacity rows (0 frees it).  A row holds
+ * SF_MLP_IN inputs then SF_MLP_OUT targets, FP32. */
+enum { SF_MLP_IN = 15, SF_MLP_OUT = 2, SF_MLP_WIDTH = 64 };
+int  sf_mdf_collect(sf_ctx* ctx, int64_t capacity);
+/* rows written so far; rows_out (may be NULL) receives them. */
+int  sf_mdf_samples(sf_ctx* ctx, int64_t* rows, float* rows_out);
+
+/* The warm-start network: SF_MLP_IN -> 64 -> 64 -> SF_MLP_OUT with GELU,
+ * FP32, with input/output normalisation (in_mean[15], in_scale[15],
+ * out_mean[2], out_scale[2]).  Parameters: W1[64][15], b1[64], W2[64][64],
+ * b2[64], W3[2][64], b3[2]. */
+typedef struct sf_mlp sf_mlp;
+int  sf_mlp_create(sf_mlp** net, int device, uint64_t seed);
+int  sf_mlp_destroy(sf_mlp* net);
+int  sf_mlp_num_params(int* n_params, int* n_norm);
+int  sf_mlp_get_params(sf_mlp* net, float* params, float* norm);
+int  sf_mlp_set_params(sf_mlp* net, const float* params, const float* norm);
+/* y[n][2] = network(x[n][15]) in physical units. */
+int  sf_mlp_predict(sf_mlp* net, int n, const float* x, float* y);
+/* Loss L = 1/(2n) sum |out - t^|^2 in normalised units and its gradient with
+ * respect to the parameters, for host samples x[n][15], t[n][2]. */
+int  sf_mlp_loss_grad(sf_mlp* net, int n, const float* x, const float* t, double* loss,
+                      float* grad);
+/* Adam (beta 0.9/0.999) over the context's sample set: epochs passes of
+ * shuffled mini-batches; fits the normalisation first if none was set.
+ * epoch_loss[epochs] (may be NULL) receives the mean loss of each pass. */
+int  sf_mlp_train(sf_mlp* net, sf_ctx* ctx, int epochs, int batch, float lr, uint64_t seed,
+                  double* epoch_loss);
+/* Use net for the first MDF correction of every step (warm = 1); the plain
+ * corrections follow as needed.  net is borrowed and must outlive the use. */
+int  sf_mdf_set_model(sf_ctx* ctx, sf_mlp* net, int warm);
+
 /* Device math probe used by the tests: op 0 sqrt, 1 divide (a/b), 2 atan2(a,b),
  * 3 pow(a, b) with integer b as used by the contact force. */
 int  sf_probe_math(int op, int n, const double* a, const double* b, double* out);

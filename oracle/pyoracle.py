@@ -108,6 +108,13 @@ def _lib(path: str, prefix: str):
         g("phi_exact").restype = C.c_double
         g("phi_exact").argtypes = [C.c_double]
         g("pair_force").argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP]
+        g("set_mdf_control").argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
+        g("mdf_stats").argtypes = [C.c_void_p, _DP]
+        g("mlp_num_params").restype = C.c_int
+        g("mlp_num_params").argtypes = [C.c_int]
+        g("mlp_predict").argtypes = [_DP, _DP, C.c_int, C.c_int, _DP, _DP]
+        g("mlp_loss_grad").restype = C.c_double
+        g("mlp_loss_grad").argtypes = [_DP, _DP, C.c_int, C.c_int, _DP, _DP, _DP]
         g("collide_cell").argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, _DP, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_double]
@@ -251,6 +258,14 @@ class Sim:
         self._keep_flags = a
         self._f("set_flags")(self.h, a.ctypes.data_as(C.POINTER(C.c_int)))
 
+    def set_mdf_control(self, tol, max_corrections, probe=False):
+        self._f("set_mdf_control")(self.h, tol, max_corrections, 1 if probe else 0)
+
+    def mdf_stats(self):
+        out = np.zeros(4)
+        self._f("mdf_stats")(self.h, _ptr(out))
+        return out
+
     def scalar(self, sp=0):
         a = np.zeros(self.nx * self.ny)
         self._f("get_scalar")(self.h, _ptr(a), sp)
@@ -275,6 +290,28 @@ class Sim:
         a = np.zeros(self.total_nodes)
         self._f("rest_lengths")(self.h, _ptr(a))
         return a
+
+
+def mlp_predict(params, norm, x, width=64):
+    """FP64 restatement of the warm-start network: y[n, 2] for x[n, 15]."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    nr = np.ascontiguousarray(norm, dtype=np.float64)
+    X = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 15)
+    Y = np.zeros((X.shape[0], 2))
+    oracle_lib().sfo_mlp_predict(_ptr(P), _ptr(nr), width, X.shape[0], _ptr(X), _ptr(Y))
+    return Y
+
+
+def mlp_loss_grad(params, norm, x, t, width=64):
+    """(loss, dloss/dparams) of the warm-start network in FP64."""
+    L = oracle_lib()
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    nr = np.ascontiguousarray(norm, dtype=np.float64)
+    X = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 15)
+    T = np.ascontiguousarray(t, dtype=np.float64).reshape(-1, 2)
+    G = np.zeros(L.sfo_mlp_num_params(width))
+    loss = L.sfo_mlp_loss_grad(_ptr(P), _ptr(nr), width, X.shape[0], _ptr(X), _ptr(T), _ptr(G))
+    return loss, G
 
 
 def collide_cell(model, tau, f9, rho, ux, uy, Fx, Fy, se=1.4, seps=1.4, sq=1.4):

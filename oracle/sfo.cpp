@@ -202,6 +202,11 @@ struct sfo_sim {
     int nsp = 0;
     std::vector<std::vector<double>> g, gtmp, C;
     std::vector<double> tau_s, c_inlet;
+    // MDF control of the device engine (no reference counterpart): residual
+    // mode stops once max_k |V_k - U(X_k)| <= tol; statistics per step
+    double mdf_tol = 0.0;
+    int mdf_max = 0, mdf_probe = 0;
+    double st_applied = 0.0, st_final = 0.0, st_max = 0.0, st_steps = 0.0;
 };
 
 namespace {
@@ -772,13 +777,23 @@ void macroscopic(sfo_sim& s) {
     }
 }
 
-// multiDirectForcing: :231-286
-void mdf(sfo_sim& s, int iterations) {
+// multiDirectForcing: :231-286, with ncorr = iterations - 1 corrections.
+// Residual mode (tol > 0, no reference counterpart): before each correction
+// the no-slip residual r = max_k |V_k - U(X_k)| is taken from the same
+// interpolation, and the loop stops once r <= tol.
+void mdf(sfo_sim& s, int ncorr) {
     spread_all(s);
-    if (iterations <= 1) return;
+    if (ncorr <= 0) return;
     Caps& c = s.caps;
-    for (int it = 1; it < iterations; ++it) {
+    const double tol = s.mdf_tol;
+    const int K = static_cast<int>(c.x.size());
+    std::vector<double> dFx(K), dFy(K);
+    bool conv = false;
+    double r_final = 0.0;
+    int applied = 0;
+    auto residual_pass = [&](bool keep) {
         macroscopic(s);
+        double r = 0.0;
         for (int cap = 0; cap < c.n(); ++cap) {
             int o = c.off[cap], n = c.off[cap + 1] - o;
             for (int k = 0; k < n; ++k) {
@@ -792,11 +807,34 @@ void mdf(sfo_sim& s, int iterations) {
                 if (ix >= s.nx) ix -= s.nx;
                 if (iy >= 0 && iy < s.ny) rl = s.rho[iy * s.nx + ix];
                 double two_rho = 2.0 * rl;
-                V2 dF{du.x * two_rho, du.y * two_rho};
-                spread_node(s, px, py, dF.x, dF.y, node_ds(s, cap, k));
+                if (keep) { dFx[o + k] = du.x * two_rho; dFy[o + k] = du.y * two_rho; }
+                double rk = std::sqrt(du.x * du.x + du.y * du.y);
+                r = (rk > r || rk != rk) ? rk : r;
             }
         }
+        return r;
+    };
+    for (int it = 1; it <= ncorr; ++it) {
+        double r = residual_pass(true);
+        r_final = r;
+        if (tol > 0.0 && r <= tol) { conv = true; break; }
+        ++applied;
+        for (int cap = 0; cap < c.n(); ++cap) {
+            int o = c.off[cap], n = c.off[cap + 1] - o;
+            for (int k = 0; k < n; ++k)
+                spread_node(s, c.x[o + k], c.y[o + k], dFx[o + k], dFy[o + k], node_ds(s, cap, k));
+        }
     }
+    if (!conv && s.mdf_probe) {
+        // diagnostic only: the fields the collision reads stay as they were
+        std::vector<double> r0 = s.rho, u0 = s.ux, v0 = s.uy;
+        r_final = residual_pass(false);
+        s.rho.swap(r0); s.ux.swap(u0); s.uy.swap(v0);
+    }
+    s.st_applied += applied;
+    s.st_final += r_final;
+    s.st_max = (r_final > s.st_max || r_final != r_final) ? r_final : s.st_max;
+    s.st_steps += 1.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1089,7 +1127,8 @@ void sfo_step(sfo_sim* s, int nsteps) {
                 for (int i = 0; i < n; ++i) { c.fx[o + i] += gx; c.fy[o + i] += gy; }
             }
         }
-        if (p.ibm_iterations > 1) mdf(*s, p.ibm_iterations);        // :531-538
+        const int ncorr = s->mdf_tol > 0.0 ? s->mdf_max : p.ibm_iterations - 1;
+        if (ncorr > 0) mdf(*s, ncorr);                              // :531-538
         else spread_all(*s);
         const bool hb = (p.body_force_x != 0.0 || p.body_force_y != 0.0);   // :541-572
         const bool hg = p.apply_gravity_to_fluid && (p.gravity_x != 0.0 || p.gravity_y != 0.0);
@@ -1252,6 +1291,124 @@ void sfo_membrane_forces_only(sfo_sim* s) {
     std::fill(c.fx.begin(), c.fx.end(), 0.0);
     std::fill(c.fy.begin(), c.fy.end(), 0.0);
     for (int cap = 0; cap < c.n(); ++cap) membrane_forces(c, cap);
+}
+
+
+void sfo_set_mdf_control(sfo_sim* s, double tol, int max_corrections, int probe) {
+    s->mdf_tol = tol > 0.0 ? tol : 0.0;
+    s->mdf_max = tol > 0.0 ? max_corrections : 0;
+    s->mdf_probe = probe ? 1 : 0;
+}
+
+void sfo_mdf_stats(sfo_sim* s, double* out4) {
+    const double n = s->st_steps;
+    out4[0] = n > 0 ? s->st_applied / n : 0.0;
+    out4[1] = n > 0 ? s->st_final / n : 0.0;
+    out4[2] = s->st_max;
+    out4[3] = n;
+    s->st_applied = s->st_final = s->st_max = s->st_steps = 0.0;
+}
+
+// The warm-start network of the device engine (no reference counterpart),
+// restated in FP64: x^ = (x - in_mean) / in_scale; a1 = GELU(W1 x^ + b1);
+// a2 = GELU(W2 a1 + b2); out = W3 a2 + b3; y = out_mean + out_scale * out.
+// GELU(z) = z Phi(z) (exact, with erf).  Parameter block W1[w][15], b1[w],
+// W2[w][w], b2[w], W3[2][w], b3[2]; norm = in_mean[15], in_scale[15],
+// out_mean[2], out_scale[2].
+namespace {
+constexpr int NI = 15, NO = 2;
+inline double gelu_(double z) { return 0.5 * z * (1.0 + std::erf(z / std::sqrt(2.0))); }
+inline double gelu_d_(double z) {
+    return 0.5 * (1.0 + std::erf(z / std::sqrt(2.0))) + z * std::exp(-0.5 * z * z) / std::sqrt(2.0 * M_PI);
+}
+struct MlpView {
+    int w;
+    const double *W1, *b1, *W2, *b2, *W3, *b3;
+    explicit MlpView(const double* P, int width) : w(width) {
+        W1 = P; b1 = W1 + w * NI; W2 = b1 + w; b2 = W2 + w * w; W3 = b2 + w; b3 = W3 + NO * w;
+    }
+};
+}  // namespace
+
+int sfo_mlp_num_params(int width) { return width * NI + width + width * width + width + NO * width + NO; }
+
+void sfo_mlp_predict(const double* P, const double* norm, int width, int n, const double* x, double* y) {
+    MlpView v(P, width);
+    std::vector<double> a1(width), a2(width);
+    for (int s = 0; s < n; ++s) {
+        double xh[NI];
+        for (int j = 0; j < NI; ++j) xh[j] = (x[s * NI + j] - norm[j]) / norm[NI + j];
+        for (int i = 0; i < width; ++i) {
+            double z = v.b1[i];
+            for (int j = 0; j < NI; ++j) z += v.W1[i * NI + j] * xh[j];
+            a1[i] = gelu_(z);
+        }
+        for (int i = 0; i < width; ++i) {
+            double z = v.b2[i];
+            for (int j = 0; j < width; ++j) z += v.W2[i * width + j] * a1[j];
+            a2[i] = gelu_(z);
+        }
+        for (int o = 0; o < NO; ++o) {
+            double z = v.b3[o];
+            for (int j = 0; j < width; ++j) z += v.W3[o * width + j] * a2[j];
+            y[s * NO + o] = norm[2 * NI + o] + norm[2 * NI + NO + o] * z;
+        }
+    }
+}
+
+// L = 1/(2n) sum_s |out_s - t^_s|^2 with t^ = (t - out_mean) / out_scale, and dL/dP.
+double sfo_mlp_loss_grad(const double* P, const double* norm, int width, int n, const double* x,
+                         const double* t, double* grad) {
+    MlpView v(P, width);
+    const int np = sfo_mlp_num_params(width);
+    std::fill(grad, grad + np, 0.0);
+    double* gW1 = grad; double* gb1 = gW1 + width * NI; double* gW2 = gb1 + width;
+    double* gb2 = gW2 + width * width; double* gW3 = gb2 + width; double* gb3 = gW3 + NO * width;
+    std::vector<double> z1(width), a1(width), z2(width), a2(width), d1(width), d2(width);
+    double L = 0.0;
+    for (int s = 0; s < n; ++s) {
+        double xh[NI], e[NO];
+        for (int j = 0; j < NI; ++j) xh[j] = (x[s * NI + j] - norm[j]) / norm[NI + j];
+        for (int i = 0; i < width; ++i) {
+            double z = v.b1[i];
+            for (int j = 0; j < NI; ++j) z += v.W1[i * NI + j] * xh[j];
+            z1[i] = z; a1[i] = gelu_(z);
+        }
+        for (int i = 0; i < width; ++i) {
+            double z = v.b2[i];
+            for (int j = 0; j < width; ++j) z += v.W2[i * width + j] * a1[j];
+            z2[i] = z; a2[i] = gelu_(z);
+        }
+        for (int o = 0; o < NO; ++o) {
+            double z = v.b3[o];
+            for (int j = 0; j < width; ++j) z += v.W3[o * width + j] * a2[j];
+            const double th = (t[s * NO + o] - norm[2 * NI + o]) / norm[2 * NI + NO + o];
+            const double d = z - th;
+            L += d * d;
+            e[o] = d / n;
+        }
+        for (int i = 0; i < width; ++i) {
+            double acc = 0.0;
+            for (int o = 0; o < NO; ++o) acc += v.W3[o * width + i] * e[o];
+            d2[i] = acc * gelu_d_(z2[i]);
+        }
+        for (int j = 0; j < width; ++j) {
+            double acc = 0.0;
+            for (int i = 0; i < width; ++i) acc += v.W2[i * width + j] * d2[i];
+            d1[j] = acc * gelu_d_(z1[j]);
+        }
+        for (int o = 0; o < NO; ++o) {
+            for (int j = 0; j < width; ++j) gW3[o * width + j] += e[o] * a2[j];
+            gb3[o] += e[o];
+        }
+        for (int i = 0; i < width; ++i) {
+            for (int j = 0; j < width; ++j) gW2[i * width + j] += d2[i] * a1[j];
+            gb2[i] += d2[i];
+            for (int j = 0; j < NI; ++j) gW1[i * NI + j] += d1[i] * xh[j];
+            gb1[i] += d1[i];
+        }
+    }
+    return 0.5 * L / n;
 }
 
 }  // extern "C"

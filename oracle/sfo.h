@@ -106,6 +106,20 @@ void   sfo_membrane_forces_only(sfo_sim* s);
 void   sfo_pair_force(const sfo_sim* s, const double* pa, const double* va,
                       const double* pb, const double* vb, double* out);
 
+/* MDF control of the device engine (no reference counterpart): tol > 0 stops
+ * the corrections once max_k |V_k - U(X_k)| <= tol (at most max_corrections);
+ * stats out4: mean corrections, mean and max final residual, steps. */
+void   sfo_set_mdf_control(sfo_sim* s, double tol, int max_corrections, int probe);
+void   sfo_mdf_stats(sfo_sim* s, double* out4);
+
+/* The device engine's warm-start network restated in FP64 (no reference
+ * counterpart): 15 -> width -> width -> 2, GELU. */
+int    sfo_mlp_num_params(int width);
+void   sfo_mlp_predict(const double* P, const double* norm, int width, int n,
+                       const double* x, double* y);
+double sfo_mlp_loss_grad(const double* P, const double* norm, int width, int n,
+                         const double* x, const double* t, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

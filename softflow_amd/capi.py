@@ -28,6 +28,9 @@ EXPORTS = (
     "sf_set_scalar", "sf_get_scalar", "sf_step", "sf_set_graph_mode", "sf_synchronize",
     "sf_stage", "sf_last_capped", "sf_field_stats", "sf_step_count", "sf_kernel_times",
     "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_probe_math",
+    "sf_mdf_set_control", "sf_mdf_stats", "sf_mdf_collect", "sf_mdf_samples", "sf_mdf_set_model",
+    "sf_mlp_create", "sf_mlp_destroy", "sf_mlp_num_params", "sf_mlp_get_params", "sf_mlp_set_params",
+    "sf_mlp_predict", "sf_mlp_loss_grad", "sf_mlp_train",
 )
 
 
@@ -60,6 +63,7 @@ class CapsuleDesc(C.Structure):
 
 
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
 _IP = C.POINTER(C.c_int32)
 _lib = None
 
@@ -101,6 +105,19 @@ def load():
         "sf_autotune": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],
         "sf_set_lbm_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int],
         "sf_probe_math": [C.c_int, C.c_int, _DP, _DP, _DP],
+        "sf_mdf_set_control": [C.c_void_p, C.c_double, C.c_int, C.c_int],
+        "sf_mdf_stats": [C.c_void_p, _DP],
+        "sf_mdf_collect": [C.c_void_p, C.c_int64],
+        "sf_mdf_samples": [C.c_void_p, C.POINTER(C.c_int64), _FP],
+        "sf_mdf_set_model": [C.c_void_p, C.c_void_p, C.c_int],
+        "sf_mlp_create": [C.POINTER(C.c_void_p), C.c_int, C.c_uint64],
+        "sf_mlp_destroy": [C.c_void_p],
+        "sf_mlp_num_params": [C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "sf_mlp_get_params": [C.c_void_p, _FP, _FP],
+        "sf_mlp_set_params": [C.c_void_p, _FP, _FP],
+        "sf_mlp_predict": [C.c_void_p, C.c_int, _FP, _FP],
+        "sf_mlp_loss_grad": [C.c_void_p, C.c_int, _FP, _FP, _DP, _FP],
+        "sf_mlp_train": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint64, _DP],
     }
     for name, args in proto.items():
         fn = getattr(L, name)

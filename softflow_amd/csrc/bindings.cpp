@@ -345,7 +345,54 @@ PYBIND11_MODULE(softflow_core, m) {
         .def("setTiming", &Simulation::setTiming, py::arg("on"))
         .def("autotune", &Simulation::autotune, py::arg("reps") = 10)
         .def("setLbmLaunch", &Simulation::setLbmLaunch, py::arg("bx"), py::arg("by"), py::arg("xcd_banded") = true)
-        .def("kernelTimes", &Simulation::kernelTimes);
+        .def("kernelTimes", &Simulation::kernelTimes)
+        .def("setMdfControl", &Simulation::setMdfControl, py::arg("tolerance"),
+             py::arg("max_corrections") = 8, py::arg("probe") = false)
+        .def("mdfStats", [](Simulation& s) {
+            auto v = s.mdfStats();
+            py::dict d;
+            d["mean_corrections"] = v[0];
+            d["mean_residual"] = v[1];
+            d["max_residual"] = v[2];
+            d["steps"] = v[3];
+            return d;
+        })
+        .def("collectMdfSamples", &Simulation::collectMdfSamples, py::arg("capacity"))
+        .def("mdfSampleCount", &Simulation::mdfSampleCount)
+        .def("mdfSamples", [](Simulation& s) {
+            auto v = s.mdfSamples();
+            const py::ssize_t w = SF_MLP_IN + SF_MLP_OUT;
+            py::array_t<float> a({static_cast<py::ssize_t>(v.size()) / w, w});
+            std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(float));
+            return a;
+        })
+        .def("setMdfModel", &Simulation::setMdfModel, py::arg("model"), py::arg("warm") = true,
+             py::keep_alive<1, 2>());
+
+    py::class_<MdfModel>(m, "MdfModel")
+        .def(py::init<int, uint64_t>(), py::arg("device") = 0, py::arg("seed") = 1)
+        .def("train", &MdfModel::train, py::arg("sim"), py::arg("epochs") = 10, py::arg("batch") = 4096,
+             py::arg("lr") = 1e-3, py::arg("seed") = 1, py::call_guard<py::gil_scoped_release>())
+        .def("params", [](const MdfModel& m) { auto v = m.params(); return py::array_t<float>(v.size(), v.data()); })
+        .def("normalization", [](const MdfModel& m) { auto v = m.normalization(); return py::array_t<float>(v.size(), v.data()); })
+        .def("setParams", [](MdfModel& m, py::array_t<float, py::array::c_style | py::array::forcecast> p) {
+            m.setParams(std::vector<float>(p.data(), p.data() + p.size()));
+        })
+        .def("setNormalization", [](MdfModel& m, py::array_t<float, py::array::c_style | py::array::forcecast> n) {
+            m.setNormalization(std::vector<float>(n.data(), n.data() + n.size()));
+        })
+        .def("predict", [](const MdfModel& m, py::array_t<float, py::array::c_style | py::array::forcecast> x) {
+            auto y = m.predict(std::vector<float>(x.data(), x.data() + x.size()));
+            py::array_t<float> a({static_cast<py::ssize_t>(y.size() / SF_MLP_OUT), static_cast<py::ssize_t>(SF_MLP_OUT)});
+            std::memcpy(a.mutable_data(), y.data(), y.size() * sizeof(float));
+            return a;
+        })
+        .def("lossGrad", [](const MdfModel& m, py::array_t<float, py::array::c_style | py::array::forcecast> x,
+                            py::array_t<float, py::array::c_style | py::array::forcecast> t) {
+            auto r = m.lossGrad(std::vector<float>(x.data(), x.data() + x.size()),
+                                std::vector<float>(t.data(), t.data() + t.size()));
+            return py::make_tuple(r.first, py::array_t<float>(r.second.size(), r.second.data()));
+        });
 
     m.def("device_count", []() {
         int n = 0;

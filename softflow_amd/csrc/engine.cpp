@@ -478,6 +478,9 @@ void Simulation::initialize() {
     caps_.dirty_ = true;
     step_count_ = 0;
     pushDirty();
+    check(sf_mdf_set_control(ctx_, mdf_tol_, mdf_max_, mdf_probe_ ? 1 : 0), "sf_mdf_set_control");
+    if (mdf_collect_ > 0) check(sf_mdf_collect(ctx_, mdf_collect_), "sf_mdf_collect");
+    if (mdf_model_) check(sf_mdf_set_model(ctx_, mdf_model_->handle(), mdf_warm_ ? 1 : 0), "sf_mdf_set_model");
     std::cout << "Simulation initialized: " << nx << "x" << ny << " lattice, " << caps_.caps_.size()
               << " capsules (" << caps_.totalNodes() << " membrane nodes)" << std::endl;
 }
@@ -677,6 +680,111 @@ std::string Simulation::autotune(int reps) {
 void Simulation::setLbmLaunch(int bx, int by, bool xcd_banded) {
     if (!ctx_) throw std::runtime_error("Simulation: initialize() has not been called");
     check(sf_set_lbm_launch(ctx_, bx, by, xcd_banded ? 1 : 0), "sf_set_lbm_launch");
+}
+
+void Simulation::setMdfControl(Real tol, int max_corrections, bool probe) {
+    mdf_tol_ = tol;
+    mdf_max_ = max_corrections;
+    mdf_probe_ = probe;
+    if (ctx_) check(sf_mdf_set_control(ctx_, tol, max_corrections, probe ? 1 : 0), "sf_mdf_set_control");
+}
+
+std::vector<double> Simulation::mdfStats() {
+    std::vector<double> out(4, 0.0);
+    if (ctx_) check(sf_mdf_stats(ctx_, out.data()), "sf_mdf_stats");
+    return out;
+}
+
+void Simulation::collectMdfSamples(int64_t capacity) {
+    mdf_collect_ = capacity;
+    if (ctx_) check(sf_mdf_collect(ctx_, capacity), "sf_mdf_collect");
+}
+
+int64_t Simulation::mdfSampleCount() {
+    int64_t n = 0;
+    if (ctx_) check(sf_mdf_samples(ctx_, &n, nullptr), "sf_mdf_samples");
+    return n;
+}
+
+std::vector<float> Simulation::mdfSamples() {
+    int64_t n = mdfSampleCount();
+    std::vector<float> out(static_cast<size_t>(n) * (SF_MLP_IN + SF_MLP_OUT));
+    if (n > 0) check(sf_mdf_samples(ctx_, &n, out.data()), "sf_mdf_samples");
+    return out;
+}
+
+void Simulation::setMdfModel(MdfModel* model, bool warm) {
+    if (warm && !model) throw std::invalid_argument("Simulation: warm start needs a model");
+    mdf_model_ = model;
+    mdf_warm_ = warm;
+    if (ctx_) check(sf_mdf_set_model(ctx_, model ? model->handle() : nullptr, warm ? 1 : 0), "sf_mdf_set_model");
+}
+
+// ---------------------------------------------------------------------------
+static void mlp_check(int rc, const char* what) {
+    if (rc != SF_OK) throw std::runtime_error(std::string(what) + ": " + sf_last_error());
+}
+
+MdfModel::MdfModel(int device, uint64_t seed) { mlp_check(sf_mlp_create(&net_, device, seed), "sf_mlp_create"); }
+MdfModel::~MdfModel() { if (net_) sf_mlp_destroy(net_); }
+
+std::vector<double> MdfModel::train(Simulation& sim, int epochs, int batch, double lr, uint64_t seed) {
+    if (!sim.deviceContext()) throw std::runtime_error("MdfModel: the simulation is not initialized");
+    std::vector<double> loss(std::max(epochs, 0));
+    mlp_check(sf_mlp_train(net_, sim.deviceContext(), epochs, batch, static_cast<float>(lr), seed,
+                           loss.data()), "sf_mlp_train");
+    return loss;
+}
+
+std::vector<float> MdfModel::params() const {
+    int np = 0, nn = 0;
+    sf_mlp_num_params(&np, &nn);
+    std::vector<float> p(np);
+    mlp_check(sf_mlp_get_params(net_, p.data(), nullptr), "sf_mlp_get_params");
+    return p;
+}
+
+std::vector<float> MdfModel::normalization() const {
+    int np = 0, nn = 0;
+    sf_mlp_num_params(&np, &nn);
+    std::vector<float> n(nn);
+    mlp_check(sf_mlp_get_params(net_, nullptr, n.data()), "sf_mlp_get_params");
+    return n;
+}
+
+void MdfModel::setParams(const std::vector<float>& p) {
+    int np = 0, nn = 0;
+    sf_mlp_num_params(&np, &nn);
+    if (static_cast<int>(p.size()) != np) throw std::invalid_argument("MdfModel: wrong parameter count");
+    mlp_check(sf_mlp_set_params(net_, p.data(), nullptr), "sf_mlp_set_params");
+}
+
+void MdfModel::setNormalization(const std::vector<float>& n) {
+    int np = 0, nn = 0;
+    sf_mlp_num_params(&np, &nn);
+    if (static_cast<int>(n.size()) != nn) throw std::invalid_argument("MdfModel: wrong normalisation size");
+    mlp_check(sf_mlp_set_params(net_, nullptr, n.data()), "sf_mlp_set_params");
+}
+
+std::vector<float> MdfModel::predict(const std::vector<float>& x) const {
+    if (x.size() % SF_MLP_IN) throw std::invalid_argument("MdfModel: inputs must be n x 15");
+    const int n = static_cast<int>(x.size() / SF_MLP_IN);
+    std::vector<float> y(static_cast<size_t>(n) * SF_MLP_OUT);
+    mlp_check(sf_mlp_predict(net_, n, x.data(), y.data()), "sf_mlp_predict");
+    return y;
+}
+
+std::pair<double, std::vector<float>> MdfModel::lossGrad(const std::vector<float>& x,
+                                                         const std::vector<float>& t) const {
+    if (x.size() % SF_MLP_IN || t.size() != x.size() / SF_MLP_IN * SF_MLP_OUT)
+        throw std::invalid_argument("MdfModel: inputs must be n x 15 and targets n x 2");
+    const int n = static_cast<int>(x.size() / SF_MLP_IN);
+    int np = 0, nn = 0;
+    sf_mlp_num_params(&np, &nn);
+    std::vector<float> g(np);
+    double loss = 0.0;
+    mlp_check(sf_mlp_loss_grad(net_, n, x.data(), t.data(), &loss, g.data()), "sf_mlp_loss_grad");
+    return {loss, g};
 }
 
 void Simulation::setTiming(bool on) { if (ctx_) check(sf_set_timing(ctx_, on ? 1 : 0), "sf_set_timing"); }

@@ -272,6 +272,35 @@ private:
     BoundaryType bt_ = BoundaryType::INLET_OUTLET;
 };
 
+class Simulation;
+
+// The learned warm start of multi-direct forcing (no counterpart in the
+// reference): a 15 -> 64 -> 64 -> 2 GELU network on the device, trained on
+// the (inputs, converged correction) rows a Simulation collects.
+class MdfModel {
+public:
+    explicit MdfModel(int device = 0, uint64_t seed = 1);
+    ~MdfModel();
+    MdfModel(const MdfModel&) = delete;
+    MdfModel& operator=(const MdfModel&) = delete;
+
+    // Adam over the simulation's sample rows; returns the mean loss per epoch.
+    std::vector<double> train(Simulation& sim, int epochs, int batch, double lr, uint64_t seed);
+    std::vector<float> params() const;
+    std::vector<float> normalization() const;
+    void setParams(const std::vector<float>& p);
+    void setNormalization(const std::vector<float>& n);
+    // y[n][2] for x[n][15]
+    std::vector<float> predict(const std::vector<float>& x) const;
+    // (loss, gradient) on host samples x[n][15], t[n][2]
+    std::pair<double, std::vector<float>> lossGrad(const std::vector<float>& x,
+                                                   const std::vector<float>& t) const;
+    sf_mlp* handle() const { return net_; }
+
+private:
+    sf_mlp* net_ = nullptr;
+};
+
 // Simulation (engine/simulation.h:37-246): construct, add capsules,
 // initialize, step/run, accessors.
 class Simulation {
@@ -309,6 +338,15 @@ public:
     std::string autotune(int reps = 10);
     void setLbmLaunch(int bx, int by, bool xcd_banded);
 
+    // multi-direct forcing control and the learned warm start (extensions)
+    void setMdfControl(Real tol, int max_corrections, bool probe);
+    std::vector<double> mdfStats();
+    void collectMdfSamples(int64_t capacity);
+    int64_t mdfSampleCount();
+    std::vector<float> mdfSamples();
+    void setMdfModel(MdfModel* model, bool warm);
+    sf_ctx* deviceContext() const { return ctx_; }
+
     // device <-> host synchronisation (used by the mirrors)
     void pullField(LatticeField& f);
     void pullNodes(CapsuleSystem& cs);
@@ -325,6 +363,13 @@ private:
     std::function<void(Simulation&, int)> cb_;
     sf_ctx* ctx_ = nullptr;
     int64_t step_count_ = 0;
+    // MDF settings kept for (re)initialisation
+    Real mdf_tol_ = 0.0;
+    int mdf_max_ = 0;
+    bool mdf_probe_ = false;
+    int64_t mdf_collect_ = 0;
+    MdfModel* mdf_model_ = nullptr;
+    bool mdf_warm_ = false;
 };
 
 }  // namespace softflow

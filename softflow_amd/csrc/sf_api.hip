@@ -10,25 +10,14 @@
 #include "sf_internal.h"
 #include "sf_math.h"
 
-struct sf_ctx {
-    sf::Ctx c;
-};
-
 namespace sf {
 
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg) {
+int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
-
-#define SF_HIP(expr)                                                                   \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(SF_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 __global__ void k_fill_int(int* p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,16 +123,31 @@ static void step_forces(Ctx& c) {
     { Scope s(c, "repulsion"); launch_repulsion(c); }
 }
 
+// MDF corrections this step may apply: residual mode allows up to
+// mdf_max_corr, fixed mode applies the reference's ibm_iterations - 1.
+static int mdf_corrections(const Ctx& c) {
+    return c.mdf_tol > 0.0 ? c.mdf_max_corr : c.p.ibm_iterations - 1;
+}
+
 static int step_ibm(Ctx& c) {
     int shift;
-    const bool mdf = c.p.ibm_iterations > 1 && c.K > 0;
+    const int ncorr = mdf_corrections(c);
+    const bool mdf = ncorr > 0 && c.K > 0;
+    if (mdf) launch_mdf_begin(c);
     { Scope s(c, "spread"); launch_spread(c, 0, !mdf); }
-    if (c.p.ibm_iterations > 1) {
+    if (ncorr > 0) {
         if (c.K > 0) {
-            for (int it = 1; it < c.p.ibm_iterations; ++it) {
-                { Scope s(c, "mdf_interp"); launch_mdf_interp(c); }
-                { Scope s(c, "mdf_spread"); launch_spread(c, 1, it == c.p.ibm_iterations - 1); }
+            const bool learned = c.mdf_warm && c.model;
+            for (int it = 1; it <= ncorr; ++it) {
+                { Scope s(c, "mdf_interp"); launch_mdf_interp(c, it); }
+                if (it == 1 && c.mdf_hist_on) {
+                    { Scope s(c, "mdf_features"); launch_mdf_features(c); }
+                    if (learned) { Scope s(c, "mlp_infer"); launch_mlp_infer(c); }
+                }
+                { Scope s(c, "mdf_spread"); launch_spread(c, 1, it == ncorr, it); }
             }
+            if (c.mdf_probe) { Scope s(c, "mdf_probe"); launch_mdf_interp(c, ncorr + 1); }
+            launch_mdf_record(c, ncorr);
             shift = SHIFT_PARTIAL;
         } else {
             shift = SHIFT_ZERO;   // computeMacroscopic with F = 0
@@ -168,7 +172,7 @@ static void one_step(Ctx& c) {
     // The scalar step n-1 runs inside LBM step n when the collision velocity
     // of step n is the one that scalar step needs: macro(f_n, F_{n-1}), i.e.
     // the shift force is the previous step's collision force.
-    const bool fuse_ok = c.nsp > 0 && c.p.ibm_iterations <= 1 && c.shift_next == SHIFT_PREV;
+    const bool fuse_ok = c.nsp > 0 && mdf_corrections(c) <= 0 && c.shift_next == SHIFT_PREV;
     if (!fuse_ok) flush_adr(c);
     // rotate the raw-force buffers: this step's force goes into the spare one
     const int spare = 3 - c.Fcur - c.Fprev;
@@ -180,7 +184,7 @@ static void one_step(Ctx& c) {
     const bool fuse = fuse_ok && c.adr_pending;
     { Scope s(c, fuse ? "lbm+adr" : "lbm"); launch_lbm(c, shift, fuse); }
     if (c.nsp > 0) {
-        if (c.p.ibm_iterations <= 1) {
+        if (mdf_corrections(c) <= 0) {
             c.adr_pending = true;
         } else {
             Scope s(c, "adr");
@@ -200,7 +204,7 @@ static int run_graph(Ctx& c, int nsteps) {
     const int period = 6;
     while (nsteps > 0) {
         if (c.shift_next != SHIFT_PREV || nsteps < period ||
-            (c.nsp > 0 && (c.c_from_store || (c.p.ibm_iterations <= 1 && !c.adr_pending)))) {
+            (c.nsp > 0 && (c.c_from_store || (mdf_corrections(c) <= 0 && !c.adr_pending)))) {
             one_step(c);
             --nsteps;
             continue;
@@ -258,8 +262,10 @@ static int dmalloc(T** p, size_t bytes) {
 
 static void free_caps(Ctx& c) {
     void* ptrs[] = {c.off, c.owner, c.desc, c.has_prev, c.x, c.y, c.vx, c.vy, c.fx, c.fy,
-                    c.L0, c.th0, c.ds0, c.Lprev, c.spread};
+                    c.L0, c.th0, c.ds0, c.Lprev, c.spread, c.mdf_total, c.mdf_u0, c.mdf_hist,
+                    c.mdf_feat};
     for (void* p : ptrs) if (p) hipFree(p);
+    c.mdf_total = nullptr; c.mdf_u0 = nullptr; c.mdf_hist = nullptr; c.mdf_feat = nullptr;
     c.off = c.owner = c.has_prev = nullptr;
     c.desc = nullptr;
     c.x = c.y = c.vx = c.vy = c.fx = c.fy = c.L0 = c.th0 = c.ds0 = c.Lprev = nullptr;
@@ -291,7 +297,8 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     if (p->collision_model < 0 || p->collision_model > 2) return fail(SF_E_ARG, "unknown collision model");
     if (p->precision != SF_FP64 && p->precision != SF_FP32) return fail(SF_E_ARG, "precision must be SF_FP64 or SF_FP32");
     if (p->scalar_enabled && (p->n_species < 1 || p->n_species > 4)) return fail(SF_E_ARG, "1..4 scalar species supported");
-    if (p->ibm_iterations < 1) return fail(SF_E_ARG, "ibm_iterations must be >= 1");
+    if (p->ibm_iterations < 1 || p->ibm_iterations > MDF_MAX_CORR + 1)
+        return fail(SF_E_ARG, "ibm_iterations must be in 1..33");
     int ndev = 0;
     SF_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(SF_E_ARG, "no such HIP device");
@@ -321,6 +328,7 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     chk(dmalloc(&c.stored_u, 2 * N * 8));
     chk(dmalloc(&c.counters, 64 * sizeof(int)));
     chk(dmalloc(&c.phi, 1025 * 8));
+    chk(dmalloc(&c.mdf, sizeof(MdfDev)));
     if (rc != SF_OK) { sf_destroy(h); return rc; }
     SF_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host_counters), 64 * sizeof(int)));
     // flags: the plain channel of LBMSolver::initialize (lbm_solver.cpp:130-158)
@@ -332,6 +340,7 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     SF_HIP(hipMemcpy(c.flags, fl.data(), N, hipMemcpyHostToDevice));
     for (int b = 0; b < 3; ++b) SF_HIP(hipMemset(c.F[b], 0, 2 * N * es));
     SF_HIP(hipMemset(c.counters, 0, 64 * sizeof(int)));
+    SF_HIP(hipMemset(c.mdf, 0, sizeof(MdfDev)));
     c.Fcur = 0; c.Fprev = 1; c.Fpart = 2;
     double tab[1025];
     for (int i = 0; i <= 1024; ++i) tab[i] = phi_exact_host(2.0 * static_cast<double>(i) / 1024);
@@ -377,7 +386,7 @@ int sf_destroy(sf_ctx* h) {
     free_caps(c);
     void* ptrs[] = {c.flags, c.f[0], c.f[1], c.F[0], c.F[1], c.F[2], c.stored_rho, c.stored_u,
                     c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_dirty, c.cell_cnt, c.cell_list,
-                    c.cell_pay, c.cell_own};
+                    c.cell_pay, c.cell_own, c.mdf, c.ds_buf};
     for (void* p : ptrs) if (p) hipFree(p);
     for (int s = 0; s < 4; ++s) {
         for (int b = 0; b < 2; ++b) if (c.g[s][b]) hipFree(c.g[s][b]);
@@ -532,6 +541,15 @@ int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_d
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.has_prev), ncaps * sizeof(int)));
     SF_HIP(hipMemcpy(c.has_prev, has_prev, ncaps * sizeof(int), hipMemcpyHostToDevice));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.spread), K * sizeof(NodeSpread)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_total), 2 * static_cast<size_t>(K) * sizeof(double)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_u0), 2 * static_cast<size_t>(K) * sizeof(double)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_hist), MLP_HIST * 2 * static_cast<size_t>(K) * sizeof(float)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_feat), MLP_IN * static_cast<size_t>(K) * sizeof(float)));
+    SF_HIP(hipMemset(c.mdf_total, 0, 2 * static_cast<size_t>(K) * sizeof(double)));
+    SF_HIP(hipMemset(c.mdf_u0, 0, 2 * static_cast<size_t>(K) * sizeof(double)));
+    SF_HIP(hipMemset(c.mdf_hist, 0, MLP_HIST * 2 * static_cast<size_t>(K) * sizeof(float)));
+    SF_HIP(hipMemset(c.mdf_feat, 0, MLP_IN * static_cast<size_t>(K) * sizeof(float)));
+    SF_HIP(hipMemset(c.mdf, 0, sizeof(MdfDev)));
     if (!c.tile_cnt) {
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_cnt), c.ntiles * sizeof(int)));
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_list), static_cast<size_t>(c.ntiles) * TILE_MAXL * sizeof(int)));
@@ -778,8 +796,8 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
     SF_HIP(hipSetDevice(c.device));
     if (reps < 1) reps = 10;
     // the steady-state launch: previous-step shift force, scalar riding along
-    const bool fuse = c.nsp > 0 && c.p.ibm_iterations <= 1;
-    const int shift = c.p.ibm_iterations > 1 ? SHIFT_PARTIAL : SHIFT_PREV;
+    const bool fuse = c.nsp > 0 && mdf_corrections(c) <= 0;
+    const int shift = mdf_corrections(c) > 0 ? SHIFT_PARTIAL : SHIFT_PREV;
     // every candidate must fit the kernel's __launch_bounds__(256)
     static const int shapes[][2] = {{64, 4}, {64, 2}, {64, 1}, {128, 2}, {128, 1},
                                     {256, 1}, {32, 8}, {32, 4}};
@@ -831,6 +849,86 @@ int sf_set_lbm_launch(sf_ctx* h, int bx, int by, int xcd_banded) {
         return fail(SF_E_ARG, "launch shape must hold 64..256 threads in whole wavefronts");
     Ctx& c = h->c;
     c.lbm_bx = bx; c.lbm_by = by; c.lbm_remap = xcd_banded ? 1 : 0;
+    invalidate_graph(c);
+    return SF_OK;
+}
+
+int sf_mdf_set_control(sf_ctx* h, double tol, int max_corrections, int probe) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    if (tol > 0.0 && (max_corrections < 1 || max_corrections > MDF_MAX_CORR))
+        return fail(SF_E_ARG, "max_corrections must be in 1..32");
+    if (!(tol == tol)) return fail(SF_E_ARG, "tolerance is NaN");
+    c.mdf_tol = tol > 0.0 ? tol : 0.0;
+    c.mdf_max_corr = tol > 0.0 ? max_corrections : 0;
+    c.mdf_probe = probe ? 1 : 0;
+    invalidate_graph(c);
+    return SF_OK;
+}
+
+int sf_mdf_stats(sf_ctx* h, double* out4) {
+    if (!h || !out4) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    MdfDev md{};
+    SF_HIP(hipMemcpy(&md, c.mdf, sizeof(MdfDev), hipMemcpyDeviceToHost));
+    const double n = md.steps;
+    out4[0] = n > 0 ? md.sum_applied / n : 0.0;
+    out4[1] = n > 0 ? md.sum_final / n : 0.0;
+    out4[2] = md.max_final;
+    out4[3] = n;
+    md.sum_applied = md.sum_final = md.max_final = md.steps = 0.0;
+    SF_HIP(hipMemcpy(c.mdf, &md, sizeof(MdfDev), hipMemcpyHostToDevice));
+    return SF_OK;
+}
+
+int sf_mdf_collect(sf_ctx* h, int64_t capacity) {
+    if (!h || capacity < 0) return fail(SF_E_ARG, "bad arguments");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    if (c.ds_buf) hipFree(c.ds_buf);
+    c.ds_buf = nullptr;
+    c.ds_cap = 0;
+    MdfDev md{};
+    SF_HIP(hipMemcpy(&md, c.mdf, sizeof(MdfDev), hipMemcpyDeviceToHost));
+    md.ds_rows = 0;
+    SF_HIP(hipMemcpy(c.mdf, &md, sizeof(MdfDev), hipMemcpyHostToDevice));
+    if (capacity > 0) {
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.ds_buf),
+                         static_cast<size_t>(capacity) * (MLP_IN + MLP_OUT) * sizeof(float)));
+        c.ds_cap = capacity;
+    }
+    c.mdf_hist_on = (c.mdf_warm || c.ds_buf) ? 1 : 0;
+    invalidate_graph(c);
+    return SF_OK;
+}
+
+int sf_mdf_samples(sf_ctx* h, int64_t* rows, float* rows_out) {
+    if (!h || !rows) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    MdfDev md{};
+    SF_HIP(hipMemcpy(&md, c.mdf, sizeof(MdfDev), hipMemcpyDeviceToHost));
+    *rows = c.ds_buf ? md.ds_rows : 0;
+    if (rows_out && c.ds_buf && md.ds_rows > 0)
+        SF_HIP(hipMemcpy(rows_out, c.ds_buf, static_cast<size_t>(md.ds_rows) * (MLP_IN + MLP_OUT) * sizeof(float),
+                         hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+int sf_mdf_set_model(sf_ctx* h, sf_mlp* net, int warm) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    if (net && net->m.device != c.device) return fail(SF_E_ARG, "network and context are on different devices");
+    if (warm && !net) return fail(SF_E_ARG, "warm start needs a network");
+    SF_HIP(hipSetDevice(c.device));
+    if (net) SF_HIP(hipStreamSynchronize(net->m.stream));
+    c.model = net ? &net->m : nullptr;
+    c.mdf_warm = warm != 0;
+    c.mdf_hist_on = (c.mdf_warm || c.ds_buf) ? 1 : 0;
     invalidate_graph(c);
     return SF_OK;
 }

@@ -8,6 +8,7 @@
 // cell's force is therefore the same sequential sum the reference forms
 // (coupling/immersed_boundary.cpp:44-80 at one thread), independent of the
 // launch and of how nodes were appended to the lists.
+#include <algorithm>
 #include <type_traits>
 
 #include "sf_internal.h"
@@ -37,12 +38,26 @@ __device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, 
 // ---------------------------------------------------------------------------
 // Node records + tile binning.  mode 0: the node's own force; mode 1: keep the
 // force already written by the MDF interpolation (only re-validate).
+// In mode 1 the MDF control decides first: once the residual of this
+// iteration is within tolerance (residual mode) nothing more is spread this
+// step; otherwise the correction is counted into the node's step total.
 __global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,
                             const int* off, const int* owner, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, int* counters,
-                            int mode) {
+                            int mode, MdfDev* md, double tol, int it, double* total) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
+    if (mode == 1 && md) {
+        if (md->conv) return;
+        if (tol > 0.0 && __longlong_as_double(static_cast<long long>(md->res[it])) <= tol) {
+            md->conv = 1;          // every thread reaches the same decision
+            md->conv_it = it;
+            return;
+        }
+        if (k == 0) md->applied += 1;
+        total[2 * k] += rec[k].fx;
+        total[2 * k + 1] += rec[k].fy;
+    }
     const double px = x[k], py = y[k];
     NodeSpread s;
     s.px = px;
@@ -123,7 +138,8 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile
                                                 const int* tile_list, const double* lut,
                                                 const void* Fin, void* Fout,
                                                 const uint8_t* dirty_in, uint8_t* dirty_out,
-                                                int nx, int ny, int ntx, int mode, int last) {
+                                                int nx, int ny, int ntx, int mode, int last,
+                                                const MdfDev* md) {
     __shared__ double tab[1025];
     __shared__ double spx[STAGE_MAX], spy[STAGE_MAX], sfx[STAGE_MAX], sfy[STAGE_MAX], sds[STAGE_MAX];
     __shared__ int si0[STAGE_MAX], sj0[STAGE_MAX], skey[STAGE_MAX];
@@ -136,6 +152,11 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile
     const int t = threadIdx.x;
     int L = tile_cnt[tile];
     if (L > TILE_MAXL) L = TILE_MAXL;
+    if (mode == 1 && md && md->conv) {
+        // converged MDF: the output is a copy of the input (F_out = F_in + 0)
+        if (last && t == 0) tile_cnt[tile] = 0;
+        L = 0;
+    }
     const int tx0 = (tile % ntx) * TILE_X, ty0 = (tile / ntx) * TILE_Y;
     const int cx = tx0 + (t % TILE_X);
     const int cy = ty0 + (t / TILE_X);
@@ -411,33 +432,129 @@ __device__ __forceinline__ void interp_group(const CellVelArgs& a, const double*
 // MDF correction forces (immersed_boundary.cpp:245-284): for every node,
 // dF = 2 rho_l (V_prev - U(X)) with U from the populations and the raw force
 // spread so far; written into the node's spread record.
+// Also records the no-slip residual max_k |V_k - U(X_k)| of this iteration
+// (md->res[it], an order-independent max) and, when u0 is set, the slip
+// V - U per node.  After convergence (residual mode) it does nothing.
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double* lut, const double* x,
                                                     const double* y, const double* vx, const double* vy,
-                                                    NodeSpread* rec, int K) {
+                                                    NodeSpread* rec, int K, MdfDev* md, int it,
+                                                    double* u0) {
     __shared__ double tx[256], ty[256];
     __shared__ int use[256];
+    __shared__ double sres[INTERP_NODES];
+    if (md->conv) return;       // uniform: no kernel writes it while this one runs
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     const int k = blockIdx.x * INTERP_NODES + g;
     const bool live = k < K;
     const double px = live ? x[k] : 0.0, py = live ? y[k] : 0.0;
     double Ux, Uy;
     interp_group<FP32>(a, lut, px, py, 0, l, tx + 16 * g, ty + 16 * g, use + 16 * g, Ux, Uy);
-    if (!live || l != 0) return;
-    const double dux = vx[k] - Ux, duy = vy[k] - Uy;
-    double rl = 1.0;
-    int ix = static_cast<int>(px + 0.5);
-    const int iy = static_cast<int>(py + 0.5);
-    if (ix < 0) ix += a.nx;
-    if (ix >= a.nx) ix -= a.nx;
-    if (iy >= 0 && iy < a.ny) {
-        double r, u, v;
-        cell_velocity<FP32>(a, ix, iy, 0, r, u, v);
-        rl = r;
+    if (l == 0) {
+        double res = 0.0;
+        if (live) {
+            const double dux = vx[k] - Ux, duy = vy[k] - Uy;
+            double rl = 1.0;
+            int ix = static_cast<int>(px + 0.5);
+            const int iy = static_cast<int>(py + 0.5);
+            if (ix < 0) ix += a.nx;
+            if (ix >= a.nx) ix -= a.nx;
+            if (iy >= 0 && iy < a.ny) {
+                double r, u, v;
+                cell_velocity<FP32>(a, ix, iy, 0, r, u, v);
+                rl = r;
+            }
+            const double two = 2.0 * rl;
+            rec[k].fx = dux * two;
+            rec[k].fy = duy * two;
+            if (u0) { u0[2 * k] = dux; u0[2 * k + 1] = duy; }
+            res = sqrt(dux * dux + duy * duy);
+        }
+        sres[g] = res;
     }
-    const double two = 2.0 * rl;
-    rec[k].fx = dux * two;
-    rec[k].fy = duy * two;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int i = 0; i < INTERP_NODES; ++i) m = (sres[i] > m || sres[i] != sres[i]) ? sres[i] : m;
+        // non-negative doubles order like their bit patterns (NaN above all)
+        atomicMax(&md->res[it], static_cast<unsigned long long>(__double_as_longlong(m)));
+    }
+}
+
+// Per-node inputs of the warm-start model, in FP32: slip V - U at the first
+// correction, the node's membrane + contact force, the signed discrete
+// curvature (turning angle / arc length) and the corrections of the last
+// MLP_HIST steps.
+__global__ void k_mdf_features(const double* x, const double* y, const double* fx, const double* fy,
+                               const int* off, const int* owner, const double* u0,
+                               const float* hist, float* feat, int K, double Lx) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int cap = owner[k];
+    const int o = off[cap], n = off[cap + 1] - o, i = k - o;
+    const int kp = o + (i - 1 + n) % n, kn = o + (i + 1) % n;
+    double ex0 = x[k] - x[kp], ey0 = y[k] - y[kp];
+    double ex1 = x[kn] - x[k], ey1 = y[kn] - y[k];
+    if (Lx > 0.0) {
+        ex0 -= Lx * rint(ex0 / Lx);
+        ex1 -= Lx * rint(ex1 / Lx);
+    }
+    const double th = atan2(ex0 * ey1 - ey0 * ex1, ex0 * ex1 + ey0 * ey1);
+    const double ds = 0.5 * (sqrt(ex0 * ex0 + ey0 * ey0) + sqrt(ex1 * ex1 + ey1 * ey1));
+    float* f = feat + static_cast<size_t>(k) * MLP_IN;
+    f[0] = static_cast<float>(u0[2 * k]);
+    f[1] = static_cast<float>(u0[2 * k + 1]);
+    f[2] = static_cast<float>(fx[k]);
+    f[3] = static_cast<float>(fy[k]);
+    f[4] = static_cast<float>(ds > 0.0 ? th / ds : 0.0);
+    for (int h = 0; h < MLP_HIST; ++h) {
+        f[5 + 2 * h] = hist[(static_cast<size_t>(h) * K + k) * 2];
+        f[6 + 2 * h] = hist[(static_cast<size_t>(h) * K + k) * 2 + 1];
+    }
+}
+
+// Clears the per-step MDF state (graph-safe: a kernel, not a memset).
+__global__ void k_mdf_begin(MdfDev* md, double* total, int K) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) { md->conv = 0; md->applied = 0; md->conv_it = 0; }
+    if (i < MDF_MAX_CORR + 2) md->res[i] = 0ull;
+    if (i < 2 * K) total[i] = 0.0;
+}
+
+// End of the MDF stage: shift the per-node history, append sample rows
+// (inputs, converged correction) and accumulate the step statistics.
+__global__ void k_mdf_record(MdfDev* md, const double* total, float* hist, const float* feat,
+                             float* ds, int64_t ds_cap, int K, int hist_on) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    if (ds) {
+        const int64_t row = md->ds_rows + k;
+        if (md->ds_rows + K <= ds_cap) {
+            float* r = ds + row * (MLP_IN + MLP_OUT);
+            const float* f = feat + static_cast<size_t>(k) * MLP_IN;
+            for (int j = 0; j < MLP_IN; ++j) r[j] = f[j];
+            r[MLP_IN] = static_cast<float>(total[2 * k]);
+            r[MLP_IN + 1] = static_cast<float>(total[2 * k + 1]);
+        }
+    }
+    if (hist_on) {
+        for (int h = MLP_HIST - 1; h > 0; --h) {
+            hist[(static_cast<size_t>(h) * K + k) * 2] = hist[(static_cast<size_t>(h - 1) * K + k) * 2];
+            hist[(static_cast<size_t>(h) * K + k) * 2 + 1] = hist[(static_cast<size_t>(h - 1) * K + k) * 2 + 1];
+        }
+        hist[2 * static_cast<size_t>(k)] = static_cast<float>(total[2 * k]);
+        hist[2 * static_cast<size_t>(k) + 1] = static_cast<float>(total[2 * k + 1]);
+    }
+}
+
+__global__ void k_mdf_finish(MdfDev* md, int ncorr, int probe, int K, int collect, int64_t ds_cap) {
+    const int it_final = md->conv ? md->conv_it : (probe ? ncorr + 1 : ncorr);
+    const double r = __longlong_as_double(static_cast<long long>(md->res[it_final]));
+    md->sum_applied += md->applied;
+    md->sum_final += r;
+    md->max_final = (r > md->max_final || r != r) ? r : md->max_final;
+    md->steps += 1.0;
+    if (collect && md->ds_rows + K <= ds_cap) md->ds_rows += K;
 }
 
 // interpolateVelocity (immersed_boundary.cpp:174-189) with the post-step
@@ -565,7 +682,7 @@ static CellVelArgs vel_args(Ctx& c, const void* f, const void* F) {
 // mode 0: bin nodes and spread node forces into F[Fcur] (overwrite).
 // mode 1: spread the MDF corrections on top of F[Fcur] into the spare buffer
 //         and make it current; the previous contents become F[Fpart].
-void launch_spread(Ctx& c, int mode, bool last) {
+void launch_spread(Ctx& c, int mode, bool last, int it) {
     if (c.K == 0) {
         if (mode == 0) launch_fill_zero(c, c.F[c.Fcur], (c.fp32 ? 8 : 16) * static_cast<size_t>(c.N));
         return;
@@ -573,11 +690,11 @@ void launch_spread(Ctx& c, int mode, bool last) {
     if (mode == 0) {
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                            c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                           c.tile_cnt, c.tile_list, c.counters, 0);
+                           c.tile_cnt, c.tile_list, c.counters, 0, nullptr, 0.0, 0, nullptr);
     } else {
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                            c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                           c.tile_cnt, c.tile_list, c.counters, 1);
+                           c.tile_cnt, c.tile_list, c.counters, 1, c.mdf, c.mdf_tol, it, c.mdf_total);
     }
     const void* Fin = c.F[c.Fcur];
     // mode 1 writes into the buffer that is neither current nor previous
@@ -588,27 +705,51 @@ void launch_spread(Ctx& c, int mode, bool last) {
     if (c.fp32)
         hipLaunchKernelGGL((k_spread<true>), dim3(c.ntiles), dim3(256), 0, c.stream, c.spread,
                            c.tile_cnt, c.tile_list, c.phi, Fin, Fout, din, dout, c.nx, c.ny, c.ntx,
-                           mode, last ? 1 : 0);
+                           mode, last ? 1 : 0, mode == 1 ? c.mdf : nullptr);
     else
         hipLaunchKernelGGL((k_spread<false>), dim3(c.ntiles), dim3(256), 0, c.stream, c.spread,
                            c.tile_cnt, c.tile_list, c.phi, Fin, Fout, din, dout, c.nx, c.ny, c.ntx,
-                           mode, last ? 1 : 0);
+                           mode, last ? 1 : 0, mode == 1 ? c.mdf : nullptr);
     if (mode == 1) {
         c.Fpart = c.Fcur;
         c.Fcur = out;
     }
 }
 
-void launch_mdf_interp(Ctx& c) {
+// MDF iteration it (1-based; MDF_MAX_CORR + 1 at most, the last being the
+// residual probe after the final correction).
+void launch_mdf_interp(Ctx& c, int it) {
     if (c.K == 0) return;
     CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
     const dim3 grid((c.K + INTERP_NODES - 1) / INTERP_NODES);
+    double* u0 = it == 1 && c.mdf_hist_on ? c.mdf_u0 : nullptr;
     if (c.fp32)
         hipLaunchKernelGGL((k_mdf_interp<true>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y,
-                           c.vx, c.vy, c.spread, c.K);
+                           c.vx, c.vy, c.spread, c.K, c.mdf, it, u0);
     else
         hipLaunchKernelGGL((k_mdf_interp<false>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y,
-                           c.vx, c.vy, c.spread, c.K);
+                           c.vx, c.vy, c.spread, c.K, c.mdf, it, u0);
+}
+
+void launch_mdf_begin(Ctx& c) {
+    const int n = std::max(2 * c.K, MDF_MAX_CORR + 2);
+    hipLaunchKernelGGL(k_mdf_begin, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.mdf, c.mdf_total, c.K);
+}
+
+void launch_mdf_features(Ctx& c) {
+    hipLaunchKernelGGL(k_mdf_features, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
+                       c.fx, c.fy, c.off, c.owner, c.mdf_u0, c.mdf_hist, c.mdf_feat, c.K,
+                       c.periodic_x ? static_cast<double>(c.nx) : 0.0);
+}
+
+void launch_mdf_record(Ctx& c, int ncorr) {
+    const bool collect = c.ds_buf != nullptr;
+    if (collect || c.mdf_hist_on)
+        hipLaunchKernelGGL(k_mdf_record, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.mdf,
+                           c.mdf_total, c.mdf_hist, c.mdf_feat, c.ds_buf, c.ds_cap, c.K,
+                           c.mdf_hist_on);
+    hipLaunchKernelGGL(k_mdf_finish, dim3(1), dim3(1), 0, c.stream, c.mdf, ncorr, c.mdf_probe, c.K,
+                       collect ? 1 : 0, c.ds_cap);
 }
 
 void launch_interp_move(Ctx& c) {

@@ -40,6 +40,48 @@ struct NodeSpread {
     int i0, j0, valid, pad;
 };
 
+// Multi-direct forcing control.  Fixed mode (tol <= 0) is the reference's
+// ibm_iterations - 1 corrections; residual mode stops as soon as the no-slip
+// residual max_k |V_k - U(X_k)| is at most tol, after at most max_corr
+// corrections.  The first correction may come from a learned model.
+constexpr int MDF_MAX_CORR = 32;
+constexpr int MLP_IN = 15;          // slip (2), node force (2), curvature (1), 5 x 2 history
+constexpr int MLP_OUT = 2;
+constexpr int MLP_HIST = 5;
+
+struct MdfDev {
+    int conv;                    // the residual reached tol this step
+    int applied;                 // corrections spread this step
+    int conv_it;                 // iteration at which it converged (0: not yet)
+    int pad;
+    unsigned long long res[MDF_MAX_CORR + 2];   // max residual (double bits) per iteration
+    double sum_applied;          // accumulated over steps since the last read
+    double sum_final, max_final;
+    double steps;
+    int64_t ds_rows;             // rows written to the sample set
+};
+
+constexpr int MLP_W = 64;           // hidden width (two hidden layers)
+constexpr int MLP_P = MLP_W * MLP_IN + MLP_W + MLP_W * MLP_W + MLP_W + MLP_OUT * MLP_W + MLP_OUT;
+constexpr int MLP_NORM = 2 * MLP_IN + 2 * MLP_OUT;   // in_mean, in_scale, out_mean, out_scale
+
+// The warm-start network: x -> GELU(W1 x^ + b1) -> GELU(W2 . + b2) -> W3 . + b3,
+// with x^ = (x - in_mean) / in_scale and y = out_mean + out_scale * out.
+// Parameters are one FP32 block: W1[W][IN], b1[W], W2[W][W], b2[W], W3[OUT][W], b3[OUT].
+struct Mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float* params = nullptr;             // MLP_P
+    float* norm = nullptr;               // MLP_NORM
+    float* adam_m = nullptr;             // MLP_P
+    float* adam_v = nullptr;             // MLP_P
+    int64_t adam_t = 0;
+    bool norm_set = false;
+    float* partial = nullptr;            // per-block gradient partials
+    int partial_blocks = 0;
+    double* acc = nullptr;               // device loss accumulator
+};
+
 struct Timer {
     const char* name;
     hipEvent_t a, b;
@@ -106,6 +148,21 @@ struct Ctx {
     int* cell_own = nullptr;             // capsule of each listed node
     int cell_cur = 0;
 
+    // multi-direct forcing control, history and the sample set
+    double mdf_tol = 0.0;
+    int mdf_max_corr = 0;
+    int mdf_probe = 0;
+    bool mdf_warm = false;
+    MdfDev* mdf = nullptr;               // device
+    double* mdf_total = nullptr;         // 2K: corrections spread this step, per node
+    double* mdf_u0 = nullptr;            // 2K: slip V - U at the first correction
+    float* mdf_hist = nullptr;           // MLP_HIST x 2K: totals of the previous steps
+    float* mdf_feat = nullptr;           // K x MLP_IN
+    int mdf_hist_on = 0;                 // history is being recorded
+    float* ds_buf = nullptr;             // sample rows (MLP_IN + MLP_OUT floats)
+    int64_t ds_cap = 0;
+    Mlp* model = nullptr;                // borrowed from the caller
+
     // counters: [0] capped cells, [1] tile overflow, [2] cell overflow
     int* counters = nullptr;
     int* host_counters = nullptr;        // pinned
@@ -128,6 +185,17 @@ struct Ctx {
     int lbm_bx = 64, lbm_by = 4, lbm_remap = 1;
 };
 
+// Error reporting shared by the C-ABI translation units: sets the message
+// returned by sf_last_error() and returns code.
+int fail(int code, const std::string& msg);
+
+#define SF_HIP(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::sf::fail(SF_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
 // -- launchers (defined in the .hip files) --------------------------------
 void launch_lbm(Ctx& c, int shift_mode, bool fuse_adr);
 // the same launch with an explicit shape and no change of buffer roles
@@ -137,11 +205,23 @@ void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, doubl
 void launch_field_stats(Ctx& c, double* dev_out);
 void launch_membrane(Ctx& c);
 void launch_repulsion(Ctx& c);
-void launch_spread(Ctx& c, int mode, bool last);   // 0 node forces into F[Fcur]; 1 MDF correction
-void launch_mdf_interp(Ctx& c);
+void launch_spread(Ctx& c, int mode, bool last, int it = 0);   // 0 node forces into F[Fcur]; 1 MDF correction
+void launch_mdf_interp(Ctx& c, int it);
+void launch_mdf_begin(Ctx& c);
+void launch_mdf_features(Ctx& c);
+void launch_mdf_record(Ctx& c, int ncorr);
+void launch_mlp_infer(Ctx& c);
 void launch_interp_move(Ctx& c);
 void launch_fill_int(Ctx& c, int* p, int n, int v);
 void launch_fill_zero(Ctx& c, void* p, size_t bytes);
 void probe_math(int op, int n, const double* a, const double* b, double* out);
 
 }  // namespace sf
+
+struct sf_ctx {
+    sf::Ctx c;
+};
+
+struct sf_mlp {
+    sf::Mlp m;
+};

@@ -1,0 +1,152 @@
+"""MDF control and the learned warm start on the HIP path.
+
+- residual-mode MDF (stop at max_k |V_k - U(X_k)| <= tol) in FP64 against
+  the oracle's restatement, at the parity bar, with equal statistics;
+- the warm-start network's forward pass and loss gradient (hand-written
+  HIP kernels, FP32) against the oracle's FP64 restatement;
+- training on rows the plain solver collected, deterministic for a seed;
+- what the trained network gains: fewer MDF corrections per step at the
+  same no-slip residual, with the field difference to the plain solver.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.cases import oracle_sim, oracle_state, product_sim, product_state, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SKALAK = dict(model=2, G_s=0.06, C_skalak=10.0)
+CASE = dict(nx=80, ny=40, tau=0.8, bfx=3e-5, coll=2, mdf=1,
+            capsules=[dict(count=5, region=(8, 8, 72, 32), rmin=3.0, rmax=3.5, nodes=20, seed=42,
+                           mem=SKALAK)])
+
+
+def test_residual_mode_matches_oracle_fp64():
+    print("residual-mode MDF parity", flush=True)
+    o = oracle_sim(CASE)
+    p = product_sim(CASE)
+    o.step(50)
+    p.run(50)
+    o.set_mdf_control(1e-300, 1, probe=True)
+    p.setMdfControl(1e-300, 1, True)
+    o.step(1)
+    p.run(1)
+    r1 = o.mdf_stats()[1]
+    p.mdfStats()
+    tol = 0.9 * r1                 # reached in some steps, not in others
+    o.set_mdf_control(tol, 6, probe=True)
+    p.setMdfControl(tol, 6, True)
+    o.step(300)
+    p.run(300)
+    so, sp = o.mdf_stats(), p.mdfStats()
+    print("oracle", so, "product", sp, flush=True)
+    assert sp["mean_corrections"] == so[0] and sp["steps"] == so[3]
+    assert 1.0 < so[0] < 6.0
+    assert abs(sp["mean_residual"] - so[1]) <= 1e-9 * so[1]
+    ref, got = oracle_state(o), product_state(p)
+    for k in ("rho", "ux", "uy", "x", "y", "vx", "vy"):
+        assert rel_l2(got[k], ref[k]) <= 1e-12, k
+
+
+def _random_net(seed):
+    rng = np.random.default_rng(seed)
+    P = rng.normal(scale=0.2, size=po.oracle_lib().sfo_mlp_num_params(64)).astype(np.float32)
+    nrm = np.r_[rng.normal(size=15), rng.uniform(0.5, 2, 15), rng.normal(size=2),
+                rng.uniform(0.5, 2, 2)].astype(np.float32)
+    return P, nrm
+
+
+def test_network_forward_and_gradient_match_oracle():
+    from softflow_amd import softflow_core as sc
+    P, nrm = _random_net(3)
+    m = sc.MdfModel(0, 1)
+    m.setParams(P)
+    m.setNormalization(nrm)
+    assert np.array_equal(m.params(), P) and np.array_equal(m.normalization(), nrm)
+    rng = np.random.default_rng(4)
+    for n in (1, 63, 64, 200):
+        x = rng.normal(size=(n, 15)).astype(np.float32)
+        t = rng.normal(size=(n, 2)).astype(np.float32)
+        y = m.predict(x)
+        yo = po.mlp_predict(P.astype(np.float64), nrm.astype(np.float64), x.astype(np.float64))
+        assert np.max(np.abs(y - yo)) <= 1e-5 * max(1.0, np.max(np.abs(yo)))
+        L, g = m.lossGrad(x, t)
+        Lo, go = po.mlp_loss_grad(P.astype(np.float64), nrm.astype(np.float64), x.astype(np.float64),
+                                  t.astype(np.float64))
+        print(f"n={n} loss {L:.6e} vs {Lo:.6e}; grad rel err {np.linalg.norm(g - go) / np.linalg.norm(go):.2e}",
+              flush=True)
+        assert abs(L - Lo) <= 1e-5 * Lo
+        assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+
+
+LEARN = dict(nx=160, ny=64, tau=0.8, bfx=3e-5, coll=2, mdf=1, precision="fp32",
+             capsules=[dict(count=20, region=(8, 8, 152, 56), rmin=4.0, rmax=4.0, nodes=25, seed=42,
+                            mem=SKALAK)])
+
+
+def _residual_after(sim, ncorr, steps=10):
+    sim.setMdfControl(1e-300, ncorr, True)
+    sim.run(steps)
+    return sim.mdfStats()["mean_residual"]
+
+
+def test_learned_warm_start_gains():
+    """Plain MDF converges slowly here (the iteration matrix of the Peskin
+    stencil on a ring with unit spacing has eigenvalues in (0, 0.38]), so the
+    network learns the total of four plain corrections -- the reference's
+    5-iteration MDF -- and replaces them with one."""
+    from softflow_amd import softflow_core as sc
+    print("learned warm start: collect", flush=True)
+    sim = product_sim(LEARN)
+    sim.setMdfControl(1e-300, 4, True)
+    sim.run(200)
+    sim.mdfStats()
+    sim.collectMdfSamples(300 * 500)
+    sim.run(300)
+    st = sim.mdfStats()
+    tol = 1.2 * st["max_residual"]             # what four plain corrections reach, with margin
+    rows = sim.mdfSamples()
+    assert rows.shape == (300 * 500, 17) and np.all(np.isfinite(rows))
+    model = sc.MdfModel(0, 7)
+    loss = model.train(sim, epochs=30, batch=1024, lr=2e-3, seed=11)
+    print("training loss per epoch", np.array(loss)[[0, 5, 10, 29]], flush=True)
+    assert loss[-1] < 0.1 * loss[0]
+    # deterministic: the same seed gives the same network
+    m2 = sc.MdfModel(0, 7)
+    m2.train(sim, epochs=2, batch=1024, lr=2e-3, seed=11)
+    m3 = sc.MdfModel(0, 7)
+    m3.train(sim, epochs=2, batch=1024, lr=2e-3, seed=11)
+    assert np.array_equal(m2.params(), m3.params())
+    sim.collectMdfSamples(0)
+    # the same state advanced three ways: four plain corrections per step (the
+    # reference's 5-iteration MDF), plain corrections until the residual is
+    # within tol, and the network's correction followed by plain ones as needed
+    runs = {k: product_sim(LEARN) for k in ("plain4", "plain", "learned")}
+    for s in runs.values():
+        s.setMdfControl(1e-300, 4, False)
+        s.run(500)
+        s.mdfStats()
+    runs["learned"].setMdfModel(model, True)
+    runs["plain4"].setMdfControl(1e-300, 4, True)
+    runs["plain"].setMdfControl(tol, 16, True)
+    runs["learned"].setMdfControl(tol, 16, True)
+    st, state = {}, {}
+    for k, s in runs.items():
+        s.run(100)
+        st[k] = s.mdfStats()
+        state[k] = product_state(s)
+
+    def du(a, b):
+        return rel_l2(np.r_[state[a]["ux"].ravel(), state[a]["uy"].ravel()],
+                      np.r_[state[b]["ux"].ravel(), state[b]["uy"].ravel()])
+    for k in runs:
+        print(f"{k}: {st[k]['mean_corrections']:.2f} corrections/step, residual mean "
+              f"{st[k]['mean_residual']:.2e} max {st[k]['max_residual']:.2e}", flush=True)
+    print(f"tol {tol:.3e}; velocity vs plain4: plain {du('plain', 'plain4'):.2e}, "
+          f"learned {du('learned', 'plain4'):.2e}", flush=True)
+    assert st["learned"]["max_residual"] <= tol and st["plain"]["max_residual"] <= tol
+    assert st["learned"]["mean_corrections"] < st["plain"]["mean_corrections"]
+    # for scale: one plain correction more or less per step moves this field
+    # by 8-12 % (relative L2) over the same 100 steps
+    assert du("learned", "plain4") < 0.05
