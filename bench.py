@@ -16,7 +16,12 @@ step MLUPS over the timed region (HIP-graph replay, inputs resident in HBM).
 duration comes from HIP events around each launch in a second timed region
 of the same length, run kernel by kernel.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+The line also carries the learned-vs-plain MDF comparison (the second half of
+the metric): steps/s of the plain 5-iteration MDF in FP64 against the FP32
+path whose first correction comes from the trained network, at the same
+no-slip residual, on a 2048 x 512 channel with 500 capsules.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mdf]
 """
 from __future__ import annotations
 
@@ -111,6 +116,113 @@ def cpu_baseline(case, steps):
     return case["nx"] * case["ny"] * steps / dt / 1e6, dt
 
 
+# Learned MDF warm start workload: 2048 x 512 channel, regularized tau = 0.8,
+# fx = 3e-5; 500 Skalak capsules (G_s = 0.06, C = 1) of radius 4 with 25 nodes
+# seeded (seed 5) over the whole channel (2.4 % area fraction).  Denser
+# seeding (30 % in a 164 x 504 band) and C = 10 both make the explicit
+# coupling of plain multi-direct forcing diverge within 120-200 steps, in the
+# FP64 restatement of the reference as on the device.
+C5 = dict(nx=2048, ny=512, tau=0.8, bfx=3e-5, count=500, radius=4.0, nodes=25, seed=5,
+          region=(4, 8, 2044, 504), mem={"C_skalak": 1.0})
+
+
+def build_c5(precision, ibm_iterations):
+    from softflow_amd import softflow_core as sc
+
+    p = sc.SimulationParams()
+    p.nx, p.ny = C5["nx"], C5["ny"]
+    p.fluid.tau = C5["tau"]
+    p.fluid.boundary_type = sc.BoundaryType.PERIODIC
+    p.fluid.body_force_x = C5["bfx"]
+    p.fluid.collision_model = sc.CollisionModel.REGULARIZED
+    p.ibm_iterations = ibm_iterations
+    p.enable_stability_checks = False
+    p.precision = sc.Precision.FP32 if precision == "fp32" else sc.Precision.FP64
+    sim = sc.Simulation(p)
+    b = sc.ChannelBuilder(p.nx, p.ny)
+    b.addWalls()
+    sim.setChannelBuilder(b)
+    mp = sc.MembraneParams()
+    mp.model = sc.MembraneModel.SKALAK
+    mp.G_s, mp.C_skalak = 0.06, 10.0
+    for k, v in C5.get("mem", {}).items():
+        setattr(mp, k, v)
+    x0, y0, x1, y1 = C5["region"]
+    placed = sim.addCapsuleRandom(C5["count"], x0, y0, x1, y1, C5["radius"], C5["radius"], C5["nodes"],
+                                  mp, 0, C5["seed"], 1.0)
+    sim.initialize()
+    return sim, placed
+
+
+def timed_steps(sim, steps):
+    sim.getMaxSpeed()
+    t = time.perf_counter()
+    sim.run(steps)
+    sim.getMaxSpeed()
+    return steps / (time.perf_counter() - t)
+
+
+def mdf_learned_bench(steps, warmup, train_steps, epochs):
+    """Steps/s of the plain 5-iteration MDF in FP64 (the reference's scheme)
+    against the FP32 path whose first correction comes from the trained
+    network, at the same no-slip residual; the network is trained here on rows
+    the plain FP32 solver collects."""
+    from softflow_amd import softflow_core as sc
+
+    out = {"workload": "2048x512 regularized tau=0.8 fx=3e-5, 500 Skalak (G_s=0.06, C=1) r=4 "
+                       "(25 nodes) seeded over the channel (2.4% area fraction)"}
+    # plain FP64, 5 MDF iterations (4 corrections), residual measured after the last
+    plain, placed = build_c5("fp64", 5)
+    out["capsules"] = placed
+    plain.run(warmup)
+    plain.setMdfControl(0.0, 0, True)
+    plain.run(10)
+    plain.mdfStats()
+    out["plain_fp64_steps_per_s"] = round(timed_steps(plain, steps), 1)
+    sp = plain.mdfStats()
+    # training rows from the plain FP32 solver (4 corrections: the same scheme)
+    t0 = time.perf_counter()
+    coll, _ = build_c5("fp32", 5)
+    coll.run(warmup)
+    K = coll.capsules().totalNodes()
+    coll.collectMdfSamples(train_steps * K)
+    coll.run(train_steps)
+    model = sc.MdfModel(0, 7)
+    loss = model.train(coll, epochs=epochs, batch=4096, lr=2e-3, seed=11)
+    train_s = time.perf_counter() - t0
+    rows = coll.mdfSampleCount()
+    del coll
+    # FP32 with the network's correction first, plain ones until the residual
+    # is at most what the plain FP64 scheme reached (here above 1e-4: the
+    # plain iteration stalls near 3e-4 on this channel)
+    tol = sp["max_residual"]
+    learned, _ = build_c5("fp32", 1)
+    learned.setMdfControl(1e-300, 4, False)
+    learned.run(warmup)
+    learned.setMdfModel(model, True)
+    learned.setMdfControl(tol, 8, True)
+    learned.run(10)
+    learned.mdfStats()
+    out["learned_fp32_steps_per_s"] = round(timed_steps(learned, steps), 1)
+    sl = learned.mdfStats()
+    out["speedup"] = round(out["learned_fp32_steps_per_s"] / out["plain_fp64_steps_per_s"], 3)
+    out["mlups_plain_fp64"] = round(out["plain_fp64_steps_per_s"] * C5["nx"] * C5["ny"] / 1e6, 1)
+    out["mlups_learned_fp32"] = round(out["learned_fp32_steps_per_s"] * C5["nx"] * C5["ny"] / 1e6, 1)
+    out["residual_tolerance"] = tol
+    out["plain"] = {"corrections_per_step": sp["mean_corrections"], "residual_mean": sp["mean_residual"],
+                    "residual_max": sp["max_residual"]}
+    out["learned"] = {"corrections_per_step": sl["mean_corrections"], "residual_mean": sl["mean_residual"],
+                      "residual_max": sl["max_residual"]}
+    # field difference after the same number of steps from the same start
+    fa, fb = plain.field(), learned.field()
+    ua = np.r_[np.ravel(fa.velocity_x()), np.ravel(fa.velocity_y())]
+    ub = np.r_[np.ravel(fb.velocity_x()), np.ravel(fb.velocity_y())]
+    out["velocity_rel_l2_vs_plain_fp64"] = float(np.linalg.norm(ub - ua) / np.linalg.norm(ua))
+    out["training"] = {"rows": int(rows), "epochs": epochs, "seconds": round(train_s, 2),
+                       "loss_first_epoch": loss[0], "loss_last_epoch": loss[-1]}
+    return out
+
+
 def calibrate_copy(mib):
     import torch
 
@@ -151,6 +263,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--no-mdf", action="store_true",
+                    help="skip the learned-vs-plain MDF comparison")
+    ap.add_argument("--mdf-steps", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -212,6 +327,14 @@ def main():
               + ", ".join(f"{k}={v:.4f}" for k, v in sorted(kt.items())), file=sys.stderr, flush=True)
 
     stats = sim.fieldStats()
+    del sim
+    mdf = None
+    if not args.no_mdf:
+        try:
+            mdf = mdf_learned_bench(args.mdf_steps, 200, 160, 6)
+        except RuntimeError as e:   # reported, so the main line is still printed
+            mdf = {"error": str(e)}
+        print("# learned vs plain MDF: " + json.dumps(mdf), file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         try:
@@ -223,7 +346,7 @@ def main():
             cpu = {"value": None, "unit": "MLUPS", "cores": 1, "kind": "port", "sample": f"unavailable: {e}"}
 
     out = {
-        "metric": "MLUPS (whole step), 4096x1024 + 2000 capsules",
+        "metric": "MLUPS + % HBM roofline, 4096x1024 + 2000 capsules; learned vs plain MDF steps/s",
         "value": round(mlups, 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -240,6 +363,7 @@ def main():
                    "nx": NX, "ny": NY, "capsules": 2000, "membrane_nodes": 800 * 38 + 1200 * 19},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mdf_learned_vs_plain": mdf,
         "kernel_ms": {k: round(v, 5) for k, v in sorted(kt.items())},
         "lbm_launch": tune.strip().splitlines()[-1],
         "mass": stats[0],

@@ -171,14 +171,14 @@ int  sf_mdf_stats(sf_ctx* ctx, double* out4);
 /* Record (inputs, converged correction) rows of every node at every step into
  * a device sample set of up to capacity rows (0 frees it).  A row holds
  * SF_MLP_IN inputs then SF_MLP_OUT targets, FP32. */
-enum { SF_MLP_IN = 15, SF_MLP_OUT = 2, SF_MLP_WIDTH = 64 };
+enum { SF_MLP_IN = 17, SF_MLP_OUT = 2, SF_MLP_WIDTH = 64 };
 int  sf_mdf_collect(sf_ctx* ctx, int64_t capacity);
 /* rows written so far; rows_out (may be NULL) receives them. */
 int  sf_mdf_samples(sf_ctx* ctx, int64_t* rows, float* rows_out);
 
 /* The warm-start network: SF_MLP_IN -> 64 -> 64 -> SF_MLP_OUT with GELU,
- * FP32, with input/output normalisation (in_mean[15], in_scale[15],
- * out_mean[2], out_scale[2]).  Parameters: W1[64][15], b1[64], W2[64][64],
+ * FP32, with input/output normalisation (in_mean[17], in_scale[17],
+ * out_mean[2], out_scale[2]).  Parameters: W1[64][17], b1[64], W2[64][64],
  * b2[64], W3[2][64], b3[2]. */
 typedef struct sf_mlp sf_mlp;
 int  sf_mlp_create(sf_mlp** net, int device, uint64_t seed);
@@ -186,10 +186,10 @@ int  sf_mlp_destroy(sf_mlp* net);
 int  sf_mlp_num_params(int* n_params, int* n_norm);
 int  sf_mlp_get_params(sf_mlp* net, float* params, float* norm);
 int  sf_mlp_set_params(sf_mlp* net, const float* params, const float* norm);
-/* y[n][2] = network(x[n][15]) in physical units. */
+/* y[n][2] = network(x[n][17]): out_mean + out_scale * net((x - in_mean) / in_scale). */
 int  sf_mlp_predict(sf_mlp* net, int n, const float* x, float* y);
 /* Loss L = 1/(2n) sum |out - t^|^2 in normalised units and its gradient with
- * respect to the parameters, for host samples x[n][15], t[n][2]. */
+ * respect to the parameters, for host samples x[n][17], t[n][2]. */
 int  sf_mlp_loss_grad(sf_mlp* net, int n, const float* x, const float* t, double* loss,
                       float* grad);
 /* Adam (beta 0.9/0.999) over the context's sample set: epochs passes of

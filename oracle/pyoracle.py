@@ -293,10 +293,10 @@ class Sim:
 
 
 def mlp_predict(params, norm, x, width=64):
-    """FP64 restatement of the warm-start network: y[n, 2] for x[n, 15]."""
+    """FP64 restatement of the warm-start network: y[n, 2] for x[n, 17]."""
     P = np.ascontiguousarray(params, dtype=np.float64)
     nr = np.ascontiguousarray(norm, dtype=np.float64)
-    X = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 15)
+    X = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 17)
     Y = np.zeros((X.shape[0], 2))
     oracle_lib().sfo_mlp_predict(_ptr(P), _ptr(nr), width, X.shape[0], _ptr(X), _ptr(Y))
     return Y
@@ -307,7 +307,7 @@ def mlp_loss_grad(params, norm, x, t, width=64):
     L = oracle_lib()
     P = np.ascontiguousarray(params, dtype=np.float64)
     nr = np.ascontiguousarray(norm, dtype=np.float64)
-    X = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 15)
+    X = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 17)
     T = np.ascontiguousarray(t, dtype=np.float64).reshape(-1, 2)
     G = np.zeros(L.sfo_mlp_num_params(width))
     loss = L.sfo_mlp_loss_grad(_ptr(P), _ptr(nr), width, X.shape[0], _ptr(X), _ptr(T), _ptr(G))

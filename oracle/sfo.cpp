@@ -1312,11 +1312,11 @@ void sfo_mdf_stats(sfo_sim* s, double* out4) {
 // The warm-start network of the device engine (no reference counterpart),
 // restated in FP64: x^ = (x - in_mean) / in_scale; a1 = GELU(W1 x^ + b1);
 // a2 = GELU(W2 a1 + b2); out = W3 a2 + b3; y = out_mean + out_scale * out.
-// GELU(z) = z Phi(z) (exact, with erf).  Parameter block W1[w][15], b1[w],
-// W2[w][w], b2[w], W3[2][w], b3[2]; norm = in_mean[15], in_scale[15],
+// GELU(z) = z Phi(z) (exact, with erf).  Parameter block W1[w][17], b1[w],
+// W2[w][w], b2[w], W3[2][w], b3[2]; norm = in_mean[17], in_scale[17],
 // out_mean[2], out_scale[2].
 namespace {
-constexpr int NI = 15, NO = 2;
+constexpr int NI = 17, NO = 2;
 inline double gelu_(double z) { return 0.5 * z * (1.0 + std::erf(z / std::sqrt(2.0))); }
 inline double gelu_d_(double z) {
     return 0.5 * (1.0 + std::erf(z / std::sqrt(2.0))) + z * std::exp(-0.5 * z * z) / std::sqrt(2.0 * M_PI);

@@ -113,7 +113,7 @@ void   sfo_set_mdf_control(sfo_sim* s, double tol, int max_corrections, int prob
 void   sfo_mdf_stats(sfo_sim* s, double* out4);
 
 /* The device engine's warm-start network restated in FP64 (no reference
- * counterpart): 15 -> width -> width -> 2, GELU. */
+ * counterpart): 17 -> width -> width -> 2, GELU. */
 int    sfo_mlp_num_params(int width);
 void   sfo_mlp_predict(const double* P, const double* norm, int width, int n,
                        const double* x, double* y);

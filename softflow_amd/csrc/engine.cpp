@@ -767,7 +767,7 @@ void MdfModel::setNormalization(const std::vector<float>& n) {
 }
 
 std::vector<float> MdfModel::predict(const std::vector<float>& x) const {
-    if (x.size() % SF_MLP_IN) throw std::invalid_argument("MdfModel: inputs must be n x 15");
+    if (x.size() % SF_MLP_IN) throw std::invalid_argument("MdfModel: inputs must be n x 17");
     const int n = static_cast<int>(x.size() / SF_MLP_IN);
     std::vector<float> y(static_cast<size_t>(n) * SF_MLP_OUT);
     mlp_check(sf_mlp_predict(net_, n, x.data(), y.data()), "sf_mlp_predict");
@@ -777,7 +777,7 @@ std::vector<float> MdfModel::predict(const std::vector<float>& x) const {
 std::pair<double, std::vector<float>> MdfModel::lossGrad(const std::vector<float>& x,
                                                          const std::vector<float>& t) const {
     if (x.size() % SF_MLP_IN || t.size() != x.size() / SF_MLP_IN * SF_MLP_OUT)
-        throw std::invalid_argument("MdfModel: inputs must be n x 15 and targets n x 2");
+        throw std::invalid_argument("MdfModel: inputs must be n x 17 and targets n x 2");
     const int n = static_cast<int>(x.size() / SF_MLP_IN);
     int np = 0, nn = 0;
     sf_mlp_num_params(&np, &nn);

@@ -275,7 +275,7 @@ private:
 class Simulation;
 
 // The learned warm start of multi-direct forcing (no counterpart in the
-// reference): a 15 -> 64 -> 64 -> 2 GELU network on the device, trained on
+// reference): a 17 -> 64 -> 64 -> 2 GELU network on the device, trained on
 // the (inputs, converged correction) rows a Simulation collects.
 class MdfModel {
 public:
@@ -290,9 +290,9 @@ public:
     std::vector<float> normalization() const;
     void setParams(const std::vector<float>& p);
     void setNormalization(const std::vector<float>& n);
-    // y[n][2] for x[n][15]
+    // y[n][2] for x[n][17]
     std::vector<float> predict(const std::vector<float>& x) const;
-    // (loss, gradient) on host samples x[n][15], t[n][2]
+    // (loss, gradient) on host samples x[n][17], t[n][2]
     std::pair<double, std::vector<float>> lossGrad(const std::vector<float>& x,
                                                    const std::vector<float>& t) const;
     sf_mlp* handle() const { return net_; }

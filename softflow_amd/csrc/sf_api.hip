@@ -263,9 +263,10 @@ static int dmalloc(T** p, size_t bytes) {
 static void free_caps(Ctx& c) {
     void* ptrs[] = {c.off, c.owner, c.desc, c.has_prev, c.x, c.y, c.vx, c.vy, c.fx, c.fy,
                     c.L0, c.th0, c.ds0, c.Lprev, c.spread, c.mdf_total, c.mdf_u0, c.mdf_hist,
-                    c.mdf_feat};
+                    c.mdf_feat, c.mdf_pred, c.mdf_f1, c.mdf_dmin, c.mdf_head};
     for (void* p : ptrs) if (p) hipFree(p);
     c.mdf_total = nullptr; c.mdf_u0 = nullptr; c.mdf_hist = nullptr; c.mdf_feat = nullptr;
+    c.mdf_pred = nullptr; c.mdf_f1 = nullptr; c.mdf_dmin = nullptr; c.mdf_head = nullptr;
     c.off = c.owner = c.has_prev = nullptr;
     c.desc = nullptr;
     c.x = c.y = c.vx = c.vy = c.fx = c.fy = c.L0 = c.th0 = c.ds0 = c.Lprev = nullptr;
@@ -543,6 +544,12 @@ int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_d
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.spread), K * sizeof(NodeSpread)));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_total), 2 * static_cast<size_t>(K) * sizeof(double)));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_u0), 2 * static_cast<size_t>(K) * sizeof(double)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_pred), 2 * static_cast<size_t>(K) * sizeof(double)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_f1), 2 * static_cast<size_t>(K) * sizeof(double)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_dmin), static_cast<size_t>(K) * sizeof(float)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_head), 2 * static_cast<size_t>(K) * sizeof(double)));
+    SF_HIP(hipMemset(c.mdf_head, 0, 2 * static_cast<size_t>(K) * sizeof(double)));
+    SF_HIP(hipMemset(c.mdf_dmin, 0, static_cast<size_t>(K) * sizeof(float)));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_hist), MLP_HIST * 2 * static_cast<size_t>(K) * sizeof(float)));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_feat), MLP_IN * static_cast<size_t>(K) * sizeof(float)));
     SF_HIP(hipMemset(c.mdf_total, 0, 2 * static_cast<size_t>(K) * sizeof(double)));

@@ -40,16 +40,20 @@ __device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, 
 // force already written by the MDF interpolation (only re-validate).
 // In mode 1 the MDF control decides first: once the residual of this
 // iteration is within tolerance (residual mode) nothing more is spread this
-// step; otherwise the correction is counted into the node's step total.
+// step -- except at iteration force_it (0: none), which is always applied;
+// otherwise the correction is counted into the node's step total, and into
+// its head total while it <= MDF_HEAD.
 __global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,
                             const int* off, const int* owner, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, int* counters,
-                            int mode, MdfDev* md, double tol, int it, double* total) {
+                            int mode, MdfDev* md, double tol, int it, int force_it, double* total,
+                            double* head) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     if (mode == 1 && md) {
         if (md->conv) return;
-        if (tol > 0.0 && __longlong_as_double(static_cast<long long>(md->res[it])) <= tol) {
+        if (tol > 0.0 && it != force_it &&
+            __longlong_as_double(static_cast<long long>(md->res[it])) <= tol) {
             md->conv = 1;          // every thread reaches the same decision
             md->conv_it = it;
             return;
@@ -57,6 +61,10 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
         if (k == 0) md->applied += 1;
         total[2 * k] += rec[k].fx;
         total[2 * k + 1] += rec[k].fy;
+        if (it <= MDF_HEAD) {
+            head[2 * k] += rec[k].fx;
+            head[2 * k + 1] += rec[k].fy;
+        }
     }
     const double px = x[k], py = y[k];
     NodeSpread s;
@@ -483,13 +491,23 @@ __global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double*
 
 // Per-node inputs of the warm-start model, in FP32: slip V - U at the first
 // correction, the node's membrane + contact force, the signed discrete
-// curvature (turning angle / arc length) and the corrections of the last
-// MLP_HIST steps.
+// curvature (turning angle / arc length), the distances to the nearest node of
+// another capsule and to the nearest wall plane (both capped at 2 r_cut: the
+// Peskin stencils of close neighbours overlap and a uniform correction then
+// moves the node's velocity more), and the slips of the last MLP_HIST steps.
+// The history holds measured slips, never the model's own outputs, so a
+// prediction cannot feed back into later inputs.
+// It also keeps the plain first correction dF1 (the reference frame of the
+// model's output).
 __global__ void k_mdf_features(const double* x, const double* y, const double* fx, const double* fy,
                                const int* off, const int* owner, const double* u0,
-                               const float* hist, float* feat, int K, double Lx) {
+                               const float* hist, float* feat, const NodeSpread* rec, double* f1,
+                               const float* dmin, int K, double Lx, double ybot, double ytop,
+                               double dcap) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
+    f1[2 * k] = rec[k].fx;
+    f1[2 * k + 1] = rec[k].fy;
     const int cap = owner[k];
     const int o = off[cap], n = off[cap + 1] - o, i = k - o;
     const int kp = o + (i - 1 + n) % n, kn = o + (i + 1) % n;
@@ -507,24 +525,46 @@ __global__ void k_mdf_features(const double* x, const double* y, const double* f
     f[2] = static_cast<float>(fx[k]);
     f[3] = static_cast<float>(fy[k]);
     f[4] = static_cast<float>(ds > 0.0 ? th / ds : 0.0);
+    f[5] = fminf(dmin[k], static_cast<float>(dcap));
+    f[6] = static_cast<float>(fmin(fmax(fmin(y[k] - ybot, ytop - y[k]), 0.0), dcap));
     for (int h = 0; h < MLP_HIST; ++h) {
-        f[5 + 2 * h] = hist[(static_cast<size_t>(h) * K + k) * 2];
-        f[6 + 2 * h] = hist[(static_cast<size_t>(h) * K + k) * 2 + 1];
+        f[7 + 2 * h] = hist[(static_cast<size_t>(h) * K + k) * 2];
+        f[8 + 2 * h] = hist[(static_cast<size_t>(h) * K + k) * 2 + 1];
     }
 }
 
 // Clears the per-step MDF state (graph-safe: a kernel, not a memset).
-__global__ void k_mdf_begin(MdfDev* md, double* total, int K) {
+__global__ void k_mdf_begin(MdfDev* md, double* total, double* head, int K) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) { md->conv = 0; md->applied = 0; md->conv_it = 0; }
     if (i < MDF_MAX_CORR + 2) md->res[i] = 0ull;
-    if (i < 2 * K) total[i] = 0.0;
+    if (i < 2 * K) { total[i] = 0.0; head[i] = 0.0; }
 }
 
-// End of the MDF stage: shift the per-node history, append sample rows
-// (inputs, converged correction) and accumulate the step statistics.
-__global__ void k_mdf_record(MdfDev* md, const double* total, float* hist, const float* feat,
-                             float* ds, int64_t ds_cap, int K, int hist_on) {
+// The model's targets: the sum T of the step's first MDF_HEAD plain
+// corrections in the frame of the first one, dF1: T = g_par dF1 +
+// g_perp rot90(dF1), clamped to [-MDF_GAIN_MAX, MDF_GAIN_MAX] (unit gain where
+// dF1 vanishes).  Predicting more of the plain iteration in one shot (the
+// total of four corrections, a gain near 2.2) overshoots the modes of stiffer
+// local coupling -- nearby membranes, where one correction already moves the
+// interpolated velocity further -- and the coupled run diverges within a few
+// hundred steps; the two-correction head stays stable and the plain
+// corrections that follow take the residual down to the tolerance.
+__device__ __forceinline__ void gains_of(double tx, double ty, double ax, double ay, float& gp,
+                                         float& gq) {
+    const double a = ax * ax + ay * ay;
+    if (!(a > 1e-40)) { gp = 1.f; gq = 0.f; return; }
+    const double p = (tx * ax + ty * ay) / a, q = (ax * ty - ay * tx) / a;
+    gp = static_cast<float>(fmin(fmax(p, -MDF_GAIN_MAX), MDF_GAIN_MAX));
+    gq = static_cast<float>(fmin(fmax(q, -MDF_GAIN_MAX), MDF_GAIN_MAX));
+}
+
+// End of the MDF stage: shift the per-node slip history, append sample rows
+// (inputs, gains of the total correction applied) and accumulate the step
+// statistics.
+__global__ void k_mdf_record(MdfDev* md, const double* total, const double* head, const double* u0,
+                             const double* f1, float* hist, const float* feat, float* ds,
+                             int64_t ds_cap, int K, int hist_on) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     if (ds) {
@@ -533,8 +573,10 @@ __global__ void k_mdf_record(MdfDev* md, const double* total, float* hist, const
             float* r = ds + row * (MLP_IN + MLP_OUT);
             const float* f = feat + static_cast<size_t>(k) * MLP_IN;
             for (int j = 0; j < MLP_IN; ++j) r[j] = f[j];
-            r[MLP_IN] = static_cast<float>(total[2 * k]);
-            r[MLP_IN + 1] = static_cast<float>(total[2 * k + 1]);
+            float gp, gq;
+            gains_of(head[2 * k], head[2 * k + 1], f1[2 * k], f1[2 * k + 1], gp, gq);
+            r[MLP_IN] = gp;
+            r[MLP_IN + 1] = gq;
         }
     }
     if (hist_on) {
@@ -542,8 +584,8 @@ __global__ void k_mdf_record(MdfDev* md, const double* total, float* hist, const
             hist[(static_cast<size_t>(h) * K + k) * 2] = hist[(static_cast<size_t>(h - 1) * K + k) * 2];
             hist[(static_cast<size_t>(h) * K + k) * 2 + 1] = hist[(static_cast<size_t>(h - 1) * K + k) * 2 + 1];
         }
-        hist[2 * static_cast<size_t>(k)] = static_cast<float>(total[2 * k]);
-        hist[2 * static_cast<size_t>(k) + 1] = static_cast<float>(total[2 * k + 1]);
+        hist[2 * static_cast<size_t>(k)] = static_cast<float>(u0[2 * k]);
+        hist[2 * static_cast<size_t>(k) + 1] = static_cast<float>(u0[2 * k + 1]);
     }
 }
 
@@ -690,11 +732,12 @@ void launch_spread(Ctx& c, int mode, bool last, int it) {
     if (mode == 0) {
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                            c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                           c.tile_cnt, c.tile_list, c.counters, 0, nullptr, 0.0, 0, nullptr);
+                           c.tile_cnt, c.tile_list, c.counters, 0, nullptr, 0.0, 0, 0, nullptr, nullptr);
     } else {
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                            c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                           c.tile_cnt, c.tile_list, c.counters, 1, c.mdf, c.mdf_tol, it, c.mdf_total);
+                           c.tile_cnt, c.tile_list, c.counters, 1, c.mdf, c.mdf_tol, it, 0,
+                           c.mdf_total, c.mdf_head);
     }
     const void* Fin = c.F[c.Fcur];
     // mode 1 writes into the buffer that is neither current nor previous
@@ -733,20 +776,24 @@ void launch_mdf_interp(Ctx& c, int it) {
 
 void launch_mdf_begin(Ctx& c) {
     const int n = std::max(2 * c.K, MDF_MAX_CORR + 2);
-    hipLaunchKernelGGL(k_mdf_begin, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.mdf, c.mdf_total, c.K);
+    hipLaunchKernelGGL(k_mdf_begin, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.mdf, c.mdf_total,
+                       c.mdf_head, c.K);
 }
 
 void launch_mdf_features(Ctx& c) {
     hipLaunchKernelGGL(k_mdf_features, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
-                       c.fx, c.fy, c.off, c.owner, c.mdf_u0, c.mdf_hist, c.mdf_feat, c.K,
-                       c.periodic_x ? static_cast<double>(c.nx) : 0.0);
+                       c.fx, c.fy, c.off, c.owner, c.mdf_u0, c.mdf_hist, c.mdf_feat, c.spread,
+                       c.mdf_f1, c.mdf_dmin, c.K, c.periodic_x ? static_cast<double>(c.nx) : 0.0,
+                       c.p.periodic_y ? -1e30 : 0.5,
+                       c.p.periodic_y ? 1e30 : static_cast<double>(c.ny) - 1.5, 2.0 * c.p.rep_r_cut);
 }
 
 void launch_mdf_record(Ctx& c, int ncorr) {
     const bool collect = c.ds_buf != nullptr;
     if (collect || c.mdf_hist_on)
         hipLaunchKernelGGL(k_mdf_record, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.mdf,
-                           c.mdf_total, c.mdf_hist, c.mdf_feat, c.ds_buf, c.ds_cap, c.K,
+                           c.mdf_total, c.mdf_head, c.mdf_u0, c.mdf_f1, c.mdf_hist, c.mdf_feat, c.ds_buf,
+                           c.ds_cap, c.K,
                            c.mdf_hist_on);
     hipLaunchKernelGGL(k_mdf_finish, dim3(1), dim3(1), 0, c.stream, c.mdf, ncorr, c.mdf_probe, c.K,
                        collect ? 1 : 0, c.ds_cap);

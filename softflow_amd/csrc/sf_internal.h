@@ -45,9 +45,13 @@ struct NodeSpread {
 // residual max_k |V_k - U(X_k)| is at most tol, after at most max_corr
 // corrections.  The first correction may come from a learned model.
 constexpr int MDF_MAX_CORR = 32;
-constexpr int MLP_IN = 15;          // slip (2), node force (2), curvature (1), 5 x 2 history
+// model inputs: slip (2), node force (2), curvature (1), distance to the
+// nearest other capsule (1), distance to the nearest wall (1), 5 x 2 slip history
+constexpr int MLP_IN = 17;
 constexpr int MLP_OUT = 2;
 constexpr int MLP_HIST = 5;
+constexpr int MDF_HEAD = 2;           // plain corrections the model's output stands for
+constexpr double MDF_GAIN_MAX = 8.0;  // |gain| bound of the targets; predictions are held to [0, 4] x [-2, 2]
 
 struct MdfDev {
     int conv;                    // the residual reached tol this step
@@ -155,8 +159,12 @@ struct Ctx {
     bool mdf_warm = false;
     MdfDev* mdf = nullptr;               // device
     double* mdf_total = nullptr;         // 2K: corrections spread this step, per node
+    double* mdf_head = nullptr;          // 2K: sum of the first MDF_HEAD corrections this step
     double* mdf_u0 = nullptr;            // 2K: slip V - U at the first correction
-    float* mdf_hist = nullptr;           // MLP_HIST x 2K: totals of the previous steps
+    double* mdf_pred = nullptr;          // 2K: the model's gains before filtering
+    double* mdf_f1 = nullptr;            // 2K: the plain first correction dF1
+    float* mdf_dmin = nullptr;           // K: distance to the nearest other capsule's node
+    float* mdf_hist = nullptr;           // MLP_HIST x 2K: slips of the previous steps
     float* mdf_feat = nullptr;           // K x MLP_IN
     int mdf_hist_on = 0;                 // history is being recorded
     float* ds_buf = nullptr;             // sample rows (MLP_IN + MLP_OUT floats)

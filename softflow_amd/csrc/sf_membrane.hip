@@ -300,7 +300,21 @@ struct RepArgs {
     double rcut2_hi;   // > rcut^2 by a margin: beyond it no pair can be within rcut
     int ncx, ncy;
     double xorg, cdx, cdy;
+    // when set: distance from each node to the nearest node of another
+    // capsule among those examined (capped at 2 r_cut)
+    float* dmin;
 };
+
+__device__ __forceinline__ double pair_dist2(const RepArgs& a, double px, double py, double bx,
+                                             double by) {
+    double dx = bx - px;
+    if (a.Lx > 0.0) {
+        if (dx > 0.5 * a.Lx) dx -= a.Lx;
+        if (dx < -0.5 * a.Lx) dx += a.Lx;
+    }
+    const double dy = by - py;
+    return dx * dx + dy * dy;
+}
 
 __device__ __forceinline__ void pair_force(const RepArgs& a, double pax, double pay, double vax,
                                            double vay, double pbx, double pby, double vbx,
@@ -395,12 +409,14 @@ __global__ void k_repulsion(RepArgs a) {
     const double px = a.x[k], py = a.y[k], vx = a.vx[k], vy = a.vy[k];
     double fx = a.fx[k], fy = a.fy[k];
     double Fx, Fy;
+    double d2min = 4.0 * a.rcut * a.rcut;
     if (!CELLS) {
         for (int j = 0; j < a.ncaps; ++j) {
             if (j == cap) continue;
             for (int b = a.off[j]; b < a.off[j + 1]; ++b) {
                 pair_force(a, px, py, vx, vy, a.x[b], a.y[b], a.vx[b], a.vy[b], Fx, Fy);
                 fx += Fx; fy += Fy;
+                if (a.dmin) d2min = fmin(d2min, pair_dist2(a, px, py, a.x[b], a.y[b]));
             }
         }
     } else {
@@ -422,6 +438,7 @@ __global__ void k_repulsion(RepArgs a) {
                     const double4 b = a.pay[base + t];
                     pair_force(a, px, py, vx, vy, b.x, b.y, b.z, b.w, Fx, Fy);
                     fx += Fx; fy += Fy;
+                    if (a.dmin) d2min = fmin(d2min, pair_dist2(a, px, py, b.x, b.y));
                 }
             }
         }
@@ -432,6 +449,7 @@ __global__ void k_repulsion(RepArgs a) {
     fx += Fx; fy += Fy;
     a.fx[k] = fx;
     a.fy[k] = fy;
+    if (a.dmin) a.dmin[k] = static_cast<float>(sqrt(d2min));
 }
 
 // x extent of the cell grid when x is not periodic (cell_list.h:50-66):
@@ -471,6 +489,7 @@ void launch_repulsion(Ctx& c) {
     a.Lx = c.periodic_x ? static_cast<double>(c.nx) : 0.0;
     a.ybot = 0.5; a.ytop = static_cast<double>(c.ny) - 1.5;
     a.rcut2_hi = a.rcut * a.rcut * (1.0 + 1e-12);
+    a.dmin = c.mdf_hist_on ? c.mdf_dmin : nullptr;
     const int bs = 128, nb = (c.K + bs - 1) / bs;
     if (c.K <= 100) {
         hipLaunchKernelGGL((k_repulsion<false>), dim3(nb), dim3(bs), 0, c.stream, a);

@@ -1,7 +1,8 @@
-// The learned warm start of multi-direct forcing: a 15 -> 64 -> 64 -> 2 GELU
-// network that predicts a node's converged MDF correction from its slip
-// velocity, its membrane + contact force, the local curvature and the
-// corrections of the previous steps.  The reference has no counterpart; its
+// The learned warm start of multi-direct forcing: a 17 -> 64 -> 64 -> 2 GELU
+// network that predicts the sum of a node's first plain MDF corrections -- as
+// two gains on the first one -- from its slip velocity, its membrane + contact
+// force, the local curvature, the distances to other capsules and walls, and
+// its slips at the previous steps.  The reference has no counterpart; its
 // only network (ml/simple_nn.cpp) is not on the stepping path.
 //
 // Everything runs on the device in FP32: inference inside the step, and
@@ -38,16 +39,20 @@ __device__ __forceinline__ float gelu_d(float z) {
            z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-// Inference inside the step: replaces the first MDF correction of every node
-// by the network's prediction (unless the residual is already within
-// tolerance, or an input or the prediction is not finite).
+// Inference inside the step: the network's two outputs are the gains of the
+// node's total correction in the frame of its first plain correction dF1,
+// T = g_par dF1 + g_perp rot90(dF1).  They go to gain[2k], held to
+// g_par in [0, 4] and g_perp in [-2, 2], and default to (1, 0), the plain
+// correction, where an input or an output is not finite.  Nothing
+// happens when the residual is already within tolerance.
 __global__ __launch_bounds__(GB) void k_mlp_infer(const float* __restrict__ P,
                                                   const float* __restrict__ norm, const float* feat,
-                                                  NodeSpread* rec, int K, const MdfDev* md,
-                                                  double tol) {
+                                                  double* gain, int K, const MdfDev* md, double tol) {
     if (tol > 0.0 && __longlong_as_double(static_cast<long long>(md->res[1])) <= tol) return;
     const int k = blockIdx.x * GB + threadIdx.x;
     if (k >= K) return;
+    gain[2 * k] = 1.0;
+    gain[2 * k + 1] = 0.0;
     float xh[MLP_IN];
     bool ok = true;
 #pragma unroll
@@ -76,11 +81,71 @@ __global__ __launch_bounds__(GB) void k_mlp_infer(const float* __restrict__ P,
 #pragma unroll
         for (int o = 0; o < MLP_OUT; ++o) out[o] += P[W3_OFF + o * MLP_W + i] * a2;
     }
-    const float fx = norm[2 * MLP_IN] + norm[2 * MLP_IN + MLP_OUT] * out[0];
-    const float fy = norm[2 * MLP_IN + 1] + norm[2 * MLP_IN + MLP_OUT + 1] * out[1];
-    if (!isfinite(fx) || !isfinite(fy)) return;
-    rec[k].fx = static_cast<double>(fx);
-    rec[k].fy = static_cast<double>(fy);
+    const float gp = norm[2 * MLP_IN] + norm[2 * MLP_IN + MLP_OUT] * out[0];
+    const float gq = norm[2 * MLP_IN + 1] + norm[2 * MLP_IN + MLP_OUT + 1] * out[1];
+    if (!isfinite(gp) || !isfinite(gq)) return;
+    gain[2 * k] = fminf(fmaxf(gp, 0.f), 4.f);
+    gain[2 * k + 1] = fminf(fmaxf(gq, -2.f), 2.f);
+}
+
+// The gains, low-passed along each membrane with the binomial weights
+// (1 4 6 4 1)/16, scale the first plain correction dF1 (smooth, since both
+// velocities it compares are interpolated).  Node-by-node prediction errors
+// would otherwise alternate between neighbours: that zig-zag barely moves the
+// interpolated velocity, so the residual cannot see it, yet it pushes the
+// fluid.
+__global__ void k_mdf_smooth(const double* gain, NodeSpread* rec, const int* off, const int* owner,
+                             int K, const MdfDev* md, double tol) {
+    if (tol > 0.0 && __longlong_as_double(static_cast<long long>(md->res[1])) <= tol) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int cap = owner[k];
+    const int o = off[cap], n = off[cap + 1] - o, i = k - o;
+    double gp = gain[2 * k], gq = gain[2 * k + 1];
+    if (n >= 5) {
+        const int j[5] = {o + (i - 2 + n) % n, o + (i - 1 + n) % n, k, o + (i + 1) % n, o + (i + 2) % n};
+        const double w[5] = {1.0, 4.0, 6.0, 4.0, 1.0};
+        gp = 0.0;
+        gq = 0.0;
+        for (int t = 0; t < 5; ++t) { gp += w[t] * gain[2 * j[t]]; gq += w[t] * gain[2 * j[t] + 1]; }
+        gp *= 0.0625;
+        gq *= 0.0625;
+    }
+    const double ax = rec[k].fx, ay = rec[k].fy;    // dF1
+    rec[k].fx = gp * ax - gq * ay;
+    rec[k].fy = gp * ay + gq * ax;
+}
+
+// The network alone on n inputs: y = out_mean + out_scale * net(x^).
+__global__ __launch_bounds__(GB) void k_mlp_forward(const float* __restrict__ P,
+                                                    const float* __restrict__ norm, const float* x,
+                                                    float* y, int n) {
+    const int k = blockIdx.x * GB + threadIdx.x;
+    if (k >= n) return;
+    float xh[MLP_IN];
+#pragma unroll
+    for (int j = 0; j < MLP_IN; ++j) xh[j] = (x[static_cast<size_t>(k) * MLP_IN + j] - norm[j]) / norm[MLP_IN + j];
+    float a1[MLP_W];
+#pragma unroll
+    for (int i = 0; i < MLP_W; ++i) {
+        float acc = P[B1_OFF + i];
+#pragma unroll
+        for (int j = 0; j < MLP_IN; ++j) acc += P[W1_OFF + i * MLP_IN + j] * xh[j];
+        a1[i] = gelu(acc);
+    }
+    float out[MLP_OUT];
+#pragma unroll
+    for (int o = 0; o < MLP_OUT; ++o) out[o] = P[B3_OFF + o];
+    for (int i = 0; i < MLP_W; ++i) {
+        float acc = P[B2_OFF + i];
+#pragma unroll
+        for (int j = 0; j < MLP_W; ++j) acc += P[W2_OFF + i * MLP_W + j] * a1[j];
+        const float a2 = gelu(acc);
+#pragma unroll
+        for (int o = 0; o < MLP_OUT; ++o) out[o] += P[W3_OFF + o * MLP_W + i] * a2;
+    }
+#pragma unroll
+    for (int o = 0; o < MLP_OUT; ++o) y[static_cast<size_t>(k) * MLP_OUT + o] = norm[2 * MLP_IN + o] + norm[2 * MLP_IN + MLP_OUT + o] * out[o];
 }
 
 // Forward + backward of GB samples; the workgroup's gradient of
@@ -252,7 +317,9 @@ __global__ void k_colstats(const float* rows, int64_t nrows, int64_t chunk, doub
 void launch_mlp_infer(Ctx& c) {
     if (!c.model || c.K == 0) return;
     hipLaunchKernelGGL(k_mlp_infer, dim3((c.K + GB - 1) / GB), dim3(GB), 0, c.stream,
-                       c.model->params, c.model->norm, c.mdf_feat, c.spread, c.K, c.mdf, c.mdf_tol);
+                       c.model->params, c.model->norm, c.mdf_feat, c.mdf_pred, c.K, c.mdf, c.mdf_tol);
+    hipLaunchKernelGGL(k_mdf_smooth, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.mdf_pred,
+                       c.spread, c.off, c.owner, c.K, c.mdf, c.mdf_tol);
 }
 
 static int ensure_partial(Mlp& m, int nblk) {
@@ -428,29 +495,18 @@ int sf_mlp_predict(sf_mlp* h, int n, const float* x, float* y) {
     if (n == 0) return SF_OK;
     Mlp& m = h->m;
     SF_HIP(hipSetDevice(m.device));
-    // reuse the in-step kernel: features in, corrections out through records
-    float* dx = nullptr;
-    NodeSpread* rec = nullptr;
-    MdfDev* md = nullptr;
+    float *dx = nullptr, *dy = nullptr;
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&dx), static_cast<size_t>(n) * MLP_IN * sizeof(float)));
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&rec), static_cast<size_t>(n) * sizeof(NodeSpread));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&md), sizeof(MdfDev));
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&dy), static_cast<size_t>(n) * MLP_OUT * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(dx, x, static_cast<size_t>(n) * MLP_IN * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(md, 0, sizeof(MdfDev));
-    if (e == hipSuccess) e = hipMemset(rec, 0xff, static_cast<size_t>(n) * sizeof(NodeSpread));   // NaN
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_mlp_infer, dim3((n + GB - 1) / GB), dim3(GB), 0, m.stream, m.params,
-                           m.norm, dx, rec, n, md, 0.0);
+        hipLaunchKernelGGL(k_mlp_forward, dim3((n + GB - 1) / GB), dim3(GB), 0, m.stream, m.params,
+                           m.norm, dx, dy, n);
         e = hipStreamSynchronize(m.stream);
     }
-    std::vector<NodeSpread> hr(n);
-    if (e == hipSuccess) e = hipMemcpy(hr.data(), rec, static_cast<size_t>(n) * sizeof(NodeSpread), hipMemcpyDeviceToHost);
-    hipFree(dx); hipFree(rec); hipFree(md);
+    if (e == hipSuccess) e = hipMemcpy(y, dy, static_cast<size_t>(n) * MLP_OUT * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(dx); hipFree(dy);
     SF_HIP(e);
-    for (int i = 0; i < n; ++i) {
-        y[2 * i] = static_cast<float>(hr[i].fx);
-        y[2 * i + 1] = static_cast<float>(hr[i].fy);
-    }
     return SF_OK;
 }
 

@@ -52,7 +52,7 @@ def test_residual_mode_matches_oracle_fp64():
 def _random_net(seed):
     rng = np.random.default_rng(seed)
     P = rng.normal(scale=0.2, size=po.oracle_lib().sfo_mlp_num_params(64)).astype(np.float32)
-    nrm = np.r_[rng.normal(size=15), rng.uniform(0.5, 2, 15), rng.normal(size=2),
+    nrm = np.r_[rng.normal(size=17), rng.uniform(0.5, 2, 17), rng.normal(size=2),
                 rng.uniform(0.5, 2, 2)].astype(np.float32)
     return P, nrm
 
@@ -66,7 +66,7 @@ def test_network_forward_and_gradient_match_oracle():
     assert np.array_equal(m.params(), P) and np.array_equal(m.normalization(), nrm)
     rng = np.random.default_rng(4)
     for n in (1, 63, 64, 200):
-        x = rng.normal(size=(n, 15)).astype(np.float32)
+        x = rng.normal(size=(n, 17)).astype(np.float32)
         t = rng.normal(size=(n, 2)).astype(np.float32)
         y = m.predict(x)
         yo = po.mlp_predict(P.astype(np.float64), nrm.astype(np.float64), x.astype(np.float64))
@@ -92,10 +92,13 @@ def _residual_after(sim, ncorr, steps=10):
 
 
 def test_learned_warm_start_gains():
-    """Plain MDF converges slowly here (the iteration matrix of the Peskin
-    stencil on a ring with unit spacing has eigenvalues in (0, 0.38]), so the
-    network learns the total of four plain corrections -- the reference's
-    5-iteration MDF -- and replaces them with one."""
+    """Plain MDF converges slowly (the iteration matrix of the Peskin stencil
+    on a ring with unit spacing has eigenvalues in (0, 0.38]).  The network
+    predicts the first two plain corrections in one -- the part that stays
+    stable when applied at once -- and the plain corrections that follow take
+    the residual to the tolerance; the targets are gains on the first plain
+    correction, nearly constant (1.62 +- 0.02), so the normalised loss mostly
+    measures the spread the network resolves."""
     from softflow_amd import softflow_core as sc
     print("learned warm start: collect", flush=True)
     sim = product_sim(LEARN)
@@ -107,11 +110,11 @@ def test_learned_warm_start_gains():
     st = sim.mdfStats()
     tol = 1.2 * st["max_residual"]             # what four plain corrections reach, with margin
     rows = sim.mdfSamples()
-    assert rows.shape == (300 * 500, 17) and np.all(np.isfinite(rows))
+    assert rows.shape == (300 * 500, 19) and np.all(np.isfinite(rows))
     model = sc.MdfModel(0, 7)
     loss = model.train(sim, epochs=30, batch=1024, lr=2e-3, seed=11)
     print("training loss per epoch", np.array(loss)[[0, 5, 10, 29]], flush=True)
-    assert loss[-1] < 0.1 * loss[0]
+    assert loss[-1] < 0.5 * loss[0]
     # deterministic: the same seed gives the same network
     m2 = sc.MdfModel(0, 7)
     m2.train(sim, epochs=2, batch=1024, lr=2e-3, seed=11)

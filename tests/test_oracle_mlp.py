@@ -2,7 +2,7 @@
 network (neither has a counterpart in the reference).
 
 - network forward: a hand-built network whose output is a closed form in
-  erf (checks the parameter layout W1[w][15], b1, W2[w][w], b2, W3[2][w], b3
+  erf (checks the parameter layout W1[w][NI], b1, W2[w][w], b2, W3[2][w], b3
   and the normalisation);
 - network gradient: central finite differences of the oracle's own loss;
 - residual-mode MDF: with an unreachable tolerance it is the reference's
@@ -19,13 +19,14 @@ from oracle import pyoracle as po
 from tests.cases import oracle_sim, oracle_state
 
 W = 64
-NP = W * 15 + W + W * W + W + 2 * W + 2
+NI = 17
+NP = W * NI + W + W * W + W + 2 * W + 2
 
 
 def offsets(w=W):
     o = {}
     o["W1"] = 0
-    o["b1"] = o["W1"] + w * 15
+    o["b1"] = o["W1"] + w * NI
     o["W2"] = o["b1"] + w
     o["b2"] = o["W2"] + w * w
     o["W3"] = o["b2"] + w
@@ -34,7 +35,7 @@ def offsets(w=W):
 
 
 def ident_norm():
-    return np.r_[np.zeros(15), np.ones(15), np.zeros(2), np.ones(2)]
+    return np.r_[np.zeros(NI), np.ones(NI), np.zeros(2), np.ones(2)]
 
 
 def gelu(z):
@@ -44,16 +45,16 @@ def gelu(z):
 def test_forward_closed_form():
     o = offsets()
     P = np.zeros(NP)
-    P[o["W1"] + 3 * 15 + 7] = 2.0        # hidden unit 3 reads input 7
+    P[o["W1"] + 3 * NI + 7] = 2.0        # hidden unit 3 reads input 7
     P[o["b1"] + 3] = -0.5
     P[o["W2"] + 10 * W + 3] = 1.5        # unit 10 of layer 2 reads unit 3
     P[o["W3"] + 1 * W + 10] = -3.0       # output 1 reads unit 10
     P[o["b3"] + 0] = 0.25
     nrm = ident_norm()
-    nrm[7], nrm[15 + 7] = 1.0, 4.0       # x^7 = (x7 - 1) / 4
-    nrm[30 + 1], nrm[32 + 1] = 0.1, 2.0  # y1 = 0.1 + 2 out1
+    nrm[7], nrm[NI + 7] = 1.0, 4.0       # x^7 = (x7 - 1) / 4
+    nrm[2 * NI + 1], nrm[2 * NI + 2 + 1] = 0.1, 2.0   # y1 = 0.1 + 2 out1
     rng = np.random.default_rng(0)
-    x = rng.normal(size=(20, 15))
+    x = rng.normal(size=(20, NI))
     y = po.mlp_predict(P, nrm, x)
     g0 = gelu(0.0)
     for s in range(20):
@@ -66,8 +67,8 @@ def test_forward_closed_form():
 def test_gradient_finite_differences():
     rng = np.random.default_rng(1)
     P = rng.normal(scale=0.3, size=NP)
-    nrm = np.r_[rng.normal(size=15), rng.uniform(0.5, 2, 15), rng.normal(size=2), rng.uniform(0.5, 2, 2)]
-    x = rng.normal(size=(7, 15))
+    nrm = np.r_[rng.normal(size=NI), rng.uniform(0.5, 2, NI), rng.normal(size=2), rng.uniform(0.5, 2, 2)]
+    x = rng.normal(size=(7, NI))
     t = rng.normal(size=(7, 2))
     L, G = po.mlp_loss_grad(P, nrm, x, t)
     assert L > 0
