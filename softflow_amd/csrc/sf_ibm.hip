@@ -46,9 +46,10 @@ __device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, 
 __global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,
                             const int* off, const int* owner, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, int* counters,
-                            int mode, MdfDev* md, double tol, int it, int force_it, double* total,
-                            double* head) {
+                            int mode, MdfDev* md, double tol, int it, int force_it,
+                            double* total, double* head) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 0 && k == 0) counters[3] = 0;     // the touched-tile list is rebuilt after binning
     if (k >= K) return;
     if (mode == 1 && md) {
         if (md->conv) return;
@@ -121,6 +122,7 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
             const int slot = atomicAdd(&tile_cnt[tile], 1);
             if (slot < TILE_MAXL) tile_list[static_cast<size_t>(tile) * TILE_MAXL + slot] = k;
             else atomicOr(&counters[1], 1);
+
         }
 }
 
@@ -128,9 +130,10 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
 // mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction).
 // Tiles no node touches are skipped when the output buffer already holds zeros
 // there (dirty flags per buffer), so the dense write covers touched tiles only.
-constexpr int STAGE_MAX = 256;     // node records staged in LDS (lists up to this length)
-constexpr int CELL_MAXPC = 24;     // contributions one cell collects in LDS before falling back
+constexpr int STAGE_MAX = 64;      // node records staged in LDS (lists up to this length)
+constexpr int CELL_MAXPC = 16;     // contributions one cell collects in LDS before falling back
 constexpr int NO_ROW = -0x40000000;
+constexpr int SPREAD_GRID = 2048;  // persistent workgroups walking the touched tiles
 
 // Unwrapped stencil column (lo .. lo+3) whose single periodic wrap lands on cx,
 // or NO_ROW if none does.
@@ -141,13 +144,71 @@ __device__ __forceinline__ int stencil_column(int cx, int lo, int nx) {
     return NO_ROW;
 }
 
+// One lane per tile.  Tiles no node touches: the wavefront clears (or, for an
+// MDF correction, copies from the input) each one that needs it -- those whose
+// output still holds non-zero values, or whose input does; after convergence
+// (skip) every tile counts as untouched.  Touched tiles (mode 0, which runs
+// once per step after the binning): appended to act as (tile, node count),
+// one atomic per wavefront.
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_spread_rest(const int* tile_cnt, const void* Fin, void* Fout,
+                                                     const uint8_t* dirty_in, uint8_t* dirty_out,
+                                                     int2* act, int* counters, int nx, int ny,
+                                                     int ntx, int ntiles, int mode,
+                                                     const MdfDev* md) {
+    using T = typename std::conditional<FP32, float, double>::type;
+    const bool skip = mode == 1 && md && md->conv;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cnt = tile < ntiles ? tile_cnt[tile] : 0;
+    bool need = false, copy = false;
+    if (tile < ntiles && (skip || cnt == 0)) {
+        copy = mode == 1 && dirty_in[tile];
+        need = copy || dirty_out[tile];
+    }
+    if (mode == 0) {
+        const unsigned long long touched = __ballot(cnt > 0);
+        int base = 0;
+        if (lane == 0 && touched) base = atomicAdd(&counters[3], __popcll(touched));
+        base = __shfl(base, 0);
+        if (cnt > 0) {
+            const int pos = base + __popcll(touched & ((1ull << lane) - 1ull));
+            if (pos < ntiles) act[pos] = make_int2(tile, cnt);
+        }
+    }
+    unsigned long long todo = __ballot(need);
+    const unsigned long long cmask = __ballot(copy);
+    const int wave_tile0 = tile - lane;
+    while (todo) {
+        const int b = __ffsll(static_cast<long long>(todo)) - 1;
+        todo &= todo - 1;
+        const int tt = wave_tile0 + b;
+        const bool cp = (cmask >> b) & 1ull;
+        const int tx0 = (tt % ntx) * TILE_X, ty0 = (tt / ntx) * TILE_Y;
+        for (int c = lane; c < TILE_X * TILE_Y; c += 64) {
+            const int cx = tx0 + (c % TILE_X), cy = ty0 + (c / TILE_X);
+            if (cx >= nx || cy >= ny) continue;
+            const size_t n = static_cast<size_t>(cy) * nx + cx;
+            T vx = 0, vy = 0;
+            if (cp) { vx = static_cast<const T*>(Fin)[2 * n]; vy = static_cast<const T*>(Fin)[2 * n + 1]; }
+            static_cast<T*>(Fout)[2 * n] = vx;
+            static_cast<T*>(Fout)[2 * n + 1] = vy;
+        }
+        if (lane == 0) dirty_out[tt] = cp ? 1 : 0;
+    }
+}
+
+// Touched tiles: a persistent grid walks the list of tiles the node binning
+// touched (act[0 .. counters[3])), one 256-thread workgroup per tile and one
+// thread per cell of its TILE_X x TILE_Y cells.
+// mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction).
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile_cnt,
-                                                const int* tile_list, const double* lut,
-                                                const void* Fin, void* Fout,
-                                                const uint8_t* dirty_in, uint8_t* dirty_out,
-                                                int nx, int ny, int ntx, int mode, int last,
-                                                const MdfDev* md) {
+                                                const int* tile_list, const int2* act,
+                                                const int* counters, const double* lut,
+                                                const void* Fin, void* Fout, uint8_t* dirty_out,
+                                                int nx, int ny, int ntx, int ntiles, int mode,
+                                                int last, const MdfDev* md) {
     __shared__ double tab[1025];
     __shared__ double spx[STAGE_MAX], spy[STAGE_MAX], sfx[STAGE_MAX], sfy[STAGE_MAX], sds[STAGE_MAX];
     __shared__ int si0[STAGE_MAX], sj0[STAGE_MAX], skey[STAGE_MAX];
@@ -156,171 +217,167 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile
     unsigned short* cslot = reinterpret_cast<unsigned short*>(cslot_raw);
     int* list = cslot_raw;   // the long-list path keeps its sorted list here
     using T = typename std::conditional<FP32, float, double>::type;
-    const int tile = blockIdx.x;
     const int t = threadIdx.x;
-    int L = tile_cnt[tile];
-    if (L > TILE_MAXL) L = TILE_MAXL;
-    if (mode == 1 && md && md->conv) {
-        // converged MDF: the output is a copy of the input (F_out = F_in + 0)
-        if (last && t == 0) tile_cnt[tile] = 0;
-        L = 0;
-    }
-    const int tx0 = (tile % ntx) * TILE_X, ty0 = (tile / ntx) * TILE_Y;
-    const int cx = tx0 + (t % TILE_X);
-    const int cy = ty0 + (t / TILE_X);
-    const bool inside = cx < nx && cy < ny;
-    const size_t n = static_cast<size_t>(cy) * nx + cx;
-    if (L == 0) {
-        const bool copy = mode == 1 && dirty_in[tile];
-        const bool clear = !copy && dirty_out[tile];
-        if (inside && (copy || clear)) {
-            T vx = 0, vy = 0;
-            if (copy) { vx = static_cast<const T*>(Fin)[2 * n]; vy = static_cast<const T*>(Fin)[2 * n + 1]; }
-            static_cast<T*>(Fout)[2 * n] = vx;
-            static_cast<T*>(Fout)[2 * n + 1] = vy;
-        }
-        if (t == 0) dirty_out[tile] = copy ? 1 : 0;
+    const int nact = min(counters[3], ntiles);
+    const bool skip = mode == 1 && md && md->conv;
+    if (skip) {
+        // converged MDF: nothing to add (the rest kernel copied F_in)
+        if (last)
+            for (int i = blockIdx.x * blockDim.x + t; i < nact; i += gridDim.x * blockDim.x) tile_cnt[act[i].x] = 0;
         return;
     }
-    const int* tl = tile_list + static_cast<size_t>(tile) * TILE_MAXL;
-    for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
-    double ax = 0.0, ay = 0.0;
-    if (mode == 1 && inside) {
-        ax = static_cast<const T*>(Fin)[2 * n];
-        ay = static_cast<const T*>(Fin)[2 * n + 1];
-    }
-    auto stage = [&](int i, int k) {
-        const NodeSpread s = rec[k];
-        spx[i] = s.px; spy[i] = s.py; sfx[i] = s.fx; sfy[i] = s.fy; sds[i] = s.ds;
-        si0[i] = s.i0;
-        sj0[i] = s.valid ? s.j0 : NO_ROW;   // invalid records touch no row
-        skey[i] = k;
-    };
-    auto contribution = [&](int i, double& w, int& ok) {
-        ok = 0;
-        w = 0.0;
-        const int dyc = cy - (sj0[i] - 1);
-        if (dyc < 0 || dyc > 3) return;
-        const int ix = stencil_column(cx, si0[i] - 1, nx);
-        if (ix == NO_ROW) return;
-        w = phi_lut(tab, spx[i] - static_cast<double>(ix)) * phi_lut(tab, spy[i] - static_cast<double>(cy));
-        ok = w > 1e-15;
-    };
+    if (nact > 0)
+        for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
+    for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
+        const int2 te = act[ai];
+        const int tile = te.x;
+        __syncthreads();          // the previous tile's LDS use is over
+        int L = te.y;
+        if (L > TILE_MAXL) L = TILE_MAXL;
+        const int tx0 = (tile % ntx) * TILE_X, ty0 = (tile / ntx) * TILE_Y;
+        const int cx = tx0 + (t % TILE_X);
+        const int cy = ty0 + (t / TILE_X);
+        const bool inside = cx < nx && cy < ny;
+        const size_t n = static_cast<size_t>(cy) * nx + cx;
+        const int* tl = tile_list + static_cast<size_t>(tile) * TILE_MAXL;
+        double ax = 0.0, ay = 0.0;
+        if (mode == 1 && inside) {
+            ax = static_cast<const T*>(Fin)[2 * n];
+            ay = static_cast<const T*>(Fin)[2 * n + 1];
+        }
+        auto stage = [&](int i, int k) {
+            const NodeSpread s = rec[k];
+            spx[i] = s.px; spy[i] = s.py; sfx[i] = s.fx; sfy[i] = s.fy; sds[i] = s.ds;
+            si0[i] = s.i0;
+            sj0[i] = s.valid ? s.j0 : NO_ROW;   // invalid records touch no row
+            skey[i] = k;
+        };
+        auto contribution = [&](int i, double& w, int& ok) {
+            ok = 0;
+            w = 0.0;
+            const int dyc = cy - (sj0[i] - 1);
+            if (dyc < 0 || dyc > 3) return;
+            const int ix = stencil_column(cx, si0[i] - 1, nx);
+            if (ix == NO_ROW) return;
+            w = phi_lut(tab, spx[i] - static_cast<double>(ix)) * phi_lut(tab, spy[i] - static_cast<double>(cy));
+            ok = w > 1e-15;
+        };
 
-    if (L <= STAGE_MAX) {
-        // stage the records, then enumerate node x stencil pairs in parallel
-        // into per-cell lists
-        for (int i = t; i < L; i += 256) stage(i, tl[i]);
-        ccnt[t] = 0;
-        __syncthreads();
-        if (t == 0) {
-            if (last) tile_cnt[tile] = 0;
-            dirty_out[tile] = 1;
-        }
-        for (int i = t; i < 16 * L; i += 256) {
-            const int p = i >> 4, s = i & 15;
-            const int j0 = sj0[p];
-            if (j0 == NO_ROW) continue;
-            const int iy = j0 - 1 + (s >> 2);
-            if (iy < 0 || iy >= ny) continue;
-            const int ly = iy - ty0;
-            if (ly < 0 || ly >= TILE_Y) continue;
-            const int ix = si0[p] - 1 + (s & 3);
-            int jx = ix;
-            if (jx < 0) jx += nx;
-            if (jx >= nx) jx -= nx;
-            const int lx = jx - tx0;
-            if (lx < 0 || lx >= TILE_X) continue;
-            const double w = phi_lut(tab, spx[p] - static_cast<double>(ix)) *
-                             phi_lut(tab, spy[p] - static_cast<double>(iy));
-            if (!(w > 1e-15)) continue;
-            const int cell = ly * TILE_X + lx;
-            const int slot = atomicAdd(&ccnt[cell], 1);
-            if (slot < CELL_MAXPC) cslot[cell * CELL_MAXPC + slot] = static_cast<unsigned short>(p);
-        }
-        __syncthreads();
-        if (!inside) return;
-        const int m = ccnt[t];
-        if (m <= CELL_MAXPC) {
-            // this cell's contributions in global node order (insertion sort)
-            unsigned short* my = cslot + t * CELL_MAXPC;
-            for (int a = 1; a < m; ++a) {
-                const unsigned short v = my[a];
-                const int kv = skey[v];
-                int b = a - 1;
-                while (b >= 0 && skey[my[b]] > kv) { my[b + 1] = my[b]; --b; }
-                my[b + 1] = v;
+        if (L <= STAGE_MAX) {
+            // stage the records, then enumerate node x stencil pairs in parallel
+            // into per-cell lists
+            for (int i = t; i < L; i += 256) stage(i, tl[i]);
+            ccnt[t] = 0;
+            __syncthreads();
+            if (t == 0) {
+                if (last) tile_cnt[tile] = 0;
+                dirty_out[tile] = 1;
             }
-            for (int j = 0; j < m; ++j) {
-                const int p = my[j];
-                double w; int ok;
-                contribution(p, w, ok);
-                ax += sfx[p] * w * sds[p];
-                ay += sfy[p] * w * sds[p];
+            for (int i = t; i < 16 * L; i += 256) {
+                const int p = i >> 4, s = i & 15;
+                const int j0 = sj0[p];
+                if (j0 == NO_ROW) continue;
+                const int iy = j0 - 1 + (s >> 2);
+                if (iy < 0 || iy >= ny) continue;
+                const int ly = iy - ty0;
+                if (ly < 0 || ly >= TILE_Y) continue;
+                const int ix = si0[p] - 1 + (s & 3);
+                int jx = ix;
+                if (jx < 0) jx += nx;
+                if (jx >= nx) jx -= nx;
+                const int lx = jx - tx0;
+                if (lx < 0 || lx >= TILE_X) continue;
+                const double w = phi_lut(tab, spx[p] - static_cast<double>(ix)) *
+                                 phi_lut(tab, spy[p] - static_cast<double>(iy));
+                if (!(w > 1e-15)) continue;
+                const int cell = ly * TILE_X + lx;
+                const int slot = atomicAdd(&ccnt[cell], 1);
+                if (slot < CELL_MAXPC) cslot[cell * CELL_MAXPC + slot] = static_cast<unsigned short>(p);
+            }
+            __syncthreads();
+            const int m = ccnt[t];
+            if (inside && m <= CELL_MAXPC) {
+                // this cell's contributions in global node order (insertion sort)
+                unsigned short* my = cslot + t * CELL_MAXPC;
+                for (int a = 1; a < m; ++a) {
+                    const unsigned short v = my[a];
+                    const int kv = skey[v];
+                    int b = a - 1;
+                    while (b >= 0 && skey[my[b]] > kv) { my[b + 1] = my[b]; --b; }
+                    my[b + 1] = v;
+                }
+                for (int j = 0; j < m; ++j) {
+                    const int p = my[j];
+                    double w; int ok;
+                    contribution(p, w, ok);
+                    ax += sfx[p] * w * sds[p];
+                    ay += sfy[p] * w * sds[p];
+                }
+            } else if (inside) {
+                // more contributions than slots: walk the staged records in node order
+                int lastk = -1;
+                for (;;) {
+                    int best = -1, bestk = 0x7fffffff;
+                    for (int i = 0; i < L; ++i) {
+                        const int k = skey[i];
+                        if (k <= lastk || k >= bestk) continue;
+                        double w; int ok;
+                        contribution(i, w, ok);
+                        if (ok) { best = i; bestk = k; }
+                    }
+                    if (best < 0) break;
+                    double w; int ok;
+                    contribution(best, w, ok);
+                    ax += sfx[best] * w * sds[best];
+                    ay += sfy[best] * w * sds[best];
+                    lastk = bestk;
+                }
             }
         } else {
-            // more contributions than slots: walk the staged records in node order
-            int lastk = -1;
-            for (;;) {
-                int best = -1, bestk = 0x7fffffff;
-                for (int i = 0; i < L; ++i) {
-                    const int k = skey[i];
-                    if (k <= lastk || k >= bestk) continue;
-                    double w; int ok;
-                    contribution(i, w, ok);
-                    if (ok) { best = i; bestk = k; }
-                }
-                if (best < 0) break;
-                double w; int ok;
-                contribution(best, w, ok);
-                ax += sfx[best] * w * sds[best];
-                ay += sfy[best] * w * sds[best];
-                lastk = bestk;
+            // long list: sort it, then every cell scans it in chunks
+            int P = 1;
+            while (P < L) P <<= 1;
+            for (int i = t; i < P; i += 256) list[i] = i < L ? tl[i] : 0x7fffffff;
+            __syncthreads();
+            if (t == 0) {
+                if (last) tile_cnt[tile] = 0;
+                dirty_out[tile] = 1;
             }
-        }
-    } else {
-        // long list: sort it, then every cell scans it in chunks
-        int P = 1;
-        while (P < L) P <<= 1;
-        for (int i = t; i < P; i += 256) list[i] = i < L ? tl[i] : 0x7fffffff;
-        __syncthreads();
-        if (t == 0) {
-            if (last) tile_cnt[tile] = 0;
-            dirty_out[tile] = 1;
-        }
-        for (int k = 2; k <= P; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int i = t; i < P; i += 256) {
-                    const int l = i ^ j;
-                    if (l > i) {
-                        const int a = list[i], b = list[l];
-                        const bool up = (i & k) == 0;
-                        if ((a > b) == up) { list[i] = b; list[l] = a; }
+            for (int k = 2; k <= P; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = t; i < P; i += 256) {
+                        const int l = i ^ j;
+                        if (l > i) {
+                            const int a = list[i], b = list[l];
+                            const bool up = (i & k) == 0;
+                            if ((a > b) == up) { list[i] = b; list[l] = a; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int base = 0; base < L; base += STAGE_MAX) {
+                const int m = min(STAGE_MAX, L - base);
+                for (int i = t; i < m; i += 256) stage(i, list[base + i]);
+                __syncthreads();
+                if (inside) {
+                    for (int i = 0; i < m; ++i) {
+                        double w; int ok;
+                        contribution(i, w, ok);
+                        if (ok) {
+                            ax += sfx[i] * w * sds[i];
+                            ay += sfy[i] * w * sds[i];
+                        }
                     }
                 }
                 __syncthreads();
             }
         }
-        for (int base = 0; base < L; base += STAGE_MAX) {
-            const int m = min(STAGE_MAX, L - base);
-            for (int i = t; i < m; i += 256) stage(i, list[base + i]);
-            __syncthreads();
-            if (inside) {
-                for (int i = 0; i < m; ++i) {
-                    double w; int ok;
-                    contribution(i, w, ok);
-                    if (ok) {
-                        ax += sfx[i] * w * sds[i];
-                        ay += sfy[i] * w * sds[i];
-                    }
-                }
-            }
-            __syncthreads();
+        if (inside) {
+            static_cast<T*>(Fout)[2 * n] = static_cast<T>(ax);
+            static_cast<T*>(Fout)[2 * n + 1] = static_cast<T>(ay);
         }
-        if (!inside) return;
     }
-    static_cast<T*>(Fout)[2 * n] = static_cast<T>(ax);
-    static_cast<T*>(Fout)[2 * n + 1] = static_cast<T>(ay);
 }
 
 // ---------------------------------------------------------------------------
@@ -732,7 +789,8 @@ void launch_spread(Ctx& c, int mode, bool last, int it) {
     if (mode == 0) {
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                            c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                           c.tile_cnt, c.tile_list, c.counters, 0, nullptr, 0.0, 0, 0, nullptr, nullptr);
+                           c.tile_cnt, c.tile_list, c.counters, 0, nullptr, 0.0, 0, 0, nullptr,
+                           nullptr);
     } else {
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                            c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
@@ -745,14 +803,22 @@ void launch_spread(Ctx& c, int mode, bool last, int it) {
     void* Fout = c.F[out];
     const uint8_t* din = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
     uint8_t* dout = c.tile_dirty + static_cast<size_t>(out) * c.ntiles;
-    if (c.fp32)
-        hipLaunchKernelGGL((k_spread<true>), dim3(c.ntiles), dim3(256), 0, c.stream, c.spread,
-                           c.tile_cnt, c.tile_list, c.phi, Fin, Fout, din, dout, c.nx, c.ny, c.ntx,
-                           mode, last ? 1 : 0, mode == 1 ? c.mdf : nullptr);
-    else
-        hipLaunchKernelGGL((k_spread<false>), dim3(c.ntiles), dim3(256), 0, c.stream, c.spread,
-                           c.tile_cnt, c.tile_list, c.phi, Fin, Fout, din, dout, c.nx, c.ny, c.ntx,
-                           mode, last ? 1 : 0, mode == 1 ? c.mdf : nullptr);
+    const MdfDev* md = mode == 1 ? c.mdf : nullptr;
+    const dim3 rgrid((c.ntiles + 255) / 256);
+    const dim3 sgrid(std::min(c.ntiles, SPREAD_GRID));
+    if (c.fp32) {
+        hipLaunchKernelGGL((k_spread_rest<true>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
+                           din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md);
+        hipLaunchKernelGGL((k_spread<true>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_cnt,
+                           c.tile_list, c.tile_act, c.counters, c.phi, Fin, Fout, dout, c.nx, c.ny,
+                           c.ntx, c.ntiles, mode, last ? 1 : 0, md);
+    } else {
+        hipLaunchKernelGGL((k_spread_rest<false>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
+                           din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md);
+        hipLaunchKernelGGL((k_spread<false>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_cnt,
+                           c.tile_list, c.tile_act, c.counters, c.phi, Fin, Fout, dout, c.nx, c.ny,
+                           c.ntx, c.ntiles, mode, last ? 1 : 0, md);
+    }
     if (mode == 1) {
         c.Fpart = c.Fcur;
         c.Fcur = out;

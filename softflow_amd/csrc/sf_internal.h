@@ -141,6 +141,7 @@ struct Ctx {
     int ntx = 0, nty = 0, ntiles = 0;
     int* tile_cnt = nullptr;             // zero between steps (the last spread resets it)
     int* tile_list = nullptr;
+    int2* tile_act = nullptr;            // (tile, node count) of the tiles touched this step (counters[3])
     uint8_t* tile_dirty = nullptr;       // [3][ntiles]: tile of force buffer b holds non-zeros
 
     // repulsion cells
@@ -171,7 +172,7 @@ struct Ctx {
     int64_t ds_cap = 0;
     Mlp* model = nullptr;                // borrowed from the caller
 
-    // counters: [0] capped cells, [1] tile overflow, [2] cell overflow
+    // counters: [0] capped cells, [1] tile overflow, [2] cell overflow, [3] touched tiles
     int* counters = nullptr;
     int* host_counters = nullptr;        // pinned
 
