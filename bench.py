@@ -259,7 +259,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing-pass", action="store_true")
     ap.add_argument("--lbm-launch", default="auto",
-                    help="'auto' (autotune) or BXxBY[xl] for a fixed collide-stream launch shape")
+                    help="'auto' (autotune) or BXxBY[xl|xpN] for a fixed collide-stream launch shape "
+                         "(l: linear workgroup order, pN: persistent grid of N workgroups per XCD)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (the known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -288,8 +289,10 @@ def main():
         parts = args.lbm_launch.split("x")
         bx, by = int(parts[0]), int(parts[1])
         banded = not (len(parts) > 2 and parts[2] == "l")
-        sim.setLbmLaunch(bx, by, banded)
-        tune = "fixed %dx%d %s" % (bx, by, "xcd-banded" if banded else "linear")
+        persist = int(parts[2][1:]) if len(parts) > 2 and parts[2].startswith("p") else 0
+        sim.setLbmLaunch(bx, by, banded, persist)
+        tune = ("fixed %dx%d persistent %d/xcd" % (bx, by, persist) if persist else
+                "fixed %dx%d %s" % (bx, by, "xcd-banded" if banded else "linear"))
     sim.run(args.warmup)
     sim.getMaxSpeed()   # synchronises
     t = time.perf_counter()

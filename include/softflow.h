@@ -147,12 +147,14 @@ int  sf_set_timing(sf_ctx* ctx, int on);
 
 /* Autotuner (no counterpart in the reference): times the collide-stream
  * kernel in place over launch shapes (block width x height, XCD-banded or
- * linear workgroup order) on the current state -- the kernel only reads its
- * source buffers, so repeated launches leave the state unchanged -- and keeps
- * the fastest.  report (may be NULL) receives one line per candidate. */
+ * linear workgroup order, one-shot or a persistent grid) on the current state
+ * -- the kernel only reads its source buffers, so repeated launches leave the
+ * state unchanged -- and keeps the fastest.  report (may be NULL) receives one
+ * line per candidate. */
 int  sf_autotune(sf_ctx* ctx, int reps, char* report, int report_len);
-/* Fix the collide-stream launch shape instead (bx*by a multiple of 64, <= 256). */
-int  sf_set_lbm_launch(sf_ctx* ctx, int bx, int by, int xcd_banded);
+/* Fix the collide-stream launch shape instead (bx*by a multiple of 64, <= 256);
+ * persist > 0 launches 8 * persist workgroups that walk the XCD bands. */
+int  sf_set_lbm_launch(sf_ctx* ctx, int bx, int by, int xcd_banded, int persist);
 
 /* ---- Multi-direct forcing control and the learned warm start ------------
  * (no counterpart in the reference; its MDF is a fixed iteration count,

@@ -103,7 +103,7 @@ def load():
         "sf_kernel_times": [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), _DP, C.POINTER(C.c_int)],
         "sf_set_timing": [C.c_void_p, C.c_int],
         "sf_autotune": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],
-        "sf_set_lbm_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int],
+        "sf_set_lbm_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int],
         "sf_probe_math": [C.c_int, C.c_int, _DP, _DP, _DP],
         "sf_mdf_set_control": [C.c_void_p, C.c_double, C.c_int, C.c_int],
         "sf_mdf_stats": [C.c_void_p, _DP],

@@ -344,7 +344,8 @@ PYBIND11_MODULE(softflow_core, m) {
         }, py::arg("callback"))
         .def("setTiming", &Simulation::setTiming, py::arg("on"))
         .def("autotune", &Simulation::autotune, py::arg("reps") = 10)
-        .def("setLbmLaunch", &Simulation::setLbmLaunch, py::arg("bx"), py::arg("by"), py::arg("xcd_banded") = true)
+        .def("setLbmLaunch", &Simulation::setLbmLaunch, py::arg("bx"), py::arg("by"), py::arg("xcd_banded") = true,
+             py::arg("persist") = 0)
         .def("kernelTimes", &Simulation::kernelTimes)
         .def("setMdfControl", &Simulation::setMdfControl, py::arg("tolerance"),
              py::arg("max_corrections") = 8, py::arg("probe") = false)

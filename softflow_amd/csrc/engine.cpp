@@ -677,9 +677,9 @@ std::string Simulation::autotune(int reps) {
     return std::string(buf.data());
 }
 
-void Simulation::setLbmLaunch(int bx, int by, bool xcd_banded) {
+void Simulation::setLbmLaunch(int bx, int by, bool xcd_banded, int persist) {
     if (!ctx_) throw std::runtime_error("Simulation: initialize() has not been called");
-    check(sf_set_lbm_launch(ctx_, bx, by, xcd_banded ? 1 : 0), "sf_set_lbm_launch");
+    check(sf_set_lbm_launch(ctx_, bx, by, xcd_banded ? 1 : 0, persist), "sf_set_lbm_launch");
 }
 
 void Simulation::setMdfControl(Real tol, int max_corrections, bool probe) {

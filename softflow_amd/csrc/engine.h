@@ -336,7 +336,7 @@ public:
     std::vector<std::pair<std::string, double>> kernelTimes();
     void setTiming(bool on);
     std::string autotune(int reps = 10);
-    void setLbmLaunch(int bx, int by, bool xcd_banded);
+    void setLbmLaunch(int bx, int by, bool xcd_banded, int persist = 0);
 
     // multi-direct forcing control and the learned warm start (extensions)
     void setMdfControl(Real tol, int max_corrections, bool probe);

@@ -806,22 +806,24 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
     // the steady-state launch: previous-step shift force, scalar riding along
     const bool fuse = c.nsp > 0 && mdf_corrections(c) <= 0;
     const int shift = mdf_corrections(c) > 0 ? SHIFT_PARTIAL : SHIFT_PREV;
-    // every candidate must fit the kernel's __launch_bounds__(256)
-    static const int shapes[][2] = {{64, 4}, {64, 2}, {64, 1}, {128, 2}, {128, 1},
-                                    {256, 1}, {32, 8}, {32, 4}};
+    // every candidate must fit the kernel's __launch_bounds__(256); the last
+    // entries are persistent grids (workgroups per XCD), XCD-banded only
+    static const int shapes[][3] = {{64, 4, 0}, {64, 2, 0}, {64, 1, 0}, {128, 2, 0}, {128, 1, 0},
+                                    {256, 1, 0}, {32, 8, 0}, {32, 4, 0}, {256, 1, 64},
+                                    {256, 1, 128}, {256, 1, 256}, {128, 2, 128}, {64, 4, 128}};
     hipEvent_t e0, e1;
     SF_HIP(hipEventCreate(&e0));
     SF_HIP(hipEventCreate(&e1));
     std::string rep;
     double best = 1e30;
-    int bbx = c.lbm_bx, bby = c.lbm_by, brm = c.lbm_remap;
+    int bbx = c.lbm_bx, bby = c.lbm_by, brm = c.lbm_remap, bps = c.lbm_persist;
     SF_HIP(hipGetLastError());
     for (const auto& sh : shapes) {
         if (sh[0] * sh[1] > 256) continue;
-        for (int remap = 0; remap <= 1; ++remap) {
-            launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap);   // warm
+        for (int remap = sh[2] ? 1 : 0; remap <= 1; ++remap) {
+            launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap, sh[2]);   // warm
             SF_HIP(hipEventRecord(e0, c.stream));
-            for (int r = 0; r < reps; ++r) launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap);
+            for (int r = 0; r < reps; ++r) launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap, sh[2]);
             SF_HIP(hipEventRecord(e1, c.stream));
             SF_HIP(hipEventSynchronize(e1));
             SF_HIP(hipGetLastError());
@@ -829,19 +831,25 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
             SF_HIP(hipEventElapsedTime(&ms, e0, e1));
             const double per = ms / reps;
             char line[96];
-            std::snprintf(line, sizeof(line), "%dx%d %s %.4f ms\n", sh[0], sh[1], remap ? "xcd-banded" : "linear", per);
+            if (sh[2])
+                std::snprintf(line, sizeof(line), "%dx%d persistent %d/xcd %.4f ms\n", sh[0], sh[1], sh[2], per);
+            else
+                std::snprintf(line, sizeof(line), "%dx%d %s %.4f ms\n", sh[0], sh[1], remap ? "xcd-banded" : "linear", per);
             rep += line;
-            if (per < best) { best = per; bbx = sh[0]; bby = sh[1]; brm = remap; }
+            if (per < best) { best = per; bbx = sh[0]; bby = sh[1]; brm = remap; bps = sh[2]; }
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     SF_HIP(hipGetLastError());
     launch_fill_int(c, c.counters, 1, 0);
-    c.lbm_bx = bbx; c.lbm_by = bby; c.lbm_remap = brm;
+    c.lbm_bx = bbx; c.lbm_by = bby; c.lbm_remap = brm; c.lbm_persist = bps;
     invalidate_graph(c);
     char line[96];
-    std::snprintf(line, sizeof(line), "chosen %dx%d %s\n", bbx, bby, brm ? "xcd-banded" : "linear");
+    if (bps)
+        std::snprintf(line, sizeof(line), "chosen %dx%d persistent %d/xcd\n", bbx, bby, bps);
+    else
+        std::snprintf(line, sizeof(line), "chosen %dx%d %s\n", bbx, bby, brm ? "xcd-banded" : "linear");
     rep += line;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), report_len - 1);
@@ -851,12 +859,14 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
     return SF_OK;
 }
 
-int sf_set_lbm_launch(sf_ctx* h, int bx, int by, int xcd_banded) {
+int sf_set_lbm_launch(sf_ctx* h, int bx, int by, int xcd_banded, int persist) {
     if (!h) return fail(SF_E_ARG, "null pointer");
     if (bx < 1 || by < 1 || bx * by > 256 || (bx * by) % 64 != 0)
         return fail(SF_E_ARG, "launch shape must hold 64..256 threads in whole wavefronts");
+    if (persist < 0) return fail(SF_E_ARG, "persistent workgroups per XCD must be >= 0");
     Ctx& c = h->c;
-    c.lbm_bx = bx; c.lbm_by = by; c.lbm_remap = xcd_banded ? 1 : 0;
+    c.lbm_bx = bx; c.lbm_by = by; c.lbm_remap = (xcd_banded || persist) ? 1 : 0;
+    c.lbm_persist = persist;
     invalidate_graph(c);
     return SF_OK;
 }

@@ -191,7 +191,7 @@ struct Ctx {
     bool lattice_ready = false;
 
     // collide-stream launch shape (chosen by sf_autotune)
-    int lbm_bx = 64, lbm_by = 4, lbm_remap = 1;
+    int lbm_bx = 64, lbm_by = 4, lbm_remap = 1, lbm_persist = 0;
 };
 
 // Error reporting shared by the C-ABI translation units: sets the message
@@ -208,7 +208,7 @@ int fail(int code, const std::string& msg);
 // -- launchers (defined in the .hip files) --------------------------------
 void launch_lbm(Ctx& c, int shift_mode, bool fuse_adr);
 // the same launch with an explicit shape and no change of buffer roles
-void launch_lbm_cfg(Ctx& c, int shift_mode, bool fuse_adr, int bx, int by, int remap);
+void launch_lbm_cfg(Ctx& c, int shift_mode, bool fuse_adr, int bx, int by, int remap, int persist);
 void launch_adr(Ctx& c, int Fidx);
 void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy);
 void launch_field_stats(Ctx& c, double* dev_out);

@@ -221,6 +221,13 @@ __device__ __forceinline__ void dcol_reg(float* d, float om, float drho, float r
 }
 
 // ---------------------------------------------------------------------------
+// Streaming accesses: every population is read once and written once per
+// step, so the loads and stores bypass cache retention (nontemporal).
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 struct LbmArgs {
     const void* fsrc;
     void* fdst;
@@ -244,8 +251,10 @@ struct LbmArgs {
     double oms[4];
     // launch geometry: 1-D grid over nbx x nby blocks of blockDim cells; with
     // remap, workgroup b (dispatched to XCD b % 8) takes tile (b % 8) * per + b / 8,
-    // so each XCD streams one contiguous band of rows
-    int nbx, nblk, per_xcd, remap;
+    // so each XCD streams one contiguous band of rows.  persist > 0: a grid of
+    // 8 * persist workgroups; the persist workgroups of an XCD walk its band
+    // with stride persist.
+    int nbx, nblk, per_xcd, remap, persist;
 };
 
 __device__ __forceinline__ bool block_origin(const LbmArgs& a, int& x, int& y) {
@@ -301,7 +310,7 @@ __device__ __forceinline__ void adr_cell(const LbmArgs& a, int x, int y, int n, 
         T* gd = static_cast<T*>(a.gdst[s]);
         T g[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) g[q] = gs[q * N + n];
+        for (int q = 0; q < 9; ++q) g[q] = ld_nt(gs + q * N + n);
         double C;
         if (a.from_store) {
             C = a.Cst[s][n];
@@ -347,16 +356,14 @@ __device__ __forceinline__ void adr_cell(const LbmArgs& a, int x, int y, int n, 
             } else {
                 bounce = (yd < 0 || yd >= a.ny) || is_solid(a, xd, yd, FLAGS);
             }
-            if (bounce) gd[dopp(q) * N + n] = g[q];
-            else gd[q * N + (yd * a.nx + xd)] = g[q];
+            if (bounce) st_nt(gd + dopp(q) * N + n, g[q]);
+            else st_nt(gd + q * N + (yd * a.nx + xd), g[q]);
         }
     }
 }
 
 template <int OP, bool FP32, bool FLAGS, bool ADRF>
-__global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
-    int x, y;
-    if (!block_origin(a, x, y)) return;
+__device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
     int capped = 0;
     if (x < a.nx && y < a.ny) {
         const int n = y * a.nx + x;
@@ -371,7 +378,7 @@ __global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
             const size_t N = static_cast<size_t>(a.N);
             T f[9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) f[q] = src[q * N + n];
+            for (int q = 0; q < 9; ++q) f[q] = ld_nt(src + q * N + n);
             if constexpr (!FP32) {
                 double r = 0.0, vx = 0.0, vy = 0.0;
 #pragma unroll
@@ -436,32 +443,53 @@ __global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
                 if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
                 else oob |= (yd < 0 || yd >= a.ny);
                 if (!oob) {
-                    if (is_solid(a, xd, yd, FLAGS)) dst[dopp(q) * N + n] = f[q];
-                    else dst[q * N + (yd * a.nx + xd)] = f[q];
+                    if (is_solid(a, xd, yd, FLAGS)) st_nt(dst + dopp(q) * N + n, f[q]);
+                    else st_nt(dst + q * N + (yd * a.nx + xd), f[q]);
                 }
                 if (q != 0) {
                     const int xs = x - dcx(q), ys = y - dcy(q);
                     const bool soob = (!a.periodic_x && (xs < 0 || xs >= a.nx)) ||
                                       (!a.periodic_y && (ys < 0 || ys >= a.ny));
-                    if (soob) dst[q * N + n] = f[q];
+                    if (soob) st_nt(dst + q * N + n, f[q]);
                 }
             }
         }
     }
-    // count capped cells: one atomic per wavefront
-    unsigned long long b = __ballot(capped);
-    if ((threadIdx.x + threadIdx.y * blockDim.x) % warpSize == 0 && b)
-        atomicAdd(&a.counters[0], __popcll(b));
+    return capped;
 }
 
-static void set_geometry(const Ctx& c, LbmArgs& a, int bx, int by, int remap, dim3& g, dim3& b) {
+template <int OP, bool FP32, bool FLAGS, bool ADRF>
+__global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
+    int capped = 0;
+    if (a.persist == 0) {
+        int x, y;
+        if (!block_origin(a, x, y)) return;
+        capped = lbm_cell<OP, FP32, FLAGS, ADRF>(a, x, y);
+    } else {
+        const int b = blockIdx.x, xcd = b % 8, l = b / 8;
+        const int t_end = min((xcd + 1) * a.per_xcd, a.nblk);
+        for (int t = xcd * a.per_xcd + l; t < t_end; t += a.persist) {
+            const int x = (t % a.nbx) * blockDim.x + threadIdx.x;
+            const int y = (t / a.nbx) * blockDim.y + threadIdx.y;
+            capped += lbm_cell<OP, FP32, FLAGS, ADRF>(a, x, y);
+        }
+    }
+    // count capped cells: one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) capped += __shfl_down(capped, off);
+    if ((threadIdx.x + threadIdx.y * blockDim.x) % warpSize == 0 && capped)
+        atomicAdd(&a.counters[0], capped);
+}
+
+static void set_geometry(const Ctx& c, LbmArgs& a, int bx, int by, int remap, int persist, dim3& g,
+                         dim3& b) {
     b = dim3(bx, by);
     a.nbx = (c.nx + bx - 1) / bx;
     const int nby = (c.ny + by - 1) / by;
     a.nblk = a.nbx * nby;
     a.per_xcd = (a.nblk + 7) / 8;
     a.remap = remap;
-    g = dim3(remap ? 8 * a.per_xcd : a.nblk);
+    a.persist = persist > 0 && persist < a.per_xcd ? persist : 0;
+    g = dim3(a.persist ? 8 * a.persist : (remap ? 8 * a.per_xcd : a.nblk));
 }
 
 template <int OP, bool FP32>
@@ -487,7 +515,7 @@ static void scalar_args(Ctx& c, LbmArgs& a) {
     }
 }
 
-void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap) {
+void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap, int persist) {
     LbmArgs a{};
     a.fsrc = c.f[c.fcur];
     a.fdst = c.f[1 - c.fcur];
@@ -508,7 +536,7 @@ void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap)
     a.se = c.p.mrt_se; a.seps = c.p.mrt_s_eps; a.sq = c.p.mrt_sq;
     if (fuse_adr) scalar_args(c, a);
     dim3 b, g;
-    set_geometry(c, a, bx, by, remap, g, b);
+    set_geometry(c, a, bx, by, remap, persist, g, b);
     const int op = c.p.collision_model;
     if (c.fp32) {
         if (op == SF_REGULARIZED) lbm_dispatch_flags<SF_REGULARIZED, true>(c, a, g, b, fuse_adr);
@@ -522,7 +550,7 @@ void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap)
 }
 
 void launch_lbm(Ctx& c, int shift, bool fuse_adr) {
-    launch_lbm_cfg(c, shift, fuse_adr, c.lbm_bx, c.lbm_by, c.lbm_remap);
+    launch_lbm_cfg(c, shift, fuse_adr, c.lbm_bx, c.lbm_by, c.lbm_remap, c.lbm_persist);
     c.fcur = 1 - c.fcur;
     if (fuse_adr) {
         c.gcur = 1 - c.gcur;
@@ -609,7 +637,7 @@ void launch_adr(Ctx& c, int Fidx) {
     scalar_args(c, a);
     const void* f = c.f[c.fcur];
     dim3 b, g;
-    set_geometry(c, a, c.lbm_bx, c.lbm_by, c.lbm_remap, g, b);
+    set_geometry(c, a, c.lbm_bx, c.lbm_by, c.lbm_remap, 0, g, b);
     if (c.fp32) {
         if (c.generic_flags) hipLaunchKernelGGL((k_adr<true, true>), g, b, 0, c.stream, a, f);
         else hipLaunchKernelGGL((k_adr<true, false>), g, b, 0, c.stream, a, f);

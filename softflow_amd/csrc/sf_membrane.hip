@@ -11,7 +11,9 @@
 
 namespace sf {
 
-constexpr int MAXN = 256;   // nodes per capsule held in LDS
+// nodes per capsule held in LDS: the kernel is instantiated for capsules of
+// up to 64 nodes (small LDS, so every capsule's workgroup can be resident at
+// once) and up to 256
 
 struct MemArgs {
     double *x, *y, *vx, *vy, *fx, *fy;
@@ -34,6 +36,7 @@ __device__ __forceinline__ void mimg(double ax, double ay, double bx, double by,
     }
 }
 
+template <int MAXN>
 __global__ __launch_bounds__(64) void k_membrane(MemArgs a) {
     __shared__ double X[MAXN], Y[MAXN], FX[MAXN], FY[MAXN];
     __shared__ double Ex[MAXN], Ey[MAXN], Gx[MAXN], Gy[MAXN], Cc[MAXN], Ln[MAXN];
@@ -280,7 +283,10 @@ void launch_membrane(Ctx& c) {
     a.off = c.off; a.desc = c.desc;
     a.periodic_nx = c.periodic_x ? c.nx : 0;
     a.counters = c.counters;
-    hipLaunchKernelGGL(k_membrane, dim3(c.ncaps), dim3(64), 0, c.stream, a);
+    if (c.max_nodes <= 64)
+        hipLaunchKernelGGL(k_membrane<64>, dim3(c.ncaps), dim3(64), 0, c.stream, a);
+    else
+        hipLaunchKernelGGL(k_membrane<256>, dim3(c.ncaps), dim3(64), 0, c.stream, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -420,25 +426,40 @@ __global__ void k_repulsion(RepArgs a) {
             }
         }
     } else {
+        // the 3 x 3 neighbour cells in the reference's order (dy outer, dx
+        // inner); their counts are loaded together, then each cell's entries
+        // in batches of four whose loads are issued before any is used
         const int ci = cell_index(a, px, py);
         const int cx0 = ci % a.ncx, cy0 = ci / a.ncx;
-        for (int dyc = -1; dyc <= 1; ++dyc) {
-            const int yy = cy0 + dyc;
-            if (yy < 0 || yy >= a.ncy) continue;
-            for (int dxc = -1; dxc <= 1; ++dxc) {
-                int xx = cx0 + dxc;
-                if (a.Lx > 0.0) { if (xx < 0) xx += a.ncx; if (xx >= a.ncx) xx -= a.ncx; }
-                else if (xx < 0 || xx >= a.ncx) continue;
-                const int cell = yy * a.ncx + xx;
-                int m = a.cnt[cell];
-                if (m > CELL_MAXC) m = CELL_MAXC;
-                const size_t base = static_cast<size_t>(cell) * CELL_MAXC;
-                for (int t = 0; t < m; ++t) {
-                    if (a.own[base + t] == cap) continue;
-                    const double4 b = a.pay[base + t];
-                    pair_force(a, px, py, vx, vy, b.x, b.y, b.z, b.w, Fx, Fy);
+        int cells[9], cnts[9];
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {
+            const int yy = cy0 + u / 3 - 1;
+            int xx = cx0 + u % 3 - 1;
+            bool ok = yy >= 0 && yy < a.ncy;
+            if (a.Lx > 0.0) { if (xx < 0) xx += a.ncx; if (xx >= a.ncx) xx -= a.ncx; }
+            else ok = ok && xx >= 0 && xx < a.ncx;
+            cells[u] = ok ? yy * a.ncx + xx : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 9; ++u) cnts[u] = cells[u] >= 0 ? min(a.cnt[cells[u]], CELL_MAXC) : 0;
+        for (int u = 0; u < 9; ++u) {
+            const int m = cnts[u];
+            const size_t base = static_cast<size_t>(cells[u]) * CELL_MAXC;
+            for (int t0 = 0; t0 < m; t0 += 4) {
+                double4 b[4];
+                int ow[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (t0 + q < m) { b[q] = a.pay[base + t0 + q]; ow[q] = a.own[base + t0 + q]; }
+                    else ow[q] = cap;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (ow[q] == cap) continue;
+                    pair_force(a, px, py, vx, vy, b[q].x, b[q].y, b[q].z, b[q].w, Fx, Fy);
                     fx += Fx; fy += Fy;
-                    if (a.dmin) d2min = fmin(d2min, pair_dist2(a, px, py, b.x, b.y));
+                    if (a.dmin) d2min = fmin(d2min, pair_dist2(a, px, py, b[q].x, b[q].y));
                 }
             }
         }
