@@ -386,7 +386,7 @@ int sf_destroy(sf_ctx* h) {
     invalidate_graph(c);
     free_caps(c);
     void* ptrs[] = {c.flags, c.f[0], c.f[1], c.F[0], c.F[1], c.F[2], c.stored_rho, c.stored_u,
-                    c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_act, c.tile_dirty, c.cell_cnt, c.cell_list,
+                    c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_act, c.tile_rec, c.tile_dirty, c.cell_cnt, c.cell_list,
                     c.cell_pay, c.cell_own, c.mdf, c.ds_buf};
     for (void* p : ptrs) if (p) hipFree(p);
     for (int s = 0; s < 4; ++s) {
@@ -561,6 +561,7 @@ int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_d
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_cnt), c.ntiles * sizeof(int)));
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_list), static_cast<size_t>(c.ntiles) * TILE_MAXL * sizeof(int)));
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_act), static_cast<size_t>(c.ntiles) * sizeof(int2)));
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_rec), static_cast<size_t>(c.ntiles) * TILE_RECS * sizeof(NodeSpread)));
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.tile_dirty), 3 * static_cast<size_t>(c.ntiles)));
         SF_HIP(hipMemset(c.tile_cnt, 0, c.ntiles * sizeof(int)));
         // the force buffers may hold anything from before capsules existed

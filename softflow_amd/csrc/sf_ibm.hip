@@ -45,8 +45,8 @@ __device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, 
 // its head total while it <= MDF_HEAD.
 __global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,
                             const int* off, const int* owner, NodeSpread* rec, int K, int nx, int ny,
-                            int ntx, int nty, int* tile_cnt, int* tile_list, int* counters,
-                            int mode, MdfDev* md, double tol, int it, int force_it,
+                            int ntx, int nty, int* tile_cnt, int* tile_list, NodeSpread* tile_rec,
+                            int* counters, int mode, MdfDev* md, double tol, int it, int force_it,
                             double* total, double* head) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (mode == 0 && k == 0) counters[3] = 0;     // the touched-tile list is rebuilt after binning
@@ -97,7 +97,7 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
     s.i0 = i0;
     s.j0 = j0;
     s.valid = ok ? 1 : 0;
-    s.pad = 0;
+    s.pad = k;
     rec[k] = s;
     if (mode != 0 || !pos_ok || j0 < -2 || j0 >= ny + 2) return;
     // tiles touched by the stencil columns i0-1..i0+2 (single wrap) and rows j0-1..j0+2
@@ -122,6 +122,7 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
             const int slot = atomicAdd(&tile_cnt[tile], 1);
             if (slot < TILE_MAXL) tile_list[static_cast<size_t>(tile) * TILE_MAXL + slot] = k;
             else atomicOr(&counters[1], 1);
+            if (slot < TILE_RECS) tile_rec[static_cast<size_t>(tile) * TILE_RECS + slot] = s;
 
         }
 }
@@ -133,7 +134,7 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
 constexpr int STAGE_MAX = 64;      // node records staged in LDS (lists up to this length)
 constexpr int CELL_MAXPC = 16;     // contributions one cell collects in LDS before falling back
 constexpr int NO_ROW = -0x40000000;
-constexpr int SPREAD_GRID = 2048;  // persistent workgroups walking the touched tiles
+constexpr int SPREAD_GRID = 1 << 30;  // workgroups walking the touched tiles (capped at the tile count)
 
 // Unwrapped stencil column (lo .. lo+3) whose single periodic wrap lands on cx,
 // or NO_ROW if none does.
@@ -203,8 +204,8 @@ __global__ __launch_bounds__(256) void k_spread_rest(const int* tile_cnt, const 
 // thread per cell of its TILE_X x TILE_Y cells.
 // mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction).
 template <bool FP32>
-__global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile_cnt,
-                                                const int* tile_list, const int2* act,
+__global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const NodeSpread* tile_rec,
+                                                int* tile_cnt, const int* tile_list, const int2* act,
                                                 const int* counters, const double* lut,
                                                 const void* Fin, void* Fout, uint8_t* dirty_out,
                                                 int nx, int ny, int ntx, int ntiles, int mode,
@@ -245,13 +246,13 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile
             ax = static_cast<const T*>(Fin)[2 * n];
             ay = static_cast<const T*>(Fin)[2 * n + 1];
         }
-        auto stage = [&](int i, int k) {
-            const NodeSpread s = rec[k];
+        auto stage_rec = [&](int i, const NodeSpread& s) {
             spx[i] = s.px; spy[i] = s.py; sfx[i] = s.fx; sfy[i] = s.fy; sds[i] = s.ds;
             si0[i] = s.i0;
             sj0[i] = s.valid ? s.j0 : NO_ROW;   // invalid records touch no row
-            skey[i] = k;
+            skey[i] = s.pad;                    // the node index
         };
+        auto stage = [&](int i, int k) { stage_rec(i, rec[k]); };
         auto contribution = [&](int i, double& w, int& ok) {
             ok = 0;
             w = 0.0;
@@ -266,7 +267,15 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, int* tile
         if (L <= STAGE_MAX) {
             // stage the records, then enumerate node x stencil pairs in parallel
             // into per-cell lists
-            for (int i = t; i < L; i += 256) stage(i, tl[i]);
+            // mode 0 reads the record copies the binning left in the tile
+            // (one dependent load less); MDF corrections change the forces
+            // after binning, so mode 1 reads the records themselves
+            if (mode == 0 && L <= TILE_RECS) {
+                const NodeSpread* tr = tile_rec + static_cast<size_t>(tile) * TILE_RECS;
+                for (int i = t; i < L; i += 256) stage_rec(i, tr[i]);
+            } else {
+                for (int i = t; i < L; i += 256) stage(i, tl[i]);
+            }
             ccnt[t] = 0;
             __syncthreads();
             if (t == 0) {
@@ -789,13 +798,13 @@ void launch_spread(Ctx& c, int mode, bool last, int it) {
     if (mode == 0) {
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                            c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                           c.tile_cnt, c.tile_list, c.counters, 0, nullptr, 0.0, 0, 0, nullptr,
-                           nullptr);
+                           c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 0, nullptr, 0.0, 0, 0,
+                           nullptr, nullptr);
     } else {
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                            c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                           c.tile_cnt, c.tile_list, c.counters, 1, c.mdf, c.mdf_tol, it, 0,
-                           c.mdf_total, c.mdf_head);
+                           c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 1, c.mdf, c.mdf_tol, it,
+                           0, c.mdf_total, c.mdf_head);
     }
     const void* Fin = c.F[c.Fcur];
     // mode 1 writes into the buffer that is neither current nor previous
@@ -809,13 +818,13 @@ void launch_spread(Ctx& c, int mode, bool last, int it) {
     if (c.fp32) {
         hipLaunchKernelGGL((k_spread_rest<true>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
                            din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md);
-        hipLaunchKernelGGL((k_spread<true>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_cnt,
+        hipLaunchKernelGGL((k_spread<true>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
                            c.tile_list, c.tile_act, c.counters, c.phi, Fin, Fout, dout, c.nx, c.ny,
                            c.ntx, c.ntiles, mode, last ? 1 : 0, md);
     } else {
         hipLaunchKernelGGL((k_spread_rest<false>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
                            din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md);
-        hipLaunchKernelGGL((k_spread<false>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_cnt,
+        hipLaunchKernelGGL((k_spread<false>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
                            c.tile_list, c.tile_act, c.counters, c.phi, Fin, Fout, dout, c.nx, c.ny,
                            c.ntx, c.ntiles, mode, last ? 1 : 0, md);
     }

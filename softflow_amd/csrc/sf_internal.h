@@ -28,13 +28,15 @@ __host__ __device__ constexpr double dw(int q) {
 constexpr int TILE_X = 32;
 constexpr int TILE_Y = 8;
 constexpr int TILE_MAXL = 1024;    // node references per tile
+constexpr int TILE_RECS = 64;      // node records copied into the tile (the spread's short lists)
 constexpr int CELL_MAXC = 32;      // nodes per contact cell
 
 // Shift-force source of the collision velocity (see Ctx::shift_next).
 enum Shift : int { SHIFT_ZERO = 0, SHIFT_PREV = 1, SHIFT_PARTIAL = 2, SHIFT_STORED = 3 };
 
-// Per-node contribution to a spread: position, force, arc length and the
-// stencil origin.  One record per node, capsule-major.
+// Per-node contribution to a spread: position, force, arc length, the
+// stencil origin and (pad) the node index.  One record per node,
+// capsule-major.
 struct NodeSpread {
     double px, py, fx, fy, ds;
     int i0, j0, valid, pad;
@@ -141,6 +143,7 @@ struct Ctx {
     int ntx = 0, nty = 0, ntiles = 0;
     int* tile_cnt = nullptr;             // zero between steps (the last spread resets it)
     int* tile_list = nullptr;
+    NodeSpread* tile_rec = nullptr;      // ntiles x TILE_RECS record copies
     int2* tile_act = nullptr;            // (tile, node count) of the tiles touched this step (counters[3])
     uint8_t* tile_dirty = nullptr;       // [3][ntiles]: tile of force buffer b holds non-zeros
 

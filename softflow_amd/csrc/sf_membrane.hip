@@ -426,40 +426,25 @@ __global__ void k_repulsion(RepArgs a) {
             }
         }
     } else {
-        // the 3 x 3 neighbour cells in the reference's order (dy outer, dx
-        // inner); their counts are loaded together, then each cell's entries
-        // in batches of four whose loads are issued before any is used
         const int ci = cell_index(a, px, py);
         const int cx0 = ci % a.ncx, cy0 = ci / a.ncx;
-        int cells[9], cnts[9];
-#pragma unroll
-        for (int u = 0; u < 9; ++u) {
-            const int yy = cy0 + u / 3 - 1;
-            int xx = cx0 + u % 3 - 1;
-            bool ok = yy >= 0 && yy < a.ncy;
-            if (a.Lx > 0.0) { if (xx < 0) xx += a.ncx; if (xx >= a.ncx) xx -= a.ncx; }
-            else ok = ok && xx >= 0 && xx < a.ncx;
-            cells[u] = ok ? yy * a.ncx + xx : -1;
-        }
-#pragma unroll
-        for (int u = 0; u < 9; ++u) cnts[u] = cells[u] >= 0 ? min(a.cnt[cells[u]], CELL_MAXC) : 0;
-        for (int u = 0; u < 9; ++u) {
-            const int m = cnts[u];
-            const size_t base = static_cast<size_t>(cells[u]) * CELL_MAXC;
-            for (int t0 = 0; t0 < m; t0 += 4) {
-                double4 b[4];
-                int ow[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (t0 + q < m) { b[q] = a.pay[base + t0 + q]; ow[q] = a.own[base + t0 + q]; }
-                    else ow[q] = cap;
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (ow[q] == cap) continue;
-                    pair_force(a, px, py, vx, vy, b[q].x, b[q].y, b[q].z, b[q].w, Fx, Fy);
+        for (int dyc = -1; dyc <= 1; ++dyc) {
+            const int yy = cy0 + dyc;
+            if (yy < 0 || yy >= a.ncy) continue;
+            for (int dxc = -1; dxc <= 1; ++dxc) {
+                int xx = cx0 + dxc;
+                if (a.Lx > 0.0) { if (xx < 0) xx += a.ncx; if (xx >= a.ncx) xx -= a.ncx; }
+                else if (xx < 0 || xx >= a.ncx) continue;
+                const int cell = yy * a.ncx + xx;
+                int m = a.cnt[cell];
+                if (m > CELL_MAXC) m = CELL_MAXC;
+                const size_t base = static_cast<size_t>(cell) * CELL_MAXC;
+                for (int t = 0; t < m; ++t) {
+                    if (a.own[base + t] == cap) continue;
+                    const double4 b = a.pay[base + t];
+                    pair_force(a, px, py, vx, vy, b.x, b.y, b.z, b.w, Fx, Fy);
                     fx += Fx; fy += Fy;
-                    if (a.dmin) d2min = fmin(d2min, pair_dist2(a, px, py, b[q].x, b[q].y));
+                    if (a.dmin) d2min = fmin(d2min, pair_dist2(a, px, py, b.x, b.y));
                 }
             }
         }
@@ -471,6 +456,106 @@ __global__ void k_repulsion(RepArgs a) {
     a.fx[k] = fx;
     a.fy[k] = fy;
     if (a.dmin) a.dmin[k] = static_cast<float>(sqrt(d2min));
+}
+
+// Cell-list contact with 16 lanes per node: lane u < 9 scans neighbour cell u
+// (the reference's dy-outer, dx-inner order), lanes 9 and 10 take the two
+// wall images, all in parallel; every non-zero pair force is kept in LDS with
+// its place in that order, and the node's lane 0 adds them to the membrane
+// force in order -- the sequence of additions of the serial loop, minus its
+// exact zeros.  A lane with more than REP_SLOTS non-zero pairs makes lane 0
+// redo the node serially.
+constexpr int REP_NODES = 16;      // nodes per 256-thread workgroup
+constexpr int REP_SLOTS = 6;       // non-zero pair forces a lane keeps
+
+__global__ __launch_bounds__(256) void k_repulsion_lanes(RepArgs a) {
+    __shared__ double sfx[REP_NODES][11][REP_SLOTS], sfy[REP_NODES][11][REP_SLOTS];
+    __shared__ int scnt[REP_NODES][16];
+    __shared__ float sdmin[REP_NODES][16];
+    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int k = blockIdx.x * REP_NODES + g;
+    const bool live = k < a.K;
+    int cap = 0;
+    double px = 0.0, py = 0.0, vx = 0.0, vy = 0.0;
+    if (live) { cap = a.owner[k]; px = a.x[k]; py = a.y[k]; vx = a.vx[k]; vy = a.vy[k]; }
+    int nz = 0;
+    double d2min = 4.0 * a.rcut * a.rcut;
+    double Fx, Fy;
+    auto keep = [&](double fx, double fy) {
+        if (fx == 0.0 && fy == 0.0) return;
+        if (nz < REP_SLOTS) { sfx[g][l][nz] = fx; sfy[g][l][nz] = fy; }
+        ++nz;
+    };
+    if (live && l < 9) {
+        const int ci = cell_index(a, px, py);
+        const int cx0 = ci % a.ncx, cy0 = ci / a.ncx;
+        const int yy = cy0 + l / 3 - 1;
+        int xx = cx0 + l % 3 - 1;
+        bool ok = yy >= 0 && yy < a.ncy;
+        if (a.Lx > 0.0) { if (xx < 0) xx += a.ncx; if (xx >= a.ncx) xx -= a.ncx; }
+        else ok = ok && xx >= 0 && xx < a.ncx;
+        if (ok) {
+            const int cell = yy * a.ncx + xx;
+            int m = a.cnt[cell];
+            if (m > CELL_MAXC) m = CELL_MAXC;
+            const size_t base = static_cast<size_t>(cell) * CELL_MAXC;
+            for (int t = 0; t < m; ++t) {
+                if (a.own[base + t] == cap) continue;
+                const double4 b = a.pay[base + t];
+                pair_force(a, px, py, vx, vy, b.x, b.y, b.z, b.w, Fx, Fy);
+                keep(Fx, Fy);
+                if (a.dmin) d2min = fmin(d2min, pair_dist2(a, px, py, b.x, b.y));
+            }
+        }
+    } else if (live && l < 11) {
+        pair_force(a, px, py, vx, vy, px, l == 9 ? a.ybot : a.ytop, 0.0, 0.0, Fx, Fy);
+        keep(Fx, Fy);
+    }
+    scnt[g][l] = nz;
+    sdmin[g][l] = static_cast<float>(sqrt(d2min));
+    __syncthreads();
+    if (!live || l != 0) return;
+    double fx = a.fx[k], fy = a.fy[k];
+    bool overflow = false;
+    for (int u = 0; u < 11; ++u) overflow |= scnt[g][u] > REP_SLOTS;
+    if (!overflow) {
+        for (int u = 0; u < 11; ++u)
+            for (int j = 0; j < scnt[g][u]; ++j) { fx += sfx[g][u][j]; fy += sfy[g][u][j]; }
+    } else {
+        // the serial loop of the reference (cells, then the wall images)
+        const int ci = cell_index(a, px, py);
+        const int cx0 = ci % a.ncx, cy0 = ci / a.ncx;
+        for (int dyc = -1; dyc <= 1; ++dyc) {
+            const int yy = cy0 + dyc;
+            if (yy < 0 || yy >= a.ncy) continue;
+            for (int dxc = -1; dxc <= 1; ++dxc) {
+                int xx = cx0 + dxc;
+                if (a.Lx > 0.0) { if (xx < 0) xx += a.ncx; if (xx >= a.ncx) xx -= a.ncx; }
+                else if (xx < 0 || xx >= a.ncx) continue;
+                const int cell = yy * a.ncx + xx;
+                int m = a.cnt[cell];
+                if (m > CELL_MAXC) m = CELL_MAXC;
+                const size_t base = static_cast<size_t>(cell) * CELL_MAXC;
+                for (int t = 0; t < m; ++t) {
+                    if (a.own[base + t] == cap) continue;
+                    const double4 b = a.pay[base + t];
+                    pair_force(a, px, py, vx, vy, b.x, b.y, b.z, b.w, Fx, Fy);
+                    fx += Fx; fy += Fy;
+                }
+            }
+        }
+        pair_force(a, px, py, vx, vy, px, a.ybot, 0.0, 0.0, Fx, Fy);
+        fx += Fx; fy += Fy;
+        pair_force(a, px, py, vx, vy, px, a.ytop, 0.0, 0.0, Fx, Fy);
+        fx += Fx; fy += Fy;
+    }
+    a.fx[k] = fx;
+    a.fy[k] = fy;
+    if (a.dmin) {
+        float dm = sdmin[g][0];
+        for (int u = 1; u < 9; ++u) dm = fminf(dm, sdmin[g][u]);
+        a.dmin[k] = dm;
+    }
 }
 
 // x extent of the cell grid when x is not periodic (cell_list.h:50-66):
@@ -542,7 +627,8 @@ void launch_repulsion(Ctx& c) {
     a.cnt = cnt;
     a.pay = c.cell_pay;
     a.own = c.cell_own;
-    hipLaunchKernelGGL((k_repulsion<true>), dim3(nb), dim3(bs), 0, c.stream, a);
+    hipLaunchKernelGGL(k_repulsion_lanes, dim3((c.K + REP_NODES - 1) / REP_NODES), dim3(256), 0,
+                       c.stream, a);
 }
 
 }  // namespace sf
