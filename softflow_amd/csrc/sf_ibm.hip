@@ -43,12 +43,12 @@ __device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, 
 // step -- except at iteration force_it (0: none), which is always applied;
 // otherwise the correction is counted into the node's step total, and into
 // its head total while it <= MDF_HEAD.
-__global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,
+__device__ __forceinline__ void node_prep_body(const double* x, const double* y, const double* fx, const double* fy,
                             const int* off, const int* owner, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, NodeSpread* tile_rec,
                             int* counters, int mode, MdfDev* md, double tol, int it, int force_it,
-                            double* total, double* head) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+                            double* total, double* head, int bx) {
+    const int k = bx * blockDim.x + threadIdx.x;
     if (mode == 0 && k == 0) counters[3] = 0;     // the touched-tile list is rebuilt after binning
     if (k >= K) return;
     if (mode == 1 && md) {
@@ -127,6 +127,14 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
         }
 }
 
+__global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,
+                            const int* off, const int* owner, NodeSpread* rec, int K, int nx, int ny,
+                            int ntx, int nty, int* tile_cnt, int* tile_list, NodeSpread* tile_rec,
+                            int* counters, int mode, MdfDev* md, double tol, int it, int force_it,
+                            double* total, double* head) {
+    node_prep_body(x, y, fx, fy, off, owner, rec, K, nx, ny, ntx, nty, tile_cnt, tile_list, tile_rec, counters, mode, md, tol, it, force_it, total, head, blockIdx.x);
+}
+
 // Gather: one 256-thread workgroup per TILE_X x TILE_Y tile.
 // mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction).
 // Tiles no node touches are skipped when the output buffer already holds zeros
@@ -152,15 +160,15 @@ __device__ __forceinline__ int stencil_column(int cx, int lo, int nx) {
 // once per step after the binning): appended to act as (tile, node count),
 // one atomic per wavefront.
 template <bool FP32>
-__global__ __launch_bounds__(256) void k_spread_rest(const int* tile_cnt, const void* Fin, void* Fout,
+__device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void* Fin, void* Fout,
                                                      const uint8_t* dirty_in, uint8_t* dirty_out,
                                                      int2* act, int* counters, int nx, int ny,
                                                      int ntx, int ntiles, int mode,
-                                                     const MdfDev* md) {
+                                                     const MdfDev* md, int bx) {
     using T = typename std::conditional<FP32, float, double>::type;
     const bool skip = mode == 1 && md && md->conv;
     const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tile = bx * blockDim.x + threadIdx.x;
     const int cnt = tile < ntiles ? tile_cnt[tile] : 0;
     bool need = false, copy = false;
     if (tile < ntiles && (skip || cnt == 0)) {
@@ -199,17 +207,26 @@ __global__ __launch_bounds__(256) void k_spread_rest(const int* tile_cnt, const 
     }
 }
 
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_spread_rest(const int* tile_cnt, const void* Fin, void* Fout,
+                                                     const uint8_t* dirty_in, uint8_t* dirty_out,
+                                                     int2* act, int* counters, int nx, int ny,
+                                                     int ntx, int ntiles, int mode,
+                                                     const MdfDev* md) {
+    spread_rest_body<FP32>(tile_cnt, Fin, Fout, dirty_in, dirty_out, act, counters, nx, ny, ntx, ntiles, mode, md, blockIdx.x);
+}
+
 // Touched tiles: a persistent grid walks the list of tiles the node binning
 // touched (act[0 .. counters[3])), one 256-thread workgroup per tile and one
 // thread per cell of its TILE_X x TILE_Y cells.
 // mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction).
 template <bool FP32>
-__global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const NodeSpread* tile_rec,
+__device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpread* tile_rec,
                                                 int* tile_cnt, const int* tile_list, const int2* act,
                                                 const int* counters, const double* lut,
                                                 const void* Fin, void* Fout, uint8_t* dirty_out,
                                                 int nx, int ny, int ntx, int ntiles, int mode,
-                                                int last, const MdfDev* md) {
+                                                int last, const MdfDev* md, int bx, int gx) {
     __shared__ double tab[1025];
     __shared__ double spx[STAGE_MAX], spy[STAGE_MAX], sfx[STAGE_MAX], sfy[STAGE_MAX], sds[STAGE_MAX];
     __shared__ int si0[STAGE_MAX], sj0[STAGE_MAX], skey[STAGE_MAX];
@@ -224,12 +241,12 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const Nod
     if (skip) {
         // converged MDF: nothing to add (the rest kernel copied F_in)
         if (last)
-            for (int i = blockIdx.x * blockDim.x + t; i < nact; i += gridDim.x * blockDim.x) tile_cnt[act[i].x] = 0;
+            for (int i = bx * blockDim.x + t; i < nact; i += gx * blockDim.x) tile_cnt[act[i].x] = 0;
         return;
     }
     if (nact > 0)
         for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
-    for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
+    for (int ai = bx; ai < nact; ai += gx) {
         const int2 te = act[ai];
         const int tile = te.x;
         __syncthreads();          // the previous tile's LDS use is over
@@ -387,6 +404,16 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const Nod
             static_cast<T*>(Fout)[2 * n + 1] = static_cast<T>(ay);
         }
     }
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const NodeSpread* tile_rec,
+                                                int* tile_cnt, const int* tile_list, const int2* act,
+                                                const int* counters, const double* lut,
+                                                const void* Fin, void* Fout, uint8_t* dirty_out,
+                                                int nx, int ny, int ntx, int ntiles, int mode,
+                                                int last, const MdfDev* md) {
+    spread_body<FP32>(rec, tile_rec, tile_cnt, tile_list, act, counters, lut, Fin, Fout, dirty_out, nx, ny, ntx, ntiles, mode, last, md, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -695,12 +722,12 @@ __device__ __forceinline__ void move_node(const MoveArgs& m, double& px, double&
 }
 
 template <bool FP32>
-__global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut, double* x,
-                                                double* y, double* vx, double* vy, int K, MoveArgs mv) {
+__device__ __forceinline__ void interp_body(CellVelArgs a, const double* lut, double* x,
+                                                double* y, double* vx, double* vy, int K, MoveArgs mv, int bx) {
     __shared__ double tx[256], ty[256];
     __shared__ int use[256];
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    const int k = blockIdx.x * INTERP_NODES + g;
+    const int k = bx * INTERP_NODES + g;
     const bool live = k < K;
     double px = live ? x[k] : 0.0, py = live ? y[k] : 0.0;
     double Ux, Uy;
@@ -714,6 +741,12 @@ __global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut
             y[k] = py;
         }
     }
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut, double* x,
+                                                double* y, double* vx, double* vy, int K, MoveArgs mv) {
+    interp_body<FP32>(a, lut, x, y, vx, vy, K, mv, blockIdx.x);
 }
 
 // moveAllNodes + y-clamp (capsule.cpp:267-297, simulation.cpp:665-675); one

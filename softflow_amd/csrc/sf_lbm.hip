@@ -257,8 +257,7 @@ struct LbmArgs {
     int nbx, nblk, per_xcd, remap, persist;
 };
 
-__device__ __forceinline__ bool block_origin(const LbmArgs& a, int& x, int& y) {
-    const int b = blockIdx.x;
+__device__ __forceinline__ bool block_origin(const LbmArgs& a, int b, int& x, int& y) {
     const int t = a.remap ? (b % 8) * a.per_xcd + b / 8 : b;
     if (t >= a.nblk) return false;
     x = (t % a.nbx) * blockDim.x + threadIdx.x;
@@ -463,7 +462,7 @@ __global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
     int capped = 0;
     if (a.persist == 0) {
         int x, y;
-        if (!block_origin(a, x, y)) return;
+        if (!block_origin(a, blockIdx.x, x, y)) return;
         capped = lbm_cell<OP, FP32, FLAGS, ADRF>(a, x, y);
     } else {
         const int b = blockIdx.x, xcd = b % 8, l = b / 8;
@@ -609,7 +608,7 @@ __device__ __forceinline__ void cell_macro(const void* fv, size_t N, int n, cons
 template <bool FP32, bool FLAGS>
 __global__ __launch_bounds__(256) void k_adr(LbmArgs a, const void* f) {
     int x, y;
-    if (!block_origin(a, x, y)) return;
+    if (!block_origin(a, blockIdx.x, x, y)) return;
     if (x >= a.nx || y >= a.ny) return;
     const int n = y * a.nx + x;
     if (is_solid(a, x, y, FLAGS)) return;

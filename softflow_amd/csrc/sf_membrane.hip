@@ -37,12 +37,12 @@ __device__ __forceinline__ void mimg(double ax, double ay, double bx, double by,
 }
 
 template <int MAXN>
-__global__ __launch_bounds__(64) void k_membrane(MemArgs a) {
+__device__ __forceinline__ void membrane_body(MemArgs a, int bx) {
     __shared__ double X[MAXN], Y[MAXN], FX[MAXN], FY[MAXN];
     __shared__ double Ex[MAXN], Ey[MAXN], Gx[MAXN], Gy[MAXN], Cc[MAXN], Ln[MAXN];
     __shared__ int skip[MAXN];
     __shared__ double red[2];
-    const int cap = blockIdx.x;
+    const int cap = bx;
     const int lane = threadIdx.x;
     if (cap == 0 && lane == 0) a.counters[0] = 0;
     const int o = a.off[cap], n = a.off[cap + 1] - o;
@@ -275,6 +275,11 @@ __global__ __launch_bounds__(64) void k_membrane(MemArgs a) {
     }
 }
 
+template <int MAXN>
+__global__ __launch_bounds__(64) void k_membrane(MemArgs a) {
+    membrane_body<MAXN>(a, blockIdx.x);
+}
+
 void launch_membrane(Ctx& c) {
     if (c.ncaps == 0) return;
     MemArgs a{};
@@ -372,8 +377,8 @@ __device__ __forceinline__ int cell_index(const RepArgs& a, double px0, double p
     return iy * a.ncx + ix;
 }
 
-__global__ void k_cells_fill(RepArgs a, int* cnt, int* list, int* counters) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cells_fill_body(RepArgs a, int* cnt, int* list, int* counters, int bx) {
+    const int k = bx * blockDim.x + threadIdx.x;
     if (k >= a.K) return;
     const int ci = cell_index(a, a.x[k], a.y[k]);
     const int slot = atomicAdd(&cnt[ci], 1);
@@ -381,13 +386,17 @@ __global__ void k_cells_fill(RepArgs a, int* cnt, int* list, int* counters) {
     else atomicOr(&counters[2], 1);
 }
 
+__global__ void k_cells_fill(RepArgs a, int* cnt, int* list, int* counters) {
+    cells_fill_body(a, cnt, list, counters, blockIdx.x);
+}
+
 // Sort each cell's list by node index (the reference's insertion order), lay
 // the listed nodes' data out in that order, and clear the other step's count
 // array for the next step.
-__global__ void k_cells_sort(int ncells, int nclear, const int* cnt, int* list, int* other,
+__device__ __forceinline__ void cells_sort_body(int ncells, int nclear, const int* cnt, int* list, int* other,
                              const double* x, const double* y, const double* vx, const double* vy,
-                             const int* owner, double4* pay, int* own) {
-    const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+                             const int* owner, double4* pay, int* own, int bx) {
+    const int ci = bx * blockDim.x + threadIdx.x;
     if (ci < nclear) other[ci] = 0;
     if (ci >= ncells) return;
     int m = cnt[ci];
@@ -407,9 +416,15 @@ __global__ void k_cells_sort(int ncells, int nclear, const int* cnt, int* list, 
     }
 }
 
+__global__ void k_cells_sort(int ncells, int nclear, const int* cnt, int* list, int* other,
+                             const double* x, const double* y, const double* vx, const double* vy,
+                             const int* owner, double4* pay, int* own) {
+    cells_sort_body(ncells, nclear, cnt, list, other, x, y, vx, vy, owner, pay, own, blockIdx.x);
+}
+
 template <bool CELLS>
-__global__ void k_repulsion(RepArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void repulsion_body(RepArgs a, int bx) {
+    const int k = bx * blockDim.x + threadIdx.x;
     if (k >= a.K) return;
     const int cap = a.owner[k];
     const double px = a.x[k], py = a.y[k], vx = a.vx[k], vy = a.vy[k];
@@ -458,6 +473,11 @@ __global__ void k_repulsion(RepArgs a) {
     if (a.dmin) a.dmin[k] = static_cast<float>(sqrt(d2min));
 }
 
+template <bool CELLS>
+__global__ void k_repulsion(RepArgs a) {
+    repulsion_body<CELLS>(a, blockIdx.x);
+}
+
 // Cell-list contact with 16 lanes per node: lane u < 9 scans neighbour cell u
 // (the reference's dy-outer, dx-inner order), lanes 9 and 10 take the two
 // wall images, all in parallel; every non-zero pair force is kept in LDS with
@@ -468,12 +488,12 @@ __global__ void k_repulsion(RepArgs a) {
 constexpr int REP_NODES = 16;      // nodes per 256-thread workgroup
 constexpr int REP_SLOTS = 6;       // non-zero pair forces a lane keeps
 
-__global__ __launch_bounds__(256) void k_repulsion_lanes(RepArgs a) {
+__device__ __forceinline__ void repulsion_lanes_body(RepArgs a, int bx) {
     __shared__ double sfx[REP_NODES][11][REP_SLOTS], sfy[REP_NODES][11][REP_SLOTS];
     __shared__ int scnt[REP_NODES][16];
     __shared__ float sdmin[REP_NODES][16];
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    const int k = blockIdx.x * REP_NODES + g;
+    const int k = bx * REP_NODES + g;
     const bool live = k < a.K;
     int cap = 0;
     double px = 0.0, py = 0.0, vx = 0.0, vy = 0.0;
@@ -556,6 +576,10 @@ __global__ __launch_bounds__(256) void k_repulsion_lanes(RepArgs a) {
         for (int u = 1; u < 9; ++u) dm = fminf(dm, sdmin[g][u]);
         a.dmin[k] = dm;
     }
+}
+
+__global__ __launch_bounds__(256) void k_repulsion_lanes(RepArgs a) {
+    repulsion_lanes_body(a, blockIdx.x);
 }
 
 // x extent of the cell grid when x is not periodic (cell_list.h:50-66):
