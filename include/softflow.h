@@ -203,6 +203,33 @@ int  sf_mlp_train(sf_mlp* net, sf_ctx* ctx, int epochs, int batch, float lr, uin
  * corrections follow as needed.  net is borrowed and must outlive the use. */
 int  sf_mdf_set_model(sf_ctx* ctx, sf_mlp* net, int warm);
 
+/* Batched ensembles (no counterpart in the reference, whose members would each
+ * be one Simulation stepped on its own, simulation.cpp:470-787).  A batch
+ * borrows n contexts on one device and advances all of them with one launch
+ * per kernel of the step, each launch covering the workgroups of every member;
+ * six steps (the period of the buffer rotation) are captured as one hipGraph.
+ * Every member's results are bit-identical to sf_step on that context alone.
+ * Members must share precision and collision model and be x-periodic plain
+ * channels (no obstacle flags) with deformable capsules, one IBM pass per step
+ * and no scalar.  The contexts must outlive the batch; calls on a member
+ * between batch steps are allowed (the batch re-plans when a member's buffers
+ * or roles changed). */
+typedef struct sf_batch sf_batch;
+int  sf_batch_create(sf_batch** out, sf_ctx* const* members, int n);
+int  sf_batch_destroy(sf_batch* b);
+int  sf_batch_step(sf_batch* b, int nsteps);
+int  sf_batch_synchronize(sf_batch* b);
+/* 0: launch every step directly; 1 (default): replay captured six-step graphs. */
+int  sf_batch_set_graph_mode(sf_batch* b, int on);
+/* Autotuner: times the batched collide-stream over workgroup shapes and
+ * linear or XCD-banded orders of the members' workgroups (repeated launches
+ * leave the state unchanged) and keeps the fastest; report may be NULL. */
+int  sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len);
+int  sf_batch_set_lbm_launch(sf_batch* b, int bx, int by, int xcd_banded);
+/* out[0] members, out[1] lattice cells, out[2] membrane nodes, out[3] workgroups
+ * of the batched collide-stream launch. */
+int  sf_batch_info(sf_batch* b, int64_t* out4);
+
 /* Device math probe used by the tests: op 0 sqrt, 1 divide (a/b), 2 atan2(a,b),
  * 3 pow(a, b) with integer b as used by the contact force. */
 int  sf_probe_math(int op, int n, const double* a, const double* b, double* out);

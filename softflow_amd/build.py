@@ -19,7 +19,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 HIP_LIB = os.path.join(LIBDIR, "libsoftflow_hip.so")
-HIP_SRCS = ["sf_api.hip", "sf_lbm.hip", "sf_ibm.hip", "sf_membrane.hip", "sf_mlp.hip"]
+HIP_SRCS = ["sf_api.hip", "sf_lbm.hip", "sf_ibm.hip", "sf_membrane.hip", "sf_mlp.hip", "sf_batch.hip"]
 ARCH = os.environ.get("SOFTFLOW_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -31,6 +31,8 @@ EXPORTS = (
     "sf_mdf_set_control", "sf_mdf_stats", "sf_mdf_collect", "sf_mdf_samples", "sf_mdf_set_model",
     "sf_mlp_create", "sf_mlp_destroy", "sf_mlp_num_params", "sf_mlp_get_params", "sf_mlp_set_params",
     "sf_mlp_predict", "sf_mlp_loss_grad", "sf_mlp_train",
+    "sf_batch_create", "sf_batch_destroy", "sf_batch_step", "sf_batch_synchronize",
+    "sf_batch_set_graph_mode", "sf_batch_autotune", "sf_batch_set_lbm_launch", "sf_batch_info",
 )
 
 
@@ -118,6 +120,14 @@ def load():
         "sf_mlp_predict": [C.c_void_p, C.c_int, _FP, _FP],
         "sf_mlp_loss_grad": [C.c_void_p, C.c_int, _FP, _FP, _DP, _FP],
         "sf_mlp_train": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint64, _DP],
+        "sf_batch_create": [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int],
+        "sf_batch_destroy": [C.c_void_p],
+        "sf_batch_step": [C.c_void_p, C.c_int],
+        "sf_batch_synchronize": [C.c_void_p],
+        "sf_batch_set_graph_mode": [C.c_void_p, C.c_int],
+        "sf_batch_autotune": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],
+        "sf_batch_set_lbm_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int],
+        "sf_batch_info": [C.c_void_p, C.POINTER(C.c_int64)],
     }
     for name, args in proto.items():
         fn = getattr(L, name)
@@ -291,6 +301,52 @@ class Context:
         n = C.c_int(0)
         _check(self.L.sf_kernel_times(self.h, 32, names, _dp(ms), C.byref(n)), "sf_kernel_times")
         return {names[i].decode(): float(ms[i]) for i in range(min(n.value, 32))}
+
+
+class Batch:
+    """A batched ensemble of Contexts (sf_batch_*); the contexts must outlive it."""
+
+    def __init__(self, contexts):
+        self.L = load()
+        self.members = list(contexts)
+        arr = (C.c_void_p * len(self.members))(*[c.h.value for c in self.members])
+        h = C.c_void_p()
+        _check(self.L.sf_batch_create(C.byref(h), arr, len(self.members)), "sf_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        _check(self.L.sf_batch_step(self.h, n), "sf_batch_step")
+
+    def synchronize(self):
+        _check(self.L.sf_batch_synchronize(self.h), "sf_batch_synchronize")
+
+    def set_graph_mode(self, on: bool):
+        _check(self.L.sf_batch_set_graph_mode(self.h, 1 if on else 0), "sf_batch_set_graph_mode")
+
+    def set_lbm_launch(self, bx, by, xcd_banded=True):
+        _check(self.L.sf_batch_set_lbm_launch(self.h, bx, by, 1 if xcd_banded else 0),
+               "sf_batch_set_lbm_launch")
+
+    def autotune(self, reps=10):
+        buf = C.create_string_buffer(4096)
+        _check(self.L.sf_batch_autotune(self.h, reps, buf, 4096), "sf_batch_autotune")
+        return buf.value.decode()
+
+    def info(self):
+        out = (C.c_int64 * 4)()
+        _check(self.L.sf_batch_info(self.h, out), "sf_batch_info")
+        return list(out)
 
 
 def probe_math(op: int, a, b=None):

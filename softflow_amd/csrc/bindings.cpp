@@ -370,6 +370,16 @@ PYBIND11_MODULE(softflow_core, m) {
         .def("setMdfModel", &Simulation::setMdfModel, py::arg("model"), py::arg("warm") = true,
              py::keep_alive<1, 2>());
 
+    py::class_<Ensemble>(m, "Ensemble")
+        .def(py::init<const std::vector<Simulation*>&>(), py::arg("members"), py::keep_alive<1, 2>())
+        .def("run", &Ensemble::run, py::arg("num_steps"), py::call_guard<py::gil_scoped_release>())
+        .def("synchronize", &Ensemble::synchronize)
+        .def("autotune", &Ensemble::autotune, py::arg("reps") = 10)
+        .def("setLbmLaunch", &Ensemble::setLbmLaunch, py::arg("bx"), py::arg("by"), py::arg("xcd_banded") = true)
+        .def("setGraphMode", &Ensemble::setGraphMode, py::arg("on"))
+        .def("info", &Ensemble::info)
+        .def("__len__", &Ensemble::size);
+
     py::class_<MdfModel>(m, "MdfModel")
         .def(py::init<int, uint64_t>(), py::arg("device") = 0, py::arg("seed") = 1)
         .def("train", &MdfModel::train, py::arg("sim"), py::arg("epochs") = 10, py::arg("batch") = 4096,

@@ -576,6 +576,13 @@ void Simulation::step() {
     step_count_++;
 }
 
+void Simulation::markAdvanced(int steps) {
+    field_->stale_ = true;
+    caps_.stale_ = true;
+    if (adr_) adr_->stale_ = true;
+    step_count_ += steps;
+}
+
 void Simulation::run(int num_steps) {
     if (cb_) {
         for (int s = 0; s < num_steps; ++s) step();
@@ -797,6 +804,54 @@ std::vector<std::pair<std::string, double>> Simulation::kernelTimes() {
     int n = 0;
     check(sf_kernel_times(ctx_, 32, names, ms, &n), "sf_kernel_times");
     for (int i = 0; i < std::min(n, 32); ++i) out.emplace_back(names[i], ms[i]);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble
+
+Ensemble::Ensemble(const std::vector<Simulation*>& members) : members_(members) {
+    if (members_.empty()) throw std::runtime_error("Ensemble: no members");
+    std::vector<sf_ctx*> ctx;
+    for (Simulation* s : members_) {
+        if (!s || !s->initialized()) throw std::runtime_error("Ensemble: members must be initialised simulations");
+        ctx.push_back(s->deviceContext());
+    }
+    check(sf_batch_create(&batch_, ctx.data(), static_cast<int>(ctx.size())), "sf_batch_create");
+}
+
+Ensemble::~Ensemble() {
+    if (batch_) sf_batch_destroy(batch_);
+}
+
+void Ensemble::check(int rc, const char* what) const {
+    if (rc != SF_OK) throw std::runtime_error(std::string(what) + ": " + sf_last_error());
+}
+
+void Ensemble::run(int num_steps) {
+    for (Simulation* s : members_) s->pushDirty();
+    check(sf_batch_step(batch_, num_steps), "sf_batch_step");
+    for (Simulation* s : members_) s->markAdvanced(num_steps);
+}
+
+void Ensemble::synchronize() { check(sf_batch_synchronize(batch_), "sf_batch_synchronize"); }
+
+std::string Ensemble::autotune(int reps) {
+    for (Simulation* s : members_) s->pushDirty();
+    std::vector<char> buf(4096, 0);
+    check(sf_batch_autotune(batch_, reps, buf.data(), static_cast<int>(buf.size())), "sf_batch_autotune");
+    return std::string(buf.data());
+}
+
+void Ensemble::setLbmLaunch(int bx, int by, bool xcd_banded) {
+    check(sf_batch_set_lbm_launch(batch_, bx, by, xcd_banded ? 1 : 0), "sf_batch_set_lbm_launch");
+}
+
+void Ensemble::setGraphMode(bool on) { check(sf_batch_set_graph_mode(batch_, on ? 1 : 0), "sf_batch_set_graph_mode"); }
+
+std::vector<int64_t> Ensemble::info() {
+    std::vector<int64_t> out(4, 0);
+    check(sf_batch_info(batch_, out.data()), "sf_batch_info");
     return out;
 }
 

@@ -353,7 +353,9 @@ public:
     void pullScalar(AdvectionDiffusion& a);
 
 private:
+    friend class Ensemble;
     void pushDirty();
+    void markAdvanced(int steps);   // device state moved on: host mirrors are stale
     void check(int rc, const char* what) const;
     SimulationParams params_;
     std::unique_ptr<LatticeField> field_;
@@ -370,6 +372,34 @@ private:
     int64_t mdf_collect_ = 0;
     MdfModel* mdf_model_ = nullptr;
     bool mdf_warm_ = false;
+};
+
+// A batched ensemble (no counterpart in the reference): independent
+// simulations of uneven size advanced together, one launch per kernel of the
+// step over all members, six steps per captured graph.  Each member ends
+// bit-identical to running it alone.  The simulations must be initialised
+// x-periodic channels with deformable capsules, one IBM pass and no scalar,
+// sharing precision and collision model; they stay usable on their own.
+class Ensemble {
+public:
+    explicit Ensemble(const std::vector<Simulation*>& members);
+    ~Ensemble();
+    Ensemble(const Ensemble&) = delete;
+    Ensemble& operator=(const Ensemble&) = delete;
+
+    void run(int num_steps);
+    void synchronize();
+    std::string autotune(int reps = 10);
+    void setLbmLaunch(int bx, int by, bool xcd_banded);
+    void setGraphMode(bool on);
+    // members, lattice cells, membrane nodes, collide-stream workgroups
+    std::vector<int64_t> info();
+    int size() const { return static_cast<int>(members_.size()); }
+
+private:
+    void check(int rc, const char* what) const;
+    std::vector<Simulation*> members_;
+    sf_batch* batch_ = nullptr;
 };
 
 }  // namespace softflow

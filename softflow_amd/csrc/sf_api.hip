@@ -1,4 +1,5 @@
 // C-ABI entry points, device state management and the per-step schedule.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -197,6 +198,17 @@ static void one_step(Ctx& c) {
     c.step_count++;
 }
 
+void ctx_one_step(Ctx& c) { one_step(c); }
+
+static void invalidate_graph(Ctx& c) {
+    c.gen++;
+    if (c.gexec) hipGraphExecDestroy(c.gexec);
+    if (c.graph) hipGraphDestroy(c.graph);
+    c.gexec = nullptr;
+    c.graph = nullptr;
+    c.graph_valid = false;
+}
+
 // Graph replay: the step's buffer rotation has period <= 6 (two population
 // buffers, a permutation of three force buffers), so six steps are captured
 // as one graph whose end state maps buffers back onto the start state.
@@ -210,6 +222,10 @@ static int run_graph(Ctx& c, int nsteps) {
             continue;
         }
         const int f0 = c.fcur, g0 = c.gcur, a0 = c.Fcur, b0 = c.Fprev, p0 = c.Fpart;
+        const int key[5] = {f0, g0, a0, b0, c.cell_cur};
+        // a graph bakes in the buffer roles it was captured with: a remainder
+        // step run eagerly since then leaves them rotated, so capture again
+        if (c.graph_valid && !std::equal(key, key + 5, c.graph_key)) invalidate_graph(c);
         if (!c.graph_valid) {
             hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
             for (int s = 0; s < period; ++s) one_step(c);
@@ -219,6 +235,7 @@ static int run_graph(Ctx& c, int nsteps) {
             if (c.fcur != f0 || c.gcur != g0 || c.Fcur != a0 || c.Fprev != b0)
                 return fail(SF_E_STATE, "step buffer rotation did not close over the graph period");
             SF_HIP(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
+            std::copy(key, key + 5, c.graph_key);
             c.graph_valid = true;
         }
         SF_HIP(hipGraphLaunch(c.gexec, c.stream));
@@ -228,14 +245,6 @@ static int run_graph(Ctx& c, int nsteps) {
         c.c_from_store = false;
     }
     return SF_OK;
-}
-
-static void invalidate_graph(Ctx& c) {
-    if (c.gexec) hipGraphExecDestroy(c.gexec);
-    if (c.graph) hipGraphDestroy(c.graph);
-    c.gexec = nullptr;
-    c.graph = nullptr;
-    c.graph_valid = false;
 }
 
 static int check_counters(Ctx& c) {

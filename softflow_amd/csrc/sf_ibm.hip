@@ -925,4 +925,141 @@ void launch_interp_move(Ctx& c) {
                            1.5, static_cast<double>(c.ny) - 1.5);
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched launches over the members of an ensemble (map[b] = member and its
+// own workgroup index), for the steady step: node forces spread once, no MDF.
+struct PrepPack {
+    const double *x, *y, *fx, *fy;
+    const int *off, *owner;
+    NodeSpread* rec;
+    int K, nx, ny, ntx, nty;
+    int* tile_cnt;
+    int* tile_list;
+    NodeSpread* tile_rec;
+    int* counters;
+};
+struct RestPack {
+    const int* tile_cnt;
+    const void* Fin;
+    void* Fout;
+    const uint8_t* din;
+    uint8_t* dout;
+    int2* act;
+    int* counters;
+    int nx, ny, ntx, ntiles;
+};
+struct SpreadPack {
+    const NodeSpread* rec;
+    const NodeSpread* tile_rec;
+    int* tile_cnt;
+    const int* tile_list;
+    const int2* act;
+    const int* counters;
+    const double* lut;
+    const void* Fin;
+    void* Fout;
+    uint8_t* dout;
+    int nx, ny, ntx, ntiles, gx;
+};
+struct InterpPack {
+    CellVelArgs a;
+    MoveArgs mv;
+    const double* lut;
+    double *x, *y, *vx, *vy;
+    int K;
+};
+
+__global__ __launch_bounds__(256) void kb_node_prep(const PrepPack* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const PrepPack p = packs[w.x];
+    node_prep_body(p.x, p.y, p.fx, p.fy, p.off, p.owner, p.rec, p.K, p.nx, p.ny, p.ntx, p.nty,
+                   p.tile_cnt, p.tile_list, p.tile_rec, p.counters, 0, nullptr, 0.0, 0, 0, nullptr,
+                   nullptr, w.y);
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(256) void kb_spread_rest(const RestPack* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const RestPack p = packs[w.x];
+    spread_rest_body<FP32>(p.tile_cnt, p.Fin, p.Fout, p.din, p.dout, p.act, p.counters, p.nx, p.ny,
+                           p.ntx, p.ntiles, 0, nullptr, w.y);
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(256) void kb_spread(const SpreadPack* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const SpreadPack p = packs[w.x];
+    spread_body<FP32>(p.rec, p.tile_rec, p.tile_cnt, p.tile_list, p.act, p.counters, p.lut, p.Fin,
+                      p.Fout, p.dout, p.nx, p.ny, p.ntx, p.ntiles, 0, 1, nullptr, w.y, p.gx);
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(256) void kb_interp(const InterpPack* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const InterpPack p = packs[w.x];
+    interp_body<FP32>(p.a, p.lut, p.x, p.y, p.vx, p.vy, p.K, p.mv, w.y);
+}
+
+// Workgroups of one member's spread gather in a batch: each walks the touched
+// tiles with this stride; the rest exit at once (touched tiles are far fewer
+// than tiles).
+constexpr int BATCH_SPREAD_GRID = 64;
+
+// The arguments of launch_spread(c, 0, true) (no capsules: not batched).
+void batch_items_spread(Ctx& c, BatchItem* it) {
+    batch_put(it[0], (c.K + 255) / 256,
+              PrepPack{c.x, c.y, c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
+                       c.tile_cnt, c.tile_list, c.tile_rec, c.counters});
+    const void* Fin = c.F[c.Fcur];
+    void* Fout = c.F[c.Fcur];
+    const uint8_t* din = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
+    uint8_t* dout = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
+    batch_put(it[1], (c.ntiles + 255) / 256,
+              RestPack{c.tile_cnt, Fin, Fout, din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx,
+                       c.ntiles});
+    const int gx = std::min(c.ntiles, BATCH_SPREAD_GRID);
+    batch_put(it[2], gx,
+              SpreadPack{c.spread, c.tile_rec, c.tile_cnt, c.tile_list, c.tile_act, c.counters, c.phi,
+                         Fin, Fout, dout, c.nx, c.ny, c.ntx, c.ntiles, gx});
+}
+
+// The arguments of launch_interp_move (deformable capsules only).
+void batch_items_interp(Ctx& c, BatchItem* it) {
+    const CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
+    const MoveArgs mv{1, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1, 1.5,
+                      static_cast<double>(c.ny) - 1.5};
+    batch_put(it[0], (c.K + INTERP_NODES - 1) / INTERP_NODES,
+              InterpPack{a, mv, c.phi, c.x, c.y, c.vx, c.vy, c.K});
+}
+
+void batch_launch_ibm(int kind, const BatchVariant& v, const void* packs, const int2* map,
+                      int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return;
+    const dim3 g(nblocks), b(256);
+    switch (kind) {
+    case BK_NODE_PREP:
+        hipLaunchKernelGGL(kb_node_prep, g, b, 0, s, static_cast<const PrepPack*>(packs), map);
+        break;
+    case BK_SPREAD_REST:
+        if (v.fp32) hipLaunchKernelGGL(kb_spread_rest<true>, g, b, 0, s, static_cast<const RestPack*>(packs), map);
+        else hipLaunchKernelGGL(kb_spread_rest<false>, g, b, 0, s, static_cast<const RestPack*>(packs), map);
+        break;
+    case BK_SPREAD:
+        if (v.fp32) hipLaunchKernelGGL(kb_spread<true>, g, b, 0, s, static_cast<const SpreadPack*>(packs), map);
+        else hipLaunchKernelGGL(kb_spread<false>, g, b, 0, s, static_cast<const SpreadPack*>(packs), map);
+        break;
+    case BK_INTERP:
+        if (v.fp32) hipLaunchKernelGGL(kb_interp<true>, g, b, 0, s, static_cast<const InterpPack*>(packs), map);
+        else hipLaunchKernelGGL(kb_interp<false>, g, b, 0, s, static_cast<const InterpPack*>(packs), map);
+        break;
+    default:
+        break;
+    }
+}
+
 }  // namespace sf

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -185,6 +186,8 @@ struct Ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_shift = -1;
+    int graph_key[5] = {};               // buffer roles at capture (the graph replays only from them)
+    int64_t gen = 0;                     // bumped whenever buffers, parameters or launch shapes change
 
     // timing
     bool timing = false;
@@ -227,6 +230,62 @@ void launch_interp_move(Ctx& c);
 void launch_fill_int(Ctx& c, int* p, int n, int v);
 void launch_fill_zero(Ctx& c, void* p, size_t bytes);
 void probe_math(int op, int n, const double* a, const double* b, double* out);
+// one step of a single context, launched on its own stream (simulation.cpp:470-787 order)
+void ctx_one_step(Ctx& c);
+
+// -- batched ensembles ------------------------------------------------------
+// One batched launch runs the workgroups of every member of an ensemble in a
+// single grid: map[b] = (member, workgroup index within that member's own
+// launch), member -1 for padding workgroups.  Each member contributes its
+// single-context kernel arguments (the "pack") at its current buffer roles.
+enum BatchKernel : int {
+    BK_MEMBRANE = 0,
+    BK_CELLS_FILL,
+    BK_CELLS_SORT,
+    BK_CONTACT,
+    BK_NODE_PREP,
+    BK_SPREAD_REST,
+    BK_SPREAD,
+    BK_LBM,
+    BK_INTERP,
+    BK_COUNT
+};
+
+struct BatchItem {
+    int blocks = 0;
+    std::vector<unsigned char> pack;
+};
+
+// Template choices shared by all members of a batch.
+struct BatchVariant {
+    int fp32 = 1;
+    int op = SF_BGK;
+    int maxn = 64;          // membrane LDS capacity (64 or 256 nodes)
+    int bx = 64, by = 4;    // collide-stream workgroup shape
+};
+
+template <typename P>
+inline void batch_put(BatchItem& it, int blocks, const P& p) {
+    it.blocks = blocks;
+    it.pack.resize(sizeof(P));
+    std::memcpy(it.pack.data(), &p, sizeof(P));
+}
+
+// Items of one steady step (previous-step shift force, one IBM pass, no
+// scalar), in launch order.  Each call advances the member's buffer roles
+// exactly as the single-context launcher does.
+void batch_items_membrane(Ctx& c, BatchItem* it);          // BK_MEMBRANE
+void batch_items_contact(Ctx& c, BatchItem* it);           // BK_CELLS_FILL .. BK_CONTACT
+void batch_items_spread(Ctx& c, BatchItem* it);            // BK_NODE_PREP .. BK_SPREAD
+void batch_items_lbm(Ctx& c, int bx, int by, BatchItem* it);   // BK_LBM
+void batch_items_interp(Ctx& c, BatchItem* it);            // BK_INTERP
+// launch kind over nblocks workgroups of the map
+void batch_launch_membrane(int kind, const BatchVariant& v, const void* packs, const int2* map,
+                           int nblocks, hipStream_t s);
+void batch_launch_ibm(int kind, const BatchVariant& v, const void* packs, const int2* map,
+                      int nblocks, hipStream_t s);
+void batch_launch_lbm(const BatchVariant& v, const void* packs, const int2* map, int nblocks,
+                      hipStream_t s);
 
 }  // namespace sf
 

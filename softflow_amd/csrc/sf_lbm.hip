@@ -514,7 +514,7 @@ static void scalar_args(Ctx& c, LbmArgs& a) {
     }
 }
 
-void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap, int persist) {
+static LbmArgs lbm_args(Ctx& c, int shift, bool fuse_adr) {
     LbmArgs a{};
     a.fsrc = c.f[c.fcur];
     a.fdst = c.f[1 - c.fcur];
@@ -534,6 +534,11 @@ void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap,
     a.om = 1.0 / c.p.tau;
     a.se = c.p.mrt_se; a.seps = c.p.mrt_s_eps; a.sq = c.p.mrt_sq;
     if (fuse_adr) scalar_args(c, a);
+    return a;
+}
+
+void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap, int persist) {
+    LbmArgs a = lbm_args(c, shift, fuse_adr);
     dim3 b, g;
     set_geometry(c, a, bx, by, remap, persist, g, b);
     const int op = c.p.collision_model;
@@ -555,6 +560,50 @@ void launch_lbm(Ctx& c, int shift, bool fuse_adr) {
         c.gcur = 1 - c.gcur;
         c.c_from_store = false;
     }
+}
+
+// Batched collide-stream over the members of an ensemble: workgroup b runs
+// workgroup map[b].y of member map[b].x's own linear grid (the map itself
+// orders the workgroups across XCDs).
+template <int OP, bool FP32>
+__global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const LbmArgs a = packs[w.x];
+    int x, y;
+    if (!block_origin(a, w.y, x, y)) return;
+    int capped = lbm_cell<OP, FP32, false, false>(a, x, y);
+    for (int off = 32; off > 0; off >>= 1) capped += __shfl_down(capped, off);
+    if ((threadIdx.x + threadIdx.y * blockDim.x) % warpSize == 0 && capped)
+        atomicAdd(&a.counters[0], capped);
+}
+
+// The steady step's launch (previous-step shift force, no scalar) with a
+// linear grid of bx x by workgroups; swaps the population buffers as
+// launch_lbm does.
+void batch_items_lbm(Ctx& c, int bx, int by, BatchItem* it) {
+    LbmArgs a = lbm_args(c, SHIFT_PREV, false);
+    dim3 g, b;
+    set_geometry(c, a, bx, by, 0, 0, g, b);
+    batch_put(it[0], a.nblk, a);
+    c.fcur = 1 - c.fcur;
+}
+
+template <bool FP32>
+static void batch_lbm_op(const BatchVariant& v, const LbmArgs* packs, const int2* map, int nblocks,
+                         hipStream_t s) {
+    const dim3 g(nblocks), b(v.bx, v.by);
+    if (v.op == SF_REGULARIZED) hipLaunchKernelGGL((kb_lbm<SF_REGULARIZED, FP32>), g, b, 0, s, packs, map);
+    else if (v.op == SF_MRT) hipLaunchKernelGGL((kb_lbm<SF_MRT, FP32>), g, b, 0, s, packs, map);
+    else hipLaunchKernelGGL((kb_lbm<SF_BGK, FP32>), g, b, 0, s, packs, map);
+}
+
+void batch_launch_lbm(const BatchVariant& v, const void* packs, const int2* map, int nblocks,
+                      hipStream_t s) {
+    if (nblocks <= 0) return;
+    const LbmArgs* p = static_cast<const LbmArgs*>(packs);
+    if (v.fp32) batch_lbm_op<true>(v, p, map, nblocks, s);
+    else batch_lbm_op<false>(v, p, map, nblocks, s);
 }
 
 // ---------------------------------------------------------------------------

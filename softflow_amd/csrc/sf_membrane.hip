@@ -280,14 +280,19 @@ __global__ __launch_bounds__(64) void k_membrane(MemArgs a) {
     membrane_body<MAXN>(a, blockIdx.x);
 }
 
-void launch_membrane(Ctx& c) {
-    if (c.ncaps == 0) return;
+static MemArgs mem_args(const Ctx& c) {
     MemArgs a{};
     a.x = c.x; a.y = c.y; a.vx = c.vx; a.vy = c.vy; a.fx = c.fx; a.fy = c.fy;
     a.L0 = c.L0; a.th0 = c.th0; a.ds0 = c.ds0; a.Lprev = c.Lprev; a.has_prev = c.has_prev;
     a.off = c.off; a.desc = c.desc;
     a.periodic_nx = c.periodic_x ? c.nx : 0;
     a.counters = c.counters;
+    return a;
+}
+
+void launch_membrane(Ctx& c) {
+    if (c.ncaps == 0) return;
+    const MemArgs a = mem_args(c);
     if (c.max_nodes <= 64)
         hipLaunchKernelGGL(k_membrane<64>, dim3(c.ncaps), dim3(64), 0, c.stream, a);
     else
@@ -609,8 +614,7 @@ __global__ void k_cell_geom(const double* x, int K, double rcut, int max_ncx, Ce
     }
 }
 
-void launch_repulsion(Ctx& c) {
-    if (c.ncaps == 0) return;
+static RepArgs rep_args(const Ctx& c) {
     RepArgs a{};
     a.x = c.x; a.y = c.y; a.vx = c.vx; a.vy = c.vy; a.fx = c.fx; a.fy = c.fy;
     a.off = c.off; a.owner = c.owner; a.K = c.K; a.ncaps = c.ncaps;
@@ -620,8 +624,17 @@ void launch_repulsion(Ctx& c) {
     a.ybot = 0.5; a.ytop = static_cast<double>(c.ny) - 1.5;
     a.rcut2_hi = a.rcut * a.rcut * (1.0 + 1e-12);
     a.dmin = c.mdf_hist_on ? c.mdf_dmin : nullptr;
+    return a;
+}
+
+// contacts are summed over all pairs up to this many nodes, over the cell list above
+constexpr int REP_ALLPAIRS_MAX = 100;
+
+void launch_repulsion(Ctx& c) {
+    if (c.ncaps == 0) return;
+    RepArgs a = rep_args(c);
     const int bs = 128, nb = (c.K + bs - 1) / bs;
-    if (c.K <= 100) {
+    if (c.K <= REP_ALLPAIRS_MAX) {
         hipLaunchKernelGGL((k_repulsion<false>), dim3(nb), dim3(bs), 0, c.stream, a);
         return;
     }
@@ -653,6 +666,122 @@ void launch_repulsion(Ctx& c) {
     a.own = c.cell_own;
     hipLaunchKernelGGL(k_repulsion_lanes, dim3((c.K + REP_NODES - 1) / REP_NODES), dim3(256), 0,
                        c.stream, a);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched launches over the members of an ensemble: each workgroup looks up
+// (member, own workgroup index) in the map and runs the single-context body
+// on that member's arguments.
+struct CellsFillPack {
+    RepArgs a;
+    int* cnt;
+    int* list;
+    int* counters;
+};
+struct CellsSortPack {
+    int ncells, nclear;
+    const int* cnt;
+    int* list;
+    int* other;
+    const double *x, *y, *vx, *vy;
+    const int* owner;
+    double4* pay;
+    int* own;
+};
+struct ContactPack {
+    RepArgs a;
+    int allpairs;    // all-pairs sum (one thread per node) instead of the cell list
+};
+constexpr int BATCH_FILL_THREADS = 128;
+constexpr int BATCH_ALLPAIRS_THREADS = 256;
+
+template <int MAXN>
+__global__ __launch_bounds__(64) void kb_membrane(const MemArgs* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    membrane_body<MAXN>(packs[w.x], w.y);
+}
+
+__global__ __launch_bounds__(BATCH_FILL_THREADS) void kb_cells_fill(const CellsFillPack* packs,
+                                                                   const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const CellsFillPack p = packs[w.x];
+    cells_fill_body(p.a, p.cnt, p.list, p.counters, w.y);
+}
+
+__global__ __launch_bounds__(256) void kb_cells_sort(const CellsSortPack* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const CellsSortPack p = packs[w.x];
+    cells_sort_body(p.ncells, p.nclear, p.cnt, p.list, p.other, p.x, p.y, p.vx, p.vy, p.owner,
+                    p.pay, p.own, w.y);
+}
+
+__global__ __launch_bounds__(256) void kb_contact(const ContactPack* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const ContactPack p = packs[w.x];
+    if (p.allpairs) repulsion_body<false>(p.a, w.y);
+    else repulsion_lanes_body(p.a, w.y);
+}
+
+void batch_items_membrane(Ctx& c, BatchItem* it) {
+    batch_put(it[0], c.ncaps, mem_args(c));
+}
+
+// The same arguments and cell-count buffer alternation as launch_repulsion
+// (x-periodic channels: the cell grid is fixed).
+void batch_items_contact(Ctx& c, BatchItem* it) {
+    RepArgs a = rep_args(c);
+    if (c.K <= REP_ALLPAIRS_MAX) {
+        batch_put(it[0], 0, CellsFillPack{});
+        batch_put(it[1], 0, CellsSortPack{});
+        batch_put(it[2], (c.K + BATCH_ALLPAIRS_THREADS - 1) / BATCH_ALLPAIRS_THREADS, ContactPack{a, 1});
+        return;
+    }
+    a.ncy = c.ncy; a.cdy = c.cdy;
+    a.ncx = c.ncx; a.xorg = 0.0; a.cdx = c.cdx;
+    const int nmax = c.ncx * c.ncy;
+    int* cnt = c.cell_cnt + static_cast<size_t>(c.cell_cur) * nmax;
+    int* other = c.cell_cnt + static_cast<size_t>(1 - c.cell_cur) * nmax;
+    batch_put(it[0], (c.K + BATCH_FILL_THREADS - 1) / BATCH_FILL_THREADS,
+              CellsFillPack{a, cnt, c.cell_list, c.counters});
+    batch_put(it[1], (nmax + 255) / 256,
+              CellsSortPack{a.ncx * a.ncy, nmax, cnt, c.cell_list, other, c.x, c.y, c.vx, c.vy,
+                            c.owner, c.cell_pay, c.cell_own});
+    c.cell_cur = 1 - c.cell_cur;
+    a.cnt = cnt;
+    a.pay = c.cell_pay;
+    a.own = c.cell_own;
+    batch_put(it[2], (c.K + REP_NODES - 1) / REP_NODES, ContactPack{a, 0});
+}
+
+void batch_launch_membrane(int kind, const BatchVariant& v, const void* packs, const int2* map,
+                           int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return;
+    const dim3 g(nblocks);
+    switch (kind) {
+    case BK_MEMBRANE:
+        if (v.maxn <= 64)
+            hipLaunchKernelGGL(kb_membrane<64>, g, dim3(64), 0, s, static_cast<const MemArgs*>(packs), map);
+        else
+            hipLaunchKernelGGL(kb_membrane<256>, g, dim3(64), 0, s, static_cast<const MemArgs*>(packs), map);
+        break;
+    case BK_CELLS_FILL:
+        hipLaunchKernelGGL(kb_cells_fill, g, dim3(BATCH_FILL_THREADS), 0, s,
+                           static_cast<const CellsFillPack*>(packs), map);
+        break;
+    case BK_CELLS_SORT:
+        hipLaunchKernelGGL(kb_cells_sort, g, dim3(256), 0, s, static_cast<const CellsSortPack*>(packs), map);
+        break;
+    case BK_CONTACT:
+        hipLaunchKernelGGL(kb_contact, g, dim3(256), 0, s, static_cast<const ContactPack*>(packs), map);
+        break;
+    default:
+        break;
+    }
 }
 
 }  // namespace sf
