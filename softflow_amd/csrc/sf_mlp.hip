@@ -45,47 +45,70 @@ __device__ __forceinline__ float gelu_d(float z) {
 // g_par in [0, 4] and g_perp in [-2, 2], and default to (1, 0), the plain
 // correction, where an input or an output is not finite.  Nothing
 // happens when the residual is already within tolerance.
-__global__ __launch_bounds__(GB) void k_mlp_infer(const float* __restrict__ P,
-                                                  const float* __restrict__ norm, const float* feat,
-                                                  double* gain, int K, const MdfDev* md, double tol) {
+//
+// One wavefront per node and pass, lane u computing hidden unit u of both
+// layers from weights staged once per workgroup in LDS (W2 rows padded to
+// avoid bank conflicts); lanes 0 and 1 form the two outputs.  Every sum runs
+// over its terms in index order, as the one-thread-per-node form of the same
+// network (k_mlp_forward) does.
+constexpr int INF_SLOTS = 4;            // nodes in flight per 256-thread workgroup
+constexpr int INF_CHUNK = 32;           // nodes per workgroup
+constexpr int W2_LD = MLP_W + 1;
+
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P,
+                                                   const float* __restrict__ norm, const float* feat,
+                                                   double* gain, int K, const MdfDev* md, double tol) {
     if (tol > 0.0 && __longlong_as_double(static_cast<long long>(md->res[1])) <= tol) return;
-    const int k = blockIdx.x * GB + threadIdx.x;
-    if (k >= K) return;
-    gain[2 * k] = 1.0;
-    gain[2 * k + 1] = 0.0;
-    float xh[MLP_IN];
-    bool ok = true;
-#pragma unroll
-    for (int j = 0; j < MLP_IN; ++j) {
-        const float v = feat[static_cast<size_t>(k) * MLP_IN + j];
-        ok &= isfinite(v);
-        xh[j] = (v - norm[j]) / norm[MLP_IN + j];
+    __shared__ float sW1[MLP_W * MLP_IN], sb1[MLP_W], sW2[MLP_W * W2_LD], sb2[MLP_W];
+    __shared__ float sW3[MLP_OUT * MLP_W], sb3[MLP_OUT], snorm[MLP_NORM];
+    __shared__ float sx[INF_SLOTS][MLP_IN], sa1[INF_SLOTS][MLP_W], sa2[INF_SLOTS][MLP_W];
+    const int t = threadIdx.x;
+    for (int i = t; i < MLP_W * MLP_IN; i += 256) sW1[i] = P[W1_OFF + i];
+    for (int i = t; i < MLP_W * MLP_W; i += 256) sW2[(i / MLP_W) * W2_LD + i % MLP_W] = P[W2_OFF + i];
+    if (t < MLP_W) { sb1[t] = P[B1_OFF + t]; sb2[t] = P[B2_OFF + t]; }
+    if (t < MLP_OUT * MLP_W) sW3[t] = P[W3_OFF + t];
+    if (t < MLP_OUT) sb3[t] = P[B3_OFF + t];
+    if (t < MLP_NORM) snorm[t] = norm[t];
+    __syncthreads();
+    const int u = t & 63, q = t >> 6;
+    const int k_end = min(K, (blockIdx.x + 1) * INF_CHUNK);
+    for (int base = blockIdx.x * INF_CHUNK; base < k_end; base += INF_SLOTS) {
+        const int k = base + q;
+        const bool live = k < k_end;
+        float v = 0.f;
+        if (live && u < MLP_IN) v = feat[static_cast<size_t>(k) * MLP_IN + u];
+        const bool ok = __all(u >= MLP_IN || isfinite(v)) != 0;   // this node's wavefront
+        if (u < MLP_IN) sx[q][u] = (v - snorm[u]) / snorm[MLP_IN + u];
+        __syncthreads();
+        float acc = sb1[u];
+        for (int j = 0; j < MLP_IN; ++j) acc += sW1[u * MLP_IN + j] * sx[q][j];
+        sa1[q][u] = gelu(acc);
+        __syncthreads();
+        acc = sb2[u];
+        for (int j = 0; j < MLP_W; ++j) acc += sW2[u * W2_LD + j] * sa1[q][j];
+        sa2[q][u] = gelu(acc);
+        __syncthreads();
+        float out = 0.f;
+        if (u < MLP_OUT) {
+            out = sb3[u];
+            for (int i = 0; i < MLP_W; ++i) out += sW3[u * MLP_W + i] * sa2[q][i];
+        }
+        const float out1 = __shfl(out, 1);
+        if (live && u == 0) {
+            double g0 = 1.0, g1 = 0.0;
+            if (ok) {
+                const float gp = snorm[2 * MLP_IN] + snorm[2 * MLP_IN + MLP_OUT] * out;
+                const float gq = snorm[2 * MLP_IN + 1] + snorm[2 * MLP_IN + MLP_OUT + 1] * out1;
+                if (isfinite(gp) && isfinite(gq)) {
+                    g0 = fminf(fmaxf(gp, 0.f), 4.f);
+                    g1 = fminf(fmaxf(gq, -2.f), 2.f);
+                }
+            }
+            gain[2 * k] = g0;
+            gain[2 * k + 1] = g1;
+        }
+        __syncthreads();   // the slots are reused by the next pass
     }
-    if (!ok) return;
-    float a1[MLP_W];
-#pragma unroll
-    for (int i = 0; i < MLP_W; ++i) {
-        float acc = P[B1_OFF + i];
-#pragma unroll
-        for (int j = 0; j < MLP_IN; ++j) acc += P[W1_OFF + i * MLP_IN + j] * xh[j];
-        a1[i] = gelu(acc);
-    }
-    float out[MLP_OUT];
-#pragma unroll
-    for (int o = 0; o < MLP_OUT; ++o) out[o] = P[B3_OFF + o];
-    for (int i = 0; i < MLP_W; ++i) {
-        float acc = P[B2_OFF + i];
-#pragma unroll
-        for (int j = 0; j < MLP_W; ++j) acc += P[W2_OFF + i * MLP_W + j] * a1[j];
-        const float a2 = gelu(acc);
-#pragma unroll
-        for (int o = 0; o < MLP_OUT; ++o) out[o] += P[W3_OFF + o * MLP_W + i] * a2;
-    }
-    const float gp = norm[2 * MLP_IN] + norm[2 * MLP_IN + MLP_OUT] * out[0];
-    const float gq = norm[2 * MLP_IN + 1] + norm[2 * MLP_IN + MLP_OUT + 1] * out[1];
-    if (!isfinite(gp) || !isfinite(gq)) return;
-    gain[2 * k] = fminf(fmaxf(gp, 0.f), 4.f);
-    gain[2 * k + 1] = fminf(fmaxf(gq, -2.f), 2.f);
 }
 
 // The gains, low-passed along each membrane with the binomial weights
@@ -316,7 +339,7 @@ __global__ void k_colstats(const float* rows, int64_t nrows, int64_t chunk, doub
 
 void launch_mlp_infer(Ctx& c) {
     if (!c.model || c.K == 0) return;
-    hipLaunchKernelGGL(k_mlp_infer, dim3((c.K + GB - 1) / GB), dim3(GB), 0, c.stream,
+    hipLaunchKernelGGL(k_mlp_infer, dim3((c.K + INF_CHUNK - 1) / INF_CHUNK), dim3(256), 0, c.stream,
                        c.model->params, c.model->norm, c.mdf_feat, c.mdf_pred, c.K, c.mdf, c.mdf_tol);
     hipLaunchKernelGGL(k_mdf_smooth, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.mdf_pred,
                        c.spread, c.off, c.owner, c.K, c.mdf, c.mdf_tol);

@@ -68,7 +68,7 @@ def test_lbm_doubly_periodic_fp64():
 
 
 def test_capsules_regularized_mdf3_fp64():
-    # C2's physics (regularized, 3 MDF iterations, Skalak C=10) on a shorter channel
+    # the 400 x 80 suspension's physics (regularized, 3 MDF iterations, Skalak C=10) on a shorter channel
     case = dict(name="c2_small", nx=200, ny=80, tau=0.8, bfx=3e-5, coll=2, mdf=3,
                 capsules=[dict(count=12, region=(20, 10, 180, 70), rmin=3.0, rmax=3.0, nodes=20,
                                seed=42, mem=SKALAK)])
@@ -154,14 +154,14 @@ def test_fp32_capsules():
 
 
 # The configurations the parity bar is stated on, over 10^4 steps.
-def test_c1_poiseuille_10k_steps():
-    case = dict(name="C1", nx=400, ny=80, tau=0.8, bfx=3e-5, coll=0)
+def test_poiseuille_400x80_10k_steps():
+    case = dict(name="poiseuille_400x80", nx=400, ny=80, tau=0.8, bfx=3e-5, coll=0)
     got, ref = _compare(case, 10000, 1e-12)
     assert np.array_equal(got["ux"], ref["ux"])
 
 
-def test_c2_capsule_suspension_10k_steps():
-    case = dict(name="C2", nx=400, ny=80, tau=0.8, bfx=3e-5, coll=2, mdf=3,
+def test_capsule_suspension_400x80_10k_steps():
+    case = dict(name="suspension_400x80", nx=400, ny=80, tau=0.8, bfx=3e-5, coll=2, mdf=3,
                 capsules=[dict(count=30, region=(20, 10, 380, 70), rmin=3.0, rmax=3.0, nodes=20,
                                seed=42, mem=SKALAK)])
     _compare(case, 10000, 1e-12)
