@@ -94,15 +94,23 @@ def build_product(case, precision="fp32", graph=True):
     return sim
 
 
-def cpu_baseline(case, steps):
-    """The oracle (single-threaded FP64 restatement) on this host: ``steps``
-    full steps of the same workload."""
+def cpu_baseline(case, steps, threads):
+    """The reference's own hot-path components (oracle/_ref: its lbm, coupling
+    and membrane translation units compiled in place, driven in
+    Simulation::step order) with ``threads`` OpenMP threads on this host:
+    ``steps`` full steps of the same workload.  Where that build is absent,
+    the single-threaded FP64 oracle restatement instead."""
     from oracle import pyoracle as po
 
+    kind = "reference" if os.path.exists(po.REF_SO) else "port"
     p = po.default_params(nx=case["nx"], ny=case["ny"], tau=case["tau"], boundary_type=1,
                           body_force_x=case["bfx"], collision_model=case["coll"],
                           scalar_enabled=1, diffusivity=[case["scalar"]["D"]])
-    s = po.Sim(p, "oracle")
+    s = po.Sim(p, "ref" if kind == "reference" else "oracle")
+    if kind == "reference":
+        s.set_threads(threads)
+    else:
+        threads = 1
     for spec in case["capsules"]:
         m = po.default_membrane(**spec["mem"])
         x0, y0, x1, y1 = spec["region"]
@@ -113,7 +121,7 @@ def cpu_baseline(case, steps):
     t = time.perf_counter()
     s.step(steps)
     dt = time.perf_counter() - t
-    return case["nx"] * case["ny"] * steps / dt / 1e6, dt
+    return case["nx"] * case["ny"] * steps / dt / 1e6, dt, kind, threads
 
 
 # Learned MDF warm start workload: 2048 x 512 channel, regularized tau = 0.8,
@@ -123,7 +131,7 @@ def cpu_baseline(case, steps):
 # coupling of plain multi-direct forcing diverge within 120-200 steps, in the
 # FP64 restatement of the reference as on the device.
 MDF_CHANNEL = dict(nx=2048, ny=512, tau=0.8, bfx=3e-5, count=500, radius=4.0, nodes=25, seed=5,
-          region=(4, 8, 2044, 504), mem={"C_skalak": 1.0})
+                   region=(4, 8, 2044, 504), mem={"C_skalak": 1.0})
 
 
 USE_GRAPH = True   # HIP-graph replay of the steps (off: every launch eager, for tracing)
@@ -452,10 +460,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            v, cdt = cpu_baseline(case, args.cpu_steps)
-            cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+            threads = max(1, min(16, os.cpu_count() or 1))
+            v, cdt, kind, cores = cpu_baseline(case, args.cpu_steps, threads)
+            what = ("the reference's own components (OpenMP, FP64)" if kind == "reference"
+                    else "the FP64 oracle restatement")
+            cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": cores, "kind": kind,
                    "sample": f"{args.cpu_steps} full steps of the same 4096x1024 + 2000-capsule "
-                             f"workload in the FP64 oracle ({cdt:.1f} s)"}
+                             f"workload in {what} ({cdt:.1f} s)"}
         except Exception as e:  # the oracle library is test infrastructure; report its absence
             cpu = {"value": None, "unit": "MLUPS", "cores": 1, "kind": "port", "sample": f"unavailable: {e}"}
 

@@ -76,6 +76,8 @@ def _lib(path: str, prefix: str):
     for n in ("initialize", "membrane_forces_only"):
         g(n).argtypes = [C.c_void_p]
     g("step").argtypes = [C.c_void_p, C.c_int]
+    if prefix == "sfr_":
+        g("set_threads").argtypes = [C.c_int]
     g("scalar_initialize").argtypes = [C.c_void_p, C.c_double]
     g("set_scalar_point").argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]
     g("num_capsules").restype = C.c_int
@@ -197,6 +199,12 @@ class Sim:
 
     def step(self, n=1):
         self._f("step")(self.h, n)
+
+    def set_threads(self, n):
+        """OpenMP threads of the reference build's parallel regions (reference only)."""
+        if self.pre != "sfr_":
+            raise ValueError("the oracle restatement is single-threaded")
+        self.L.sfr_set_threads(n)
 
     def membrane_forces_only(self):
         self._f("membrane_forces_only")(self.h)

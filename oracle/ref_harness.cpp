@@ -97,6 +97,10 @@ sfr_sim* sfr_create(const sfo_params* p) {
 
 void sfr_destroy(sfr_sim* s) { delete s; }
 
+// OpenMP threads for the reference's parallel regions from here on (the
+// CPU-baseline timing; every comparison runs at the default single thread).
+void sfr_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 void sfr_add_capsule(sfr_sim* s, double cx, double cy, double radius, int num_nodes,
                      const sfo_membrane* m, int type) {
     s->caps.addCapsule(Vec2d{cx, cy}, radius, num_nodes, to_mem(m), type);
