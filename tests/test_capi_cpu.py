@@ -61,6 +61,26 @@ def test_argument_validation_before_device():
     assert L.sf_create(None, 0, None) == capi.SF_E_ARG
 
 
+def test_batch_argument_validation_before_device():
+    from softflow_amd import capi
+    L = capi.load()
+    b = C.c_void_p()
+    assert L.sf_batch_create(C.byref(b), None, 1) == capi.SF_E_ARG
+    arr = (C.c_void_p * 2)(None, None)
+    assert L.sf_batch_create(C.byref(b), arr, 0) == capi.SF_E_ARG
+    assert "at least one member" in L.sf_last_error().decode()
+    assert L.sf_batch_create(C.byref(b), arr, 2) == capi.SF_E_ARG
+    assert not b.value
+    fake = (C.c_void_p * 2)(0x1000, 0x1000)     # never dereferenced: duplicates are refused first
+    assert L.sf_batch_create(C.byref(b), fake, 2) == capi.SF_E_ARG
+    assert "distinct" in L.sf_last_error().decode()
+    for fn, args in (("sf_batch_step", (None, 1)), ("sf_batch_synchronize", (None,)),
+                     ("sf_batch_set_graph_mode", (None, 1)), ("sf_batch_set_lbm_launch", (None, 64, 4, 1)),
+                     ("sf_batch_info", (None, None))):
+        assert getattr(L, fn)(*args) == capi.SF_E_ARG, fn
+    assert L.sf_batch_destroy(None) == capi.SF_OK
+
+
 @pytest.mark.skipif(os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"),
                     reason="a GPU may be present")
 def test_create_fails_cleanly_without_device():
