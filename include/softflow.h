@@ -109,6 +109,13 @@ int  sf_get_nodes(sf_ctx* ctx, double* x, double* y, double* vx, double* vy,
 int  sf_set_nodes(sf_ctx* ctx, const double* x, const double* y,
                   const double* vx, const double* vy);
 int  sf_get_membrane_state(sf_ctx* ctx, double* prev_len, int32_t* has_prev);
+/* Checkpoint extension (no counterpart in the reference, whose .sfck files keep
+ * f, rho/u, flags and nodes, io/checkpoint.cpp:88-190): the raw IBM force of the
+ * last step, (Fx, Fy) pairs per cell, from which the next collision's velocity
+ * is computed.  Setting it makes the next step use it instead of a stored
+ * macroscopic field, so a run resumed from a checkpoint continues bit for bit. */
+int  sf_get_force_state(sf_ctx* ctx, double* F);
+int  sf_set_force_state(sf_ctx* ctx, const double* F);
 
 /* Scalar transport (AdvectionDiffusion, advection_diffusion.h:17-99). */
 int  sf_set_scalar(sf_ctx* ctx, int species, const double* g, const double* C);

@@ -25,6 +25,7 @@ EXPORTS = (
     "sf_last_error", "sf_device_count", "sf_create", "sf_destroy", "sf_set_flags",
     "sf_set_populations", "sf_get_populations", "sf_get_macroscopic", "sf_set_macroscopic",
     "sf_set_capsules", "sf_get_nodes", "sf_set_nodes", "sf_get_membrane_state",
+    "sf_get_force_state", "sf_set_force_state",
     "sf_set_scalar", "sf_get_scalar", "sf_step", "sf_set_graph_mode", "sf_synchronize",
     "sf_stage", "sf_last_capped", "sf_field_stats", "sf_step_count", "sf_kernel_times",
     "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_probe_math",
@@ -93,6 +94,8 @@ def load():
         "sf_get_nodes": [C.c_void_p, _DP, _DP, _DP, _DP, _DP, _DP],
         "sf_set_nodes": [C.c_void_p, _DP, _DP, _DP, _DP],
         "sf_get_membrane_state": [C.c_void_p, _DP, _IP],
+        "sf_get_force_state": [C.c_void_p, _DP],
+        "sf_set_force_state": [C.c_void_p, _DP],
         "sf_set_scalar": [C.c_void_p, C.c_int, _DP, _DP],
         "sf_get_scalar": [C.c_void_p, C.c_int, _DP, _DP],
         "sf_step": [C.c_void_p, C.c_int],

@@ -323,6 +323,9 @@ PYBIND11_MODULE(softflow_core, m) {
         .def("lastCappedNodes", &Simulation::lastCappedNodes)
         .def("checkStability", &Simulation::checkStability)
         .def("fieldStats", &Simulation::fieldStats)
+        .def("saveCheckpoint", &Simulation::saveCheckpoint, py::arg("filename"))
+        .def("loadCheckpoint", &Simulation::loadCheckpoint, py::arg("filename"))
+        .def("setCurrentStep", &Simulation::setCurrentStep, py::arg("step"))
         .def("lbmSolver", [](Simulation& s) { return LBMSolverView{&s}; }, py::keep_alive<0, 1>())
         .def("field", [](Simulation& s) -> LatticeField& { return s.field(); },
              py::return_value_policy::reference_internal)

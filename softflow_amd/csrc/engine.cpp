@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <random>
 
@@ -525,6 +526,11 @@ void Simulation::pushDirty() {
             check(sf_set_scalar(ctx_, s, adr_->g_[s].data(), adr_->C_[s].data()), "sf_set_scalar");
         adr_->dirty_ = false;
     }
+    if (!pending_force_.empty()) {
+        // after the stored macroscopic field: the next collision uses this force instead
+        check(sf_set_force_state(ctx_, pending_force_.data()), "sf_set_force_state");
+        pending_force_.clear();
+    }
 }
 
 void Simulation::pullField(LatticeField& F) {
@@ -572,6 +578,7 @@ void Simulation::step() {
     if (params_.enable_stability_checks && params_.stability_check_interval > 0 &&
         step_count_ % params_.stability_check_interval == 0)
         checkStability();
+    intervalCheckpoint(step_count_);
     if (cb_) cb_(*this, static_cast<int>(step_count_));
     step_count_++;
 }
@@ -599,6 +606,12 @@ void Simulation::run(int num_steps) {
             const int to_check = static_cast<int>(r == 0 ? 1 : iv - r + 1);
             chunk = std::min(chunk, to_check);
         }
+        const int civ = params_.checkpoint_interval;
+        if (civ > 0) {
+            // end the chunk on the next step whose index is a multiple of the interval
+            const int64_t r = step_count_ % civ;
+            chunk = std::min(chunk, static_cast<int>(r == 0 ? 1 : civ - r + 1));
+        }
         pushDirty();
         check(sf_step(ctx_, chunk), "sf_step");
         field_->stale_ = true;
@@ -606,6 +619,7 @@ void Simulation::run(int num_steps) {
         if (adr_) adr_->stale_ = true;
         step_count_ += chunk;
         if (checks && (step_count_ - 1) % iv == 0) checkStability();
+        intervalCheckpoint(step_count_ - 1);
         done += chunk;
     }
 }
@@ -805,6 +819,252 @@ std::vector<std::pair<std::string, double>> Simulation::kernelTimes() {
     check(sf_kernel_times(ctx_, 32, names, ms, &n), "sf_kernel_times");
     for (int i = 0; i < std::min(n, 32); ++i) out.emplace_back(names[i], ms[i]);
     return out;
+}
+
+// ---------------------------------------------------------------------------
+// Checkpoints in the reference's .sfck layout, version 4 (io/checkpoint.cpp:10-58
+// documents it): header, f (SoA), rho, ux, uy, cell types, per capsule id,
+// type, node count, positions, velocities, area, perimeter and periodic nx,
+// adhesion bonds (none: adhesion is refused at initialize()), then the scalar
+// block.  After it comes an extension block the reference's loader never reads:
+// "SFXT", its version, the raw IBM force of the last step and, per capsule, the
+// Kelvin-Voigt state.  A checkpoint without it restarts as the reference does:
+// the next collision takes rho and u from the file.
+namespace {
+
+template <typename T>
+void put(std::ofstream& o, const T& v) { o.write(reinterpret_cast<const char*>(&v), sizeof(T)); }
+template <typename T>
+void put_n(std::ofstream& o, const T* p, size_t n) { o.write(reinterpret_cast<const char*>(p), n * sizeof(T)); }
+template <typename T>
+bool get(std::ifstream& i, T& v) { i.read(reinterpret_cast<char*>(&v), sizeof(T)); return i.good(); }
+template <typename T>
+bool get_n(std::ifstream& i, T* p, size_t n) {
+    i.read(reinterpret_cast<char*>(p), static_cast<std::streamsize>(n * sizeof(T)));
+    return i.good();
+}
+bool skip(std::ifstream& i, size_t bytes) {
+    i.seekg(static_cast<std::streamoff>(bytes), std::ios::cur);
+    return i.good();
+}
+
+constexpr int SFCK_VERSION = 4;
+constexpr int SFXT_VERSION = 1;
+
+}  // namespace
+
+bool Simulation::saveCheckpoint(const std::string& filename) {
+    return writeCheckpoint(filename, static_cast<int>(step_count_));
+}
+
+// Simulation::step writes checkpoint_<step>.sfck after the step whose index is
+// a positive multiple of checkpoint_interval (simulation.cpp:773-779).
+void Simulation::intervalCheckpoint(int64_t step) {
+    const int iv = params_.checkpoint_interval;
+    if (iv > 0 && step > 0 && step % iv == 0)
+        writeCheckpoint(params_.output_dir + "/checkpoint_" + std::to_string(step) + ".sfck",
+                        static_cast<int>(step));
+}
+
+bool Simulation::writeCheckpoint(const std::string& filename, int step) {
+    if (!ctx_) throw std::runtime_error("Simulation: initialize() has not been called");
+    pushDirty();
+    std::ofstream out(filename, std::ios::binary);
+    if (!out.is_open()) {
+        std::cerr << "Checkpoint: cannot open " << filename << " for writing\n";
+        return false;
+    }
+    LatticeField& fl = *field_;
+    const int nx = params_.nx, ny = params_.ny;
+    const size_t N = static_cast<size_t>(nx) * ny;
+    std::vector<Capsule>& cs = caps_.capsules();
+    const int ncaps = static_cast<int>(cs.size());
+    out.write("SFCK", 4);
+    put(out, SFCK_VERSION);
+    put(out, step);
+    put(out, nx);
+    put(out, ny);
+    put(out, ncaps);
+    put_n(out, fl.fData(), 9 * N);
+    put_n(out, fl.rhoData(), N);
+    put_n(out, fl.uxData(), N);
+    put_n(out, fl.uyData(), N);
+    put_n(out, fl.cellTypeData(), N);
+    for (Capsule& c : cs) {
+        put(out, c.getId());
+        put(out, c.getType());
+        put(out, c.numNodes());
+        for (const Vec2d& p : c.pos_) { put(out, p.x); put(out, p.y); }
+        for (const Vec2d& v : c.vel_) { put(out, v.x); put(out, v.y); }
+        put(out, c.area());
+        put(out, c.perimeter());
+        put(out, c.getPeriodicX());
+    }
+    put(out, 0);                                  // adhesion bonds
+    const int has_scalar = adr_ ? 1 : 0;
+    put(out, has_scalar);
+    if (adr_) {
+        put(out, adr_->nsp_);
+        for (int s = 0; s < adr_->nsp_; ++s) {
+            const Real* C = adr_->concentrationData(s);   // pulls g and C
+            put_n(out, adr_->g_[s].data(), 9 * N);
+            put_n(out, C, N);
+        }
+        put(out, 0);                              // capsule release / absorb trackers
+        put(out, 0);                              // particle masses
+        put(out, 0);                              // surface coverage
+    }
+    out.write("SFXT", 4);
+    put(out, SFXT_VERSION);
+    std::vector<double> F(2 * N);
+    check(sf_get_force_state(ctx_, F.data()), "sf_get_force_state");
+    put_n(out, F.data(), F.size());
+    for (Capsule& c : cs) {
+        put(out, c.has_prev_ ? 1 : 0);
+        put_n(out, c.prev_len_.data(), c.prev_len_.size());
+    }
+    out.close();
+    if (!out) {
+        std::cerr << "Checkpoint: writing " << filename << " failed\n";
+        return false;
+    }
+    std::cout << "Checkpoint saved: step " << step << ", " << ncaps << " capsules"
+              << (has_scalar ? ", scalar state (v4)" : "") << " -> " << filename << std::endl;
+    return true;
+}
+
+bool Simulation::loadCheckpoint(const std::string& filename) {
+    if (!ctx_) throw std::runtime_error("Simulation: initialize() has not been called");
+    std::ifstream in(filename, std::ios::binary);
+    if (!in.is_open()) {
+        std::cerr << "Checkpoint: cannot open " << filename << " for reading\n";
+        return false;
+    }
+    auto bad = [&](const std::string& why) {
+        std::cerr << "Checkpoint: " << why << " (" << filename << ")\n";
+        return false;
+    };
+    char magic[4];
+    in.read(magic, 4);
+    if (!in.good() || std::strncmp(magic, "SFCK", 4) != 0) return bad("invalid magic (not a SoftFlow checkpoint)");
+    int version = 0, step = 0, nx = 0, ny = 0, ncaps = 0;
+    if (!get(in, version)) return bad("truncated header");
+    if (version < 2 || version > 4) return bad("unsupported version " + std::to_string(version) + " (expected 2, 3, or 4)");
+    if (version == 2)
+        std::cerr << "Checkpoint: loading version 2 file - scalar transport state will be reset "
+                     "to initial values (no scalar data in v2 files)\n";
+    if (!get(in, step) || !get(in, nx) || !get(in, ny) || !get(in, ncaps)) return bad("truncated header");
+    if (nx != params_.nx || ny != params_.ny)
+        return bad("domain mismatch: file has " + std::to_string(nx) + "x" + std::to_string(ny));
+    std::vector<Capsule>& cs = caps_.capsules();
+    if (ncaps != static_cast<int>(cs.size()))
+        return bad("capsule count mismatch: file has " + std::to_string(ncaps));
+    // read everything first: a short or mismatched file changes nothing
+    const size_t N = static_cast<size_t>(nx) * ny;
+    std::vector<Real> f(9 * N), rho(N), ux(N), uy(N);
+    std::vector<int> flags(N);
+    if (!get_n(in, f.data(), f.size()) || !get_n(in, rho.data(), N) || !get_n(in, ux.data(), N) ||
+        !get_n(in, uy.data(), N) || !get_n(in, flags.data(), N))
+        return bad("truncated lattice state");
+    std::vector<std::vector<Vec2d>> pos(ncaps), vel(ncaps);
+    for (int c = 0; c < ncaps; ++c) {
+        int id = 0, tid = 0, nn = 0, pnx = 0;
+        if (!get(in, id) || !get(in, tid) || !get(in, nn)) return bad("truncated capsule record");
+        if (nn != cs[c].numNodes())
+            return bad("node count mismatch for capsule " + std::to_string(c) + ": file has " + std::to_string(nn));
+        pos[c].resize(nn);
+        vel[c].resize(nn);
+        for (Vec2d& p : pos[c]) if (!get(in, p.x) || !get(in, p.y)) return bad("truncated capsule record");
+        for (Vec2d& v : vel[c]) if (!get(in, v.x) || !get(in, v.y)) return bad("truncated capsule record");
+        Real area = 0.0, perim = 0.0;
+        if (!get(in, area) || !get(in, perim) || !get(in, pnx)) return bad("truncated capsule record");
+    }
+    int nbonds = 0;
+    if (!get(in, nbonds)) return bad("truncated bond count");
+    // bonds are {int, int, int, int, Real, Real}; adhesion is not on the device path
+    if (nbonds > 0 && !skip(in, static_cast<size_t>(nbonds) * (4 * sizeof(int) + 2 * sizeof(Real))))
+        return bad("truncated bonds");
+    std::vector<std::vector<Real>> g, C;
+    bool scalar_loaded = false;
+    if (version >= 3) {
+        int has_scalar = 0;
+        if (!get(in, has_scalar)) return bad("truncated scalar block");
+        if (has_scalar) {
+            int nsp = 0;
+            if (!get(in, nsp)) return bad("truncated scalar block");
+            if (adr_ && nsp != adr_->nsp_) return bad("scalar species count mismatch: file has " + std::to_string(nsp));
+            g.assign(nsp, std::vector<Real>(9 * N));
+            C.assign(nsp, std::vector<Real>(N));
+            for (int s = 0; s < nsp; ++s)
+                if (!get_n(in, g[s].data(), 9 * N) || !get_n(in, C[s].data(), N)) return bad("truncated scalar block");
+            int n_tracked = 0;
+            if (!get(in, n_tracked) || (n_tracked > 0 && !skip(in, 2 * static_cast<size_t>(n_tracked) * sizeof(Real))))
+                return bad("truncated scalar trackers");
+            if (version >= 4) {
+                int n_mp = 0, n_gamma = 0;
+                if (!get(in, n_mp) || (n_mp > 0 && !skip(in, static_cast<size_t>(n_mp) * sizeof(Real))) ||
+                    !get(in, n_gamma) || (n_gamma > 0 && !skip(in, static_cast<size_t>(n_gamma) * sizeof(Real))))
+                    return bad("truncated scalar trackers");
+            }
+            if (adr_) scalar_loaded = true;
+            else std::cerr << "Checkpoint: file contains scalar state but sim has no scalar solver - "
+                              "scalar state will be ignored\n";
+        }
+    }
+    // the extension block, when present
+    std::vector<double> force;
+    std::vector<int> has_prev;
+    std::vector<std::vector<Real>> prev_len;
+    char ext[4];
+    in.read(ext, 4);
+    if (in.good() && std::strncmp(ext, "SFXT", 4) == 0) {
+        int ev = 0;
+        if (!get(in, ev) || ev != SFXT_VERSION) return bad("unsupported extension block");
+        force.resize(2 * N);
+        if (!get_n(in, force.data(), force.size())) return bad("truncated extension block");
+        has_prev.resize(ncaps);
+        prev_len.resize(ncaps);
+        for (int c = 0; c < ncaps; ++c) {
+            prev_len[c].resize(cs[c].numNodes());
+            if (!get(in, has_prev[c]) || !get_n(in, prev_len[c].data(), prev_len[c].size()))
+                return bad("truncated extension block");
+        }
+    }
+    // apply: the host mirrors are current (pulled above), then overwritten and pushed
+    LatticeField& fl = *field_;
+    fl.pull();
+    fl.f_ = std::move(f);
+    fl.rho_ = std::move(rho);
+    fl.ux_ = std::move(ux);
+    fl.uy_ = std::move(uy);
+    if (fl.flags_ != flags) {
+        fl.flags_ = std::move(flags);
+        fl.flags_dirty_ = true;
+    }
+    fl.f_dirty_ = fl.macro_dirty_ = true;
+    for (int c = 0; c < ncaps; ++c) {
+        cs[c].pos_ = pos[c];
+        cs[c].vel_ = vel[c];
+        if (!force.empty()) {
+            cs[c].has_prev_ = has_prev[c] != 0;
+            cs[c].prev_len_ = prev_len[c];
+        }
+    }
+    caps_.dirty_ = true;
+    if (scalar_loaded) {
+        adr_->pull();
+        for (int s = 0; s < adr_->nsp_; ++s) {
+            adr_->g_[s] = g[s];
+            adr_->C_[s] = C[s];
+        }
+        adr_->dirty_ = true;
+    }
+    pending_force_ = std::move(force);
+    step_count_ = step;
+    pushDirty();
+    std::cout << "Checkpoint loaded: step " << step << ", " << ncaps << " capsules <- " << filename
+              << std::endl;
+    return true;
 }
 
 // ---------------------------------------------------------------------------

@@ -347,6 +347,14 @@ public:
     void setMdfModel(MdfModel* model, bool warm);
     sf_ctx* deviceContext() const { return ctx_; }
 
+    // Checkpoint::save / load (io/checkpoint.cpp:88-369): the .sfck format,
+    // version 4, from and into the device state, plus an extension block for
+    // bit-exact resume.  false (with a message on stderr) on failure, as the
+    // reference returns.
+    bool saveCheckpoint(const std::string& filename);
+    bool loadCheckpoint(const std::string& filename);
+    void setCurrentStep(int step) { step_count_ = step; }
+
     // device <-> host synchronisation (used by the mirrors)
     void pullField(LatticeField& f);
     void pullNodes(CapsuleSystem& cs);
@@ -356,6 +364,8 @@ private:
     friend class Ensemble;
     void pushDirty();
     void markAdvanced(int steps);   // device state moved on: host mirrors are stale
+    bool writeCheckpoint(const std::string& filename, int step);
+    void intervalCheckpoint(int64_t step);
     void check(int rc, const char* what) const;
     SimulationParams params_;
     std::unique_ptr<LatticeField> field_;
@@ -372,6 +382,8 @@ private:
     int64_t mdf_collect_ = 0;
     MdfModel* mdf_model_ = nullptr;
     bool mdf_warm_ = false;
+    // raw force of the last step read from a checkpoint, set after the other state
+    std::vector<double> pending_force_;
 };
 
 // A batched ensemble (no counterpart in the reference): independent

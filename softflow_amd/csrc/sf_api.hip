@@ -501,6 +501,43 @@ int sf_set_macroscopic(sf_ctx* h, const double* rho, const double* ux, const dou
     return SF_OK;
 }
 
+int sf_get_force_state(sf_ctx* h, double* F) {
+    if (!h || !F) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    const size_t n2 = 2 * static_cast<size_t>(c.N);
+    if (c.fp32) {
+        std::vector<float> t(n2);
+        SF_HIP(hipMemcpy(t.data(), c.F[c.Fcur], n2 * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n2; ++i) F[i] = t[i];
+    } else {
+        SF_HIP(hipMemcpy(F, c.F[c.Fcur], n2 * 8, hipMemcpyDeviceToHost));
+    }
+    return SF_OK;
+}
+
+int sf_set_force_state(sf_ctx* h, const double* F) {
+    if (!h || !F) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    const size_t n2 = 2 * static_cast<size_t>(c.N);
+    if (c.fp32) {
+        std::vector<float> t(n2);
+        for (size_t i = 0; i < n2; ++i) t[i] = static_cast<float>(F[i]);
+        SF_HIP(hipMemcpy(c.F[c.Fcur], t.data(), n2 * 4, hipMemcpyHostToDevice));
+    } else {
+        SF_HIP(hipMemcpy(c.F[c.Fcur], F, n2 * 8, hipMemcpyHostToDevice));
+    }
+    // the buffer may now hold non-zeros anywhere
+    if (c.tile_dirty)
+        SF_HIP(hipMemset(c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles, 1, c.ntiles));
+    c.shift_next = SHIFT_PREV;
+    invalidate_graph(c);
+    return SF_OK;
+}
+
 int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_desc* desc,
                     const double* x, const double* y, const double* vx, const double* vy,
                     const double* rest_len, const double* rest_angle, const double* rest_ds,

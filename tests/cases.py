@@ -9,6 +9,7 @@ A case is a dict:
     rep: {damping_normal, friction_coeff}  (optional)
     solids: list of ('rect', x0, y0, x1, y1) | ('circle', cx, cy, r): extra SOLID
             cells set after initialize (LatticeField::setCellType)   (optional)
+    params: {name: value} further SimulationParams fields (product only)   (optional)
 Membrane keys follow the oracle's struct (model, G_s, C_skalak, k_bend, use_helfrich, ...).
 """
 from __future__ import annotations
@@ -105,6 +106,8 @@ def product_sim(case, graph=True):
     p.enable_stability_checks = False
     p.precision = sc.Precision.FP32 if case.get("precision") == "fp32" else sc.Precision.FP64
     p.use_graph = graph
+    for k, v in case.get("params", {}).items():     # further SimulationParams fields
+        setattr(p, k, v)
     sim = sc.Simulation(p)
     b = sc.ChannelBuilder(p.nx, p.ny)
     b.addWalls()
