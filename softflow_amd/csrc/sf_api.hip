@@ -120,7 +120,15 @@ static void harvest_timing(Ctx& c) {
 // One step in simulation.cpp:470-787 order.
 static void step_forces(Ctx& c) {
     if (c.ncaps == 0) return;
-    { Scope s(c, "membrane"); launch_membrane(c); }
+    if (contact_uses_cells(c)) {
+        // the cell sort needs only node positions: it shares a launch with the
+        // membrane forces
+        { Scope s(c, "contact_fill"); launch_contact_fill(c); }
+        { Scope s(c, "membrane+sort"); launch_membrane_sort(c); }
+    } else {
+        Scope s(c, "membrane");
+        launch_membrane(c);
+    }
     { Scope s(c, "repulsion"); launch_repulsion(c); }
 }
 

@@ -215,9 +215,20 @@ static int build_plan(sf_batch& B) {
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&B.dpacks), std::max<size_t>(total, 256)));
     SF_HIP(hipMemcpy(B.dpacks, host.data(), total, hipMemcpyHostToDevice));
     for (int k = 0; k < BK_COUNT; ++k) {
+        if (k == BK_CELLS_SORT) continue;     // its workgroups follow the membrane ones
         std::vector<int> blocks(M);
         for (int m = 0; m < M; ++m) blocks[m] = item(0, m, k).blocks;
-        const std::vector<int2> map = make_map(blocks, k == BK_LBM && B.banded);
+        std::vector<int2> map;
+        if (k == BK_MEMBRANE) {
+            // each member's membrane workgroups, then its sort workgroups: the two
+            // kinds are dispatched side by side across the batch
+            for (int m = 0; m < M; ++m) {
+                for (int b = 0; b < blocks[m]; ++b) map.push_back(make_int2(m, b));
+                for (int b = 0; b < item(0, m, BK_CELLS_SORT).blocks; ++b) map.push_back(make_int2(M + m, b));
+            }
+        } else {
+            map = make_map(blocks, k == BK_LBM && B.banded);
+        }
         B.nblk[k] = static_cast<int>(map.size());
         if (map.empty()) continue;
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&B.dmap[k]), map.size() * sizeof(int2)));
@@ -230,9 +241,15 @@ static int build_plan(sf_batch& B) {
     return SF_OK;
 }
 
+// One step: the contact cell fill, then the membrane forces and the cell sorts
+// in one launch (the sort needs only node positions), then the rest in order.
 static void launch_step(sf_batch& B, int s) {
-    for (int k = 0; k < BK_COUNT; ++k)
-        launch_kind(k, B.v, B.dpacks + B.off[s][k], B.dmap[k], B.nblk[k], B.stream);
+    const int M = static_cast<int>(B.mem.size());
+    auto run = [&](int k) { launch_kind(k, B.v, B.dpacks + B.off[s][k], B.dmap[k], B.nblk[k], B.stream); };
+    run(BK_CELLS_FILL);
+    batch_launch_membrane_sort(B.v, B.dpacks + B.off[s][BK_MEMBRANE], B.dpacks + B.off[s][BK_CELLS_SORT],
+                               B.dmap[BK_MEMBRANE], B.nblk[BK_MEMBRANE], M, B.stream);
+    for (int k = BK_CONTACT; k < BK_COUNT; ++k) run(k);
 }
 
 // The plan still describes every member: same buffers, parameters and roles.

@@ -156,6 +156,8 @@ struct Ctx {
     double4* cell_pay = nullptr;         // (x, y, vx, vy) of each listed node, in list order
     int* cell_own = nullptr;             // capsule of each listed node
     int cell_cur = 0;
+    int* cell_built = nullptr;           // count array filled ahead this step (launch_contact_fill)
+    int* cell_other = nullptr;           // the other step's count array, cleared by the sort
 
     // multi-direct forcing control, history and the sample set
     double mdf_tol = 0.0;
@@ -220,6 +222,12 @@ void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, doubl
 void launch_field_stats(Ctx& c, double* dev_out);
 void launch_membrane(Ctx& c);
 void launch_repulsion(Ctx& c);
+// Where the contact cell list applies (x-periodic, more than the all-pairs
+// limit of nodes) a step runs launch_contact_fill, then launch_membrane_sort
+// (membrane forces and the cell sort in one launch), then launch_repulsion.
+bool contact_uses_cells(const Ctx& c);
+void launch_contact_fill(Ctx& c);
+void launch_membrane_sort(Ctx& c);
 void launch_spread(Ctx& c, int mode, bool last, int it = 0);   // 0 node forces into F[Fcur]; 1 MDF correction
 void launch_mdf_interp(Ctx& c, int it);
 void launch_mdf_begin(Ctx& c);
@@ -282,6 +290,10 @@ void batch_items_interp(Ctx& c, BatchItem* it);            // BK_INTERP
 // launch kind over nblocks workgroups of the map
 void batch_launch_membrane(int kind, const BatchVariant& v, const void* packs, const int2* map,
                            int nblocks, hipStream_t s);
+// membrane forces and cell sorts in one launch: map entries (m, b) with m < M are
+// membrane workgroups of member m, (M + m, b) sort workgroups of member m
+void batch_launch_membrane_sort(const BatchVariant& v, const void* mem_packs, const void* sort_packs,
+                                const int2* map, int nblocks, int M, hipStream_t s);
 void batch_launch_ibm(int kind, const BatchVariant& v, const void* packs, const int2* map,
                       int nblocks, hipStream_t s);
 void batch_launch_lbm(const BatchVariant& v, const void* packs, const int2* map, int nblocks,

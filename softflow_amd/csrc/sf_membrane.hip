@@ -630,6 +630,50 @@ static RepArgs rep_args(const Ctx& c) {
 // contacts are summed over all pairs up to this many nodes, over the cell list above
 constexpr int REP_ALLPAIRS_MAX = 100;
 
+bool contact_uses_cells(const Ctx& c) {
+    return c.ncaps > 0 && c.K > REP_ALLPAIRS_MAX && c.periodic_x;
+}
+
+void launch_contact_fill(Ctx& c) {
+    RepArgs a = rep_args(c);
+    a.ncy = c.ncy; a.cdy = c.cdy;
+    a.ncx = c.ncx; a.xorg = 0.0; a.cdx = c.cdx;
+    const int nmax = c.ncx * c.ncy;
+    int* cnt = c.cell_cnt + static_cast<size_t>(c.cell_cur) * nmax;
+    const int bs = 128, nb = (c.K + bs - 1) / bs;
+    hipLaunchKernelGGL(k_cells_fill, dim3(nb), dim3(bs), 0, c.stream, a, cnt, c.cell_list, c.counters);
+    c.cell_built = cnt;
+    c.cell_other = c.cell_cnt + static_cast<size_t>(1 - c.cell_cur) * nmax;
+    c.cell_cur = 1 - c.cell_cur;
+}
+
+// Membrane forces (workgroups [0, ncaps)) and the contact cell sort (the rest,
+// one thread per cell) in one launch of 64-thread workgroups.
+template <int MAXN>
+__global__ __launch_bounds__(64) void k_membrane_sort(MemArgs a, int ncaps, int ncells, int nclear,
+                                                      const int* cnt, int* list, int* other,
+                                                      const double* x, const double* y,
+                                                      const double* vx, const double* vy,
+                                                      const int* owner, double4* pay, int* own) {
+    if (static_cast<int>(blockIdx.x) < ncaps) membrane_body<MAXN>(a, blockIdx.x);
+    else cells_sort_body(ncells, nclear, cnt, list, other, x, y, vx, vy, owner, pay, own,
+                         blockIdx.x - ncaps);
+}
+
+void launch_membrane_sort(Ctx& c) {
+    const MemArgs a = mem_args(c);
+    const int nmax = c.ncx * c.ncy;
+    const dim3 g(c.ncaps + (nmax + 63) / 64);
+    if (c.max_nodes <= 64)
+        hipLaunchKernelGGL(k_membrane_sort<64>, g, dim3(64), 0, c.stream, a, c.ncaps, nmax, nmax,
+                           c.cell_built, c.cell_list, c.cell_other, c.x, c.y, c.vx, c.vy, c.owner,
+                           c.cell_pay, c.cell_own);
+    else
+        hipLaunchKernelGGL(k_membrane_sort<256>, g, dim3(64), 0, c.stream, a, c.ncaps, nmax, nmax,
+                           c.cell_built, c.cell_list, c.cell_other, c.x, c.y, c.vx, c.vy, c.owner,
+                           c.cell_pay, c.cell_own);
+}
+
 void launch_repulsion(Ctx& c) {
     if (c.ncaps == 0) return;
     RepArgs a = rep_args(c);
@@ -653,14 +697,18 @@ void launch_repulsion(Ctx& c) {
         hipStreamSynchronize(c.stream);
         a.ncx = h.ncx; a.xorg = h.xorg; a.cdx = h.cdx;
     }
-    const int ncells = a.ncx * a.ncy;
-    const int nmax = c.ncx * c.ncy;
-    int* cnt = c.cell_cnt + static_cast<size_t>(c.cell_cur) * nmax;
-    int* other = c.cell_cnt + static_cast<size_t>(1 - c.cell_cur) * nmax;
-    hipLaunchKernelGGL(k_cells_fill, dim3(nb), dim3(bs), 0, c.stream, a, cnt, c.cell_list, c.counters);
-    hipLaunchKernelGGL(k_cells_sort, dim3((nmax + 255) / 256), dim3(256), 0, c.stream, ncells, nmax, cnt,
-                       c.cell_list, other, c.x, c.y, c.vx, c.vy, c.owner, c.cell_pay, c.cell_own);
-    c.cell_cur = 1 - c.cell_cur;
+    int* cnt = c.cell_built;
+    c.cell_built = nullptr;
+    if (!cnt) {
+        const int ncells = a.ncx * a.ncy;
+        const int nmax = c.ncx * c.ncy;
+        cnt = c.cell_cnt + static_cast<size_t>(c.cell_cur) * nmax;
+        int* other = c.cell_cnt + static_cast<size_t>(1 - c.cell_cur) * nmax;
+        hipLaunchKernelGGL(k_cells_fill, dim3(nb), dim3(bs), 0, c.stream, a, cnt, c.cell_list, c.counters);
+        hipLaunchKernelGGL(k_cells_sort, dim3((nmax + 255) / 256), dim3(256), 0, c.stream, ncells, nmax, cnt,
+                           c.cell_list, other, c.x, c.y, c.vx, c.vy, c.owner, c.cell_pay, c.cell_own);
+        c.cell_cur = 1 - c.cell_cur;
+    }
     a.cnt = cnt;
     a.pay = c.cell_pay;
     a.own = c.cell_own;
@@ -696,13 +744,6 @@ struct ContactPack {
 constexpr int BATCH_FILL_THREADS = 128;
 constexpr int BATCH_ALLPAIRS_THREADS = 256;
 
-template <int MAXN>
-__global__ __launch_bounds__(64) void kb_membrane(const MemArgs* packs, const int2* map) {
-    const int2 w = map[blockIdx.x];
-    if (w.x < 0) return;
-    membrane_body<MAXN>(packs[w.x], w.y);
-}
-
 __global__ __launch_bounds__(BATCH_FILL_THREADS) void kb_cells_fill(const CellsFillPack* packs,
                                                                    const int2* map) {
     const int2 w = map[blockIdx.x];
@@ -711,12 +752,18 @@ __global__ __launch_bounds__(BATCH_FILL_THREADS) void kb_cells_fill(const CellsF
     cells_fill_body(p.a, p.cnt, p.list, p.counters, w.y);
 }
 
-__global__ __launch_bounds__(256) void kb_cells_sort(const CellsSortPack* packs, const int2* map) {
+template <int MAXN>
+__global__ __launch_bounds__(64) void kb_membrane_sort(const MemArgs* mp, const CellsSortPack* sp,
+                                                       const int2* map, int M) {
     const int2 w = map[blockIdx.x];
     if (w.x < 0) return;
-    const CellsSortPack p = packs[w.x];
-    cells_sort_body(p.ncells, p.nclear, p.cnt, p.list, p.other, p.x, p.y, p.vx, p.vy, p.owner,
-                    p.pay, p.own, w.y);
+    if (w.x < M) {
+        membrane_body<MAXN>(mp[w.x], w.y);
+    } else {
+        const CellsSortPack p = sp[w.x - M];
+        cells_sort_body(p.ncells, p.nclear, p.cnt, p.list, p.other, p.x, p.y, p.vx, p.vy, p.owner,
+                        p.pay, p.own, w.y);
+    }
 }
 
 __global__ __launch_bounds__(256) void kb_contact(const ContactPack* packs, const int2* map) {
@@ -748,7 +795,7 @@ void batch_items_contact(Ctx& c, BatchItem* it) {
     int* other = c.cell_cnt + static_cast<size_t>(1 - c.cell_cur) * nmax;
     batch_put(it[0], (c.K + BATCH_FILL_THREADS - 1) / BATCH_FILL_THREADS,
               CellsFillPack{a, cnt, c.cell_list, c.counters});
-    batch_put(it[1], (nmax + 255) / 256,
+    batch_put(it[1], (nmax + 63) / 64,           // 64-thread workgroups, beside the membrane ones
               CellsSortPack{a.ncx * a.ncy, nmax, cnt, c.cell_list, other, c.x, c.y, c.vx, c.vy,
                             c.owner, c.cell_pay, c.cell_own});
     c.cell_cur = 1 - c.cell_cur;
@@ -758,23 +805,23 @@ void batch_items_contact(Ctx& c, BatchItem* it) {
     batch_put(it[2], (c.K + REP_NODES - 1) / REP_NODES, ContactPack{a, 0});
 }
 
+void batch_launch_membrane_sort(const BatchVariant& v, const void* mem_packs, const void* sort_packs,
+                                const int2* map, int nblocks, int M, hipStream_t s) {
+    if (nblocks <= 0) return;
+    const MemArgs* mp = static_cast<const MemArgs*>(mem_packs);
+    const CellsSortPack* sp = static_cast<const CellsSortPack*>(sort_packs);
+    if (v.maxn <= 64) hipLaunchKernelGGL(kb_membrane_sort<64>, dim3(nblocks), dim3(64), 0, s, mp, sp, map, M);
+    else hipLaunchKernelGGL(kb_membrane_sort<256>, dim3(nblocks), dim3(64), 0, s, mp, sp, map, M);
+}
+
 void batch_launch_membrane(int kind, const BatchVariant& v, const void* packs, const int2* map,
                            int nblocks, hipStream_t s) {
     if (nblocks <= 0) return;
     const dim3 g(nblocks);
     switch (kind) {
-    case BK_MEMBRANE:
-        if (v.maxn <= 64)
-            hipLaunchKernelGGL(kb_membrane<64>, g, dim3(64), 0, s, static_cast<const MemArgs*>(packs), map);
-        else
-            hipLaunchKernelGGL(kb_membrane<256>, g, dim3(64), 0, s, static_cast<const MemArgs*>(packs), map);
-        break;
     case BK_CELLS_FILL:
         hipLaunchKernelGGL(kb_cells_fill, g, dim3(BATCH_FILL_THREADS), 0, s,
                            static_cast<const CellsFillPack*>(packs), map);
-        break;
-    case BK_CELLS_SORT:
-        hipLaunchKernelGGL(kb_cells_sort, g, dim3(256), 0, s, static_cast<const CellsSortPack*>(packs), map);
         break;
     case BK_CONTACT:
         hipLaunchKernelGGL(kb_contact, g, dim3(256), 0, s, static_cast<const ContactPack*>(packs), map);
