@@ -153,12 +153,21 @@ __device__ __forceinline__ int stencil_column(int cx, int lo, int nx) {
     return NO_ROW;
 }
 
-// One lane per tile.  Tiles no node touches: the wavefront clears (or, for an
-// MDF correction, copies from the input) each one that needs it -- those whose
-// output still holds non-zero values, or whose input does; after convergence
-// (skip) every tile counts as untouched.  Touched tiles (mode 0, which runs
-// once per step after the binning): appended to act as (tile, node count),
-// one atomic per wavefront.
+// rest_tpw(mode) tiles per wavefront, one per lane of its first lanes.  Tiles no
+// node touches: the wavefront clears (or, for an MDF correction, copies from
+// the input) each one that needs it -- those whose output still holds non-zero
+// values, or whose input does; after convergence (skip) every tile counts as
+// untouched.  Touched tiles (mode 0, which runs once per step after the
+// binning): appended to act as (tile, node count), one atomic per wavefront.
+// The pass after the binning (mode 0) mostly clears: a whole wavefront of
+// tiles each, so that few wavefronts take part in the touched-tile list.  MDF
+// corrections (mode 1) copy whole tiles once converged: 16 tiles each, for more
+// wavefronts in flight.
+constexpr int REST_TPW0 = 64;
+constexpr int REST_TPW1 = 16;
+__host__ __device__ constexpr int rest_tpw(int mode) { return mode == 0 ? REST_TPW0 : REST_TPW1; }
+__host__ __device__ constexpr int rest_tiles_per_block(int mode) { return 4 * rest_tpw(mode); }
+
 template <bool FP32>
 __device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void* Fin, void* Fout,
                                                      const uint8_t* dirty_in, uint8_t* dirty_out,
@@ -168,7 +177,9 @@ __device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void
     using T = typename std::conditional<FP32, float, double>::type;
     const bool skip = mode == 1 && md && md->conv;
     const int lane = threadIdx.x & 63;
-    const int tile = bx * blockDim.x + threadIdx.x;
+    const int tpw = rest_tpw(mode);
+    const int wave_tile0 = (bx * static_cast<int>(blockDim.x >> 6) + static_cast<int>(threadIdx.x >> 6)) * tpw;
+    const int tile = lane < tpw ? wave_tile0 + lane : ntiles;   // the other lanes hold none
     const int cnt = tile < ntiles ? tile_cnt[tile] : 0;
     bool need = false, copy = false;
     if (tile < ntiles && (skip || cnt == 0)) {
@@ -187,23 +198,47 @@ __device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void
     }
     unsigned long long todo = __ballot(need);
     const unsigned long long cmask = __ballot(copy);
-    const int wave_tile0 = tile - lane;
+    // The wavefront clears or copies its tiles two at a time; every lane issues
+    // all its loads of both tiles before its stores.  A copy reads the input
+    // buffer and writes the output buffer, which is always a different one.
+    using T2 = typename std::conditional<FP32, float2, double2>::type;
+    constexpr int PER = TILE_X * TILE_Y / 64;    // cells per lane and tile
+    const T2* fin = static_cast<const T2*>(Fin);
+    T2* fout = static_cast<T2*>(Fout);
     while (todo) {
-        const int b = __ffsll(static_cast<long long>(todo)) - 1;
-        todo &= todo - 1;
-        const int tt = wave_tile0 + b;
-        const bool cp = (cmask >> b) & 1ull;
-        const int tx0 = (tt % ntx) * TILE_X, ty0 = (tt / ntx) * TILE_Y;
-        for (int c = lane; c < TILE_X * TILE_Y; c += 64) {
-            const int cx = tx0 + (c % TILE_X), cy = ty0 + (c / TILE_X);
-            if (cx >= nx || cy >= ny) continue;
-            const size_t n = static_cast<size_t>(cy) * nx + cx;
-            T vx = 0, vy = 0;
-            if (cp) { vx = static_cast<const T*>(Fin)[2 * n]; vy = static_cast<const T*>(Fin)[2 * n + 1]; }
-            static_cast<T*>(Fout)[2 * n] = vx;
-            static_cast<T*>(Fout)[2 * n + 1] = vy;
+        int tl[2] = {0, 0};
+        bool cps[2] = {false, false};
+        int nt = 0;
+        for (; nt < 2 && todo; ++nt) {
+            const int b = __ffsll(static_cast<long long>(todo)) - 1;
+            todo &= todo - 1;
+            tl[nt] = wave_tile0 + b;
+            cps[nt] = (cmask >> b) & 1ull;
         }
-        if (lane == 0) dirty_out[tt] = cp ? 1 : 0;
+        T2 v[2][PER];
+        size_t at[2][PER];
+        bool in[2][PER];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int tx0 = (tl[t] % ntx) * TILE_X, ty0 = (tl[t] / ntx) * TILE_Y;
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int c = lane + 64 * i;
+                const int cx = tx0 + (c % TILE_X), cy = ty0 + (c / TILE_X);
+                in[t][i] = t < nt && cx < nx && cy < ny;
+                at[t][i] = static_cast<size_t>(cy) * nx + cx;
+                v[t][i].x = T(0);
+                v[t][i].y = T(0);
+                if (in[t][i] && cps[t]) v[t][i] = fin[at[t][i]];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int i = 0; i < PER; ++i)
+                if (in[t][i]) fout[at[t][i]] = v[t][i];
+        if (lane == 0)
+            for (int t = 0; t < nt; ++t) dirty_out[tl[t]] = cps[t] ? 1 : 0;
     }
 }
 
@@ -846,7 +881,7 @@ void launch_spread(Ctx& c, int mode, bool last, int it) {
     const uint8_t* din = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
     uint8_t* dout = c.tile_dirty + static_cast<size_t>(out) * c.ntiles;
     const MdfDev* md = mode == 1 ? c.mdf : nullptr;
-    const dim3 rgrid((c.ntiles + 255) / 256);
+    const dim3 rgrid((c.ntiles + rest_tiles_per_block(mode) - 1) / rest_tiles_per_block(mode));
     const dim3 sgrid(std::min(c.ntiles, SPREAD_GRID));
     if (c.fp32) {
         hipLaunchKernelGGL((k_spread_rest<true>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
@@ -1019,7 +1054,7 @@ void batch_items_spread(Ctx& c, BatchItem* it) {
     void* Fout = c.F[c.Fcur];
     const uint8_t* din = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
     uint8_t* dout = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
-    batch_put(it[1], (c.ntiles + 255) / 256,
+    batch_put(it[1], (c.ntiles + rest_tiles_per_block(0) - 1) / rest_tiles_per_block(0),
               RestPack{c.tile_cnt, Fin, Fout, din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx,
                        c.ntiles});
     const int gx = std::min(c.ntiles, BATCH_SPREAD_GRID);
