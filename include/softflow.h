@@ -144,6 +144,12 @@ int  sf_last_capped(sf_ctx* ctx, int32_t* capped);
  * out[3] = max |u| (getMaxSpeed, simulation.cpp:892-907),
  * out[4] = number of non-finite cells (checkStability, :820-890). */
 int  sf_field_stats(sf_ctx* ctx, double* out5);
+/* FP64 device reductions over the non-solid cells for GlobalStatsWriter
+ * (io/global_stats_writer.cpp:55-80) and checkStability (simulation.cpp:820-890):
+ * out[0] cells, [1] sum rho, [2] min rho, [3] max rho, [4] max |u|^2,
+ * [5] sum |u|, [6] sum rho |u|^2 / 2, [7] non-finite cells (left out of the
+ * rest), [8] the density farthest from rho0, [9] sum rho ux, [10] sum rho uy. */
+int  sf_global_stats(sf_ctx* ctx, double rho0, double* out11);
 int  sf_step_count(sf_ctx* ctx, int64_t* n);
 
 /* Per-kernel timing of the last sf_step call, measured with HIP events on the

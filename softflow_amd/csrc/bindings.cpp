@@ -323,6 +323,11 @@ PYBIND11_MODULE(softflow_core, m) {
         .def("lastCappedNodes", &Simulation::lastCappedNodes)
         .def("checkStability", &Simulation::checkStability)
         .def("fieldStats", &Simulation::fieldStats)
+        .def("globalStats", [](Simulation& s) {
+            py::dict d;
+            for (auto& kv : s.globalStats()) d[py::str(kv.first)] = kv.second;
+            return d;
+        })
         .def("saveCheckpoint", &Simulation::saveCheckpoint, py::arg("filename"))
         .def("loadCheckpoint", &Simulation::loadCheckpoint, py::arg("filename"))
         .def("setCurrentStep", &Simulation::setCurrentStep, py::arg("step"))

@@ -644,34 +644,72 @@ int Simulation::lastCappedNodes() {
     return c;
 }
 
-// simulation.cpp:820-890 on the host mirror
+// simulation.cpp:820-890, from FP64 device reductions (no copy of the field)
 bool Simulation::checkStability() {
-    LatticeField& F = *field_;
-    F.pull();
+    pushDirty();
     const Real rho0 = params_.fluid.rho0;
-    bool nan = false, dwarn = false, vwarn = false;
-    Real worst_rho = rho0, worst_vel = 0.0;
-    for (int n = 0; n < F.N_ && !nan; ++n) {
-        if (F.flags_[n] == static_cast<int>(CellType::SOLID) || F.flags_[n] == static_cast<int>(CellType::EMPTY)) continue;
-        const Real r = F.rho_[n], u = F.ux_[n], v = F.uy_[n];
-        if (!std::isfinite(r) || !std::isfinite(u) || !std::isfinite(v)) {
-            std::cerr << "*** STABILITY WARNING: NaN/Inf detected at step " << step_count_ << std::endl;
-            nan = true;
-            break;
-        }
-        if (std::abs(r - rho0) / rho0 > params_.max_density_deviation) {
-            if (!dwarn || std::abs(r - rho0) > std::abs(worst_rho - rho0)) worst_rho = r;
-            dwarn = true;
-        }
-        const Real vel = std::sqrt(u * u + v * v);
-        if (vel > params_.max_velocity) { if (vel > worst_vel) worst_vel = vel; vwarn = true; }
-    }
-    if (dwarn) std::cerr << "*** STABILITY WARNING: density deviation at step " << step_count_ << " - rho = " << worst_rho << std::endl;
-    if (vwarn) std::cerr << "*** STABILITY WARNING: high velocity at step " << step_count_ << " - |u| = " << worst_vel << std::endl;
+    double g[11];
+    check(sf_global_stats(ctx_, rho0, g), "sf_global_stats");
+    const bool nan = g[7] > 0.0;
+    const bool dwarn = std::abs(g[8] - rho0) / rho0 > params_.max_density_deviation;
+    const Real worst_vel = std::sqrt(g[4]);
+    const bool vwarn = worst_vel > params_.max_velocity;
+    if (nan)
+        std::cerr << "*** STABILITY WARNING: NaN/Inf detected at step " << step_count_ << " in "
+                  << static_cast<long long>(g[7]) << " cells" << std::endl;
+    if (dwarn)
+        std::cerr << "*** STABILITY WARNING: density deviation at step " << step_count_ << " - rho = " << g[8]
+                  << " (rho0 = " << rho0 << ", deviation = " << std::abs(g[8] - rho0) / rho0 * 100.0 << "%)"
+                  << std::endl;
+    if (vwarn)
+        std::cerr << "*** STABILITY WARNING: high velocity at step " << step_count_ << " - |u| = " << worst_vel
+                  << " (limit = " << params_.max_velocity << ", Ma = " << worst_vel / std::sqrt(1.0 / 3.0) << ")"
+                  << std::endl;
     const int capped = lastCappedNodes();
     if (capped > 0)
-        std::cerr << "*** STABILITY NOTICE: max_lattice_force cap rescaled " << capped << " fluid nodes at step " << step_count_ << std::endl;
+        std::cerr << "*** STABILITY NOTICE: max_lattice_force cap rescaled " << capped << " fluid nodes at step "
+                  << step_count_ << " (cap = " << params_.max_lattice_force << ")" << std::endl;
     return !(nan || dwarn || vwarn);
+}
+
+std::vector<std::pair<std::string, double>> Simulation::globalStats() {
+    pushDirty();
+    double g[11];
+    check(sf_global_stats(ctx_, params_.fluid.rho0, g), "sf_global_stats");
+    const double n = g[0];
+    std::vector<std::pair<std::string, double>> out;
+    out.emplace_back("timestep", static_cast<double>(step_count_));
+    out.emplace_back("time", static_cast<double>(step_count_) * params_.dt);
+    out.emplace_back("mean_rho", n > 0 ? g[1] / n : 0.0);
+    out.emplace_back("max_velocity", std::sqrt(g[4]));
+    out.emplace_back("mean_velocity", n > 0 ? g[5] / n : 0.0);
+    out.emplace_back("total_kinetic_energy", g[6]);
+    std::vector<Capsule>& cs = caps_.capsules();
+    double mean_def = 0.0, max_def = 0.0;
+    for (Capsule& c : cs) {
+        const double D = c.deformationIndex();
+        mean_def += D;
+        if (D > max_def) max_def = D;
+    }
+    if (!cs.empty()) mean_def /= static_cast<double>(cs.size());
+    out.emplace_back("n_active_particles", static_cast<double>(cs.size()));
+    out.emplace_back("mean_deformation", mean_def);
+    out.emplace_back("max_deformation", max_def);
+    out.emplace_back("min_rho", n > 0 ? g[2] : 1e30);
+    out.emplace_back("max_rho", n > 0 ? g[3] : -1e30);
+    if (adr_) {
+        // every cell and species, as the writer sums them
+        double mass = 0.0;
+        for (int s = 0; s < adr_->nsp_; ++s) {
+            const Real* C = adr_->concentrationData(s);
+            for (int k = 0; k < params_.nx * params_.ny; ++k) mass += C[k];
+        }
+        out.emplace_back("total_dissolved_mass", mass);
+        out.emplace_back("total_particle_mass", 0.0);
+        out.emplace_back("total_adsorbed_mass", 0.0);
+    }
+    out.emplace_back("nonfinite_cells", g[7]);
+    return out;
 }
 
 // LBMSolver::setFluidRegion: lbm_solver.cpp:339-350

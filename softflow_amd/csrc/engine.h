@@ -325,6 +325,9 @@ public:
     int lastCappedNodes();
     bool checkStability();
     std::vector<double> fieldStats();
+    // the quantities of GlobalStatsWriter::writeTimestep (io/global_stats_writer.cpp:41-124),
+    // from FP64 device reductions; scalar masses from the scalar mirror
+    std::vector<std::pair<std::string, double>> globalStats();
     LatticeField& field() { return *field_; }
     CapsuleSystem& capsules() { return caps_; }
     AdvectionDiffusion* advectionDiffusion() { return adr_.get(); }

@@ -821,6 +821,20 @@ int sf_field_stats(sf_ctx* h, double* out5) {
     return SF_OK;
 }
 
+int sf_global_stats(sf_ctx* h, double rho0, double* out11) {
+    if (!h || !out11) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    double* buf = nullptr;
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&buf), (16 + 3 * static_cast<size_t>(c.N) + 11 * 256) * 8));
+    launch_global_stats(c, rho0, buf);
+    hipError_t e = hipStreamSynchronize(c.stream);
+    if (e == hipSuccess) e = hipMemcpy(out11, buf, 11 * 8, hipMemcpyDeviceToHost);
+    hipFree(buf);
+    SF_HIP(e);
+    return SF_OK;
+}
+
 int sf_step_count(sf_ctx* h, int64_t* n) {
     if (!h || !n) return fail(SF_E_ARG, "null pointer");
     *n = h->c.step_count;

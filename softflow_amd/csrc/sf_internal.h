@@ -220,6 +220,7 @@ void launch_lbm_cfg(Ctx& c, int shift_mode, bool fuse_adr, int bx, int by, int r
 void launch_adr(Ctx& c, int Fidx);
 void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy);
 void launch_field_stats(Ctx& c, double* dev_out);
+void launch_global_stats(Ctx& c, double rho0, double* dev_out);
 void launch_membrane(Ctx& c);
 void launch_repulsion(Ctx& c);
 // Where the contact cell list applies (x-periodic, more than the all-pairs

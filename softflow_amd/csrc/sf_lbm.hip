@@ -789,6 +789,75 @@ __global__ void k_stats_final(const double* part, int nb, double* out) {
     out[0] = m; out[1] = px; out[2] = py; out[3] = sqrt(um); out[4] = bad;
 }
 
+// Global statistics over the non-solid cells (io/global_stats_writer.cpp:55-80,
+// simulation.cpp:820-907), FP64: count, sum rho, min rho, max rho, max |u|^2,
+// sum |u|, sum rho |u|^2 / 2, non-finite cells, the density farthest from
+// rho0, sum rho ux, sum rho uy.  Non-finite cells are counted and left out of
+// the rest.
+constexpr int GSTATS = 11;
+
+__device__ __forceinline__ void gstats_combine(double* a, const double* b, double rho0) {
+    a[0] += b[0]; a[1] += b[1];
+    a[2] = fmin(a[2], b[2]); a[3] = fmax(a[3], b[3]); a[4] = fmax(a[4], b[4]);
+    a[5] += b[5]; a[6] += b[6]; a[7] += b[7];
+    if (fabs(b[8] - rho0) > fabs(a[8] - rho0)) a[8] = b[8];
+    a[9] += b[9]; a[10] += b[10];
+}
+
+__global__ void k_gstats_partial(const double* rho, const double* ux, const double* uy,
+                                 const uint8_t* flags, LbmArgs a, double rho0, double* part) {
+    __shared__ double s[GSTATS][256];
+    const int t = threadIdx.x;
+    double v[GSTATS] = {0.0, 0.0, 1e300, -1e300, 0.0, 0.0, 0.0, 0.0, rho0, 0.0, 0.0};
+    for (int n = blockIdx.x * blockDim.x + t; n < a.N; n += gridDim.x * blockDim.x) {
+        const int x = n % a.nx, y = n / a.nx;
+        if (is_solid(a, x, y, flags != nullptr)) continue;
+        const double r = rho[n], u = ux[n], w = uy[n];
+        if (!isfinite(r) || !isfinite(u) || !isfinite(w)) { v[7] += 1.0; continue; }
+        const double v2 = u * u + w * w;
+        v[0] += 1.0; v[1] += r;
+        v[2] = fmin(v[2], r); v[3] = fmax(v[3], r); v[4] = fmax(v[4], v2);
+        v[5] += sqrt(v2); v[6] += 0.5 * r * v2;
+        if (fabs(r - rho0) > fabs(v[8] - rho0)) v[8] = r;
+        v[9] += r * u; v[10] += r * w;
+    }
+    for (int k = 0; k < GSTATS; ++k) s[k][t] = v[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            double a0[GSTATS], b0[GSTATS];
+            for (int k = 0; k < GSTATS; ++k) { a0[k] = s[k][t]; b0[k] = s[k][t + h]; }
+            gstats_combine(a0, b0, rho0);
+            for (int k = 0; k < GSTATS; ++k) s[k][t] = a0[k];
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < GSTATS; ++k) part[blockIdx.x * GSTATS + k] = s[k][0];
+}
+
+__global__ void k_gstats_final(const double* part, int nb, double rho0, double* out) {
+    if (threadIdx.x != 0) return;
+    double a0[GSTATS];
+    for (int k = 0; k < GSTATS; ++k) a0[k] = part[k];
+    for (int b = 1; b < nb; ++b) gstats_combine(a0, part + b * GSTATS, rho0);
+    for (int k = 0; k < GSTATS; ++k) out[k] = a0[k];
+}
+
+void launch_global_stats(Ctx& c, double rho0, double* dev_out) {
+    // dev_out: GSTATS results (padded to 16), then 3N + GSTATS * nb doubles of scratch
+    double* rho = dev_out + 16;
+    double* ux = rho + c.N;
+    double* uy = ux + c.N;
+    double* part = uy + c.N;
+    launch_macro(c, rho, ux, uy, nullptr, nullptr);
+    LbmArgs a = view_args(c);
+    const int nb = 256;
+    hipLaunchKernelGGL(k_gstats_partial, dim3(nb), dim3(256), 0, c.stream, rho, ux, uy,
+                       c.generic_flags ? c.flags : nullptr, a, rho0, part);
+    hipLaunchKernelGGL(k_gstats_final, dim3(1), dim3(64), 0, c.stream, part, nb, rho0, dev_out);
+}
+
 void launch_field_stats(Ctx& c, double* dev_out) {
     // dev_out holds 5 results followed by scratch: 3N + 5*nb doubles
     double* rho = dev_out + 8;
