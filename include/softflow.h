@@ -53,6 +53,8 @@ typedef struct {
     double inlet_concentration[4];
     int    scalar_periodic_y;
     int    precision;               /* SF_FP64 or SF_FP32 */
+    double top_wall_velocity;       /* moving walls (lbm/moving_wall.cpp): x velocity of the */
+    double bottom_wall_velocity;    /* solid rows y = ny-1 and y = 0 */
 } sf_params;
 
 /* Per-capsule constants resolved on the host (rest geometry and WLC constants

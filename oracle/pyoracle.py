@@ -35,6 +35,7 @@ class Params(C.Structure):
         ("scalar_enabled", C.c_int), ("n_species", C.c_int),
         ("diffusivity", C.c_double * 4), ("inlet_concentration", C.c_double * 4),
         ("scalar_periodic_y", C.c_int),
+        ("top_wall_velocity", C.c_double), ("bottom_wall_velocity", C.c_double),
     ]
 
 

@@ -61,6 +61,8 @@ static SimulationParams to_params(const sfo_params* p) {
     s.scalar.inlet_concentration.assign(p->inlet_concentration,
                                         p->inlet_concentration + p->n_species);
     s.scalar.periodic_y = p->scalar_periodic_y != 0;
+    s.fluid.top_wall_velocity = p->top_wall_velocity;
+    s.fluid.bottom_wall_velocity = p->bottom_wall_velocity;
     return s;
 }
 

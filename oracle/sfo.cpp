@@ -856,6 +856,35 @@ void build_neighbors(sfo_sim& s) {                             // :72-104
         }
 }
 
+// MovingWall::apply (lbm/moving_wall.cpp:14-63), after the bounce-back: for a
+// wall row moving with velocity U along x, each link from a non-solid cell
+// into a solid cell of that row returns f_q - 2 w_q rho (c_q . U) / cs^2, with
+// rho the fluid cell's density from the last macroscopic pass.
+static void moving_wall_row(sfo_sim& s, int yw, double U) {
+    const int nx = s.nx, ny = s.ny, N = s.N;
+    const double cs2 = 1.0 / 3.0;
+    for (int x = 0; x < nx; ++x) {
+        if (s.flags[yw * nx + x] != SOLID) continue;
+        for (int q = 1; q < Q; ++q) {
+            int xn = x - CX[q];
+            const int yn = yw - CY[q];
+            if (s.periodic_x) xn = (xn + nx) % nx;
+            if (xn < 0 || xn >= nx || yn < 0 || yn >= ny) continue;
+            const int m = yn * nx + xn;
+            if (s.flags[m] == SOLID) continue;
+            const double eu = CX[q] * U;
+            s.f[OPP[q] * N + m] = s.f[q * N + yw * nx + x] - 2.0 * W[q] * s.rho[m] * eu / cs2;
+        }
+    }
+}
+
+static void moving_walls(sfo_sim& s) {
+    const double top = s.p.top_wall_velocity, bot = s.p.bottom_wall_velocity;
+    if (!(std::abs(top) > 1e-15 || std::abs(bot) > 1e-15)) return;
+    if (std::abs(bot) > 1e-15) moving_wall_row(s, 0, bot);
+    if (std::abs(top) > 1e-15) moving_wall_row(s, s.ny - 1, top);
+}
+
 void lbm_step(sfo_sim& s) {
     const int N = s.N;
     const sfo_params& p = s.p;
@@ -887,6 +916,7 @@ void lbm_step(sfo_sim& s) {
             s.f[OPP[q] * N + yn * s.nx + xn] = s.f[q * N + sn];
         }
     }
+    moving_walls(s);
     macroscopic(s);
 }
 

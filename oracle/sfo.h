@@ -39,6 +39,8 @@ typedef struct {
     double diffusivity[4];
     double inlet_concentration[4];
     int    scalar_periodic_y;
+    /* moving walls: x velocity of the solid rows y = ny-1 and y = 0 */
+    double top_wall_velocity, bottom_wall_velocity;
 } sfo_params;
 
 typedef struct {

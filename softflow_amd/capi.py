@@ -51,6 +51,7 @@ class Params(C.Structure):
         ("scalar_enabled", C.c_int), ("n_species", C.c_int),
         ("diffusivity", C.c_double * 4), ("inlet_concentration", C.c_double * 4),
         ("scalar_periodic_y", C.c_int), ("precision", C.c_int),
+        ("top_wall_velocity", C.c_double), ("bottom_wall_velocity", C.c_double),
     ]
 
 

@@ -414,8 +414,6 @@ void Simulation::initialize() {
         throw std::invalid_argument("Simulation: Zou-He inlet/outlet boundaries are not implemented on the MI355X path; use PERIODIC or CLOSED");
     if ((P.fluid.gravity_x != 0.0 || P.fluid.gravity_y != 0.0))
         throw std::invalid_argument("Simulation: gravity is not implemented on the MI355X path");
-    if (P.fluid.top_wall_velocity != 0.0 || P.fluid.bottom_wall_velocity != 0.0)
-        throw std::invalid_argument("Simulation: moving walls are not implemented on the MI355X path");
     if (P.lubrication.enabled || P.adhesion.enabled)
         throw std::invalid_argument("Simulation: lubrication/adhesion are not implemented on the MI355X path");
     if (P.fluid.use_interpolated_bb)
@@ -445,6 +443,8 @@ void Simulation::initialize() {
     }
     sp.scalar_periodic_y = P.scalar.periodic_y ? 1 : 0;
     sp.precision = static_cast<int>(P.precision);
+    sp.top_wall_velocity = P.fluid.top_wall_velocity;
+    sp.bottom_wall_velocity = P.fluid.bottom_wall_velocity;
 
     if (ctx_) { sf_destroy(ctx_); ctx_ = nullptr; }
     check(sf_create(&ctx_, P.device, &sp), "sf_create");

@@ -255,6 +255,9 @@ struct LbmArgs {
     // 8 * persist workgroups; the persist workgroups of an XCD walk its band
     // with stride persist.
     int nbx, nblk, per_xcd, remap, persist;
+    // moving walls (lbm/moving_wall.cpp): x velocities of the solid rows y = 0 and y = ny-1
+    int moving;
+    double wall_bot, wall_top;
 };
 
 __device__ __forceinline__ bool block_origin(const LbmArgs& a, int b, int& x, int& y) {
@@ -378,6 +381,7 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
             T f[9];
 #pragma unroll
             for (int q = 0; q < 9; ++q) f[q] = ld_nt(src + q * N + n);
+            double rho_wall = 0.0;    // the density a moving wall reflects with
             if constexpr (!FP32) {
                 double r = 0.0, vx = 0.0, vy = 0.0;
 #pragma unroll
@@ -402,6 +406,7 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                     ux = 0.0;
                     uy = 0.0;
                 }
+                rho_wall = r;
                 if constexpr (ADRF) adr_cell<false, FLAGS>(a, x, y, n, ux, uy);
                 if (OP == SF_REGULARIZED) col_reg(f, a.om, r, ux, uy, Fx, Fy);
                 else if (OP == SF_MRT) col_mrt(f, a.om, a.se, a.seps, a.sq, r, ux, uy, Fx, Fy);
@@ -424,6 +429,7 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                     ux = (jx + 0.5f * sx) * inv;
                     uy = (jy + 0.5f * sy) * inv;
                 }
+                rho_wall = static_cast<double>(rho);
                 if constexpr (ADRF) adr_cell<true, FLAGS>(a, x, y, n, static_cast<double>(ux), static_cast<double>(uy));
                 const float om = static_cast<float>(a.om);
                 if (OP == SF_REGULARIZED) dcol_reg(f, om, drho, rho, ux, uy, Fx, Fy);
@@ -442,8 +448,22 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                 if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
                 else oob |= (yd < 0 || yd >= a.ny);
                 if (!oob) {
-                    if (is_solid(a, xd, yd, FLAGS)) st_nt(dst + dopp(q) * N + n, f[q]);
-                    else st_nt(dst + q * N + (yd * a.nx + xd), f[q]);
+                    if (is_solid(a, xd, yd, FLAGS)) {
+                        T v = f[q];
+                        if (a.moving && (yd == 0 || yd == a.ny - 1)) {
+                            // a moving wall row (moving_wall.cpp:20-38): minus
+                            // 2 w_q rho (c_q . U) / cs^2, in the reference's order
+                            const double U = yd == 0 ? a.wall_bot : a.wall_top;
+                            if (fabs(U) > 1e-15) {
+                                const double eu = static_cast<double>(dcx(q)) * U;
+                                v = static_cast<T>(static_cast<double>(f[q]) -
+                                                   2.0 * dw(q) * rho_wall * eu / (1.0 / 3.0));
+                            }
+                        }
+                        st_nt(dst + dopp(q) * N + n, v);
+                    } else {
+                        st_nt(dst + q * N + (yd * a.nx + xd), f[q]);
+                    }
                 }
                 if (q != 0) {
                     const int xs = x - dcx(q), ys = y - dcy(q);
@@ -533,6 +553,9 @@ static LbmArgs lbm_args(Ctx& c, int shift, bool fuse_adr) {
     a.fmax = c.p.max_lattice_force; a.fmax2 = a.fmax * a.fmax;
     a.om = 1.0 / c.p.tau;
     a.se = c.p.mrt_se; a.seps = c.p.mrt_s_eps; a.sq = c.p.mrt_sq;
+    a.wall_bot = c.p.bottom_wall_velocity;
+    a.wall_top = c.p.top_wall_velocity;
+    a.moving = (fabs(a.wall_bot) > 1e-15 || fabs(a.wall_top) > 1e-15) ? 1 : 0;
     if (fuse_adr) scalar_args(c, a);
     return a;
 }

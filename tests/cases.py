@@ -10,6 +10,7 @@ A case is a dict:
     solids: list of ('rect', x0, y0, x1, y1) | ('circle', cx, cy, r): extra SOLID
             cells set after initialize (LatticeField::setCellType)   (optional)
     params: {name: value} further SimulationParams fields (product only)   (optional)
+    walls: (top, bottom) x velocities of moving wall rows   (optional)
 Membrane keys follow the oracle's struct (model, G_s, C_skalak, k_bend, use_helfrich, ...).
 """
 from __future__ import annotations
@@ -58,6 +59,8 @@ def oracle_sim(case, which="oracle"):
         diffusivity=[s["D"]] if s else [0.01],
         rep_damping_normal=rep.get("damping_normal", 0.0),
         rep_friction_coeff=rep.get("friction_coeff", 0.0),
+        top_wall_velocity=case.get("walls", (0.0, 0.0))[0],
+        bottom_wall_velocity=case.get("walls", (0.0, 0.0))[1],
     )
     sim = po.Sim(p, which)
     for spec in case.get("capsules", []):
@@ -100,6 +103,7 @@ def product_sim(case, graph=True):
     p.ibm_iterations = case.get("mdf", 1)
     p.repulsion.damping_normal = rep.get("damping_normal", 0.0)
     p.repulsion.friction_coeff = rep.get("friction_coeff", 0.0)
+    p.fluid.top_wall_velocity, p.fluid.bottom_wall_velocity = case.get("walls", (0.0, 0.0))
     if s:
         p.scalar.enabled = True
         p.scalar.diffusivity = [s["D"]]

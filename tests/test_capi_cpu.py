@@ -143,7 +143,7 @@ def test_random_placement_and_rest_geometry_match_oracle(shape, nodes):
         assert cp.deformationIndex() == o.geometry(c, 4)
 
 
-@pytest.mark.parametrize("what", ["inlet", "gravity", "moving_wall", "lubrication", "ibb"])
+@pytest.mark.parametrize("what", ["inlet", "gravity", "lubrication", "ibb"])
 def test_unsupported_features_refused(what):
     from softflow_amd import softflow_core as sc
     p = sc.SimulationParams()
@@ -153,8 +153,6 @@ def test_unsupported_features_refused(what):
         p.fluid.boundary_type = sc.BoundaryType.INLET_OUTLET
     elif what == "gravity":
         p.fluid.gravity_y = -1e-6
-    elif what == "moving_wall":
-        p.fluid.top_wall_velocity = 0.01
     elif what == "lubrication":
         p.lubrication.enabled = True
     else:

@@ -110,6 +110,19 @@ def test_helfrich_mrt_scalar_fp64():
     _compare(case, 600, 1e-12)
 
 
+@pytest.mark.parametrize("variant", ["channel_bgk", "closed_mrt", "caps_reg_mdf3"])
+def test_moving_walls_fp64(variant):
+    # moving wall rows (lbm/moving_wall.cpp) after the bounce-back
+    case = {"channel_bgk": dict(nx=64, ny=30, tau=0.8, bfx=1e-5, coll=0, walls=(0.02, -0.01)),
+            "closed_mrt": dict(nx=48, ny=30, tau=0.7, coll=1, boundary="closed", walls=(0.03, 0.0)),
+            "caps_reg_mdf3": dict(nx=96, ny=34, tau=0.8, bfx=3e-5, coll=2, mdf=3, walls=(0.01, -0.01),
+                                  capsules=[dict(count=5, region=(8, 6, 88, 28), rmin=3.0, rmax=3.5,
+                                                 nodes=20, seed=4, mem=SKALAK)])}[variant]
+    case["name"] = "walls_" + variant
+    got, ref = _compare(case, 1500, 1e-12)
+    assert np.array_equal(got["ux"], ref["ux"]) or "caps" in variant
+
+
 def test_graph_replay_matches_eager():
     case = dict(name="graph", nx=64, ny=30, tau=0.8, bfx=3e-5, coll=2, mdf=2,
                 capsules=[dict(count=4, region=(8, 8, 56, 22), rmin=3.0, rmax=3.0, nodes=16,

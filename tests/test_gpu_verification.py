@@ -18,7 +18,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def make(nx, ny, tau=0.8, coll=0, G=0.0, periodic_y=False, precision="fp64", D=None):
+def make(nx, ny, tau=0.8, coll=0, G=0.0, periodic_y=False, precision="fp64", D=None, walls=(0.0, 0.0)):
     from softflow_amd import softflow_core as sc
     p = sc.SimulationParams()
     p.nx, p.ny = nx, ny
@@ -26,6 +26,7 @@ def make(nx, ny, tau=0.8, coll=0, G=0.0, periodic_y=False, precision="fp64", D=N
     p.fluid.boundary_type = sc.BoundaryType.PERIODIC
     p.fluid.periodic_y = periodic_y
     p.fluid.body_force_x = G
+    p.fluid.top_wall_velocity, p.fluid.bottom_wall_velocity = walls
     p.fluid.collision_model = [sc.CollisionModel.BGK, sc.CollisionModel.MRT,
                                sc.CollisionModel.REGULARIZED][coll]
     if D is not None:
@@ -61,6 +62,22 @@ def test_poiseuille_profile_and_flow_rate(precision, coll, tau):
     l2 = math.sqrt(np.sum((u - ua) ** 2) / np.sum(ua ** 2))
     q = u.sum() / (G * H ** 3 / (12 * nu))
     assert l2 < 0.02 and abs(q - 1) < 0.02, (l2, q)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("coll", [0, 1, 2])
+def test_couette_profile(precision, coll):
+    # moving wall rows at Ub and Ut (lbm/moving_wall.cpp); the steady profile
+    # between the half-way walls is linear
+    nx, ny, tau, Ut, Ub = 6, 22, 0.8, 0.02, -0.01
+    H = ny - 2
+    sim = make(nx, ny, tau, coll, 0.0, precision=precision, walls=(Ut, Ub))
+    sim.run(int(8 * H * H / ((tau - 0.5) / 3)))
+    u = ux(sim)[1:H + 1, nx // 2]
+    ua = Ub + (Ut - Ub) * (np.arange(H) + 0.5) / H
+    err = np.max(np.abs(u - ua)) / abs(Ut - Ub)
+    print(f"couette {precision} coll={coll}: max rel err {err:.2e}", flush=True)
+    assert err < (1e-6 if precision == "fp64" else 1e-4)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

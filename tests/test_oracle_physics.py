@@ -157,6 +157,27 @@ def test_poiseuille_profile_and_flow_rate(coll, tau):
     assert l2 < 0.02 and abs(q - 1) < 0.02, (l2, q)
 
 
+@pytest.mark.parametrize("coll", [0, 1, 2])
+def test_couette_profile(coll):
+    # plane Couette flow between moving wall rows (lbm/moving_wall.cpp): the
+    # half-way walls sit at y = 0.5 and y = ny - 1.5 and move at Ub and Ut; the
+    # steady profile is linear, which the moving-wall bounce-back keeps exact
+    nx, ny, tau, Ut, Ub = 6, 22, 0.8, 0.02, -0.01
+    nu = (tau - 0.5) / 3
+    H = ny - 2
+    p = po.default_params(nx=nx, ny=ny, tau=tau, boundary_type=1, body_force_x=0.0,
+                          collision_model=coll, top_wall_velocity=Ut, bottom_wall_velocity=Ub)
+    s = po.Sim(p)
+    s.initialize()
+    s.step(int(8 * H * H / nu))
+    u = s.field()["ux"][1:H + 1, nx // 2]
+    ua = Ub + (Ut - Ub) * (np.arange(H) + 0.5) / H
+    err = np.max(np.abs(u - ua)) / abs(Ut - Ub)
+    assert err < 1e-6, err
+    # the walls drag the fluid: the mean velocity is the walls' mean
+    assert abs(u.mean() - 0.5 * (Ut + Ub)) < 1e-6 * abs(Ut - Ub)
+
+
 def test_periodic_channel_translation_invariant():
     # test_poiseuille_collision_operators.cpp:115-169
     def run(nx):

@@ -17,11 +17,12 @@ pytestmark = pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not 
 
 def run(which, coll=0, mdf=1, model=2, helf=0, ncap=8, steps=150, scalar=0, nx=100, ny=40,
         seed=42, gvisc=0.01, damp=0.0, fric=0.0, shape=0, boundary=1, periodic_y=0, nodes=0,
-        eta=0.0, k_area=None, k_per=None):
+        eta=0.0, k_area=None, k_per=None, walls=(0.0, 0.0)):
     p = po.default_params(nx=nx, ny=ny, tau=0.8, boundary_type=boundary, body_force_x=3e-5,
                           collision_model=coll, ibm_iterations=mdf, scalar_enabled=scalar,
                           diffusivity=[0.05], rep_damping_normal=damp, rep_friction_coeff=fric,
-                          periodic_y=periodic_y)
+                          periodic_y=periodic_y, top_wall_velocity=walls[0],
+                          bottom_wall_velocity=walls[1])
     s = po.Sim(p, which)
     kw = dict(model=model, G_s=0.06, C_skalak=10.0, use_helfrich=helf,
               k_bend=1e-3 if helf else 0.005, gamma_visc=gvisc, shape=shape, aspect_ratio=0.6,
@@ -64,6 +65,9 @@ CASES = dict(
     area_perimeter_penalty=dict(model=2, ncap=6, k_area=0.2, k_per=0.1),
     allpairs_few_nodes=dict(ncap=2, nodes=12),
     dem_contact=dict(ncap=12, damp=0.01, fric=0.1),
+    moving_walls=dict(coll=0, ncap=0, walls=(0.02, -0.01)),
+    moving_walls_closed_mrt=dict(coll=1, ncap=0, boundary=2, walls=(0.03, 0.0)),
+    moving_walls_caps_mdf3=dict(coll=2, mdf=3, ncap=6, walls=(0.01, -0.01)),
 )
 
 
