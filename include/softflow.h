@@ -39,7 +39,7 @@ typedef struct {
     double dt;
     int    ibm_iterations;          /* parameters.h:286 */
     double tau, rho0;
-    int    boundary_type;           /* SF_PERIODIC or SF_CLOSED */
+    int    boundary_type;           /* SF_PERIODIC, SF_CLOSED or SF_INLET_OUTLET */
     double body_force_x, body_force_y;
     int    collision_model;         /* SF_BGK / SF_MRT / SF_REGULARIZED */
     double mrt_se, mrt_s_eps, mrt_sq;
@@ -55,6 +55,8 @@ typedef struct {
     int    precision;               /* SF_FP64 or SF_FP32 */
     double top_wall_velocity;       /* moving walls (lbm/moving_wall.cpp): x velocity of the */
     double bottom_wall_velocity;    /* solid rows y = ny-1 and y = 0 */
+    double inlet_velocity;          /* SF_INLET_OUTLET (lbm/zou_he_boundary.cpp): peak of the */
+    double outlet_density;          /* parabolic inlet profile; outlet density */
 } sf_params;
 
 /* Per-capsule constants resolved on the host (rest geometry and WLC constants
@@ -79,7 +81,8 @@ int  sf_device_count(int* n);
 int  sf_create(sf_ctx** ctx, int device, const sf_params* p);
 int  sf_destroy(sf_ctx* ctx);
 
-/* Cell types (LatticeField::cellTypeData, lattice_field.h:87).  flags[N]. */
+/* Cell types (LatticeField::cellTypeData, lattice_field.h:87).  flags[N]:
+ * FLUID, SOLID, INLET, OUTLET (EMPTY is refused). */
 int  sf_set_flags(sf_ctx* ctx, const int32_t* flags);
 
 /* Populations in the reference layout (LatticeField::fData, lattice_field.h:41). */

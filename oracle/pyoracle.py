@@ -36,6 +36,7 @@ class Params(C.Structure):
         ("diffusivity", C.c_double * 4), ("inlet_concentration", C.c_double * 4),
         ("scalar_periodic_y", C.c_int),
         ("top_wall_velocity", C.c_double), ("bottom_wall_velocity", C.c_double),
+        ("inlet_velocity", C.c_double), ("outlet_density", C.c_double),
     ]
 
 

@@ -63,6 +63,8 @@ static SimulationParams to_params(const sfo_params* p) {
     s.scalar.periodic_y = p->scalar_periodic_y != 0;
     s.fluid.top_wall_velocity = p->top_wall_velocity;
     s.fluid.bottom_wall_velocity = p->bottom_wall_velocity;
+    s.fluid.inlet_velocity = p->inlet_velocity;
+    s.fluid.outlet_density = p->outlet_density;
     return s;
 }
 

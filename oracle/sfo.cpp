@@ -885,6 +885,42 @@ static void moving_walls(sfo_sim& s) {
     if (std::abs(top) > 1e-15) moving_wall_row(s, s.ny - 1, top);
 }
 
+// ZouHeBoundary::apply over the inlet and outlet node lists
+// (lbm/zou_he_boundary.cpp:18-75): a parabolic velocity inlet on the INLET
+// cells of column 0 and a fixed-density outlet on the OUTLET cells of column
+// nx-1, each rebuilding the three populations that stream in from outside.
+static void zou_he(sfo_sim& s) {
+    const int nx = s.nx, ny = s.ny, N = s.N;
+    auto F = [&](int x, int y, int q) -> double& { return s.f[static_cast<size_t>(q) * N + y * nx + x]; };
+    for (int y = 0; y < ny; ++y) {
+        if (s.flags[y * nx] != INLET) continue;
+        const int x = 0;
+        const double y_center = static_cast<double>(ny) * 0.5;
+        const double H = static_cast<double>(ny - 2);
+        const double y_norm = (static_cast<double>(y) - y_center) / (H * 0.5);
+        const double ux = s.p.inlet_velocity * (1.0 - y_norm * y_norm);
+        const double uy = 0.0;
+        const double f0 = F(x, y, 0), f2 = F(x, y, 2), f3 = F(x, y, 3), f4 = F(x, y, 4);
+        const double f6 = F(x, y, 6), f7 = F(x, y, 7);
+        const double rho = (f0 + f2 + f4 + 2.0 * (f3 + f6 + f7)) / (1.0 - ux);
+        F(x, y, 1) = f3 + (2.0 / 3.0) * rho * ux;
+        F(x, y, 5) = f7 + 0.5 * (f4 - f2) + (1.0 / 6.0) * rho * ux + 0.5 * rho * uy;
+        F(x, y, 8) = f6 - 0.5 * (f4 - f2) + (1.0 / 6.0) * rho * ux - 0.5 * rho * uy;
+    }
+    for (int y = 0; y < ny; ++y) {
+        const int x = nx - 1;
+        if (s.flags[y * nx + x] != OUTLET) continue;
+        const double rho = s.p.outlet_density;
+        const double f0 = F(x, y, 0), f1 = F(x, y, 1), f2 = F(x, y, 2), f4 = F(x, y, 4);
+        const double f5 = F(x, y, 5), f8 = F(x, y, 8);
+        const double ux = -1.0 + (f0 + f2 + f4 + 2.0 * (f1 + f5 + f8)) / rho;
+        const double uy = 0.0;
+        F(x, y, 3) = f1 - (2.0 / 3.0) * rho * ux;
+        F(x, y, 7) = f5 - 0.5 * (f4 - f2) - (1.0 / 6.0) * rho * ux - 0.5 * rho * uy;
+        F(x, y, 6) = f8 + 0.5 * (f4 - f2) - (1.0 / 6.0) * rho * ux + 0.5 * rho * uy;
+    }
+}
+
 void lbm_step(sfo_sim& s) {
     const int N = s.N;
     const sfo_params& p = s.p;
@@ -917,6 +953,7 @@ void lbm_step(sfo_sim& s) {
         }
     }
     moving_walls(s);
+    if (p.boundary_type == 0) zou_he(s);
     macroscopic(s);
 }
 
@@ -1008,6 +1045,7 @@ void sfo_default_params(sfo_params* p) {                       // core/parameter
     std::memset(p, 0, sizeof(*p));
     p->nx = 200; p->ny = 50; p->dt = 1.0; p->ibm_iterations = 1;
     p->tau = 0.8; p->rho0 = 1.0; p->boundary_type = 0;
+    p->inlet_velocity = 0.02; p->outlet_density = 1.0;          /* parameters.h:44-45 */
     p->apply_gravity_to_fluid = 1; p->apply_gravity_to_capsules = 1;
     p->mrt_se = 1.4; p->mrt_s_eps = 1.4; p->mrt_sq = 1.4;
     p->max_lattice_force = 0.01;

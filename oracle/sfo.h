@@ -41,6 +41,8 @@ typedef struct {
     int    scalar_periodic_y;
     /* moving walls: x velocity of the solid rows y = ny-1 and y = 0 */
     double top_wall_velocity, bottom_wall_velocity;
+    /* Zou-He inlet/outlet (boundary_type 0) */
+    double inlet_velocity, outlet_density;
 } sfo_params;
 
 typedef struct {

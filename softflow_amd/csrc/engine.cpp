@@ -410,8 +410,6 @@ int Simulation::addCapsuleRandom(int count, Real x0, Real y0, Real x1, Real y1, 
 void Simulation::initialize() {
     const SimulationParams& P = params_;
     // features of SimulationParams outside the device path
-    if (P.fluid.boundary_type == BoundaryType::INLET_OUTLET)
-        throw std::invalid_argument("Simulation: Zou-He inlet/outlet boundaries are not implemented on the MI355X path; use PERIODIC or CLOSED");
     if ((P.fluid.gravity_x != 0.0 || P.fluid.gravity_y != 0.0))
         throw std::invalid_argument("Simulation: gravity is not implemented on the MI355X path");
     if (P.lubrication.enabled || P.adhesion.enabled)
@@ -445,6 +443,8 @@ void Simulation::initialize() {
     sp.precision = static_cast<int>(P.precision);
     sp.top_wall_velocity = P.fluid.top_wall_velocity;
     sp.bottom_wall_velocity = P.fluid.bottom_wall_velocity;
+    sp.inlet_velocity = P.fluid.inlet_velocity;
+    sp.outlet_density = P.fluid.outlet_density;
 
     if (ctx_) { sf_destroy(ctx_); ctx_ = nullptr; }
     check(sf_create(&ctx_, P.device, &sp), "sf_create");
@@ -458,6 +458,8 @@ void Simulation::initialize() {
             int ct = static_cast<int>(CellType::FLUID);
             if (y == 0 || y == ny - 1) ct = static_cast<int>(CellType::SOLID);
             if (P.fluid.boundary_type == BoundaryType::CLOSED && (x == 0 || x == nx - 1)) ct = static_cast<int>(CellType::SOLID);
+            if (P.fluid.boundary_type == BoundaryType::INLET_OUTLET && y >= 1 && y <= ny - 2 && (x == 0 || x == nx - 1))
+                ct = static_cast<int>(x == 0 ? CellType::INLET : CellType::OUTLET);
             if (P.fluid.periodic_y && (y == 0 || y == ny - 1) &&
                 !(P.fluid.boundary_type == BoundaryType::CLOSED && (x == 0 || x == nx - 1)))
                 ct = static_cast<int>(CellType::FLUID);

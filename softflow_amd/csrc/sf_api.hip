@@ -309,8 +309,10 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     if (!out || !p) return fail(SF_E_ARG, "null pointer");
     *out = nullptr;
     if (p->nx < 3 || p->ny < 3) return fail(SF_E_ARG, "domain must be at least 3x3");
-    if (p->boundary_type != SF_PERIODIC && p->boundary_type != SF_CLOSED)
-        return fail(SF_E_ARG, "device path supports periodic or closed x boundaries (Zou-He inlet/outlet is not implemented)");
+    if (p->boundary_type != SF_PERIODIC && p->boundary_type != SF_CLOSED && p->boundary_type != SF_INLET_OUTLET)
+        return fail(SF_E_ARG, "unknown boundary type");
+    if (p->boundary_type == SF_INLET_OUTLET && !(p->outlet_density > 0.0))
+        return fail(SF_E_ARG, "outlet density must be positive");
     if (p->tau <= 0.5) return fail(SF_E_ARG, "tau must exceed 0.5");
     if (p->collision_model < 0 || p->collision_model > 2) return fail(SF_E_ARG, "unknown collision model");
     if (p->precision != SF_FP64 && p->precision != SF_FP32) return fail(SF_E_ARG, "precision must be SF_FP64 or SF_FP32");
@@ -349,12 +351,21 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     chk(dmalloc(&c.mdf, sizeof(MdfDev)));
     if (rc != SF_OK) { sf_destroy(h); return rc; }
     SF_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host_counters), 64 * sizeof(int)));
-    // flags: the plain channel of LBMSolver::initialize (lbm_solver.cpp:130-158)
+    // flags: the channel of LBMSolver::initialize (lbm_solver.cpp:130-158);
+    // with inlet/outlet, INLET and OUTLET columns between the walls
     std::vector<uint8_t> fl(N, SF_FLUID);
     for (int y = 0; y < c.ny; ++y)
         for (int x = 0; x < c.nx; ++x)
             if ((c.walls_y && (y == 0 || y == c.ny - 1)) || (c.walls_x && (x == 0 || x == c.nx - 1)))
                 fl[static_cast<size_t>(y) * c.nx + x] = SF_SOLID;
+    if (p->boundary_type == SF_INLET_OUTLET) {
+        for (int y = 1; y < c.ny - 1; ++y) {
+            fl[static_cast<size_t>(y) * c.nx] = SF_INLET;
+            fl[static_cast<size_t>(y) * c.nx + c.nx - 1] = SF_OUTLET;
+        }
+        c.inlet_outlet = true;
+        c.generic_flags = true;
+    }
     SF_HIP(hipMemcpy(c.flags, fl.data(), N, hipMemcpyHostToDevice));
     for (int b = 0; b < 3; ++b) SF_HIP(hipMemset(c.F[b], 0, 2 * N * es));
     SF_HIP(hipMemset(c.counters, 0, 64 * sizeof(int)));
@@ -427,19 +438,21 @@ int sf_set_flags(sf_ctx* h, const int32_t* flags) {
     Ctx& c = h->c;
     SF_HIP(hipSetDevice(c.device));
     std::vector<uint8_t> fl(c.N);
-    bool channel = true;
+    bool channel = true, io = false;
     for (int y = 0; y < c.ny; ++y)
         for (int x = 0; x < c.nx; ++x) {
             const size_t n = static_cast<size_t>(y) * c.nx + x;
             const int v = flags[n];
-            if (v != SF_FLUID && v != SF_SOLID)
-                return fail(SF_E_ARG, "device path supports FLUID and SOLID cells only");
+            if (v != SF_FLUID && v != SF_SOLID && v != SF_INLET && v != SF_OUTLET)
+                return fail(SF_E_ARG, "device path supports FLUID, SOLID, INLET and OUTLET cells");
             fl[n] = static_cast<uint8_t>(v);
             const bool wall = (c.walls_y && (y == 0 || y == c.ny - 1)) || (c.walls_x && (x == 0 || x == c.nx - 1));
-            if ((v == SF_SOLID) != wall) channel = false;
+            if ((v == SF_SOLID) != wall || v == SF_INLET || v == SF_OUTLET) channel = false;
+            io |= v == SF_INLET || v == SF_OUTLET;
         }
     SF_HIP(hipMemcpy(c.flags, fl.data(), c.N, hipMemcpyHostToDevice));
     c.generic_flags = !channel;
+    c.inlet_outlet = io;
     invalidate_graph(c);
     return SF_OK;
 }

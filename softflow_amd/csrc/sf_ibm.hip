@@ -466,6 +466,11 @@ struct CellVelArgs {
     double bfx, bfy, fmax, fmax2;
 };
 
+// FLUID cells take the body force (INLET and OUTLET cells do not).
+__device__ __forceinline__ bool cell_fluid(const CellVelArgs& a, int x, int y) {
+    return !a.generic_flags || a.flags[y * a.nx + x] == SF_FLUID;
+}
+
 __device__ __forceinline__ bool cell_solid(const CellVelArgs& a, int x, int y) {
     if (a.generic_flags) return a.flags[y * a.nx + x] == SF_SOLID;
     return (a.walls_y && (y == 0 || y == a.ny - 1)) || (a.walls_x && (x == 0 || x == a.nx - 1));
@@ -490,7 +495,7 @@ __device__ __forceinline__ void cell_velocity(const CellVelArgs& a, int x, int y
         const double* Fp = static_cast<const double*>(a.F) + 2 * static_cast<size_t>(n);
         double fx = Fp[0], fy = Fp[1];
         if (force_mode == 1) {
-            if (a.has_body) { fx += a.bfx; fy += a.bfy; }
+            if (a.has_body && cell_fluid(a, x, y)) { fx += a.bfx; fy += a.bfy; }
             double m2 = fx * fx + fy * fy;
             if (m2 > a.fmax2) { double sc = a.fmax / sqrt(m2); fx *= sc; fy *= sc; }
         }
@@ -508,7 +513,7 @@ __device__ __forceinline__ void cell_velocity(const CellVelArgs& a, int x, int y
         const float* Fp = static_cast<const float*>(a.F) + 2 * static_cast<size_t>(n);
         float fx = Fp[0], fy = Fp[1];
         if (force_mode == 1) {
-            if (a.has_body) { fx += static_cast<float>(a.bfx); fy += static_cast<float>(a.bfy); }
+            if (a.has_body && cell_fluid(a, x, y)) { fx += static_cast<float>(a.bfx); fy += static_cast<float>(a.bfy); }
             float m2 = fx * fx + fy * fy;
             if (m2 > static_cast<float>(a.fmax2)) { float sc = static_cast<float>(a.fmax) / sqrtf(m2); fx *= sc; fy *= sc; }
         }

@@ -103,6 +103,7 @@ struct Ctx {
     bool periodic_x = false, periodic_y = false;
     bool generic_flags = false;          // flags differ from the plain channel
     bool walls_y = true, walls_x = false;
+    bool inlet_outlet = false;           // INLET / OUTLET cells present: Zou-He after each collide-stream
     bool fp32 = false;
     hipStream_t stream = nullptr;
 

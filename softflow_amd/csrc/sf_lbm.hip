@@ -258,7 +258,17 @@ struct LbmArgs {
     // moving walls (lbm/moving_wall.cpp): x velocities of the solid rows y = 0 and y = ny-1
     int moving;
     double wall_bot, wall_top;
+    // Zou-He inlet / outlet (lbm/zou_he_boundary.cpp) and the scalar's inlet concentration
+    double inlet_ux, outlet_rho;
+    double cin[4];
 };
+
+// FLUID cells take the body force and collide (bgk_collision.cpp:43,
+// simulation.cpp:541-609); INLET and OUTLET cells only stream.
+__device__ __forceinline__ bool is_fluid_type(const LbmArgs& a, int x, int y, bool use_flags) {
+    if (use_flags) return a.flags[y * a.nx + x] == SF_FLUID;
+    return !((a.walls_y && (y == 0 || y == a.ny - 1)) || (a.walls_x && (x == 0 || x == a.nx - 1)));
+}
 
 __device__ __forceinline__ bool block_origin(const LbmArgs& a, int b, int& x, int& y) {
     const int t = a.remap ? (b % 8) * a.per_xcd + b / 8 : b;
@@ -323,12 +333,15 @@ __device__ __forceinline__ void adr_cell(const LbmArgs& a, int x, int y, int n, 
             C = acc;
         }
         const double oms = a.oms[s];
-        if constexpr (!FP32) {
+        // INLET cells relax towards the inlet concentration (advection_diffusion.cpp:111-150)
+        const bool inlet = FLAGS && a.flags[n] == SF_INLET;
+        if (inlet) C = a.cin[s];
+        if (!FP32 || inlet) {
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 const double eu = static_cast<double>(dcx(q)) * ux + static_cast<double>(dcy(q)) * uy;
                 const double ge = dw(q) * C * (1.0 + eu * 3.0);
-                g[q] = g[q] - oms * (g[q] - ge);
+                g[q] = static_cast<T>(static_cast<double>(g[q]) - oms * (static_cast<double>(g[q]) - ge));
             }
         } else {
             // FP32: the eight moving populations relax in FP32; the rest
@@ -371,9 +384,10 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
         const int n = y * a.nx + x;
         const bool solid = is_solid(a, x, y, FLAGS);
         const bool fluid = !solid;
+        const bool collide = FLAGS ? is_fluid_type(a, x, y, true) : fluid;
         using T = typename std::conditional<FP32, float, double>::type;
         T Fx, Fy;
-        capped = collision_force<T>(a, a.Fcur, n, fluid, Fx, Fy);
+        capped = collision_force<T>(a, a.Fcur, n, collide, Fx, Fy);
         if (fluid) {
             const T* src = static_cast<const T*>(a.fsrc);
             T* dst = static_cast<T*>(a.fdst);
@@ -392,7 +406,7 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                     vy += fi * static_cast<double>(dcy(q));
                 }
                 double sx = 0.0, sy = 0.0;
-                if (a.shift == SHIFT_PREV) collision_force<double>(a, a.Fshift, n, true, sx, sy);
+                if (a.shift == SHIFT_PREV) collision_force<double>(a, a.Fshift, n, collide, sx, sy);
                 else if (a.shift == SHIFT_PARTIAL) load_force<double>(a.Fshift, n, sx, sy);
                 double ux, uy;
                 if (a.shift == SHIFT_STORED) {
@@ -408,7 +422,8 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                 }
                 rho_wall = r;
                 if constexpr (ADRF) adr_cell<false, FLAGS>(a, x, y, n, ux, uy);
-                if (OP == SF_REGULARIZED) col_reg(f, a.om, r, ux, uy, Fx, Fy);
+                if (!collide) {
+                } else if (OP == SF_REGULARIZED) col_reg(f, a.om, r, ux, uy, Fx, Fy);
                 else if (OP == SF_MRT) col_mrt(f, a.om, a.se, a.seps, a.sq, r, ux, uy, Fx, Fy);
                 else col_bgk(f, a.om, r, ux, uy, Fx, Fy);
             } else {
@@ -416,7 +431,7 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                 float jx = (f[1] - f[3]) + (f[5] - f[6]) + (f[8] - f[7]);
                 float jy = (f[2] - f[4]) + (f[5] + f[6]) - (f[7] + f[8]);
                 float sx = 0.f, sy = 0.f;
-                if (a.shift == SHIFT_PREV) collision_force<float>(a, a.Fshift, n, true, sx, sy);
+                if (a.shift == SHIFT_PREV) collision_force<float>(a, a.Fshift, n, collide, sx, sy);
                 else if (a.shift == SHIFT_PARTIAL) load_force<float>(a.Fshift, n, sx, sy);
                 float rho = 1.0f + drho, ux, uy;
                 if (a.shift == SHIFT_STORED) {
@@ -432,7 +447,8 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                 rho_wall = static_cast<double>(rho);
                 if constexpr (ADRF) adr_cell<true, FLAGS>(a, x, y, n, static_cast<double>(ux), static_cast<double>(uy));
                 const float om = static_cast<float>(a.om);
-                if (OP == SF_REGULARIZED) dcol_reg(f, om, drho, rho, ux, uy, Fx, Fy);
+                if (!collide) {
+                } else if (OP == SF_REGULARIZED) dcol_reg(f, om, drho, rho, ux, uy, Fx, Fy);
                 else if (OP == SF_MRT)
                     dcol_mrt(f, om, static_cast<float>(a.se), static_cast<float>(a.seps),
                              static_cast<float>(a.sq), drho, rho, ux, uy, Fx, Fy);
@@ -531,6 +547,7 @@ static void scalar_args(Ctx& c, LbmArgs& a) {
         a.gdst[s] = c.g[s][1 - c.gcur];
         a.Cst[s] = c.Cstore[s];
         a.oms[s] = 1.0 / (3.0 * c.p.diffusivity[s] + 0.5);
+        a.cin[s] = c.p.inlet_concentration[s];
     }
 }
 
@@ -556,6 +573,8 @@ static LbmArgs lbm_args(Ctx& c, int shift, bool fuse_adr) {
     a.wall_bot = c.p.bottom_wall_velocity;
     a.wall_top = c.p.top_wall_velocity;
     a.moving = (fabs(a.wall_bot) > 1e-15 || fabs(a.wall_top) > 1e-15) ? 1 : 0;
+    a.inlet_ux = c.p.inlet_velocity;
+    a.outlet_rho = c.p.outlet_density;
     if (fuse_adr) scalar_args(c, a);
     return a;
 }
@@ -576,8 +595,61 @@ void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap,
     }
 }
 
+// ZouHeBoundary::apply (lbm/zou_he_boundary.cpp:18-75) on the streamed
+// populations: a parabolic velocity inlet on the INLET cells of column 0 and a
+// fixed-density outlet on the OUTLET cells of column nx-1, each rebuilding the
+// three populations that enter from outside (one thread per row, FP64).
+template <bool FP32>
+__global__ void k_zou_he(LbmArgs a) {
+    using T = typename std::conditional<FP32, float, double>::type;
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= a.ny) return;
+    T* fd = static_cast<T*>(a.fdst);
+    const size_t N = static_cast<size_t>(a.N);
+    // full populations (FP32 storage holds f - w)
+    auto get = [&](int x, int q) -> double {
+        const double v = static_cast<double>(fd[q * N + static_cast<size_t>(y) * a.nx + x]);
+        return FP32 ? v + dw(q) : v;
+    };
+    auto put = [&](int x, int q, double v) {
+        fd[q * N + static_cast<size_t>(y) * a.nx + x] = static_cast<T>(FP32 ? v - dw(q) : v);
+    };
+    if (a.flags[static_cast<size_t>(y) * a.nx] == SF_INLET) {
+        const int x = 0;
+        const double y_center = static_cast<double>(a.ny) * 0.5;
+        const double H = static_cast<double>(a.ny - 2);
+        const double y_norm = (static_cast<double>(y) - y_center) / (H * 0.5);
+        const double ux = a.inlet_ux * (1.0 - y_norm * y_norm);
+        const double uy = 0.0;
+        const double f0 = get(x, 0), f2 = get(x, 2), f3 = get(x, 3), f4 = get(x, 4);
+        const double f6 = get(x, 6), f7 = get(x, 7);
+        const double rho = (f0 + f2 + f4 + 2.0 * (f3 + f6 + f7)) / (1.0 - ux);
+        put(x, 1, f3 + (2.0 / 3.0) * rho * ux);
+        put(x, 5, f7 + 0.5 * (f4 - f2) + (1.0 / 6.0) * rho * ux + 0.5 * rho * uy);
+        put(x, 8, f6 - 0.5 * (f4 - f2) + (1.0 / 6.0) * rho * ux - 0.5 * rho * uy);
+    }
+    if (a.flags[static_cast<size_t>(y) * a.nx + a.nx - 1] == SF_OUTLET) {
+        const int x = a.nx - 1;
+        const double rho = a.outlet_rho;
+        const double f0 = get(x, 0), f1 = get(x, 1), f2 = get(x, 2), f4 = get(x, 4);
+        const double f5 = get(x, 5), f8 = get(x, 8);
+        const double ux = -1.0 + (f0 + f2 + f4 + 2.0 * (f1 + f5 + f8)) / rho;
+        const double uy = 0.0;
+        put(x, 3, f1 - (2.0 / 3.0) * rho * ux);
+        put(x, 7, f5 - 0.5 * (f4 - f2) - (1.0 / 6.0) * rho * ux - 0.5 * rho * uy);
+        put(x, 6, f8 + 0.5 * (f4 - f2) - (1.0 / 6.0) * rho * ux + 0.5 * rho * uy);
+    }
+}
+
 void launch_lbm(Ctx& c, int shift, bool fuse_adr) {
     launch_lbm_cfg(c, shift, fuse_adr, c.lbm_bx, c.lbm_by, c.lbm_remap, c.lbm_persist);
+    if (c.inlet_outlet) {
+        LbmArgs a = lbm_args(c, shift, false);
+        a.fdst = c.f[1 - c.fcur];
+        const dim3 g((c.ny + 63) / 64), b(64);
+        if (c.fp32) hipLaunchKernelGGL(k_zou_he<true>, g, b, 0, c.stream, a);
+        else hipLaunchKernelGGL(k_zou_he<false>, g, b, 0, c.stream, a);
+    }
     c.fcur = 1 - c.fcur;
     if (fuse_adr) {
         c.gcur = 1 - c.gcur;
@@ -634,7 +706,7 @@ void batch_launch_lbm(const BatchVariant& v, const void* packs, const int2* map,
 // collision force): used by the scalar step and the macroscopic views.
 template <bool FP32>
 __device__ __forceinline__ void cell_macro(const void* fv, size_t N, int n, const void* F,
-                                           const LbmArgs& a, bool zero_force, double& r,
+                                           const LbmArgs& a, bool zero_force, bool body, double& r,
                                            double& ux, double& uy) {
     if constexpr (!FP32) {
         const double* f = static_cast<const double*>(fv);
@@ -647,7 +719,7 @@ __device__ __forceinline__ void cell_macro(const void* fv, size_t N, int n, cons
             vy += fi * static_cast<double>(dcy(q));
         }
         double sx = 0.0, sy = 0.0;
-        if (!zero_force) collision_force<double>(a, F, n, true, sx, sy);
+        if (!zero_force) collision_force<double>(a, F, n, body, sx, sy);
         r = rr;
         if (rr > 1e-15) {
             ux = (vx + 0.5 * sx) / rr;
@@ -665,7 +737,7 @@ __device__ __forceinline__ void cell_macro(const void* fv, size_t N, int n, cons
         float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
         float jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
         float sx = 0.f, sy = 0.f;
-        if (!zero_force) collision_force<float>(a, F, n, true, sx, sy);
+        if (!zero_force) collision_force<float>(a, F, n, body, sx, sy);
         float rho = 1.0f + drho;
         r = rho;
         ux = (jx + 0.5f * sx) / rho;
@@ -685,7 +757,8 @@ __global__ __launch_bounds__(256) void k_adr(LbmArgs a, const void* f) {
     const int n = y * a.nx + x;
     if (is_solid(a, x, y, FLAGS)) return;
     double r, ux, uy;
-    cell_macro<FP32>(f, static_cast<size_t>(a.N), n, a.Fcur, a, false, r, ux, uy);
+    cell_macro<FP32>(f, static_cast<size_t>(a.N), n, a.Fcur, a, false, is_fluid_type(a, x, y, FLAGS), r,
+                     ux, uy);
     adr_cell<FP32, FLAGS>(a, x, y, n, ux, uy);
 }
 
@@ -736,11 +809,11 @@ __global__ void k_macro(LbmArgs a, const void* f, int mode, double* rho, double*
             Fx[n] = 0.0; Fy[n] = 0.0;
         } else if (FP32) {
             float fx, fy;
-            collision_force<float>(a, a.Fcur, n, !solid, fx, fy);
+            collision_force<float>(a, a.Fcur, n, !solid && is_fluid_type(a, x, y, FLAGS), fx, fy);
             Fx[n] = fx; Fy[n] = fy;
         } else {
             double fx, fy;
-            collision_force<double>(a, a.Fcur, n, !solid, fx, fy);
+            collision_force<double>(a, a.Fcur, n, !solid && is_fluid_type(a, x, y, FLAGS), fx, fy);
             Fx[n] = fx; Fy[n] = fy;
         }
     }
@@ -754,7 +827,8 @@ __global__ void k_macro(LbmArgs a, const void* f, int mode, double* rho, double*
         return;
     }
     double r, u, v;
-    cell_macro<FP32>(f, static_cast<size_t>(a.N), n, a.Fcur, a, mode == 1, r, u, v);
+    cell_macro<FP32>(f, static_cast<size_t>(a.N), n, a.Fcur, a, mode == 1, is_fluid_type(a, x, y, FLAGS), r,
+                     u, v);
     rho[n] = r; ux[n] = u; uy[n] = v;
 }
 

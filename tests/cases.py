@@ -3,7 +3,8 @@
 
 A case is a dict:
     nx, ny, tau, coll (0 BGK, 1 MRT, 2 regularized), bfx, mdf (ibm_iterations),
-    boundary ('periodic' | 'closed'), periodic_y, precision ('fp64' | 'fp32'),
+    boundary ('periodic' | 'closed' | 'inlet_outlet'), periodic_y, precision ('fp64' | 'fp32'),
+    inlet: (inlet peak velocity, outlet density) for 'inlet_outlet' (optional),
     capsules: list of {count, region:(x0,y0,x1,y1), rmin, rmax, nodes, seed, min_gap, mem:{...}}
     scalar: {D, blob:(cx, cy, sigma)}      (optional)
     rep: {damping_normal, friction_coeff}  (optional)
@@ -52,7 +53,8 @@ def oracle_sim(case, which="oracle"):
     rep = case.get("rep", {})
     p = po.default_params(
         nx=case["nx"], ny=case["ny"], tau=case.get("tau", 0.8),
-        boundary_type=2 if case.get("boundary") == "closed" else 1,
+        boundary_type={"closed": 2, "inlet_outlet": 0}.get(case.get("boundary"), 1),
+        inlet_velocity=case.get("inlet", (0.02, 1.0))[0], outlet_density=case.get("inlet", (0.02, 1.0))[1],
         periodic_y=1 if case.get("periodic_y") else 0,
         body_force_x=case.get("bfx", 0.0), collision_model=case.get("coll", 0),
         ibm_iterations=case.get("mdf", 1), scalar_enabled=1 if s else 0,
@@ -95,7 +97,10 @@ def product_sim(case, graph=True):
     p = sc.SimulationParams()
     p.nx, p.ny = case["nx"], case["ny"]
     p.fluid.tau = case.get("tau", 0.8)
-    p.fluid.boundary_type = sc.BoundaryType.CLOSED if case.get("boundary") == "closed" else sc.BoundaryType.PERIODIC
+    p.fluid.boundary_type = {"closed": sc.BoundaryType.CLOSED,
+                             "inlet_outlet": sc.BoundaryType.INLET_OUTLET}.get(case.get("boundary"),
+                                                                              sc.BoundaryType.PERIODIC)
+    p.fluid.inlet_velocity, p.fluid.outlet_density = case.get("inlet", (0.02, 1.0))
     p.fluid.periodic_y = bool(case.get("periodic_y", False))
     p.fluid.body_force_x = case.get("bfx", 0.0)
     p.fluid.collision_model = [sc.CollisionModel.BGK, sc.CollisionModel.MRT,

@@ -48,7 +48,7 @@ def test_argument_validation_before_device():
     from softflow_amd import capi
     L = capi.load()
     h = C.c_void_p()
-    for kw, msg in ((dict(nx=2), "3x3"), (dict(tau=0.5), "tau"), (dict(boundary_type=0), "inlet"),
+    for kw, msg in ((dict(nx=2), "3x3"), (dict(tau=0.5), "tau"), (dict(boundary_type=0), "outlet density"), (dict(boundary_type=7), "boundary"),
                     (dict(collision_model=5), "collision"), (dict(precision=7), "precision"),
                     (dict(ibm_iterations=0), "ibm_iterations")):
         p = capi.make_params(nx=16, ny=8, **{k: v for k, v in kw.items() if k != "nx"})
@@ -143,15 +143,13 @@ def test_random_placement_and_rest_geometry_match_oracle(shape, nodes):
         assert cp.deformationIndex() == o.geometry(c, 4)
 
 
-@pytest.mark.parametrize("what", ["inlet", "gravity", "lubrication", "ibb"])
+@pytest.mark.parametrize("what", ["gravity", "lubrication", "ibb"])
 def test_unsupported_features_refused(what):
     from softflow_amd import softflow_core as sc
     p = sc.SimulationParams()
     p.nx, p.ny = 32, 16
     p.fluid.boundary_type = sc.BoundaryType.PERIODIC
-    if what == "inlet":
-        p.fluid.boundary_type = sc.BoundaryType.INLET_OUTLET
-    elif what == "gravity":
+    if what == "gravity":
         p.fluid.gravity_y = -1e-6
     elif what == "lubrication":
         p.lubrication.enabled = True

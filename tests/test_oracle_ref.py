@@ -68,6 +68,9 @@ CASES = dict(
     moving_walls=dict(coll=0, ncap=0, walls=(0.02, -0.01)),
     moving_walls_closed_mrt=dict(coll=1, ncap=0, boundary=2, walls=(0.03, 0.0)),
     moving_walls_caps_mdf3=dict(coll=2, mdf=3, ncap=6, walls=(0.01, -0.01)),
+    inlet_outlet_bgk=dict(coll=0, ncap=0, boundary=0),
+    inlet_outlet_mrt_caps=dict(coll=1, ncap=8, boundary=0),
+    inlet_outlet_reg_mdf3_adr=dict(coll=2, mdf=3, ncap=6, boundary=0, scalar=1),
 )
 
 
