@@ -274,11 +274,12 @@ def ensemble_bench(steps, count=256, seed=7, seq_steps=60):
     ens.run(7)
     tune = ens.autotune(10)
     chosen = tune.strip().splitlines()[-1]
-    shape = chosen.split()[1] + " " + chosen.split()[2]
+    want = chosen[len("chosen "):]
     lbm_ms = None
     for line in tune.strip().splitlines()[:-1]:
-        if line.startswith(shape):
-            lbm_ms = float(line.split()[-2])
+        desc, ms, _ = line.rsplit(" ", 2)          # "<launch description> <ms> ms"
+        if desc == want:
+            lbm_ms = float(ms)
     ens.run(12)
     ens.synchronize()
     t = time.perf_counter()
