@@ -57,6 +57,7 @@ typedef struct {
     double bottom_wall_velocity;    /* solid rows y = ny-1 and y = 0 */
     double inlet_velocity;          /* SF_INLET_OUTLET (lbm/zou_he_boundary.cpp): peak of the */
     double outlet_density;          /* parabolic inlet profile; outlet density */
+    int    use_interpolated_bb;     /* parameters.h:89: Bouzidi links towards the obstacles */
 } sf_params;
 
 /* Per-capsule constants resolved on the host (rest geometry and WLC constants
@@ -84,6 +85,26 @@ int  sf_destroy(sf_ctx* ctx);
 /* Cell types (LatticeField::cellTypeData, lattice_field.h:87).  flags[N]:
  * FLUID, SOLID, INLET, OUTLET (EMPTY is refused). */
 int  sf_set_flags(sf_ctx* ctx, const int32_t* flags);
+
+/* Obstacles (geometry/obstacle.h and its three shapes), as the channel builder
+ * holds them (ChannelBuilder::getObstacles, channel_builder.h:52-54): they act
+ * in the contact forces (RepulsionForce::computeObstacleRepulsion,
+ * membrane/repulsion.cpp:180-236, after the wall images, in list order) and,
+ * with use_interpolated_bb, place the wall of every link from a non-SOLID cell
+ * into a SOLID cell (InterpolatedBounceBack::initialize,
+ * lbm/interpolated_bounce_back.cpp:7-53).  They do not change the cell types:
+ * the caller sets the SOLID cells (ChannelBuilder::applyToField) through
+ * sf_set_flags.  The links are rebuilt from the current cell types whenever the
+ * obstacles or the cell types change.
+ *   kinds[i]: SF_OBS_CIRCLE   coords (cx, cy, r)            (circle_obstacle.h)
+ *             SF_OBS_RECT     coords (x0, y0, x1, y1)       (rect_obstacle.h)
+ *             SF_OBS_POLYGON  coords (x, y) per vertex, >= 3 (polygon_obstacle.h)
+ *   obstacle i takes coords[offsets[i] .. offsets[i+1]).  count 0 clears them.
+ * A link whose wall lies nearer than half a cell and whose upstream cell is
+ * SOLID (a fluid gap one cell wide) is refused with SF_E_ARG. */
+enum { SF_OBS_CIRCLE = 0, SF_OBS_RECT = 1, SF_OBS_POLYGON = 2 };
+int  sf_set_obstacles(sf_ctx* ctx, int count, const int32_t* kinds,
+                      const int32_t* offsets, const double* coords);
 
 /* Populations in the reference layout (LatticeField::fData, lattice_field.h:41). */
 int  sf_set_populations(sf_ctx* ctx, const double* f);

@@ -37,6 +37,7 @@ class Params(C.Structure):
         ("scalar_periodic_y", C.c_int),
         ("top_wall_velocity", C.c_double), ("bottom_wall_velocity", C.c_double),
         ("inlet_velocity", C.c_double), ("outlet_density", C.c_double),
+        ("use_interpolated_bb", C.c_int),
     ]
 
 
@@ -100,7 +101,12 @@ def _lib(path: str, prefix: str):
     g("set_nodes").argtypes = [C.c_void_p] + [_DP] * 4
     g("set_flags").argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     g("get_flags").argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    g("add_obstacle").argtypes = [C.c_void_p, C.c_int, _DP, C.c_int]
+    g("obstacle_distance").restype = C.c_double
+    g("obstacle_distance").argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _DP]
     if prefix == "sfo_":
+        g("ibb_links").restype = C.c_int
+        g("ibb_links").argtypes = [C.c_void_p, C.POINTER(C.c_int), _DP]
         for n in ("default_params",):
             g(n).argtypes = [C.POINTER(Params)]
         g("default_membrane").argtypes = [C.POINTER(Membrane)]
@@ -189,6 +195,28 @@ class Sim:
                            seed=12345, min_gap=1.0, max_attempts=0):
         return self._f("add_capsule_random")(self.h, count, x0, y0, x1, y1, rmin, rmax, nodes,
                                              C.byref(mem), typ, seed, min_gap, max_attempts)
+
+    def add_obstacle(self, obstacle):
+        """("circle", cx, cy, r) | ("rect", x0, y0, x1, y1) | ("polygon", [(x, y), ...])"""
+        kind = {"circle": 0, "rect": 1, "polygon": 2}[obstacle[0]]
+        vals = ([float(v) for xy in obstacle[1] for v in xy] if kind == 2
+                else [float(v) for v in obstacle[1:]])
+        a = np.asarray(vals, dtype=np.float64)
+        self._f("add_obstacle")(self.h, kind, _ptr(a), a.size)
+
+    def obstacle_distance(self, i, x, y):
+        """(signedDistance, normalAt) of obstacle i at (x, y)."""
+        nn = np.zeros(2)
+        d = self._f("obstacle_distance")(self.h, i, x, y, _ptr(nn))
+        return d, nn
+
+    def ibb_links(self):
+        """The interpolated bounce-back links (oracle only): int (L, 3) of (x, y, q), qf (L,)."""
+        n = self.L.sfo_ibb_links(self.h, None, None)
+        xyq = np.zeros(3 * max(n, 1), dtype=np.int32)
+        qf = np.zeros(max(n, 1))
+        self.L.sfo_ibb_links(self.h, xyq.ctypes.data_as(C.POINTER(C.c_int)), _ptr(qf))
+        return xyq[:3 * n].reshape(n, 3), qf[:n]
 
     def initialize(self):
         self._f("initialize")(self.h)

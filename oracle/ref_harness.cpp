@@ -14,6 +14,8 @@
 
 #include "core/parameters.h"
 #include "coupling/immersed_boundary.h"
+#include "geometry/channel_builder.h"
+#include "geometry/polygon_obstacle.h"
 #include "lbm/advection_diffusion.h"
 #include "lbm/lbm_solver.h"
 #include "membrane/capsule_system.h"
@@ -34,6 +36,8 @@ struct sfr_sim {
     ImmersedBoundary ibm;
     std::unique_ptr<RepulsionForce> rep;
     std::unique_ptr<AdvectionDiffusion> adr;
+    std::vector<std::shared_ptr<Obstacle>> obs;     // in the order added
+    std::vector<int> obs_kind;
     int capped = 0;
 };
 
@@ -65,6 +69,7 @@ static SimulationParams to_params(const sfo_params* p) {
     s.fluid.bottom_wall_velocity = p->bottom_wall_velocity;
     s.fluid.inlet_velocity = p->inlet_velocity;
     s.fluid.outlet_density = p->outlet_density;
+    s.fluid.use_interpolated_bb = p->use_interpolated_bb != 0;
     return s;
 }
 
@@ -130,6 +135,8 @@ int sfr_add_capsule_random(sfr_sim* s, int count, double x0, double y0, double x
             if ((Vec2d{cx, cy} - oc).norm() < r + orr + min_gap) { overlap = true; break; }
         }
         if (cy - r < wb + min_gap || cy + r > wt - min_gap) overlap = true;
+        for (const auto& o : s->obs)
+            if (o->signedDistance(cx, cy) < r + min_gap) { overlap = true; break; }
         if (!overlap) {
             int nn = (num_nodes > 0) ? num_nodes : Capsule::computeOptimalNodes(r);
             s->caps.addCapsule({cx, cy}, r, nn, mp, type);
@@ -140,8 +147,51 @@ int sfr_add_capsule_random(sfr_sim* s, int count, double x0, double y0, double x
     return placed;
 }
 
+void sfr_add_obstacle(sfr_sim* s, int kind, const double* c, int n) {
+    if (kind == 0) s->obs.push_back(std::make_shared<CircleObstacle>(c[0], c[1], c[2]));
+    else if (kind == 1) s->obs.push_back(std::make_shared<RectObstacle>(c[0], c[1], c[2], c[3]));
+    else {
+        std::vector<Vec2d> v;
+        for (int i = 0; i + 1 < n; i += 2) v.push_back(Vec2d{c[i], c[i + 1]});
+        s->obs.push_back(std::make_shared<PolygonObstacle>(v));
+    }
+    s->obs_kind.push_back(kind);
+}
+
+double sfr_obstacle_distance(const sfr_sim* s, int i, double x, double y, double* normal) {
+    if (normal) { Vec2d nn = s->obs[i]->normalAt(x, y); normal[0] = nn.x; normal[1] = nn.y; }
+    return s->obs[i]->signedDistance(x, y);
+}
+
+// Simulation::initialize (simulation.cpp:429-442): the channel builder's cell
+// types, then the solver's rows and columns, then the IBB links.  The builder
+// takes circles and rectangles; polygons have no builder call and are marked
+// SOLID here the way applyToField marks its obstacles.
 void sfr_initialize(sfr_sim* s) {
+    if (!s->obs.empty()) {
+        ChannelBuilder b(s->params.nx, s->params.ny);
+        b.addWalls();
+        for (size_t i = 0; i < s->obs.size(); ++i) {
+            if (s->obs_kind[i] == 0) {
+                auto* o = static_cast<CircleObstacle*>(s->obs[i].get());
+                b.addCirclePillar(o->getCx(), o->getCy(), o->getRadius());
+            } else if (s->obs_kind[i] == 1) {
+                auto* o = static_cast<RectObstacle*>(s->obs[i].get());
+                b.addRectPillar(o->getX0(), o->getY0(), o->getX1(), o->getY1());
+            }
+        }
+        b.setBoundaryType(s->params.fluid.boundary_type);
+        LatticeField& f = s->lbm->field();
+        b.applyToField(f);
+        for (size_t i = 0; i < s->obs.size(); ++i)
+            if (s->obs_kind[i] == 2)
+                for (int y = 0; y < s->params.ny; ++y)
+                    for (int x = 0; x < s->params.nx; ++x)
+                        if (s->obs[i]->contains(static_cast<Real>(x), static_cast<Real>(y)))
+                            f.setCellType(x, y, CellType::SOLID);
+    }
     s->lbm->initialize();
+    if (s->params.fluid.use_interpolated_bb) s->lbm->setObstacles(s->obs);
     if (s->adr) s->adr->initialize(0.0);
     if (s->params.fluid.boundary_type == BoundaryType::PERIODIC) s->caps.setPeriodicX(s->params.nx);
 }
@@ -159,6 +209,7 @@ void sfr_step(sfr_sim* s, int nsteps) {
         s->caps.clearAllForces();
         s->caps.computeAllMembraneForces();
         s->rep->computeAll(s->caps, 0.5, static_cast<Real>(P.ny) - 1.5);
+        for (const auto& o : s->obs) s->rep->computeObstacleRepulsion(s->caps, *o);
         if (P.fluid.apply_gravity_to_capsules &&
             (P.fluid.gravity_x != 0.0 || P.fluid.gravity_y != 0.0))
             for (int c = 0; c < s->caps.numCapsules(); ++c)

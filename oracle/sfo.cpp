@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <random>
 #include <vector>
 
@@ -185,10 +186,119 @@ struct Caps {
 
 };
 
+// An obstacle: geometry/circle_obstacle.h, rect_obstacle.h,
+// polygon_obstacle.h/.cpp.  Each query keeps the reference's operations.
+struct OObstacle {
+    int kind;                       // 0 circle, 1 rectangle, 2 polygon
+    double a[4];                    // circle cx, cy, r; rectangle x0, y0, x1, y1
+    std::vector<V2> poly;
+
+    bool contains(double x, double y) const {
+        if (kind == 0) {
+            double dx = x - a[0], dy = y - a[1];
+            return (dx * dx + dy * dy) <= a[2] * a[2];
+        }
+        if (kind == 1) return x >= a[0] && x <= a[2] && y >= a[1] && y <= a[3];
+        bool in = false;                                   // polygon_obstacle.cpp:10-24
+        const int n = static_cast<int>(poly.size());
+        for (int i = 0, j = n - 1; i < n; j = i++) {
+            const V2 &pi = poly[i], &pj = poly[j];
+            if (((pi.y > y) != (pj.y > y)) && (x < (pj.x - pi.x) * (y - pi.y) / (pj.y - pi.y) + pi.x))
+                in = !in;
+        }
+        return in;
+    }
+    V2 nearest(double x, double y) const {
+        if (kind == 0) {
+            double dx = x - a[0], dy = y - a[1];
+            double d = std::sqrt(dx * dx + dy * dy);
+            if (d < 1e-15) return V2{a[0] + a[2], a[1]};
+            return V2{a[0] + a[2] * dx / d, a[1] + a[2] * dy / d};
+        }
+        if (kind == 1) {                                   // rect_obstacle.h:27-41
+            double cx = std::clamp(x, a[0], a[2]), cy = std::clamp(y, a[1], a[3]);
+            if (cx == x && cy == y) {
+                double dl = x - a[0], dr = a[2] - x, db = y - a[1], dt = a[3] - y;
+                double m = std::min({dl, dr, db, dt});
+                if (m == dl) return V2{a[0], y};
+                if (m == dr) return V2{a[2], y};
+                if (m == db) return V2{x, a[1]};
+                return V2{x, a[3]};
+            }
+            return V2{cx, cy};
+        }
+        double best = std::numeric_limits<double>::max();  // polygon_obstacle.cpp:26-54
+        V2 out{0.0, 0.0};
+        const int n = static_cast<int>(poly.size());
+        for (int i = 0; i < n; ++i) {
+            const V2 v0 = poly[i], v1 = poly[(i + 1) % n];
+            V2 e{v1.x - v0.x, v1.y - v0.y};
+            double len2 = e.x * e.x + e.y * e.y;
+            V2 q = v0;
+            if (!(len2 < 1e-30)) {
+                double t = ((x - v0.x) * e.x + (y - v0.y) * e.y) / len2;
+                t = std::max(0.0, std::min(1.0, t));
+                q = V2{v0.x + t * e.x, v0.y + t * e.y};
+            }
+            double ddx = q.x - x, ddy = q.y - y;
+            double d2 = ddx * ddx + ddy * ddy;
+            if (d2 < best) { best = d2; out = q; }
+        }
+        return out;
+    }
+    double signed_distance(double x, double y) const {
+        if (kind == 0) {
+            double dx = x - a[0], dy = y - a[1];
+            return std::sqrt(dx * dx + dy * dy) - a[2];
+        }
+        if (kind == 1) {                                   // rect_obstacle.h:18-25
+            double dx = std::max({a[0] - x, x - a[2], 0.0});
+            double dy = std::max({a[1] - y, y - a[3], 0.0});
+            double out = std::sqrt(dx * dx + dy * dy);
+            if (out > 1e-15) return out;
+            return -std::min({x - a[0], a[2] - x, y - a[1], a[3] - y});
+        }
+        V2 q = nearest(x, y);
+        double d = std::sqrt((q.x - x) * (q.x - x) + (q.y - y) * (q.y - y));
+        return contains(x, y) ? -d : d;
+    }
+    V2 normal(double x, double y) const {
+        if (kind == 0) {
+            double dx = x - a[0], dy = y - a[1];
+            double d = std::sqrt(dx * dx + dy * dy);
+            if (d < 1e-15) return V2{1.0, 0.0};
+            return V2{dx / d, dy / d};
+        }
+        V2 q = nearest(x, y);
+        double dx = x - q.x, dy = y - q.y;
+        double d = std::sqrt(dx * dx + dy * dy);
+        if (kind == 1) {                                   // rect_obstacle.h:43-61
+            if (d < 1e-15) {
+                double dl = std::abs(x - a[0]), dr = std::abs(x - a[2]);
+                double db = std::abs(y - a[1]), dt = std::abs(y - a[3]);
+                double m = std::min({dl, dr, db, dt});
+                if (m == dl) return V2{-1.0, 0.0};
+                if (m == dr) return V2{1.0, 0.0};
+                if (m == db) return V2{0.0, -1.0};
+                return V2{0.0, 1.0};
+            }
+            return V2{dx / d, dy / d};
+        }
+        if (d < 1e-15) return V2{0.0, 1.0};                // polygon_obstacle.cpp:62-77
+        V2 nn{dx / d, dy / d};
+        if (contains(x, y)) { nn.x = -nn.x; nn.y = -nn.y; }
+        return nn;
+    }
+};
+
+struct IbbLink { int x, y, q; double qf; };
+
 }  // namespace
 
 struct sfo_sim {
     sfo_params p;
+    std::vector<OObstacle> obs;
+    std::vector<IbbLink> ibb;
     int nx, ny, N;
     bool periodic_x, periodic_y;
     std::vector<double> f, ftmp, rho, ux, uy, Fx, Fy;
@@ -921,6 +1031,87 @@ static void zou_he(sfo_sim& s) {
     }
 }
 
+// InterpolatedBounceBack::initialize (lbm/interpolated_bounce_back.cpp:7-53)
+void ibb_build(sfo_sim& s) {
+    s.ibb.clear();
+    for (int y = 0; y < s.ny; ++y)
+        for (int x = 0; x < s.nx; ++x) {
+            if (s.flags[y * s.nx + x] == SOLID) continue;
+            for (int q = 1; q < Q; ++q) {
+                int xn = x + CX[q], yn = y + CY[q];
+                if (xn < 0 || xn >= s.nx || yn < 0 || yn >= s.ny) continue;
+                if (s.flags[yn * s.nx + xn] != SOLID) continue;
+                double best = 1.0;
+                for (const OObstacle& o : s.obs) {
+                    double lo = 0.0, hi = 1.0;
+                    for (int it = 0; it < 20; ++it) {
+                        double mid = 0.5 * (lo + hi);
+                        if (o.contains(static_cast<double>(x) + mid * static_cast<double>(CX[q]),
+                                       static_cast<double>(y) + mid * static_cast<double>(CY[q])))
+                            hi = mid;
+                        else
+                            lo = mid;
+                    }
+                    if (hi < best) best = hi;
+                }
+                s.ibb.push_back(IbbLink{x, y, q, best});
+            }
+        }
+}
+
+// InterpolatedBounceBack::apply (lbm/interpolated_bounce_back.cpp:55-96), links
+// in list order
+void ibb_apply(sfo_sim& s) {
+    const int N = s.N, nx = s.nx, ny = s.ny;
+    auto F = [&](int x, int y, int q) -> double& { return s.f[static_cast<size_t>(q) * N + y * nx + x]; };
+    for (const IbbLink& L : s.ibb) {
+        const int o = OPP[L.q];
+        if (L.qf < 0.5) {
+            int xu = L.x - CX[L.q], yu = L.y - CY[L.q];
+            if (s.periodic_x) xu = ((xu % nx) + nx) % nx;
+            if (xu >= 0 && xu < nx && yu >= 0 && yu < ny)
+                F(L.x, L.y, o) = 2.0 * L.qf * F(L.x, L.y, L.q) + (1.0 - 2.0 * L.qf) * F(xu, yu, L.q);
+            else
+                F(L.x, L.y, o) = F(L.x, L.y, L.q);
+        } else {
+            double inv2q = 1.0 / (2.0 * L.qf);
+            F(L.x, L.y, o) = inv2q * F(L.x, L.y, L.q) + (1.0 - inv2q) * F(L.x, L.y, o);
+        }
+    }
+}
+
+// RepulsionForce::computeObstacleRepulsion (membrane/repulsion.cpp:180-236)
+void obstacle_repulsion(Caps& c, const sfo_params& p, const OObstacle& ob) {
+    const int K = c.off[c.n()];
+    for (int k = 0; k < K; ++k) {
+        double sd = ob.signed_distance(c.x[k], c.y[k]);
+        if (sd >= p.rep_r_cut) continue;
+        double r = std::abs(sd);
+        if (r < 1e-15) r = 1e-15;
+        V2 nn = ob.normal(c.x[k], c.y[k]);
+        double Frep = p.rep_epsilon * std::pow(p.rep_sigma / r, p.rep_power);
+        if (sd < 0.0) {
+            c.fx[k] += nn.x * (Frep * 10.0);
+            c.fy[k] += nn.y * (Frep * 10.0);
+            continue;
+        }
+        double Fn = Frep;
+        V2 Ft{0.0, 0.0};
+        if (p.rep_damping_normal > 0.0 || p.rep_friction_coeff > 0.0) {
+            double vn = c.vx[k] * nn.x + c.vy[k] * nn.y;
+            Fn = Frep - p.rep_damping_normal * vn;
+            if (Fn < 0.0) Fn = 0.0;
+            if (p.rep_friction_coeff > 0.0) {
+                V2 vt{c.vx[k] - nn.x * vn, c.vy[k] - nn.y * vn};
+                double m = norm(vt);
+                if (m > 1e-12) Ft = V2{vt.x / m * (-p.rep_friction_coeff * Fn), vt.y / m * (-p.rep_friction_coeff * Fn)};
+            }
+        }
+        c.fx[k] += nn.x * Fn + Ft.x;
+        c.fy[k] += nn.y * Fn + Ft.y;
+    }
+}
+
 void lbm_step(sfo_sim& s) {
     const int N = s.N;
     const sfo_params& p = s.p;
@@ -953,6 +1144,7 @@ void lbm_step(sfo_sim& s) {
         }
     }
     moving_walls(s);
+    if (p.use_interpolated_bb) ibb_apply(s);                       // lbm_solver.cpp:193-196
     if (p.boundary_type == 0) zou_he(s);
     macroscopic(s);
 }
@@ -1117,6 +1309,8 @@ int sfo_add_capsule_random(sfo_sim* s, int count, double x0, double y0, double x
             if (dist < r + orr + min_gap) { overlap = true; break; }
         }
         if (cy - r < wall_bot + min_gap || cy + r > wall_top - min_gap) overlap = true;
+        for (const OObstacle& ob : s->obs)
+            if (ob.signed_distance(cx, cy) < r + min_gap) { overlap = true; break; }
         if (!overlap) {
             int nn = num_nodes > 0 ? num_nodes
                                    : std::max(static_cast<int>(std::ceil(2.0 * kPI * r / 0.75)), 12);
@@ -1130,8 +1324,43 @@ int sfo_add_capsule_random(sfo_sim* s, int count, double x0, double y0, double x
 
 // Simulation::initialize + LBMSolver::initialize: simulation.cpp:429-468,
 // lbm_solver.cpp:130-170, channel_builder.cpp:6-36
+void sfo_add_obstacle(sfo_sim* s, int kind, const double* c, int n) {
+    OObstacle o{};
+    o.kind = kind;
+    if (kind == 0) { o.a[0] = c[0]; o.a[1] = c[1]; o.a[2] = c[2]; }
+    else if (kind == 1) {                                          // rect_obstacle.h:10-12
+        o.a[0] = std::min(c[0], c[2]); o.a[1] = std::min(c[1], c[3]);
+        o.a[2] = std::max(c[0], c[2]); o.a[3] = std::max(c[1], c[3]);
+    } else {
+        for (int i = 0; i + 1 < n; i += 2) o.poly.push_back(V2{c[i], c[i + 1]});
+    }
+    s->obs.push_back(o);
+}
+
+double sfo_obstacle_distance(const sfo_sim* s, int i, double x, double y, double* normal) {
+    const OObstacle& o = s->obs[i];
+    if (normal) { V2 nn = o.normal(x, y); normal[0] = nn.x; normal[1] = nn.y; }
+    return o.signed_distance(x, y);
+}
+
+int sfo_ibb_links(const sfo_sim* s, int* xyq, double* qf) {
+    const int n = static_cast<int>(s->ibb.size());
+    for (int i = 0; i < n; ++i) {
+        if (xyq) { xyq[3 * i] = s->ibb[i].x; xyq[3 * i + 1] = s->ibb[i].y; xyq[3 * i + 2] = s->ibb[i].q; }
+        if (qf) qf[i] = s->ibb[i].qf;
+    }
+    return n;
+}
+
 void sfo_initialize(sfo_sim* s) {
     const int nx = s->nx, ny = s->ny;
+    for (int y = 0; y < ny; ++y)                                   // channel_builder.cpp:27-33
+        for (int x = 0; x < nx; ++x)
+            for (const OObstacle& o : s->obs)
+                if (o.contains(static_cast<double>(x), static_cast<double>(y))) {
+                    s->flags[y * nx + x] = SOLID;
+                    break;
+                }
     for (int x = 0; x < nx; ++x) { s->flags[x] = SOLID; s->flags[(ny - 1) * nx + x] = SOLID; }
     if (s->p.boundary_type == 0) {
         for (int y = 1; y < ny - 1; ++y) { s->flags[y * nx] = INLET; s->flags[y * nx + nx - 1] = OUTLET; }
@@ -1150,6 +1379,7 @@ void sfo_initialize(sfo_sim* s) {
     macroscopic(*s);
     s->solid.clear();
     for (int n = 0; n < N; ++n) if (s->flags[n] == SOLID) s->solid.push_back(n);
+    if (s->p.use_interpolated_bb) ibb_build(*s);                   // simulation.cpp:440-442
     if (s->nsp) sfo_scalar_initialize(s, 0.0);
     if (s->periodic_x) for (auto& v : s->caps.periodic_nx) v = s->nx;
     s->step_count = 0;
@@ -1184,6 +1414,7 @@ void sfo_step(sfo_sim* s, int nsteps) {
         std::fill(c.fy.begin(), c.fy.end(), 0.0);
         for (int cap = 0; cap < c.n(); ++cap) membrane_forces(c, cap);   // :481
         repulsion_all(c, R, 0.5, static_cast<double>(p.ny) - 1.5);   // :486-497
+        for (const OObstacle& ob : s->obs) obstacle_repulsion(c, p, ob);
         if (p.apply_gravity_to_capsules && (p.gravity_x != 0.0 || p.gravity_y != 0.0)) {
             for (int cap = 0; cap < c.n(); ++cap) {                 // :516-527, capsule.cpp:673-690
                 double drho = c.density[cap] - p.rho0;

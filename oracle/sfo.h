@@ -43,6 +43,8 @@ typedef struct {
     double top_wall_velocity, bottom_wall_velocity;
     /* Zou-He inlet/outlet (boundary_type 0) */
     double inlet_velocity, outlet_density;
+    /* Bouzidi links towards the obstacles (parameters.h:89) */
+    int    use_interpolated_bb;
 } sfo_params;
 
 typedef struct {
@@ -72,6 +74,16 @@ int    sfo_add_capsule_random(sfo_sim* s, int count, double x0, double y0,
                               double x1, double y1, double rmin, double rmax,
                               int num_nodes, const sfo_membrane* m, int type,
                               int seed, double min_gap, int max_attempts);
+/* An obstacle of the channel builder (geometry/{circle,rect,polygon}_obstacle.h):
+ * kind 0 circle (cx, cy, r), 1 rectangle (x0, y0, x1, y1), 2 polygon (x, y)
+ * per vertex; n = number of coordinates.  Added before sfo_initialize, which
+ * marks the cells it contains SOLID (channel_builder.cpp:27-33). */
+void   sfo_add_obstacle(sfo_sim* s, int kind, const double* coords, int n);
+/* Obstacle::signedDistance / normalAt (out[2]) of obstacle i. */
+double sfo_obstacle_distance(const sfo_sim* s, int i, double x, double y, double* normal);
+/* The interpolated bounce-back links built by sfo_initialize: count, and
+ * (x, y, q) + wall distance of each when the arrays are not NULL. */
+int    sfo_ibb_links(const sfo_sim* s, int* xyq, double* qf);
 void   sfo_initialize(sfo_sim* s);
 void   sfo_scalar_initialize(sfo_sim* s, double c0);
 void   sfo_set_scalar_point(sfo_sim* s, int x, int y, double c, int species);

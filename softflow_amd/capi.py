@@ -22,7 +22,7 @@ STAGES = {"forces": 0, "ibm": 1, "lbm": 2, "scalar": 3, "move": 4}
 
 # every entry point the header declares (checked by tests/test_capi_cpu.py)
 EXPORTS = (
-    "sf_last_error", "sf_device_count", "sf_create", "sf_destroy", "sf_set_flags",
+    "sf_last_error", "sf_device_count", "sf_create", "sf_destroy", "sf_set_flags", "sf_set_obstacles",
     "sf_set_populations", "sf_get_populations", "sf_get_macroscopic", "sf_set_macroscopic",
     "sf_set_capsules", "sf_get_nodes", "sf_set_nodes", "sf_get_membrane_state",
     "sf_get_force_state", "sf_set_force_state",
@@ -53,6 +53,7 @@ class Params(C.Structure):
         ("scalar_periodic_y", C.c_int), ("precision", C.c_int),
         ("top_wall_velocity", C.c_double), ("bottom_wall_velocity", C.c_double),
         ("inlet_velocity", C.c_double), ("outlet_density", C.c_double),
+        ("use_interpolated_bb", C.c_int),
     ]
 
 
@@ -87,6 +88,7 @@ def load():
         "sf_create": [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Params)],
         "sf_destroy": [C.c_void_p],
         "sf_set_flags": [C.c_void_p, _IP],
+        "sf_set_obstacles": [C.c_void_p, C.c_int, _IP, _IP, _DP],
         "sf_set_populations": [C.c_void_p, _DP],
         "sf_get_populations": [C.c_void_p, _DP],
         "sf_get_macroscopic": [C.c_void_p, _DP, _DP, _DP, _DP, _DP],
@@ -215,6 +217,21 @@ class Context:
     def set_flags(self, flags):
         a = np.ascontiguousarray(flags, dtype=np.int32).reshape(-1)
         _check(self.L.sf_set_flags(self.h, a.ctypes.data_as(_IP)), "sf_set_flags")
+
+    def set_obstacles(self, obstacles):
+        """obstacles: [("circle", cx, cy, r) | ("rect", x0, y0, x1, y1) | ("polygon", [(x, y), ...])]"""
+        kinds, offs, coords = [], [0], []
+        for o in obstacles:
+            kind = {"circle": 0, "rect": 1, "polygon": 2}[o[0]]
+            vals = [float(v) for xy in o[1] for v in xy] if kind == 2 else [float(v) for v in o[1:]]
+            kinds.append(kind)
+            coords.extend(vals)
+            offs.append(len(coords))
+        k = np.asarray(kinds, dtype=np.int32)
+        f = np.asarray(offs, dtype=np.int32)
+        c = np.asarray(coords if coords else [0.0], dtype=np.float64)
+        _check(self.L.sf_set_obstacles(self.h, len(kinds), k.ctypes.data_as(_IP), f.ctypes.data_as(_IP),
+                                       _dp(c)), "sf_set_obstacles")
 
     def set_populations(self, f):
         a = _f64(f).reshape(-1)

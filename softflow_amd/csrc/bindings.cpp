@@ -289,11 +289,43 @@ PYBIND11_MODULE(softflow_core, m) {
             a.setField(C.data(), species);
         }, py::arg("C"), py::arg("species") = 0);
 
+    // geometry: obstacles and the channel builder (python/bindings.cpp:602-652)
+    py::class_<Obstacle, std::shared_ptr<Obstacle>>(m, "Obstacle")
+        .def("contains", &Obstacle::contains, py::arg("x"), py::arg("y"))
+        .def("signedDistance", &Obstacle::signedDistance, py::arg("x"), py::arg("y"))
+        .def("nearestPoint", &Obstacle::nearestPoint, py::arg("x"), py::arg("y"))
+        .def("normalAt", &Obstacle::normalAt, py::arg("x"), py::arg("y"));
+    py::class_<CircleObstacle, Obstacle, std::shared_ptr<CircleObstacle>>(m, "CircleObstacle")
+        .def(py::init<Real, Real, Real>(), py::arg("cx"), py::arg("cy"), py::arg("radius"))
+        .def("getCx", &CircleObstacle::getCx)
+        .def("getCy", &CircleObstacle::getCy)
+        .def("getRadius", &CircleObstacle::getRadius);
+    py::class_<RectObstacle, Obstacle, std::shared_ptr<RectObstacle>>(m, "RectObstacle")
+        .def(py::init<Real, Real, Real, Real>(), py::arg("x0"), py::arg("y0"), py::arg("x1"), py::arg("y1"))
+        .def("getX0", &RectObstacle::getX0)
+        .def("getY0", &RectObstacle::getY0)
+        .def("getX1", &RectObstacle::getX1)
+        .def("getY1", &RectObstacle::getY1);
+    py::class_<PolygonObstacle, Obstacle, std::shared_ptr<PolygonObstacle>>(m, "PolygonObstacle")
+        .def(py::init<const std::vector<Vec2d>&>(), py::arg("vertices"));
+
     py::class_<ChannelBuilder>(m, "ChannelBuilder")
         .def(py::init<int, int>(), py::arg("nx"), py::arg("ny"))
         .def("addWalls", &ChannelBuilder::addWalls, py::return_value_policy::reference_internal)
+        .def("addCirclePillar", &ChannelBuilder::addCirclePillar, py::arg("cx"), py::arg("cy"),
+             py::arg("radius"), py::return_value_policy::reference_internal)
+        .def("addRectPillar", &ChannelBuilder::addRectPillar, py::arg("x0"), py::arg("y0"), py::arg("x1"),
+             py::arg("y1"), py::return_value_policy::reference_internal)
+        .def("addPolygonPillar", &ChannelBuilder::addPolygonPillar, py::arg("vertices"),
+             py::return_value_policy::reference_internal)
         .def("setBoundaryType", &ChannelBuilder::setBoundaryType, py::arg("bt"),
              py::return_value_policy::reference_internal)
+        .def("getBoundaryType", &ChannelBuilder::getBoundaryType)
+        .def("hasTopWall", &ChannelBuilder::hasTopWall)
+        .def("hasBottomWall", &ChannelBuilder::hasBottomWall)
+        .def("getNx", &ChannelBuilder::getNx)
+        .def("getNy", &ChannelBuilder::getNy)
+        .def("getObstacles", &ChannelBuilder::getObstacles)
         .def("wallBottom", &ChannelBuilder::wallBottom)
         .def("wallTop", &ChannelBuilder::wallTop);
 

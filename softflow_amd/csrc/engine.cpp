@@ -369,6 +369,48 @@ void Simulation::check(int rc, const char* what) const {
     if (rc != SF_OK) throw std::runtime_error(std::string(what) + ": " + sf_last_error());
 }
 
+CircleObstacle::CircleObstacle(Real cx, Real cy, Real radius) {
+    o_.kind = sf::OBS_CIRCLE;
+    o_.p[0] = cx; o_.p[1] = cy; o_.p[2] = radius;
+}
+
+RectObstacle::RectObstacle(Real x0, Real y0, Real x1, Real y1) {   // rect_obstacle.h:10-12
+    o_.kind = sf::OBS_RECT;
+    o_.p[0] = std::min(x0, x1); o_.p[1] = std::min(y0, y1);
+    o_.p[2] = std::max(x0, x1); o_.p[3] = std::max(y0, y1);
+}
+
+PolygonObstacle::PolygonObstacle(const std::vector<Vec2d>& vertices) {
+    if (vertices.size() < 3) throw std::invalid_argument("PolygonObstacle: at least 3 vertices");
+    o_.kind = sf::OBS_POLYGON;
+    o_.nv = static_cast<int>(vertices.size());
+    for (const Vec2d& v : vertices) { verts_.push_back(v.x); verts_.push_back(v.y); }
+}
+
+std::vector<double> Obstacle::coords() const {
+    if (o_.kind == sf::OBS_CIRCLE) return {o_.p[0], o_.p[1], o_.p[2]};
+    if (o_.kind == sf::OBS_RECT) return {o_.p[0], o_.p[1], o_.p[2], o_.p[3]};
+    return verts_;
+}
+
+std::vector<int32_t> ChannelBuilder::cellTypes(int nx, int ny, BoundaryType bt) const {
+    std::vector<int32_t> fl(static_cast<size_t>(nx) * ny);
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            CellType ct = CellType::FLUID;
+            if (bottom_ && y == 0) ct = CellType::SOLID;
+            if (top_ && y == ny - 1) ct = CellType::SOLID;
+            if (bt == BoundaryType::INLET_OUTLET) {
+                if (x == 0 && ct == CellType::FLUID) ct = CellType::INLET;
+                if (x == nx - 1 && ct == CellType::FLUID) ct = CellType::OUTLET;
+            }
+            for (const auto& o : obstacles_)
+                if (o->contains(static_cast<Real>(x), static_cast<Real>(y))) { ct = CellType::SOLID; break; }
+            fl[static_cast<size_t>(y) * nx + x] = static_cast<int32_t>(ct);
+        }
+    return fl;
+}
+
 void Simulation::setChannelBuilder(const ChannelBuilder& b) { builder_ = std::make_unique<ChannelBuilder>(b); }
 
 void Simulation::addCapsule(Vec2d center, Real radius, int num_nodes, const MembraneParams& mp, int type) {
@@ -395,6 +437,8 @@ int Simulation::addCapsuleRandom(int count, Real x0, Real y0, Real x1, Real y1, 
         }
         if (builder_) {
             if (cy - r < builder_->wallBottom() + min_gap || cy + r > builder_->wallTop() - min_gap) overlap = true;
+            for (const auto& o : builder_->getObstacles())
+                if (o->signedDistance(cx, cy) < r + min_gap) { overlap = true; break; }
         }
         if (!overlap) {
             int nn = (num_nodes > 0) ? num_nodes : Capsule::computeOptimalNodes(r);
@@ -414,8 +458,6 @@ void Simulation::initialize() {
         throw std::invalid_argument("Simulation: gravity is not implemented on the MI355X path");
     if (P.lubrication.enabled || P.adhesion.enabled)
         throw std::invalid_argument("Simulation: lubrication/adhesion are not implemented on the MI355X path");
-    if (P.fluid.use_interpolated_bb)
-        throw std::invalid_argument("Simulation: interpolated bounce-back is not implemented on the MI355X path");
 
     sf_params sp{};
     sp.nx = P.nx; sp.ny = P.ny; sp.dt = P.dt; sp.ibm_iterations = P.ibm_iterations;
@@ -445,27 +487,45 @@ void Simulation::initialize() {
     sp.bottom_wall_velocity = P.fluid.bottom_wall_velocity;
     sp.inlet_velocity = P.fluid.inlet_velocity;
     sp.outlet_density = P.fluid.outlet_density;
+    // links are only built from a channel builder's geometry (simulation.cpp:440-442)
+    sp.use_interpolated_bb = (P.fluid.use_interpolated_bb && builder_) ? 1 : 0;
 
     if (ctx_) { sf_destroy(ctx_); ctx_ = nullptr; }
     check(sf_create(&ctx_, P.device, &sp), "sf_create");
     check(sf_set_graph_mode(ctx_, P.use_graph ? 1 : 0), "sf_set_graph_mode");
 
-    // cell types: lbm_solver.cpp:130-158 (with the channel builder's walls)
+    // cell types: the channel builder's (simulation.cpp:431-434), then
+    // LBMSolver::initialize's rows and columns over them (lbm_solver.cpp:130-158)
     LatticeField& F = *field_;
     const int nx = P.nx, ny = P.ny;
-    for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-            int ct = static_cast<int>(CellType::FLUID);
-            if (y == 0 || y == ny - 1) ct = static_cast<int>(CellType::SOLID);
-            if (P.fluid.boundary_type == BoundaryType::CLOSED && (x == 0 || x == nx - 1)) ct = static_cast<int>(CellType::SOLID);
-            if (P.fluid.boundary_type == BoundaryType::INLET_OUTLET && y >= 1 && y <= ny - 2 && (x == 0 || x == nx - 1))
-                ct = static_cast<int>(x == 0 ? CellType::INLET : CellType::OUTLET);
-            if (P.fluid.periodic_y && (y == 0 || y == ny - 1) &&
-                !(P.fluid.boundary_type == BoundaryType::CLOSED && (x == 0 || x == nx - 1)))
-                ct = static_cast<int>(CellType::FLUID);
-            F.flags_[static_cast<size_t>(y) * nx + x] = ct;
+    const int SOLID = static_cast<int>(CellType::SOLID), FLUID = static_cast<int>(CellType::FLUID);
+    if (builder_) F.flags_ = builder_->cellTypes(nx, ny, P.fluid.boundary_type);
+    else std::fill(F.flags_.begin(), F.flags_.end(), FLUID);
+    auto at = [&](int x, int y) -> int32_t& { return F.flags_[static_cast<size_t>(y) * nx + x]; };
+    for (int x = 0; x < nx; ++x) at(x, 0) = at(x, ny - 1) = SOLID;
+    if (P.fluid.boundary_type == BoundaryType::INLET_OUTLET) {
+        for (int y = 1; y < ny - 1; ++y) {
+            at(0, y) = static_cast<int>(CellType::INLET);
+            at(nx - 1, y) = static_cast<int>(CellType::OUTLET);
         }
+    } else if (P.fluid.boundary_type == BoundaryType::CLOSED) {
+        for (int y = 0; y < ny; ++y) at(0, y) = at(nx - 1, y) = SOLID;
+    }
+    if (P.fluid.periodic_y)
+        for (int x = 0; x < nx; ++x) at(x, 0) = at(x, ny - 1) = FLUID;
     check(sf_set_flags(ctx_, F.flags_.data()), "sf_set_flags");
+    if (builder_ && !builder_->getObstacles().empty()) {
+        std::vector<int32_t> kinds, offs{0};
+        std::vector<double> coords;
+        for (const auto& o : builder_->getObstacles()) {
+            kinds.push_back(o->kind());
+            const std::vector<double> c = o->coords();
+            coords.insert(coords.end(), c.begin(), c.end());
+            offs.push_back(static_cast<int32_t>(coords.size()));
+        }
+        check(sf_set_obstacles(ctx_, static_cast<int>(kinds.size()), kinds.data(), offs.data(), coords.data()),
+              "sf_set_obstacles");
+    }
     // equilibrium at rest (lattice_field.cpp:75-90)
     F.stale_ = false;
     for (int n = 0; n < F.N_; ++n) {

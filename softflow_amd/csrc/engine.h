@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/softflow.h"
+#include "sf_geometry.h"
 
 namespace softflow {
 
@@ -255,21 +256,94 @@ private:
     bool stale_ = false, dirty_ = true;
 };
 
-// Walls and boundary type of the channel (geometry/channel_builder.h).
+// Obstacles (geometry/obstacle.h, circle_obstacle.h, rect_obstacle.h,
+// polygon_obstacle.h).  The geometry is the one the device path evaluates in
+// its contact forces (sf_geometry.h), so placement, cell flags and forces agree.
+class Obstacle {
+public:
+    virtual ~Obstacle() = default;
+    bool contains(Real x, Real y) const { return sf::obstacle_contains(geo(), x, y); }
+    Real signedDistance(Real x, Real y) const { return sf::obstacle_signed_distance(geo(), x, y); }
+    Vec2d nearestPoint(Real x, Real y) const {
+        const sf::ONearest q = sf::obstacle_nearest_point(geo(), x, y);
+        return {q.x, q.y};
+    }
+    Vec2d normalAt(Real x, Real y) const {
+        const sf::ONearest q = sf::obstacle_normal(geo(), x, y);
+        return {q.x, q.y};
+    }
+    // the C-ABI description: kind and coordinates (sf_set_obstacles)
+    int kind() const { return o_.kind; }
+    std::vector<double> coords() const;
+
+protected:
+    sf::Obstacle geo() const {
+        sf::Obstacle o = o_;
+        o.v = verts_.data();
+        return o;
+    }
+    sf::Obstacle o_{};
+    std::vector<double> verts_;
+};
+
+class CircleObstacle : public Obstacle {
+public:
+    CircleObstacle(Real cx, Real cy, Real radius);
+    Real getCx() const { return o_.p[0]; }
+    Real getCy() const { return o_.p[1]; }
+    Real getRadius() const { return o_.p[2]; }
+};
+
+class RectObstacle : public Obstacle {
+public:
+    RectObstacle(Real x0, Real y0, Real x1, Real y1);
+    Real getX0() const { return o_.p[0]; }
+    Real getY0() const { return o_.p[1]; }
+    Real getX1() const { return o_.p[2]; }
+    Real getY1() const { return o_.p[3]; }
+};
+
+class PolygonObstacle : public Obstacle {
+public:
+    explicit PolygonObstacle(const std::vector<Vec2d>& vertices);
+};
+
+// Walls, boundary type and obstacles of the channel (geometry/channel_builder.h).
 class ChannelBuilder {
 public:
     ChannelBuilder(int nx, int ny) : nx_(nx), ny_(ny) {}
     ChannelBuilder& addWalls() { bottom_ = top_ = true; return *this; }
+    ChannelBuilder& addCirclePillar(Real cx, Real cy, Real radius) {
+        obstacles_.push_back(std::make_shared<CircleObstacle>(cx, cy, radius));
+        return *this;
+    }
+    ChannelBuilder& addRectPillar(Real x0, Real y0, Real x1, Real y1) {
+        obstacles_.push_back(std::make_shared<RectObstacle>(x0, y0, x1, y1));
+        return *this;
+    }
+    // no counterpart in the reference's builder, which holds polygons only
+    // through its obstacle list: a polygonal pillar
+    ChannelBuilder& addPolygonPillar(const std::vector<Vec2d>& vertices) {
+        obstacles_.push_back(std::make_shared<PolygonObstacle>(vertices));
+        return *this;
+    }
     ChannelBuilder& setBoundaryType(BoundaryType bt) { bt_ = bt; return *this; }
+    BoundaryType getBoundaryType() const { return bt_; }
     Real wallBottom() const { return 0.0; }
     Real wallTop() const { return static_cast<Real>(ny_ - 1); }
     bool hasBottomWall() const { return bottom_; }
     bool hasTopWall() const { return top_; }
+    int getNx() const { return nx_; }
+    int getNy() const { return ny_; }
+    const std::vector<std::shared_ptr<Obstacle>>& getObstacles() const { return obstacles_; }
+    // ChannelBuilder::applyToField (channel_builder.cpp:6-36) on an nx x ny field
+    std::vector<int32_t> cellTypes(int nx, int ny, BoundaryType bt) const;
 
 private:
     int nx_, ny_;
     bool bottom_ = false, top_ = false;
     BoundaryType bt_ = BoundaryType::INLET_OUTLET;
+    std::vector<std::shared_ptr<Obstacle>> obstacles_;
 };
 
 class Simulation;

@@ -277,6 +277,116 @@ static int dmalloc(T** p, size_t bytes) {
     return SF_OK;
 }
 
+// InterpolatedBounceBack::initialize (lbm/interpolated_bounce_back.cpp:7-53):
+// a link for every direction from a non-SOLID cell into an in-bounds SOLID
+// cell, its wall distance the smallest over the obstacles of a 20-step
+// bisection along the link (1 when no obstacle is met).  Groups (a cell's
+// links) that read a slot another group writes -- the upstream slot of a link
+// nearer than half a cell, written by the opposite link of the upstream cell
+// -- go to the serial tail, in the reference's cell order.
+int build_ibb_links(Ctx& c) {
+    invalidate_graph(c);
+    for (int** p : {&c.ibb_cell, &c.ibb_off, &c.ibb_q, &c.ibb_up})
+        if (*p) { hipFree(*p); *p = nullptr; }
+    if (c.ibb_qf) { hipFree(c.ibb_qf); c.ibb_qf = nullptr; }
+    c.ibb_groups = c.ibb_par = 0;
+    const int nx = c.nx, ny = c.ny;
+    const std::vector<uint8_t>& fl = c.h_flags;
+    std::vector<Obstacle> obs = c.h_obs;
+    for (Obstacle& o : obs)
+        if (o.kind == OBS_POLYGON) o.v = c.h_obs_v.data() + static_cast<size_t>(o.p[0]);
+    struct Link { int q, up; double qf; };
+    std::vector<int> gcell, goff{0};
+    std::vector<Link> links;
+    std::vector<int> group_of(static_cast<size_t>(c.N), -1);
+    std::vector<uint16_t> dirs;   // per group: bit q set when the group has a link in direction q
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            const int n = y * nx + x;
+            if (fl[n] == SF_SOLID) continue;
+            const size_t first = links.size();
+            uint16_t mask = 0;
+            for (int q = 1; q < 9; ++q) {
+                const int xn = x + dcx(q), yn = y + dcy(q);
+                if (xn < 0 || xn >= nx || yn < 0 || yn >= ny) continue;
+                if (fl[static_cast<size_t>(yn) * nx + xn] != SF_SOLID) continue;
+                const double x0 = static_cast<double>(x), y0 = static_cast<double>(y);
+                const double dx = static_cast<double>(dcx(q)), dy = static_cast<double>(dcy(q));
+                double min_q = 1.0;
+                for (const Obstacle& o : obs) {
+                    double lo = 0.0, hi = 1.0;
+                    for (int it = 0; it < 20; ++it) {
+                        const double mid = 0.5 * (lo + hi);
+                        if (obstacle_contains(o, x0 + mid * dx, y0 + mid * dy)) hi = mid;
+                        else lo = mid;
+                    }
+                    if (hi < min_q) min_q = hi;
+                }
+                int up = -1;
+                if (min_q < 0.5) {
+                    int xu = x - dcx(q);
+                    const int yu = y - dcy(q);
+                    if (c.periodic_x) xu = ((xu % nx) + nx) % nx;
+                    if (xu >= 0 && xu < nx && yu >= 0 && yu < ny) {
+                        up = yu * nx + xu;
+                        if (fl[up] == SF_SOLID)
+                            return fail(SF_E_ARG, "interpolated bounce-back: the fluid gap at cell (" +
+                                                      std::to_string(x) + ", " + std::to_string(y) +
+                                                      ") is one cell wide (its upstream cell is SOLID)");
+                    }
+                }
+                links.push_back({q, up, min_q});
+                mask |= static_cast<uint16_t>(1u << q);
+            }
+            if (links.size() > first) {
+                group_of[n] = static_cast<int>(gcell.size());
+                gcell.push_back(n);
+                goff.push_back(static_cast<int>(links.size()));
+                dirs.push_back(mask);
+            }
+        }
+    const int G = static_cast<int>(gcell.size());
+    if (G == 0) return SF_OK;
+    std::vector<uint8_t> coupled(G, 0);
+    for (int g = 0; g < G; ++g)
+        for (int l = goff[g]; l < goff[g + 1]; ++l) {
+            const Link& L = links[l];
+            if (L.qf >= 0.5 || L.up < 0) continue;
+            const int w = group_of[L.up];
+            // the upstream cell's link in the opposite direction writes slot (up, q)
+            if (w >= 0 && (dirs[w] >> dopp(L.q) & 1)) coupled[g] = coupled[w] = 1;
+        }
+    std::vector<int> cell2, off2{0}, q2, up2;
+    std::vector<double> qf2;
+    int par = 0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int g = 0; g < G; ++g) {
+            if (coupled[g] != pass) continue;
+            cell2.push_back(gcell[g]);
+            for (int l = goff[g]; l < goff[g + 1]; ++l) {
+                q2.push_back(links[l].q);
+                up2.push_back(links[l].up);
+                qf2.push_back(links[l].qf);
+            }
+            off2.push_back(static_cast<int>(q2.size()));
+            if (pass == 0) ++par;
+        }
+    const size_t L = q2.size();
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.ibb_cell), G * sizeof(int)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.ibb_off), (G + 1) * sizeof(int)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.ibb_q), L * sizeof(int)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.ibb_up), L * sizeof(int)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.ibb_qf), L * sizeof(double)));
+    SF_HIP(hipMemcpy(c.ibb_cell, cell2.data(), G * sizeof(int), hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(c.ibb_off, off2.data(), (G + 1) * sizeof(int), hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(c.ibb_q, q2.data(), L * sizeof(int), hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(c.ibb_up, up2.data(), L * sizeof(int), hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(c.ibb_qf, qf2.data(), L * sizeof(double), hipMemcpyHostToDevice));
+    c.ibb_groups = G;
+    c.ibb_par = par;
+    return SF_OK;
+}
+
 static void free_caps(Ctx& c) {
     void* ptrs[] = {c.off, c.owner, c.desc, c.has_prev, c.x, c.y, c.vx, c.vy, c.fx, c.fy,
                     c.L0, c.th0, c.ds0, c.Lprev, c.spread, c.mdf_total, c.mdf_u0, c.mdf_hist,
@@ -366,6 +476,14 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
         c.inlet_outlet = true;
         c.generic_flags = true;
     }
+    if (c.walls_x && c.periodic_y) {
+        // periodic y clears both rows after the closed columns are set
+        // (lbm_solver.cpp:146-158): the four corners are FLUID
+        for (int x : {0, c.nx - 1})
+            for (int y : {0, c.ny - 1}) fl[static_cast<size_t>(y) * c.nx + x] = SF_FLUID;
+        c.generic_flags = true;
+    }
+    c.h_flags = fl;
     SF_HIP(hipMemcpy(c.flags, fl.data(), N, hipMemcpyHostToDevice));
     for (int b = 0; b < 3; ++b) SF_HIP(hipMemset(c.F[b], 0, 2 * N * es));
     SF_HIP(hipMemset(c.counters, 0, 64 * sizeof(int)));
@@ -402,6 +520,10 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
         c.c_from_store = true;
     }
     c.shift_next = SHIFT_ZERO;
+    if (p->use_interpolated_bb) {
+        const int r = build_ibb_links(c);
+        if (r != SF_OK) { sf_destroy(h); return r; }
+    }
     *out = h;
     return SF_OK;
 }
@@ -415,7 +537,8 @@ int sf_destroy(sf_ctx* h) {
     free_caps(c);
     void* ptrs[] = {c.flags, c.f[0], c.f[1], c.F[0], c.F[1], c.F[2], c.stored_rho, c.stored_u,
                     c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_act, c.tile_rec, c.tile_dirty, c.cell_cnt, c.cell_list,
-                    c.cell_pay, c.cell_own, c.mdf, c.ds_buf};
+                    c.cell_pay, c.cell_own, c.mdf, c.ds_buf, c.obs, c.obs_v, c.ibb_cell, c.ibb_off,
+                    c.ibb_q, c.ibb_up, c.ibb_qf};
     for (void* p : ptrs) if (p) hipFree(p);
     for (int s = 0; s < 4; ++s) {
         for (int b = 0; b < 2; ++b) if (c.g[s][b]) hipFree(c.g[s][b]);
@@ -451,9 +574,67 @@ int sf_set_flags(sf_ctx* h, const int32_t* flags) {
             io |= v == SF_INLET || v == SF_OUTLET;
         }
     SF_HIP(hipMemcpy(c.flags, fl.data(), c.N, hipMemcpyHostToDevice));
+    c.h_flags = fl;
     c.generic_flags = !channel;
     c.inlet_outlet = io;
     invalidate_graph(c);
+    if (c.p.use_interpolated_bb) return build_ibb_links(c);
+    return SF_OK;
+}
+
+int sf_set_obstacles(sf_ctx* h, int count, const int32_t* kinds, const int32_t* offsets,
+                     const double* coords) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    if (count < 0) return fail(SF_E_ARG, "negative obstacle count");
+    if (count > 0 && (!kinds || !offsets || !coords)) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    SF_HIP(hipSetDevice(c.device));
+    std::vector<Obstacle> obs(count);
+    std::vector<double> verts;
+    for (int i = 0; i < count; ++i) {
+        const int b = offsets[i], e = offsets[i + 1], n = e - b;
+        if (b < 0 || n < 0 || (i == 0 && b != 0)) return fail(SF_E_ARG, "obstacle offsets must start at 0 and not decrease");
+        Obstacle& o = obs[i];
+        o = Obstacle{};
+        o.kind = kinds[i];
+        if (o.kind == OBS_CIRCLE) {
+            if (n != 3) return fail(SF_E_ARG, "a circle takes (cx, cy, r)");
+            for (int k = 0; k < 3; ++k) o.p[k] = coords[b + k];
+        } else if (o.kind == OBS_RECT) {
+            if (n != 4) return fail(SF_E_ARG, "a rectangle takes (x0, y0, x1, y1)");
+            // rect_obstacle.h:10-12
+            o.p[0] = std::min(coords[b], coords[b + 2]);
+            o.p[1] = std::min(coords[b + 1], coords[b + 3]);
+            o.p[2] = std::max(coords[b], coords[b + 2]);
+            o.p[3] = std::max(coords[b + 1], coords[b + 3]);
+        } else if (o.kind == OBS_POLYGON) {
+            if (n < 6 || n % 2) return fail(SF_E_ARG, "a polygon takes at least 3 (x, y) vertices");
+            o.nv = n / 2;
+            o.p[0] = static_cast<double>(verts.size());   // offset, resolved below
+            verts.insert(verts.end(), coords + b, coords + e);
+        } else {
+            return fail(SF_E_ARG, "unknown obstacle kind");
+        }
+    }
+    if (c.obs) { hipFree(c.obs); c.obs = nullptr; }
+    if (c.obs_v) { hipFree(c.obs_v); c.obs_v = nullptr; }
+    c.h_obs = obs;
+    c.h_obs_v = verts;
+    c.nobs = 0;
+    if (count > 0) {
+        if (!verts.empty()) {
+            SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.obs_v), verts.size() * sizeof(double)));
+            SF_HIP(hipMemcpy(c.obs_v, verts.data(), verts.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        std::vector<Obstacle> dev = obs;
+        for (Obstacle& o : dev)
+            if (o.kind == OBS_POLYGON) o.v = c.obs_v + static_cast<size_t>(o.p[0]);
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.obs), dev.size() * sizeof(Obstacle)));
+        SF_HIP(hipMemcpy(c.obs, dev.data(), dev.size() * sizeof(Obstacle), hipMemcpyHostToDevice));
+        c.nobs = count;
+    }
+    invalidate_graph(c);
+    if (c.p.use_interpolated_bb) return build_ibb_links(c);
     return SF_OK;
 }
 

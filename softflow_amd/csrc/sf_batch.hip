@@ -156,6 +156,8 @@ static int validate(sf_batch& B) {
             return fail(SF_E_ARG, who + "collision model differs from member 0");
         if (!c.periodic_x || c.generic_flags)
             return fail(SF_E_ARG, who + "batches take x-periodic plain channels (no obstacle flags)");
+        if (c.nobs > 0 || c.p.use_interpolated_bb)
+            return fail(SF_E_ARG, who + "obstacles and interpolated bounce-back are not batched");
         if (c.nsp > 0) return fail(SF_E_ARG, who + "scalar transport is not batched");
         if (mdf_corrections_of(c) > 0) return fail(SF_E_ARG, who + "multi-direct forcing is not batched");
         if (c.ncaps == 0 || c.any_rigid)

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/softflow.h"
+#include "sf_geometry.h"
 
 namespace sf {
 
@@ -149,6 +150,22 @@ struct Ctx {
     int2* tile_act = nullptr;            // (tile, node count) of the tiles touched this step (counters[3])
     uint8_t* tile_dirty = nullptr;       // [3][ntiles]: tile of force buffer b holds non-zeros
 
+    // obstacles (contact forces) and interpolated bounce-back links
+    std::vector<uint8_t> h_flags;        // host copy of the cell types
+    std::vector<Obstacle> h_obs;         // polygon vertices: offsets into h_obs_v until uploaded
+    std::vector<double> h_obs_v;
+    int nobs = 0;
+    Obstacle* obs = nullptr;             // device copies, vertices pointing into obs_v
+    double* obs_v = nullptr;
+    bool ibb_dirty = false;              // links must be rebuilt before the next step
+    int ibb_groups = 0;                  // cells owning links, in the reference's (y, x) order
+    int ibb_par = 0;                     // the first ibb_par groups touch no other group's slots
+    int* ibb_cell = nullptr;             // cell of each group
+    int* ibb_off = nullptr;              // groups' link ranges (ibb_groups + 1)
+    int* ibb_q = nullptr;                // link direction into the solid
+    int* ibb_up = nullptr;               // upstream cell for q < 1/2, -1: halfway fallback
+    double* ibb_qf = nullptr;            // wall distance along the link, in [0, 1]
+
     // repulsion cells
     int ncx = 0, ncy = 0;
     double cdx = 1.0, cdy = 1.0, xorg = 0.0, Lx = 0.0;
@@ -222,6 +239,9 @@ void launch_adr(Ctx& c, int Fidx);
 void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy);
 void launch_field_stats(Ctx& c, double* dev_out);
 void launch_global_stats(Ctx& c, double rho0, double* dev_out);
+// interpolated bounce-back on the populations the last collide-stream wrote
+void launch_ibb(Ctx& c);
+int build_ibb_links(Ctx& c);
 void launch_membrane(Ctx& c);
 void launch_repulsion(Ctx& c);
 // Where the contact cell list applies (x-periodic, more than the all-pairs

@@ -641,8 +641,68 @@ __global__ void k_zou_he(LbmArgs a) {
     }
 }
 
+// InterpolatedBounceBack::apply (lbm/interpolated_bounce_back.cpp:55-96) on the
+// streamed populations.  A group is one cell's links in direction order, the
+// order of the reference's link list within a cell.  Groups [0, par) read and
+// write only their own cell's slots and slots no link writes: one thread each.
+// The rest (cells that read a slot another cell's link writes: fluid runs two
+// cells long between solids) run on one thread in the reference's cell order.
+// FP32 storage holds f - w: the update is affine with weights summing to one
+// and w is equal for opposite directions, so it applies to f - w unchanged.
+template <typename T>
+__device__ __forceinline__ void ibb_group(T* f, size_t N, int cell, int l0, int l1, const int* lq,
+                                          const int* lup, const double* lqf) {
+    for (int l = l0; l < l1; ++l) {
+        const int q = lq[l], o = dopp(q);
+        const double qf = lqf[l];
+        const double fq = static_cast<double>(f[q * N + cell]);
+        double r;
+        if (qf < 0.5) {
+            const int u = lup[l];
+            if (u >= 0) r = 2.0 * qf * fq + (1.0 - 2.0 * qf) * static_cast<double>(f[q * N + u]);
+            else r = fq;
+        } else {
+            const double inv2q = 1.0 / (2.0 * qf);
+            r = inv2q * fq + (1.0 - inv2q) * static_cast<double>(f[o * N + cell]);
+        }
+        f[o * N + cell] = static_cast<T>(r);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ibb(T* f, int N, int g0, int g1, int serial, const int* gcell,
+                                             const int* goff, const int* lq, const int* lup,
+                                             const double* lqf) {
+    if (serial) {
+        if (blockIdx.x != 0 || threadIdx.x != 0) return;
+        for (int g = g0; g < g1; ++g)
+            ibb_group<T>(f, static_cast<size_t>(N), gcell[g], goff[g], goff[g + 1], lq, lup, lqf);
+        return;
+    }
+    const int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= g1) return;
+    ibb_group<T>(f, static_cast<size_t>(N), gcell[g], goff[g], goff[g + 1], lq, lup, lqf);
+}
+
+void launch_ibb(Ctx& c) {
+    if (c.ibb_groups == 0) return;
+    void* f = c.f[1 - c.fcur];
+    auto run = [&](auto* fp) {
+        using T = std::remove_pointer_t<decltype(fp)>;
+        if (c.ibb_par > 0)
+            hipLaunchKernelGGL(k_ibb<T>, dim3((c.ibb_par + 255) / 256), dim3(256), 0, c.stream, fp, c.N, 0,
+                               c.ibb_par, 0, c.ibb_cell, c.ibb_off, c.ibb_q, c.ibb_up, c.ibb_qf);
+        if (c.ibb_groups > c.ibb_par)
+            hipLaunchKernelGGL(k_ibb<T>, dim3(1), dim3(64), 0, c.stream, fp, c.N, c.ibb_par, c.ibb_groups, 1,
+                               c.ibb_cell, c.ibb_off, c.ibb_q, c.ibb_up, c.ibb_qf);
+    };
+    if (c.fp32) run(static_cast<float*>(f));
+    else run(static_cast<double*>(f));
+}
+
 void launch_lbm(Ctx& c, int shift, bool fuse_adr) {
     launch_lbm_cfg(c, shift, fuse_adr, c.lbm_bx, c.lbm_by, c.lbm_remap, c.lbm_persist);
+    launch_ibb(c);
     if (c.inlet_outlet) {
         LbmArgs a = lbm_args(c, shift, false);
         a.fdst = c.f[1 - c.fcur];

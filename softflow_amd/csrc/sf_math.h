@@ -14,9 +14,10 @@ namespace sf {
 // product (error ~2^-104 relative before the final rounding), so the result is
 // the correctly rounded power except in vanishingly rare near-tie cases.  The
 // contact force evaluates pow(sigma/r, power) with an integer power
-// (membrane/repulsion.cpp:15); glibc's pow is correctly rounded for these
-// arguments in all but rare cases, and this reproduces it without depending on
-// the device libm's ulp behaviour.
+// (membrane/repulsion.cpp:15).  glibc's pow (0.52 ulp) agrees with the
+// correctly rounded value for all but about 1 in 1200 arguments of x^4 over
+// x in [0.28, 6.7]; this gives the correctly rounded value without depending
+// on the device libm's ulp behaviour.
 SF_HD inline double pow_int_cr(double x, int p) {
     if (p == 0) return 1.0;
     if (p == 1) return x;

@@ -319,7 +319,16 @@ struct RepArgs {
     // when set: distance from each node to the nearest node of another
     // capsule among those examined (capped at 2 r_cut)
     float* dmin;
+    // obstacles, after the wall images (simulation.cpp:495-499)
+    const Obstacle* obs;
+    int nobs;
 };
+
+__device__ __forceinline__ void obstacle_forces(const RepArgs& a, double px, double py, double vx,
+                                                double vy, double& fx, double& fy) {
+    for (int o = 0; o < a.nobs; ++o)
+        obstacle_contact(a.obs[o], px, py, vx, vy, a.eps, a.sig, a.pw, a.rcut, a.gn, a.mu, fx, fy);
+}
 
 __device__ __forceinline__ double pair_dist2(const RepArgs& a, double px, double py, double bx,
                                              double by) {
@@ -473,6 +482,7 @@ __device__ __forceinline__ void repulsion_body(RepArgs a, int bx) {
     fx += Fx; fy += Fy;
     pair_force(a, px, py, vx, vy, px, a.ytop, 0.0, 0.0, Fx, Fy);
     fx += Fx; fy += Fy;
+    obstacle_forces(a, px, py, vx, vy, fx, fy);
     a.fx[k] = fx;
     a.fy[k] = fy;
     if (a.dmin) a.dmin[k] = static_cast<float>(sqrt(d2min));
@@ -574,6 +584,7 @@ __device__ __forceinline__ void repulsion_lanes_body(RepArgs a, int bx) {
         pair_force(a, px, py, vx, vy, px, a.ytop, 0.0, 0.0, Fx, Fy);
         fx += Fx; fy += Fy;
     }
+    obstacle_forces(a, px, py, vx, vy, fx, fy);
     a.fx[k] = fx;
     a.fy[k] = fy;
     if (a.dmin) {
@@ -624,6 +635,8 @@ static RepArgs rep_args(const Ctx& c) {
     a.ybot = 0.5; a.ytop = static_cast<double>(c.ny) - 1.5;
     a.rcut2_hi = a.rcut * a.rcut * (1.0 + 1e-12);
     a.dmin = c.mdf_hist_on ? c.mdf_dmin : nullptr;
+    a.obs = c.obs;
+    a.nobs = c.nobs;
     return a;
 }
 

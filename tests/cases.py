@@ -12,6 +12,9 @@ A case is a dict:
             cells set after initialize (LatticeField::setCellType)   (optional)
     params: {name: value} further SimulationParams fields (product only)   (optional)
     walls: (top, bottom) x velocities of moving wall rows   (optional)
+    obstacles: list of ('circle', cx, cy, r) | ('rect', x0, y0, x1, y1) |
+               ('polygon', [(x, y), ...]): the channel builder's pillars   (optional)
+    ibb: True for interpolated bounce-back towards them   (optional)
 Membrane keys follow the oracle's struct (model, G_s, C_skalak, k_bend, use_helfrich, ...).
 """
 from __future__ import annotations
@@ -63,8 +66,11 @@ def oracle_sim(case, which="oracle"):
         rep_friction_coeff=rep.get("friction_coeff", 0.0),
         top_wall_velocity=case.get("walls", (0.0, 0.0))[0],
         bottom_wall_velocity=case.get("walls", (0.0, 0.0))[1],
+        use_interpolated_bb=1 if case.get("ibb") else 0,
     )
     sim = po.Sim(p, which)
+    for ob in case.get("obstacles", []):
+        sim.add_obstacle(ob)
     for spec in case.get("capsules", []):
         m = po.default_membrane(**spec.get("mem", {}))
         if "centers" in spec:
@@ -109,6 +115,7 @@ def product_sim(case, graph=True):
     p.repulsion.damping_normal = rep.get("damping_normal", 0.0)
     p.repulsion.friction_coeff = rep.get("friction_coeff", 0.0)
     p.fluid.top_wall_velocity, p.fluid.bottom_wall_velocity = case.get("walls", (0.0, 0.0))
+    p.fluid.use_interpolated_bb = bool(case.get("ibb", False))
     if s:
         p.scalar.enabled = True
         p.scalar.diffusivity = [s["D"]]
@@ -120,6 +127,13 @@ def product_sim(case, graph=True):
     sim = sc.Simulation(p)
     b = sc.ChannelBuilder(p.nx, p.ny)
     b.addWalls()
+    for ob in case.get("obstacles", []):
+        if ob[0] == "circle":
+            b.addCirclePillar(*ob[1:])
+        elif ob[0] == "rect":
+            b.addRectPillar(*ob[1:])
+        else:
+            b.addPolygonPillar([sc.Vec2d(float(x), float(y)) for x, y in ob[1]])
     sim.setChannelBuilder(b)
     models = [sc.MembraneModel.HOOKEAN, sc.MembraneModel.NEO_HOOKEAN, sc.MembraneModel.SKALAK,
               sc.MembraneModel.WLC]
@@ -196,3 +210,40 @@ def rel_l2(a, b):
     den = np.linalg.norm(b)
     num = np.linalg.norm(a - b)
     return float(num / den) if den > 0 else float(num)
+
+
+# Channels with the builder's pillars (circle, rectangle, polygon), with and
+# without interpolated bounce-back; run by the oracle-vs-reference and the
+# device parity tests.
+PILLARS = [("circle", 40.3, 20.6, 6.2), ("rect", 70.0, 9.5, 78.0, 17.2),
+           ("polygon", [(100.0, 24.0), (112.0, 21.5), (108.0, 31.0)])]
+OBSTACLE_CASES = dict(
+    pillars_caps=dict(nx=130, ny=40, coll=0, obstacles=PILLARS,
+                      capsules=[dict(count=10, region=(5, 5, 125, 35), rmin=3.0, rmax=4.0, seed=3,
+                                     mem=dict(model=2, G_s=0.06, C_skalak=10.0))],
+                      rep=dict(damping_normal=0.01, friction_coeff=0.1)),
+    ibb_pillars_mrt_caps=dict(nx=130, ny=40, coll=1, obstacles=PILLARS, ibb=True,
+                              capsules=[dict(count=8, region=(5, 5, 125, 35), rmin=3.0, rmax=4.0, seed=5,
+                                             mem=dict(model=2, G_s=0.06, C_skalak=10.0))]),
+    ibb_reg_mdf3_adr=dict(nx=130, ny=40, coll=2, mdf=3, obstacles=PILLARS[1:], ibb=True,
+                          scalar=dict(D=0.05, blob=(30, 20, 5.0)),
+                          capsules=[dict(count=6, region=(5, 5, 125, 35), rmin=3.0, rmax=4.0, seed=9,
+                                         mem=dict(model=2, G_s=0.06, C_skalak=10.0))]),
+    ibb_walls_only=dict(nx=60, ny=24, coll=0, ibb=True),
+    ibb_periodic_seam=dict(nx=60, ny=24, coll=0, obstacles=[("circle", 3.5, 11.8, 3.2)], ibb=True),
+    # a fluid run two cells long under a block: the links nearer than half a
+    # cell read slots the opposite links of their upstream cells write
+    ibb_two_cell_gap=dict(nx=40, ny=8, coll=1, obstacles=[("rect", 10.0, 2.3, 30.0, 5.0)], ibb=True),
+    ibb_inlet_outlet=dict(nx=90, ny=30, coll=0, boundary="inlet_outlet", inlet=(0.03, 1.0),
+                          obstacles=[("circle", 45.2, 14.7, 5.3)], ibb=True),
+    # capsules placed against each pillar: contact with damping and friction
+    obstacle_contact=dict(nx=130, ny=40, coll=0, obstacles=PILLARS,
+                          capsules=[dict(centers=[(40.3, 31.4), (74.0, 5.2), (81.9, 13.0), (104.0, 19.3)],
+                                         rmin=4.0, nodes=26, mem=dict(model=2, G_s=0.06, C_skalak=10.0))],
+                          rep=dict(damping_normal=0.01, friction_coeff=0.1)),
+    # capsules overlapping the pillars: nodes inside take the push-out branch
+    obstacle_inside=dict(nx=130, ny=40, coll=0, obstacles=PILLARS, steps=60,
+                         capsules=[dict(centers=[(40.3, 29.6), (74.0, 21.5), (109.5, 34.0)],
+                                        rmin=4.0, nodes=26, mem=dict(model=2, G_s=0.06, C_skalak=10.0))],
+                         rep=dict(damping_normal=0.01, friction_coeff=0.1)),
+)

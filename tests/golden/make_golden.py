@@ -5,7 +5,7 @@ the reference's own hot-path components (compiled from its sources by
 ``make -C oracle ref``) driven in Simulation::step order at one OpenMP thread,
 and stores inputs (the case dict) and outputs (fields, node state) as .npz.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--force]
 """
 import json
 import os
@@ -47,6 +47,19 @@ CASES = {
                                               dict(count=1, region=(8, 8, 56, 24), rmin=3.0, rmax=3.0,
                                                    nodes=16, seed=13, mem=dict(model=0, k_stretch=0.05,
                                                                                shape=2))]), 300),
+    # the channel builder's pillars with interpolated bounce-back; capsules
+    # against them (contact with damping and friction)
+    "pillars_ibb_mrt_caps": (dict(nx=96, ny=36, tau=0.8, bfx=3e-5, coll=1, ibb=True,
+                                  obstacles=[("circle", 30.3, 18.6, 5.2), ("rect", 55.0, 8.5, 62.0, 15.2),
+                                             ("polygon", [(75.0, 22.0), (86.0, 19.5), (82.0, 29.0)])],
+                                  capsules=[dict(centers=[(30.3, 28.2), (58.5, 4.6), (80.0, 16.0)],
+                                                 rmin=3.5, nodes=22, mem=SKALAK)],
+                                  rep=dict(damping_normal=0.01, friction_coeff=0.1)), 300),
+    # a fluid run two cells long between a wall and a block (cells whose links
+    # read slots another cell's links write), periodic seam pillar
+    "ibb_gap_seam": (dict(nx=48, ny=10, tau=0.8, bfx=2e-5, coll=0, ibb=True,
+                          obstacles=[("rect", 10.0, 2.3, 30.0, 6.0), ("circle", 40.5, 5.0, 1.6),
+                                     ("circle", 1.5, 4.5, 1.2)]), 400),
 }
 
 
@@ -55,15 +68,19 @@ def main():
 
     if not pyoracle.ref_available():
         sys.exit("oracle/_ref/libsfref.so is not built (make -C oracle ref)")
+    force = "--force" in sys.argv
     index = {}
     for name, (case, steps) in CASES.items():
+        index[name] = {"case": case, "steps": steps}
+        path = os.path.join(HERE, f"{name}.npz")
+        if os.path.exists(path) and not force:
+            continue            # vectors already written (--force rewrites them)
         sim = oracle_sim(case, which="ref")
         sim.step(steps)
         st = oracle_state(sim)
         arrays = {k: np.asarray(v) for k, v in st.items() if k != "capped"}
         arrays["capped"] = np.array(st["capped"])
-        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **arrays)
-        index[name] = {"case": case, "steps": steps}
+        np.savez_compressed(path, **arrays)
         print(name, "saved", flush=True)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(index, f, indent=1)

@@ -59,6 +59,7 @@ def test_argument_validation_before_device():
         assert msg in L.sf_last_error().decode()
         assert not h.value
     assert L.sf_create(None, 0, None) == capi.SF_E_ARG
+    assert L.sf_set_obstacles(None, 0, None, None, None) == capi.SF_E_ARG
 
 
 def test_batch_argument_validation_before_device():
@@ -143,7 +144,69 @@ def test_random_placement_and_rest_geometry_match_oracle(shape, nodes):
         assert cp.deformationIndex() == o.geometry(c, 4)
 
 
-@pytest.mark.parametrize("what", ["gravity", "lubrication", "ibb"])
+def _pillar_builder(sc, nx, ny):
+    from tests.cases import PILLARS
+    b = sc.ChannelBuilder(nx, ny)
+    b.addWalls()
+    for ob in PILLARS:
+        if ob[0] == "circle":
+            b.addCirclePillar(*ob[1:])
+        elif ob[0] == "rect":
+            b.addRectPillar(*ob[1:])
+        else:
+            b.addPolygonPillar([sc.Vec2d(x, y) for x, y in ob[1]])
+    return b
+
+
+def test_obstacle_geometry_matches_oracle():
+    # the engine's obstacles (the geometry the device path evaluates) against
+    # the oracle's restatement, on a grid crossing every face, vertex and interior
+    from softflow_amd import softflow_core as sc
+    from tests.cases import PILLARS, oracle_sim
+    obs = _pillar_builder(sc, 130, 40).getObstacles()
+    o = oracle_sim(dict(nx=130, ny=40, obstacles=PILLARS))
+    for i, ob in enumerate(obs):
+        for x in np.linspace(0.0, 130.0, 263):
+            for y in np.linspace(0.0, 40.0, 83):
+                d, n = o.obstacle_distance(i, x, y)
+                nn = ob.normalAt(x, y)
+                assert ob.signedDistance(x, y) == d and (nn.x, nn.y) == (n[0], n[1]), (i, x, y)
+                np_ = ob.nearestPoint(x, y)
+                assert np.isfinite(np_.x) and np.isfinite(np_.y)
+
+
+def test_capsule_placement_avoids_obstacles():
+    # Simulation::addCapsuleRandom with a channel builder: the obstacle test
+    # (signedDistance < r + min_gap, simulation.cpp:256-261) rejects the same
+    # draws as the oracle's
+    from softflow_amd import softflow_core as sc
+    from tests.cases import PILLARS
+    p = sc.SimulationParams()
+    p.nx, p.ny = 130, 40
+    p.fluid.boundary_type = sc.BoundaryType.PERIODIC
+    sim = sc.Simulation(p)
+    sim.setChannelBuilder(_pillar_builder(sc, 130, 40))
+    mp = sc.MembraneParams()
+    mp.model = sc.MembraneModel.SKALAK
+    placed = sim.addCapsuleRandom(25, 2, 2, 128, 38, 3.0, 4.5, 0, mp, 0, 77, 1.0)
+    from oracle import pyoracle as po
+    o = po.Sim(po.default_params(nx=130, ny=40, boundary_type=1))
+    for ob in PILLARS:
+        o.add_obstacle(ob)
+    oplaced = o.add_capsule_random(25, 2, 2, 128, 38, 3.0, 4.5, 0, po.default_membrane(model=2), 0, 77, 1.0, 0)
+    assert placed == oplaced > 5
+    nd = o.nodes()
+    xs = np.concatenate([np.array(c.node_positions_array())[:, 0] for c in sim.capsules()])
+    assert np.array_equal(xs, nd["x"])
+    # no capsule centre within its radius + gap of a pillar
+    caps = sim.capsules()
+    for c in range(len(caps)):
+        cx, cy = caps[c].centroid().x, caps[c].centroid().y
+        for i in range(len(PILLARS)):
+            assert o.obstacle_distance(i, cx, cy)[0] > 3.0
+
+
+@pytest.mark.parametrize("what", ["gravity", "lubrication"])
 def test_unsupported_features_refused(what):
     from softflow_amd import softflow_core as sc
     p = sc.SimulationParams()
@@ -151,10 +214,8 @@ def test_unsupported_features_refused(what):
     p.fluid.boundary_type = sc.BoundaryType.PERIODIC
     if what == "gravity":
         p.fluid.gravity_y = -1e-6
-    elif what == "lubrication":
-        p.lubrication.enabled = True
     else:
-        p.fluid.use_interpolated_bb = True
+        p.lubrication.enabled = True
     sim = sc.Simulation(p)
     with pytest.raises(ValueError, match="not implemented"):
         sim.initialize()

@@ -11,6 +11,14 @@ correctly rounded) by an ulp in some turning angles, which leaves the state
 (k (lambda - 1), k_b (theta - theta_0) with lambda - 1 and theta - theta_0 of
 order 1e-3 .. 1e-5), so it amplifies that state difference by 1e3 .. 1e5 and
 the force bar is 1e-8 for those cases.
+
+Contact forces take pow(sigma / r, power) (membrane/repulsion.cpp:15, :208).
+The device evaluates the correctly rounded power; glibc's pow, which the
+vectors were written with, is off by one ulp in about 1 of 1200 arguments
+(checked against quad precision).  Where capsules press on obstacles the
+contact forces are large (the push-out branch multiplies them by 10), so one
+such ulp shows in the force fields at ~2e-12 while the state stays within
+1e-12: the force bar is 1e-11 for the cases with obstacles and capsules.
 """
 import json
 import os
@@ -35,6 +43,14 @@ def uses_helfrich(case):
     return any(c.get("mem", {}).get("use_helfrich") for c in case.get("capsules", []))
 
 
+def force_bar(case):
+    if uses_helfrich(case):
+        return 1e-8
+    if case.get("obstacles") and case.get("capsules"):
+        return 1e-11
+    return 1e-12
+
+
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_product_matches_golden(name, graph):
@@ -52,6 +68,6 @@ def test_product_matches_golden(name, graph):
         b = np.concatenate([np.ravel(ref[p]) for p in parts])
         assert a.shape == b.shape, g
         err = rel_l2(a, b)
-        tol = 1e-8 if g in FORCES and uses_helfrich(spec["case"]) else 1e-12
+        tol = force_bar(spec["case"]) if g in FORCES else 1e-12
         assert err <= tol, f"{name}: {g} rel L2 {err:.3e}"
     assert int(got["capped"]) == int(ref["capped"])
