@@ -59,6 +59,7 @@ def build_product(case, precision="fp32", graph=True):
     from softflow_amd import softflow_core as sc
 
     p = sc.SimulationParams()
+    p.output_interval, p.probe_dump_every, p.stats_dump_every = 0, 0, 0   # no output files
     p.nx, p.ny = case["nx"], case["ny"]
     p.fluid.tau = case["tau"]
     p.fluid.boundary_type = sc.BoundaryType.PERIODIC
@@ -141,6 +142,7 @@ def build_mdf_channel(precision, ibm_iterations):
     from softflow_amd import softflow_core as sc
 
     p = sc.SimulationParams()
+    p.output_interval, p.probe_dump_every, p.stats_dump_every = 0, 0, 0   # no output files
     p.nx, p.ny = MDF_CHANNEL["nx"], MDF_CHANNEL["ny"]
     p.fluid.tau = MDF_CHANNEL["tau"]
     p.fluid.boundary_type = sc.BoundaryType.PERIODIC

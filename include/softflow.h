@@ -178,6 +178,36 @@ int  sf_field_stats(sf_ctx* ctx, double* out5);
 int  sf_global_stats(sf_ctx* ctx, double rho0, double* out11);
 int  sf_step_count(sf_ctx* ctx, int64_t* n);
 
+/* Output snapshots for the writers (the reference's writers read LatticeField,
+ * CapsuleSystem and AdvectionDiffusion on the host between steps,
+ * simulation.cpp:736-771; this is how they get the same data from the device).
+ * sf_snapshot_async enqueues on the context's stream, behind the steps already
+ * enqueued, the requested state into the caller's host buffers and returns a
+ * ticket at once; steps enqueued afterwards leave the copies intact.  Buffers
+ * from sf_host_alloc (pinned) keep the copies off the host's critical path.
+ * sf_snapshot_wait blocks until a ticket's copies have landed (the buffers must
+ * not be reused before).  Any pointer may be NULL.
+ *   rho..Fy   N each, as sf_get_macroscopic gives them;
+ *   C[s]      N each, the concentration of species s (NULL for the others);
+ *   x..fy     K each, as sf_get_nodes gives them;
+ *   probes    3 per probe cell (rho, ux, uy at the cells of sf_set_probes,
+ *             FluidProbeWriter::writeTimestep, fluid_probe_writer.cpp:46-68);
+ *   stats     12 values: the 11 of sf_global_stats, then the concentration
+ *             summed over every cell and species (global_stats_writer.cpp:86-97). */
+typedef struct {
+    double *rho, *ux, *uy, *Fx, *Fy;
+    double *C[4];
+    double *x, *y, *vx, *vy, *fx, *fy;
+    double *probes;
+    double *stats;
+    double  rho0;
+} sf_snapshot;
+int  sf_host_alloc(void** p, uint64_t bytes);
+int  sf_host_free(void* p);
+int  sf_set_probes(sf_ctx* ctx, int n, const int32_t* cells);
+int  sf_snapshot_async(sf_ctx* ctx, const sf_snapshot* s, int64_t* ticket);
+int  sf_snapshot_wait(sf_ctx* ctx, int64_t ticket);
+
 /* Per-kernel timing of the last sf_step call, measured with HIP events on the
  * context's stream: names[i] (static strings) and mean ms per launch. */
 int  sf_kernel_times(sf_ctx* ctx, int max, const char** names, double* ms,

@@ -81,6 +81,9 @@ def _lib(path: str, prefix: str):
     g("step").argtypes = [C.c_void_p, C.c_int]
     if prefix == "sfr_":
         g("set_threads").argtypes = [C.c_int]
+        g("output_open").argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        g("output_write").argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        g("output_close").argtypes = [C.c_void_p]
     g("scalar_initialize").argtypes = [C.c_void_p, C.c_double]
     g("set_scalar_point").argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]
     g("num_capsules").restype = C.c_int
@@ -235,6 +238,23 @@ class Sim:
         if self.pre != "sfr_":
             raise ValueError("the oracle restatement is single-threaded")
         self.L.sfr_set_threads(n)
+
+    def output_open(self, directory, legacy=False, all_csv=False, extra_fluid=False, probes=()):
+        """The reference's writers into `directory` (reference only)."""
+        if self.pre != "sfr_":
+            raise ValueError("the writers are the reference's own")
+        ij = np.asarray([v for p in probes for v in p] or [0], dtype=np.int32)
+        self._probe_ij = ij
+        opts = (1 if legacy else 0) | (2 if all_csv else 0) | (4 if extra_fluid else 0)
+        self.L.sfr_output_open(self.h, str(directory).encode(), opts, len(probes),
+                               ij.ctypes.data_as(C.POINTER(C.c_int)))
+
+    def output_write(self, step, time, vtk=False, csv=False, probes=False, stats=False):
+        mask = (1 if vtk else 0) | (2 if csv else 0) | (4 if probes else 0) | (8 if stats else 0)
+        self.L.sfr_output_write(self.h, step, time, mask)
+
+    def output_close(self):
+        self.L.sfr_output_close(self.h)
 
     def membrane_forces_only(self):
         self._f("membrane_forces_only")(self.h)

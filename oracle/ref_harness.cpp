@@ -16,6 +16,10 @@
 #include "coupling/immersed_boundary.h"
 #include "geometry/channel_builder.h"
 #include "geometry/polygon_obstacle.h"
+#include "io/csv_writer.h"
+#include "io/fluid_probe_writer.h"
+#include "io/global_stats_writer.h"
+#include "io/vtk_writer.h"
 #include "lbm/advection_diffusion.h"
 #include "lbm/lbm_solver.h"
 #include "membrane/capsule_system.h"
@@ -39,6 +43,11 @@ struct sfr_sim {
     std::vector<std::shared_ptr<Obstacle>> obs;     // in the order added
     std::vector<int> obs_kind;
     int capped = 0;
+    // the reference's writers (simulation.cpp:176-191)
+    std::unique_ptr<VTKWriter> vtk;
+    std::unique_ptr<CSVWriter> csv;
+    std::unique_ptr<FluidProbeWriter> probes;
+    std::unique_ptr<GlobalStatsWriter> stats;
 };
 
 static SimulationParams to_params(const sfo_params* p) {
@@ -156,6 +165,55 @@ void sfr_add_obstacle(sfr_sim* s, int kind, const double* c, int n) {
         s->obs.push_back(std::make_shared<PolygonObstacle>(v));
     }
     s->obs_kind.push_back(kind);
+}
+
+// The reference's writers on this simulation's state, set up as the
+// Simulation constructor does (simulation.cpp:173-191).  options: bit 0 legacy
+// VTK, bit 1 every CSV column, bit 2 the optional fluid fields (vorticity,
+// strain rate, IBM force).  probes: n (i, j) pairs labelled p0, p1, ...
+void sfr_output_open(sfr_sim* s, const char* dir, int options, int nprobes, const int* ij) {
+    const std::string d(dir);
+    s->vtk = std::make_unique<VTKWriter>(d);
+    if (s->params.fluid.boundary_type == BoundaryType::PERIODIC) s->vtk->setPeriodicX(s->params.nx);
+    if (options & 1) s->vtk->setLegacyMode(true);
+    if (s->adr) s->vtk->setAdvectionDiffusion(s->adr.get());
+    if (options & 4) {
+        FluidOutputFields ff;
+        ff.vorticity = ff.strain_rate = ff.ibm_force = true;
+        s->vtk->setFluidFields(ff);
+    }
+    s->csv = std::make_unique<CSVWriter>(d);
+    if (options & 2) {
+        ParticleOutputFields pf;
+        pf.force = pf.diameter = pf.group = pf.deformation = pf.area_volume = true;
+        pf.orientation = pf.angular_vel = true;
+        s->csv->setFields(pf);
+    }
+    s->probes = std::make_unique<FluidProbeWriter>(d);
+    for (int i = 0; i < nprobes; ++i) s->probes->addProbe(ij[2 * i], ij[2 * i + 1], "p" + std::to_string(i));
+    s->stats = std::make_unique<GlobalStatsWriter>(d);
+}
+
+// Simulation::step's output block (simulation.cpp:736-767) for the writers in mask
+// (bit 0 VTK, 1 CSV, 2 probes, 3 statistics)
+void sfr_output_write(sfr_sim* s, int step, double time, int mask) {
+    LatticeField& f = s->lbm->field();
+    if (mask & 1) {
+        s->vtk->writeFluidField(f, step);
+        s->vtk->writeCapsules(s->caps, step);
+        s->vtk->recordTimestep(step, time);
+    }
+    if (mask & 2) s->csv->writeTimestep(s->caps, step, time);
+    if (mask & 4) s->probes->writeTimestep(f, step, time);
+    if (mask & 8) s->stats->writeTimestep(f, s->caps, step, time, s->adr.get());
+}
+
+// Simulation::finalize (simulation.cpp:801-812)
+void sfr_output_close(sfr_sim* s) {
+    if (s->vtk) s->vtk->writePVDFiles();
+    if (s->csv) s->csv->close();
+    if (s->probes) s->probes->close();
+    if (s->stats) s->stats->close();
 }
 
 double sfr_obstacle_distance(const sfr_sim* s, int i, double x, double y, double* normal) {

@@ -62,7 +62,8 @@ def build_core() -> str:
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
            "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
            "-I", os.path.join(ROOT, "include"),
-           os.path.join(CSRC, "engine.cpp"), os.path.join(CSRC, "bindings.cpp"),
+           os.path.join(CSRC, "engine.cpp"), os.path.join(CSRC, "output.cpp"),
+           os.path.join(CSRC, "bindings.cpp"), "-pthread",
            "-L", LIBDIR, "-lsoftflow_hip", "-Wl,-rpath,$ORIGIN/lib",
            "-o", out]
     _run(cmd)

@@ -28,6 +28,7 @@ EXPORTS = (
     "sf_get_force_state", "sf_set_force_state",
     "sf_set_scalar", "sf_get_scalar", "sf_step", "sf_set_graph_mode", "sf_synchronize",
     "sf_stage", "sf_last_capped", "sf_field_stats", "sf_global_stats", "sf_step_count", "sf_kernel_times",
+    "sf_host_alloc", "sf_host_free", "sf_set_probes", "sf_snapshot_async", "sf_snapshot_wait",
     "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_probe_math",
     "sf_mdf_set_control", "sf_mdf_stats", "sf_mdf_collect", "sf_mdf_samples", "sf_mdf_set_model",
     "sf_mlp_create", "sf_mlp_destroy", "sf_mlp_num_params", "sf_mlp_get_params", "sf_mlp_set_params",
@@ -110,6 +111,11 @@ def load():
         "sf_field_stats": [C.c_void_p, _DP],
         "sf_global_stats": [C.c_void_p, C.c_double, _DP],
         "sf_step_count": [C.c_void_p, C.POINTER(C.c_int64)],
+        "sf_host_alloc": [C.POINTER(C.c_void_p), C.c_uint64],
+        "sf_host_free": [C.c_void_p],
+        "sf_set_probes": [C.c_void_p, C.c_int, _IP],
+        "sf_snapshot_async": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)],
+        "sf_snapshot_wait": [C.c_void_p, C.c_int64],
         "sf_kernel_times": [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), _DP, C.POINTER(C.c_int)],
         "sf_set_timing": [C.c_void_p, C.c_int],
         "sf_autotune": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],

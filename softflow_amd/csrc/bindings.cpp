@@ -146,6 +146,90 @@ PYBIND11_MODULE(softflow_core, m) {
         .def_readwrite("inlet_concentration", &ScalarParams::inlet_concentration)
         .def_readwrite("periodic_y", &ScalarParams::periodic_y);
 
+    // output configuration (io/output_config.h; python/bindings.cpp:720-801)
+    py::class_<FluidOutputFields>(m, "FluidOutputFields")
+        .def(py::init<>())
+        .def_readwrite("density", &FluidOutputFields::density)
+        .def_readwrite("velocity", &FluidOutputFields::velocity)
+        .def_readwrite("pressure", &FluidOutputFields::pressure)
+        .def_readwrite("vorticity", &FluidOutputFields::vorticity)
+        .def_readwrite("strain_rate", &FluidOutputFields::strain_rate)
+        .def_readwrite("ibm_force", &FluidOutputFields::ibm_force)
+        .def_readwrite("node_type", &FluidOutputFields::node_type)
+        .def_readwrite("velocity_mag", &FluidOutputFields::velocity_mag)
+        .def_readwrite("component_density", &FluidOutputFields::component_density)
+        .def_readwrite("concentration", &FluidOutputFields::concentration);
+    py::class_<ParticleVTKFields>(m, "ParticleVTKFields")
+        .def(py::init<>())
+        .def_readwrite("velocity", &ParticleVTKFields::velocity)
+        .def_readwrite("force", &ParticleVTKFields::force)
+        .def_readwrite("particle_id", &ParticleVTKFields::particle_id)
+        .def_readwrite("particle_type", &ParticleVTKFields::particle_type)
+        .def_readwrite("particle_group", &ParticleVTKFields::particle_group)
+        .def_readwrite("local_strain", &ParticleVTKFields::local_strain)
+        .def_readwrite("local_curvature", &ParticleVTKFields::local_curvature);
+    py::class_<ParticleOutputFields>(m, "ParticleOutputFields")
+        .def(py::init<>())
+        .def_readwrite("position", &ParticleOutputFields::position)
+        .def_readwrite("velocity", &ParticleOutputFields::velocity)
+        .def_readwrite("force", &ParticleOutputFields::force)
+        .def_readwrite("diameter", &ParticleOutputFields::diameter)
+        .def_readwrite("type", &ParticleOutputFields::type)
+        .def_readwrite("group", &ParticleOutputFields::group)
+        .def_readwrite("deformation", &ParticleOutputFields::deformation)
+        .def_readwrite("area_volume", &ParticleOutputFields::area_volume)
+        .def_readwrite("orientation", &ParticleOutputFields::orientation)
+        .def_readwrite("angular_vel", &ParticleOutputFields::angular_vel)
+        .def_readwrite("com_position", &ParticleOutputFields::com_position)
+        .def_readwrite("com_velocity", &ParticleOutputFields::com_velocity);
+    py::enum_<ParticleFilter::FilterType>(m, "FilterType")
+        .value("ALL", ParticleFilter::ALL)
+        .value("BY_ID", ParticleFilter::BY_ID)
+        .value("BY_TYPE", ParticleFilter::BY_TYPE)
+        .value("BY_GROUP", ParticleFilter::BY_GROUP)
+        .value("BY_REGION", ParticleFilter::BY_REGION)
+        .value("COMBINED", ParticleFilter::COMBINED);
+    py::class_<ParticleFilter>(m, "ParticleFilter")
+        .def(py::init<>())
+        .def_readwrite("filter_type", &ParticleFilter::filter_type)
+        .def_readwrite("selected_ids", &ParticleFilter::selected_ids)
+        .def_readwrite("selected_types", &ParticleFilter::selected_types)
+        .def_readwrite("selected_groups", &ParticleFilter::selected_groups)
+        .def_readwrite("region_xmin", &ParticleFilter::region_xmin)
+        .def_readwrite("region_xmax", &ParticleFilter::region_xmax)
+        .def_readwrite("region_ymin", &ParticleFilter::region_ymin)
+        .def_readwrite("region_ymax", &ParticleFilter::region_ymax)
+        .def("passes", &ParticleFilter::passes, py::arg("id"), py::arg("type_id"), py::arg("group"),
+             py::arg("cx"), py::arg("cy"));
+    py::class_<FluidProbe>(m, "FluidProbe")
+        .def(py::init<>())
+        .def_readwrite("i", &FluidProbe::i)
+        .def_readwrite("j", &FluidProbe::j)
+        .def_readwrite("label", &FluidProbe::label);
+    py::class_<ExtraCSVConfig>(m, "ExtraCSVConfig")
+        .def(py::init<>())
+        .def_readwrite("filename", &ExtraCSVConfig::filename)
+        .def_readwrite("dump_every", &ExtraCSVConfig::dump_every)
+        .def_readwrite("fields", &ExtraCSVConfig::fields)
+        .def_readwrite("filter", &ExtraCSVConfig::filter)
+        .def_readwrite("append", &ExtraCSVConfig::append);
+    py::class_<OutputConfig>(m, "OutputConfig")
+        .def(py::init<>())
+        .def_readwrite("output_dir", &OutputConfig::output_dir)
+        .def_readwrite("vtk_dump_every", &OutputConfig::vtk_dump_every)
+        .def_readwrite("vtk_format", &OutputConfig::vtk_format)
+        .def_readwrite("csv_dump_every", &OutputConfig::csv_dump_every)
+        .def_readwrite("csv_format", &OutputConfig::csv_format)
+        .def_readwrite("csv_append", &OutputConfig::csv_append)
+        .def_readwrite("probe_dump_every", &OutputConfig::probe_dump_every)
+        .def_readwrite("stats_dump_every", &OutputConfig::stats_dump_every)
+        .def_readwrite("fluid_fields", &OutputConfig::fluid_fields)
+        .def_readwrite("particle_vtk_fields", &OutputConfig::particle_vtk_fields)
+        .def_readwrite("csv_fields", &OutputConfig::csv_fields)
+        .def_readwrite("csv_filter", &OutputConfig::csv_filter)
+        .def_readwrite("extra_csv", &OutputConfig::extra_csv)
+        .def_readwrite("probes", &OutputConfig::probes);
+
     py::class_<SimulationParams>(m, "SimulationParams")
         .def(py::init<>())
         .def_readwrite("nx", &SimulationParams::nx)
@@ -161,6 +245,7 @@ PYBIND11_MODULE(softflow_core, m) {
         .def_readwrite("adhesion", &SimulationParams::adhesion)
         .def_readwrite("scalar", &SimulationParams::scalar)
         .def_readwrite("output_dir", &SimulationParams::output_dir)
+        .def_readwrite("vtk_format", &SimulationParams::vtk_format)
         .def_readwrite("checkpoint_interval", &SimulationParams::checkpoint_interval)
         .def_readwrite("metrics_interval", &SimulationParams::metrics_interval)
         .def_readwrite("vtk_dump_every", &SimulationParams::vtk_dump_every)
@@ -348,7 +433,15 @@ PYBIND11_MODULE(softflow_core, m) {
              py::arg("seed") = 12345, py::arg("min_gap") = 1.0, py::arg("max_attempts") = 0)
         .def("initialize", &Simulation::initialize)
         .def("step", &Simulation::step)
-        .def("finalize", &Simulation::finalize)
+        .def("finalize", &Simulation::finalize, py::call_guard<py::gil_scoped_release>())
+        .def("setOutputConfig", &Simulation::setOutputConfig, py::arg("config"))
+        .def("setVTKOutput", &Simulation::setVTKOutput, py::arg("dump_every"), py::arg("format"),
+             py::arg("fluid_fields"), py::arg("particle_fields"))
+        .def("setCSVOutput", &Simulation::setCSVOutput, py::arg("dump_every"), py::arg("format"),
+             py::arg("fields"), py::arg("filter"), py::arg("append") = true)
+        .def("addExtraCSVOutput", &Simulation::addExtraCSVOutput, py::arg("filename"), py::arg("dump_every"),
+             py::arg("fields"), py::arg("filter"))
+        .def("addFluidProbe", &Simulation::addFluidProbe, py::arg("i"), py::arg("j"), py::arg("label"))
         .def("run", &Simulation::run, py::arg("num_steps"), py::call_guard<py::gil_scoped_release>())
         .def("currentStep", &Simulation::currentStep)
         .def("getMaxSpeed", &Simulation::getMaxSpeed)

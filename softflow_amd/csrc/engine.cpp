@@ -359,11 +359,43 @@ Simulation::Simulation(const SimulationParams& p) : params_(p) {
         adr_ = std::make_unique<AdvectionDiffusion>(p.nx, p.ny, p.scalar);
         adr_->sim_ = this;
     }
+    out_ = std::make_unique<OutputManager>(p, adr_ ? adr_->getNumSpecies() : 0);
 }
 
 Simulation::~Simulation() {
+    out_.reset();            // the writer thread waits on the context's copies
     if (ctx_) sf_destroy(ctx_);
 }
+
+void Simulation::output(int64_t step) {
+    if (out_ && out_->due(step)) out_->capture(step, caps_.caps_, field_->flags_);
+}
+
+void Simulation::finalize() {
+    if (out_ && ctx_) out_->finalize();
+}
+
+void Simulation::setOutputConfig(const OutputConfig& config) { out_->setOutputConfig(config); }
+
+void Simulation::setVTKOutput(int dump_every, const std::string& format, const FluidOutputFields& ff,
+                              const ParticleVTKFields& pf) {
+    params_.vtk_dump_every = dump_every;
+    out_->setVTKOutput(dump_every, format, ff, pf);
+}
+
+void Simulation::setCSVOutput(int dump_every, const std::string& format, const ParticleOutputFields& fields,
+                              const ParticleFilter& filter, bool append) {
+    params_.csv_dump_every = dump_every;
+    out_->setCSVOutput(dump_every, format, fields, filter, append);
+}
+
+void Simulation::addExtraCSVOutput(const std::string& filename, int dump_every, const ParticleOutputFields& fields,
+                                   const ParticleFilter& filter) {
+    (void)dump_every;   // the extra writers follow the CSV interval (simulation.cpp:1015-1023)
+    out_->addExtraCSVOutput(filename, fields, filter);
+}
+
+void Simulation::addFluidProbe(int i, int j, const std::string& label) { out_->addFluidProbe(i, j, label); }
 
 void Simulation::check(int rc, const char* what) const {
     if (rc != SF_OK) throw std::runtime_error(std::string(what) + ": " + sf_last_error());
@@ -490,8 +522,13 @@ void Simulation::initialize() {
     // links are only built from a channel builder's geometry (simulation.cpp:440-442)
     sp.use_interpolated_bb = (P.fluid.use_interpolated_bb && builder_) ? 1 : 0;
 
-    if (ctx_) { sf_destroy(ctx_); ctx_ = nullptr; }
+    if (ctx_) {
+        out_->attach(nullptr);
+        sf_destroy(ctx_);
+        ctx_ = nullptr;
+    }
     check(sf_create(&ctx_, P.device, &sp), "sf_create");
+    out_->attach(ctx_);
     check(sf_set_graph_mode(ctx_, P.use_graph ? 1 : 0), "sf_set_graph_mode");
 
     // cell types: the channel builder's (simulation.cpp:431-434), then
@@ -640,6 +677,7 @@ void Simulation::step() {
     if (params_.enable_stability_checks && params_.stability_check_interval > 0 &&
         step_count_ % params_.stability_check_interval == 0)
         checkStability();
+    output(step_count_);
     intervalCheckpoint(step_count_);
     if (cb_) cb_(*this, static_cast<int>(step_count_));
     step_count_++;
@@ -655,6 +693,7 @@ void Simulation::markAdvanced(int steps) {
 void Simulation::run(int num_steps) {
     if (cb_) {
         for (int s = 0; s < num_steps; ++s) step();
+        finalize();
         return;
     }
     int done = 0;
@@ -674,6 +713,9 @@ void Simulation::run(int num_steps) {
             const int64_t r = step_count_ % civ;
             chunk = std::min(chunk, static_cast<int>(r == 0 ? 1 : civ - r + 1));
         }
+        // and on the next step index at which a writer is due
+        const int64_t due = out_->nextDue(step_count_);
+        if (due >= 0) chunk = std::min(chunk, static_cast<int>(due - step_count_ + 1));
         pushDirty();
         check(sf_step(ctx_, chunk), "sf_step");
         field_->stale_ = true;
@@ -681,9 +723,11 @@ void Simulation::run(int num_steps) {
         if (adr_) adr_->stale_ = true;
         step_count_ += chunk;
         if (checks && (step_count_ - 1) % iv == 0) checkStability();
+        output(step_count_ - 1);
         intervalCheckpoint(step_count_ - 1);
         done += chunk;
     }
+    finalize();
 }
 
 Real Simulation::getMaxSpeed() {

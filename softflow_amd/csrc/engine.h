@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/softflow.h"
+#include "output.h"
 #include "sf_geometry.h"
 
 namespace softflow {
@@ -102,6 +103,7 @@ struct SimulationParams {
     AdhesionParams adhesion;
     ScalarParams scalar;
     std::string output_dir = "output";
+    std::string vtk_format = "ascii";      // "ascii" (XML .vti/.vtp) or "legacy" (.vtk)
     int checkpoint_interval = 0, metrics_interval = 0;
     int vtk_dump_every = 0, csv_dump_every = 0, probe_dump_every = 10, stats_dump_every = 100;
     bool enable_stability_checks = true;
@@ -197,6 +199,7 @@ public:
 private:
     friend class Simulation;
     friend class CapsuleSystem;
+    friend class OutputManager;
     Vec2d minImageDiff(const Vec2d& a, const Vec2d& b) const;
     void secondMoments(Real& mxx, Real& myy, Real& mxy) const;
     sf_capsule_desc desc() const;
@@ -392,7 +395,19 @@ public:
     void initialize();
     void step();
     void run(int num_steps);
-    void finalize() {}
+    // Simulation::finalize (simulation.cpp:801-812): outstanding output
+    // written, .pvd collections, writers closed; run() ends with it
+    void finalize();
+
+    // output configuration (simulation.h:180-200)
+    void setOutputConfig(const OutputConfig& config);
+    void setVTKOutput(int dump_every, const std::string& format, const FluidOutputFields& fluid_fields,
+                      const ParticleVTKFields& particle_fields);
+    void setCSVOutput(int dump_every, const std::string& format, const ParticleOutputFields& fields,
+                      const ParticleFilter& filter, bool append = true);
+    void addExtraCSVOutput(const std::string& filename, int dump_every, const ParticleOutputFields& fields,
+                           const ParticleFilter& filter);
+    void addFluidProbe(int i, int j, const std::string& label);
 
     int currentStep() const { return static_cast<int>(step_count_); }
     Real getMaxSpeed();
@@ -461,6 +476,8 @@ private:
     bool mdf_warm_ = false;
     // raw force of the last step read from a checkpoint, set after the other state
     std::vector<double> pending_force_;
+    std::unique_ptr<OutputManager> out_;
+    void output(int64_t step);       // the writers due after step index `step`
 };
 
 // A batched ensemble (no counterpart in the reference): independent

@@ -538,7 +538,9 @@ int sf_destroy(sf_ctx* h) {
     void* ptrs[] = {c.flags, c.f[0], c.f[1], c.F[0], c.F[1], c.F[2], c.stored_rho, c.stored_u,
                     c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_act, c.tile_rec, c.tile_dirty, c.cell_cnt, c.cell_list,
                     c.cell_pay, c.cell_own, c.mdf, c.ds_buf, c.obs, c.obs_v, c.ibb_cell, c.ibb_off,
-                    c.ibb_q, c.ibb_up, c.ibb_qf};
+                    c.ibb_q, c.ibb_up, c.ibb_qf, c.snap_dev, c.probe_cells};
+    for (auto& t : c.snap_tickets) hipEventDestroy(t.second);
+    c.snap_tickets.clear();
     for (void* p : ptrs) if (p) hipFree(p);
     for (int s = 0; s < 4; ++s) {
         for (int b = 0; b < 2; ++b) if (c.g[s][b]) hipFree(c.g[s][b]);
@@ -1027,6 +1029,118 @@ int sf_global_stats(sf_ctx* h, double rho0, double* out11) {
     hipFree(buf);
     SF_HIP(e);
     return SF_OK;
+}
+
+int sf_host_alloc(void** p, uint64_t bytes) {
+    if (!p) return fail(SF_E_ARG, "null pointer");
+    *p = nullptr;
+    if (bytes == 0) return SF_OK;
+    SF_HIP(hipHostMalloc(p, bytes, hipHostMallocDefault));
+    return SF_OK;
+}
+
+int sf_host_free(void* p) {
+    if (p) SF_HIP(hipHostFree(p));
+    return SF_OK;
+}
+
+int sf_set_probes(sf_ctx* h, int n, const int32_t* cells) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    if (n < 0 || (n > 0 && !cells)) return fail(SF_E_ARG, "probe cells");
+    Ctx& c = h->c;
+    for (int i = 0; i < n; ++i)
+        if (cells[i] < 0 || cells[i] >= c.N) return fail(SF_E_ARG, "probe cell outside the lattice");
+    SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));
+    if (c.probe_cells) { hipFree(c.probe_cells); c.probe_cells = nullptr; }
+    c.nprobes = 0;
+    if (n > 0) {
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.probe_cells), n * sizeof(int)));
+        SF_HIP(hipMemcpy(c.probe_cells, cells, n * sizeof(int), hipMemcpyHostToDevice));
+        c.nprobes = n;
+    }
+    return SF_OK;
+}
+
+int sf_snapshot_async(sf_ctx* h, const sf_snapshot* s, int64_t* ticket) {
+    if (!h || !s || !ticket) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    for (int sp = c.nsp; sp < 4; ++sp)
+        if (s->C[sp]) return fail(SF_E_ARG, "no such scalar species");
+    if (s->probes && c.nprobes == 0) return fail(SF_E_ARG, "no probe cells set");
+    SF_HIP(hipSetDevice(c.device));
+    const size_t N = c.N;
+    const bool fields = s->rho || s->ux || s->uy || s->Fx || s->Fy;
+    bool anyC = s->stats && c.nsp > 0;
+    for (int sp = 0; sp < c.nsp; ++sp) anyC |= s->C[sp] != nullptr;
+    // staging: macro 5N | C nsp*N | probes | stats (16 + 3N + 11*256) | sum scratch 256 + 4
+    const size_t o_C = 5 * N, o_pr = o_C + c.nsp * N, o_st = o_pr + 3 * static_cast<size_t>(c.nprobes);
+    const size_t o_sum = o_st + 16 + 3 * N + 11 * 256, need = o_sum + 260;
+    if (need > c.snap_cap) {
+        SF_HIP(hipStreamSynchronize(c.stream));
+        if (c.snap_dev) hipFree(c.snap_dev);
+        c.snap_dev = nullptr;
+        c.snap_cap = 0;
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.snap_dev), need * 8));
+        c.snap_cap = need;
+    }
+    double* st = c.snap_dev;
+    if (fields) launch_macro(c, st, st + N, st + 2 * N, st + 3 * N, st + 4 * N);
+    if (anyC) {
+        flush_adr(c);
+        for (int sp = 0; sp < c.nsp; ++sp) {
+            double* Cd = st + o_C + sp * N;
+            if (c.c_from_store)
+                SF_HIP(hipMemcpyAsync(Cd, c.Cstore[sp], N * 8, hipMemcpyDeviceToDevice, c.stream));
+            else if (c.fp32)
+                hipLaunchKernelGGL(k_csum<float>, dim3((c.N + 255) / 256), dim3(256), 0, c.stream,
+                                   static_cast<const float*>(c.g[sp][c.gcur]), c.N, Cd);
+            else
+                hipLaunchKernelGGL(k_csum<double>, dim3((c.N + 255) / 256), dim3(256), 0, c.stream,
+                                   static_cast<const double*>(c.g[sp][c.gcur]), c.N, Cd);
+        }
+    }
+    if (s->probes) launch_macro_cells(c, c.probe_cells, c.nprobes, st + o_pr);
+    if (s->stats) {
+        launch_global_stats(c, s->rho0, st + o_st);
+        double* mass = st + o_st + 11;
+        if (c.nsp > 0) launch_sum(c, st + o_C, static_cast<int64_t>(c.nsp) * c.N, st + o_sum, mass);
+        else SF_HIP(hipMemsetAsync(mass, 0, 8, c.stream));
+    }
+    auto d2h = [&](double* dst, const double* src, size_t count) -> hipError_t {
+        if (!dst || count == 0) return hipSuccess;
+        return hipMemcpyAsync(dst, src, count * 8, hipMemcpyDeviceToHost, c.stream);
+    };
+    double* outs[5] = {s->rho, s->ux, s->uy, s->Fx, s->Fy};
+    for (int k = 0; k < 5; ++k) SF_HIP(d2h(outs[k], st + k * N, N));
+    for (int sp = 0; sp < c.nsp; ++sp) SF_HIP(d2h(s->C[sp], st + o_C + sp * N, N));
+    const size_t K = c.K;
+    SF_HIP(d2h(s->x, c.x, K)); SF_HIP(d2h(s->y, c.y, K));
+    SF_HIP(d2h(s->vx, c.vx, K)); SF_HIP(d2h(s->vy, c.vy, K));
+    SF_HIP(d2h(s->fx, c.fx, K)); SF_HIP(d2h(s->fy, c.fy, K));
+    SF_HIP(d2h(s->probes, st + o_pr, 3 * static_cast<size_t>(c.nprobes)));
+    SF_HIP(d2h(s->stats, st + o_st, 12));
+    hipEvent_t ev;
+    SF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    SF_HIP(hipEventRecord(ev, c.stream));
+    *ticket = c.snap_next++;
+    c.snap_tickets.emplace_back(*ticket, ev);
+    return SF_OK;
+}
+
+int sf_snapshot_wait(sf_ctx* h, int64_t ticket) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    for (size_t i = 0; i < c.snap_tickets.size(); ++i) {
+        if (c.snap_tickets[i].first != ticket) continue;
+        hipEvent_t ev = c.snap_tickets[i].second;
+        c.snap_tickets.erase(c.snap_tickets.begin() + static_cast<long>(i));
+        hipError_t e = hipEventSynchronize(ev);
+        hipEventDestroy(ev);
+        SF_HIP(e);
+        return SF_OK;
+    }
+    return fail(SF_E_ARG, "unknown snapshot ticket");
 }
 
 int sf_step_count(sf_ctx* h, int64_t* n) {

@@ -166,6 +166,14 @@ struct Ctx {
     int* ibb_up = nullptr;               // upstream cell for q < 1/2, -1: halfway fallback
     double* ibb_qf = nullptr;            // wall distance along the link, in [0, 1]
 
+    // output snapshots: staging on the device, probe cells, tickets
+    double* snap_dev = nullptr;
+    size_t snap_cap = 0;                 // doubles
+    int nprobes = 0;
+    int* probe_cells = nullptr;
+    std::vector<std::pair<int64_t, hipEvent_t>> snap_tickets;
+    int64_t snap_next = 1;
+
     // repulsion cells
     int ncx = 0, ncy = 0;
     double cdx = 1.0, cdy = 1.0, xorg = 0.0, Lx = 0.0;
@@ -237,6 +245,8 @@ void launch_lbm(Ctx& c, int shift_mode, bool fuse_adr);
 void launch_lbm_cfg(Ctx& c, int shift_mode, bool fuse_adr, int bx, int by, int remap, int persist);
 void launch_adr(Ctx& c, int Fidx);
 void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy);
+void launch_macro_cells(Ctx& c, const int* cells, int count, double* out);
+void launch_sum(Ctx& c, const double* v, int64_t n, double* ws, double* out);
 void launch_field_stats(Ctx& c, double* dev_out);
 void launch_global_stats(Ctx& c, double rho0, double* dev_out);
 // interpolated bounce-back on the populations the last collide-stream wrote

@@ -858,38 +858,105 @@ void launch_adr(Ctx& c, int Fidx) {
 // the last step.  mode: 0 = force of the last collision (after a step),
 // 1 = zero force (fresh initialisation), 2 = stored macro arrays.
 template <bool FP32, bool FLAGS>
+__device__ __forceinline__ void macro_force(const LbmArgs& a, int mode, int n, double& Fx, double& Fy) {
+    const int x = n % a.nx, y = n / a.nx;
+    const bool solid = is_solid(a, x, y, FLAGS);
+    if (mode == 1) {
+        Fx = 0.0; Fy = 0.0;
+    } else if (FP32) {
+        float fx, fy;
+        collision_force<float>(a, a.Fcur, n, !solid && is_fluid_type(a, x, y, FLAGS), fx, fy);
+        Fx = fx; Fy = fy;
+    } else {
+        collision_force<double>(a, a.Fcur, n, !solid && is_fluid_type(a, x, y, FLAGS), Fx, Fy);
+    }
+}
+
+template <bool FP32, bool FLAGS>
+__device__ __forceinline__ void macro_state(const LbmArgs& a, const void* f, int mode, int n, double& r,
+                                            double& u, double& v) {
+    const int x = n % a.nx, y = n / a.nx;
+    if (mode == 2) {
+        r = a.srho[n]; u = a.su[2 * n]; v = a.su[2 * n + 1];
+        return;
+    }
+    if (is_solid(a, x, y, FLAGS)) {
+        r = 0.0; u = 0.0; v = 0.0;
+        return;
+    }
+    cell_macro<FP32>(f, static_cast<size_t>(a.N), n, a.Fcur, a, mode == 1, is_fluid_type(a, x, y, FLAGS), r,
+                     u, v);
+}
+
+template <bool FP32, bool FLAGS>
 __global__ void k_macro(LbmArgs a, const void* f, int mode, double* rho, double* ux, double* uy,
                         double* Fx, double* Fy) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.N) return;
-    const int x = n % a.nx, y = n / a.nx;
-    const bool solid = is_solid(a, x, y, FLAGS);
     if (Fx) {
-        if (mode == 1) {
-            Fx[n] = 0.0; Fy[n] = 0.0;
-        } else if (FP32) {
-            float fx, fy;
-            collision_force<float>(a, a.Fcur, n, !solid && is_fluid_type(a, x, y, FLAGS), fx, fy);
-            Fx[n] = fx; Fy[n] = fy;
-        } else {
-            double fx, fy;
-            collision_force<double>(a, a.Fcur, n, !solid && is_fluid_type(a, x, y, FLAGS), fx, fy);
-            Fx[n] = fx; Fy[n] = fy;
-        }
+        double fx, fy;
+        macro_force<FP32, FLAGS>(a, mode, n, fx, fy);
+        Fx[n] = fx; Fy[n] = fy;
     }
     if (!rho) return;
-    if (mode == 2) {
-        rho[n] = a.srho[n]; ux[n] = a.su[2 * n]; uy[n] = a.su[2 * n + 1];
-        return;
-    }
-    if (solid) {
-        rho[n] = 0.0; ux[n] = 0.0; uy[n] = 0.0;
-        return;
-    }
     double r, u, v;
-    cell_macro<FP32>(f, static_cast<size_t>(a.N), n, a.Fcur, a, mode == 1, is_fluid_type(a, x, y, FLAGS), r,
-                     u, v);
+    macro_state<FP32, FLAGS>(a, f, mode, n, r, u, v);
     rho[n] = r; ux[n] = u; uy[n] = v;
+}
+
+// rho, ux, uy at listed cells, the values k_macro gives there
+template <bool FP32, bool FLAGS>
+__global__ void k_macro_cells(LbmArgs a, const void* f, int mode, const int* cells, int count, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    double r, u, v;
+    macro_state<FP32, FLAGS>(a, f, mode, cells[i], r, u, v);
+    out[3 * i] = r; out[3 * i + 1] = u; out[3 * i + 2] = v;
+}
+
+void launch_macro_cells(Ctx& c, const int* cells, int count, double* out) {
+    if (count <= 0) return;
+    LbmArgs a = view_args(c);
+    a.srho = c.stored_rho;
+    a.su = c.stored_u;
+    const int mode = c.shift_next == SHIFT_STORED ? 2 : (c.step_count == 0 ? 1 : 0);
+    dim3 b(64), g((count + 63) / 64);
+    const void* f = c.f[c.fcur];
+    if (c.fp32) {
+        if (c.generic_flags) hipLaunchKernelGGL((k_macro_cells<true, true>), g, b, 0, c.stream, a, f, mode, cells, count, out);
+        else hipLaunchKernelGGL((k_macro_cells<true, false>), g, b, 0, c.stream, a, f, mode, cells, count, out);
+    } else {
+        if (c.generic_flags) hipLaunchKernelGGL((k_macro_cells<false, true>), g, b, 0, c.stream, a, f, mode, cells, count, out);
+        else hipLaunchKernelGGL((k_macro_cells<false, false>), g, b, 0, c.stream, a, f, mode, cells, count, out);
+    }
+}
+
+// Sum of n values: 256 fixed partials, then one thread adds them in order
+// (deterministic).  ws holds 256 doubles.
+__global__ void k_sum_partial(const double* v, int64_t n, double* ws) {
+    __shared__ double s[256];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + t; i < n; i += 256 * 256) acc += v[i];
+    s[t] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t == 0) ws[blockIdx.x] = s[0];
+}
+
+__global__ void k_sum_final(const double* ws, double* out) {
+    if (threadIdx.x != 0) return;
+    double acc = 0.0;
+    for (int b = 0; b < 256; ++b) acc += ws[b];
+    *out = acc;
+}
+
+void launch_sum(Ctx& c, const double* v, int64_t n, double* ws, double* out) {
+    hipLaunchKernelGGL(k_sum_partial, dim3(256), dim3(256), 0, c.stream, v, n, ws);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, c.stream, ws, out);
 }
 
 void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy) {

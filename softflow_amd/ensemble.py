@@ -39,6 +39,7 @@ def build_member(spec, precision="fp32", graph=True):
     from softflow_amd import softflow_core as sc
 
     p = sc.SimulationParams()
+    p.output_interval, p.probe_dump_every, p.stats_dump_every = 0, 0, 0   # no output files
     p.nx, p.ny = spec["nx"], spec["ny"]
     p.fluid.tau = spec["tau"]
     p.fluid.boundary_type = sc.BoundaryType.PERIODIC

@@ -101,6 +101,7 @@ def product_sim(case, graph=True):
     s = case.get("scalar")
     rep = case.get("rep", {})
     p = sc.SimulationParams()
+    p.output_interval, p.probe_dump_every, p.stats_dump_every = 0, 0, 0   # no output files
     p.nx, p.ny = case["nx"], case["ny"]
     p.fluid.tau = case.get("tau", 0.8)
     p.fluid.boundary_type = {"closed": sc.BoundaryType.CLOSED,

@@ -63,6 +63,55 @@ CASES = {
 }
 
 
+# The reference's writers (io/*_writer.cpp) on reference-component runs: each
+# case's files, gzipped, under writers/<case>/, written at every output step of
+# Simulation::step's schedule (simulation.cpp:736-767) and closed as
+# Simulation::finalize does.
+WRITER_CASES = {
+    "periodic_caps_scalar": dict(
+        case=dict(nx=32, ny=14, tau=0.8, bfx=3e-5, coll=0, scalar=dict(D=0.05, blob=(14, 7, 3.0)),
+                  capsules=[dict(centers=[(2.5, 7.0), (19.0, 6.5)], rmin=3.0, nodes=16, mem=SKALAK)]),
+        steps=11, every=10, legacy=False, all_csv=True, extra_fluid=True, probes=[(4, 4), (16, 7), (40, -3)]),
+    "closed_legacy_mrt": dict(
+        case=dict(nx=30, ny=16, tau=0.9, bfx=2e-5, coll=1, boundary="closed",
+                  capsules=[dict(centers=[(12.0, 8.0)], rmin=3.5, nodes=18, mem=SKALAK)]),
+        steps=11, every=5, legacy=True, all_csv=False, extra_fluid=False, probes=[(10, 8)]),
+}
+
+
+def write_writer_cases(force):
+    import gzip
+    import shutil
+    import tempfile
+
+    for name, w in WRITER_CASES.items():
+        dest = os.path.join(HERE, "writers", name)
+        if os.path.isdir(dest) and not force:
+            continue
+        tmp = tempfile.mkdtemp()
+        sim = oracle_sim(w["case"], which="ref")
+        sim.output_open(tmp, legacy=w["legacy"], all_csv=w["all_csv"], extra_fluid=w["extra_fluid"],
+                        probes=w["probes"])
+        for st in range(w["steps"]):
+            sim.step(1)
+            if st % w["every"] == 0:
+                sim.output_write(st, float(st), vtk=True, csv=True, probes=True, stats=True)
+        sim.output_close()
+        shutil.rmtree(dest, ignore_errors=True)
+        for root, _, files in os.walk(tmp):
+            for fn in files:
+                rel = os.path.relpath(os.path.join(root, fn), tmp)
+                out = os.path.join(dest, rel + ".gz")
+                os.makedirs(os.path.dirname(out), exist_ok=True)
+                with open(os.path.join(root, fn), "rb") as src, open(out, "wb") as raw:
+                    with gzip.GzipFile(filename="", mode="wb", fileobj=raw, mtime=0) as gz:
+                        gz.write(src.read())
+        shutil.rmtree(tmp)
+        print("writers", name, "saved", flush=True)
+    with open(os.path.join(HERE, "writers", "cases.json"), "w") as f:
+        json.dump(WRITER_CASES, f, indent=1)
+
+
 def main():
     from oracle import pyoracle
 
@@ -84,6 +133,7 @@ def main():
         print(name, "saved", flush=True)
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(index, f, indent=1)
+    write_writer_cases(force)
 
 
 if __name__ == "__main__":

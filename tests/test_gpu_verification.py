@@ -21,6 +21,7 @@ pytestmark = pytest.mark.gpu
 def make(nx, ny, tau=0.8, coll=0, G=0.0, periodic_y=False, precision="fp64", D=None, walls=(0.0, 0.0)):
     from softflow_amd import softflow_core as sc
     p = sc.SimulationParams()
+    p.output_interval, p.probe_dump_every, p.stats_dump_every = 0, 0, 0   # no output files
     p.nx, p.ny = nx, ny
     p.fluid.tau = tau
     p.fluid.boundary_type = sc.BoundaryType.PERIODIC
