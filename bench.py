@@ -60,6 +60,8 @@ def build_product(case, precision="fp32", graph=True):
 
     p = sc.SimulationParams()
     p.output_interval, p.probe_dump_every, p.stats_dump_every = 0, 0, 0   # no output files
+    for k, v in case.get("output", {}).items():
+        setattr(p, k, v)
     p.nx, p.ny = case["nx"], case["ny"]
     p.fluid.tau = case["tau"]
     p.fluid.boundary_type = sc.BoundaryType.PERIODIC
@@ -316,6 +318,77 @@ def ensemble_bench(steps, count=256, seed=7, seq_steps=60):
     return out
 
 
+def output_bench(case, base_ms, steps):
+    """The writers on the 4096x1024 workload.  (1) The step rate with the
+    reference's default statistics (every 100 steps) and probe (every 10 steps,
+    4 probes) output against the same run without output.  (2) One VTK + CSV
+    dump: wall time from the snapshot to the files closed, bytes written.
+    (3) The reference's own VTK writer (oracle/_ref, one thread) on a 1024x256
+    field of the same kind, where that build exists, as the CPU rate."""
+    import shutil
+    import tempfile
+    from softflow_amd import softflow_core as sc
+
+    out = {}
+    tmp = tempfile.mkdtemp(prefix="sf_out_")
+    try:
+        # default statistics and probe intervals of SimulationParams
+        p2 = dict(output_dir=tmp, stats_dump_every=100, probe_dump_every=10)
+        sim = build_product(dict(case, output=p2), "fp32", graph=USE_GRAPH)
+        for k, (i, j) in enumerate([(100, 100), (2048, 512), (3000, 900), (4000, 20)]):
+            sim.addFluidProbe(i, j, f"p{k}")
+        sim.run(20)
+        sim.getMaxSpeed()
+        t = time.perf_counter()
+        sim.run(steps)
+        sim.getMaxSpeed()
+        dt = time.perf_counter() - t
+        ms = dt * 1e3 / steps
+        out["stats_probes_ms_per_step"] = round(ms, 5)
+        out["stats_probes_overhead"] = round(ms / base_ms - 1.0, 4)
+        # one VTK + CSV dump at the next step (the writer thread formats on 16 threads)
+        sim.setVTKOutput(1, "ascii", sc.FluidOutputFields(), sc.ParticleVTKFields())
+        sim.setCSVOutput(1, "csv", sc.ParticleOutputFields(), sc.ParticleFilter(), True)
+        t = time.perf_counter()
+        sim.run(1)                       # finalize(): waits for the files
+        dump_s = time.perf_counter() - t
+        nbytes = 0
+        for root, _, files in os.walk(tmp):
+            for fn in files:
+                if fn.endswith((".vti", ".vtp")) or fn == "particle_data.csv":
+                    nbytes += os.path.getsize(os.path.join(root, fn))
+        out["vtk_csv_dump_s"] = round(dump_s, 3)
+        out["vtk_csv_dump_mb"] = round(nbytes / 1e6, 1)
+        out["vtk_csv_dump_mb_per_s"] = round(nbytes / 1e6 / dump_s, 1)
+        del sim
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    try:
+        from oracle import pyoracle as po
+        if os.path.exists(po.REF_SO):
+            rtmp = tempfile.mkdtemp(prefix="sf_ref_out_")
+            try:
+                r = po.Sim(po.default_params(nx=1024, ny=256, tau=0.7, boundary_type=1, body_force_x=1e-5,
+                                             collision_model=1, scalar_enabled=1, diffusivity=[0.05]), "ref")
+                r.initialize()
+                r.step(2)
+                r.output_open(rtmp)
+                t = time.perf_counter()
+                r.output_write(0, 0.0, vtk=True)
+                rdt = time.perf_counter() - t
+                r.output_close()
+                rb = sum(os.path.getsize(os.path.join(a, f)) for a, _, fs in os.walk(rtmp) for f in fs
+                         if f.endswith(".vti"))
+                out["reference_vtk_writer"] = {"mb_per_s": round(rb / 1e6 / rdt, 1), "cores": 1,
+                                               "sample": f"VTKWriter::writeFluidField on 1024x256 with the "
+                                                         f"scalar ({rb / 1e6:.1f} MB in {rdt:.2f} s)"}
+            finally:
+                shutil.rmtree(rtmp, ignore_errors=True)
+    except Exception as e:  # the reference build is optional on this host
+        out["reference_vtk_writer"] = {"error": str(e)}
+    return out
+
+
 def calibrate_copy(mib):
     import torch
 
@@ -367,7 +440,10 @@ def main():
                     help="steps of the members run one after another for comparison (0: skip)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every kernel eagerly instead of replaying HIP graphs (for tracing)")
-    ap.add_argument("--only", choices=["ensemble", "mdf"], default=None,
+    ap.add_argument("--no-output", action="store_true",
+                    help="skip the writer measurement")
+    ap.add_argument("--output-steps", type=int, default=500)
+    ap.add_argument("--only", choices=["ensemble", "mdf", "output"], default=None,
                     help="run only the batched-ensemble or the learned-vs-plain MDF measurement "
                          "and print its object as the JSON line (for profiling)")
     args = ap.parse_args()
@@ -383,9 +459,21 @@ def main():
     json_fd = os.dup(1)
     os.dup2(2, 1)
     if args.only:
-        part = (ensemble_bench(args.ensemble_steps, seq_steps=args.ensemble_sequential_steps)
-                if args.only == "ensemble"
-                else mdf_learned_bench(args.mdf_steps, 200, 160, 6))
+        if args.only == "ensemble":
+            part = ensemble_bench(args.ensemble_steps, seq_steps=args.ensemble_sequential_steps)
+        elif args.only == "mdf":
+            part = mdf_learned_bench(args.mdf_steps, 200, 160, 6)
+        else:
+            case = large_domain_case()
+            sim = build_product(case, "fp32", graph=USE_GRAPH)
+            sim.run(20)
+            sim.getMaxSpeed()
+            t = time.perf_counter()
+            sim.run(args.output_steps)
+            sim.getMaxSpeed()
+            base = (time.perf_counter() - t) * 1e3 / args.output_steps
+            del sim
+            part = dict(output_bench(case, base, args.output_steps), base_ms_per_step=round(base, 5))
         sys.stdout.flush()
         os.write(json_fd, (json.dumps(part) + "\n").encode())
         return
@@ -446,6 +534,13 @@ def main():
 
     stats = sim.fieldStats()
     del sim
+    outb = None
+    if not args.no_output:
+        try:
+            outb = output_bench(case, ms_per_step, args.output_steps)
+        except RuntimeError as e:
+            outb = {"error": str(e)}
+        print("# writers: " + json.dumps(outb), file=sys.stderr, flush=True)
     mdf = None
     if not args.no_mdf:
         try:
@@ -493,6 +588,7 @@ def main():
         "cpu_baseline": cpu,
         "mdf_learned_vs_plain": mdf,
         "batched_ensemble": ens,
+        "writers": outb,
         "kernel_ms": {k: round(v, 5) for k, v in sorted(kt.items())},
         "lbm_launch": tune.strip().splitlines()[-1],
         "mass": stats[0],

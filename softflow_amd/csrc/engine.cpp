@@ -727,6 +727,7 @@ void Simulation::run(int num_steps) {
         intervalCheckpoint(step_count_ - 1);
         done += chunk;
     }
+    check(sf_synchronize(ctx_), "sf_synchronize");   // capacity overflows of the last calls
     finalize();
 }
 

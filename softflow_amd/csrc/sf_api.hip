@@ -210,58 +210,109 @@ void ctx_one_step(Ctx& c) { one_step(c); }
 
 static void invalidate_graph(Ctx& c) {
     c.gen++;
-    if (c.gexec) hipGraphExecDestroy(c.gexec);
-    if (c.graph) hipGraphDestroy(c.graph);
-    c.gexec = nullptr;
-    c.graph = nullptr;
-    c.graph_valid = false;
+    for (Ctx::GraphEntry& g : c.graphs) {
+        hipGraphExecDestroy(g.exec);
+        hipGraphDestroy(g.graph);
+    }
+    c.graphs.clear();
 }
+
+constexpr size_t MAX_GRAPHS = 24;
 
 // Graph replay: the step's buffer rotation has period <= 6 (two population
 // buffers, a permutation of three force buffers), so six steps are captured
-// as one graph whose end state maps buffers back onto the start state.
+// as one graph whose end state maps buffers back onto the start state.  A
+// shorter remainder (a run cut at an output step, say) is a graph of its own
+// length, which leaves the roles rotated as the same steps run eagerly would.
 static int run_graph(Ctx& c, int nsteps) {
     const int period = 6;
     while (nsteps > 0) {
-        if (c.shift_next != SHIFT_PREV || nsteps < period ||
+        if (c.shift_next != SHIFT_PREV ||
             (c.nsp > 0 && (c.c_from_store || (mdf_corrections(c) <= 0 && !c.adr_pending)))) {
             one_step(c);
             --nsteps;
             continue;
         }
-        const int f0 = c.fcur, g0 = c.gcur, a0 = c.Fcur, b0 = c.Fprev, p0 = c.Fpart;
+        const int len = std::min(nsteps, period);
+        const int f0 = c.fcur, g0 = c.gcur, a0 = c.Fcur, b0 = c.Fprev;
         const int key[5] = {f0, g0, a0, b0, c.cell_cur};
-        // a graph bakes in the buffer roles it was captured with: a remainder
-        // step run eagerly since then leaves them rotated, so capture again
-        if (c.graph_valid && !std::equal(key, key + 5, c.graph_key)) invalidate_graph(c);
-        if (!c.graph_valid) {
+        // a graph bakes in the buffer roles it was captured with: replay the
+        // one captured from these roles, or capture it
+        Ctx::GraphEntry* hit = nullptr;
+        for (Ctx::GraphEntry& g : c.graphs)
+            if (g.len == len && std::equal(key, key + 5, g.key)) hit = &g;
+        if (!hit) {
+            Ctx::GraphEntry g{};
+            std::copy(key, key + 5, g.key);
+            g.len = len;
             hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
-            for (int s = 0; s < period; ++s) one_step(c);
-            hipError_t e = hipStreamEndCapture(c.stream, &c.graph);
-            c.step_count -= period;   // counted again below
+            for (int s = 0; s < len; ++s) one_step(c);
+            hipError_t e = hipStreamEndCapture(c.stream, &g.graph);
+            c.step_count -= len;   // counted again below
             if (e != hipSuccess) return fail(SF_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-            if (c.fcur != f0 || c.gcur != g0 || c.Fcur != a0 || c.Fprev != b0)
+            if (len == period && (c.fcur != f0 || c.gcur != g0 || c.Fcur != a0 || c.Fprev != b0)) {
+                hipGraphDestroy(g.graph);
                 return fail(SF_E_STATE, "step buffer rotation did not close over the graph period");
-            SF_HIP(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
-            std::copy(key, key + 5, c.graph_key);
-            c.graph_valid = true;
+            }
+            const int out[6] = {c.fcur, c.gcur, c.Fcur, c.Fprev, c.Fpart, c.cell_cur};
+            std::copy(out, out + 6, g.out);
+            e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                hipGraphDestroy(g.graph);
+                SF_HIP(e);
+            }
+            if (c.graphs.size() >= MAX_GRAPHS) {
+                hipGraphExecDestroy(c.graphs.front().exec);
+                hipGraphDestroy(c.graphs.front().graph);
+                c.graphs.erase(c.graphs.begin());
+            }
+            c.graphs.push_back(g);
+            hit = &c.graphs.back();
         }
-        SF_HIP(hipGraphLaunch(c.gexec, c.stream));
-        c.step_count += period;
-        c.Fpart = p0;
-        nsteps -= period;
+        SF_HIP(hipGraphLaunch(hit->exec, c.stream));
+        c.fcur = hit->out[0]; c.gcur = hit->out[1]; c.Fcur = hit->out[2];
+        c.Fprev = hit->out[3]; c.Fpart = hit->out[4]; c.cell_cur = hit->out[5];
+        c.step_count += len;
+        nsteps -= len;
         c.c_from_store = false;
     }
     return SF_OK;
 }
 
-static int check_counters(Ctx& c) {
-    SF_HIP(hipMemcpyAsync(c.host_counters, c.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    SF_HIP(hipStreamSynchronize(c.stream));
+static int counters_verdict(Ctx& c) {
     if (c.host_counters[1] & 1) return fail(SF_E_CAPACITY, "IBM tile node list overflowed (more than 1024 nodes near one 32x8 tile)");
     if (c.host_counters[2] & 1) return fail(SF_E_CAPACITY, "contact cell list overflowed (more than 32 nodes in one cell)");
     if (c.host_counters[2] & 2) return fail(SF_E_CAPACITY, "contact cell grid exceeded its capacity");
     return SF_OK;
+}
+
+static int poll_counters(Ctx& c, bool wait) {
+    if (!c.ctr_pending) return SF_OK;
+    if (wait) {
+        SF_HIP(hipEventSynchronize(c.ctr_ev));
+    } else {
+        const hipError_t q = hipEventQuery(c.ctr_ev);
+        if (q == hipErrorNotReady) return SF_OK;
+        SF_HIP(q);
+    }
+    c.ctr_pending = false;
+    for (int k = 0; k < 4; ++k) c.host_counters[k] = c.ctr_host[k];
+    return counters_verdict(c);
+}
+
+// The overflow flags are sticky on the device.  After each sf_step their copy
+// is enqueued behind the steps and read once it has landed: at the next call
+// that finds it complete, or at a synchronisation (sync = true waits for it).
+// Short sf_step calls (stepping between output steps) therefore do not drain
+// the stream each time.
+static int check_counters(Ctx& c, bool sync) {
+    if (!c.ctr_ev) SF_HIP(hipEventCreateWithFlags(&c.ctr_ev, hipEventDisableTiming));
+    if (!sync) {
+        SF_HIP(hipMemcpyAsync(c.ctr_host, c.counters, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        SF_HIP(hipEventRecord(c.ctr_ev, c.stream));
+        c.ctr_pending = true;
+    }
+    return poll_counters(c, sync);
 }
 
 static double phi_exact_host(double r) {       // coupling/delta_function.h:18-26
@@ -461,6 +512,7 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     chk(dmalloc(&c.mdf, sizeof(MdfDev)));
     if (rc != SF_OK) { sf_destroy(h); return rc; }
     SF_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host_counters), 64 * sizeof(int)));
+    c.ctr_host = c.host_counters + 32;
     // flags: the channel of LBMSolver::initialize (lbm_solver.cpp:130-158);
     // with inlet/outlet, INLET and OUTLET columns between the walls
     std::vector<uint8_t> fl(N, SF_FLUID);
@@ -539,14 +591,15 @@ int sf_destroy(sf_ctx* h) {
                     c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_act, c.tile_rec, c.tile_dirty, c.cell_cnt, c.cell_list,
                     c.cell_pay, c.cell_own, c.mdf, c.ds_buf, c.obs, c.obs_v, c.ibb_cell, c.ibb_off,
                     c.ibb_q, c.ibb_up, c.ibb_qf, c.snap_dev, c.probe_cells};
-    for (auto& t : c.snap_tickets) hipEventDestroy(t.second);
-    c.snap_tickets.clear();
+    delete c.snap_sync;     // the stream has drained: no host function is pending
+    c.snap_sync = nullptr;
     for (void* p : ptrs) if (p) hipFree(p);
     for (int s = 0; s < 4; ++s) {
         for (int b = 0; b < 2; ++b) if (c.g[s][b]) hipFree(c.g[s][b]);
         if (c.Cstore[s]) hipFree(c.Cstore[s]);
     }
     if (c.host_counters) hipHostFree(c.host_counters);
+    if (c.ctr_ev) hipEventDestroy(c.ctr_ev);
     auto it = pools().find(&c);
     if (it != pools().end()) {
         for (auto& pr : it->second.freelist) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
@@ -948,7 +1001,7 @@ int sf_step(sf_ctx* h, int nsteps) {
         for (int s = 0; s < nsteps; ++s) one_step(c);
     }
     SF_HIP(hipGetLastError());
-    int rc = check_counters(c);
+    int rc = check_counters(c, false);
     if (c.timing) harvest_timing(c);
     return rc;
 }
@@ -963,7 +1016,7 @@ int sf_set_graph_mode(sf_ctx* h, int on) {
 int sf_synchronize(sf_ctx* h) {
     if (!h) return fail(SF_E_ARG, "null pointer");
     SF_HIP(hipStreamSynchronize(h->c.stream));
-    return SF_OK;
+    return poll_counters(h->c, true);
 }
 
 int sf_stage(sf_ctx* h, int stage) {
@@ -990,7 +1043,8 @@ int sf_stage(sf_ctx* h, int stage) {
     default: return fail(SF_E_ARG, "unknown stage");
     }
     SF_HIP(hipGetLastError());
-    return check_counters(c);
+    const int rc = check_counters(c, false);
+    return rc != SF_OK ? rc : poll_counters(c, true);
 }
 
 int sf_last_capped(sf_ctx* h, int32_t* capped) {
@@ -1120,27 +1174,43 @@ int sf_snapshot_async(sf_ctx* h, const sf_snapshot* s, int64_t* ticket) {
     SF_HIP(d2h(s->fx, c.fx, K)); SF_HIP(d2h(s->fy, c.fy, K));
     SF_HIP(d2h(s->probes, st + o_pr, 3 * static_cast<size_t>(c.nprobes)));
     SF_HIP(d2h(s->stats, st + o_st, 12));
-    hipEvent_t ev;
-    SF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    SF_HIP(hipEventRecord(ev, c.stream));
+    if (!c.snap_sync) c.snap_sync = new SnapSync();
+    struct Note { SnapSync* s; int64_t t; };
+    Note* note = new Note{c.snap_sync, c.snap_next};
+    {
+        std::lock_guard<std::mutex> lk(c.snap_sync->mu);
+        c.snap_sync->issued.push_back(note->t);
+    }
+    hipError_t e = hipLaunchHostFunc(c.stream, [](void* p) {
+        Note* n = static_cast<Note*>(p);
+        {
+            std::lock_guard<std::mutex> lk(n->s->mu);
+            n->s->done.push_back(n->t);
+        }
+        n->s->cv.notify_all();
+        delete n;
+    }, note);
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> lk(c.snap_sync->mu);
+        c.snap_sync->issued.pop_back();
+        delete note;
+        SF_HIP(e);
+    }
     *ticket = c.snap_next++;
-    c.snap_tickets.emplace_back(*ticket, ev);
     return SF_OK;
 }
 
 int sf_snapshot_wait(sf_ctx* h, int64_t ticket) {
     if (!h) return fail(SF_E_ARG, "null pointer");
-    Ctx& c = h->c;
-    for (size_t i = 0; i < c.snap_tickets.size(); ++i) {
-        if (c.snap_tickets[i].first != ticket) continue;
-        hipEvent_t ev = c.snap_tickets[i].second;
-        c.snap_tickets.erase(c.snap_tickets.begin() + static_cast<long>(i));
-        hipError_t e = hipEventSynchronize(ev);
-        hipEventDestroy(ev);
-        SF_HIP(e);
-        return SF_OK;
-    }
-    return fail(SF_E_ARG, "unknown snapshot ticket");
+    SnapSync* s = h->c.snap_sync;
+    if (!s) return fail(SF_E_ARG, "unknown snapshot ticket");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = std::find(s->issued.begin(), s->issued.end(), ticket);
+    if (it == s->issued.end()) return fail(SF_E_ARG, "unknown snapshot ticket");
+    s->cv.wait(lk, [&] { return std::find(s->done.begin(), s->done.end(), ticket) != s->done.end(); });
+    s->issued.erase(std::find(s->issued.begin(), s->issued.end(), ticket));
+    s->done.erase(std::find(s->done.begin(), s->done.end(), ticket));
+    return SF_OK;
 }
 
 int sf_step_count(sf_ctx* h, int64_t* n) {

@@ -2,8 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -97,6 +99,15 @@ struct Timer {
     int launches;
 };
 
+// Completion of output snapshots, signalled by a host function the stream runs
+// after a snapshot's copies, so that a thread waiting for one makes no HIP call
+// (the stepping thread may be capturing a graph on the same stream meanwhile).
+struct SnapSync {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<int64_t> issued, done;   // tickets enqueued and not yet waited for / completed
+};
+
 struct Ctx {
     sf_params p{};
     int device = 0;
@@ -171,7 +182,7 @@ struct Ctx {
     size_t snap_cap = 0;                 // doubles
     int nprobes = 0;
     int* probe_cells = nullptr;
-    std::vector<std::pair<int64_t, hipEvent_t>> snap_tickets;
+    SnapSync* snap_sync = nullptr;       // tickets completed by host callbacks on the stream
     int64_t snap_next = 1;
 
     // repulsion cells
@@ -206,15 +217,24 @@ struct Ctx {
 
     // counters: [0] capped cells, [1] tile overflow, [2] cell overflow, [3] touched tiles
     int* counters = nullptr;
-    int* host_counters = nullptr;        // pinned
+    int* host_counters = nullptr;        // pinned (64 ints; the last 32 take the lagged copy)
+    int* ctr_host = nullptr;             // host_counters + 32: the copy enqueued after sf_step
+    hipEvent_t ctr_ev = nullptr;         // that copy has landed
+    bool ctr_pending = false;
 
     // graph
     bool graph_mode = false;
-    bool graph_valid = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_shift = -1;
-    int graph_key[5] = {};               // buffer roles at capture (the graph replays only from them)
+    // captured step graphs, one per (buffer roles at capture, length): a graph
+    // replays only from the roles it was captured with, and the rotation
+    // cycles through a few such sets
+    struct GraphEntry {
+        int key[5];
+        int len;                         // steps in the graph (1..6)
+        int out[6];                      // fcur, gcur, Fcur, Fprev, Fpart, cell_cur after them
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
     int64_t gen = 0;                     // bumped whenever buffers, parameters or launch shapes change
 
     // timing
