@@ -1002,7 +1002,12 @@ int sf_step(sf_ctx* h, int nsteps) {
     }
     SF_HIP(hipGetLastError());
     int rc = check_counters(c, false);
-    if (c.timing) harvest_timing(c);
+    if (c.timing) {
+        // the per-launch events are read once every launch has completed
+        if (rc == SF_OK) rc = poll_counters(c, true);
+        SF_HIP(hipStreamSynchronize(c.stream));
+        harvest_timing(c);
+    }
     return rc;
 }
 

@@ -279,8 +279,10 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
             for (int i = bx * blockDim.x + t; i < nact; i += gx * blockDim.x) tile_cnt[act[i].x] = 0;
         return;
     }
-    if (nact > 0)
-        for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
+    // workgroups past the touched-tile count have nothing to do: they leave
+    // before loading the table (the grid is sized for every tile)
+    if (bx >= nact) return;
+    for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
     for (int ai = bx; ai < nact; ai += gx) {
         const int2 te = act[ai];
         const int tile = te.x;
