@@ -318,32 +318,43 @@ def ensemble_bench(steps, count=256, seed=7, seq_steps=60):
     return out
 
 
-def output_bench(case, base_ms, steps):
+def output_bench(case, steps):
     """The writers on the 4096x1024 workload.  (1) The step rate with the
     reference's default statistics (every 100 steps) and probe (every 10 steps,
-    4 probes) output against the same run without output.  (2) One VTK + CSV
-    dump: wall time from the snapshot to the files closed, bytes written.
-    (3) The reference's own VTK writer (oracle/_ref, one thread) on a 1024x256
-    field of the same kind, where that build exists, as the CPU rate."""
+    4 probes) output against the same workload without output, each measured
+    twice, interleaved, in this process.  (2) One VTK + CSV dump: wall time
+    from the snapshot to the files closed, bytes written.  (3) The reference's
+    own VTK writer (oracle/_ref, one thread) on a 1024x256 field of the same
+    kind, where that build exists, as the CPU rate."""
     import shutil
     import tempfile
     from softflow_amd import softflow_core as sc
 
+    def rate(sim):
+        sim.getMaxSpeed()
+        t = time.perf_counter()
+        sim.run(steps)
+        sim.getMaxSpeed()
+        return (time.perf_counter() - t) * 1e3 / steps
+
     out = {}
     tmp = tempfile.mkdtemp(prefix="sf_out_")
     try:
+        plain = build_product(case, "fp32", graph=USE_GRAPH)
+        plain.run(20)
         # default statistics and probe intervals of SimulationParams
         p2 = dict(output_dir=tmp, stats_dump_every=100, probe_dump_every=10)
         sim = build_product(dict(case, output=p2), "fp32", graph=USE_GRAPH)
         for k, (i, j) in enumerate([(100, 100), (2048, 512), (3000, 900), (4000, 20)]):
             sim.addFluidProbe(i, j, f"p{k}")
         sim.run(20)
-        sim.getMaxSpeed()
-        t = time.perf_counter()
-        sim.run(steps)
-        sim.getMaxSpeed()
-        dt = time.perf_counter() - t
-        ms = dt * 1e3 / steps
+        base = [rate(plain)]
+        ms = [rate(sim)]
+        base.append(rate(plain))
+        ms.append(rate(sim))
+        del plain
+        base_ms, ms = sum(base) / 2, sum(ms) / 2
+        out["base_ms_per_step"] = round(base_ms, 5)
         out["stats_probes_ms_per_step"] = round(ms, 5)
         out["stats_probes_overhead"] = round(ms / base_ms - 1.0, 4)
         # one VTK + CSV dump at the next step (the writer thread formats on 16 threads)
@@ -464,16 +475,7 @@ def main():
         elif args.only == "mdf":
             part = mdf_learned_bench(args.mdf_steps, 200, 160, 6)
         else:
-            case = large_domain_case()
-            sim = build_product(case, "fp32", graph=USE_GRAPH)
-            sim.run(20)
-            sim.getMaxSpeed()
-            t = time.perf_counter()
-            sim.run(args.output_steps)
-            sim.getMaxSpeed()
-            base = (time.perf_counter() - t) * 1e3 / args.output_steps
-            del sim
-            part = dict(output_bench(case, base, args.output_steps), base_ms_per_step=round(base, 5))
+            part = output_bench(large_domain_case(), args.output_steps)
         sys.stdout.flush()
         os.write(json_fd, (json.dumps(part) + "\n").encode())
         return
@@ -537,7 +539,7 @@ def main():
     outb = None
     if not args.no_output:
         try:
-            outb = output_bench(case, ms_per_step, args.output_steps)
+            outb = output_bench(case, args.output_steps)
         except RuntimeError as e:
             outb = {"error": str(e)}
         print("# writers: " + json.dumps(outb), file=sys.stderr, flush=True)

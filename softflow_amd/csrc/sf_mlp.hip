@@ -39,76 +39,198 @@ __device__ __forceinline__ float gelu_d(float z) {
            z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
+// The network on the matrix cores.  A wavefront evaluates 32 nodes at once:
+// each layer is a [32 x K] x [K x 64] product done as two 32x32 output tiles
+// of v_mfma_f32_32x32x2f32 (K in steps of 2), with the weights staged once
+// per workgroup in LDS and the activations passed between layers through
+// LDS.  Operand layout of that instruction (wave64): lane l supplies
+// A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator r of lane l holds
+// D[r % 4 + 8 (r / 4) + 4 (l / 32)][l % 32].  Rows of odd length keep the
+// LDS accesses of a 32-lane half free of bank conflicts.  The sums run in the
+// matrix cores' order; the training forward (k_mlp_grad) runs them in index
+// order, so the two agree to FP32 rounding.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int NODES_PER_WAVE = 32;
+constexpr int X_LD = 19;                 // 17 normalised inputs + zero padding
+constexpr int W1_LD = 19;
+constexpr int H_LD = MLP_W + 1;
+constexpr int MLP_WAVES = 4;             // wavefronts per workgroup
+constexpr int MLP_NODES_WG = MLP_WAVES * NODES_PER_WAVE;
+
+struct MlpStage {
+    float w1[MLP_W * W1_LD], b1[MLP_W], w2[MLP_W * H_LD], b2[MLP_W];
+    float w3[MLP_OUT * MLP_W], b3[MLP_OUT], norm[MLP_NORM];
+};
+struct MlpWave {
+    float x[NODES_PER_WAVE * X_LD], h1[NODES_PER_WAVE * H_LD], h2[NODES_PER_WAVE * H_LD];
+    int bad[NODES_PER_WAVE];
+};
+
+// Every thread issues all its loads before its LDS stores (fixed trip counts),
+// so a workgroup waits for the weights about once, not once per row.
+constexpr int STAGE_THREADS = MLP_WAVES * 64;
+constexpr int W1_PER = (MLP_W * MLP_IN + STAGE_THREADS - 1) / STAGE_THREADS;
+constexpr int W2_PER = (MLP_W * MLP_W + STAGE_THREADS - 1) / STAGE_THREADS;
+
+__device__ void mlp_stage(MlpStage& s, const float* P, const float* norm) {
+    const int t = threadIdx.x;
+    float r1[W1_PER], r2[W2_PER];
+#pragma unroll
+    for (int u = 0; u < W1_PER; ++u) {
+        const int i = t + u * STAGE_THREADS;
+        r1[u] = i < MLP_W * MLP_IN ? P[W1_OFF + i] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < W2_PER; ++u) {
+        const int i = t + u * STAGE_THREADS;
+        r2[u] = i < MLP_W * MLP_W ? P[W2_OFF + i] : 0.f;
+    }
+    float rb1 = 0.f, rb2 = 0.f, rw3 = 0.f, rb3 = 0.f, rn = 0.f;
+    if (t < MLP_W) { rb1 = P[B1_OFF + t]; rb2 = P[B2_OFF + t]; }
+    if (t < MLP_OUT * MLP_W) rw3 = P[W3_OFF + t];
+    if (t < MLP_OUT) rb3 = P[B3_OFF + t];
+    if (t < MLP_NORM) rn = norm[t];
+#pragma unroll
+    for (int u = 0; u < W1_PER; ++u) {
+        const int i = t + u * STAGE_THREADS;
+        if (i < MLP_W * MLP_IN) s.w1[(i / MLP_IN) * W1_LD + i % MLP_IN] = r1[u];
+    }
+    if (t < MLP_W)
+        for (int k = MLP_IN; k < W1_LD; ++k) s.w1[t * W1_LD + k] = 0.f;   // the padding the inputs meet
+#pragma unroll
+    for (int u = 0; u < W2_PER; ++u) {
+        const int i = t + u * STAGE_THREADS;
+        if (i < MLP_W * MLP_W) s.w2[(i / MLP_W) * H_LD + i % MLP_W] = r2[u];
+    }
+    if (t < MLP_W) { s.b1[t] = rb1; s.b2[t] = rb2; }
+    if (t < MLP_OUT * MLP_W) s.w3[t] = rw3;
+    if (t < MLP_OUT) s.b3[t] = rb3;
+    if (t < MLP_NORM) s.norm[t] = rn;
+}
+
+// Inputs of nodes k0 .. k0+31 (rows of x, clipped at n) normalised into the
+// wave's LDS; bad[node] is set where an input is not finite.  The caller
+// synchronises the workgroup before the values are read.
+constexpr int LOAD_PER = (NODES_PER_WAVE * MLP_IN + 63) / 64;
+
+__device__ void mlp_load(const MlpStage& s, MlpWave& w, const float* x, int k0, int n) {
+    const int l = threadIdx.x & 63;
+    if (l < NODES_PER_WAVE) {
+        w.bad[l] = 0;
+#pragma unroll
+        for (int k = MLP_IN; k < X_LD; ++k) w.x[l * X_LD + k] = 0.f;
+    }
+    const int avail = max(0, min(NODES_PER_WAVE, n - k0)) * MLP_IN;   // contiguous input values
+    const float* src = x + static_cast<size_t>(max(k0, 0)) * MLP_IN;
+    float v[LOAD_PER];
+#pragma unroll
+    for (int u = 0; u < LOAD_PER; ++u) {
+        const int idx = l + 64 * u;
+        v[u] = idx < avail ? src[idx] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < LOAD_PER; ++u) {
+        const int idx = l + 64 * u;
+        if (idx < NODES_PER_WAVE * MLP_IN) {
+            const int node = idx / MLP_IN, f = idx % MLP_IN;
+            w.x[node * X_LD + f] = (v[u] - s.norm[f]) / s.norm[MLP_IN + f];
+            if (!isfinite(v[u])) w.bad[node] = 1;
+        }
+    }
+}
+
+// The network on the wave's 32 loaded nodes.  Returns, on lane l, output l / 32
+// of node l % 32 (before denormalisation).  Every wave of the workgroup calls it.
+__device__ float mlp_wave(const MlpStage& s, MlpWave& w) {
+    const int l = threadIdx.x & 63, lo = l & 31, kh = l >> 5;
+    f32x16 c0 = {}, c1 = {};
+    for (int kk = 0; kk < (MLP_IN + 1) / 2; ++kk) {
+        const int k = 2 * kk + kh;
+        const float a = w.x[lo * X_LD + k];
+        c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, s.w1[lo * W1_LD + k], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, s.w1[(32 + lo) * W1_LD + k], c1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = (r & 3) + 8 * (r >> 2) + 4 * kh;
+        w.h1[i * H_LD + lo] = gelu(c0[r] + s.b1[lo]);
+        w.h1[i * H_LD + 32 + lo] = gelu(c1[r] + s.b1[32 + lo]);
+    }
+    __syncthreads();
+    c0 = f32x16{};
+    c1 = f32x16{};
+    for (int kk = 0; kk < MLP_W / 2; ++kk) {
+        const int k = 2 * kk + kh;
+        const float a = w.h1[lo * H_LD + k];
+        c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, s.w2[lo * H_LD + k], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, s.w2[(32 + lo) * H_LD + k], c1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = (r & 3) + 8 * (r >> 2) + 4 * kh;
+        w.h2[i * H_LD + lo] = gelu(c0[r] + s.b2[lo]);
+        w.h2[i * H_LD + 32 + lo] = gelu(c1[r] + s.b2[32 + lo]);
+    }
+    __syncthreads();
+    float out = s.b3[kh];
+    for (int k = 0; k < MLP_W; ++k) out += s.w3[kh * MLP_W + k] * w.h2[lo * H_LD + k];
+    return out;
+}
+
 // Inference inside the step: the network's two outputs are the gains of the
 // node's total correction in the frame of its first plain correction dF1,
 // T = g_par dF1 + g_perp rot90(dF1).  They go to gain[2k], held to
 // g_par in [0, 4] and g_perp in [-2, 2], and default to (1, 0), the plain
 // correction, where an input or an output is not finite.  Nothing
 // happens when the residual is already within tolerance.
-//
-// One wavefront per node and pass, lane u computing hidden unit u of both
-// layers from weights staged once per workgroup in LDS (W2 rows padded to
-// avoid bank conflicts); lanes 0 and 1 form the two outputs.  Every sum runs
-// over its terms in index order, as the one-thread-per-node form of the same
-// network (k_mlp_forward) does.
-constexpr int INF_SLOTS = 4;            // nodes in flight per 256-thread workgroup
-constexpr int INF_CHUNK = 32;           // nodes per workgroup
-constexpr int W2_LD = MLP_W + 1;
-
-__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ P,
-                                                   const float* __restrict__ norm, const float* feat,
-                                                   double* gain, int K, const MdfDev* md, double tol) {
+__global__ __launch_bounds__(MLP_WAVES * 64) void k_mlp_infer(const float* __restrict__ P,
+                                                              const float* __restrict__ norm,
+                                                              const float* feat, double* gain, int K,
+                                                              const MdfDev* md, double tol) {
     if (tol > 0.0 && __longlong_as_double(static_cast<long long>(md->res[1])) <= tol) return;
-    __shared__ float sW1[MLP_W * MLP_IN], sb1[MLP_W], sW2[MLP_W * W2_LD], sb2[MLP_W];
-    __shared__ float sW3[MLP_OUT * MLP_W], sb3[MLP_OUT], snorm[MLP_NORM];
-    __shared__ float sx[INF_SLOTS][MLP_IN], sa1[INF_SLOTS][MLP_W], sa2[INF_SLOTS][MLP_W];
-    const int t = threadIdx.x;
-    for (int i = t; i < MLP_W * MLP_IN; i += 256) sW1[i] = P[W1_OFF + i];
-    for (int i = t; i < MLP_W * MLP_W; i += 256) sW2[(i / MLP_W) * W2_LD + i % MLP_W] = P[W2_OFF + i];
-    if (t < MLP_W) { sb1[t] = P[B1_OFF + t]; sb2[t] = P[B2_OFF + t]; }
-    if (t < MLP_OUT * MLP_W) sW3[t] = P[W3_OFF + t];
-    if (t < MLP_OUT) sb3[t] = P[B3_OFF + t];
-    if (t < MLP_NORM) snorm[t] = norm[t];
+    __shared__ MlpStage s;
+    __shared__ MlpWave waves[MLP_WAVES];
+    mlp_stage(s, P, norm);
     __syncthreads();
-    const int u = t & 63, q = t >> 6;
-    const int k_end = min(K, (blockIdx.x + 1) * INF_CHUNK);
-    for (int base = blockIdx.x * INF_CHUNK; base < k_end; base += INF_SLOTS) {
-        const int k = base + q;
-        const bool live = k < k_end;
-        float v = 0.f;
-        if (live && u < MLP_IN) v = feat[static_cast<size_t>(k) * MLP_IN + u];
-        const bool ok = __all(u >= MLP_IN || isfinite(v)) != 0;   // this node's wavefront
-        if (u < MLP_IN) sx[q][u] = (v - snorm[u]) / snorm[MLP_IN + u];
-        __syncthreads();
-        float acc = sb1[u];
-        for (int j = 0; j < MLP_IN; ++j) acc += sW1[u * MLP_IN + j] * sx[q][j];
-        sa1[q][u] = gelu(acc);
-        __syncthreads();
-        acc = sb2[u];
-        for (int j = 0; j < MLP_W; ++j) acc += sW2[u * W2_LD + j] * sa1[q][j];
-        sa2[q][u] = gelu(acc);
-        __syncthreads();
-        float out = 0.f;
-        if (u < MLP_OUT) {
-            out = sb3[u];
-            for (int i = 0; i < MLP_W; ++i) out += sW3[u * MLP_W + i] * sa2[q][i];
-        }
-        const float out1 = __shfl(out, 1);
-        if (live && u == 0) {
-            double g0 = 1.0, g1 = 0.0;
-            if (ok) {
-                const float gp = snorm[2 * MLP_IN] + snorm[2 * MLP_IN + MLP_OUT] * out;
-                const float gq = snorm[2 * MLP_IN + 1] + snorm[2 * MLP_IN + MLP_OUT + 1] * out1;
-                if (isfinite(gp) && isfinite(gq)) {
-                    g0 = fminf(fmaxf(gp, 0.f), 4.f);
-                    g1 = fminf(fmaxf(gq, -2.f), 2.f);
-                }
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int k0 = blockIdx.x * MLP_NODES_WG + wv * NODES_PER_WAVE;
+    mlp_load(s, waves[wv], feat, k0, K);
+    __syncthreads();
+    const bool ok = l < NODES_PER_WAVE && waves[wv].bad[l] == 0;
+    const float out = mlp_wave(s, waves[wv]);
+    const float out1 = __shfl(out, (l & 31) + 32);
+    const int k = k0 + l;
+    if (l < NODES_PER_WAVE && k < K) {
+        double g0 = 1.0, g1 = 0.0;
+        if (ok) {
+            const float gp = s.norm[2 * MLP_IN] + s.norm[2 * MLP_IN + MLP_OUT] * out;
+            const float gq = s.norm[2 * MLP_IN + 1] + s.norm[2 * MLP_IN + MLP_OUT + 1] * out1;
+            if (isfinite(gp) && isfinite(gq)) {
+                g0 = fminf(fmaxf(gp, 0.f), 4.f);
+                g1 = fminf(fmaxf(gq, -2.f), 2.f);
             }
-            gain[2 * k] = g0;
-            gain[2 * k + 1] = g1;
         }
-        __syncthreads();   // the slots are reused by the next pass
+        gain[2 * k] = g0;
+        gain[2 * k + 1] = g1;
     }
+}
+
+// The network alone on n inputs, on the matrix cores: y = out_mean + out_scale * net(x^).
+__global__ __launch_bounds__(MLP_WAVES * 64) void k_mlp_predict(const float* __restrict__ P,
+                                                                const float* __restrict__ norm,
+                                                                const float* x, float* y, int n) {
+    __shared__ MlpStage s;
+    __shared__ MlpWave waves[MLP_WAVES];
+    mlp_stage(s, P, norm);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int k0 = blockIdx.x * MLP_NODES_WG + wv * NODES_PER_WAVE;
+    mlp_load(s, waves[wv], x, k0, n);
+    __syncthreads();
+    const float out = mlp_wave(s, waves[wv]);
+    const int k = k0 + (l & 31), o = l >> 5;
+    if (k < n) y[static_cast<size_t>(k) * MLP_OUT + o] = s.norm[2 * MLP_IN + o] + s.norm[2 * MLP_IN + MLP_OUT + o] * out;
 }
 
 // The gains, low-passed along each membrane with the binomial weights
@@ -137,38 +259,6 @@ __global__ void k_mdf_smooth(const double* gain, NodeSpread* rec, const int* off
     const double ax = rec[k].fx, ay = rec[k].fy;    // dF1
     rec[k].fx = gp * ax - gq * ay;
     rec[k].fy = gp * ay + gq * ax;
-}
-
-// The network alone on n inputs: y = out_mean + out_scale * net(x^).
-__global__ __launch_bounds__(GB) void k_mlp_forward(const float* __restrict__ P,
-                                                    const float* __restrict__ norm, const float* x,
-                                                    float* y, int n) {
-    const int k = blockIdx.x * GB + threadIdx.x;
-    if (k >= n) return;
-    float xh[MLP_IN];
-#pragma unroll
-    for (int j = 0; j < MLP_IN; ++j) xh[j] = (x[static_cast<size_t>(k) * MLP_IN + j] - norm[j]) / norm[MLP_IN + j];
-    float a1[MLP_W];
-#pragma unroll
-    for (int i = 0; i < MLP_W; ++i) {
-        float acc = P[B1_OFF + i];
-#pragma unroll
-        for (int j = 0; j < MLP_IN; ++j) acc += P[W1_OFF + i * MLP_IN + j] * xh[j];
-        a1[i] = gelu(acc);
-    }
-    float out[MLP_OUT];
-#pragma unroll
-    for (int o = 0; o < MLP_OUT; ++o) out[o] = P[B3_OFF + o];
-    for (int i = 0; i < MLP_W; ++i) {
-        float acc = P[B2_OFF + i];
-#pragma unroll
-        for (int j = 0; j < MLP_W; ++j) acc += P[W2_OFF + i * MLP_W + j] * a1[j];
-        const float a2 = gelu(acc);
-#pragma unroll
-        for (int o = 0; o < MLP_OUT; ++o) out[o] += P[W3_OFF + o * MLP_W + i] * a2;
-    }
-#pragma unroll
-    for (int o = 0; o < MLP_OUT; ++o) y[static_cast<size_t>(k) * MLP_OUT + o] = norm[2 * MLP_IN + o] + norm[2 * MLP_IN + MLP_OUT + o] * out[o];
 }
 
 // Forward + backward of GB samples; the workgroup's gradient of
@@ -339,8 +429,9 @@ __global__ void k_colstats(const float* rows, int64_t nrows, int64_t chunk, doub
 
 void launch_mlp_infer(Ctx& c) {
     if (!c.model || c.K == 0) return;
-    hipLaunchKernelGGL(k_mlp_infer, dim3((c.K + INF_CHUNK - 1) / INF_CHUNK), dim3(256), 0, c.stream,
-                       c.model->params, c.model->norm, c.mdf_feat, c.mdf_pred, c.K, c.mdf, c.mdf_tol);
+    hipLaunchKernelGGL(k_mlp_infer, dim3((c.K + MLP_NODES_WG - 1) / MLP_NODES_WG), dim3(MLP_WAVES * 64), 0,
+                       c.stream, c.model->params, c.model->norm, c.mdf_feat, c.mdf_pred, c.K, c.mdf,
+                       c.mdf_tol);
     hipLaunchKernelGGL(k_mdf_smooth, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.mdf_pred,
                        c.spread, c.off, c.owner, c.K, c.mdf, c.mdf_tol);
 }
@@ -523,8 +614,8 @@ int sf_mlp_predict(sf_mlp* h, int n, const float* x, float* y) {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&dy), static_cast<size_t>(n) * MLP_OUT * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(dx, x, static_cast<size_t>(n) * MLP_IN * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_mlp_forward, dim3((n + GB - 1) / GB), dim3(GB), 0, m.stream, m.params,
-                           m.norm, dx, dy, n);
+        hipLaunchKernelGGL(k_mlp_predict, dim3((n + MLP_NODES_WG - 1) / MLP_NODES_WG), dim3(MLP_WAVES * 64),
+                           0, m.stream, m.params, m.norm, dx, dy, n);
         e = hipStreamSynchronize(m.stream);
     }
     if (e == hipSuccess) e = hipMemcpy(y, dy, static_cast<size_t>(n) * MLP_OUT * sizeof(float), hipMemcpyDeviceToHost);
