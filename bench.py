@@ -55,6 +55,25 @@ def large_domain_case():
                 scalar=dict(D=0.05, blob=(NX / 2, NY / 2, 20.0)))
 
 
+def pillar_case():
+    """4096x1024 channel through a staggered array of 200 circular pillars (radius 24,
+    160 apart in x, 128 in y) with interpolated bounce-back, 1000 capsules placed
+    around them, MRT, the scalar on: the large domain with obstacles."""
+    pillars = []
+    for row in range(8):
+        y = 64.0 + 128.0 * row + 0.21
+        x = (80.0 if row % 2 == 0 else 160.0) + 0.37
+        while x < NX - 40:
+            pillars.append(("circle", x, y, 24.0))
+            x += 160.0
+    big = dict(count=400, region=(0, 0, NX, NY), rmin=6.0, rmax=6.0, nodes=38, seed=303,
+               mem=dict(model=2, G_s=0.03, C_skalak=10.0, use_helfrich=1, k_bend=1e-3))
+    small = dict(count=600, region=(0, 0, NX, NY), rmin=3.0, rmax=3.0, nodes=19, seed=404,
+                 mem=dict(model=2, G_s=0.06, C_skalak=10.0, use_helfrich=1, k_bend=1e-3))
+    return dict(nx=NX, ny=NY, tau=0.7, bfx=1e-5, coll=1, capsules=[big, small], obstacles=pillars,
+                ibb=True, scalar=dict(D=0.05, blob=(NX / 2, NY / 2, 20.0)))
+
+
 def build_product(case, precision="fp32", graph=True):
     from softflow_amd import softflow_core as sc
 
@@ -73,9 +92,12 @@ def build_product(case, precision="fp32", graph=True):
     p.enable_stability_checks = False
     p.precision = sc.Precision.FP32 if precision == "fp32" else sc.Precision.FP64
     p.use_graph = graph
+    p.fluid.use_interpolated_bb = bool(case.get("ibb", False))
     sim = sc.Simulation(p)
     b = sc.ChannelBuilder(p.nx, p.ny)
     b.addWalls()
+    for ob in case.get("obstacles", []):
+        b.addCirclePillar(*ob[1:])
     sim.setChannelBuilder(b)
     for spec in case["capsules"]:
         mp = sc.MembraneParams()
@@ -108,8 +130,11 @@ def cpu_baseline(case, steps, threads):
     kind = "reference" if os.path.exists(po.REF_SO) else "port"
     p = po.default_params(nx=case["nx"], ny=case["ny"], tau=case["tau"], boundary_type=1,
                           body_force_x=case["bfx"], collision_model=case["coll"],
-                          scalar_enabled=1, diffusivity=[case["scalar"]["D"]])
+                          scalar_enabled=1, diffusivity=[case["scalar"]["D"]],
+                          use_interpolated_bb=1 if case.get("ibb") else 0)
     s = po.Sim(p, "ref" if kind == "reference" else "oracle")
+    for ob in case.get("obstacles", []):
+        s.add_obstacle(ob)
     if kind == "reference":
         s.set_threads(threads)
     else:
@@ -400,6 +425,33 @@ def output_bench(case, steps):
     return out
 
 
+def geometry_bench(steps, cpu_steps, threads, with_cpu=True):
+    """The pillar array (pillar_case): whole-step rate with obstacles, their contact
+    forces and interpolated bounce-back, and the reference's own components on the
+    same workload on the host cores (where oracle/_ref exists)."""
+    case = pillar_case()
+    sim = build_product(case, "fp32", graph=USE_GRAPH)
+    sim.run(20)
+    sim.getMaxSpeed()
+    t = time.perf_counter()
+    sim.run(steps)
+    sim.getMaxSpeed()
+    dt = time.perf_counter() - t
+    out = {"workload": "4096x1024 MRT tau=0.7 fx=1e-5 through 200 staggered circular pillars (r 24) "
+                       "with interpolated bounce-back, 1000 Skalak+Helfrich capsules, ADR D=0.05; FP32",
+           "pillars": len(case["obstacles"]), "capsules": sum(c["count"] for c in case["capsules"]),
+           "ms_per_step": round(dt * 1e3 / steps, 5), "mlups": round(NX * NY * steps / dt / 1e6, 1)}
+    del sim
+    if with_cpu:
+        try:
+            v, cdt, kind, cores = cpu_baseline(case, cpu_steps, threads)
+            out["cpu_baseline"] = {"value": round(v, 3), "unit": "MLUPS", "cores": cores, "kind": kind,
+                                   "sample": f"{cpu_steps} full steps of the pillar workload ({cdt:.1f} s)"}
+        except Exception as e:
+            out["cpu_baseline"] = {"value": None, "sample": f"unavailable: {e}"}
+    return out
+
+
 def calibrate_copy(mib):
     import torch
 
@@ -454,7 +506,9 @@ def main():
     ap.add_argument("--no-output", action="store_true",
                     help="skip the writer measurement")
     ap.add_argument("--output-steps", type=int, default=500)
-    ap.add_argument("--only", choices=["ensemble", "mdf", "output"], default=None,
+    ap.add_argument("--no-geometry", action="store_true",
+                    help="skip the pillar-array measurement")
+    ap.add_argument("--only", choices=["ensemble", "mdf", "output", "geometry"], default=None,
                     help="run only the batched-ensemble or the learned-vs-plain MDF measurement "
                          "and print its object as the JSON line (for profiling)")
     args = ap.parse_args()
@@ -474,6 +528,8 @@ def main():
             part = ensemble_bench(args.ensemble_steps, seq_steps=args.ensemble_sequential_steps)
         elif args.only == "mdf":
             part = mdf_learned_bench(args.mdf_steps, 200, 160, 6)
+        elif args.only == "geometry":
+            part = geometry_bench(args.steps, 5, max(1, min(16, os.cpu_count() or 1)), not args.no_cpu)
         else:
             part = output_bench(large_domain_case(), args.output_steps)
         sys.stdout.flush()
@@ -543,6 +599,14 @@ def main():
         except RuntimeError as e:
             outb = {"error": str(e)}
         print("# writers: " + json.dumps(outb), file=sys.stderr, flush=True)
+    geo = None
+    if not args.no_geometry:
+        try:
+            geo = geometry_bench(min(args.steps, 1000), 5, max(1, min(16, os.cpu_count() or 1)),
+                                 not args.no_cpu)
+        except RuntimeError as e:
+            geo = {"error": str(e)}
+        print("# pillar array: " + json.dumps(geo), file=sys.stderr, flush=True)
     mdf = None
     if not args.no_mdf:
         try:
@@ -591,6 +655,7 @@ def main():
         "mdf_learned_vs_plain": mdf,
         "batched_ensemble": ens,
         "writers": outb,
+        "pillar_array": geo,
         "kernel_ms": {k: round(v, 5) for k, v in sorted(kt.items())},
         "lbm_launch": tune.strip().splitlines()[-1],
         "mass": stats[0],

@@ -328,6 +328,27 @@ static int dmalloc(T** p, size_t bytes) {
     return SF_OK;
 }
 
+// Per-cell mask for the collide-stream with generic flags: bit q when the
+// neighbour in direction q, both axes wrapped, is SOLID; the cell's own type
+// in bits 9-11.
+static int build_nbm(Ctx& c) {
+    if (!c.generic_flags) return SF_OK;
+    const int nx = c.nx, ny = c.ny;
+    std::vector<uint16_t> m(static_cast<size_t>(c.N));
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            unsigned v = static_cast<unsigned>(c.h_flags[static_cast<size_t>(y) * nx + x]) << 9;
+            for (int q = 0; q < 9; ++q) {
+                const int xw = (x + dcx(q) + nx) % nx, yw = (y + dcy(q) + ny) % ny;
+                if (c.h_flags[static_cast<size_t>(yw) * nx + xw] == SF_SOLID) v |= 1u << q;
+            }
+            m[static_cast<size_t>(y) * nx + x] = static_cast<uint16_t>(v);
+        }
+    if (!c.nbm) SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.nbm), m.size() * sizeof(uint16_t)));
+    SF_HIP(hipMemcpy(c.nbm, m.data(), m.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    return SF_OK;
+}
+
 // InterpolatedBounceBack::initialize (lbm/interpolated_bounce_back.cpp:7-53):
 // a link for every direction from a non-SOLID cell into an in-bounds SOLID
 // cell, its wall distance the smallest over the obstacles of a 20-step
@@ -537,6 +558,7 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     }
     c.h_flags = fl;
     SF_HIP(hipMemcpy(c.flags, fl.data(), N, hipMemcpyHostToDevice));
+    if (build_nbm(c) != SF_OK) { sf_destroy(h); return SF_E_HIP; }
     for (int b = 0; b < 3; ++b) SF_HIP(hipMemset(c.F[b], 0, 2 * N * es));
     SF_HIP(hipMemset(c.counters, 0, 64 * sizeof(int)));
     SF_HIP(hipMemset(c.mdf, 0, sizeof(MdfDev)));
@@ -590,7 +612,7 @@ int sf_destroy(sf_ctx* h) {
     void* ptrs[] = {c.flags, c.f[0], c.f[1], c.F[0], c.F[1], c.F[2], c.stored_rho, c.stored_u,
                     c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_act, c.tile_rec, c.tile_dirty, c.cell_cnt, c.cell_list,
                     c.cell_pay, c.cell_own, c.mdf, c.ds_buf, c.obs, c.obs_v, c.ibb_cell, c.ibb_off,
-                    c.ibb_q, c.ibb_up, c.ibb_qf, c.snap_dev, c.probe_cells};
+                    c.ibb_q, c.ibb_up, c.ibb_qf, c.snap_dev, c.probe_cells, c.nbm};
     delete c.snap_sync;     // the stream has drained: no host function is pending
     c.snap_sync = nullptr;
     for (void* p : ptrs) if (p) hipFree(p);
@@ -633,6 +655,7 @@ int sf_set_flags(sf_ctx* h, const int32_t* flags) {
     c.generic_flags = !channel;
     c.inlet_outlet = io;
     invalidate_graph(c);
+    if (build_nbm(c) != SF_OK) return SF_E_HIP;
     if (c.p.use_interpolated_bb) return build_ibb_links(c);
     return SF_OK;
 }
@@ -682,8 +705,10 @@ int sf_set_obstacles(sf_ctx* h, int count, const int32_t* kinds, const int32_t* 
             SF_HIP(hipMemcpy(c.obs_v, verts.data(), verts.size() * sizeof(double), hipMemcpyHostToDevice));
         }
         std::vector<Obstacle> dev = obs;
-        for (Obstacle& o : dev)
+        for (Obstacle& o : dev) {
+            obstacle_bound(o, o.kind == OBS_POLYGON ? verts.data() + static_cast<size_t>(o.p[0]) : nullptr);
             if (o.kind == OBS_POLYGON) o.v = c.obs_v + static_cast<size_t>(o.p[0]);
+        }
         SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.obs), dev.size() * sizeof(Obstacle)));
         SF_HIP(hipMemcpy(c.obs, dev.data(), dev.size() * sizeof(Obstacle), hipMemcpyHostToDevice));
         c.nobs = count;

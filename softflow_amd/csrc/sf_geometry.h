@@ -22,12 +22,47 @@ struct Obstacle {
     int kind, nv;
     double p[4];
     const double* v;   // polygon vertices, (x, y) pairs
+    double bc[3];      // a disc (x, y, radius) holding the whole obstacle
 };
 
 struct ONearest { double x, y; };
 
 SF_HD inline double omin(double a, double b) { return b < a ? b : a; }   // std::min
 SF_HD inline double omax(double a, double b) { return a < b ? b : a; }   // std::max
+
+// A node farther than r_cut from the disc that holds the obstacle is farther
+// than r_cut from the obstacle: its contact term is zero and is skipped (the
+// margin covers the rounding of the exact distance).
+SF_HD inline bool obstacle_far(const Obstacle& o, double px, double py, double rcut) {
+    const double dx = px - o.bc[0], dy = py - o.bc[1];
+    const double lim = o.bc[2] + rcut;
+    return dx * dx + dy * dy > lim * lim * (1.0 + 1e-9);
+}
+
+// The holding disc of an obstacle, a little larger than the tightest one.
+inline void obstacle_bound(Obstacle& o, const double* verts) {
+    double cx, cy, r;
+    if (o.kind == OBS_CIRCLE) {
+        cx = o.p[0]; cy = o.p[1]; r = o.p[2];
+    } else if (o.kind == OBS_RECT) {
+        cx = 0.5 * (o.p[0] + o.p[2]); cy = 0.5 * (o.p[1] + o.p[3]);
+        r = 0.5 * std::sqrt((o.p[2] - o.p[0]) * (o.p[2] - o.p[0]) + (o.p[3] - o.p[1]) * (o.p[3] - o.p[1]));
+    } else {
+        double x0 = verts[0], x1 = verts[0], y0 = verts[1], y1 = verts[1];
+        for (int i = 1; i < o.nv; ++i) {
+            x0 = omin(x0, verts[2 * i]); x1 = omax(x1, verts[2 * i]);
+            y0 = omin(y0, verts[2 * i + 1]); y1 = omax(y1, verts[2 * i + 1]);
+        }
+        cx = 0.5 * (x0 + x1); cy = 0.5 * (y0 + y1);
+        r = 0.0;
+        for (int i = 0; i < o.nv; ++i) {
+            const double dx = verts[2 * i] - cx, dy = verts[2 * i + 1] - cy;
+            r = omax(r, std::sqrt(dx * dx + dy * dy));
+        }
+    }
+    o.bc[0] = cx; o.bc[1] = cy;
+    o.bc[2] = std::fabs(r) * (1.0 + 1e-12) + 1e-12;
+}
 
 // circle_obstacle.h:11-38
 SF_HD inline bool circle_contains(const Obstacle& o, double x, double y) {

@@ -121,6 +121,7 @@ struct Ctx {
 
     // lattice
     uint8_t* flags = nullptr;            // N
+    uint16_t* nbm = nullptr;             // N: solid-neighbour bits and own type (generic flags)
     void* f[2] = {nullptr, nullptr};     // 9*N populations (double, or float f-w)
     int fcur = 0;
     void* F[3] = {nullptr, nullptr, nullptr};   // raw spread force, (x,y) pairs

@@ -234,6 +234,9 @@ struct LbmArgs {
     const void* Fcur;        // raw spread force of this step
     const void* Fshift;      // raw force feeding the collision velocity
     const uint8_t* flags;
+    // with generic flags: per cell, bit q set when the neighbour in direction q
+    // (both axes wrapped) is SOLID, and the cell's own type in bits 9-11
+    const uint16_t* nbm;
     const double* srho;      // stored macro (SHIFT_STORED)
     const double* su;
     int* counters;
@@ -283,6 +286,13 @@ __device__ __forceinline__ bool is_solid(const LbmArgs& a, int x, int y, bool us
     return (a.walls_y && (y == 0 || y == a.ny - 1)) || (a.walls_x && (x == 0 || x == a.nx - 1));
 }
 
+// The collide-stream's view of the cell types: one 16-bit mask per cell
+// instead of a flag load per neighbour.  A neighbour is looked up only where
+// the push consults it (in bounds, or wrapped by a periodic axis), where the
+// wrapped position is the neighbour's own.
+__device__ __forceinline__ int mask_type(unsigned m) { return static_cast<int>((m >> 9) & 7u); }
+__device__ __forceinline__ bool mask_solid_nb(unsigned m, int q) { return (m >> q) & 1u; }
+
 template <typename T>
 __device__ __forceinline__ void load_force(const void* F, int n, T& fx, T& fy) {
     const T* p = static_cast<const T*>(F) + 2 * static_cast<size_t>(n);
@@ -314,7 +324,8 @@ __device__ __forceinline__ int collision_force(const LbmArgs& a, const void* F, 
 // BGK with first-order equilibrium from C and the post-LBM velocity, then a
 // push with half-way bounce-back; x is always periodic for the scalar.
 template <bool FP32, bool FLAGS>
-__device__ __forceinline__ void adr_cell(const LbmArgs& a, int x, int y, int n, double ux, double uy) {
+__device__ __forceinline__ void adr_cell(const LbmArgs& a, int x, int y, int n, double ux, double uy,
+                                         unsigned m) {
     using T = typename std::conditional<FP32, float, double>::type;
     const size_t N = static_cast<size_t>(a.N);
     for (int s = 0; s < a.nsp; ++s) {
@@ -334,7 +345,7 @@ __device__ __forceinline__ void adr_cell(const LbmArgs& a, int x, int y, int n, 
         }
         const double oms = a.oms[s];
         // INLET cells relax towards the inlet concentration (advection_diffusion.cpp:111-150)
-        const bool inlet = FLAGS && a.flags[n] == SF_INLET;
+        const bool inlet = FLAGS && mask_type(m) == SF_INLET;
         if (inlet) C = a.cin[s];
         if (!FP32 || inlet) {
 #pragma unroll
@@ -365,11 +376,12 @@ __device__ __forceinline__ void adr_cell(const LbmArgs& a, int x, int y, int n, 
             xd = xd < 0 ? xd + a.nx : (xd >= a.nx ? xd - a.nx : xd);
             int yd = y + dcy(q);
             bool bounce;
+            const bool nb_solid = FLAGS ? mask_solid_nb(m, q) : false;
             if (a.scalar_periodic_y) {
                 yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
-                bounce = is_solid(a, xd, yd, FLAGS);
+                bounce = FLAGS ? nb_solid : is_solid(a, xd, yd, false);
             } else {
-                bounce = (yd < 0 || yd >= a.ny) || is_solid(a, xd, yd, FLAGS);
+                bounce = (yd < 0 || yd >= a.ny) || (FLAGS ? nb_solid : is_solid(a, xd, yd, false));
             }
             if (bounce) st_nt(gd + dopp(q) * N + n, g[q]);
             else st_nt(gd + q * N + (yd * a.nx + xd), g[q]);
@@ -382,9 +394,11 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
     int capped = 0;
     if (x < a.nx && y < a.ny) {
         const int n = y * a.nx + x;
-        const bool solid = is_solid(a, x, y, FLAGS);
+        unsigned m = 0;
+        if constexpr (FLAGS) m = a.nbm[n];
+        const bool solid = FLAGS ? mask_type(m) == SF_SOLID : is_solid(a, x, y, false);
         const bool fluid = !solid;
-        const bool collide = FLAGS ? is_fluid_type(a, x, y, true) : fluid;
+        const bool collide = FLAGS ? mask_type(m) == SF_FLUID : fluid;
         using T = typename std::conditional<FP32, float, double>::type;
         T Fx, Fy;
         capped = collision_force<T>(a, a.Fcur, n, collide, Fx, Fy);
@@ -421,7 +435,7 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                     uy = 0.0;
                 }
                 rho_wall = r;
-                if constexpr (ADRF) adr_cell<false, FLAGS>(a, x, y, n, ux, uy);
+                if constexpr (ADRF) adr_cell<false, FLAGS>(a, x, y, n, ux, uy, m);
                 if (!collide) {
                 } else if (OP == SF_REGULARIZED) col_reg(f, a.om, r, ux, uy, Fx, Fy);
                 else if (OP == SF_MRT) col_mrt(f, a.om, a.se, a.seps, a.sq, r, ux, uy, Fx, Fy);
@@ -445,7 +459,7 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                     uy = (jy + 0.5f * sy) * inv;
                 }
                 rho_wall = static_cast<double>(rho);
-                if constexpr (ADRF) adr_cell<true, FLAGS>(a, x, y, n, static_cast<double>(ux), static_cast<double>(uy));
+                if constexpr (ADRF) adr_cell<true, FLAGS>(a, x, y, n, static_cast<double>(ux), static_cast<double>(uy), m);
                 const float om = static_cast<float>(a.om);
                 if (!collide) {
                 } else if (OP == SF_REGULARIZED) dcol_reg(f, om, drho, rho, ux, uy, Fx, Fy);
@@ -464,7 +478,7 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                 if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
                 else oob |= (yd < 0 || yd >= a.ny);
                 if (!oob) {
-                    if (is_solid(a, xd, yd, FLAGS)) {
+                    if (FLAGS ? mask_solid_nb(m, q) : is_solid(a, xd, yd, false)) {
                         T v = f[q];
                         if (a.moving && (yd == 0 || yd == a.ny - 1)) {
                             // a moving wall row (moving_wall.cpp:20-38): minus
@@ -558,6 +572,7 @@ static LbmArgs lbm_args(Ctx& c, int shift, bool fuse_adr) {
     a.Fcur = c.F[c.Fcur];
     a.Fshift = shift == SHIFT_PARTIAL ? c.F[c.Fpart] : c.F[c.Fprev];
     a.flags = c.flags;
+    a.nbm = c.nbm;
     a.srho = c.stored_rho;
     a.su = c.stored_u;
     a.counters = c.counters;
@@ -819,12 +834,13 @@ __global__ __launch_bounds__(256) void k_adr(LbmArgs a, const void* f) {
     double r, ux, uy;
     cell_macro<FP32>(f, static_cast<size_t>(a.N), n, a.Fcur, a, false, is_fluid_type(a, x, y, FLAGS), r,
                      ux, uy);
-    adr_cell<FP32, FLAGS>(a, x, y, n, ux, uy);
+    adr_cell<FP32, FLAGS>(a, x, y, n, ux, uy, FLAGS ? static_cast<unsigned>(a.nbm[n]) : 0u);
 }
 
 static LbmArgs view_args(Ctx& c) {
     LbmArgs a{};
     a.flags = c.flags;
+    a.nbm = c.nbm;
     a.nx = c.nx; a.ny = c.ny; a.N = c.N;
     a.periodic_x = c.periodic_x; a.periodic_y = c.periodic_y;
     a.walls_x = c.walls_x; a.walls_y = c.walls_y;

@@ -326,8 +326,11 @@ struct RepArgs {
 
 __device__ __forceinline__ void obstacle_forces(const RepArgs& a, double px, double py, double vx,
                                                 double vy, double& fx, double& fy) {
-    for (int o = 0; o < a.nobs; ++o)
-        obstacle_contact(a.obs[o], px, py, vx, vy, a.eps, a.sig, a.pw, a.rcut, a.gn, a.mu, fx, fy);
+    for (int o = 0; o < a.nobs; ++o) {
+        const Obstacle& ob = a.obs[o];
+        if (obstacle_far(ob, px, py, a.rcut)) continue;
+        obstacle_contact(ob, px, py, vx, vy, a.eps, a.sig, a.pw, a.rcut, a.gn, a.mu, fx, fy);
+    }
 }
 
 __device__ __forceinline__ double pair_dist2(const RepArgs& a, double px, double py, double bx,
@@ -503,16 +506,41 @@ __global__ void k_repulsion(RepArgs a) {
 constexpr int REP_NODES = 16;      // nodes per 256-thread workgroup
 constexpr int REP_SLOTS = 6;       // non-zero pair forces a lane keeps
 
+// Obstacles: the 16 lanes of a node screen the obstacles' holding discs (staged
+// in LDS) in parallel and mark the ones within reach; the node's lane 0 then
+// adds their terms in list order.  Up to REP_OBS_LDS obstacles; beyond that
+// lane 0 screens them all itself.
+constexpr int REP_OBS_LDS = 256;
+
 __device__ __forceinline__ void repulsion_lanes_body(RepArgs a, int bx) {
     __shared__ double sfx[REP_NODES][11][REP_SLOTS], sfy[REP_NODES][11][REP_SLOTS];
     __shared__ int scnt[REP_NODES][16];
     __shared__ float sdmin[REP_NODES][16];
+    __shared__ double sbc[REP_OBS_LDS][3];
+    __shared__ unsigned long long snear[REP_NODES][REP_OBS_LDS / 64];
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     const int k = bx * REP_NODES + g;
     const bool live = k < a.K;
     int cap = 0;
     double px = 0.0, py = 0.0, vx = 0.0, vy = 0.0;
     if (live) { cap = a.owner[k]; px = a.x[k]; py = a.y[k]; vx = a.vx[k]; vy = a.vy[k]; }
+    const bool obs_lds = a.nobs > 0 && a.nobs <= REP_OBS_LDS;
+    if (obs_lds) {
+        for (int i = threadIdx.x; i < a.nobs; i += blockDim.x) {
+            sbc[i][0] = a.obs[i].bc[0];
+            sbc[i][1] = a.obs[i].bc[1];
+            sbc[i][2] = a.obs[i].bc[2];
+        }
+        if (l < REP_OBS_LDS / 64) snear[g][l] = 0ull;
+        __syncthreads();
+        if (live)
+            for (int o = l; o < a.nobs; o += 16) {
+                const double dx = px - sbc[o][0], dy = py - sbc[o][1];
+                const double lim = sbc[o][2] + a.rcut;
+                if (!(dx * dx + dy * dy > lim * lim * (1.0 + 1e-9)))   // obstacle_far, negated
+                    atomicOr(&snear[g][o >> 6], 1ull << (o & 63));
+            }
+    }
     int nz = 0;
     double d2min = 4.0 * a.rcut * a.rcut;
     double Fx, Fy;
@@ -584,7 +612,18 @@ __device__ __forceinline__ void repulsion_lanes_body(RepArgs a, int bx) {
         pair_force(a, px, py, vx, vy, px, a.ytop, 0.0, 0.0, Fx, Fy);
         fx += Fx; fy += Fy;
     }
-    obstacle_forces(a, px, py, vx, vy, fx, fy);
+    if (obs_lds) {
+        for (int w = 0; w < REP_OBS_LDS / 64; ++w) {
+            unsigned long long bits = snear[g][w];
+            while (bits) {
+                const int o = w * 64 + __ffsll(static_cast<long long>(bits)) - 1;
+                bits &= bits - 1ull;
+                obstacle_contact(a.obs[o], px, py, vx, vy, a.eps, a.sig, a.pw, a.rcut, a.gn, a.mu, fx, fy);
+            }
+        }
+    } else {
+        obstacle_forces(a, px, py, vx, vy, fx, fy);
+    }
     a.fx[k] = fx;
     a.fy[k] = fy;
     if (a.dmin) {
