@@ -262,7 +262,9 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
                                                 const void* Fin, void* Fout, uint8_t* dirty_out,
                                                 int nx, int ny, int ntx, int ntiles, int mode,
                                                 int last, const MdfDev* md, int bx, int gx) {
-    __shared__ double tab[1025];
+    // the delta table is read through the caches: staged in LDS it held the
+    // spread to 7 workgroups per CU, and each tile is short, latency-bound work
+    const double* tab = lut;
     __shared__ double spx[STAGE_MAX], spy[STAGE_MAX], sfx[STAGE_MAX], sfy[STAGE_MAX], sds[STAGE_MAX];
     __shared__ int si0[STAGE_MAX], sj0[STAGE_MAX], skey[STAGE_MAX];
     __shared__ int ccnt[TILE_X * TILE_Y];
@@ -279,10 +281,9 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
             for (int i = bx * blockDim.x + t; i < nact; i += gx * blockDim.x) tile_cnt[act[i].x] = 0;
         return;
     }
-    // workgroups past the touched-tile count have nothing to do: they leave
-    // before loading the table (the grid is sized for every tile)
+    // workgroups past the touched-tile count have nothing to do (the grid is
+    // sized for every tile)
     if (bx >= nact) return;
-    for (int i = t; i < 1025; i += 256) tab[i] = lut[i];
     for (int ai = bx; ai < nact; ai += gx) {
         const int2 te = act[ai];
         const int tile = te.x;
