@@ -592,6 +592,17 @@ def main():
 
     stats = sim.fieldStats()
     del sim
+    # the achievable streaming rate on this device, beside the 8 TB/s peak
+    copy_ref = None
+    try:
+        from softflow_amd import capi
+        gbs = capi.copy_ceiling(1 << 30, 10)
+        copy_ref = {"value": round(gbs, 1), "unit": "GB/s",
+                    "kernel": "nontemporal 16-byte streaming copy, 1 GiB each way (sf_copy_ceiling)",
+                    "collide_stream_frac": round(roof["achieved"] / gbs, 4) if roof and gbs > 0 else None}
+    except Exception as e:   # reported, so the main line is still printed
+        copy_ref = {"value": None, "error": str(e)}
+    print("# copy ceiling: " + json.dumps(copy_ref), file=sys.stderr, flush=True)
     outb = None
     if not args.no_output:
         try:
@@ -651,6 +662,7 @@ def main():
                                "Skalak+Helfrich capsules, ADR D=0.05; FP32 lattice storage",
                    "nx": NX, "ny": NY, "capsules": 2000, "membrane_nodes": 800 * 38 + 1200 * 19},
         "roofline": roof,
+        "copy_ceiling": copy_ref,
         "cpu_baseline": cpu,
         "mdf_learned_vs_plain": mdf,
         "batched_ensemble": ens,

@@ -224,6 +224,12 @@ int  sf_autotune(sf_ctx* ctx, int reps, char* report, int report_len);
 /* Fix the collide-stream launch shape instead (bx*by a multiple of 64, <= 256);
  * persist > 0 launches 8 * persist workgroups that walk the XCD bands. */
 int  sf_set_lbm_launch(sf_ctx* ctx, int bx, int by, int xcd_banded, int persist);
+/* Two cells per lane in the collide-stream (no counterpart in the reference):
+ * on = 1 (the default) uses the paired kernel wherever it applies -- FP32, x
+ * periodic without x walls, no generic cell types, no moving walls, nx even --
+ * and one cell per lane elsewhere; on = 0 always uses one cell per lane.
+ * Results are identical either way; sf_autotune times both. */
+int  sf_set_lbm_pairs(sf_ctx* ctx, int on);
 
 /* ---- Multi-direct forcing control and the learned warm start ------------
  * (no counterpart in the reference; its MDF is a fixed iteration count,
@@ -302,6 +308,13 @@ int  sf_batch_info(sf_batch* b, int64_t* out4);
 /* Device math probe used by the tests: op 0 sqrt, 1 divide (a/b), 2 atan2(a,b),
  * 3 pow(a, b) with integer b as used by the contact force. */
 int  sf_probe_math(int op, int n, const double* a, const double* b, double* out);
+
+/* Achievable HBM bandwidth on this device (no counterpart in the reference):
+ * copies `bytes` between two device buffers it allocates, 16 bytes per thread
+ * with nontemporal loads and stores, reps times after one warm launch, and
+ * writes (bytes read + bytes written) / time in GB/s to *gbps.  bytes: a
+ * multiple of 16 in [1 MiB, 8 GiB]. */
+int  sf_copy_ceiling(int device, uint64_t bytes, int reps, double* gbps);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,7 @@ EXPORTS = (
     "sf_set_scalar", "sf_get_scalar", "sf_step", "sf_set_graph_mode", "sf_synchronize",
     "sf_stage", "sf_last_capped", "sf_field_stats", "sf_global_stats", "sf_step_count", "sf_kernel_times",
     "sf_host_alloc", "sf_host_free", "sf_set_probes", "sf_snapshot_async", "sf_snapshot_wait",
-    "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_probe_math",
+    "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_set_lbm_pairs", "sf_probe_math", "sf_copy_ceiling",
     "sf_mdf_set_control", "sf_mdf_stats", "sf_mdf_collect", "sf_mdf_samples", "sf_mdf_set_model",
     "sf_mlp_create", "sf_mlp_destroy", "sf_mlp_num_params", "sf_mlp_get_params", "sf_mlp_set_params",
     "sf_mlp_predict", "sf_mlp_loss_grad", "sf_mlp_train",
@@ -120,7 +120,9 @@ def load():
         "sf_set_timing": [C.c_void_p, C.c_int],
         "sf_autotune": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],
         "sf_set_lbm_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int],
+        "sf_set_lbm_pairs": [C.c_void_p, C.c_int],
         "sf_probe_math": [C.c_int, C.c_int, _DP, _DP, _DP],
+        "sf_copy_ceiling": [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double)],
         "sf_mdf_set_control": [C.c_void_p, C.c_double, C.c_int, C.c_int],
         "sf_mdf_stats": [C.c_void_p, _DP],
         "sf_mdf_collect": [C.c_void_p, C.c_int64],
@@ -384,3 +386,10 @@ def probe_math(op: int, a, b=None):
     out = np.empty_like(a)
     _check(load().sf_probe_math(op, a.size, _dp(a), _dp(bb), _dp(out)), "sf_probe_math")
     return out
+
+
+def copy_ceiling(nbytes: int = 1 << 30, reps: int = 10, device: int = 0) -> float:
+    """Achievable HBM bandwidth (GB/s) of a nontemporal 16-byte streaming copy."""
+    out = C.c_double()
+    _check(load().sf_copy_ceiling(device, nbytes, reps, C.byref(out)), "sf_copy_ceiling")
+    return out.value

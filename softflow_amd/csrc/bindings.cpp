@@ -477,6 +477,7 @@ PYBIND11_MODULE(softflow_core, m) {
         }, py::arg("callback"))
         .def("setTiming", &Simulation::setTiming, py::arg("on"))
         .def("autotune", &Simulation::autotune, py::arg("reps") = 10)
+        .def("setLbmPairs", &Simulation::setLbmPairs, py::arg("on"))
         .def("setLbmLaunch", &Simulation::setLbmLaunch, py::arg("bx"), py::arg("by"), py::arg("xcd_banded") = true,
              py::arg("persist") = 0)
         .def("kernelTimes", &Simulation::kernelTimes)

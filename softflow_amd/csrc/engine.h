@@ -429,6 +429,7 @@ public:
     void setTiming(bool on);
     std::string autotune(int reps = 10);
     void setLbmLaunch(int bx, int by, bool xcd_banded, int persist = 0);
+    void setLbmPairs(bool on);
 
     // multi-direct forcing control and the learned warm start (extensions)
     void setMdfControl(Real tol, int max_corrections, bool probe);

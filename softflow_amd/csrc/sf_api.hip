@@ -52,6 +52,14 @@ __global__ void k_csum(const T* g, int N, double* C) {
     C[n] = s;
 }
 
+// Streaming copy for the achievable-bandwidth reference: 16 bytes per thread,
+// nontemporal both ways, as the collide-stream moves its populations.
+typedef float copy_v4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy_nt(const copy_v4* a, copy_v4* b, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+}
+
 __global__ void k_probe(int op, int n, const double* a, const double* b, double* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1293,40 +1301,47 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
     SF_HIP(hipEventCreate(&e1));
     std::string rep;
     double best = 1e30;
-    int bbx = c.lbm_bx, bby = c.lbm_by, brm = c.lbm_remap, bps = c.lbm_persist;
+    int bbx = c.lbm_bx, bby = c.lbm_by, brm = c.lbm_remap, bps = c.lbm_persist, bpr = c.lbm_pair;
+    const int pair_max = lbm_pair_ok(c) ? 1 : 0;
     SF_HIP(hipGetLastError());
     for (const auto& sh : shapes) {
         if (sh[0] * sh[1] > 256) continue;
         for (int remap = sh[2] ? 1 : 0; remap <= 1; ++remap) {
-            launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap, sh[2]);   // warm
-            SF_HIP(hipEventRecord(e0, c.stream));
-            for (int r = 0; r < reps; ++r) launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap, sh[2]);
-            SF_HIP(hipEventRecord(e1, c.stream));
-            SF_HIP(hipEventSynchronize(e1));
-            SF_HIP(hipGetLastError());
-            float ms = 0.f;
-            SF_HIP(hipEventElapsedTime(&ms, e0, e1));
-            const double per = ms / reps;
-            char line[96];
-            if (sh[2])
-                std::snprintf(line, sizeof(line), "%dx%d persistent %d/xcd %.4f ms\n", sh[0], sh[1], sh[2], per);
-            else
-                std::snprintf(line, sizeof(line), "%dx%d %s %.4f ms\n", sh[0], sh[1], remap ? "xcd-banded" : "linear", per);
-            rep += line;
-            if (per < best) { best = per; bbx = sh[0]; bby = sh[1]; brm = remap; bps = sh[2]; }
+            for (int pair = 0; pair <= pair_max; ++pair) {
+                launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap, sh[2], pair);   // warm
+                SF_HIP(hipEventRecord(e0, c.stream));
+                for (int r = 0; r < reps; ++r) launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap, sh[2], pair);
+                SF_HIP(hipEventRecord(e1, c.stream));
+                SF_HIP(hipEventSynchronize(e1));
+                SF_HIP(hipGetLastError());
+                float ms = 0.f;
+                SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+                const double per = ms / reps;
+                const char* cl = pair ? " 2 cells/lane" : "";
+                char line[112];
+                if (sh[2])
+                    std::snprintf(line, sizeof(line), "%dx%d persistent %d/xcd%s %.4f ms\n", sh[0], sh[1], sh[2],
+                                  cl, per);
+                else
+                    std::snprintf(line, sizeof(line), "%dx%d %s%s %.4f ms\n", sh[0], sh[1],
+                                  remap ? "xcd-banded" : "linear", cl, per);
+                rep += line;
+                if (per < best) { best = per; bbx = sh[0]; bby = sh[1]; brm = remap; bps = sh[2]; bpr = pair; }
+            }
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     SF_HIP(hipGetLastError());
     launch_fill_int(c, c.counters, 1, 0);
-    c.lbm_bx = bbx; c.lbm_by = bby; c.lbm_remap = brm; c.lbm_persist = bps;
+    c.lbm_bx = bbx; c.lbm_by = bby; c.lbm_remap = brm; c.lbm_persist = bps; c.lbm_pair = bpr;
     invalidate_graph(c);
-    char line[96];
+    char line[112];
+    const char* cl = (bpr && pair_max) ? " 2 cells/lane" : "";
     if (bps)
-        std::snprintf(line, sizeof(line), "chosen %dx%d persistent %d/xcd\n", bbx, bby, bps);
+        std::snprintf(line, sizeof(line), "chosen %dx%d persistent %d/xcd%s\n", bbx, bby, bps, cl);
     else
-        std::snprintf(line, sizeof(line), "chosen %dx%d %s\n", bbx, bby, brm ? "xcd-banded" : "linear");
+        std::snprintf(line, sizeof(line), "chosen %dx%d %s%s\n", bbx, bby, brm ? "xcd-banded" : "linear", cl);
     rep += line;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), report_len - 1);
@@ -1344,6 +1359,14 @@ int sf_set_lbm_launch(sf_ctx* h, int bx, int by, int xcd_banded, int persist) {
     Ctx& c = h->c;
     c.lbm_bx = bx; c.lbm_by = by; c.lbm_remap = (xcd_banded || persist) ? 1 : 0;
     c.lbm_persist = persist;
+    invalidate_graph(c);
+    return SF_OK;
+}
+
+int sf_set_lbm_pairs(sf_ctx* h, int on) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    c.lbm_pair = on ? 1 : 0;
     invalidate_graph(c);
     return SF_OK;
 }
@@ -1441,6 +1464,45 @@ int sf_probe_math(int op, int n, const double* a, const double* b, double* out) 
     if (e == hipSuccess) e = hipMemcpy(out, dout, n * 8, hipMemcpyDeviceToHost);
     hipFree(da); hipFree(db); hipFree(dout);
     SF_HIP(e);
+    return SF_OK;
+}
+
+int sf_copy_ceiling(int device, uint64_t bytes, int reps, double* gbps) {
+    if (!gbps) return fail(SF_E_ARG, "null pointer");
+    if (bytes < (1ull << 20) || bytes > (8ull << 30) || bytes % 16 != 0)
+        return fail(SF_E_ARG, "bytes must be a multiple of 16 in [1 MiB, 8 GiB]");
+    if (reps < 1 || reps > 1000) return fail(SF_E_ARG, "reps must be in 1..1000");
+    SF_HIP(hipSetDevice(device));
+    const int64_t n = static_cast<int64_t>(bytes / 16);
+    const int64_t nblk = (n + 255) / 256;
+    if (nblk > 0x7fffffffLL) return fail(SF_E_ARG, "copy too large for one grid");
+    copy_v4 *a = nullptr, *b = nullptr;
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&a), bytes));
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&b), bytes);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t st = nullptr;
+    if (e == hipSuccess) e = hipStreamCreate(&st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, bytes, st);
+    float ms = 0.f;
+    if (e == hipSuccess) {
+        const dim3 g(static_cast<unsigned>(nblk)), blk(256);
+        hipLaunchKernelGGL(k_copy_nt, g, blk, 0, st, a, b, n);   // warm
+        hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy_nt, g, blk, 0, st, a, b, n);
+        hipEventRecord(e1, st);
+        e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    hipFree(a);
+    if (b) hipFree(b);
+    SF_HIP(e);
+    *gbps = ms > 0.f ? 2.0 * static_cast<double>(bytes) * reps / (static_cast<double>(ms) * 1e-3) / 1e9 : 0.0;
     return SF_OK;
 }
 

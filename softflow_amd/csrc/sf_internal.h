@@ -247,6 +247,8 @@ struct Ctx {
 
     // collide-stream launch shape (chosen by sf_autotune)
     int lbm_bx = 64, lbm_by = 4, lbm_remap = 1, lbm_persist = 0;
+    // two cells per lane where the paired collide-stream applies (lbm_pair_ok)
+    int lbm_pair = 1;
 };
 
 // Error reporting shared by the C-ABI translation units: sets the message
@@ -263,7 +265,9 @@ int fail(int code, const std::string& msg);
 // -- launchers (defined in the .hip files) --------------------------------
 void launch_lbm(Ctx& c, int shift_mode, bool fuse_adr);
 // the same launch with an explicit shape and no change of buffer roles
-void launch_lbm_cfg(Ctx& c, int shift_mode, bool fuse_adr, int bx, int by, int remap, int persist);
+void launch_lbm_cfg(Ctx& c, int shift_mode, bool fuse_adr, int bx, int by, int remap, int persist,
+                    int pair);
+bool lbm_pair_ok(const Ctx& c);
 void launch_adr(Ctx& c, int Fidx);
 void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy);
 void launch_macro_cells(Ctx& c, const int* cells, int count, double* out);

@@ -529,16 +529,233 @@ __global__ __launch_bounds__(256) void k_lbm(LbmArgs a) {
         atomicAdd(&a.counters[0], capped);
 }
 
+// lanes_x: lanes across a row (nx, or nx / 2 for the paired collide-stream)
 static void set_geometry(const Ctx& c, LbmArgs& a, int bx, int by, int remap, int persist, dim3& g,
-                         dim3& b) {
+                         dim3& b, int lanes_x = 0) {
     b = dim3(bx, by);
-    a.nbx = (c.nx + bx - 1) / bx;
+    a.nbx = ((lanes_x > 0 ? lanes_x : c.nx) + bx - 1) / bx;
     const int nby = (c.ny + by - 1) / by;
     a.nblk = a.nbx * nby;
     a.per_xcd = (a.nblk + 7) / 8;
     a.remap = remap;
     a.persist = persist > 0 && persist < a.per_xcd ? persist : 0;
     g = dim3(a.persist ? 8 * a.persist : (remap ? 8 * a.per_xcd : a.nblk));
+}
+
+
+// ---------------------------------------------------------------------------
+// Paired collide-stream for FP32 channels: each lane takes the cells (x, x+1),
+// x even, of a lattice that is periodic in x with no x walls, no generic cell
+// types and no moving walls.  There every solid, bounce-back and out-of-bounds
+// decision depends on the row alone, so the two cells share it: their loads,
+// the unshifted pushes, the bounce-backs and the self-copies move as aligned
+// 8-byte pairs, and the pushes along x as 8-byte pairs one cell over (two
+// 4-byte stores where one of them wraps).  The arithmetic of each cell is lbm_cell's FP32 branch and
+// adr_cell's FP32 relaxation, expression for expression, so results are
+// bit-identical to one cell per lane.  With 4-byte lanes a wave moves half the
+// bytes per memory instruction, which holds a streaming kernel well below the
+// bandwidth the same traffic reaches with 8-byte lanes.
+typedef float pair_f2 __attribute__((ext_vector_type(2)));
+
+// the same pair at a 4-byte-aligned address (the pushes along x, one 8-byte
+// store where neither cell's push wraps; the code object runs in the
+// runtime's unaligned access mode)
+typedef float pair_f2u __attribute__((ext_vector_type(2), aligned(4)));
+__device__ __forceinline__ void st_pair_u(float* p, float v0, float v1) {
+    pair_f2u v;
+    v.x = v0;
+    v.y = v1;
+    __builtin_nontemporal_store(v, reinterpret_cast<pair_f2u*>(p));
+}
+
+__device__ __forceinline__ pair_f2 ld_pair(const float* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const pair_f2*>(p));
+}
+__device__ __forceinline__ void st_pair(float* p, float v0, float v1) {
+    pair_f2 v;
+    v.x = v0;
+    v.y = v1;
+    __builtin_nontemporal_store(v, reinterpret_cast<pair_f2*>(p));
+}
+
+// lbm_cell's FP32 macroscopic state and collision of one cell
+template <int OP>
+__device__ __forceinline__ void pair_collide(const LbmArgs& a, float (&f)[9], int n, bool collide, float Fx,
+                                             float Fy, float& ux, float& uy) {
+    float drho = ((f[0] + (f[1] + f[3])) + (f[2] + f[4])) + ((f[5] + f[7]) + (f[6] + f[8]));
+    float jx = (f[1] - f[3]) + (f[5] - f[6]) + (f[8] - f[7]);
+    float jy = (f[2] - f[4]) + (f[5] + f[6]) - (f[7] + f[8]);
+    float sx = 0.f, sy = 0.f;
+    if (a.shift == SHIFT_PREV) collision_force<float>(a, a.Fshift, n, collide, sx, sy);
+    else if (a.shift == SHIFT_PARTIAL) load_force<float>(a.Fshift, n, sx, sy);
+    float rho = 1.0f + drho;
+    if (a.shift == SHIFT_STORED) {
+        rho = static_cast<float>(a.srho[n]);
+        drho = static_cast<float>(a.srho[n] - 1.0);
+        ux = static_cast<float>(a.su[2 * n]);
+        uy = static_cast<float>(a.su[2 * n + 1]);
+    } else {
+        const float inv = 1.0f / rho;
+        ux = (jx + 0.5f * sx) * inv;
+        uy = (jy + 0.5f * sy) * inv;
+    }
+    const float om = static_cast<float>(a.om);
+    if (!collide) {
+    } else if (OP == SF_REGULARIZED) dcol_reg(f, om, drho, rho, ux, uy, Fx, Fy);
+    else if (OP == SF_MRT)
+        dcol_mrt(f, om, static_cast<float>(a.se), static_cast<float>(a.seps), static_cast<float>(a.sq), drho,
+                 rho, ux, uy, Fx, Fy);
+    else dcol_bgk(f, om, drho, rho, ux, uy, Fx, Fy);
+}
+
+// adr_cell's FP32 relaxation of one cell (no INLET cells on this path)
+__device__ __forceinline__ void pair_relax(const LbmArgs& a, int s, float (&g)[9], int n, double ux, double uy) {
+    double C;
+    if (a.from_store) {
+        C = a.Cst[s][n];
+    } else {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc += static_cast<double>(g[q]);
+        C = acc;
+    }
+    const float Cf = static_cast<float>(C), omf = static_cast<float>(a.oms[s]);
+    double moving = 0.0;
+#pragma unroll
+    for (int q = 1; q < 9; ++q) {
+        const float eu = static_cast<float>(dcx(q) * ux + dcy(q) * uy);
+        const float ge = static_cast<float>(dw(q)) * Cf * (1.0f + 3.0f * eu);
+        g[q] = g[q] - omf * (g[q] - ge);
+        moving += static_cast<double>(g[q]);
+    }
+    g[0] = static_cast<float>(C - moving);
+}
+
+__device__ __forceinline__ int wrap_x(int x, int nx) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); }
+
+template <int OP, bool ADRF>
+__device__ __forceinline__ int lbm_pair(const LbmArgs& a, int x, int y) {
+    if (x >= a.nx || y >= a.ny) return 0;     // nx is even: x + 1 < nx as well
+    const int n = y * a.nx + x;
+    const bool fluid = !is_solid(a, x, y, false);
+    float Fx0, Fy0, Fx1, Fy1;
+    int capped = collision_force<float>(a, a.Fcur, n, fluid, Fx0, Fy0);
+    capped += collision_force<float>(a, a.Fcur, n + 1, fluid, Fx1, Fy1);
+    if (!fluid) return capped;
+    const float* src = static_cast<const float*>(a.fsrc);
+    float* dst = static_cast<float*>(a.fdst);
+    const size_t N = static_cast<size_t>(a.N);
+    float f0[9], f1[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const pair_f2 v = ld_pair(src + q * N + n);
+        f0[q] = v.x;
+        f1[q] = v.y;
+    }
+    float ux0, uy0, ux1, uy1;
+    pair_collide<OP>(a, f0, n, true, Fx0, Fy0, ux0, uy0);
+    pair_collide<OP>(a, f1, n + 1, true, Fx1, Fy1, ux1, uy1);
+    const int xp[2] = {x, x + 1};
+    if constexpr (ADRF) {
+        for (int s = 0; s < a.nsp; ++s) {
+            const float* gs = static_cast<const float*>(a.gsrc[s]);
+            float* gd = static_cast<float*>(a.gdst[s]);
+            float g0[9], g1[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const pair_f2 v = ld_pair(gs + q * N + n);
+                g0[q] = v.x;
+                g1[q] = v.y;
+            }
+            pair_relax(a, s, g0, n, static_cast<double>(ux0), static_cast<double>(uy0));
+            pair_relax(a, s, g1, n + 1, static_cast<double>(ux1), static_cast<double>(uy1));
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                int yd = y + dcy(q);
+                bool bounce;
+                if (a.scalar_periodic_y) {
+                    yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
+                    bounce = is_solid(a, 0, yd, false);
+                } else {
+                    bounce = (yd < 0 || yd >= a.ny) || is_solid(a, 0, yd, false);
+                }
+                if (bounce) {
+                    st_pair(gd + dopp(q) * N + n, g0[q], g1[q]);
+                } else if (dcx(q) == 0) {
+                    st_pair(gd + q * N + (static_cast<size_t>(yd) * a.nx + x), g0[q], g1[q]);
+                } else {
+                    const size_t row = static_cast<size_t>(yd) * a.nx;
+                    const int xa = xp[0] + dcx(q);
+                    if (xa >= 0 && xa + 1 < a.nx) {
+                        st_pair_u(gd + q * N + row + xa, g0[q], g1[q]);
+                    } else {
+                        st_nt(gd + q * N + row + wrap_x(xp[0] + dcx(q), a.nx), g0[q]);
+                        st_nt(gd + q * N + row + wrap_x(xp[1] + dcx(q), a.nx), g1[q]);
+                    }
+                }
+            }
+        }
+    }
+    // push with halfway bounce-back (lbm_solver.cpp:212-235, bounce_back.cpp:12-40)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        int yd = y + dcy(q);
+        bool oob = false;
+        if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
+        else oob = (yd < 0 || yd >= a.ny);
+        if (!oob) {
+            if (is_solid(a, 0, yd, false)) {
+                st_pair(dst + dopp(q) * N + n, f0[q], f1[q]);
+            } else if (dcx(q) == 0) {
+                st_pair(dst + q * N + (static_cast<size_t>(yd) * a.nx + x), f0[q], f1[q]);
+            } else {
+                const size_t row = static_cast<size_t>(yd) * a.nx;
+                const int xa = xp[0] + dcx(q);
+                if (xa >= 0 && xa + 1 < a.nx) {
+                    st_pair_u(dst + q * N + row + xa, f0[q], f1[q]);
+                } else {
+                    st_nt(dst + q * N + row + wrap_x(xp[0] + dcx(q), a.nx), f0[q]);
+                    st_nt(dst + q * N + row + wrap_x(xp[1] + dcx(q), a.nx), f1[q]);
+                }
+            }
+        }
+        if (q != 0) {
+            const int ys = y - dcy(q);
+            if (!a.periodic_y && (ys < 0 || ys >= a.ny)) st_pair(dst + q * N + n, f0[q], f1[q]);
+        }
+    }
+    return capped;
+}
+
+template <int OP, bool ADRF>
+__global__ __launch_bounds__(256) void k_lbm_pair(LbmArgs a) {
+    int capped = 0;
+    if (a.persist == 0) {
+        const int b = blockIdx.x;
+        const int t = a.remap ? (b % 8) * a.per_xcd + b / 8 : b;
+        if (t >= a.nblk) return;
+        const int x = 2 * ((t % a.nbx) * blockDim.x + threadIdx.x);
+        const int y = (t / a.nbx) * blockDim.y + threadIdx.y;
+        capped = lbm_pair<OP, ADRF>(a, x, y);
+    } else {
+        const int b = blockIdx.x, xcd = b % 8, l = b / 8;
+        const int t_end = min((xcd + 1) * a.per_xcd, a.nblk);
+        for (int t = xcd * a.per_xcd + l; t < t_end; t += a.persist) {
+            const int x = 2 * ((t % a.nbx) * blockDim.x + threadIdx.x);
+            const int y = (t / a.nbx) * blockDim.y + threadIdx.y;
+            capped += lbm_pair<OP, ADRF>(a, x, y);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) capped += __shfl_down(capped, off);
+    if ((threadIdx.x + threadIdx.y * blockDim.x) % warpSize == 0 && capped)
+        atomicAdd(&a.counters[0], capped);
+}
+
+// The paired form applies: FP32, x periodic without x walls, no generic cell
+// types, no moving walls, nx even.
+bool lbm_pair_ok(const Ctx& c) {
+    const bool moving = fabs(c.p.bottom_wall_velocity) > 1e-15 || fabs(c.p.top_wall_velocity) > 1e-15;
+    return c.fp32 && !c.generic_flags && c.periodic_x && !c.walls_x && !moving && c.nx % 2 == 0;
 }
 
 template <int OP, bool FP32>
@@ -594,11 +811,25 @@ static LbmArgs lbm_args(Ctx& c, int shift, bool fuse_adr) {
     return a;
 }
 
-void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap, int persist) {
+void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap, int persist, int pair) {
     LbmArgs a = lbm_args(c, shift, fuse_adr);
     dim3 b, g;
-    set_geometry(c, a, bx, by, remap, persist, g, b);
     const int op = c.p.collision_model;
+    if (pair && lbm_pair_ok(c)) {
+        set_geometry(c, a, bx, by, remap, persist, g, b, c.nx / 2);
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, b, 0, c.stream, a); };
+        if (fuse_adr) {
+            if (op == SF_REGULARIZED) go(k_lbm_pair<SF_REGULARIZED, true>);
+            else if (op == SF_MRT) go(k_lbm_pair<SF_MRT, true>);
+            else go(k_lbm_pair<SF_BGK, true>);
+        } else {
+            if (op == SF_REGULARIZED) go(k_lbm_pair<SF_REGULARIZED, false>);
+            else if (op == SF_MRT) go(k_lbm_pair<SF_MRT, false>);
+            else go(k_lbm_pair<SF_BGK, false>);
+        }
+        return;
+    }
+    set_geometry(c, a, bx, by, remap, persist, g, b);
     if (c.fp32) {
         if (op == SF_REGULARIZED) lbm_dispatch_flags<SF_REGULARIZED, true>(c, a, g, b, fuse_adr);
         else if (op == SF_MRT) lbm_dispatch_flags<SF_MRT, true>(c, a, g, b, fuse_adr);
@@ -716,7 +947,7 @@ void launch_ibb(Ctx& c) {
 }
 
 void launch_lbm(Ctx& c, int shift, bool fuse_adr) {
-    launch_lbm_cfg(c, shift, fuse_adr, c.lbm_bx, c.lbm_by, c.lbm_remap, c.lbm_persist);
+    launch_lbm_cfg(c, shift, fuse_adr, c.lbm_bx, c.lbm_by, c.lbm_remap, c.lbm_persist, c.lbm_pair);
     launch_ibb(c);
     if (c.inlet_outlet) {
         LbmArgs a = lbm_args(c, shift, false);

@@ -60,6 +60,12 @@ def test_argument_validation_before_device():
         assert not h.value
     assert L.sf_create(None, 0, None) == capi.SF_E_ARG
     assert L.sf_set_obstacles(None, 0, None, None, None) == capi.SF_E_ARG
+    g = C.c_double()
+    for nbytes, reps, out, msg in ((1 << 30, 10, None, "null"), (1000, 10, C.byref(g), "multiple of 16"),
+                                   ((1 << 20) + 8, 10, C.byref(g), "multiple of 16"),
+                                   (1 << 30, 0, C.byref(g), "reps")):
+        assert L.sf_copy_ceiling(0, nbytes, reps, out) == capi.SF_E_ARG
+        assert msg in L.sf_last_error().decode()
 
 
 def test_batch_argument_validation_before_device():

@@ -47,3 +47,11 @@ def test_contact_power_vs_glibc_pow():
     ulps = np.abs(got.view(np.int64) - ref.view(np.int64))
     assert ulps.max() <= 1
     assert np.count_nonzero(ulps) < 0.01 * x.size
+
+
+def test_copy_ceiling_is_a_plausible_hbm_rate():
+    # the streaming-copy reference beside the 8 TB/s peak: a real HBM rate,
+    # below the peak, and above what any cache-resident copy would hide
+    from softflow_amd import capi
+    g = capi.copy_ceiling(1 << 28, 5)
+    assert 1000.0 < g < 8000.0, g
