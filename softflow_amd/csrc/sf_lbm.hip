@@ -727,8 +727,11 @@ __device__ __forceinline__ int lbm_pair(const LbmArgs& a, int x, int y) {
     return capped;
 }
 
+// Four waves per SIMD: the MRT form with the scalar would otherwise take 137
+// VGPRs (three waves); at 128 it spills 32 bytes per lane and still runs
+// faster, the extra wave keeping more bytes in flight.
 template <int OP, bool ADRF>
-__global__ __launch_bounds__(256) void k_lbm_pair(LbmArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_lbm_pair(LbmArgs a) {
     int capped = 0;
     if (a.persist == 0) {
         const int b = blockIdx.x;
