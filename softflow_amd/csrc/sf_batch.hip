@@ -38,14 +38,14 @@ static void set_roles(Ctx& c, const Roles& r) {
 // The items of one steady step of member c, in one_step's launch order
 // (force-buffer rotation, membrane, contact, spread, collide-stream,
 // interpolation + move); advances c's buffer roles as one_step does.
-static void record_step(Ctx& c, int bx, int by, BatchItem* it) {
+static void record_step(Ctx& c, int bx, int by, int pair, BatchItem* it) {
     const int spare = 3 - c.Fcur - c.Fprev;
     c.Fprev = c.Fcur;
     c.Fcur = spare;
     batch_items_membrane(c, it + BK_MEMBRANE);
     batch_items_contact(c, it + BK_CELLS_FILL);
     batch_items_spread(c, it + BK_NODE_PREP);
-    batch_items_lbm(c, bx, by, it + BK_LBM);
+    batch_items_lbm(c, bx, by, pair, it + BK_LBM);
     batch_items_interp(c, it + BK_INTERP);
     c.shift_next = SHIFT_PREV;
     c.step_count++;
@@ -182,7 +182,7 @@ static int build_plan(sf_batch& B) {
     for (int s = 0; s < BATCH_PERIOD; ++s)
         for (int m = 0; m < M; ++m) {
             B.roles[s][m] = roles_of(*B.mem[m]);
-            record_step(*B.mem[m], B.v.bx, B.v.by, &items[(static_cast<size_t>(s) * M + m) * BK_COUNT]);
+            record_step(*B.mem[m], B.v.bx, B.v.by, B.v.pair, &items[(static_cast<size_t>(s) * M + m) * BK_COUNT]);
         }
     for (int m = 0; m < M; ++m) {
         B.roles[BATCH_PERIOD][m] = roles_of(*B.mem[m]);
@@ -428,11 +428,12 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
     SF_HIP(hipEventCreate(&e1));
     std::string rep;
     double best = 1e30;
-    int bbx = B.v.bx, bby = B.v.by, bband = B.banded;
+    int bbx = B.v.bx, bby = B.v.by, bband = B.banded, bpair = B.v.pair;
     BatchVariant v = B.v;
     v.fp32 = B.mem[0]->fp32 ? 1 : 0;
     v.op = B.mem[0]->p.collision_model;
-    for (const auto& sh : shapes) {
+    for (const auto& sh : shapes)
+    for (int pair = 0; pair <= 1; ++pair) {
         // the collide-stream of the members' current roles (the launch only
         // reads the source populations, so repeats leave the state as it is)
         std::vector<BatchItem> it(M);
@@ -440,7 +441,7 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
         for (int m = 0; m < M; ++m) {
             Ctx& c = *B.mem[m];
             const int f0 = c.fcur;
-            batch_items_lbm(c, sh[0], sh[1], &it[m]);
+            batch_items_lbm(c, sh[0], sh[1], pair, &it[m]);
             c.fcur = f0;
             blocks[m] = it[m].blocks;
         }
@@ -468,11 +469,11 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
             SF_HIP(hipEventElapsedTime(&ms, e0, e1));
             SF_HIP(hipFree(dm));
             const double per = ms / reps;
-            char line[96];
-            std::snprintf(line, sizeof(line), "%dx%d %s %.4f ms\n", sh[0], sh[1],
-                          band ? "xcd-banded" : "linear", per);
+            char line[112];
+            std::snprintf(line, sizeof(line), "%dx%d %s%s %.4f ms\n", sh[0], sh[1],
+                          band ? "xcd-banded" : "linear", pair ? " 2 cells/lane" : "", per);
             rep += line;
-            if (per < best) { best = per; bbx = sh[0]; bby = sh[1]; bband = band; }
+            if (per < best) { best = per; bbx = sh[0]; bby = sh[1]; bband = band; bpair = pair; }
         }
         SF_HIP(hipFree(dp));
     }
@@ -485,11 +486,13 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
     if (rc != SF_OK) return rc;
     SF_HIP(hipStreamSynchronize(B.stream));
     B.v.bx = bbx;
+    B.v.pair = bpair;
     B.v.by = bby;
     B.banded = bband;
     B.planned = false;
-    char line[96];
-    std::snprintf(line, sizeof(line), "chosen %dx%d %s\n", bbx, bby, bband ? "xcd-banded" : "linear");
+    char line[112];
+    std::snprintf(line, sizeof(line), "chosen %dx%d %s%s\n", bbx, bby, bband ? "xcd-banded" : "linear",
+                  bpair ? " 2 cells/lane" : "");
     rep += line;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), report_len - 1);

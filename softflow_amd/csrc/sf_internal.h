@@ -327,6 +327,7 @@ struct BatchVariant {
     int op = SF_BGK;
     int maxn = 64;          // membrane LDS capacity (64 or 256 nodes)
     int bx = 64, by = 4;    // collide-stream workgroup shape
+    int pair = 1;           // two cells per lane for the members where it applies
 };
 
 template <typename P>
@@ -342,7 +343,7 @@ inline void batch_put(BatchItem& it, int blocks, const P& p) {
 void batch_items_membrane(Ctx& c, BatchItem* it);          // BK_MEMBRANE
 void batch_items_contact(Ctx& c, BatchItem* it);           // BK_CELLS_FILL .. BK_CONTACT
 void batch_items_spread(Ctx& c, BatchItem* it);            // BK_NODE_PREP .. BK_SPREAD
-void batch_items_lbm(Ctx& c, int bx, int by, BatchItem* it);   // BK_LBM
+void batch_items_lbm(Ctx& c, int bx, int by, int pair, BatchItem* it);   // BK_LBM
 void batch_items_interp(Ctx& c, BatchItem* it);            // BK_INTERP
 // launch kind over nblocks workgroups of the map
 void batch_launch_membrane(int kind, const BatchVariant& v, const void* packs, const int2* map,

@@ -258,6 +258,8 @@ struct LbmArgs {
     // 8 * persist workgroups; the persist workgroups of an XCD walk its band
     // with stride persist.
     int nbx, nblk, per_xcd, remap, persist;
+    // batched collide-stream: this member takes two cells per lane (k_lbm_pair's body)
+    int pair;
     // moving walls (lbm/moving_wall.cpp): x velocities of the solid rows y = 0 and y = ny-1
     int moving;
     double wall_bot, wall_top;
@@ -974,9 +976,17 @@ __global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* 
     const int2 w = map[blockIdx.x];
     if (w.x < 0) return;
     const LbmArgs a = packs[w.x];
-    int x, y;
-    if (!block_origin(a, w.y, x, y)) return;
-    int capped = lbm_cell<OP, FP32, false, false>(a, x, y);
+    int capped;
+    if (FP32 && a.pair) {
+        if (w.y >= a.nblk) return;
+        const int x = 2 * ((w.y % a.nbx) * blockDim.x + threadIdx.x);
+        const int y = (w.y / a.nbx) * blockDim.y + threadIdx.y;
+        capped = lbm_pair<OP, false>(a, x, y);
+    } else {
+        int x, y;
+        if (!block_origin(a, w.y, x, y)) return;
+        capped = lbm_cell<OP, FP32, false, false>(a, x, y);
+    }
     for (int off = 32; off > 0; off >>= 1) capped += __shfl_down(capped, off);
     if ((threadIdx.x + threadIdx.y * blockDim.x) % warpSize == 0 && capped)
         atomicAdd(&a.counters[0], capped);
@@ -985,10 +995,11 @@ __global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* 
 // The steady step's launch (previous-step shift force, no scalar) with a
 // linear grid of bx x by workgroups; swaps the population buffers as
 // launch_lbm does.
-void batch_items_lbm(Ctx& c, int bx, int by, BatchItem* it) {
+void batch_items_lbm(Ctx& c, int bx, int by, int pair, BatchItem* it) {
     LbmArgs a = lbm_args(c, SHIFT_PREV, false);
     dim3 g, b;
-    set_geometry(c, a, bx, by, 0, 0, g, b);
+    a.pair = pair && lbm_pair_ok(c) ? 1 : 0;
+    set_geometry(c, a, bx, by, 0, 0, g, b, a.pair ? c.nx / 2 : 0);
     batch_put(it[0], a.nblk, a);
     c.fcur = 1 - c.fcur;
 }
