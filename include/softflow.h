@@ -224,11 +224,12 @@ int  sf_autotune(sf_ctx* ctx, int reps, char* report, int report_len);
 /* Fix the collide-stream launch shape instead (bx*by a multiple of 64, <= 256);
  * persist > 0 launches 8 * persist workgroups that walk the XCD bands. */
 int  sf_set_lbm_launch(sf_ctx* ctx, int bx, int by, int xcd_banded, int persist);
-/* Two cells per lane in the collide-stream (no counterpart in the reference):
- * on = 1 (the default) uses the paired kernel wherever it applies -- FP32, x
- * periodic without x walls, no generic cell types, no moving walls, nx even --
- * and one cell per lane elsewhere; on = 0 always uses one cell per lane.
- * Results are identical either way; sf_autotune times both. */
+/* Two cells per lane in the collide-stream (no counterpart in the reference).
+ * The paired kernel applies to FP32 lattices, x periodic without x walls, no
+ * moving walls, nx even.  By default it runs on channels and one cell per lane
+ * runs elsewhere; on = 1 also gives it generic cell types, on = 0 turns it
+ * off.  sf_autotune times both forms and keeps the faster.  Results are
+ * identical either way. */
 int  sf_set_lbm_pairs(sf_ctx* ctx, int on);
 
 /* ---- Multi-direct forcing control and the learned warm start ------------

@@ -1307,7 +1307,8 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
     for (const auto& sh : shapes) {
         if (sh[0] * sh[1] > 256) continue;
         for (int remap = sh[2] ? 1 : 0; remap <= 1; ++remap) {
-            for (int pair = 0; pair <= pair_max; ++pair) {
+            for (int pair_on = 0; pair_on <= pair_max; ++pair_on) {
+                const int pair = pair_on ? (c.generic_flags ? 2 : 1) : 0;
                 launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap, sh[2], pair);   // warm
                 SF_HIP(hipEventRecord(e0, c.stream));
                 for (int r = 0; r < reps; ++r) launch_lbm_cfg(c, shift, fuse, sh[0], sh[1], remap, sh[2], pair);
@@ -1366,7 +1367,7 @@ int sf_set_lbm_launch(sf_ctx* h, int bx, int by, int xcd_banded, int persist) {
 int sf_set_lbm_pairs(sf_ctx* h, int on) {
     if (!h) return fail(SF_E_ARG, "null pointer");
     Ctx& c = h->c;
-    c.lbm_pair = on ? 1 : 0;
+    c.lbm_pair = on ? 2 : 0;
     invalidate_graph(c);
     return SF_OK;
 }

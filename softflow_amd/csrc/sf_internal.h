@@ -247,7 +247,8 @@ struct Ctx {
 
     // collide-stream launch shape (chosen by sf_autotune)
     int lbm_bx = 64, lbm_by = 4, lbm_remap = 1, lbm_persist = 0;
-    // two cells per lane where the paired collide-stream applies (lbm_pair_ok)
+    // two cells per lane where the paired collide-stream applies (lbm_pair_ok):
+    // 0 never, 1 channels, 2 also generic cell types
     int lbm_pair = 1;
 };
 

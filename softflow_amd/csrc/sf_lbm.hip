@@ -610,8 +610,10 @@ __device__ __forceinline__ void pair_collide(const LbmArgs& a, float (&f)[9], in
     else dcol_bgk(f, om, drho, rho, ux, uy, Fx, Fy);
 }
 
-// adr_cell's FP32 relaxation of one cell (no INLET cells on this path)
-__device__ __forceinline__ void pair_relax(const LbmArgs& a, int s, float (&g)[9], int n, double ux, double uy) {
+// adr_cell's FP32 relaxation of one cell (INLET cells: its FP64 form towards
+// the inlet concentration)
+__device__ __forceinline__ void pair_relax(const LbmArgs& a, int s, float (&g)[9], int n, double ux, double uy,
+                                           bool inlet = false) {
     double C;
     if (a.from_store) {
         C = a.Cst[s][n];
@@ -620,6 +622,17 @@ __device__ __forceinline__ void pair_relax(const LbmArgs& a, int s, float (&g)[9
 #pragma unroll
         for (int q = 0; q < 9; ++q) acc += static_cast<double>(g[q]);
         C = acc;
+    }
+    if (inlet) {
+        C = a.cin[s];
+        const double oms = a.oms[s];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const double eu = static_cast<double>(dcx(q)) * ux + static_cast<double>(dcy(q)) * uy;
+            const double ge = dw(q) * C * (1.0 + eu * 3.0);
+            g[q] = static_cast<float>(static_cast<double>(g[q]) - oms * (static_cast<double>(g[q]) - ge));
+        }
+        return;
     }
     const float Cf = static_cast<float>(C), omf = static_cast<float>(a.oms[s]);
     double moving = 0.0;
@@ -635,10 +648,126 @@ __device__ __forceinline__ void pair_relax(const LbmArgs& a, int s, float (&g)[9
 
 __device__ __forceinline__ int wrap_x(int x, int nx) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); }
 
+// One cell's store of population q: the push to (x + cx, yd), or the bounce
+// into its own opposite slot (generic cell types, where the pair's cells may
+// differ).
+__device__ __forceinline__ void pair_store_one(float* d, size_t N, int n, int x, int nx, int q, int yd, bool bounce,
+                                               float v) {
+    if (bounce) st_nt(d + dopp(q) * N + n, v);
+    else st_nt(d + q * N + (static_cast<size_t>(yd) * nx + wrap_x(x + dcx(q), nx)), v);
+}
+
+// The pair's stores of population q when both cells push (or both bounce):
+// 8-byte pairs as in the channel form.
+__device__ __forceinline__ void pair_store_both(float* d, size_t N, int n, int x, int nx, int q, int yd, bool bounce,
+                                                float v0, float v1) {
+    if (bounce) {
+        st_pair(d + dopp(q) * N + n, v0, v1);
+    } else if (dcx(q) == 0) {
+        st_pair(d + q * N + (static_cast<size_t>(yd) * nx + x), v0, v1);
+    } else {
+        const size_t row = static_cast<size_t>(yd) * nx;
+        const int xa = x + dcx(q);
+        if (xa >= 0 && xa + 1 < nx) {
+            st_pair_u(d + q * N + row + xa, v0, v1);
+        } else {
+            st_nt(d + q * N + row + wrap_x(x + dcx(q), nx), v0);
+            st_nt(d + q * N + row + wrap_x(x + 1 + dcx(q), nx), v1);
+        }
+    }
+}
+
+
+// Generic cell types: each cell of the pair takes its own type and neighbour
+// mask (lbm_cell with FLAGS); where both cells are fluid and decide a
+// direction alike, the two stores still move as one pair.
 template <int OP, bool ADRF>
+__device__ __forceinline__ int lbm_pair_types(const LbmArgs& a, int x, int y, int n) {
+    const uint32_t mm = *reinterpret_cast<const uint32_t*>(a.nbm + n);   // n even: 4-byte aligned
+    const unsigned m0 = mm & 0xffffu, m1 = mm >> 16;
+    const bool fl0 = mask_type(m0) != SF_SOLID, fl1 = mask_type(m1) != SF_SOLID;
+    const bool co0 = mask_type(m0) == SF_FLUID, co1 = mask_type(m1) == SF_FLUID;
+    float Fx0, Fy0, Fx1, Fy1;
+    int capped = collision_force<float>(a, a.Fcur, n, co0, Fx0, Fy0);
+    capped += collision_force<float>(a, a.Fcur, n + 1, co1, Fx1, Fy1);
+    if (!fl0 && !fl1) return capped;
+    const float* src = static_cast<const float*>(a.fsrc);
+    float* dst = static_cast<float*>(a.fdst);
+    const size_t N = static_cast<size_t>(a.N);
+    float f0[9], f1[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const pair_f2 v = ld_pair(src + q * N + n);
+        f0[q] = v.x;
+        f1[q] = v.y;
+    }
+    float ux0 = 0.f, uy0 = 0.f, ux1 = 0.f, uy1 = 0.f;
+    if (fl0) pair_collide<OP>(a, f0, n, co0, Fx0, Fy0, ux0, uy0);
+    if (fl1) pair_collide<OP>(a, f1, n + 1, co1, Fx1, Fy1, ux1, uy1);
+    const bool both = fl0 && fl1;
+    if constexpr (ADRF) {
+        for (int s = 0; s < a.nsp; ++s) {
+            const float* gs = static_cast<const float*>(a.gsrc[s]);
+            float* gd = static_cast<float*>(a.gdst[s]);
+            float g0[9], g1[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const pair_f2 v = ld_pair(gs + q * N + n);
+                g0[q] = v.x;
+                g1[q] = v.y;
+            }
+            if (fl0) pair_relax(a, s, g0, n, static_cast<double>(ux0), static_cast<double>(uy0),
+                                mask_type(m0) == SF_INLET);
+            if (fl1) pair_relax(a, s, g1, n + 1, static_cast<double>(ux1), static_cast<double>(uy1),
+                                mask_type(m1) == SF_INLET);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                int yd = y + dcy(q);
+                bool yout = false;
+                if (a.scalar_periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
+                else yout = yd < 0 || yd >= a.ny;
+                const bool b0 = yout || mask_solid_nb(m0, q), b1 = yout || mask_solid_nb(m1, q);
+                if (both && b0 == b1) {
+                    pair_store_both(gd, N, n, x, a.nx, q, yd, b0, g0[q], g1[q]);
+                } else {
+                    if (fl0) pair_store_one(gd, N, n, x, a.nx, q, yd, b0, g0[q]);
+                    if (fl1) pair_store_one(gd, N, n + 1, x + 1, a.nx, q, yd, b1, g1[q]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        int yd = y + dcy(q);
+        bool oob = false;
+        if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
+        else oob = (yd < 0 || yd >= a.ny);
+        if (!oob) {
+            const bool b0 = mask_solid_nb(m0, q), b1 = mask_solid_nb(m1, q);
+            if (both && b0 == b1) {
+                pair_store_both(dst, N, n, x, a.nx, q, yd, b0, f0[q], f1[q]);
+            } else {
+                if (fl0) pair_store_one(dst, N, n, x, a.nx, q, yd, b0, f0[q]);
+                if (fl1) pair_store_one(dst, N, n + 1, x + 1, a.nx, q, yd, b1, f1[q]);
+            }
+        }
+        if (q != 0) {
+            const int ys = y - dcy(q);
+            if (!a.periodic_y && (ys < 0 || ys >= a.ny)) {
+                if (both) st_pair(dst + q * N + n, f0[q], f1[q]);
+                else if (fl0) st_nt(dst + q * N + n, f0[q]);
+                else st_nt(dst + q * N + n + 1, f1[q]);
+            }
+        }
+    }
+    return capped;
+}
+
+template <int OP, bool ADRF, bool FLAGS>
 __device__ __forceinline__ int lbm_pair(const LbmArgs& a, int x, int y) {
     if (x >= a.nx || y >= a.ny) return 0;     // nx is even: x + 1 < nx as well
     const int n = y * a.nx + x;
+    if constexpr (FLAGS) return lbm_pair_types<OP, ADRF>(a, x, y, n);
     const bool fluid = !is_solid(a, x, y, false);
     float Fx0, Fy0, Fx1, Fy1;
     int capped = collision_force<float>(a, a.Fcur, n, fluid, Fx0, Fy0);
@@ -729,11 +858,8 @@ __device__ __forceinline__ int lbm_pair(const LbmArgs& a, int x, int y) {
     return capped;
 }
 
-// Four waves per SIMD: the MRT form with the scalar would otherwise take 137
-// VGPRs (three waves); at 128 it spills 32 bytes per lane and still runs
-// faster, the extra wave keeping more bytes in flight.
-template <int OP, bool ADRF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_lbm_pair(LbmArgs a) {
+template <int OP, bool ADRF, bool FLAGS>
+__device__ __forceinline__ void lbm_pair_grid(const LbmArgs& a) {
     int capped = 0;
     if (a.persist == 0) {
         const int b = blockIdx.x;
@@ -741,14 +867,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (t >= a.nblk) return;
         const int x = 2 * ((t % a.nbx) * blockDim.x + threadIdx.x);
         const int y = (t / a.nbx) * blockDim.y + threadIdx.y;
-        capped = lbm_pair<OP, ADRF>(a, x, y);
+        capped = lbm_pair<OP, ADRF, FLAGS>(a, x, y);
     } else {
         const int b = blockIdx.x, xcd = b % 8, l = b / 8;
         const int t_end = min((xcd + 1) * a.per_xcd, a.nblk);
         for (int t = xcd * a.per_xcd + l; t < t_end; t += a.persist) {
             const int x = 2 * ((t % a.nbx) * blockDim.x + threadIdx.x);
             const int y = (t / a.nbx) * blockDim.y + threadIdx.y;
-            capped += lbm_pair<OP, ADRF>(a, x, y);
+            capped += lbm_pair<OP, ADRF, FLAGS>(a, x, y);
         }
     }
     for (int off = 32; off > 0; off >>= 1) capped += __shfl_down(capped, off);
@@ -756,11 +882,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         atomicAdd(&a.counters[0], capped);
 }
 
-// The paired form applies: FP32, x periodic without x walls, no generic cell
-// types, no moving walls, nx even.
+// Channels: four waves per SIMD.  The MRT form with the scalar would otherwise
+// take 137 VGPRs (three waves); at 128 it spills 32 bytes per lane and still
+// runs faster, the extra wave keeping more bytes in flight.
+template <int OP, bool ADRF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_lbm_pair(LbmArgs a) {
+    lbm_pair_grid<OP, ADRF, false>(a);
+}
+
+// Generic cell types: held to four waves, the MRT form with the scalar spills
+// 200 bytes per lane, so the allocator keeps its own count here.
+template <int OP, bool ADRF>
+__global__ __launch_bounds__(256) void k_lbm_pair_types(LbmArgs a) {
+    lbm_pair_grid<OP, ADRF, true>(a);
+}
+
+// The paired form applies: FP32, x periodic without x walls, no moving walls,
+// nx even (generic cell types take the per-cell form, lbm_pair_types).
 bool lbm_pair_ok(const Ctx& c) {
     const bool moving = fabs(c.p.bottom_wall_velocity) > 1e-15 || fabs(c.p.top_wall_velocity) > 1e-15;
-    return c.fp32 && !c.generic_flags && c.periodic_x && !c.walls_x && !moving && c.nx % 2 == 0;
+    return c.fp32 && c.periodic_x && !c.walls_x && !moving && c.nx % 2 == 0;
 }
 
 template <int OP, bool FP32>
@@ -820,17 +961,30 @@ void launch_lbm_cfg(Ctx& c, int shift, bool fuse_adr, int bx, int by, int remap,
     LbmArgs a = lbm_args(c, shift, fuse_adr);
     dim3 b, g;
     const int op = c.p.collision_model;
-    if (pair && lbm_pair_ok(c)) {
+    // pair: 1 channels, 2 also generic cell types (lbm_pair_types)
+    if (pair && lbm_pair_ok(c) && (!c.generic_flags || pair == 2)) {
         set_geometry(c, a, bx, by, remap, persist, g, b, c.nx / 2);
         auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, b, 0, c.stream, a); };
+        auto by_op = [&](auto adr, auto flags) {
+            constexpr bool AD = decltype(adr)::value, FL = decltype(flags)::value;
+            if constexpr (FL) {
+                if (op == SF_REGULARIZED) go(k_lbm_pair_types<SF_REGULARIZED, AD>);
+                else if (op == SF_MRT) go(k_lbm_pair_types<SF_MRT, AD>);
+                else go(k_lbm_pair_types<SF_BGK, AD>);
+            } else {
+                if (op == SF_REGULARIZED) go(k_lbm_pair<SF_REGULARIZED, AD>);
+                else if (op == SF_MRT) go(k_lbm_pair<SF_MRT, AD>);
+                else go(k_lbm_pair<SF_BGK, AD>);
+            }
+        };
+        using T_ = std::true_type;
+        using F_ = std::false_type;
         if (fuse_adr) {
-            if (op == SF_REGULARIZED) go(k_lbm_pair<SF_REGULARIZED, true>);
-            else if (op == SF_MRT) go(k_lbm_pair<SF_MRT, true>);
-            else go(k_lbm_pair<SF_BGK, true>);
+            if (c.generic_flags) by_op(T_{}, T_{});
+            else by_op(T_{}, F_{});
         } else {
-            if (op == SF_REGULARIZED) go(k_lbm_pair<SF_REGULARIZED, false>);
-            else if (op == SF_MRT) go(k_lbm_pair<SF_MRT, false>);
-            else go(k_lbm_pair<SF_BGK, false>);
+            if (c.generic_flags) by_op(F_{}, T_{});
+            else by_op(F_{}, F_{});
         }
         return;
     }
@@ -981,7 +1135,7 @@ __global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* 
         if (w.y >= a.nblk) return;
         const int x = 2 * ((w.y % a.nbx) * blockDim.x + threadIdx.x);
         const int y = (w.y / a.nbx) * blockDim.y + threadIdx.y;
-        capped = lbm_pair<OP, false>(a, x, y);
+        capped = lbm_pair<OP, false, false>(a, x, y);
     } else {
         int x, y;
         if (!block_origin(a, w.y, x, y)) return;
@@ -998,7 +1152,7 @@ __global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* 
 void batch_items_lbm(Ctx& c, int bx, int by, int pair, BatchItem* it) {
     LbmArgs a = lbm_args(c, SHIFT_PREV, false);
     dim3 g, b;
-    a.pair = pair && lbm_pair_ok(c) ? 1 : 0;
+    a.pair = pair && lbm_pair_ok(c) && !c.generic_flags ? 1 : 0;
     set_geometry(c, a, bx, by, 0, 0, g, b, a.pair ? c.nx / 2 : 0);
     batch_put(it[0], a.nblk, a);
     c.fcur = 1 - c.fcur;

@@ -1,12 +1,13 @@
 """Two cells per lane in the collide-stream against one cell per lane.
 
-The paired kernel takes cells (x, x+1) of FP32 channels (x periodic, no x
-walls, no generic cell types, no moving walls, nx even) and evaluates each
-cell with the single-cell kernel's expressions, so every case here must end
+The paired kernel takes cells (x, x+1) of FP32 lattices (x periodic without x
+walls, no moving walls, nx even; generic cell types per cell) and evaluates
+each cell with the single-cell kernel's expressions, so every case here must end
 bit-identical either way: walls and periodic y, the three collision models,
 capsules, the fused scalar with and without periodic y, an MDF run (the
-partial shift force) and a run whose macroscopic fields were written from the
-host (the stored shift).  Launch shapes cover rows that end inside a
+partial shift force), a run whose populations were written from the host (the
+stored shift), and generic cell types (pillars with interpolated bounce-back,
+extra solids, where the two cells of a pair differ).  Launch shapes cover rows that end inside a
 workgroup (nx = 130 gives 65 lanes per row), blocks taller than one row and a
 persistent grid walking several tiles.  A configuration outside the paired
 form (FP64) is checked to run the single-cell kernel unchanged.
@@ -14,7 +15,7 @@ form (FP64) is checked to run the single-cell kernel unchanged.
 import numpy as np
 import pytest
 
-from tests.cases import product_sim, product_state
+from tests.cases import PILLARS, product_sim, product_state
 
 pytestmark = pytest.mark.gpu
 
@@ -30,6 +31,12 @@ CASES = {
                                   params=dict()),
     "reg_mdf3_caps": dict(nx=130, ny=40, coll=2, bfx=2e-5, mdf=3, precision="fp32", capsules=CAPS),
     "bgk_fp64_not_paired": dict(nx=130, ny=40, coll=0, bfx=2e-5, capsules=CAPS),
+    # generic cell types: the two cells of a pair take their own types and masks
+    "pillars_ibb_mrt_caps_scalar": dict(nx=130, ny=40, coll=1, bfx=2e-5, precision="fp32", obstacles=PILLARS,
+                                        ibb=True, capsules=CAPS, scalar=SCALAR),
+    "solids_bgk_periodic_y_scalar": dict(nx=64, ny=32, coll=0, bfx=1e-5, periodic_y=True, precision="fp32",
+                                         solids=[("circle", 31.0, 16.0, 5.0), ("rect", 10, 4, 10, 27)],
+                                         scalar=SCALAR | dict(blob=(45, 16, 4.0))),
 }
 
 # (bx, by, xcd_banded, persistent workgroups per XCD)
