@@ -119,20 +119,49 @@ def build_product(case, precision="fp32", graph=True):
     return sim
 
 
+def host_cpu():
+    """CPU model and logical core count of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
+def cpu_threads():
+    """OpenMP threads for the CPU baseline: every core this process is given
+    (OMP_NUM_THREADS when set, which a shared GPU host uses to allot its
+    share), else every core of the host."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+    return max(1, min(n, os.cpu_count() or 1))
+
+
 def cpu_baseline(case, steps, threads):
     """The reference's own hot-path components (oracle/_ref: its lbm, coupling
-    and membrane translation units compiled in place, driven in
-    Simulation::step order) with ``threads`` OpenMP threads on this host:
-    ``steps`` full steps of the same workload.  Where that build is absent,
-    the single-threaded FP64 oracle restatement instead."""
+    and membrane translation units compiled in place under its Release flags,
+    driven in Simulation::step order) with ``threads`` OpenMP threads on this
+    host: ``steps`` full steps of the same workload.  Where the Release build is
+    absent, the strict (parity) build; where neither exists, the
+    single-threaded FP64 oracle restatement."""
     from oracle import pyoracle as po
 
-    kind = "reference" if os.path.exists(po.REF_SO) else "port"
+    if os.path.exists(po.REF_RELEASE_SO):
+        kind, which = "reference", "ref-release"
+    elif os.path.exists(po.REF_SO):
+        kind, which = "reference", "ref"
+    else:
+        kind, which = "port", "oracle"
     p = po.default_params(nx=case["nx"], ny=case["ny"], tau=case["tau"], boundary_type=1,
                           body_force_x=case["bfx"], collision_model=case["coll"],
                           scalar_enabled=1, diffusivity=[case["scalar"]["D"]],
                           use_interpolated_bb=1 if case.get("ibb") else 0)
-    s = po.Sim(p, "ref" if kind == "reference" else "oracle")
+    s = po.Sim(p, which)
     for ob in case.get("obstacles", []):
         s.add_obstacle(ob)
     if kind == "reference":
@@ -149,7 +178,9 @@ def cpu_baseline(case, steps, threads):
     t = time.perf_counter()
     s.step(steps)
     dt = time.perf_counter() - t
-    return case["nx"] * case["ny"] * steps / dt / 1e6, dt, kind, threads
+    flags = {"ref-release": po.REF_RELEASE_FLAGS, "ref": "-O2 -ffp-contract=off -fno-fast-math -fopenmp",
+             "oracle": "-O2 -ffp-contract=off (single thread)"}[which]
+    return case["nx"] * case["ny"] * steps / dt / 1e6, dt, kind, threads, flags
 
 
 # Learned MDF warm start workload: 2048 x 512 channel, regularized tau = 0.8,
@@ -195,12 +226,37 @@ def build_mdf_channel(precision, ibm_iterations):
     return sim, placed
 
 
-def timed_steps(sim, steps):
-    sim.getMaxSpeed()
+GRAPH_PERIOD = 6   # the step's buffer roles repeat every six steps
+
+
+def prime_graphs(sim, steps):
+    """Capture, outside any timed region, every graph that ``sim.run(steps)``
+    will replay from the current buffer phase.  A run of ``steps`` from phase P
+    replays the six-step graph of phase P and a (steps mod 6)-step graph of
+    phase P: one untimed run of the same length captures both, and
+    (6 - steps mod 6) mod 6 more steps bring the roles back to phase P."""
+    sim.run(steps)
+    r = steps % GRAPH_PERIOD
+    if r:
+        sim.run(GRAPH_PERIOD - r)
+    sim.synchronize()
+
+
+def timed_run(sim, steps):
+    """Seconds for ``sim.run(steps)`` (which returns once the device is done),
+    with its graphs primed first, and the number of graphs captured inside the
+    timed region (0 when every step was a replay)."""
+    prime_graphs(sim, steps)
+    cap0 = sim.graphCaptures()
     t = time.perf_counter()
     sim.run(steps)
-    sim.getMaxSpeed()
-    return steps / (time.perf_counter() - t)
+    dt = time.perf_counter() - t
+    return dt, sim.graphCaptures() - cap0
+
+
+def timed_steps(sim, steps):
+    dt, _ = timed_run(sim, steps)
+    return steps / dt
 
 
 def mdf_learned_bench(steps, warmup, train_steps, epochs, probe_steps=100):
@@ -272,7 +328,8 @@ def mdf_learned_bench(steps, warmup, train_steps, epochs, probe_steps=100):
     out["velocity_rel_l2_vs_plain_fp64"] = float(np.linalg.norm(vel(learned) - ua) / np.linalg.norm(ua))
     del learned
     three, _ = build_mdf_channel("fp64", 4)
-    three.run(warmup + 10 + probe_steps + steps)
+    # the same step count as the timed runs, their graph priming included
+    three.run(warmup + 10 + probe_steps + 2 * steps + (-steps) % GRAPH_PERIOD)
     out["velocity_rel_l2_plain_3_vs_4_corrections"] = float(np.linalg.norm(vel(three) - ua) / np.linalg.norm(ua))
     out["training"] = {"rows": int(rows), "epochs": epochs, "seconds": round(train_s, 2),
                        "loss_first_epoch": loss[0], "loss_last_epoch": loss[-1]}
@@ -356,11 +413,8 @@ def output_bench(case, steps):
     from softflow_amd import softflow_core as sc
 
     def rate(sim):
-        sim.getMaxSpeed()
-        t = time.perf_counter()
-        sim.run(steps)
-        sim.getMaxSpeed()
-        return (time.perf_counter() - t) * 1e3 / steps
+        dt, _ = timed_run(sim, steps)
+        return dt * 1e3 / steps
 
     out = {}
     tmp = tempfile.mkdtemp(prefix="sf_out_")
@@ -432,11 +486,7 @@ def geometry_bench(steps, cpu_steps, threads, with_cpu=True):
     case = pillar_case()
     sim = build_product(case, "fp32", graph=USE_GRAPH)
     sim.run(20)
-    sim.getMaxSpeed()
-    t = time.perf_counter()
-    sim.run(steps)
-    sim.getMaxSpeed()
-    dt = time.perf_counter() - t
+    dt, _ = timed_run(sim, steps)
     out = {"workload": "4096x1024 MRT tau=0.7 fx=1e-5 through 200 staggered circular pillars (r 24) "
                        "with interpolated bounce-back, 1000 Skalak+Helfrich capsules, ADR D=0.05; FP32",
            "pillars": len(case["obstacles"]), "capsules": sum(c["count"] for c in case["capsules"]),
@@ -444,8 +494,9 @@ def geometry_bench(steps, cpu_steps, threads, with_cpu=True):
     del sim
     if with_cpu:
         try:
-            v, cdt, kind, cores = cpu_baseline(case, cpu_steps, threads)
+            v, cdt, kind, cores, flags = cpu_baseline(case, cpu_steps, threads)
             out["cpu_baseline"] = {"value": round(v, 3), "unit": "MLUPS", "cores": cores, "kind": kind,
+                                   "flags": flags,
                                    "sample": f"{cpu_steps} full steps of the pillar workload ({cdt:.1f} s)"}
         except Exception as e:
             out["cpu_baseline"] = {"value": None, "sample": f"unavailable: {e}"}
@@ -529,7 +580,7 @@ def main():
         elif args.only == "mdf":
             part = mdf_learned_bench(args.mdf_steps, 200, 160, 6)
         elif args.only == "geometry":
-            part = geometry_bench(args.steps, 5, max(1, min(16, os.cpu_count() or 1)), not args.no_cpu)
+            part = geometry_bench(args.steps, 5, cpu_threads(), not args.no_cpu)
         else:
             part = output_bench(large_domain_case(), args.output_steps)
         sys.stdout.flush()
@@ -557,11 +608,10 @@ def main():
         # this workload (FP32, x-periodic channel, nx even) takes the paired kernel by default
         tune += " 2 cells/lane"
     sim.run(args.warmup)
-    sim.getMaxSpeed()   # synchronises
-    t = time.perf_counter()
-    sim.run(args.steps)
-    sim.getMaxSpeed()
-    dt = time.perf_counter() - t
+    # the timed region replays graphs only: those it needs are captured first
+    dt, captures = timed_run(sim, args.steps)
+    if captures:
+        raise RuntimeError(f"{captures} step graphs were captured inside the timed region")
     ms_per_step = dt * 1e3 / args.steps
     mlups = NX * NY * args.steps / dt / 1e6
     print(f"# graph replay: {ms_per_step:.4f} ms/step, {mlups:.0f} MLUPS", file=sys.stderr, flush=True)
@@ -572,7 +622,6 @@ def main():
         sim.setTiming(True)
         t = time.perf_counter()
         sim.run(args.steps)
-        sim.getMaxSpeed()
         dt_eager = time.perf_counter() - t
         kt = dict(sim.kernelTimes())
         sim.setTiming(False)
@@ -618,8 +667,7 @@ def main():
     geo = None
     if not args.no_geometry:
         try:
-            geo = geometry_bench(min(args.steps, 1000), 5, max(1, min(16, os.cpu_count() or 1)),
-                                 not args.no_cpu)
+            geo = geometry_bench(min(args.steps, 1000), 5, cpu_threads(), not args.no_cpu)
         except RuntimeError as e:
             geo = {"error": str(e)}
         print("# pillar array: " + json.dumps(geo), file=sys.stderr, flush=True)
@@ -640,11 +688,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            threads = max(1, min(16, os.cpu_count() or 1))
-            v, cdt, kind, cores = cpu_baseline(case, args.cpu_steps, threads)
+            threads = cpu_threads()
+            v, cdt, kind, cores, flags = cpu_baseline(case, args.cpu_steps, threads)
             what = ("the reference's own components (OpenMP, FP64)" if kind == "reference"
                     else "the FP64 oracle restatement")
+            model, nproc = host_cpu()
             cpu = {"value": round(v, 3), "unit": "MLUPS", "cores": cores, "kind": kind,
+                   "flags": flags, "host_cpu": model, "host_nproc": nproc,
                    "sample": f"{args.cpu_steps} full steps of the same 4096x1024 + 2000-capsule "
                              f"workload in {what} ({cdt:.1f} s)"}
         except Exception as e:  # the oracle library is test infrastructure; report its absence
@@ -675,6 +725,7 @@ def main():
         "pillar_array": geo,
         "kernel_ms": {k: round(v, 5) for k, v in sorted(kt.items())},
         "lbm_launch": tune.strip().splitlines()[-1],
+        "graph_captures_in_timed_region": captures,
         "mass": stats[0],
         "max_speed": stats[3],
     }

@@ -152,6 +152,10 @@ int  sf_step(sf_ctx* ctx, int nsteps);
 /* Replay the step as a captured HIP graph (1) or launch kernel by kernel (0). */
 int  sf_set_graph_mode(sf_ctx* ctx, int on);
 int  sf_synchronize(sf_ctx* ctx);
+/* Step graphs captured and instantiated by this context so far (no
+ * counterpart in the reference): a timed run whose count does not change
+ * replayed graphs only. */
+int  sf_graph_captures(sf_ctx* ctx, int64_t* n);
 
 /* Stage-level entry points of one step, in simulation.cpp:470-787 order.
  * SF_STAGE_FORCES  = clearAllForces + computeAllMembraneForces + repulsion

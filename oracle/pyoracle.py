@@ -17,6 +17,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libsfo.so")
 REF_SO = os.path.join(HERE, "_ref", "libsfref.so")
+# the same sources under the reference's own Release flags
+# (cmake/CompilerFlags.cmake:42,49; -march=x86-64-v3 in place of -march=native,
+# since it is built on one host and timed on another): the CPU baseline, never
+# a parity checker
+REF_RELEASE_SO = os.path.join(HERE, "_ref", "libsfref_release.so")
+REF_RELEASE_FLAGS = "-O3 -march=x86-64-v3 -ffast-math -fopenmp -DNDEBUG"
 
 
 class Params(C.Structure):
@@ -176,6 +182,8 @@ class Sim:
             self.L, self.pre = oracle_lib(), "sfo_"
         elif which == "ref":
             self.L, self.pre = _lib(REF_SO, "sfr_"), "sfr_"
+        elif which == "ref-release":
+            self.L, self.pre = _lib(REF_RELEASE_SO, "sfr_"), "sfr_"
         else:
             raise ValueError(which)
         self.params = params

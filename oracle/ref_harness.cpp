@@ -1,9 +1,11 @@
 // Driver for the reference engine's own hot-path components, built from the
 // reference sources in place into oracle/_ref/libsfref.so (see Makefile).
 //
-// TEST INFRASTRUCTURE ONLY: used in the build container to certify the
-// oracle restatement (sfo.cpp) and to write golden vectors.  It never ships
-// to the GPU machine and is never loaded by the product.
+// TEST INFRASTRUCTURE ONLY: used to certify the oracle restatement (sfo.cpp),
+// to write golden vectors and, built with the reference's Release flags
+// (libsfref_release.so), as the CPU baseline that bench.py times.  The built
+// libraries travel with the tree to the GPU machine for that baseline; the
+// reference sources do not.  The product never loads either library.
 //
 // engine/simulation.cpp cannot be compiled here: it includes the CMake-
 // generated <softflow/build_info.h>.  This driver therefore composes the

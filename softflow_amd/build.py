@@ -73,7 +73,7 @@ def build_core() -> str:
 def build_oracle(ref: bool = False) -> None:
     _run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "oracle"])
     if ref and os.path.isdir("/root/reference/src"):
-        _run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "ref"])
+        _run(["make", "-B", "-C", os.path.join(ROOT, "oracle"), "ref", "ref-release"])
 
 
 def build_all(oracle: bool = True) -> None:

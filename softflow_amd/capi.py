@@ -26,7 +26,7 @@ EXPORTS = (
     "sf_set_populations", "sf_get_populations", "sf_get_macroscopic", "sf_set_macroscopic",
     "sf_set_capsules", "sf_get_nodes", "sf_set_nodes", "sf_get_membrane_state",
     "sf_get_force_state", "sf_set_force_state",
-    "sf_set_scalar", "sf_get_scalar", "sf_step", "sf_set_graph_mode", "sf_synchronize",
+    "sf_set_scalar", "sf_get_scalar", "sf_step", "sf_set_graph_mode", "sf_synchronize", "sf_graph_captures",
     "sf_stage", "sf_last_capped", "sf_field_stats", "sf_global_stats", "sf_step_count", "sf_kernel_times",
     "sf_host_alloc", "sf_host_free", "sf_set_probes", "sf_snapshot_async", "sf_snapshot_wait",
     "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_set_lbm_pairs", "sf_probe_math", "sf_copy_ceiling",
@@ -111,6 +111,7 @@ def load():
         "sf_field_stats": [C.c_void_p, _DP],
         "sf_global_stats": [C.c_void_p, C.c_double, _DP],
         "sf_step_count": [C.c_void_p, C.POINTER(C.c_int64)],
+        "sf_graph_captures": [C.c_void_p, C.POINTER(C.c_int64)],
         "sf_host_alloc": [C.POINTER(C.c_void_p), C.c_uint64],
         "sf_host_free": [C.c_void_p],
         "sf_set_probes": [C.c_void_p, C.c_int, _IP],

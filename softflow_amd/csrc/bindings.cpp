@@ -476,6 +476,8 @@ PYBIND11_MODULE(softflow_core, m) {
             });
         }, py::arg("callback"))
         .def("setTiming", &Simulation::setTiming, py::arg("on"))
+        .def("synchronize", &Simulation::synchronize, py::call_guard<py::gil_scoped_release>())
+        .def("graphCaptures", &Simulation::graphCaptures)
         .def("autotune", &Simulation::autotune, py::arg("reps") = 10)
         .def("setLbmPairs", &Simulation::setLbmPairs, py::arg("on"))
         .def("setLbmLaunch", &Simulation::setLbmLaunch, py::arg("bx"), py::arg("by"), py::arg("xcd_banded") = true,

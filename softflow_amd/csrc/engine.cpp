@@ -960,6 +960,14 @@ std::pair<double, std::vector<float>> MdfModel::lossGrad(const std::vector<float
 
 void Simulation::setTiming(bool on) { if (ctx_) check(sf_set_timing(ctx_, on ? 1 : 0), "sf_set_timing"); }
 
+void Simulation::synchronize() { if (ctx_) check(sf_synchronize(ctx_), "sf_synchronize"); }
+
+int64_t Simulation::graphCaptures() {
+    int64_t n = 0;
+    if (ctx_) check(sf_graph_captures(ctx_, &n), "sf_graph_captures");
+    return n;
+}
+
 std::vector<std::pair<std::string, double>> Simulation::kernelTimes() {
     std::vector<std::pair<std::string, double>> out;
     if (!ctx_) return out;

@@ -427,6 +427,8 @@ public:
     bool initialized() const { return ctx_ != nullptr; }
     std::vector<std::pair<std::string, double>> kernelTimes();
     void setTiming(bool on);
+    void synchronize();
+    int64_t graphCaptures();
     std::string autotune(int reps = 10);
     void setLbmLaunch(int bx, int by, bool xcd_banded, int persist = 0);
     void setLbmPairs(bool on);

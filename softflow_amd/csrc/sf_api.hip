@@ -216,12 +216,21 @@ static void one_step(Ctx& c) {
 
 void ctx_one_step(Ctx& c) { one_step(c); }
 
+// An executable graph may still be queued or running on the stream (sf_step
+// returns once the launches are enqueued).  The runtime keeps per-launch state
+// that refers to the executable graph until the launch retires, so it is
+// destroyed only after the stream has drained.
+static void destroy_graph_entry(Ctx& c, Ctx::GraphEntry& g, bool drained) {
+    if (!drained) hipStreamSynchronize(c.stream);
+    hipGraphExecDestroy(g.exec);
+    hipGraphDestroy(g.graph);
+}
+
 static void invalidate_graph(Ctx& c) {
     c.gen++;
-    for (Ctx::GraphEntry& g : c.graphs) {
-        hipGraphExecDestroy(g.exec);
-        hipGraphDestroy(g.graph);
-    }
+    if (c.graphs.empty()) return;
+    hipStreamSynchronize(c.stream);
+    for (Ctx::GraphEntry& g : c.graphs) destroy_graph_entry(c, g, true);
     c.graphs.clear();
 }
 
@@ -253,7 +262,7 @@ static int run_graph(Ctx& c, int nsteps) {
             Ctx::GraphEntry g{};
             std::copy(key, key + 5, g.key);
             g.len = len;
-            hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+            SF_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
             for (int s = 0; s < len; ++s) one_step(c);
             hipError_t e = hipStreamEndCapture(c.stream, &g.graph);
             c.step_count -= len;   // counted again below
@@ -270,11 +279,11 @@ static int run_graph(Ctx& c, int nsteps) {
                 SF_HIP(e);
             }
             if (c.graphs.size() >= MAX_GRAPHS) {
-                hipGraphExecDestroy(c.graphs.front().exec);
-                hipGraphDestroy(c.graphs.front().graph);
+                destroy_graph_entry(c, c.graphs.front(), false);
                 c.graphs.erase(c.graphs.begin());
             }
             c.graphs.push_back(g);
+            c.graph_captures++;
             hit = &c.graphs.back();
         }
         SF_HIP(hipGraphLaunch(hit->exec, c.stream));
@@ -1254,6 +1263,12 @@ int sf_snapshot_wait(sf_ctx* h, int64_t ticket) {
 int sf_step_count(sf_ctx* h, int64_t* n) {
     if (!h || !n) return fail(SF_E_ARG, "null pointer");
     *n = h->c.step_count;
+    return SF_OK;
+}
+
+int sf_graph_captures(sf_ctx* h, int64_t* n) {
+    if (!h || !n) return fail(SF_E_ARG, "null pointer");
+    *n = h->c.graph_captures;
     return SF_OK;
 }
 

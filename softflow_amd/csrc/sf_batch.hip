@@ -120,6 +120,8 @@ struct sf_batch {
 };
 
 static void drop_graph(sf_batch& B) {
+    // a launch of the executable graph may still be in flight on the stream
+    if (B.gexec) hipStreamSynchronize(B.stream);
     if (B.gexec) hipGraphExecDestroy(B.gexec);
     if (B.graph) hipGraphDestroy(B.graph);
     B.gexec = nullptr;

@@ -237,6 +237,7 @@ struct Ctx {
     };
     std::vector<GraphEntry> graphs;
     int64_t gen = 0;                     // bumped whenever buffers, parameters or launch shapes change
+    int64_t graph_captures = 0;          // step graphs captured and instantiated so far
 
     // timing
     bool timing = false;
