@@ -36,7 +36,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-NX, NY = 4096, 1024
+from softflow_amd.workloads import (MDF_CHANNEL, NX, NY, build_mdf_channel,  # noqa: E402
+                                   build_product, large_domain_case, pillar_case)
+
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per lattice update of the fused collide-stream in FP32:
 # 9 populations read + 9 written (72 B), the step's raw force (8 B) and the
@@ -44,79 +46,6 @@ HBM_PEAK_GBS = 8000.0
 LBM_BYTES_PER_LUP = 88
 # the scalar step carried by the same kernel: 9 g read + 9 g written (FP32)
 ADR_BYTES_PER_LUP = 72
-
-
-def large_domain_case():
-    big = dict(count=800, region=(0, 0, NX, NY), rmin=6.0, rmax=6.0, nodes=38, seed=101,
-               mem=dict(model=2, G_s=0.03, C_skalak=10.0, use_helfrich=1, k_bend=1e-3))
-    small = dict(count=1200, region=(0, 0, NX, NY), rmin=3.0, rmax=3.0, nodes=19, seed=202,
-                 mem=dict(model=2, G_s=0.06, C_skalak=10.0, use_helfrich=1, k_bend=1e-3))
-    return dict(nx=NX, ny=NY, tau=0.7, bfx=1e-5, coll=1, capsules=[big, small],
-                scalar=dict(D=0.05, blob=(NX / 2, NY / 2, 20.0)))
-
-
-def pillar_case():
-    """4096x1024 channel through a staggered array of 200 circular pillars (radius 24,
-    160 apart in x, 128 in y) with interpolated bounce-back, 1000 capsules placed
-    around them, MRT, the scalar on: the large domain with obstacles."""
-    pillars = []
-    for row in range(8):
-        y = 64.0 + 128.0 * row + 0.21
-        x = (80.0 if row % 2 == 0 else 160.0) + 0.37
-        while x < NX - 40:
-            pillars.append(("circle", x, y, 24.0))
-            x += 160.0
-    big = dict(count=400, region=(0, 0, NX, NY), rmin=6.0, rmax=6.0, nodes=38, seed=303,
-               mem=dict(model=2, G_s=0.03, C_skalak=10.0, use_helfrich=1, k_bend=1e-3))
-    small = dict(count=600, region=(0, 0, NX, NY), rmin=3.0, rmax=3.0, nodes=19, seed=404,
-                 mem=dict(model=2, G_s=0.06, C_skalak=10.0, use_helfrich=1, k_bend=1e-3))
-    return dict(nx=NX, ny=NY, tau=0.7, bfx=1e-5, coll=1, capsules=[big, small], obstacles=pillars,
-                ibb=True, scalar=dict(D=0.05, blob=(NX / 2, NY / 2, 20.0)))
-
-
-def build_product(case, precision="fp32", graph=True):
-    from softflow_amd import softflow_core as sc
-
-    p = sc.SimulationParams()
-    p.output_interval, p.probe_dump_every, p.stats_dump_every = 0, 0, 0   # no output files
-    for k, v in case.get("output", {}).items():
-        setattr(p, k, v)
-    p.nx, p.ny = case["nx"], case["ny"]
-    p.fluid.tau = case["tau"]
-    p.fluid.boundary_type = sc.BoundaryType.PERIODIC
-    p.fluid.body_force_x = case["bfx"]
-    p.fluid.collision_model = [sc.CollisionModel.BGK, sc.CollisionModel.MRT,
-                               sc.CollisionModel.REGULARIZED][case["coll"]]
-    p.scalar.enabled = True
-    p.scalar.diffusivity = [case["scalar"]["D"]]
-    p.enable_stability_checks = False
-    p.precision = sc.Precision.FP32 if precision == "fp32" else sc.Precision.FP64
-    p.use_graph = graph
-    p.fluid.use_interpolated_bb = bool(case.get("ibb", False))
-    sim = sc.Simulation(p)
-    b = sc.ChannelBuilder(p.nx, p.ny)
-    b.addWalls()
-    for ob in case.get("obstacles", []):
-        b.addCirclePillar(*ob[1:])
-    sim.setChannelBuilder(b)
-    for spec in case["capsules"]:
-        mp = sc.MembraneParams()
-        m = spec["mem"]
-        mp.model = sc.MembraneModel.SKALAK
-        mp.G_s, mp.C_skalak = m["G_s"], m["C_skalak"]
-        mp.use_helfrich_bending = bool(m["use_helfrich"])
-        mp.k_bend = m["k_bend"]
-        x0, y0, x1, y1 = spec["region"]
-        sim.addCapsuleRandom(spec["count"], x0, y0, x1, y1, spec["rmin"], spec["rmax"],
-                             spec["nodes"], mp, 0, spec["seed"], 1.0)
-    sim.initialize()
-    cx, cy, sg = case["scalar"]["blob"]
-    yy, xx = np.mgrid[0:p.ny, 0:p.nx]
-    blob = np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / (2.0 * sg * sg))
-    ad = sim.advectionDiffusion()
-    ad.initialize(0.0)
-    ad.setField(blob.ravel(), 0)
-    return sim
 
 
 def host_cpu():
@@ -183,49 +112,7 @@ def cpu_baseline(case, steps, threads):
     return case["nx"] * case["ny"] * steps / dt / 1e6, dt, kind, threads, flags
 
 
-# Learned MDF warm start workload: 2048 x 512 channel, regularized tau = 0.8,
-# fx = 3e-5; 500 Skalak capsules (G_s = 0.06, C = 1) of radius 4 with 25 nodes
-# seeded (seed 5) over the whole channel (2.4 % area fraction).  Denser
-# seeding (30 % in a 164 x 504 band) and C = 10 both make the explicit
-# coupling of plain multi-direct forcing diverge within 120-200 steps, in the
-# FP64 restatement of the reference as on the device.
-MDF_CHANNEL = dict(nx=2048, ny=512, tau=0.8, bfx=3e-5, count=500, radius=4.0, nodes=25, seed=5,
-                   region=(4, 8, 2044, 504), mem={"C_skalak": 1.0})
-
-
 USE_GRAPH = True   # HIP-graph replay of the steps (off: every launch eager, for tracing)
-
-
-def build_mdf_channel(precision, ibm_iterations):
-    from softflow_amd import softflow_core as sc
-
-    p = sc.SimulationParams()
-    p.output_interval, p.probe_dump_every, p.stats_dump_every = 0, 0, 0   # no output files
-    p.nx, p.ny = MDF_CHANNEL["nx"], MDF_CHANNEL["ny"]
-    p.fluid.tau = MDF_CHANNEL["tau"]
-    p.fluid.boundary_type = sc.BoundaryType.PERIODIC
-    p.fluid.body_force_x = MDF_CHANNEL["bfx"]
-    p.fluid.collision_model = sc.CollisionModel.REGULARIZED
-    p.ibm_iterations = ibm_iterations
-    p.enable_stability_checks = False
-    p.precision = sc.Precision.FP32 if precision == "fp32" else sc.Precision.FP64
-    p.use_graph = USE_GRAPH
-    sim = sc.Simulation(p)
-    b = sc.ChannelBuilder(p.nx, p.ny)
-    b.addWalls()
-    sim.setChannelBuilder(b)
-    mp = sc.MembraneParams()
-    mp.model = sc.MembraneModel.SKALAK
-    mp.G_s, mp.C_skalak = 0.06, 10.0
-    for k, v in MDF_CHANNEL.get("mem", {}).items():
-        setattr(mp, k, v)
-    x0, y0, x1, y1 = MDF_CHANNEL["region"]
-    placed = sim.addCapsuleRandom(MDF_CHANNEL["count"], x0, y0, x1, y1, MDF_CHANNEL["radius"], MDF_CHANNEL["radius"], MDF_CHANNEL["nodes"],
-                                  mp, 0, MDF_CHANNEL["seed"], 1.0)
-    sim.initialize()
-    return sim, placed
-
-
 GRAPH_PERIOD = 6   # the step's buffer roles repeat every six steps
 
 
@@ -276,7 +163,7 @@ def mdf_learned_bench(steps, warmup, train_steps, epochs, probe_steps=100):
                        "(25 nodes) seeded over the channel (2.4% area fraction)"}
     # plain FP64, 5 MDF iterations (4 corrections); residual after the last
     # correction measured over probe_steps, then timed without the probe
-    plain, placed = build_mdf_channel("fp64", 5)
+    plain, placed = build_mdf_channel("fp64", 5, USE_GRAPH)
     out["capsules"] = placed
     plain.run(warmup)
     plain.setMdfControl(0.0, 0, True)
@@ -288,7 +175,7 @@ def mdf_learned_bench(steps, warmup, train_steps, epochs, probe_steps=100):
     out["plain_fp64_steps_per_s"] = round(timed_steps(plain, steps), 1)
     # training rows from the plain FP32 solver (4 corrections: the same scheme)
     t0 = time.perf_counter()
-    coll, _ = build_mdf_channel("fp32", 5)
+    coll, _ = build_mdf_channel("fp32", 5, USE_GRAPH)
     coll.run(warmup)
     K = coll.capsules().totalNodes()
     coll.collectMdfSamples(train_steps * K)
@@ -302,7 +189,7 @@ def mdf_learned_bench(steps, warmup, train_steps, epochs, probe_steps=100):
     # is at most what the plain FP64 scheme reached (here above 1e-4: the
     # plain iteration stalls near 3e-4 on this channel), at most four
     tol = sp["max_residual"]
-    learned, _ = build_mdf_channel("fp32", 1)
+    learned, _ = build_mdf_channel("fp32", 1, USE_GRAPH)
     learned.setMdfControl(1e-300, 4, False)
     learned.run(warmup)
     learned.setMdfModel(model, True)
@@ -327,7 +214,7 @@ def mdf_learned_bench(steps, warmup, train_steps, epochs, probe_steps=100):
     ua = vel(plain)
     out["velocity_rel_l2_vs_plain_fp64"] = float(np.linalg.norm(vel(learned) - ua) / np.linalg.norm(ua))
     del learned
-    three, _ = build_mdf_channel("fp64", 4)
+    three, _ = build_mdf_channel("fp64", 4, USE_GRAPH)
     # the same step count as the timed runs, their graph priming included
     three.run(warmup + 10 + probe_steps + 2 * steps + (-steps) % GRAPH_PERIOD)
     out["velocity_rel_l2_plain_3_vs_4_corrections"] = float(np.linalg.norm(vel(three) - ua) / np.linalg.norm(ua))

@@ -58,6 +58,8 @@ typedef struct {
     double inlet_velocity;          /* SF_INLET_OUTLET (lbm/zou_he_boundary.cpp): peak of the */
     double outlet_density;          /* parabolic inlet profile; outlet density */
     int    use_interpolated_bb;     /* parameters.h:89: Bouzidi links towards the obstacles */
+    int    capsule_periodic_x;      /* parameters.h: capsules recirculate in x over a non-periodic
+                                       fluid (Simulation::step 10a, simulation.cpp:687-700) */
 } sf_params;
 
 /* Per-capsule constants resolved on the host (rest geometry and WLC constants

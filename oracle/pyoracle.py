@@ -44,6 +44,7 @@ class Params(C.Structure):
         ("top_wall_velocity", C.c_double), ("bottom_wall_velocity", C.c_double),
         ("inlet_velocity", C.c_double), ("outlet_density", C.c_double),
         ("use_interpolated_bb", C.c_int),
+        ("capsule_periodic_x", C.c_int),
     ]
 
 
@@ -113,6 +114,11 @@ def _lib(path: str, prefix: str):
     g("add_obstacle").argtypes = [C.c_void_p, C.c_int, _DP, C.c_int]
     g("obstacle_distance").restype = C.c_double
     g("obstacle_distance").argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _DP]
+    g("cap_forces").restype = C.c_int
+    g("cap_forces").argtypes = [_DP, _DP, C.c_int, C.c_double]
+    g("clamp_nodes_y").argtypes = [_DP, C.c_int, C.c_int]
+    g("recirculate_x").restype = C.c_int
+    g("recirculate_x").argtypes = [_DP, _DP, C.POINTER(C.c_int), C.c_int, C.c_int]
     if prefix == "sfo_":
         g("ibb_links").restype = C.c_int
         g("ibb_links").argtypes = [C.c_void_p, C.POINTER(C.c_int), _DP]
@@ -147,6 +153,38 @@ def oracle_lib():
 
 def ref_available() -> bool:
     return os.path.exists(REF_SO)
+
+
+def _step_lib(which):
+    return (oracle_lib(), "sfo_") if which == "oracle" else (_lib(REF_SO, "sfr_"), "sfr_")
+
+
+def cap_forces(Fx, Fy, max_force, which="oracle"):
+    """The step's force cap on copies of (Fx, Fy): (Fx', Fy', cells capped)."""
+    L, pre = _step_lib(which)
+    fx = np.ascontiguousarray(Fx, dtype=np.float64).copy()
+    fy = np.ascontiguousarray(Fy, dtype=np.float64).copy()
+    n = getattr(L, pre + "cap_forces")(_ptr(fx), _ptr(fy), fx.size, float(max_force))
+    return fx, fy, n
+
+
+def clamp_nodes_y(y, ny, which="oracle"):
+    """The step's node clamp into [1.5, ny - 1.5] on a copy of y."""
+    L, pre = _step_lib(which)
+    yy = np.ascontiguousarray(y, dtype=np.float64).copy()
+    getattr(L, pre + "clamp_nodes_y")(_ptr(yy), yy.size, int(ny))
+    return yy
+
+
+def recirculate_x(x, y, off, nx, which="oracle"):
+    """The step's capsule recirculation in x on a copy of x: (x', capsules moved)."""
+    L, pre = _step_lib(which)
+    xx = np.ascontiguousarray(x, dtype=np.float64).copy()
+    yy = np.ascontiguousarray(y, dtype=np.float64)
+    oo = np.ascontiguousarray(off, dtype=np.int32)
+    n = getattr(L, pre + "recirculate_x")(_ptr(xx), _ptr(yy), oo.ctypes.data_as(C.POINTER(C.c_int)),
+                                          oo.size - 1, int(nx))
+    return xx, n
 
 
 def default_params(**kw) -> Params:

@@ -81,7 +81,55 @@ static SimulationParams to_params(const sfo_params* p) {
     s.fluid.inlet_velocity = p->inlet_velocity;
     s.fluid.outlet_density = p->outlet_density;
     s.fluid.use_interpolated_bb = p->use_interpolated_bb != 0;
+    s.fluid.capsule_periodic_x = p->capsule_periodic_x != 0;
     return s;
+}
+
+// Simulation::step 6b (simulation.cpp:588-609), restated: simulation.cpp is
+// not compiled here.
+static int cap_forces(Real* Fx, Real* Fy, int N, Real Fxy_max) {
+    const Real Fxy_max2 = Fxy_max * Fxy_max;
+    int capped = 0;
+    for (int n = 0; n < N; ++n) {
+        Real fx = Fx[n], fy = Fy[n];
+        Real fmag2 = fx * fx + fy * fy;
+        if (fmag2 > Fxy_max2) {
+            Real scale = Fxy_max / std::sqrt(fmag2);
+            Fx[n] *= scale;
+            Fy[n] *= scale;
+            ++capped;
+        }
+    }
+    return capped;
+}
+
+// Simulation::step 10 (simulation.cpp:665-675), restated.
+static void clamp_y(CapsuleSystem& caps, int ny) {
+    Real y_min = 1.5;
+    Real y_max = static_cast<Real>(ny) - 1.5;
+    for (int c = 0; c < caps.numCapsules(); ++c)
+        for (auto& p : caps[c].positions()) {
+            if (p.y < y_min) p.y = y_min;
+            if (p.y > y_max) p.y = y_max;
+        }
+}
+
+// Simulation::step 10a (simulation.cpp:687-700), restated over the
+// reference's own Capsule::centroid.
+static int recirculate_x(CapsuleSystem& caps, int nx) {
+    const Real Lx = static_cast<Real>(nx);
+    int moved = 0;
+    for (int c = 0; c < caps.numCapsules(); ++c) {
+        const Vec2d cen = caps[c].centroid();
+        Real shift = 0.0;
+        if (cen.x >= Lx) shift = -Lx;
+        else if (cen.x < 0.0) shift = Lx;
+        if (shift != 0.0) {
+            for (auto& p : caps[c].positions()) p.x += shift;
+            ++moved;
+        }
+    }
+    return moved;
 }
 
 static MembraneParams to_mem(const sfo_membrane* m) {
@@ -290,32 +338,14 @@ void sfr_step(sfr_sim* s, int nsteps) {
                         field.FxData()[n] += rho * gx; field.FyData()[n] += rho * gy;
                     }
                 }
-        {
-            const Real Fm = P.max_lattice_force, Fm2 = Fm * Fm;
-            int capped = 0;
-            for (int n = 0; n < N; ++n) {
-                Real fx = field.FxData()[n], fy = field.FyData()[n];
-                Real m2 = fx * fx + fy * fy;
-                if (m2 > Fm2) {
-                    Real sc = Fm / std::sqrt(m2);
-                    field.FxData()[n] *= sc; field.FyData()[n] *= sc;
-                    ++capped;
-                }
-            }
-            s->capped = capped;
-        }
+        s->capped = cap_forces(field.FxData(), field.FyData(), N, P.max_lattice_force);
         s->lbm->step();
         if (s->adr) s->adr->step(field);
         s->ibm.interpolateVelocity(field, s->caps);
         s->caps.moveAllNodes(P.dt);
-        if (!P.fluid.periodic_y) {
-            Real ymin = 1.5, ymax = static_cast<Real>(P.ny) - 1.5;
-            for (int c = 0; c < s->caps.numCapsules(); ++c)
-                for (auto& p : s->caps[c].positions()) {
-                    if (p.y < ymin) p.y = ymin;
-                    if (p.y > ymax) p.y = ymax;
-                }
-        }
+        if (!P.fluid.periodic_y) clamp_y(s->caps, P.ny);
+        if (P.fluid.capsule_periodic_x && P.fluid.boundary_type != BoundaryType::PERIODIC)
+            recirculate_x(s->caps, P.nx);
     }
 }
 
@@ -384,6 +414,38 @@ void sfr_get_scalar(const sfr_sim* s, double* C, int sp) {
     if (s->adr) std::memcpy(C, s->adr->concentrationData(sp), static_cast<size_t>(s->params.nx) * s->params.ny * 8);
 }
 int sfr_last_capped(const sfr_sim* s) { return s->capped; }
+
+int sfr_cap_forces(double* Fx, double* Fy, int N, double max_force) {
+    return cap_forces(Fx, Fy, N, max_force);
+}
+
+void sfr_clamp_nodes_y(double* y, int K, int ny) {
+    // one capsule holding the K nodes, clamped by the step's restatement
+    CapsuleSystem cs;
+    MembraneParams mp;
+    cs.addCapsule(Vec2d{0.0, 0.0}, 1.0, K > 2 ? K : 3, mp);
+    auto& pos = cs[0].positions();
+    for (int k = 0; k < K; ++k) pos[k] = Vec2d{0.0, y[k]};
+    clamp_y(cs, ny);
+    for (int k = 0; k < K; ++k) y[k] = pos[k].y;
+}
+
+int sfr_recirculate_x(double* x, const double* y, const int* off, int ncaps, int nx) {
+    CapsuleSystem cs;
+    MembraneParams mp;
+    for (int c = 0; c < ncaps; ++c) {
+        const int n = off[c + 1] - off[c];
+        cs.addCapsule(Vec2d{0.0, 0.0}, 1.0, n, mp);
+        auto& pos = cs[c].positions();
+        for (int i = 0; i < n; ++i) pos[i] = Vec2d{x[off[c] + i], y[off[c] + i]};
+    }
+    const int moved = recirculate_x(cs, nx);
+    for (int c = 0; c < ncaps; ++c) {
+        auto& pos = cs[c].positions();
+        for (int i = 0; i < off[c + 1] - off[c]; ++i) x[off[c] + i] = pos[i].x;
+    }
+    return moved;
+}
 double sfr_capsule_geometry(const sfr_sim* s, int c, int what) {
     const Capsule& cp = s->caps[c];
     switch (what) {

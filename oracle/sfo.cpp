@@ -1438,20 +1438,7 @@ void sfo_step(sfo_sim* s, int nsteps) {
                 if (hg) { double r = s->rho[n]; s->Fx[n] += r * p.gravity_x; s->Fy[n] += r * p.gravity_y; }
             }
         }
-        {                                                           // :588-609
-            const double Fm = p.max_lattice_force, Fm2 = Fm * Fm;
-            int capped = 0;
-            for (int n = 0; n < s->N; ++n) {
-                double fx = s->Fx[n], fy = s->Fy[n];
-                double m2 = fx * fx + fy * fy;
-                if (m2 > Fm2) {
-                    double sc = Fm / std::sqrt(m2);
-                    s->Fx[n] *= sc; s->Fy[n] *= sc;
-                    ++capped;
-                }
-            }
-            s->capped = capped;
-        }
+        s->capped = sfo_cap_forces(s->Fx.data(), s->Fy.data(), s->N, p.max_lattice_force);   // :588-609
         lbm_step(*s);                                               // :627-629
         if (s->nsp) adr_step(*s);                                   // :637-654
         for (int cap = 0; cap < c.n(); ++cap) {                     // :657-659
@@ -1461,15 +1448,61 @@ void sfo_step(sfo_sim* s, int nsteps) {
             }
         }
         for (int cap = 0; cap < c.n(); ++cap) move_capsule(c, cap, p.dt);   // :662
-        if (!p.periodic_y) {                                        // :665-675
-            double ymin = 1.5, ymax = static_cast<double>(p.ny) - 1.5;
-            for (size_t k = 0; k < c.y.size(); ++k) {
-                if (c.y[k] < ymin) c.y[k] = ymin;
-                if (c.y[k] > ymax) c.y[k] = ymax;
-            }
-        }
+        if (!p.periodic_y)                                          // :665-675
+            sfo_clamp_nodes_y(c.y.data(), static_cast<int>(c.y.size()), p.ny);
+        if (p.capsule_periodic_x && p.boundary_type != 1)           // :687-700
+            sfo_recirculate_x(c.x.data(), c.y.data(), c.off.data(), c.n(), p.nx);
         s->step_count++;
     }
+}
+
+// Simulation::step 6b, simulation.cpp:588-609: every cell (any type) whose
+// |F|^2 exceeds max^2 is scaled onto |F| = max; the count is reported.
+int sfo_cap_forces(double* Fx, double* Fy, int N, double max_force) {
+    const double Fm = max_force, Fm2 = Fm * Fm;
+    int capped = 0;
+    for (int n = 0; n < N; ++n) {
+        double fx = Fx[n], fy = Fy[n];
+        double m2 = fx * fx + fy * fy;
+        if (m2 > Fm2) {
+            double sc = Fm / std::sqrt(m2);
+            Fx[n] *= sc; Fy[n] *= sc;
+            ++capped;
+        }
+    }
+    return capped;
+}
+
+// Simulation::step 10, simulation.cpp:665-675 (walls in y only).
+void sfo_clamp_nodes_y(double* y, int K, int ny) {
+    const double ymin = 1.5, ymax = static_cast<double>(ny) - 1.5;
+    for (int k = 0; k < K; ++k) {
+        if (y[k] < ymin) y[k] = ymin;
+        if (y[k] > ymax) y[k] = ymax;
+    }
+}
+
+// Simulation::step 10a, simulation.cpp:687-700.  Over a non-periodic fluid the
+// capsules carry no periodic length, so Capsule::centroid is the plain mean
+// of the nodes (membrane/capsule.cpp:168-172).
+int sfo_recirculate_x(double* x, const double* y, const int* off, int ncaps, int nx) {
+    (void)y;
+    const double Lx = static_cast<double>(nx);
+    int moved = 0;
+    for (int c = 0; c < ncaps; ++c) {
+        const int o = off[c], n = off[c + 1] - o;
+        double sx = 0.0;
+        for (int i = 0; i < n; ++i) sx += x[o + i];
+        const double cx = sx / static_cast<double>(n);
+        double shift = 0.0;
+        if (cx >= Lx) shift = -Lx;
+        else if (cx < 0.0) shift = Lx;
+        if (shift != 0.0) {
+            for (int i = 0; i < n; ++i) x[o + i] += shift;
+            ++moved;
+        }
+    }
+    return moved;
 }
 
 int sfo_num_capsules(const sfo_sim* s) { return s->caps.n(); }

@@ -45,6 +45,8 @@ typedef struct {
     double inlet_velocity, outlet_density;
     /* Bouzidi links towards the obstacles (parameters.h:89) */
     int    use_interpolated_bb;
+    /* capsules recirculate in x over a non-periodic fluid (simulation.cpp:687-700) */
+    int    capsule_periodic_x;
 } sfo_params;
 
 typedef struct {
@@ -106,6 +108,15 @@ void   sfo_get_flags(const sfo_sim* s, int* flags);
 void   sfo_set_flags(sfo_sim* s, const int* flags);
 void   sfo_get_scalar(const sfo_sim* s, double* C, int species);
 int    sfo_last_capped(const sfo_sim* s);
+/* The step's force cap (simulation.cpp:588-609) on N cells: returns the
+ * number of cells rescaled. */
+int    sfo_cap_forces(double* Fx, double* Fy, int N, double max_force);
+/* The step's node clamp into [1.5, ny - 1.5] (simulation.cpp:665-675). */
+void   sfo_clamp_nodes_y(double* y, int K, int ny);
+/* The step's capsule recirculation in x (simulation.cpp:687-700): a capsule
+ * whose centroid (plain node mean) left [0, nx) moves rigidly by +-nx.
+ * off[ncaps+1] are the capsules' node offsets; returns the capsules moved. */
+int    sfo_recirculate_x(double* x, const double* y, const int* off, int ncaps, int nx);
 double sfo_max_speed(const sfo_sim* s);
 double sfo_capsule_geometry(const sfo_sim* s, int c, int what);
 void   sfo_rest_lengths(const sfo_sim* s, double* out);

@@ -55,6 +55,7 @@ class Params(C.Structure):
         ("top_wall_velocity", C.c_double), ("bottom_wall_velocity", C.c_double),
         ("inlet_velocity", C.c_double), ("outlet_density", C.c_double),
         ("use_interpolated_bb", C.c_int),
+        ("capsule_periodic_x", C.c_int),
     ]
 
 

@@ -504,6 +504,7 @@ void Simulation::initialize() {
     sp.mrt_se = P.fluid.mrt_se; sp.mrt_s_eps = P.fluid.mrt_s_eps; sp.mrt_sq = P.fluid.mrt_sq;
     sp.periodic_y = P.fluid.periodic_y ? 1 : 0;
     sp.max_lattice_force = P.max_lattice_force;
+    sp.capsule_periodic_x = P.fluid.capsule_periodic_x ? 1 : 0;
     sp.rep_epsilon = P.repulsion.epsilon; sp.rep_sigma = P.repulsion.sigma; sp.rep_r_cut = P.repulsion.r_cut;
     sp.rep_power = P.repulsion.power; sp.rep_damping_normal = P.repulsion.damping_normal;
     sp.rep_friction_coeff = P.repulsion.friction_coeff;

@@ -950,6 +950,25 @@ void launch_mdf_record(Ctx& c, int ncorr) {
                        collect ? 1 : 0, c.ds_cap);
 }
 
+// Capsule recirculation in x over a non-periodic fluid (Simulation::step 10a,
+// simulation.cpp:687-700): a capsule whose centroid -- the plain node mean,
+// since capsules carry no periodic length there (capsule.cpp:168-172) -- has
+// left [0, nx) moves rigidly by -nx or +nx.  One thread per capsule sums its
+// nodes in index order, as the reference's centroid does.
+__global__ void k_recirculate_x(double* x, const int* off, int ncaps, double Lx) {
+    const int cap = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cap >= ncaps) return;
+    const int o = off[cap], n = off[cap + 1] - o;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += x[o + i];
+    const double cx = s / static_cast<double>(n);
+    double shift = 0.0;
+    if (cx >= Lx) shift = -Lx;
+    else if (cx < 0.0) shift = Lx;
+    if (shift != 0.0)
+        for (int i = 0; i < n; ++i) x[o + i] += shift;
+}
+
 void launch_interp_move(Ctx& c) {
     if (c.K == 0) return;
     CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
@@ -966,6 +985,9 @@ void launch_interp_move(Ctx& c) {
         hipLaunchKernelGGL(k_move, dim3(c.ncaps), dim3(64), 0, c.stream, c.x, c.y, c.vx, c.vy, c.off,
                            c.desc, c.ncaps, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
                            1.5, static_cast<double>(c.ny) - 1.5);
+    if (c.p.capsule_periodic_x && !c.periodic_x)
+        hipLaunchKernelGGL(k_recirculate_x, dim3((c.ncaps + 255) / 256), dim3(256), 0, c.stream, c.x,
+                           c.off, c.ncaps, static_cast<double>(c.nx));
 }
 
 

@@ -890,8 +890,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     lbm_pair_grid<OP, ADRF, false>(a);
 }
 
-// Generic cell types: held to four waves, the MRT form with the scalar spills
-// 200 bytes per lane, so the allocator keeps its own count here.
+// Generic cell types: not held to four waves.  With that bound the MRT form
+// with the scalar would spill 200 bytes per lane, so the allocator keeps its
+// own count (177 VGPRs, two waves per SIMD).
 template <int OP, bool ADRF>
 __global__ __launch_bounds__(256) void k_lbm_pair_types(LbmArgs a) {
     lbm_pair_grid<OP, ADRF, true>(a);

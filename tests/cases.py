@@ -15,6 +15,8 @@ A case is a dict:
     obstacles: list of ('circle', cx, cy, r) | ('rect', x0, y0, x1, y1) |
                ('polygon', [(x, y), ...]): the channel builder's pillars   (optional)
     ibb: True for interpolated bounce-back towards them   (optional)
+    fmax: max_lattice_force, the step's force cap (default 0.01)   (optional)
+    capsule_periodic_x: True for capsule recirculation over a non-periodic fluid   (optional)
 Membrane keys follow the oracle's struct (model, G_s, C_skalak, k_bend, use_helfrich, ...).
 """
 from __future__ import annotations
@@ -67,6 +69,8 @@ def oracle_sim(case, which="oracle"):
         top_wall_velocity=case.get("walls", (0.0, 0.0))[0],
         bottom_wall_velocity=case.get("walls", (0.0, 0.0))[1],
         use_interpolated_bb=1 if case.get("ibb") else 0,
+        max_lattice_force=case.get("fmax", 0.01),
+        capsule_periodic_x=1 if case.get("capsule_periodic_x") else 0,
     )
     sim = po.Sim(p, which)
     for ob in case.get("obstacles", []):
@@ -117,6 +121,8 @@ def product_sim(case, graph=True):
     p.repulsion.friction_coeff = rep.get("friction_coeff", 0.0)
     p.fluid.top_wall_velocity, p.fluid.bottom_wall_velocity = case.get("walls", (0.0, 0.0))
     p.fluid.use_interpolated_bb = bool(case.get("ibb", False))
+    p.max_lattice_force = case.get("fmax", 0.01)
+    p.fluid.capsule_periodic_x = bool(case.get("capsule_periodic_x", False))
     if s:
         p.scalar.enabled = True
         p.scalar.diffusivity = [s["D"]]
@@ -247,4 +253,28 @@ OBSTACLE_CASES = dict(
                          capsules=[dict(centers=[(40.3, 29.6), (74.0, 21.5), (109.5, 34.0)],
                                         rmin=4.0, nodes=26, mem=dict(model=2, G_s=0.06, C_skalak=10.0))],
                          rep=dict(damping_normal=0.01, friction_coeff=0.1)),
+)
+
+
+# Cases that drive the three operations of Simulation::step itself: the force
+# cap fires in every step (capsules pressed into both walls), nodes are pushed
+# past the wall lines and clamped, and capsules recirculate in x over a
+# non-periodic fluid (inlet/outlet, closed box).  Run by the oracle-vs-
+# reference, golden and device parity tests.
+_SK = dict(model=2, G_s=0.06, C_skalak=10.0)
+STEP_PIN_CASES = dict(
+    walls_capped_clamped=dict(nx=48, ny=24, tau=0.8, bfx=3e-5, coll=0, steps=300,
+                              capsules=[dict(centers=[(12.0, 2.0), (30.0, 23.0)], rmin=3.5, nodes=20,
+                                             mem=_SK)]),
+    walls_capped_mrt=dict(nx=48, ny=24, tau=0.7, bfx=1e-5, coll=1, fmax=2e-3, steps=200,
+                          capsules=[dict(centers=[(10.0, 2.5), (26.0, 12.0), (40.0, 21.5)], rmin=3.5,
+                                         nodes=20, mem=_SK)]),
+    inlet_outlet_recirculate=dict(nx=40, ny=20, tau=0.8, bfx=0.0, coll=0, boundary="inlet_outlet",
+                                  inlet=(0.05, 1.0), capsule_periodic_x=True, steps=100,
+                                  capsules=[dict(centers=[(37.5, 10.0), (-0.05, 10.0), (20.0, 10.0)],
+                                                 rmin=3.0, nodes=16, mem=_SK)]),
+    closed_recirculate=dict(nx=40, ny=20, tau=0.8, bfx=3e-5, coll=2, boundary="closed",
+                            capsule_periodic_x=True, steps=60,
+                            capsules=[dict(centers=[(40.02, 10.0), (-0.3, 8.0)], rmin=3.0, nodes=16,
+                                           mem=_SK)]),
 )

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from tests.cases import oracle_sim, oracle_state  # noqa: E402
+from tests.cases import STEP_PIN_CASES, oracle_sim, oracle_state  # noqa: E402
 
 SKALAK = dict(model=2, G_s=0.06, C_skalak=10.0)
 
@@ -61,6 +61,11 @@ CASES = {
                           obstacles=[("rect", 10.0, 2.3, 30.0, 6.0), ("circle", 40.5, 5.0, 1.6),
                                      ("circle", 1.5, 4.5, 1.2)]), 400),
 }
+# the step's own operations: force cap firing at the last step, nodes clamped
+# at the wall lines, capsules recirculating over a non-periodic fluid
+for _name, _c in STEP_PIN_CASES.items():
+    _c = dict(_c)
+    CASES[_name] = (_c, _c.pop("steps"))
 
 
 # The reference's writers (io/*_writer.cpp) on reference-component runs: each

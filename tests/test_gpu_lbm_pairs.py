@@ -37,6 +37,12 @@ CASES = {
     "solids_bgk_periodic_y_scalar": dict(nx=64, ny=32, coll=0, bfx=1e-5, periodic_y=True, precision="fp32",
                                          solids=[("circle", 31.0, 16.0, 5.0), ("rect", 10, 4, 10, 27)],
                                          scalar=SCALAR | dict(blob=(45, 16, 4.0))),
+    # the regularized and MDF (partial shift force) forms with per-cell types,
+    # which the autotuner may select in production
+    "pillars_ibb_reg_caps": dict(nx=130, ny=40, coll=2, bfx=2e-5, precision="fp32", obstacles=PILLARS,
+                                 ibb=True, capsules=CAPS),
+    "pillars_reg_mdf3_caps_scalar": dict(nx=130, ny=40, coll=2, bfx=2e-5, mdf=3, precision="fp32",
+                                         obstacles=PILLARS[1:], capsules=CAPS, scalar=SCALAR),
 }
 
 # (bx, by, xcd_banded, persistent workgroups per XCD)

@@ -242,3 +242,39 @@ def test_fp32_error_report():
     print(f"fp32 vs fp64 after 2000 steps: velocity rel L2 {eu:.3e}, centroid max |dx| {ecen:.3e}, "
           f"Taylor D max |dD| {eD:.3e}", flush=True)
     assert eu < 1e-3 and ecen < 1e-2 and eD < 1e-3
+
+
+# The operations of Simulation::step itself on the device: the force cap
+# (firing in every step, wall cells included), the node clamp at the wall
+# lines and capsule recirculation in x over a non-periodic fluid.  Compared
+# with the oracle after the first step, the third and the last, FP64.
+from tests.cases import STEP_PIN_CASES  # noqa: E402
+
+
+@pytest.mark.parametrize("name", sorted(STEP_PIN_CASES))
+def test_step_operations_match_oracle(name):
+    case = dict(STEP_PIN_CASES[name], name=name)
+    last = case.pop("steps")
+    o = oracle_sim(case)
+    sim = product_sim(case)
+    done = 0
+    for upto in (1, 3, last):
+        print(f"{name} to step {upto}", flush=True)
+        o.step(upto - done)
+        sim.run(upto - done)
+        done = upto
+        ref, got = oracle_state(o), product_state(sim)
+        worst = metrics(got, ref)
+        for k, v in worst.items():
+            assert v <= 1e-12, f"{name} step {upto}: {k} rel L2 {v:.3e}"
+        assert got["capped"] == ref["capped"], (upto, got["capped"], ref["capped"])
+        lo, hi = 1.5, case["ny"] - 1.5
+        assert np.sum(got["y"] == lo) == np.sum(ref["y"] == lo)
+        assert np.sum(got["y"] == hi) == np.sum(ref["y"] == hi)
+        if "walls" in name:
+            assert got["capped"] > 0
+            if upto == 1:
+                assert np.sum(got["y"] == lo) > 0 and np.sum(got["y"] == hi) > 0
+        if "recirculate" in name:
+            # the device moved the same capsules across the seam as the oracle
+            assert np.array_equal(np.floor(got["x"] / case["nx"]), np.floor(ref["x"] / case["nx"]))

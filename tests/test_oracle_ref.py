@@ -165,3 +165,27 @@ def test_obstacle_geometry_equals_reference():
                 da, na = a.obstacle_distance(i, x, y)
                 db, nb = b.obstacle_distance(i, x, y)
                 assert da == db and np.array_equal(na, nb), (i, x, y)
+
+
+# The step operations of Simulation::step itself (force cap, node clamp,
+# capsule recirculation), restated in both libraries, on runs that exercise
+# them; see tests/test_oracle_step_pins.py for their closed-form pins.
+from tests.cases import STEP_PIN_CASES  # noqa: E402
+
+
+@pytest.mark.parametrize("name", sorted(STEP_PIN_CASES))
+def test_step_pin_cases_equal_reference(name):
+    from tests.cases import oracle_sim, oracle_state
+    case = dict(STEP_PIN_CASES[name])
+    steps = case.pop("steps")
+    a = oracle_sim(case)
+    b = oracle_sim(case, which="ref")
+    a.step(steps)
+    b.step(steps)
+    sa, sb = oracle_state(a), oracle_state(b)
+    for k in sa:
+        assert np.array_equal(np.asarray(sa[k]), np.asarray(sb[k])), k
+    assert np.array_equal(a.f(), b.f())
+    if "walls" in name:
+        assert sa["capped"] > 0
+        assert np.any(sa["y"] == 1.5) or np.any(sa["y"] == case["ny"] - 1.5)
