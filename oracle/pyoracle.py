@@ -114,6 +114,7 @@ def _lib(path: str, prefix: str):
     g("add_obstacle").argtypes = [C.c_void_p, C.c_int, _DP, C.c_int]
     g("obstacle_distance").restype = C.c_double
     g("obstacle_distance").argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _DP]
+    g("set_concentration").argtypes = [C.c_void_p, _DP, C.c_int]
     g("cap_forces").restype = C.c_int
     g("cap_forces").argtypes = [_DP, _DP, C.c_int, C.c_double]
     g("clamp_nodes_y").argtypes = [_DP, C.c_int, C.c_int]
@@ -374,6 +375,11 @@ class Sim:
         a = np.zeros(self.nx * self.ny)
         self._f("get_scalar")(self.h, _ptr(a), sp)
         return a.reshape(self.ny, self.nx)
+
+    def set_concentration(self, C_, sp=0):
+        a = np.ascontiguousarray(C_, dtype=np.float64).ravel()
+        assert a.size == self.nx * self.ny
+        self._f("set_concentration")(self.h, _ptr(a), sp)
 
     def last_capped(self):
         return self._f("last_capped")(self.h)

@@ -415,6 +415,13 @@ void sfr_get_scalar(const sfr_sim* s, double* C, int sp) {
 }
 int sfr_last_capped(const sfr_sim* s) { return s->capped; }
 
+// a write through the reference's concentration() view: its own array
+void sfr_set_concentration(sfr_sim* s, const double* C, int sp) {
+    if (!s->adr || sp < 0 || sp >= s->params.scalar.n_species) return;
+    std::memcpy(s->adr->concentrationData(sp), C,
+                sizeof(Real) * static_cast<size_t>(s->params.nx) * s->params.ny);
+}
+
 int sfr_cap_forces(double* Fx, double* Fy, int N, double max_force) {
     return cap_forces(Fx, Fy, N, max_force);
 }

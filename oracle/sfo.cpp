@@ -1558,6 +1558,9 @@ void sfo_set_flags(sfo_sim* s, const int* flags) {
 void sfo_get_scalar(const sfo_sim* s, double* C, int sp) {
     if (sp >= 0 && sp < s->nsp) std::memcpy(C, s->C[sp].data(), s->N * 8);
 }
+void sfo_set_concentration(sfo_sim* s, const double* C, int sp) {
+    if (sp >= 0 && sp < s->nsp) std::memcpy(s->C[sp].data(), C, s->N * 8);
+}
 int sfo_last_capped(const sfo_sim* s) { return s->capped; }
 
 double sfo_max_speed(const sfo_sim* s) {                        // simulation.cpp:892-907

@@ -107,6 +107,10 @@ void   sfo_set_macro(sfo_sim* s, const double* rho, const double* ux,
 void   sfo_get_flags(const sfo_sim* s, int* flags);
 void   sfo_set_flags(sfo_sim* s, const int* flags);
 void   sfo_get_scalar(const sfo_sim* s, double* C, int species);
+/* Overwrite the concentration array only (what a write through the
+ * reference's concentration() view does, python/bindings.cpp:435-440): the
+ * next collision relaxes g towards w C (1 + 3 e.u) with it. */
+void   sfo_set_concentration(sfo_sim* s, const double* C, int species);
 int    sfo_last_capped(const sfo_sim* s);
 /* The step's force cap (simulation.cpp:588-609) on N cells: returns the
  * number of cells rescaled. */

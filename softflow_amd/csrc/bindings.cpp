@@ -1,9 +1,10 @@
 // pybind11 module softflow_core: the reference's Python surface
 // (python/bindings.cpp) over the MI355X engine.
 //
-// Field views are zero-copy numpy arrays over the host mirror, refreshed from
-// the device when the view is requested; they are read-only because a write
-// into them would not reach the device (use the setters).
+// Field views are zero-copy, writable numpy arrays over the host mirror.
+// While a view is alive it holds a lease on its array: the mirror is refreshed
+// after every step, so the view shows the current state, and writes into it
+// reach the device before the next step (LatticeField::leaseView).
 #include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -16,13 +17,33 @@ using namespace softflow;
 
 namespace {
 
-py::array view2d(Real* data, int ny, int nx, py::handle owner) {
-    py::array a(py::dtype::of<Real>(), {ny, nx},
-                {static_cast<py::ssize_t>(nx) * static_cast<py::ssize_t>(sizeof(Real)),
-                 static_cast<py::ssize_t>(sizeof(Real))},
-                data, owner);
-    py::detail::array_proxy(a.ptr())->flags &= ~py::detail::npy_api::NPY_ARRAY_WRITEABLE_;
-    return a;
+// The lease a view holds on one of the field's arrays; released when the
+// last array derived from the view is gone.
+
+template <class Owner>
+py::capsule lease(Owner& f, int kind, Real** data) {
+    *data = f.leaseView(kind);
+    if (!*data) throw py::index_error("no such array");
+    struct Lease {
+        Owner* owner;
+        int kind;
+    };
+    return py::capsule(new Lease{&f, kind}, [](void* p) {
+        auto* l = static_cast<Lease*>(p);
+        l->owner->releaseView(l->kind);
+        delete l;
+    });
+}
+
+template <class Owner>
+py::array view2d(Owner& f, int kind) {
+    Real* data = nullptr;
+    py::capsule owner = lease(f, kind, &data);
+    const int ny = f.getNy(), nx = f.getNx();
+    return py::array(py::dtype::of<Real>(), {ny, nx},
+                     {static_cast<py::ssize_t>(nx) * static_cast<py::ssize_t>(sizeof(Real)),
+                      static_cast<py::ssize_t>(sizeof(Real))},
+                     data, owner);
 }
 
 // LBMSolver handle exposing field() as the reference does (bindings.cpp:456-475)
@@ -136,7 +157,38 @@ PYBIND11_MODULE(softflow_core, m) {
         .def_readwrite("h_min", &LubricationParams::h_min);
     py::class_<AdhesionParams>(m, "AdhesionParams")
         .def(py::init<>())
-        .def_readwrite("enabled", &AdhesionParams::enabled);
+        .def_readwrite("enabled", &AdhesionParams::enabled)
+        .def_readwrite("k_on", &AdhesionParams::k_on)
+        .def_readwrite("k_off", &AdhesionParams::k_off)
+        .def_readwrite("k_bond", &AdhesionParams::k_bond)
+        .def_readwrite("d_bond", &AdhesionParams::d_bond)
+        .def_readwrite("F_crit", &AdhesionParams::F_crit)
+        .def_readwrite("max_bonds_per_node", &AdhesionParams::max_bonds_per_node)
+        .def_readwrite("use_catch_slip", &AdhesionParams::use_catch_slip)
+        .def_readwrite("k_off_catch", &AdhesionParams::k_off_catch)
+        .def_readwrite("F_catch", &AdhesionParams::F_catch)
+        .def_readwrite("k_off_slip", &AdhesionParams::k_off_slip)
+        .def_readwrite("F_slip", &AdhesionParams::F_slip)
+        .def_readwrite("wall_adhesion", &AdhesionParams::wall_adhesion)
+        .def_readwrite("wall_k_on", &AdhesionParams::wall_k_on)
+        .def_readwrite("wall_k_off", &AdhesionParams::wall_k_off)
+        .def_readwrite("wall_k_bond", &AdhesionParams::wall_k_bond)
+        .def_readwrite("wall_receptor_spacing", &AdhesionParams::wall_receptor_spacing)
+        .def_readwrite("adhesion_matrix", &AdhesionParams::adhesion_matrix);
+    py::class_<ShanChenParams>(m, "ShanChenParams")
+        .def(py::init<>())
+        .def_readwrite("enabled", &ShanChenParams::enabled)
+        .def_readwrite("G", &ShanChenParams::G)
+        .def_readwrite("rho0_sc", &ShanChenParams::rho0_sc)
+        .def_readwrite("eos_type", &ShanChenParams::eos_type)
+        .def_readwrite("cs_a", &ShanChenParams::cs_a)
+        .def_readwrite("cs_b", &ShanChenParams::cs_b)
+        .def_readwrite("cs_T", &ShanChenParams::cs_T)
+        .def_readwrite("cs_R", &ShanChenParams::cs_R)
+        .def_readwrite("wetting_angle", &ShanChenParams::wetting_angle)
+        .def_readwrite("n_components", &ShanChenParams::n_components)
+        .def_readwrite("G_12", &ShanChenParams::G_12)
+        .def_readwrite("tau_2", &ShanChenParams::tau_2);
 
     py::class_<ScalarParams>(m, "ScalarParams")
         .def(py::init<>())
@@ -243,6 +295,8 @@ PYBIND11_MODULE(softflow_core, m) {
         .def_readwrite("repulsion", &SimulationParams::repulsion)
         .def_readwrite("lubrication", &SimulationParams::lubrication)
         .def_readwrite("adhesion", &SimulationParams::adhesion)
+        .def_readwrite("shan_chen", &SimulationParams::shan_chen)
+        .def_readwrite("output_format", &SimulationParams::output_format)
         .def_readwrite("scalar", &SimulationParams::scalar)
         .def_readwrite("output_dir", &SimulationParams::output_dir)
         .def_readwrite("vtk_format", &SimulationParams::vtk_format)
@@ -268,34 +322,23 @@ PYBIND11_MODULE(softflow_core, m) {
         .def("getNx", &LatticeField::getNx)
         .def("getNy", &LatticeField::getNy)
         .def("size", &LatticeField::size)
-        .def("density", [](py::object self) {
-            auto& f = self.cast<LatticeField&>();
-            return view2d(f.rhoData(), f.getNy(), f.getNx(), self);
-        })
-        .def("velocity_x", [](py::object self) {
-            auto& f = self.cast<LatticeField&>();
-            return view2d(f.uxData(), f.getNy(), f.getNx(), self);
-        })
-        .def("velocity_y", [](py::object self) {
-            auto& f = self.cast<LatticeField&>();
-            return view2d(f.uyData(), f.getNy(), f.getNx(), self);
-        })
-        .def("force_x", [](py::object self) {
-            auto& f = self.cast<LatticeField&>();
-            return view2d(f.FxData(), f.getNy(), f.getNx(), self);
-        })
-        .def("force_y", [](py::object self) {
-            auto& f = self.cast<LatticeField&>();
-            return view2d(f.FyData(), f.getNy(), f.getNx(), self);
-        })
-        .def("distributions", [](py::object self) {
-            auto& f = self.cast<LatticeField&>();
+        .def("density", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_RHO); },
+             py::keep_alive<0, 1>())
+        .def("velocity_x", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_UX); },
+             py::keep_alive<0, 1>())
+        .def("velocity_y", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_UY); },
+             py::keep_alive<0, 1>())
+        .def("force_x", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_FX); },
+             py::keep_alive<0, 1>())
+        .def("force_y", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_FY); },
+             py::keep_alive<0, 1>())
+        .def("distributions", [](LatticeField& f) {
+            Real* data = nullptr;
+            py::capsule owner = lease(f, LatticeField::VIEW_F, &data);
             const py::ssize_t N = f.size(), nx = f.getNx(), s = sizeof(Real);
-            py::array a(py::dtype::of<Real>(), {static_cast<py::ssize_t>(9), static_cast<py::ssize_t>(f.getNy()), nx},
-                        {N * s, nx * s, s}, f.fData(), self);
-            py::detail::array_proxy(a.ptr())->flags &= ~py::detail::npy_api::NPY_ARRAY_WRITEABLE_;
-            return a;
-        })
+            return py::array(py::dtype::of<Real>(), {static_cast<py::ssize_t>(9), static_cast<py::ssize_t>(f.getNy()), nx},
+                             {N * s, nx * s, s}, data, owner);
+        }, py::keep_alive<0, 1>())
         .def("cell_types", [](LatticeField& f) {
             py::array_t<int> a({f.getNy(), f.getNx()});
             std::memcpy(a.mutable_data(), f.cellTypeData(), sizeof(int) * f.size());
@@ -360,12 +403,10 @@ PYBIND11_MODULE(softflow_core, m) {
              py::arg("concentration"), py::arg("species") = 0)
         .def("getConcentration", &AdvectionDiffusion::getConcentration, py::arg("x"), py::arg("y"),
              py::arg("species") = 0)
-        .def("concentration", [](py::object self, int species) {
-            auto& a = self.cast<AdvectionDiffusion&>();
-            Real* d = a.concentrationData(species);
-            if (!d) throw py::index_error("no such species");
-            return view2d(d, a.getNy(), a.getNx(), self);
-        }, py::arg("species") = 0)
+        .def("concentration", [](AdvectionDiffusion& a, int species) {
+            if (species < 0 || species >= a.getNumSpecies()) throw py::index_error("no such species");
+            return view2d(a, species);
+        }, py::arg("species") = 0, py::keep_alive<0, 1>())
         .def("getNumSpecies", &AdvectionDiffusion::getNumSpecies)
         .def("setField", [](AdvectionDiffusion& a, py::array_t<Real, py::array::c_style | py::array::forcecast> C,
                             int species) {

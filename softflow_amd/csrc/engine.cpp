@@ -35,7 +35,64 @@ LatticeField::LatticeField(int nx, int ny)
       flags_(rho_.size(), static_cast<int>(CellType::FLUID)) {}
 
 void LatticeField::pull() {
-    if (stale_ && sim_) sim_->pullField(*this);
+    if (stale_ && sim_) {
+        sim_->pullField(*this);
+        refreshShadows();
+    }
+}
+
+std::vector<Real>& LatticeField::viewArray(int kind) {
+    switch (kind) {
+    case VIEW_F: return f_;
+    case VIEW_RHO: return rho_;
+    case VIEW_UX: return ux_;
+    case VIEW_UY: return uy_;
+    case VIEW_FX: return Fx_;
+    case VIEW_FY: return Fy_;
+    default: throw std::invalid_argument("LatticeField: unknown view");
+    }
+}
+
+Real* LatticeField::leaseView(int kind) {
+    std::vector<Real>& a = viewArray(kind);
+    pull();
+    if (leases_[kind]++ == 0) shadow_[kind] = a;
+    return a.data();
+}
+
+void LatticeField::releaseView(int kind) {
+    if (kind < 0 || kind >= VIEW_COUNT || leases_[kind] == 0) return;
+    // writes made through the last view of this array still count
+    if (!stale_ && viewArray(kind) != shadow_[kind]) {
+        if (kind == VIEW_F) f_dirty_ = true;
+        else if (kind <= VIEW_UY) macro_dirty_ = true;
+    }
+    if (--leases_[kind] == 0) std::vector<Real>().swap(shadow_[kind]);
+}
+
+bool LatticeField::anyLease() const {
+    for (int k = 0; k < VIEW_COUNT; ++k)
+        if (leases_[k]) return true;
+    return false;
+}
+
+// Writes through leased views since the last step or refresh (the mirror is
+// current whenever a lease is held, see Simulation::afterAdvance).
+void LatticeField::collectViewWrites() {
+    if (stale_) return;
+    for (int k = 0; k < VIEW_COUNT; ++k) {
+        if (!leases_[k]) continue;
+        std::vector<Real>& a = viewArray(k);
+        if (a == shadow_[k]) continue;
+        if (k == VIEW_F) f_dirty_ = true;
+        else if (k <= VIEW_UY) macro_dirty_ = true;
+        shadow_[k] = a;
+    }
+}
+
+void LatticeField::refreshShadows() {
+    for (int k = 0; k < VIEW_COUNT; ++k)
+        if (leases_[k]) shadow_[k] = viewArray(k);
 }
 void LatticeField::set_f(int x, int y, int q, Real v) {
     touch_f();
@@ -298,7 +355,46 @@ AdvectionDiffusion::AdvectionDiffusion(int nx, int ny, const ScalarParams& p)
     C_.assign(nsp_, std::vector<Real>(static_cast<size_t>(nx) * ny, 0.0));
 }
 void AdvectionDiffusion::pull() {
-    if (stale_ && sim_) sim_->pullScalar(*this);
+    if (stale_ && sim_) {
+        sim_->pullScalar(*this);
+        refreshShadows();
+    }
+}
+
+Real* AdvectionDiffusion::leaseView(int species) {
+    if (species < 0 || species >= nsp_) return nullptr;
+    pull();
+    if (leases_.empty()) { leases_.assign(nsp_, 0); shadow_.assign(nsp_, {}); }
+    if (leases_[species]++ == 0) shadow_[species] = C_[species];
+    return C_[species].data();
+}
+
+void AdvectionDiffusion::releaseView(int species) {
+    if (leases_.empty() || species < 0 || species >= nsp_ || leases_[species] == 0) return;
+    if (!stale_ && C_[species] != shadow_[species]) dirty_ = true;
+    if (--leases_[species] == 0) std::vector<Real>().swap(shadow_[species]);
+}
+
+bool AdvectionDiffusion::anyLease() const {
+    for (int n : leases_)
+        if (n) return true;
+    return false;
+}
+
+// A write into a concentration view sets C; the populations keep theirs (the
+// reference's collision relaxes g towards w C(1 + 3 e.u) with the array's C).
+void AdvectionDiffusion::collectViewWrites() {
+    if (stale_) return;
+    for (size_t s = 0; s < leases_.size(); ++s) {
+        if (!leases_[s] || C_[s] == shadow_[s]) continue;
+        dirty_ = true;
+        shadow_[s] = C_[s];
+    }
+}
+
+void AdvectionDiffusion::refreshShadows() {
+    for (size_t s = 0; s < leases_.size(); ++s)
+        if (leases_[s]) shadow_[s] = C_[s];
 }
 void AdvectionDiffusion::initialize(Real c0) {        // advection_diffusion.cpp:46-57
     stale_ = false;
@@ -488,6 +584,8 @@ void Simulation::initialize() {
     // features of SimulationParams outside the device path
     if ((P.fluid.gravity_x != 0.0 || P.fluid.gravity_y != 0.0))
         throw std::invalid_argument("Simulation: gravity is not implemented on the MI355X path");
+    if (P.shan_chen.enabled)
+        throw std::invalid_argument("Simulation: Shan-Chen multiphase is not implemented on the MI355X path");
     if (P.lubrication.enabled || P.adhesion.enabled)
         throw std::invalid_argument("Simulation: lubrication/adhesion are not implemented on the MI355X path");
 
@@ -589,6 +687,8 @@ void Simulation::initialize() {
 void Simulation::pushDirty() {
     if (!ctx_) throw std::runtime_error("Simulation: initialize() has not been called");
     LatticeField& F = *field_;
+    F.collectViewWrites();
+    if (adr_) adr_->collectViewWrites();
     if (F.flags_dirty_) { check(sf_set_flags(ctx_, F.flags_.data()), "sf_set_flags"); F.flags_dirty_ = false; }
     if (F.f_dirty_) { check(sf_set_populations(ctx_, F.f_.data()), "sf_set_populations"); }
     if (F.f_dirty_ || F.macro_dirty_) {
@@ -669,12 +769,20 @@ void Simulation::pullScalar(AdvectionDiffusion& a) {
     for (int s = 0; s < a.nsp_; ++s) check(sf_get_scalar(ctx_, s, a.C_[s].data(), a.g_[s].data()), "sf_get_scalar");
 }
 
-void Simulation::step() {
-    pushDirty();
-    check(sf_step(ctx_, 1), "sf_step");
+// After the device advanced: host mirrors are stale; a held field view is
+// refreshed at once so it keeps showing the live state.
+void Simulation::afterAdvance() {
     field_->stale_ = true;
     caps_.stale_ = true;
     if (adr_) adr_->stale_ = true;
+    if (field_->anyLease()) field_->pull();
+    if (adr_ && adr_->anyLease()) adr_->pull();
+}
+
+void Simulation::step() {
+    pushDirty();
+    check(sf_step(ctx_, 1), "sf_step");
+    afterAdvance();
     if (params_.enable_stability_checks && params_.stability_check_interval > 0 &&
         step_count_ % params_.stability_check_interval == 0)
         checkStability();
@@ -685,9 +793,7 @@ void Simulation::step() {
 }
 
 void Simulation::markAdvanced(int steps) {
-    field_->stale_ = true;
-    caps_.stale_ = true;
-    if (adr_) adr_->stale_ = true;
+    afterAdvance();
     step_count_ += steps;
 }
 
@@ -719,9 +825,7 @@ void Simulation::run(int num_steps) {
         if (due >= 0) chunk = std::min(chunk, static_cast<int>(due - step_count_ + 1));
         pushDirty();
         check(sf_step(ctx_, chunk), "sf_step");
-        field_->stale_ = true;
-        caps_.stale_ = true;
-        if (adr_) adr_->stale_ = true;
+        afterAdvance();
         step_count_ += chunk;
         if (checks && (step_count_ - 1) % iv == 0) checkStability();
         output(step_count_ - 1);

@@ -81,7 +81,27 @@ struct RepulsionParams {
 };
 
 struct LubricationParams { bool enabled = false; Real h_threshold = 2.0, h_min = 0.1; };
-struct AdhesionParams { bool enabled = false; };
+// Off the hot path (refused at initialize() when enabled); the fields and
+// defaults are core/parameters.h's, so a front end can fill them.
+struct AdhesionParams {
+    bool enabled = false;
+    Real k_on = 0.001, k_off = 0.01, k_bond = 0.05, d_bond = 2.0, F_crit = 0.01;
+    int max_bonds_per_node = 3;
+    bool use_catch_slip = false;
+    Real k_off_catch = 0.05, F_catch = 0.02, k_off_slip = 0.001, F_slip = 0.01;
+    bool wall_adhesion = false;
+    Real wall_k_on = 0.001, wall_k_off = 0.01, wall_k_bond = 0.05, wall_receptor_spacing = 2.0;
+    std::vector<std::vector<bool>> adhesion_matrix;
+};
+struct ShanChenParams {
+    bool enabled = false;
+    Real G = -5.0, rho0_sc = 1.0;
+    int eos_type = 0;
+    Real cs_a = 1.0, cs_b = 4.0, cs_T = 0.06, cs_R = 1.0;
+    Real wetting_angle = 90.0;
+    int n_components = 1;
+    Real G_12 = -5.0, tau_2 = 0.8;
+};
 
 struct ScalarParams {
     bool enabled = false;
@@ -101,8 +121,10 @@ struct SimulationParams {
     RepulsionParams repulsion;
     LubricationParams lubrication;
     AdhesionParams adhesion;
+    ShanChenParams shan_chen;
     ScalarParams scalar;
     std::string output_dir = "output";
+    std::string output_format = "vtk";     // "vtk" or "csv" (parameters.h)
     std::string vtk_format = "ascii";      // "ascii" (XML .vti/.vtp) or "legacy" (.vtk)
     int checkpoint_interval = 0, metrics_interval = 0;
     int vtk_dump_every = 0, csv_dump_every = 0, probe_dump_every = 10, stats_dump_every = 100;
@@ -155,17 +177,35 @@ public:
     void initializeAllEquilibrium();
     void computeMacroscopic();
 
+    // Writable numpy views (python/bindings.cpp:313-355 hands out zero-copy
+    // views of the live arrays).  Here a view aliases the host mirror; while
+    // one is held (leased), the mirror is refreshed after every step so the
+    // view shows the current state, and writes into it are found by comparison
+    // with a shadow copy and pushed to the device before the next step:
+    // populations as a host write of f, rho/ux/uy as the stored macroscopic
+    // state the next collision reads.  Writes into the force views stay on the
+    // host, as the reference's are cleared at the start of the next step.
+    enum ViewKind { VIEW_F = 0, VIEW_RHO, VIEW_UX, VIEW_UY, VIEW_FX, VIEW_FY, VIEW_COUNT };
+    Real* leaseView(int kind);
+    void releaseView(int kind);
+    bool anyLease() const;
+
 private:
     friend class Simulation;
     void pull();
     void touch_f() { pull(); f_dirty_ = true; }
     void touch_macro() { pull(); macro_dirty_ = true; }
+    std::vector<Real>& viewArray(int kind);
+    void collectViewWrites();
+    void refreshShadows();
     int nx_, ny_, N_;
     std::vector<Real> f_, rho_, ux_, uy_, Fx_, Fy_;
     std::vector<int> flags_;
     Simulation* sim_ = nullptr;
     bool stale_ = false;
     bool f_dirty_ = false, macro_dirty_ = false, flags_dirty_ = false;
+    int leases_[VIEW_COUNT] = {};
+    std::vector<Real> shadow_[VIEW_COUNT];
 };
 
 // Capsule with rest geometry built as membrane/capsule.cpp:7-141.
@@ -249,14 +289,23 @@ public:
     int getNx() const { return nx_; }
     int getNy() const { return ny_; }
     int getNumSpecies() const { return nsp_; }
+    // Writable concentration views, leased as LatticeField's are: writes reach
+    // the device (with g) before the next step.
+    Real* leaseView(int species);
+    void releaseView(int species);
+    bool anyLease() const;
 
 private:
     friend class Simulation;
     void pull();
+    void collectViewWrites();
+    void refreshShadows();
     int nx_, ny_, nsp_;
     std::vector<std::vector<Real>> g_, C_;
     Simulation* sim_ = nullptr;
     bool stale_ = false, dirty_ = true;
+    std::vector<int> leases_;
+    std::vector<std::vector<Real>> shadow_;
 };
 
 // Obstacles (geometry/obstacle.h, circle_obstacle.h, rect_obstacle.h,
@@ -459,6 +508,7 @@ private:
     friend class Ensemble;
     void pushDirty();
     void markAdvanced(int steps);   // device state moved on: host mirrors are stale
+    void afterAdvance();
     bool writeCheckpoint(const std::string& filename, int step);
     void intervalCheckpoint(int64_t step);
     void check(int rc, const char* what) const;

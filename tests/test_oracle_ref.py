@@ -189,3 +189,21 @@ def test_step_pin_cases_equal_reference(name):
     if "walls" in name:
         assert sa["capped"] > 0
         assert np.any(sa["y"] == 1.5) or np.any(sa["y"] == case["ny"] - 1.5)
+
+
+def test_concentration_write_equals_reference():
+    # a write into the concentration array between steps (what the
+    # reference's concentration() view allows): the next collision relaxes
+    # towards the written C while g keeps its values
+    from tests.cases import oracle_sim, oracle_state
+    case = dict(nx=40, ny=20, tau=0.8, bfx=3e-5, coll=1, scalar=dict(D=0.05, blob=(20, 10, 4.0)))
+    sims = [oracle_sim(case), oracle_sim(case, which="ref")]
+    for s in sims:
+        s.step(20)
+        C = s.scalar().reshape(case["ny"], case["nx"]).copy()
+        C[5:9, 10:30] += 0.25
+        s.set_concentration(C)
+        s.step(30)
+    a, b = oracle_state(sims[0]), oracle_state(sims[1])
+    for k in ("C", "rho", "ux", "uy"):
+        assert np.array_equal(a[k], b[k]), k
