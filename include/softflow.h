@@ -5,7 +5,10 @@
  * are borrowed only for the duration of the call; device state is owned by the
  * context.  Layouts at this boundary are the reference engine's:
  *   populations  f[q*N + y*nx + x], q in D2Q9 order (lattice.h:22-27), as
- *                left by LBMSolver::step (post-stream, post-bounce-back);
+ *                left by LBMSolver::step (post-stream, post-bounce-back) in
+ *                every non-SOLID cell; SOLID cells keep what was last written
+ *                there (the reference's pull stream copies neighbour values
+ *                into them, which no stage reads: their rho and u are 0);
  *   fields       rho/ux/uy/Fx/Fy[y*nx + x];
  *   scalar       g[n*9 + q] (advection_diffusion.h:149) and C[n];
  *   nodes        capsule-major flat arrays with CSR offsets off[ncaps+1].

@@ -20,25 +20,34 @@ namespace {
 // The lease a view holds on one of the field's arrays; released when the
 // last array derived from the view is gone.
 
+// The lease holds a strong reference to the Python object of the field (and
+// through it the Simulation), so the C++ object outlives every view: the
+// release runs before that reference is dropped, whatever order the
+// interpreter tears objects down in.
 template <class Owner>
-py::capsule lease(Owner& f, int kind, Real** data) {
+py::capsule lease(py::handle self, int kind, Real** data) {
+    Owner& f = self.cast<Owner&>();
     *data = f.leaseView(kind);
     if (!*data) throw py::index_error("no such array");
     struct Lease {
         Owner* owner;
+        PyObject* pyowner;
         int kind;
     };
-    return py::capsule(new Lease{&f, kind}, [](void* p) {
+    self.inc_ref();
+    return py::capsule(new Lease{&f, self.ptr(), kind}, [](void* p) {
         auto* l = static_cast<Lease*>(p);
         l->owner->releaseView(l->kind);
+        Py_DECREF(l->pyowner);
         delete l;
     });
 }
 
 template <class Owner>
-py::array view2d(Owner& f, int kind) {
+py::array view2d(py::handle self, int kind) {
+    Owner& f = self.cast<Owner&>();
     Real* data = nullptr;
-    py::capsule owner = lease(f, kind, &data);
+    py::capsule owner = lease<Owner>(self, kind, &data);
     const int ny = f.getNy(), nx = f.getNx();
     return py::array(py::dtype::of<Real>(), {ny, nx},
                      {static_cast<py::ssize_t>(nx) * static_cast<py::ssize_t>(sizeof(Real)),
@@ -322,23 +331,19 @@ PYBIND11_MODULE(softflow_core, m) {
         .def("getNx", &LatticeField::getNx)
         .def("getNy", &LatticeField::getNy)
         .def("size", &LatticeField::size)
-        .def("density", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_RHO); },
-             py::keep_alive<0, 1>())
-        .def("velocity_x", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_UX); },
-             py::keep_alive<0, 1>())
-        .def("velocity_y", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_UY); },
-             py::keep_alive<0, 1>())
-        .def("force_x", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_FX); },
-             py::keep_alive<0, 1>())
-        .def("force_y", [](LatticeField& f) { return view2d(f, LatticeField::VIEW_FY); },
-             py::keep_alive<0, 1>())
-        .def("distributions", [](LatticeField& f) {
+        .def("density", [](py::object self) { return view2d<LatticeField>(self, LatticeField::VIEW_RHO); })
+        .def("velocity_x", [](py::object self) { return view2d<LatticeField>(self, LatticeField::VIEW_UX); })
+        .def("velocity_y", [](py::object self) { return view2d<LatticeField>(self, LatticeField::VIEW_UY); })
+        .def("force_x", [](py::object self) { return view2d<LatticeField>(self, LatticeField::VIEW_FX); })
+        .def("force_y", [](py::object self) { return view2d<LatticeField>(self, LatticeField::VIEW_FY); })
+        .def("distributions", [](py::object self) {
+            LatticeField& f = self.cast<LatticeField&>();
             Real* data = nullptr;
-            py::capsule owner = lease(f, LatticeField::VIEW_F, &data);
+            py::capsule owner = lease<LatticeField>(self, LatticeField::VIEW_F, &data);
             const py::ssize_t N = f.size(), nx = f.getNx(), s = sizeof(Real);
             return py::array(py::dtype::of<Real>(), {static_cast<py::ssize_t>(9), static_cast<py::ssize_t>(f.getNy()), nx},
                              {N * s, nx * s, s}, data, owner);
-        }, py::keep_alive<0, 1>())
+        })
         .def("cell_types", [](LatticeField& f) {
             py::array_t<int> a({f.getNy(), f.getNx()});
             std::memcpy(a.mutable_data(), f.cellTypeData(), sizeof(int) * f.size());
@@ -403,10 +408,11 @@ PYBIND11_MODULE(softflow_core, m) {
              py::arg("concentration"), py::arg("species") = 0)
         .def("getConcentration", &AdvectionDiffusion::getConcentration, py::arg("x"), py::arg("y"),
              py::arg("species") = 0)
-        .def("concentration", [](AdvectionDiffusion& a, int species) {
+        .def("concentration", [](py::object self, int species) {
+            AdvectionDiffusion& a = self.cast<AdvectionDiffusion&>();
             if (species < 0 || species >= a.getNumSpecies()) throw py::index_error("no such species");
-            return view2d(a, species);
-        }, py::arg("species") = 0, py::keep_alive<0, 1>())
+            return view2d<AdvectionDiffusion>(self, species);
+        }, py::arg("species") = 0)
         .def("getNumSpecies", &AdvectionDiffusion::getNumSpecies)
         .def("setField", [](AdvectionDiffusion& a, py::array_t<Real, py::array::c_style | py::array::forcecast> C,
                             int species) {

@@ -46,7 +46,10 @@ def test_held_views_follow_the_steps():
         ref = oracle_state(o)
         assert rel_l2(rho, ref["rho"]) <= 1e-12 and rel_l2(ux, ref["ux"]) <= 1e-12
         assert rel_l2(C, ref["C"]) <= 1e-12
-        assert rel_l2(d, o.f().reshape(9, NY, NX)) <= 1e-12
+        # populations of the fluid rows: SOLID cells' populations are not
+        # state (the push stream never writes them, the reference's pull
+        # stream copies neighbour values there, and no stage reads them)
+        assert rel_l2(d[:, 1:-1], o.f().reshape(9, NY, NX)[:, 1:-1]) <= 1e-12
 
 
 @pytest.mark.parametrize("what", ["velocity", "density", "distributions", "concentration"])
