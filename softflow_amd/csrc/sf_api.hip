@@ -739,6 +739,9 @@ int sf_set_populations(sf_ctx* h, const double* f) {
     if (!h || !f) return fail(SF_E_ARG, "null pointer");
     Ctx& c = h->c;
     SF_HIP(hipSetDevice(c.device));
+    // a scalar step still pending needs the populations it was due with
+    flush_adr(c);
+    SF_HIP(hipStreamSynchronize(c.stream));
     const size_t N = c.N;
     if (!c.fp32) {
         SF_HIP(hipMemcpy(c.f[c.fcur], f, 9 * N * 8, hipMemcpyHostToDevice));
@@ -792,6 +795,7 @@ int sf_set_macroscopic(sf_ctx* h, const double* rho, const double* ux, const dou
     if (!h || !rho || !ux || !uy) return fail(SF_E_ARG, "null pointer");
     Ctx& c = h->c;
     SF_HIP(hipSetDevice(c.device));
+    SF_HIP(hipStreamSynchronize(c.stream));   // an enqueued step may still read the stored state
     std::vector<double> u(2 * static_cast<size_t>(c.N));
     for (int n = 0; n < c.N; ++n) { u[2 * n] = ux[n]; u[2 * n + 1] = uy[n]; }
     SF_HIP(hipMemcpy(c.stored_rho, rho, c.N * 8, hipMemcpyHostToDevice));
