@@ -179,6 +179,16 @@ int  sf_last_capped(sf_ctx* ctx, int32_t* capped);
  * out[3] = max |u| (getMaxSpeed, simulation.cpp:892-907),
  * out[4] = number of non-finite cells (checkStability, :820-890). */
 int  sf_field_stats(sf_ctx* ctx, double* out5);
+/* The same five values as sf_field_stats, computed without allocating:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, hosts that forbid
+ * allocation).  The reduction computes each cell's rho and u in place and
+ * keeps its partial sums in a fixed device array of 5 * 256 + 8 doubles
+ * (10.3 KiB) per device, held until the process ends; lattices of up to
+ * 2^31 - 1 cells, larger ones are refused with SF_E_ARG.  Results are
+ * identical to sf_field_stats'.  Calls that overlap on one device share that
+ * array and race: serialise them. */
+int  sf_field_stats_noalloc(sf_ctx* ctx, double* out5);
 /* FP64 device reductions over the non-solid cells for GlobalStatsWriter
  * (io/global_stats_writer.cpp:55-80) and checkStability (simulation.cpp:820-890):
  * out[0] cells, [1] sum rho, [2] min rho, [3] max rho, [4] max |u|^2,

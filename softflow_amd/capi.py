@@ -27,7 +27,7 @@ EXPORTS = (
     "sf_set_capsules", "sf_get_nodes", "sf_set_nodes", "sf_get_membrane_state",
     "sf_get_force_state", "sf_set_force_state",
     "sf_set_scalar", "sf_get_scalar", "sf_step", "sf_set_graph_mode", "sf_synchronize", "sf_graph_captures",
-    "sf_stage", "sf_last_capped", "sf_field_stats", "sf_global_stats", "sf_step_count", "sf_kernel_times",
+    "sf_stage", "sf_last_capped", "sf_field_stats", "sf_field_stats_noalloc", "sf_global_stats", "sf_step_count", "sf_kernel_times",
     "sf_host_alloc", "sf_host_free", "sf_set_probes", "sf_snapshot_async", "sf_snapshot_wait",
     "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_set_lbm_pairs", "sf_probe_math", "sf_copy_ceiling",
     "sf_mdf_set_control", "sf_mdf_stats", "sf_mdf_collect", "sf_mdf_samples", "sf_mdf_set_model",
@@ -110,6 +110,7 @@ def load():
         "sf_stage": [C.c_void_p, C.c_int],
         "sf_last_capped": [C.c_void_p, _IP],
         "sf_field_stats": [C.c_void_p, _DP],
+        "sf_field_stats_noalloc": [C.c_void_p, _DP],
         "sf_global_stats": [C.c_void_p, C.c_double, _DP],
         "sf_step_count": [C.c_void_p, C.POINTER(C.c_int64)],
         "sf_graph_captures": [C.c_void_p, C.POINTER(C.c_int64)],
@@ -318,6 +319,11 @@ class Context:
     def field_stats(self):
         out = np.empty(5)
         _check(self.L.sf_field_stats(self.h, _dp(out)), "sf_field_stats")
+        return out
+
+    def field_stats_noalloc(self):
+        out = np.empty(5)
+        _check(self.L.sf_field_stats_noalloc(self.h, _dp(out)), "sf_field_stats_noalloc")
         return out
 
     def set_timing(self, on: bool):

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,37 @@ __global__ void k_fill_int(int* p, int n, int v) {
 __global__ void k_fill_zero(uint32_t* p, size_t n4) {
     const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
     if (i < n4) p[i] = 0u;
+}
+
+// ---- Allocation-free field statistics ---------------------------------------
+// A file-scope device array holds the partial sums and the results of
+// sf_field_stats_noalloc: 5 doubles per reduction workgroup, then the 5
+// results.  Each device holds its own copy; its address comes from
+// hipGetSymbolAddress once per device, with that device current, and only that
+// address is handed to kernels and copies (the array's name is a host-side
+// stand-in).  Calls on one device share the array: calls that overlap on one
+// device race.  Nothing in it needs clearing: every partial and every result is
+// written before it is read.
+constexpr int NOALLOC_SCRATCH_DOUBLES = 5 * STATS_BLOCKS + 8;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the scratch array's device address on the current device (device id dev)
+static int scratch_on_device(int dev, double** out) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(SF_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return fail(SF_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SF_OK;
 }
 
 void launch_fill_int(Ctx& c, int* p, int n, int v) {
@@ -550,6 +582,11 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     chk(dmalloc(&c.mdf, sizeof(MdfDev)));
     if (rc != SF_OK) { sf_destroy(h); return rc; }
     SF_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host_counters), 64 * sizeof(int)));
+    {   // the allocation-free entry point's scratch address on this device
+        double* sp = nullptr;
+        rc = scratch_on_device(c.device, &sp);
+        if (rc != SF_OK) { sf_destroy(h); return rc; }
+    }
     c.ctr_host = c.host_counters + 32;
     // flags: the channel of LBMSolver::initialize (lbm_solver.cpp:130-158);
     // with inlet/outlet, INLET and OUTLET columns between the walls
@@ -1119,6 +1156,27 @@ int sf_field_stats(sf_ctx* h, double* out5) {
     if (e == hipSuccess) e = hipMemcpy(out5, buf, 5 * 8, hipMemcpyDeviceToHost);
     hipFree(buf);
     SF_HIP(e);
+    return SF_OK;
+}
+
+int sf_field_stats_noalloc(sf_ctx* h, double* out5) {
+    if (!h || !out5) return fail(SF_E_ARG, "null pointer");
+    Ctx& c = h->c;
+    // the reduction walks the lattice with a fixed grid; its partials fill the
+    // scratch array whatever the lattice size, and cells are indexed by int
+    if (static_cast<int64_t>(c.nx) * c.ny > INT32_MAX)
+        return fail(SF_E_ARG, "lattice larger than 2^31 - 1 cells");
+    static_assert(5 * STATS_BLOCKS + 5 <= NOALLOC_SCRATCH_DOUBLES, "scratch holds partials and results");
+    SF_HIP(hipSetDevice(c.device));
+    double* ptr = nullptr;
+    const int rc = scratch_on_device(c.device, &ptr);
+    if (rc != SF_OK) return rc;
+    double* part = ptr;
+    double* res = ptr + 5 * STATS_BLOCKS;
+    launch_field_stats_fused(c, part, res);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipMemcpyAsync(out5, res, 5 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SF_HIP(hipStreamSynchronize(c.stream));
     return SF_OK;
 }
 

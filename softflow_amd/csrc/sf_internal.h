@@ -274,7 +274,11 @@ void launch_adr(Ctx& c, int Fidx);
 void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, double* Fy);
 void launch_macro_cells(Ctx& c, const int* cells, int count, double* out);
 void launch_sum(Ctx& c, const double* v, int64_t n, double* ws, double* out);
+// workgroups of the field-statistics reductions (fixed: the summation order
+// depends on it)
+constexpr int STATS_BLOCKS = 256;
 void launch_field_stats(Ctx& c, double* dev_out);
+void launch_field_stats_fused(Ctx& c, double* part, double* dev_out);
 void launch_global_stats(Ctx& c, double rho0, double* dev_out);
 // interpolated bounce-back on the populations the last collide-stream wrote
 void launch_ibb(Ctx& c);

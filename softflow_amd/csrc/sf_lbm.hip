@@ -1393,20 +1393,16 @@ void launch_macro(Ctx& c, double* rho, double* ux, double* uy, double* Fx, doubl
 
 // FP64 reductions over FLUID cells: per-block partials in a fixed order, then
 // one block folds the partials (deterministic for a given launch shape).
-__global__ void k_stats_partial(const double* rho, const double* ux, const double* uy,
-                                const uint8_t* flags, LbmArgs a, double* part) {
+__device__ __forceinline__ void stats_add(double r, double u, double v, double (&acc)[5]) {
+    if (!isfinite(r) || !isfinite(u) || !isfinite(v)) { acc[4] += 1.0; return; }
+    acc[0] += r; acc[1] += r * u; acc[2] += r * v;
+    acc[3] = fmax(acc[3], u * u + v * v);
+}
+
+__device__ __forceinline__ void stats_block_store(const double (&acc)[5], double* part) {
     __shared__ double s[5][256];
     const int t = threadIdx.x;
-    double m = 0, px = 0, py = 0, um = 0, bad = 0;
-    for (int n = blockIdx.x * blockDim.x + t; n < a.N; n += gridDim.x * blockDim.x) {
-        const int x = n % a.nx, y = n / a.nx;
-        if (is_solid(a, x, y, flags != nullptr)) continue;
-        const double r = rho[n], u = ux[n], v = uy[n];
-        if (!isfinite(r) || !isfinite(u) || !isfinite(v)) { bad += 1.0; continue; }
-        m += r; px += r * u; py += r * v;
-        um = fmax(um, u * u + v * v);
-    }
-    s[0][t] = m; s[1][t] = px; s[2][t] = py; s[3][t] = um; s[4][t] = bad;
+    for (int k = 0; k < 5; ++k) s[k][t] = acc[k];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (t < w) {
@@ -1417,6 +1413,34 @@ __global__ void k_stats_partial(const double* rho, const double* ux, const doubl
     }
     if (t == 0)
         for (int k = 0; k < 5; ++k) part[blockIdx.x * 5 + k] = s[k][0];
+}
+
+__global__ __launch_bounds__(256) void k_stats_partial(const double* rho, const double* ux, const double* uy,
+                                                       const uint8_t* flags, LbmArgs a, double* part) {
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += gridDim.x * blockDim.x) {
+        const int x = n % a.nx, y = n / a.nx;
+        if (is_solid(a, x, y, flags != nullptr)) continue;
+        stats_add(rho[n], ux[n], uy[n], acc);
+    }
+    stats_block_store(acc, part);
+}
+
+// The same partials with each cell's rho and u computed in place, as k_macro
+// computes them, instead of read from arrays: the per-cell values and the
+// summation order are those of k_stats_partial over k_macro's output, so the
+// results are identical, and no N-sized buffer is needed.
+template <bool FP32, bool FLAGS>
+__global__ __launch_bounds__(256) void k_stats_fused(LbmArgs a, const void* f, int mode, double* part) {
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += gridDim.x * blockDim.x) {
+        const int x = n % a.nx, y = n / a.nx;
+        if (is_solid(a, x, y, FLAGS)) continue;
+        double r, u, v;
+        macro_state<FP32, FLAGS>(a, f, mode, n, r, u, v);
+        stats_add(r, u, v, acc);
+    }
+    stats_block_store(acc, part);
 }
 
 __global__ void k_stats_final(const double* part, int nb, double* out) {
@@ -1506,10 +1530,28 @@ void launch_field_stats(Ctx& c, double* dev_out) {
     double* part = uy + c.N;
     launch_macro(c, rho, ux, uy, nullptr, nullptr);
     LbmArgs a = view_args(c);
-    const int nb = 256;
+    const int nb = STATS_BLOCKS;
     hipLaunchKernelGGL(k_stats_partial, dim3(nb), dim3(256), 0, c.stream, rho, ux, uy,
                        c.generic_flags ? c.flags : nullptr, a, part);
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, c.stream, part, nb, dev_out);
+}
+
+void launch_field_stats_fused(Ctx& c, double* part, double* dev_out) {
+    // part: 5 * STATS_BLOCKS doubles; dev_out: 5 doubles
+    LbmArgs a = view_args(c);
+    a.srho = c.stored_rho;
+    a.su = c.stored_u;
+    const int mode = c.shift_next == SHIFT_STORED ? 2 : (c.step_count == 0 ? 1 : 0);
+    const void* f = c.f[c.fcur];
+    const dim3 g(STATS_BLOCKS), b(256);
+    if (c.fp32) {
+        if (c.generic_flags) hipLaunchKernelGGL((k_stats_fused<true, true>), g, b, 0, c.stream, a, f, mode, part);
+        else hipLaunchKernelGGL((k_stats_fused<true, false>), g, b, 0, c.stream, a, f, mode, part);
+    } else {
+        if (c.generic_flags) hipLaunchKernelGGL((k_stats_fused<false, true>), g, b, 0, c.stream, a, f, mode, part);
+        else hipLaunchKernelGGL((k_stats_fused<false, false>), g, b, 0, c.stream, a, f, mode, part);
+    }
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, c.stream, part, STATS_BLOCKS, dev_out);
 }
 
 }  // namespace sf
