@@ -120,11 +120,15 @@ def prime_graphs(sim, steps):
     """Capture, outside any timed region, every graph that ``sim.run(steps)``
     will replay from the current buffer phase.  A run of ``steps`` from phase P
     replays the six-step graph of phase P and a (steps mod 6)-step graph of
-    phase P: one untimed run of the same length captures both, and
-    (6 - steps mod 6) mod 6 more steps bring the roles back to phase P."""
-    sim.run(steps)
+    phase P: six steps capture the first and end at phase P again, r = steps
+    mod 6 more capture the second, and 6 - r more bring the roles back to
+    phase P (12 steps at most)."""
+    sim.run(min(steps, GRAPH_PERIOD))
     r = steps % GRAPH_PERIOD
-    if r:
+    if r and steps > GRAPH_PERIOD:
+        sim.run(r)
+        sim.run(GRAPH_PERIOD - r)
+    elif r:
         sim.run(GRAPH_PERIOD - r)
     sim.synchronize()
 
@@ -146,80 +150,147 @@ def timed_steps(sim, steps):
     return steps / dt
 
 
-def mdf_learned_bench(steps, warmup, train_steps, epochs, probe_steps=100):
-    """Steps/s of the plain 5-iteration MDF in FP64 (the reference's scheme)
-    against the FP32 path whose first correction comes from the trained
-    network, at the same no-slip residual; the network is trained here on rows
-    the plain FP32 solver collects.  Both timed runs apply at most four
-    corrections per step and measure no extra residual probe; the plain
-    scheme's residual comes from a probed segment before its timed run.  Both
-    runs take the same number of steps, so their velocity fields can be
-    compared; the plain scheme with three corrections instead of four is
-    reported beside it (the reference's MDF flow depends strongly on the
-    number of corrections)."""
+MDF_TOL = 1e-5        # the no-slip residual both MDF schemes are held to (<= 1e-4)
+MDF_MAX_CORR = 16     # corrections allowed per step before a step gives up on the tolerance
+MDF_DEVELOP = 3000    # steps of the reference's 5-iteration MDF from rest before timing
+
+
+def _mdf_stats(st):
+    return {"corrections_per_step": round(st["mean_corrections"], 3), "residual_mean": st["mean_residual"],
+            "residual_max": st["max_residual"]}
+
+
+def _residual_curve(sim, kmax=6, steps=60):
+    """Mean and max no-slip residual after k = 1 .. kmax corrections per step
+    (the learned correction counts as the first when a model is set)."""
+    out = []
+    for k in range(1, kmax + 1):
+        sim.setMdfControl(1e-300, k, True)
+        sim.run(6)
+        sim.mdfStats()
+        sim.run(steps)
+        st = sim.mdfStats()
+        out.append({"corrections": k, "residual_mean": st["mean_residual"], "residual_max": st["max_residual"]})
+    return out
+
+
+def mdf_learned_bench(steps, train_steps, epochs, curve=True):
+    """The learned MDF warm start against plain MDF on the MDF channel
+    (MDF_CHANNEL), both held to the same no-slip residual MDF_TOL.
+
+    Protocol, the same for both schemes: MDF_DEVELOP steps of the reference's
+    5-iteration MDF from rest, then ``steps`` timed steps.
+    - plain, FP64: plain corrections until the residual r = max_k |V_k - U(X_k)|
+      is <= MDF_TOL (at most MDF_MAX_CORR per step); the reference's fixed
+      5-iteration MDF (4 corrections) is timed beside it;
+    - learned, FP32: the network's correction first, then plain corrections
+      until r <= MDF_TOL (same cap).  The network is trained here, by the
+      device's own kernels, on (state, converged correction) rows the plain
+      FP32 solver collects over ``train_steps`` steps after its development.
+    Reported: steps/s, corrections per step, residuals, the velocity field of
+    the learned FP32 run against the plain FP64 run after the same number of
+    steps, and the residual after k corrections for both schemes."""
     from softflow_amd import softflow_core as sc
 
-    out = {"workload": "2048x512 regularized tau=0.8 fx=3e-5, 500 Skalak (G_s=0.06, C=1) r=4 "
-                       "(25 nodes) seeded over the channel (2.4% area fraction)"}
-    # plain FP64, 5 MDF iterations (4 corrections); residual after the last
-    # correction measured over probe_steps, then timed without the probe
+    ch = MDF_CHANNEL
+    out = {"workload": f"{ch['nx']}x{ch['ny']} regularized tau={ch['tau']} fx={ch['bfx']:g}, {ch['count']} Skalak "
+                       f"(G_s=0.06, C={ch['mem']['C_skalak']:g}) r={ch['radius']:g} ({ch['nodes']} nodes) seeded "
+                       "over the channel (2.4% area fraction)",
+           "residual_tolerance": MDF_TOL, "max_corrections": MDF_MAX_CORR, "develop_steps": MDF_DEVELOP,
+           "timed_steps": steps}
+    # plain FP64: develop, the reference's fixed scheme, then residual control
     plain, placed = build_mdf_channel("fp64", 5, USE_GRAPH)
     out["capsules"] = placed
-    plain.run(warmup)
-    plain.setMdfControl(0.0, 0, True)
-    plain.run(10)
+    plain.run(MDF_DEVELOP)
+    plain.setMdfControl(1e-300, 4, True)        # 4 corrections, residual measured
+    plain.run(6)
     plain.mdfStats()
-    plain.run(probe_steps)
+    dt, _ = timed_run(plain, steps)
+    out["plain_fp64_5_iterations"] = dict(steps_per_s=round(steps / dt, 1), **_mdf_stats(plain.mdfStats()))
+    plain.setMdfControl(MDF_TOL, MDF_MAX_CORR, True)
+    plain.run(6)
+    plain.mdfStats()
+    dt, _ = timed_run(plain, steps)
     sp = plain.mdfStats()
-    plain.setMdfControl(0.0, 0, False)
-    out["plain_fp64_steps_per_s"] = round(timed_steps(plain, steps), 1)
-    # training rows from the plain FP32 solver (4 corrections: the same scheme)
+    out["plain_fp64_steps_per_s"] = round(steps / dt, 1)
+    out["plain"] = _mdf_stats(sp)
+    # learned FP32: develop, collect rows from the plain solver, train
     t0 = time.perf_counter()
-    coll, _ = build_mdf_channel("fp32", 5, USE_GRAPH)
-    coll.run(warmup)
-    K = coll.capsules().totalNodes()
-    coll.collectMdfSamples(train_steps * K)
-    coll.run(train_steps)
+    learned, _ = build_mdf_channel("fp32", 5, USE_GRAPH)
+    learned.run(MDF_DEVELOP)
+    learned.setMdfControl(1e-300, 4, True)
+    K = learned.capsules().totalNodes()
+    learned.collectMdfSamples(train_steps * K)
+    learned.run(train_steps)
+    rows = learned.mdfSampleCount()
     model = sc.MdfModel(0, 7)
-    loss = model.train(coll, epochs=epochs, batch=4096, lr=2e-3, seed=11)
+    loss = model.train(learned, epochs=epochs, batch=4096, lr=2e-3, seed=11)
+    learned.collectMdfSamples(0)
     train_s = time.perf_counter() - t0
-    rows = coll.mdfSampleCount()
-    del coll
-    # FP32 with the network's correction first, plain ones until the residual
-    # is at most what the plain FP64 scheme reached (here above 1e-4: the
-    # plain iteration stalls near 3e-4 on this channel), at most four
-    tol = sp["max_residual"]
-    learned, _ = build_mdf_channel("fp32", 1, USE_GRAPH)
-    learned.setMdfControl(1e-300, 4, False)
-    learned.run(warmup)
+    # the same step count as the plain run before its residual-controlled segment
+    gap = (steps + len_prime(steps) + 6) - train_steps
+    if gap > 0:
+        learned.run(gap)
     learned.setMdfModel(model, True)
-    learned.setMdfControl(tol, 4, False)
-    learned.run(10 + probe_steps)
+    learned.setMdfControl(MDF_TOL, MDF_MAX_CORR, True)
+    learned.run(6)
     learned.mdfStats()
-    out["learned_fp32_steps_per_s"] = round(timed_steps(learned, steps), 1)
+    dt, _ = timed_run(learned, steps)
     sl = learned.mdfStats()
+    out["learned_fp32_steps_per_s"] = round(steps / dt, 1)
+    out["learned"] = _mdf_stats(sl)
     out["speedup"] = round(out["learned_fp32_steps_per_s"] / out["plain_fp64_steps_per_s"], 3)
-    out["mlups_plain_fp64"] = round(out["plain_fp64_steps_per_s"] * MDF_CHANNEL["nx"] * MDF_CHANNEL["ny"] / 1e6, 1)
-    out["mlups_learned_fp32"] = round(out["learned_fp32_steps_per_s"] * MDF_CHANNEL["nx"] * MDF_CHANNEL["ny"] / 1e6, 1)
-    out["residual_tolerance"] = tol
-    out["plain"] = {"corrections_per_step": sp["mean_corrections"], "residual_mean": sp["mean_residual"],
-                    "residual_max": sp["max_residual"]}
-    # steps that reach the cap report the residual before their last correction
-    out["learned"] = {"corrections_per_step": sl["mean_corrections"], "residual_mean": sl["mean_residual"],
-                      "residual_max": sl["max_residual"]}
+    out["speedup_vs_5_iterations"] = round(out["learned_fp32_steps_per_s"] /
+                                           out["plain_fp64_5_iterations"]["steps_per_s"], 3)
+    out["mlups_plain_fp64"] = round(out["plain_fp64_steps_per_s"] * ch["nx"] * ch["ny"] / 1e6, 1)
+    out["mlups_learned_fp32"] = round(out["learned_fp32_steps_per_s"] * ch["nx"] * ch["ny"] / 1e6, 1)
 
     def vel(sim):
         f = sim.field()
         return np.r_[np.ravel(f.velocity_x()), np.ravel(f.velocity_y())]
     ua = vel(plain)
     out["velocity_rel_l2_vs_plain_fp64"] = float(np.linalg.norm(vel(learned) - ua) / np.linalg.norm(ua))
-    del learned
-    three, _ = build_mdf_channel("fp64", 4, USE_GRAPH)
-    # the same step count as the timed runs, their graph priming included
-    three.run(warmup + 10 + probe_steps + 2 * steps + (-steps) % GRAPH_PERIOD)
-    out["velocity_rel_l2_plain_3_vs_4_corrections"] = float(np.linalg.norm(vel(three) - ua) / np.linalg.norm(ua))
     out["training"] = {"rows": int(rows), "epochs": epochs, "seconds": round(train_s, 2),
                        "loss_first_epoch": loss[0], "loss_last_epoch": loss[-1]}
+    if curve:
+        out["residual_after_k_corrections"] = {"plain_fp64": _residual_curve(plain),
+                                               "learned_fp32": _residual_curve(learned)}
+    return out
+
+
+def len_prime(steps):
+    """Steps prime_graphs runs before a timed run of ``steps``."""
+    r = steps % GRAPH_PERIOD
+    return min(steps, GRAPH_PERIOD) + (GRAPH_PERIOD if r and steps > GRAPH_PERIOD else (GRAPH_PERIOD - r if r else 0))
+
+
+def mdf_stability_bench(steps=4000, chunk=250):
+    """The reference's plain 5-iteration MDF (FP64, the oracle's arithmetic) on
+    the MDF channel at the original specification's body force 3e-5, with
+    C = 10, and with the capsules seeded into a 30 % band: the step at which
+    each first shows non-finite cells or a speed above 0.5, or None when it
+    ran ``steps`` steps.  The benchmark channel (body force 3e-6) is run the
+    same way for comparison."""
+    cases = {"benchmark channel (fx 3e-6, C 1, whole channel)": {},
+             "fx 3e-5": {"bfx": 3e-5},
+             "C 10": {"mem": {"C_skalak": 10.0}},
+             "30% band (164 x 504)": {"region": (942, 4, 1106, 508)}}
+    out = {}
+    for name, changes in cases.items():
+        sim, placed = build_mdf_channel("fp64", 5, USE_GRAPH, **changes)
+        at, umax = None, 0.0
+        try:
+            for s in range(0, steps, chunk):
+                sim.run(chunk)
+                fs = sim.fieldStats()
+                umax = fs[3]
+                if fs[4] > 0 or not np.isfinite(fs[3]) or fs[3] > 0.5:
+                    at = s + chunk
+                    break
+        except RuntimeError as e:   # a capacity overflow as the capsules collapse
+            at = f"{s + chunk} ({e})"
+        out[name] = {"capsules": placed, "diverged_by_step": at, "max_speed": umax, "steps": steps}
+        del sim
     return out
 
 
@@ -433,7 +504,8 @@ def main():
                          "reference for the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--no-mdf", action="store_true",
                     help="skip the learned-vs-plain MDF comparison")
-    ap.add_argument("--mdf-steps", type=int, default=400)
+    ap.add_argument("--mdf-steps", type=int, default=20000)
+    ap.add_argument("--mdf-train-steps", type=int, default=160)
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=2000)
@@ -446,7 +518,7 @@ def main():
     ap.add_argument("--output-steps", type=int, default=500)
     ap.add_argument("--no-geometry", action="store_true",
                     help="skip the pillar-array measurement")
-    ap.add_argument("--only", choices=["ensemble", "mdf", "output", "geometry"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "mdf", "mdf-stability", "output", "geometry"], default=None,
                     help="run only the batched-ensemble or the learned-vs-plain MDF measurement "
                          "and print its object as the JSON line (for profiling)")
     args = ap.parse_args()
@@ -465,7 +537,9 @@ def main():
         if args.only == "ensemble":
             part = ensemble_bench(args.ensemble_steps, seq_steps=args.ensemble_sequential_steps)
         elif args.only == "mdf":
-            part = mdf_learned_bench(args.mdf_steps, 200, 160, 6)
+            part = mdf_learned_bench(args.mdf_steps, args.mdf_train_steps, 6)
+        elif args.only == "mdf-stability":
+            part = mdf_stability_bench()
         elif args.only == "geometry":
             part = geometry_bench(args.steps, 5, cpu_threads(), not args.no_cpu)
         else:
@@ -561,7 +635,7 @@ def main():
     mdf = None
     if not args.no_mdf:
         try:
-            mdf = mdf_learned_bench(args.mdf_steps, 200, 160, 6)
+            mdf = mdf_learned_bench(args.mdf_steps, args.mdf_train_steps, 6)
         except RuntimeError as e:   # reported, so the main line is still printed
             mdf = {"error": str(e)}
         print("# learned vs plain MDF: " + json.dumps(mdf), file=sys.stderr, flush=True)

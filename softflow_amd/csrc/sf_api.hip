@@ -193,7 +193,7 @@ static int step_ibm(Ctx& c) {
                     { Scope s(c, "mdf_features"); launch_mdf_features(c); }
                     if (learned) { Scope s(c, "mlp_infer"); launch_mlp_infer(c); }
                 }
-                { Scope s(c, "mdf_spread"); launch_spread(c, 1, it == ncorr, it); }
+                { Scope s(c, "mdf_spread"); launch_spread(c, 1, it == ncorr, it, ncorr); }
             }
             if (c.mdf_probe) { Scope s(c, "mdf_probe"); launch_mdf_interp(c, ncorr + 1); }
             launch_mdf_record(c, ncorr);

@@ -168,14 +168,35 @@ constexpr int REST_TPW1 = 16;
 __host__ __device__ constexpr int rest_tpw(int mode) { return mode == 0 ? REST_TPW0 : REST_TPW1; }
 __host__ __device__ constexpr int rest_tiles_per_block(int mode) { return 4 * rest_tpw(mode); }
 
+// After convergence (residual mode) the remaining corrections add nothing, so
+// the force both buffers of the correction ping-pong must end with is the one
+// the last applied correction left.  Rather than copying it forward at every
+// remaining iteration, the iterations before the last do nothing and the last
+// one copies it once into the buffer that does not hold it: the result
+// equals the copied-forward chain's.  Convergence detected at iteration j
+// leaves the force in the input buffer of iteration j, which is the input
+// buffer of the last iteration when (last - j) is even and its output
+// buffer otherwise (the copy then runs output -> input).
 template <bool FP32>
 __device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void* Fin, void* Fout,
                                                      const uint8_t* dirty_in, uint8_t* dirty_out,
                                                      int2* act, int* counters, int nx, int ny,
                                                      int ntx, int ntiles, int mode,
-                                                     const MdfDev* md, int bx) {
+                                                     const MdfDev* md, int bx, int it = 0, int last_it = 0) {
     using T = typename std::conditional<FP32, float, double>::type;
     const bool skip = mode == 1 && md && md->conv;
+    if (skip && last_it > 0) {
+        if (it < last_it) return;
+        if (((last_it - md->conv_it) & 1) != 0) {
+            // the force is in this iteration's output buffer: copy it back
+            const void* t = Fin;
+            Fin = Fout;
+            Fout = const_cast<void*>(t);
+            const uint8_t* d = dirty_in;
+            dirty_in = dirty_out;
+            dirty_out = const_cast<uint8_t*>(d);
+        }
+    }
     const int lane = threadIdx.x & 63;
     const int tpw = rest_tpw(mode);
     const int wave_tile0 = (bx * static_cast<int>(blockDim.x >> 6) + static_cast<int>(threadIdx.x >> 6)) * tpw;
@@ -247,8 +268,9 @@ __global__ __launch_bounds__(256) void k_spread_rest(const int* tile_cnt, const 
                                                      const uint8_t* dirty_in, uint8_t* dirty_out,
                                                      int2* act, int* counters, int nx, int ny,
                                                      int ntx, int ntiles, int mode,
-                                                     const MdfDev* md) {
-    spread_rest_body<FP32>(tile_cnt, Fin, Fout, dirty_in, dirty_out, act, counters, nx, ny, ntx, ntiles, mode, md, blockIdx.x);
+                                                     const MdfDev* md, int it, int last_it) {
+    spread_rest_body<FP32>(tile_cnt, Fin, Fout, dirty_in, dirty_out, act, counters, nx, ny, ntx, ntiles, mode, md,
+                           blockIdx.x, it, last_it);
 }
 
 // Touched tiles: a persistent grid walks the list of tiles the node binning
@@ -866,7 +888,7 @@ static CellVelArgs vel_args(Ctx& c, const void* f, const void* F) {
 // mode 0: bin nodes and spread node forces into F[Fcur] (overwrite).
 // mode 1: spread the MDF corrections on top of F[Fcur] into the spare buffer
 //         and make it current; the previous contents become F[Fpart].
-void launch_spread(Ctx& c, int mode, bool last, int it) {
+void launch_spread(Ctx& c, int mode, bool last, int it, int last_it) {
     if (c.K == 0) {
         if (mode == 0) launch_fill_zero(c, c.F[c.Fcur], (c.fp32 ? 8 : 16) * static_cast<size_t>(c.N));
         return;
@@ -893,13 +915,15 @@ void launch_spread(Ctx& c, int mode, bool last, int it) {
     const dim3 sgrid(std::min(c.ntiles, SPREAD_GRID));
     if (c.fp32) {
         hipLaunchKernelGGL((k_spread_rest<true>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
-                           din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md);
+                           din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md,
+                           it, last_it);
         hipLaunchKernelGGL((k_spread<true>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
                            c.tile_list, c.tile_act, c.counters, c.phi, Fin, Fout, dout, c.nx, c.ny,
                            c.ntx, c.ntiles, mode, last ? 1 : 0, md);
     } else {
         hipLaunchKernelGGL((k_spread_rest<false>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
-                           din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md);
+                           din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md,
+                           it, last_it);
         hipLaunchKernelGGL((k_spread<false>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
                            c.tile_list, c.tile_act, c.counters, c.phi, Fin, Fout, dout, c.nx, c.ny,
                            c.ntx, c.ntiles, mode, last ? 1 : 0, md);

@@ -291,7 +291,8 @@ void launch_repulsion(Ctx& c);
 bool contact_uses_cells(const Ctx& c);
 void launch_contact_fill(Ctx& c);
 void launch_membrane_sort(Ctx& c);
-void launch_spread(Ctx& c, int mode, bool last, int it = 0);   // 0 node forces into F[Fcur]; 1 MDF correction
+// mode 0: node forces into F[Fcur]; mode 1: MDF correction it of last_it
+void launch_spread(Ctx& c, int mode, bool last, int it = 0, int last_it = 0);
 void launch_mdf_interp(Ctx& c, int it);
 void launch_mdf_begin(Ctx& c);
 void launch_mdf_features(Ctx& c);

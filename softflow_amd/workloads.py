@@ -103,27 +103,30 @@ def build_product(case, precision="fp32", graph=True):
 
 
 
-# Learned MDF warm start workload: 2048 x 512 channel, regularized tau = 0.8,
-# fx = 3e-5; 500 Skalak capsules (G_s = 0.06, C = 1) of radius 4 with 25 nodes
-# seeded (seed 5) over the whole channel (2.4 % area fraction).  Denser
-# seeding (30 % in a 164 x 504 band) and C = 10 both make the explicit
-# coupling of plain multi-direct forcing diverge within 120-200 steps, in the
-# FP64 restatement of the reference as on the device.
-MDF_CHANNEL = dict(nx=2048, ny=512, tau=0.8, bfx=3e-5, count=500, radius=4.0, nodes=25, seed=5,
+# Learned MDF warm start workload: 2048 x 512 channel, regularized tau = 0.8;
+# 500 Skalak capsules (G_s = 0.06, C = 1) of radius 4 with 25 nodes seeded
+# (seed 5) over the whole channel (2.4 % area fraction).  Body force 3e-6: at
+# the 3e-5 of the original specification the reference's plain multi-direct
+# forcing (five iterations) diverges within about 1750 steps as the plug
+# accelerates, and so do C = 10 and a 30 % seeding band sooner (the stability
+# record of bench.py --only mdf-stability); at 3e-6 it runs 20 000 steps.
+MDF_CHANNEL = dict(nx=2048, ny=512, tau=0.8, bfx=3e-6, count=500, radius=4.0, nodes=25, seed=5,
                    region=(4, 8, 2044, 504), mem={"C_skalak": 1.0})
 
 
-def build_mdf_channel(precision, ibm_iterations, graph=True):
-    """The learned-MDF channel (MDF_CHANNEL) with ``ibm_iterations`` fixed MDF
-    iterations; returns (simulation, capsules placed)."""
+def build_mdf_channel(precision, ibm_iterations, graph=True, **changes):
+    """The learned-MDF channel (MDF_CHANNEL, with ``changes`` applied: bfx,
+    region, mem) with ``ibm_iterations`` fixed MDF iterations; returns
+    (simulation, capsules placed)."""
     from softflow_amd import softflow_core as sc
 
+    ch = dict(MDF_CHANNEL, **changes)
     p = sc.SimulationParams()
     p.output_interval, p.probe_dump_every, p.stats_dump_every = 0, 0, 0   # no output files
-    p.nx, p.ny = MDF_CHANNEL["nx"], MDF_CHANNEL["ny"]
-    p.fluid.tau = MDF_CHANNEL["tau"]
+    p.nx, p.ny = ch["nx"], ch["ny"]
+    p.fluid.tau = ch["tau"]
     p.fluid.boundary_type = sc.BoundaryType.PERIODIC
-    p.fluid.body_force_x = MDF_CHANNEL["bfx"]
+    p.fluid.body_force_x = ch["bfx"]
     p.fluid.collision_model = sc.CollisionModel.REGULARIZED
     p.ibm_iterations = ibm_iterations
     p.enable_stability_checks = False
@@ -136,10 +139,10 @@ def build_mdf_channel(precision, ibm_iterations, graph=True):
     mp = sc.MembraneParams()
     mp.model = sc.MembraneModel.SKALAK
     mp.G_s, mp.C_skalak = 0.06, 10.0
-    for k, v in MDF_CHANNEL.get("mem", {}).items():
+    for k, v in ch.get("mem", {}).items():
         setattr(mp, k, v)
-    x0, y0, x1, y1 = MDF_CHANNEL["region"]
-    placed = sim.addCapsuleRandom(MDF_CHANNEL["count"], x0, y0, x1, y1, MDF_CHANNEL["radius"], MDF_CHANNEL["radius"], MDF_CHANNEL["nodes"],
-                                  mp, 0, MDF_CHANNEL["seed"], 1.0)
+    x0, y0, x1, y1 = ch["region"]
+    placed = sim.addCapsuleRandom(ch["count"], x0, y0, x1, y1, ch["radius"], ch["radius"], ch["nodes"],
+                                  mp, 0, ch["seed"], 1.0)
     sim.initialize()
     return sim, placed
