@@ -21,18 +21,27 @@ constexpr int BATCH_PERIOD = 6;   // steps after which every member's buffer rol
 
 // Buffer roles of a member between steps.
 struct Roles {
-    int fcur, Fcur, Fprev, Fpart, cell_cur;
+    int fcur, gcur, Fcur, Fprev, Fpart, cell_cur;
     bool operator==(const Roles& o) const {
-        return fcur == o.fcur && Fcur == o.Fcur && Fprev == o.Fprev && Fpart == o.Fpart &&
-               cell_cur == o.cell_cur;
+        return fcur == o.fcur && gcur == o.gcur && Fcur == o.Fcur && Fprev == o.Fprev &&
+               Fpart == o.Fpart && cell_cur == o.cell_cur;
     }
     bool operator!=(const Roles& o) const { return !(*this == o); }
 };
 
-static Roles roles_of(const Ctx& c) { return Roles{c.fcur, c.Fcur, c.Fprev, c.Fpart, c.cell_cur}; }
+static Roles roles_of(const Ctx& c) { return Roles{c.fcur, c.gcur, c.Fcur, c.Fprev, c.Fpart, c.cell_cur}; }
 
 static void set_roles(Ctx& c, const Roles& r) {
-    c.fcur = r.fcur; c.Fcur = r.Fcur; c.Fprev = r.Fprev; c.Fpart = r.Fpart; c.cell_cur = r.cell_cur;
+    c.fcur = r.fcur; c.gcur = r.gcur; c.Fcur = r.Fcur; c.Fprev = r.Fprev; c.Fpart = r.Fpart;
+    c.cell_cur = r.cell_cur;
+}
+
+// A member is at the batched (steady) step when its next collision takes the
+// previous step's force and, with a scalar, the previous step's scalar step is
+// pending and its concentration is the one its populations hold -- the state
+// in which one_step fuses the scalar step into the collide-stream.
+static bool steady_member(const Ctx& c) {
+    return c.shift_next == SHIFT_PREV && (c.nsp == 0 || (c.adr_pending && !c.c_from_store));
 }
 
 // The items of one steady step of member c, in one_step's launch order
@@ -160,7 +169,6 @@ static int validate(sf_batch& B) {
             return fail(SF_E_ARG, who + "batches take x-periodic plain channels (no obstacle flags)");
         if (c.nobs > 0 || c.p.use_interpolated_bb)
             return fail(SF_E_ARG, who + "obstacles and interpolated bounce-back are not batched");
-        if (c.nsp > 0) return fail(SF_E_ARG, who + "scalar transport is not batched");
         if (mdf_corrections_of(c) > 0) return fail(SF_E_ARG, who + "multi-direct forcing is not batched");
         if (c.ncaps == 0 || c.any_rigid)
             return fail(SF_E_ARG, who + "batched members carry deformable capsules");
@@ -176,6 +184,8 @@ static int build_plan(sf_batch& B) {
     int maxn = 0;
     for (Ctx* c : B.mem) maxn = std::max(maxn, c->max_nodes);
     B.v.maxn = maxn <= 64 ? 64 : 256;
+    B.v.adr = 0;
+    for (Ctx* c : B.mem) B.v.adr |= c->nsp > 0 ? 1 : 0;
 
     std::vector<BatchItem> items(static_cast<size_t>(BATCH_PERIOD) * M * BK_COUNT);
     std::vector<int64_t> steps0(M);
@@ -348,11 +358,17 @@ int sf_batch_step(sf_batch* b, int nsteps) {
     if (nsteps == 0) return SF_OK;
     // the batched step is the steady one (previous-step shift force); a
     // member's first step after initialisation runs on its own first
-    bool steady = true;
-    for (Ctx* c : B.mem) steady = steady && c->shift_next == SHIFT_PREV;
-    if (!steady) {
+    // (members with a scalar may need a second such step after a host write)
+    for (;;) {
+        bool steady = true;
+        for (Ctx* c : B.mem) steady = steady && steady_member(*c);
+        if (steady || nsteps == 0) break;
         for (Ctx* c : B.mem) ctx_one_step(*c);
         --nsteps;
+    }
+    if (nsteps == 0) {
+        SF_HIP(hipGetLastError());
+        return SF_OK;
     }
     rc = wait_members(B);
     if (rc != SF_OK) return rc;
@@ -434,6 +450,8 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
     BatchVariant v = B.v;
     v.fp32 = B.mem[0]->fp32 ? 1 : 0;
     v.op = B.mem[0]->p.collision_model;
+    v.adr = 0;
+    for (Ctx* c : B.mem) v.adr |= c->nsp > 0 ? 1 : 0;
     for (const auto& sh : shapes)
     for (int pair = 0; pair <= 1; ++pair) {
         // the collide-stream of the members' current roles (the launch only
@@ -442,9 +460,13 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
         std::vector<int> blocks(M);
         for (int m = 0; m < M; ++m) {
             Ctx& c = *B.mem[m];
-            const int f0 = c.fcur;
+            const int f0 = c.fcur, g0 = c.gcur;
+            const bool store0 = c.c_from_store, pend0 = c.adr_pending;
             batch_items_lbm(c, sh[0], sh[1], pair, &it[m]);
             c.fcur = f0;
+            c.gcur = g0;
+            c.c_from_store = store0;
+            c.adr_pending = pend0;
             blocks[m] = it[m].blocks;
         }
         const size_t sz = it[0].pack.size();

@@ -335,6 +335,7 @@ struct BatchVariant {
     int maxn = 64;          // membrane LDS capacity (64 or 256 nodes)
     int bx = 64, by = 4;    // collide-stream workgroup shape
     int pair = 1;           // two cells per lane for the members where it applies
+    int adr = 0;            // some member carries a scalar: the collide-stream runs its lagged step
 };
 
 template <typename P>

@@ -1125,8 +1125,9 @@ void launch_lbm(Ctx& c, int shift, bool fuse_adr) {
 
 // Batched collide-stream over the members of an ensemble: workgroup b runs
 // workgroup map[b].y of member map[b].x's own linear grid (the map itself
-// orders the workgroups across XCDs).
-template <int OP, bool FP32>
+// orders the workgroups across XCDs).  ADRF: the members' lagged scalar steps
+// ride along (a member without a scalar has nsp = 0 and runs none).
+template <int OP, bool FP32, bool ADRF>
 __global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* map) {
     const int2 w = map[blockIdx.x];
     if (w.x < 0) return;
@@ -1136,44 +1137,56 @@ __global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* 
         if (w.y >= a.nblk) return;
         const int x = 2 * ((w.y % a.nbx) * blockDim.x + threadIdx.x);
         const int y = (w.y / a.nbx) * blockDim.y + threadIdx.y;
-        capped = lbm_pair<OP, false, false>(a, x, y);
+        capped = lbm_pair<OP, ADRF, false>(a, x, y);
     } else {
         int x, y;
         if (!block_origin(a, w.y, x, y)) return;
-        capped = lbm_cell<OP, FP32, false, false>(a, x, y);
+        capped = lbm_cell<OP, FP32, false, ADRF>(a, x, y);
     }
     for (int off = 32; off > 0; off >>= 1) capped += __shfl_down(capped, off);
     if ((threadIdx.x + threadIdx.y * blockDim.x) % warpSize == 0 && capped)
         atomicAdd(&a.counters[0], capped);
 }
 
-// The steady step's launch (previous-step shift force, no scalar) with a
-// linear grid of bx x by workgroups; swaps the population buffers as
-// launch_lbm does.
+// The steady step's launch (previous-step shift force; with a scalar, the
+// lagged scalar step of the previous step fused in, as one_step fuses it) with
+// a linear grid of bx x by workgroups; swaps the population (and scalar)
+// buffers as launch_lbm does.
 void batch_items_lbm(Ctx& c, int bx, int by, int pair, BatchItem* it) {
-    LbmArgs a = lbm_args(c, SHIFT_PREV, false);
+    const bool fuse = c.nsp > 0;
+    LbmArgs a = lbm_args(c, SHIFT_PREV, fuse);
     dim3 g, b;
     a.pair = pair && lbm_pair_ok(c) && !c.generic_flags ? 1 : 0;
     set_geometry(c, a, bx, by, 0, 0, g, b, a.pair ? c.nx / 2 : 0);
     batch_put(it[0], a.nblk, a);
     c.fcur = 1 - c.fcur;
+    if (fuse) {
+        c.gcur = 1 - c.gcur;
+        c.c_from_store = false;
+        c.adr_pending = true;       // this step's scalar step runs in the next collide-stream
+    }
 }
 
-template <bool FP32>
+template <bool FP32, bool ADRF>
 static void batch_lbm_op(const BatchVariant& v, const LbmArgs* packs, const int2* map, int nblocks,
                          hipStream_t s) {
     const dim3 g(nblocks), b(v.bx, v.by);
-    if (v.op == SF_REGULARIZED) hipLaunchKernelGGL((kb_lbm<SF_REGULARIZED, FP32>), g, b, 0, s, packs, map);
-    else if (v.op == SF_MRT) hipLaunchKernelGGL((kb_lbm<SF_MRT, FP32>), g, b, 0, s, packs, map);
-    else hipLaunchKernelGGL((kb_lbm<SF_BGK, FP32>), g, b, 0, s, packs, map);
+    if (v.op == SF_REGULARIZED) hipLaunchKernelGGL((kb_lbm<SF_REGULARIZED, FP32, ADRF>), g, b, 0, s, packs, map);
+    else if (v.op == SF_MRT) hipLaunchKernelGGL((kb_lbm<SF_MRT, FP32, ADRF>), g, b, 0, s, packs, map);
+    else hipLaunchKernelGGL((kb_lbm<SF_BGK, FP32, ADRF>), g, b, 0, s, packs, map);
 }
 
 void batch_launch_lbm(const BatchVariant& v, const void* packs, const int2* map, int nblocks,
                       hipStream_t s) {
     if (nblocks <= 0) return;
     const LbmArgs* p = static_cast<const LbmArgs*>(packs);
-    if (v.fp32) batch_lbm_op<true>(v, p, map, nblocks, s);
-    else batch_lbm_op<false>(v, p, map, nblocks, s);
+    if (v.fp32) {
+        if (v.adr) batch_lbm_op<true, true>(v, p, map, nblocks, s);
+        else batch_lbm_op<true, false>(v, p, map, nblocks, s);
+    } else {
+        if (v.adr) batch_lbm_op<false, true>(v, p, map, nblocks, s);
+        else batch_lbm_op<false, false>(v, p, map, nblocks, s);
+    }
 }
 
 // ---------------------------------------------------------------------------

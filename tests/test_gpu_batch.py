@@ -7,7 +7,8 @@ member-by-member first step, eager steps, captured six-step graphs, a change of
 launch shape and a member stepped on its own in between.  One member is also
 held to the FP64 oracle bar (relative L2 <= 1e-12).  Members differ in lattice
 size, viscosity, body force and capsule count; they mix the all-pairs and the
-cell-list contact paths and capsules of more and fewer than 64 nodes.
+cell-list contact paths and capsules of more and fewer than 64 nodes; some
+carry an advection-diffusion scalar.
 """
 import numpy as np
 import pytest
@@ -113,13 +114,49 @@ def test_batch_autotune_keeps_results():
     assert info[1] == sum(c["nx"] * c["ny"] for c in cases)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_members_with_scalar_match_single_runs(precision):
+    # members with an advection-diffusion scalar beside members without one:
+    # the batched collide-stream carries each scalar member's lagged scalar
+    # step as one_step fuses it; a host write of the scalar between batch runs
+    # sends the members through single steps until the fused state is back
+    cases = members(precision)
+    for i, c in enumerate(cases):
+        if i % 2 == 0:
+            c["scalar"] = dict(D=0.02 + 0.01 * i, blob=(c["nx"] / 2, c["ny"] / 2, 5.0))
+    print(f"batch with scalar members {precision}", flush=True)
+    sims = [product_sim(c) for c in cases]
+    twins = [product_sim(c) for c in cases]
+    ens = ensemble(sims)
+    for n in (1, 2, 17):
+        ens.run(n)
+    for t in twins:
+        t.run(20)
+    for i, (s, t) in enumerate(zip(sims, twins)):
+        assert_same(s, t, f"member {i} after 20 steps")
+        if "scalar" in cases[i]:
+            a = np.array(s.advectionDiffusion().concentration(0))
+            b = np.array(t.advectionDiffusion().concentration(0))
+            assert np.array_equal(a, b), f"member {i}: scalar differs"
+    for s in (sims[2], twins[2]):
+        s.advectionDiffusion().setPoint(10, 10, 2.0, 0)
+    ens.run(13)
+    for t in twins:
+        t.run(13)
+    for i, (s, t) in enumerate(zip(sims, twins)):
+        assert_same(s, t, f"member {i} after the host write")
+        if "scalar" in cases[i]:
+            a = np.array(s.advectionDiffusion().concentration(0))
+            b = np.array(t.advectionDiffusion().concentration(0))
+            assert np.array_equal(a, b), f"member {i}: scalar differs after the host write"
+
+
 def test_batch_rejects_unsupported_members():
     ok = members("fp32")[:2]
-    bad_scalar = dict(ok[0], name="scalar", scalar=dict(D=0.05, blob=(40, 17, 5.0)))
     bad_coll = dict(ok[0], name="mrt", coll=1)
     bad_mdf = dict(ok[0], name="mdf", mdf=3)
     bad_prec = dict(ok[0], name="fp64", precision="fp64")
-    for bad, msg in ((bad_scalar, "scalar"), (bad_coll, "collision model"), (bad_mdf, "multi-direct"),
+    for bad, msg in ((bad_coll, "collision model"), (bad_mdf, "multi-direct"),
                      (bad_prec, "precision")):
         sims = [product_sim(ok[1]), product_sim(bad)]
         ens = ensemble(sims)

@@ -206,3 +206,75 @@ def test_checkpoint_interval_files(tmp_path):
     for k in ("x", "y", "vx", "vy"):
         assert np.array_equal(sb[k], st[k]), k
     assert np.array_equal(np.asarray(b.field().distributions()), np.asarray(twin.field().distributions()))
+
+
+def _expected_layout(nx, ny, node_counts, nsp):
+    """Byte offsets the .sfck format fixes (io/checkpoint.cpp:10-58, 76-196):
+    the end of the version-4 content (where the extension block starts) and the
+    end of the file with the extension block, from the sizes alone."""
+    N = nx * ny
+    size = 4 + 5 * 4                      # magic; version, step, nx, ny, ncaps
+    size += 9 * N * 8 + 3 * N * 8 + N * 4  # f (SoA), rho, ux, uy, cell types
+    for n in node_counts:                 # id, type, nodes; positions, velocities; area, perimeter; periodic nx
+        size += 3 * 4 + 2 * (2 * n * 8) + 2 * 8 + 4
+    size += 4 + 4                         # adhesion bond count (0), has_scalar
+    if nsp:
+        size += 4 + nsp * (9 * N * 8 + N * 8) + 3 * 4   # species; g, C per species; 3 tracker counts
+    ext = 4 + 4 + 2 * N * 8 + sum(4 + n * 8 for n in node_counts)   # SFXT, version, force, KV state
+    return size, size + ext
+
+
+@pytest.mark.parametrize("scalar", [True, False])
+def test_file_size_follows_from_the_sizes(tmp_path, scalar):
+    case = dict(CASE, precision="fp64")
+    if not scalar:
+        case.pop("scalar")
+    a = product_sim(case)
+    a.run(9)
+    path = str(tmp_path / "sz.sfck")
+    assert a.saveCheckpoint(path)
+    caps = a.capsules()
+    counts = [caps[i].numNodes() for i in range(len(caps))]
+    v4_end, total = _expected_layout(96, 40, counts, 1 if scalar else 0)
+    d = _read_sfck(path)
+    assert d["ext_offset"] == v4_end and d["size"] == total == os.path.getsize(path)
+    assert d["has_scalar"] == (1 if scalar else 0)
+
+
+def test_versions_2_and_3_are_accepted_and_others_refused(tmp_path):
+    # a version-3 file is the version-4 content without the two particle
+    # counts after the tracker count; a version-2 file ends after the bond
+    # count (io/checkpoint.cpp:213-222, 317-356)
+    case = dict(CASE, precision="fp64")
+    a = product_sim(case)
+    a.run(12)
+    path = str(tmp_path / "v4.sfck")
+    assert a.saveCheckpoint(path)
+    d = _read_sfck(path)
+    raw = open(path, "rb").read()
+    v4 = raw[:d["ext_offset"]]
+    ver = np.frombuffer(v4, dtype="<i4", count=1, offset=4)[0]
+    assert ver == 4
+
+    def with_version(body, v):
+        return body[:4] + np.array([v], dtype="<i4").tobytes() + body[8:]
+    v3 = with_version(v4[:-8], 3)
+    bonds_end = d["ext_offset"] - (4 + 4 + 9 * 96 * 40 * 8 + 96 * 40 * 8 + 12)   # before has_scalar
+    v2 = with_version(v4[:bonds_end], 2)
+    fl = np.asarray(a.field().distributions()).copy()
+    for name, body in (("v3", v3), ("v2", v2)):
+        p = str(tmp_path / f"{name}.sfck")
+        open(p, "wb").write(body)
+        b = product_sim(case)
+        b.run(3)
+        assert b.loadCheckpoint(p), name
+        assert b.currentStep() == 12
+        assert np.array_equal(np.asarray(b.field().distributions()), fl), name
+        if name == "v3":
+            assert np.array_equal(np.asarray(b.advectionDiffusion().concentration(0)),
+                                  np.asarray(a.advectionDiffusion().concentration(0)))
+    for bad in (1, 5):
+        p = str(tmp_path / f"v{bad}.sfck")
+        open(p, "wb").write(with_version(v4, bad))
+        b = product_sim(case)
+        assert not b.loadCheckpoint(p), bad
