@@ -305,9 +305,11 @@ int  sf_mdf_set_model(sf_ctx* ctx, sf_mlp* net, int warm);
  * six steps (the period of the buffer rotation) are captured as one hipGraph.
  * Every member's results are bit-identical to sf_step on that context alone.
  * Members must share precision and collision model and be x-periodic plain
- * channels (no obstacle flags) with deformable capsules and one IBM pass per
- * step; members may carry an advection-diffusion scalar (its lagged step rides
- * in the batched collide-stream, as in sf_step).  The contexts must outlive the batch; calls on a member
+ * channels (no obstacle flags) with deformable capsules.  Members may carry an
+ * advection-diffusion scalar (its lagged step rides in the batched
+ * collide-stream, as in sf_step) or apply multi-direct forcing, fixed or
+ * residual-controlled (each member its own count and tolerance), but not both,
+ * and not the learned warm start or sample collection.  The contexts must outlive the batch; calls on a member
  * between batch steps are allowed (the batch re-plans when a member's buffers
  * or roles changed). */
 typedef struct sf_batch sf_batch;

@@ -44,18 +44,73 @@ static bool steady_member(const Ctx& c) {
     return c.shift_next == SHIFT_PREV && (c.nsp == 0 || (c.adr_pending && !c.c_from_store));
 }
 
-// The items of one steady step of member c, in one_step's launch order
-// (force-buffer rotation, membrane, contact, spread, collide-stream,
-// interpolation + move); advances c's buffer roles as one_step does.
-static void record_step(Ctx& c, int bx, int by, int pair, BatchItem* it) {
+// MDF corrections a member applies per step (mdf_corrections in sf_api.hip)
+static int member_corrections(const Ctx& c) {
+    const int n = c.mdf_tol > 0.0 ? c.mdf_max_corr : c.p.ibm_iterations - 1;
+    return c.K > 0 && n > 0 ? n : 0;
+}
+
+// The launches of one batched step, in one_step's order: slot k launches the
+// kernel kinds[k] over every member's items in that slot.  The membrane and
+// cell-sort slots (0 and 2) share one launch.  With multi-direct forcing
+// (C corrections at most over the members) the spread is followed by C
+// corrections of four launches each, an optional residual probe and the
+// statistics fold; members with fewer corrections, or none, take no part in
+// the slots beyond theirs.
+struct StepLayout {
+    int C = 0;        // corrections of the member that applies the most
+    bool probe = false;
+    std::vector<int> kinds;
+    int lbm = 0;      // slot of the collide-stream
+    void build(int C_, bool probe_) {
+        C = C_;
+        probe = probe_;
+        kinds = {BK_MEMBRANE, BK_CELLS_FILL, BK_CELLS_SORT, BK_CONTACT};
+        if (C > 0) kinds.push_back(BK_MDF_BEGIN);
+        kinds.insert(kinds.end(), {BK_NODE_PREP, BK_SPREAD_REST, BK_SPREAD});
+        for (int i = 0; i < C; ++i) kinds.insert(kinds.end(), {BK_MDF_INTERP, BK_MDF_PREP, BK_MDF_REST, BK_MDF_SPREAD});
+        if (probe) kinds.push_back(BK_MDF_INTERP);
+        if (C > 0) kinds.push_back(BK_MDF_FINISH);
+        lbm = static_cast<int>(kinds.size());
+        kinds.insert(kinds.end(), {BK_LBM, BK_INTERP});
+    }
+    int slots() const { return static_cast<int>(kinds.size()); }
+};
+
+// The items of one steady step of member c in the layout's slots; advances
+// c's buffer roles as one_step does (force-buffer rotation, membrane, contact,
+// spread and the MDF corrections, collide-stream, interpolation + move).
+static void record_step(Ctx& c, const StepLayout& L, int bx, int by, int pair, BatchItem* it) {
     const int spare = 3 - c.Fcur - c.Fprev;
     c.Fprev = c.Fcur;
     c.Fcur = spare;
     batch_items_membrane(c, it + BK_MEMBRANE);
     batch_items_contact(c, it + BK_CELLS_FILL);
-    batch_items_spread(c, it + BK_NODE_PREP);
-    batch_items_lbm(c, bx, by, pair, it + BK_LBM);
-    batch_items_interp(c, it + BK_INTERP);
+    int k = BK_CONTACT + 1;
+    const int ncorr = member_corrections(c);
+    if (L.C > 0) {
+        if (ncorr > 0) batch_items_mdf_begin(c, it[k]);
+        else batch_items_none(BK_MDF_BEGIN, it[k]);
+        ++k;
+    }
+    batch_items_spread(c, it + k, ncorr == 0);
+    k += 3;
+    for (int i = 1; i <= L.C; ++i, k += 4) {
+        if (i <= ncorr) batch_items_mdf_correction(c, i, ncorr, it + k);
+        else for (int j = 0; j < 4; ++j) batch_items_none(L.kinds[k + j], it[k + j]);
+    }
+    if (L.probe) {
+        if (ncorr > 0 && c.mdf_probe) batch_items_mdf_interp(c, ncorr + 1, it[k]);
+        else batch_items_none(BK_MDF_INTERP, it[k]);
+        ++k;
+    }
+    if (L.C > 0) {
+        if (ncorr > 0) batch_items_mdf_finish(c, ncorr, it[k]);
+        else batch_items_none(BK_MDF_FINISH, it[k]);
+        ++k;
+    }
+    batch_items_lbm(c, bx, by, pair, ncorr > 0 ? SHIFT_PARTIAL : SHIFT_PREV, it + k);
+    batch_items_interp(c, it + k + 1);
     c.shift_next = SHIFT_PREV;
     c.step_count++;
 }
@@ -115,9 +170,10 @@ struct sf_batch {
     std::vector<Roles> roles[BATCH_PERIOD + 1];   // roles before step s of the period
     int phase = 0;
     unsigned char* dpacks = nullptr;
-    size_t off[BATCH_PERIOD][BK_COUNT] = {};
-    int2* dmap[BK_COUNT] = {};
-    int nblk[BK_COUNT] = {};
+    StepLayout layout;
+    std::vector<size_t> off[BATCH_PERIOD];   // per slot
+    std::vector<int2*> dmap;
+    std::vector<int> nblk;
     bool graph_valid = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -142,16 +198,11 @@ static void drop_plan(sf_batch& B) {
     drop_graph(B);
     if (B.dpacks) hipFree(B.dpacks);
     B.dpacks = nullptr;
-    for (int k = 0; k < BK_COUNT; ++k) {
-        if (B.dmap[k]) hipFree(B.dmap[k]);
-        B.dmap[k] = nullptr;
-        B.nblk[k] = 0;
-    }
+    for (int2* m : B.dmap)
+        if (m) hipFree(m);
+    B.dmap.clear();
+    B.nblk.clear();
     B.planned = false;
-}
-
-static int mdf_corrections_of(const Ctx& c) {
-    return c.mdf_tol > 0.0 ? c.mdf_max_corr : c.p.ibm_iterations - 1;
 }
 
 // Members the batched step covers (the checks name the first one that fails).
@@ -169,7 +220,12 @@ static int validate(sf_batch& B) {
             return fail(SF_E_ARG, who + "batches take x-periodic plain channels (no obstacle flags)");
         if (c.nobs > 0 || c.p.use_interpolated_bb)
             return fail(SF_E_ARG, who + "obstacles and interpolated bounce-back are not batched");
-        if (mdf_corrections_of(c) > 0) return fail(SF_E_ARG, who + "multi-direct forcing is not batched");
+        if (member_corrections(c) > 0) {
+            if (c.nsp > 0)
+                return fail(SF_E_ARG, who + "multi-direct forcing with a scalar is not batched");
+            if ((c.mdf_warm && c.model) || c.mdf_hist_on || c.ds_buf)
+                return fail(SF_E_ARG, who + "the learned warm start and sample collection are not batched");
+        }
         if (c.ncaps == 0 || c.any_rigid)
             return fail(SF_E_ARG, who + "batched members carry deformable capsules");
     }
@@ -187,14 +243,22 @@ static int build_plan(sf_batch& B) {
     B.v.adr = 0;
     for (Ctx* c : B.mem) B.v.adr |= c->nsp > 0 ? 1 : 0;
 
-    std::vector<BatchItem> items(static_cast<size_t>(BATCH_PERIOD) * M * BK_COUNT);
+    int C = 0;
+    bool probe = false;
+    for (Ctx* c : B.mem) {
+        C = std::max(C, member_corrections(*c));
+        probe = probe || (member_corrections(*c) > 0 && c->mdf_probe);
+    }
+    B.layout.build(C, probe);
+    const int S = B.layout.slots();
+    std::vector<BatchItem> items(static_cast<size_t>(BATCH_PERIOD) * M * S);
     std::vector<int64_t> steps0(M);
     for (int m = 0; m < M; ++m) steps0[m] = B.mem[m]->step_count;
     for (int s = 0; s <= BATCH_PERIOD; ++s) B.roles[s].assign(M, Roles{});
     for (int s = 0; s < BATCH_PERIOD; ++s)
         for (int m = 0; m < M; ++m) {
             B.roles[s][m] = roles_of(*B.mem[m]);
-            record_step(*B.mem[m], B.v.bx, B.v.by, B.v.pair, &items[(static_cast<size_t>(s) * M + m) * BK_COUNT]);
+            record_step(*B.mem[m], B.layout, B.v.bx, B.v.by, B.v.pair, &items[(static_cast<size_t>(s) * M + m) * S]);
         }
     for (int m = 0; m < M; ++m) {
         B.roles[BATCH_PERIOD][m] = roles_of(*B.mem[m]);
@@ -205,12 +269,13 @@ static int build_plan(sf_batch& B) {
         }
     }
     auto item = [&](int s, int m, int k) -> const BatchItem& {
-        return items[(static_cast<size_t>(s) * M + m) * BK_COUNT + k];
+        return items[(static_cast<size_t>(s) * M + m) * S + k];
     };
-    // pack layout: [step][kind] -> M packs of one size
+    // pack layout: [step][slot] -> M packs of one size
     size_t total = 0;
+    for (int s = 0; s < BATCH_PERIOD; ++s) B.off[s].assign(S, 0);
     for (int s = 0; s < BATCH_PERIOD; ++s)
-        for (int k = 0; k < BK_COUNT; ++k) {
+        for (int k = 0; k < S; ++k) {
             const size_t sz = item(s, 0, k).pack.size();
             for (int m = 0; m < M; ++m) {
                 if (item(s, m, k).pack.size() != sz || item(s, m, k).blocks != item(0, m, k).blocks)
@@ -221,14 +286,16 @@ static int build_plan(sf_batch& B) {
         }
     std::vector<unsigned char> host(total, 0);
     for (int s = 0; s < BATCH_PERIOD; ++s)
-        for (int k = 0; k < BK_COUNT; ++k) {
+        for (int k = 0; k < S; ++k) {
             const size_t sz = item(s, 0, k).pack.size();
             for (int m = 0; m < M; ++m)
                 if (sz) std::memcpy(host.data() + B.off[s][k] + m * sz, item(s, m, k).pack.data(), sz);
         }
+    B.dmap.assign(S, nullptr);
+    B.nblk.assign(S, 0);
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&B.dpacks), std::max<size_t>(total, 256)));
     SF_HIP(hipMemcpy(B.dpacks, host.data(), total, hipMemcpyHostToDevice));
-    for (int k = 0; k < BK_COUNT; ++k) {
+    for (int k = 0; k < S; ++k) {
         if (k == BK_CELLS_SORT) continue;     // its workgroups follow the membrane ones
         std::vector<int> blocks(M);
         for (int m = 0; m < M; ++m) blocks[m] = item(0, m, k).blocks;
@@ -241,7 +308,7 @@ static int build_plan(sf_batch& B) {
                 for (int b = 0; b < item(0, m, BK_CELLS_SORT).blocks; ++b) map.push_back(make_int2(M + m, b));
             }
         } else {
-            map = make_map(blocks, k == BK_LBM && B.banded);
+            map = make_map(blocks, k == B.layout.lbm && B.banded);
         }
         B.nblk[k] = static_cast<int>(map.size());
         if (map.empty()) continue;
@@ -259,11 +326,13 @@ static int build_plan(sf_batch& B) {
 // in one launch (the sort needs only node positions), then the rest in order.
 static void launch_step(sf_batch& B, int s) {
     const int M = static_cast<int>(B.mem.size());
-    auto run = [&](int k) { launch_kind(k, B.v, B.dpacks + B.off[s][k], B.dmap[k], B.nblk[k], B.stream); };
+    auto run = [&](int k) {
+        launch_kind(B.layout.kinds[k], B.v, B.dpacks + B.off[s][k], B.dmap[k], B.nblk[k], B.stream);
+    };
     run(BK_CELLS_FILL);
     batch_launch_membrane_sort(B.v, B.dpacks + B.off[s][BK_MEMBRANE], B.dpacks + B.off[s][BK_CELLS_SORT],
                                B.dmap[BK_MEMBRANE], B.nblk[BK_MEMBRANE], M, B.stream);
-    for (int k = BK_CONTACT; k < BK_COUNT; ++k) run(k);
+    for (int k = BK_CONTACT; k < B.layout.slots(); ++k) run(k);
 }
 
 // The plan still describes every member: same buffers, parameters and roles.
@@ -462,7 +531,8 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
             Ctx& c = *B.mem[m];
             const int f0 = c.fcur, g0 = c.gcur;
             const bool store0 = c.c_from_store, pend0 = c.adr_pending;
-            batch_items_lbm(c, sh[0], sh[1], pair, &it[m]);
+            batch_items_lbm(c, sh[0], sh[1], pair, member_corrections(c) > 0 ? SHIFT_PARTIAL : SHIFT_PREV,
+                            &it[m]);
             c.fcur = f0;
             c.gcur = g0;
             c.c_from_store = store0;

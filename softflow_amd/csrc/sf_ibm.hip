@@ -602,16 +602,16 @@ __device__ __forceinline__ void interp_group(const CellVelArgs& a, const double*
 // (md->res[it], an order-independent max) and, when u0 is set, the slip
 // V - U per node.  After convergence (residual mode) it does nothing.
 template <bool FP32>
-__global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double* lut, const double* x,
-                                                    const double* y, const double* vx, const double* vy,
-                                                    NodeSpread* rec, int K, MdfDev* md, int it,
-                                                    double* u0) {
+__device__ __forceinline__ void mdf_interp_body(const CellVelArgs& a, const double* lut, const double* x,
+                                                const double* y, const double* vx, const double* vy,
+                                                NodeSpread* rec, int K, MdfDev* md, int it, double* u0,
+                                                int bx) {
     __shared__ double tx[256], ty[256];
     __shared__ int use[256];
     __shared__ double sres[INTERP_NODES];
     if (md->conv) return;       // uniform: no kernel writes it while this one runs
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    const int k = blockIdx.x * INTERP_NODES + g;
+    const int k = bx * INTERP_NODES + g;
     const bool live = k < K;
     const double px = live ? x[k] : 0.0, py = live ? y[k] : 0.0;
     double Ux, Uy;
@@ -645,6 +645,14 @@ __global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double*
         // non-negative doubles order like their bit patterns (NaN above all)
         atomicMax(&md->res[it], static_cast<unsigned long long>(__double_as_longlong(m)));
     }
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double* lut, const double* x,
+                                                    const double* y, const double* vx, const double* vy,
+                                                    NodeSpread* rec, int K, MdfDev* md, int it,
+                                                    double* u0) {
+    mdf_interp_body<FP32>(a, lut, x, y, vx, vy, rec, K, md, it, u0, blockIdx.x);
 }
 
 // Per-node inputs of the warm-start model, in FP32: slip V - U at the first
@@ -692,11 +700,15 @@ __global__ void k_mdf_features(const double* x, const double* y, const double* f
 }
 
 // Clears the per-step MDF state (graph-safe: a kernel, not a memset).
-__global__ void k_mdf_begin(MdfDev* md, double* total, double* head, int K) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void mdf_begin_body(MdfDev* md, double* total, double* head, int K, int bx) {
+    const int i = bx * blockDim.x + threadIdx.x;
     if (i == 0) { md->conv = 0; md->applied = 0; md->conv_it = 0; }
     if (i < MDF_MAX_CORR + 2) md->res[i] = 0ull;
     if (i < 2 * K) { total[i] = 0.0; head[i] = 0.0; }
+}
+
+__global__ void k_mdf_begin(MdfDev* md, double* total, double* head, int K) {
+    mdf_begin_body(md, total, head, K, blockIdx.x);
 }
 
 // The model's targets: the sum T of the step's first MDF_HEAD plain
@@ -747,7 +759,8 @@ __global__ void k_mdf_record(MdfDev* md, const double* total, const double* head
     }
 }
 
-__global__ void k_mdf_finish(MdfDev* md, int ncorr, int probe, int K, int collect, int64_t ds_cap) {
+__device__ __forceinline__ void mdf_finish_body(MdfDev* md, int ncorr, int probe, int K, int collect,
+                                                int64_t ds_cap) {
     const int it_final = md->conv ? md->conv_it : (probe ? ncorr + 1 : ncorr);
     const double r = __longlong_as_double(static_cast<long long>(md->res[it_final]));
     md->sum_applied += md->applied;
@@ -755,6 +768,10 @@ __global__ void k_mdf_finish(MdfDev* md, int ncorr, int probe, int K, int collec
     md->max_final = (r > md->max_final || r != r) ? r : md->max_final;
     md->steps += 1.0;
     if (collect && md->ds_rows + K <= ds_cap) md->ds_rows += K;
+}
+
+__global__ void k_mdf_finish(MdfDev* md, int ncorr, int probe, int K, int collect, int64_t ds_cap) {
+    mdf_finish_body(md, ncorr, probe, K, collect, ds_cap);
 }
 
 // interpolateVelocity (immersed_boundary.cpp:174-189) with the post-step
@@ -1018,6 +1035,8 @@ void launch_interp_move(Ctx& c) {
 // ---------------------------------------------------------------------------
 // Batched launches over the members of an ensemble (map[b] = member and its
 // own workgroup index), for the steady step: node forces spread once, no MDF.
+// mode, md, tol, it, total, head: node_prep_body's MDF arguments (mode 1: an
+// MDF correction; md null in mode 0)
 struct PrepPack {
     const double *x, *y, *fx, *fy;
     const int *off, *owner;
@@ -1027,6 +1046,11 @@ struct PrepPack {
     int* tile_list;
     NodeSpread* tile_rec;
     int* counters;
+    int mode;
+    MdfDev* md;
+    double tol;
+    int it;
+    double *total, *head;
 };
 struct RestPack {
     const int* tile_cnt;
@@ -1037,6 +1061,9 @@ struct RestPack {
     int2* act;
     int* counters;
     int nx, ny, ntx, ntiles;
+    int mode;
+    const MdfDev* md;
+    int it, last_it;
 };
 struct SpreadPack {
     const NodeSpread* rec;
@@ -1050,7 +1077,39 @@ struct SpreadPack {
     void* Fout;
     uint8_t* dout;
     int nx, ny, ntx, ntiles, gx;
+    int mode, last;
+    const MdfDev* md;
 };
+struct MdfBeginPack {
+    MdfDev* md;
+    double *total, *head;
+    int K;
+};
+struct MdfInterpPack {
+    CellVelArgs a;
+    const double* lut;
+    const double *x, *y, *vx, *vy;
+    NodeSpread* rec;
+    int K;
+    MdfDev* md;
+    int it;
+};
+struct MdfFinishPack {
+    MdfDev* md;
+    int ncorr, probe, K;
+};
+
+size_t batch_pack_size(int kind) {
+    switch (kind) {
+    case BK_NODE_PREP: case BK_MDF_PREP: return sizeof(PrepPack);
+    case BK_SPREAD_REST: case BK_MDF_REST: return sizeof(RestPack);
+    case BK_SPREAD: case BK_MDF_SPREAD: return sizeof(SpreadPack);
+    case BK_MDF_BEGIN: return sizeof(MdfBeginPack);
+    case BK_MDF_INTERP: return sizeof(MdfInterpPack);
+    case BK_MDF_FINISH: return sizeof(MdfFinishPack);
+    default: return 0;
+    }
+}
 struct InterpPack {
     CellVelArgs a;
     MoveArgs mv;
@@ -1064,8 +1123,8 @@ __global__ __launch_bounds__(256) void kb_node_prep(const PrepPack* packs, const
     if (w.x < 0) return;
     const PrepPack p = packs[w.x];
     node_prep_body(p.x, p.y, p.fx, p.fy, p.off, p.owner, p.rec, p.K, p.nx, p.ny, p.ntx, p.nty,
-                   p.tile_cnt, p.tile_list, p.tile_rec, p.counters, 0, nullptr, 0.0, 0, 0, nullptr,
-                   nullptr, w.y);
+                   p.tile_cnt, p.tile_list, p.tile_rec, p.counters, p.mode, p.md, p.tol, p.it, 0, p.total,
+                   p.head, w.y);
 }
 
 template <bool FP32>
@@ -1074,7 +1133,7 @@ __global__ __launch_bounds__(256) void kb_spread_rest(const RestPack* packs, con
     if (w.x < 0) return;
     const RestPack p = packs[w.x];
     spread_rest_body<FP32>(p.tile_cnt, p.Fin, p.Fout, p.din, p.dout, p.act, p.counters, p.nx, p.ny,
-                           p.ntx, p.ntiles, 0, nullptr, w.y);
+                           p.ntx, p.ntiles, p.mode, p.md, w.y, p.it, p.last_it);
 }
 
 template <bool FP32>
@@ -1083,7 +1142,30 @@ __global__ __launch_bounds__(256) void kb_spread(const SpreadPack* packs, const 
     if (w.x < 0) return;
     const SpreadPack p = packs[w.x];
     spread_body<FP32>(p.rec, p.tile_rec, p.tile_cnt, p.tile_list, p.act, p.counters, p.lut, p.Fin,
-                      p.Fout, p.dout, p.nx, p.ny, p.ntx, p.ntiles, 0, 1, nullptr, w.y, p.gx);
+                      p.Fout, p.dout, p.nx, p.ny, p.ntx, p.ntiles, p.mode, p.last, p.md, w.y, p.gx);
+}
+
+__global__ __launch_bounds__(256) void kb_mdf_begin(const MdfBeginPack* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const MdfBeginPack p = packs[w.x];
+    mdf_begin_body(p.md, p.total, p.head, p.K, w.y);
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(256) void kb_mdf_interp(const MdfInterpPack* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0) return;
+    const MdfInterpPack p = packs[w.x];
+    mdf_interp_body<FP32>(p.a, p.lut, p.x, p.y, p.vx, p.vy, p.rec, p.K, p.md, p.it, nullptr, w.y);
+}
+
+// one workgroup per member; its first thread folds the step's MDF statistics
+__global__ __launch_bounds__(256) void kb_mdf_finish(const MdfFinishPack* packs, const int2* map) {
+    const int2 w = map[blockIdx.x];
+    if (w.x < 0 || threadIdx.x != 0) return;
+    const MdfFinishPack p = packs[w.x];
+    mdf_finish_body(p.md, p.ncorr, p.probe, p.K, 0, 0);
 }
 
 template <bool FP32>
@@ -1099,22 +1181,70 @@ __global__ __launch_bounds__(256) void kb_interp(const InterpPack* packs, const 
 // than tiles).
 constexpr int BATCH_SPREAD_GRID = 64;
 
-// The arguments of launch_spread(c, 0, true) (no capsules: not batched).
-void batch_items_spread(Ctx& c, BatchItem* it) {
+// The arguments of launch_spread(c, 0, last) (no capsules: not batched).
+void batch_items_spread(Ctx& c, BatchItem* it, bool last) {
     batch_put(it[0], (c.K + 255) / 256,
               PrepPack{c.x, c.y, c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                       c.tile_cnt, c.tile_list, c.tile_rec, c.counters});
+                       c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 0, nullptr, 0.0, 0, nullptr, nullptr});
     const void* Fin = c.F[c.Fcur];
     void* Fout = c.F[c.Fcur];
     const uint8_t* din = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
     uint8_t* dout = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
     batch_put(it[1], (c.ntiles + rest_tiles_per_block(0) - 1) / rest_tiles_per_block(0),
               RestPack{c.tile_cnt, Fin, Fout, din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx,
-                       c.ntiles});
+                       c.ntiles, 0, nullptr, 0, 0});
     const int gx = std::min(c.ntiles, BATCH_SPREAD_GRID);
     batch_put(it[2], gx,
               SpreadPack{c.spread, c.tile_rec, c.tile_cnt, c.tile_list, c.tile_act, c.counters, c.phi,
-                         Fin, Fout, dout, c.nx, c.ny, c.ntx, c.ntiles, gx});
+                         Fin, Fout, dout, c.nx, c.ny, c.ntx, c.ntiles, gx, 0, last ? 1 : 0, nullptr});
+}
+
+// launch_mdf_begin's arguments
+void batch_items_mdf_begin(Ctx& c, BatchItem& it) {
+    const int n = std::max(2 * c.K, MDF_MAX_CORR + 2);
+    batch_put(it, (n + 255) / 256, MdfBeginPack{c.mdf, c.mdf_total, c.mdf_head, c.K});
+}
+
+// launch_mdf_interp(c, i)'s arguments (no model: no first-slip record)
+void batch_items_mdf_interp(Ctx& c, int i, BatchItem& it) {
+    batch_put(it, (c.K + INTERP_NODES - 1) / INTERP_NODES,
+              MdfInterpPack{vel_args(c, c.f[c.fcur], c.F[c.Fcur]), c.phi, c.x, c.y, c.vx, c.vy, c.spread,
+                            c.K, c.mdf, i});
+}
+
+// MDF correction i of ncorr: launch_mdf_interp(c, i) then launch_spread(c, 1,
+// i == ncorr, i, ncorr); rotates the force roles as launch_spread does
+void batch_items_mdf_correction(Ctx& c, int i, int ncorr, BatchItem* it) {
+    batch_items_mdf_interp(c, i, it[0]);
+    batch_put(it[1], (c.K + 255) / 256,
+              PrepPack{c.x, c.y, c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
+                       c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 1, c.mdf, c.mdf_tol, i,
+                       c.mdf_total, c.mdf_head});
+    const void* Fin = c.F[c.Fcur];
+    const int out = 3 - c.Fcur - c.Fprev;
+    void* Fout = c.F[out];
+    const uint8_t* din = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
+    uint8_t* dout = c.tile_dirty + static_cast<size_t>(out) * c.ntiles;
+    batch_put(it[2], (c.ntiles + rest_tiles_per_block(1) - 1) / rest_tiles_per_block(1),
+              RestPack{c.tile_cnt, Fin, Fout, din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx,
+                       c.ntiles, 1, c.mdf, i, ncorr});
+    const int gx = std::min(c.ntiles, BATCH_SPREAD_GRID);
+    batch_put(it[3], gx,
+              SpreadPack{c.spread, c.tile_rec, c.tile_cnt, c.tile_list, c.tile_act, c.counters, c.phi,
+                         Fin, Fout, dout, c.nx, c.ny, c.ntx, c.ntiles, gx, 1, i == ncorr ? 1 : 0, c.mdf});
+    c.Fpart = c.Fcur;
+    c.Fcur = out;
+}
+
+// launch_mdf_record's statistics (no model, no sample collection)
+void batch_items_mdf_finish(Ctx& c, int ncorr, BatchItem& it) {
+    batch_put(it, 1, MdfFinishPack{c.mdf, ncorr, c.mdf_probe, c.K});
+}
+
+// a slot this member takes no part in: no workgroups, a pack of the slot's size
+void batch_items_none(int kind, BatchItem& it) {
+    it.blocks = 0;
+    it.pack.assign(batch_pack_size(kind), 0);
 }
 
 // The arguments of launch_interp_move (deformable capsules only).
@@ -1145,6 +1275,27 @@ void batch_launch_ibm(int kind, const BatchVariant& v, const void* packs, const 
     case BK_INTERP:
         if (v.fp32) hipLaunchKernelGGL(kb_interp<true>, g, b, 0, s, static_cast<const InterpPack*>(packs), map);
         else hipLaunchKernelGGL(kb_interp<false>, g, b, 0, s, static_cast<const InterpPack*>(packs), map);
+        break;
+    case BK_MDF_BEGIN:
+        hipLaunchKernelGGL(kb_mdf_begin, g, b, 0, s, static_cast<const MdfBeginPack*>(packs), map);
+        break;
+    case BK_MDF_INTERP:
+        if (v.fp32) hipLaunchKernelGGL(kb_mdf_interp<true>, g, b, 0, s, static_cast<const MdfInterpPack*>(packs), map);
+        else hipLaunchKernelGGL(kb_mdf_interp<false>, g, b, 0, s, static_cast<const MdfInterpPack*>(packs), map);
+        break;
+    case BK_MDF_PREP:
+        hipLaunchKernelGGL(kb_node_prep, g, b, 0, s, static_cast<const PrepPack*>(packs), map);
+        break;
+    case BK_MDF_REST:
+        if (v.fp32) hipLaunchKernelGGL(kb_spread_rest<true>, g, b, 0, s, static_cast<const RestPack*>(packs), map);
+        else hipLaunchKernelGGL(kb_spread_rest<false>, g, b, 0, s, static_cast<const RestPack*>(packs), map);
+        break;
+    case BK_MDF_SPREAD:
+        if (v.fp32) hipLaunchKernelGGL(kb_spread<true>, g, b, 0, s, static_cast<const SpreadPack*>(packs), map);
+        else hipLaunchKernelGGL(kb_spread<false>, g, b, 0, s, static_cast<const SpreadPack*>(packs), map);
+        break;
+    case BK_MDF_FINISH:
+        hipLaunchKernelGGL(kb_mdf_finish, g, b, 0, s, static_cast<const MdfFinishPack*>(packs), map);
         break;
     default:
         break;

@@ -320,7 +320,15 @@ enum BatchKernel : int {
     BK_SPREAD,
     BK_LBM,
     BK_INTERP,
-    BK_COUNT
+    BK_COUNT,
+    // multi-direct forcing (a step holds begin, the corrections' four launches
+    // each, an optional residual probe and the statistics fold)
+    BK_MDF_BEGIN = BK_COUNT,
+    BK_MDF_INTERP,
+    BK_MDF_PREP,
+    BK_MDF_REST,
+    BK_MDF_SPREAD,
+    BK_MDF_FINISH
 };
 
 struct BatchItem {
@@ -345,13 +353,18 @@ inline void batch_put(BatchItem& it, int blocks, const P& p) {
     std::memcpy(it.pack.data(), &p, sizeof(P));
 }
 
-// Items of one steady step (previous-step shift force, one IBM pass, no
-// scalar), in launch order.  Each call advances the member's buffer roles
-// exactly as the single-context launcher does.
+// Items of one steady step, in launch order.  Each call advances the member's
+// buffer roles exactly as the single-context launcher does.
 void batch_items_membrane(Ctx& c, BatchItem* it);          // BK_MEMBRANE
 void batch_items_contact(Ctx& c, BatchItem* it);           // BK_CELLS_FILL .. BK_CONTACT
-void batch_items_spread(Ctx& c, BatchItem* it);            // BK_NODE_PREP .. BK_SPREAD
-void batch_items_lbm(Ctx& c, int bx, int by, int pair, BatchItem* it);   // BK_LBM
+void batch_items_spread(Ctx& c, BatchItem* it, bool last); // BK_NODE_PREP .. BK_SPREAD
+void batch_items_mdf_begin(Ctx& c, BatchItem& it);         // BK_MDF_BEGIN
+void batch_items_mdf_interp(Ctx& c, int i, BatchItem& it); // BK_MDF_INTERP (the residual probe)
+void batch_items_mdf_correction(Ctx& c, int i, int ncorr, BatchItem* it);   // BK_MDF_INTERP .. BK_MDF_SPREAD
+void batch_items_mdf_finish(Ctx& c, int ncorr, BatchItem& it);              // BK_MDF_FINISH
+void batch_items_none(int kind, BatchItem& it);            // a slot the member takes no part in
+size_t batch_pack_size(int kind);
+void batch_items_lbm(Ctx& c, int bx, int by, int pair, int shift, BatchItem* it);   // BK_LBM
 void batch_items_interp(Ctx& c, BatchItem* it);            // BK_INTERP
 // launch kind over nblocks workgroups of the map
 void batch_launch_membrane(int kind, const BatchVariant& v, const void* packs, const int2* map,

@@ -1152,9 +1152,9 @@ __global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* 
 // lagged scalar step of the previous step fused in, as one_step fuses it) with
 // a linear grid of bx x by workgroups; swaps the population (and scalar)
 // buffers as launch_lbm does.
-void batch_items_lbm(Ctx& c, int bx, int by, int pair, BatchItem* it) {
+void batch_items_lbm(Ctx& c, int bx, int by, int pair, int shift, BatchItem* it) {
     const bool fuse = c.nsp > 0;
-    LbmArgs a = lbm_args(c, SHIFT_PREV, fuse);
+    LbmArgs a = lbm_args(c, shift, fuse);
     dim3 g, b;
     a.pair = pair && lbm_pair_ok(c) && !c.generic_flags ? 1 : 0;
     set_geometry(c, a, bx, by, 0, 0, g, b, a.pair ? c.nx / 2 : 0);

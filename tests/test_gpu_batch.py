@@ -8,7 +8,7 @@ launch shape and a member stepped on its own in between.  One member is also
 held to the FP64 oracle bar (relative L2 <= 1e-12).  Members differ in lattice
 size, viscosity, body force and capsule count; they mix the all-pairs and the
 cell-list contact paths and capsules of more and fewer than 64 nodes; some
-carry an advection-diffusion scalar.
+carry an advection-diffusion scalar, some apply multi-direct forcing.
 """
 import numpy as np
 import pytest
@@ -151,12 +151,52 @@ def test_batch_members_with_scalar_match_single_runs(precision):
             assert np.array_equal(a, b), f"member {i}: scalar differs after the host write"
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_members_with_mdf_match_single_runs(precision):
+    # multi-direct forcing members beside plain ones: fixed 2 and 4
+    # corrections, and residual control (each member its own tolerance and
+    # cap, corrections stopping at different iterations in different steps)
+    cases = members(precision)
+    cases[0]["mdf"] = 3
+    cases[1]["mdf"] = 5
+    control = {2: (None, 6), 3: (None, 3)}      # member: (tolerance from its first residual, cap)
+    print(f"batch with MDF members {precision}", flush=True)
+    sims = [product_sim(c) for c in cases]
+    twins = [product_sim(c) for c in cases]
+    for i, (cap_tol, cap) in control.items():
+        for s in (sims[i], twins[i]):
+            s.run(20)
+            s.setMdfControl(1e-300, 1, True)
+            s.run(1)
+        r1 = sims[i].mdfStats()["mean_residual"]
+        twins[i].mdfStats()
+        control[i] = (0.8 * r1, cap)
+        for s in (sims[i], twins[i]):
+            s.setMdfControl(control[i][0], cap, True)
+    for i in range(len(cases)):
+        if i not in control:
+            for s in (sims[i], twins[i]):
+                s.run(21)
+    ens = ensemble(sims)
+    for n in (1, 5, 12, 7):
+        ens.run(n)
+    for t in twins:
+        t.run(25)
+    for i, (s, t) in enumerate(zip(sims, twins)):
+        assert_same(s, t, f"member {i} with MDF")
+    for i in control:
+        a, b = sims[i].mdfStats(), twins[i].mdfStats()
+        print(f"member {i}: {a}", flush=True)
+        assert a == b, (i, a, b)
+        assert 0.0 < a["mean_corrections"] <= control[i][1]
+
+
 def test_batch_rejects_unsupported_members():
     ok = members("fp32")[:2]
     bad_coll = dict(ok[0], name="mrt", coll=1)
-    bad_mdf = dict(ok[0], name="mdf", mdf=3)
+    bad_mdf = dict(ok[0], name="mdf_scalar", mdf=3, scalar=dict(D=0.05, blob=(40, 17, 5.0)))
     bad_prec = dict(ok[0], name="fp64", precision="fp64")
-    for bad, msg in ((bad_coll, "collision model"), (bad_mdf, "multi-direct"),
+    for bad, msg in ((bad_coll, "collision model"), (bad_mdf, "multi-direct forcing with a scalar"),
                      (bad_prec, "precision")):
         sims = [product_sim(ok[1]), product_sim(bad)]
         ens = ensemble(sims)
