@@ -647,12 +647,24 @@ __device__ __forceinline__ void mdf_interp_body(const CellVelArgs& a, const doub
     }
 }
 
+// XCD-aware order of the node-parallel interpolations.  Workgroup b runs on
+// XCD b mod 8; with band = ceil(blocks / 8) it takes node block
+// (b mod 8) * band + b / 8, so each XCD walks one contiguous eighth of the
+// nodes.  A capsule's nodes (and the lattice rows around them) are then read
+// through one XCD's L2 instead of two or three: the stencils of neighbouring
+// nodes share most of their cache lines.  band = 0 keeps blockIdx order.
+__device__ __forceinline__ int band_block(int b, int band) {
+    return band > 0 ? (b % 8) * band + b / 8 : b;
+}
+
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double* lut, const double* x,
                                                     const double* y, const double* vx, const double* vy,
                                                     NodeSpread* rec, int K, MdfDev* md, int it,
-                                                    double* u0) {
-    mdf_interp_body<FP32>(a, lut, x, y, vx, vy, rec, K, md, it, u0, blockIdx.x);
+                                                    double* u0, int band) {
+    const int bx = band_block(blockIdx.x, band);
+    if (bx * INTERP_NODES >= K) return;       // padding workgroup of the banded grid
+    mdf_interp_body<FP32>(a, lut, x, y, vx, vy, rec, K, md, it, u0, bx);
 }
 
 // Per-node inputs of the warm-start model, in FP32: slip V - U at the first
@@ -827,8 +839,11 @@ __device__ __forceinline__ void interp_body(CellVelArgs a, const double* lut, do
 
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut, double* x,
-                                                double* y, double* vx, double* vy, int K, MoveArgs mv) {
-    interp_body<FP32>(a, lut, x, y, vx, vy, K, mv, blockIdx.x);
+                                                double* y, double* vx, double* vy, int K, MoveArgs mv,
+                                                int band) {
+    const int bx = band_block(blockIdx.x, band);
+    if (bx * INTERP_NODES >= K) return;       // padding workgroup of the banded grid
+    interp_body<FP32>(a, lut, x, y, vx, vy, K, mv, bx);
 }
 
 // moveAllNodes + y-clamp (capsule.cpp:267-297, simulation.cpp:665-675); one
@@ -902,6 +917,12 @@ static CellVelArgs vel_args(Ctx& c, const void* f, const void* F) {
     return a;
 }
 
+// workgroups per XCD band of the node-parallel interpolations (K nodes)
+static int interp_band(int K) {
+    const int blocks = (K + INTERP_NODES - 1) / INTERP_NODES;
+    return (blocks + 7) / 8;
+}
+
 // mode 0: bin nodes and spread node forces into F[Fcur] (overwrite).
 // mode 1: spread the MDF corrections on top of F[Fcur] into the spare buffer
 //         and make it current; the previous contents become F[Fpart].
@@ -956,14 +977,15 @@ void launch_spread(Ctx& c, int mode, bool last, int it, int last_it) {
 void launch_mdf_interp(Ctx& c, int it) {
     if (c.K == 0) return;
     CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
-    const dim3 grid((c.K + INTERP_NODES - 1) / INTERP_NODES);
+    const int band = interp_band(c.K);
+    const dim3 grid(8 * band);
     double* u0 = it == 1 && c.mdf_hist_on ? c.mdf_u0 : nullptr;
     if (c.fp32)
         hipLaunchKernelGGL((k_mdf_interp<true>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y,
-                           c.vx, c.vy, c.spread, c.K, c.mdf, it, u0);
+                           c.vx, c.vy, c.spread, c.K, c.mdf, it, u0, band);
     else
         hipLaunchKernelGGL((k_mdf_interp<false>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y,
-                           c.vx, c.vy, c.spread, c.K, c.mdf, it, u0);
+                           c.vx, c.vy, c.spread, c.K, c.mdf, it, u0, band);
 }
 
 void launch_mdf_begin(Ctx& c) {
@@ -1013,15 +1035,16 @@ __global__ void k_recirculate_x(double* x, const int* off, int ncaps, double Lx)
 void launch_interp_move(Ctx& c) {
     if (c.K == 0) return;
     CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
-    const dim3 grid((c.K + INTERP_NODES - 1) / INTERP_NODES);
+    const int band = interp_band(c.K);
+    const dim3 grid(8 * band);
     const MoveArgs mv{c.any_rigid ? 0 : 1, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
                       1.5, static_cast<double>(c.ny) - 1.5};
     if (c.fp32)
         hipLaunchKernelGGL((k_interp<true>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, c.vx,
-                           c.vy, c.K, mv);
+                           c.vy, c.K, mv, band);
     else
         hipLaunchKernelGGL((k_interp<false>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, c.vx,
-                           c.vy, c.K, mv);
+                           c.vy, c.K, mv, band);
     if (c.any_rigid)
         hipLaunchKernelGGL(k_move, dim3(c.ncaps), dim3(64), 0, c.stream, c.x, c.y, c.vx, c.vy, c.off,
                            c.desc, c.ncaps, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
