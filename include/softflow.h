@@ -324,6 +324,11 @@ int  sf_batch_set_graph_mode(sf_batch* b, int on);
  * leave the state unchanged) and keeps the fastest; report may be NULL. */
 int  sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len);
 int  sf_batch_set_lbm_launch(sf_batch* b, int bx, int by, int xcd_banded);
+/* on = 1: the batched collide-stream's workgroups take consecutive runs of
+ * bx * by cells (pairs) of each member's row-major cell index, across row ends,
+ * instead of bx x by rectangles, so rows narrower than a workgroup leave no
+ * lanes idle; results are identical either way (the autotuner times both). */
+int  sf_batch_set_lbm_flat(sf_batch* b, int on);
 /* out[0] members, out[1] lattice cells, out[2] membrane nodes, out[3] workgroups
  * of the batched collide-stream launch. */
 int  sf_batch_info(sf_batch* b, int64_t* out4);

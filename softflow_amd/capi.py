@@ -34,7 +34,8 @@ EXPORTS = (
     "sf_mlp_create", "sf_mlp_destroy", "sf_mlp_num_params", "sf_mlp_get_params", "sf_mlp_set_params",
     "sf_mlp_predict", "sf_mlp_loss_grad", "sf_mlp_train",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_step", "sf_batch_synchronize",
-    "sf_batch_set_graph_mode", "sf_batch_autotune", "sf_batch_set_lbm_launch", "sf_batch_info",
+    "sf_batch_set_graph_mode", "sf_batch_autotune", "sf_batch_set_lbm_launch", "sf_batch_set_lbm_flat",
+    "sf_batch_info",
 )
 
 
@@ -146,6 +147,7 @@ def load():
         "sf_batch_set_graph_mode": [C.c_void_p, C.c_int],
         "sf_batch_autotune": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],
         "sf_batch_set_lbm_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int],
+        "sf_batch_set_lbm_flat": [C.c_void_p, C.c_int],
         "sf_batch_info": [C.c_void_p, C.POINTER(C.c_int64)],
     }
     for name, args in proto.items():

@@ -559,6 +559,7 @@ PYBIND11_MODULE(softflow_core, m) {
         .def("synchronize", &Ensemble::synchronize)
         .def("autotune", &Ensemble::autotune, py::arg("reps") = 10)
         .def("setLbmLaunch", &Ensemble::setLbmLaunch, py::arg("bx"), py::arg("by"), py::arg("xcd_banded") = true)
+        .def("setLbmFlat", &Ensemble::setLbmFlat, py::arg("on"))
         .def("setGraphMode", &Ensemble::setGraphMode, py::arg("on"))
         .def("info", &Ensemble::info)
         .def("__len__", &Ensemble::size);

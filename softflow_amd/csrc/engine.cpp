@@ -1370,6 +1370,8 @@ void Ensemble::setLbmLaunch(int bx, int by, bool xcd_banded) {
     check(sf_batch_set_lbm_launch(batch_, bx, by, xcd_banded ? 1 : 0), "sf_batch_set_lbm_launch");
 }
 
+void Ensemble::setLbmFlat(bool on) { check(sf_batch_set_lbm_flat(batch_, on ? 1 : 0), "sf_batch_set_lbm_flat"); }
+
 void Ensemble::setGraphMode(bool on) { check(sf_batch_set_graph_mode(batch_, on ? 1 : 0), "sf_batch_set_graph_mode"); }
 
 std::vector<int64_t> Ensemble::info() {

@@ -550,6 +550,7 @@ public:
     void synchronize();
     std::string autotune(int reps = 10);
     void setLbmLaunch(int bx, int by, bool xcd_banded);
+    void setLbmFlat(bool on);
     void setGraphMode(bool on);
     // members, lattice cells, membrane nodes, collide-stream workgroups
     std::vector<int64_t> info();

@@ -80,7 +80,7 @@ struct StepLayout {
 // The items of one steady step of member c in the layout's slots; advances
 // c's buffer roles as one_step does (force-buffer rotation, membrane, contact,
 // spread and the MDF corrections, collide-stream, interpolation + move).
-static void record_step(Ctx& c, const StepLayout& L, int bx, int by, int pair, BatchItem* it) {
+static void record_step(Ctx& c, const StepLayout& L, int bx, int by, int pair, int flat, BatchItem* it) {
     const int spare = 3 - c.Fcur - c.Fprev;
     c.Fprev = c.Fcur;
     c.Fcur = spare;
@@ -109,7 +109,7 @@ static void record_step(Ctx& c, const StepLayout& L, int bx, int by, int pair, B
         else batch_items_none(BK_MDF_FINISH, it[k]);
         ++k;
     }
-    batch_items_lbm(c, bx, by, pair, ncorr > 0 ? SHIFT_PARTIAL : SHIFT_PREV, it + k);
+    batch_items_lbm(c, bx, by, pair, flat, ncorr > 0 ? SHIFT_PARTIAL : SHIFT_PREV, it + k);
     batch_items_interp(c, it + k + 1);
     c.shift_next = SHIFT_PREV;
     c.step_count++;
@@ -258,7 +258,8 @@ static int build_plan(sf_batch& B) {
     for (int s = 0; s < BATCH_PERIOD; ++s)
         for (int m = 0; m < M; ++m) {
             B.roles[s][m] = roles_of(*B.mem[m]);
-            record_step(*B.mem[m], B.layout, B.v.bx, B.v.by, B.v.pair, &items[(static_cast<size_t>(s) * M + m) * S]);
+            record_step(*B.mem[m], B.layout, B.v.bx, B.v.by, B.v.pair, B.v.flat,
+                        &items[(static_cast<size_t>(s) * M + m) * S]);
         }
     for (int m = 0; m < M; ++m) {
         B.roles[BATCH_PERIOD][m] = roles_of(*B.mem[m]);
@@ -498,6 +499,13 @@ int sf_batch_set_lbm_launch(sf_batch* b, int bx, int by, int xcd_banded) {
     return SF_OK;
 }
 
+int sf_batch_set_lbm_flat(sf_batch* b, int on) {
+    if (!b) return fail(SF_E_ARG, "null pointer");
+    b->v.flat = on ? 1 : 0;
+    b->planned = false;
+    return SF_OK;
+}
+
 int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
     if (!b) return fail(SF_E_ARG, "null pointer");
     sf_batch& B = *b;
@@ -515,14 +523,17 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
     SF_HIP(hipEventCreate(&e1));
     std::string rep;
     double best = 1e30;
-    int bbx = B.v.bx, bby = B.v.by, bband = B.banded, bpair = B.v.pair;
+    int bbx = B.v.bx, bby = B.v.by, bband = B.banded, bpair = B.v.pair, bflat = B.v.flat;
     BatchVariant v = B.v;
     v.fp32 = B.mem[0]->fp32 ? 1 : 0;
     v.op = B.mem[0]->p.collision_model;
     v.adr = 0;
     for (Ctx* c : B.mem) v.adr |= c->nsp > 0 ? 1 : 0;
     for (const auto& sh : shapes)
+    for (int flat = 0; flat <= 1; ++flat)
     for (int pair = 0; pair <= 1; ++pair) {
+        // flat runs depend only on the thread count: the one-row shapes cover them
+        if (flat && sh[1] != 1) continue;
         // the collide-stream of the members' current roles (the launch only
         // reads the source populations, so repeats leave the state as it is)
         std::vector<BatchItem> it(M);
@@ -531,7 +542,7 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
             Ctx& c = *B.mem[m];
             const int f0 = c.fcur, g0 = c.gcur;
             const bool store0 = c.c_from_store, pend0 = c.adr_pending;
-            batch_items_lbm(c, sh[0], sh[1], pair, member_corrections(c) > 0 ? SHIFT_PARTIAL : SHIFT_PREV,
+            batch_items_lbm(c, sh[0], sh[1], pair, flat, member_corrections(c) > 0 ? SHIFT_PARTIAL : SHIFT_PREV,
                             &it[m]);
             c.fcur = f0;
             c.gcur = g0;
@@ -564,10 +575,11 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
             SF_HIP(hipFree(dm));
             const double per = ms / reps;
             char line[112];
-            std::snprintf(line, sizeof(line), "%dx%d %s%s %.4f ms\n", sh[0], sh[1],
-                          band ? "xcd-banded" : "linear", pair ? " 2 cells/lane" : "", per);
+            std::snprintf(line, sizeof(line), "%dx%d %s%s%s %.4f ms\n", sh[0], sh[1],
+                          band ? "xcd-banded" : "linear", pair ? " 2 cells/lane" : "",
+                          flat ? " flat" : "", per);
             rep += line;
-            if (per < best) { best = per; bbx = sh[0]; bby = sh[1]; bband = band; bpair = pair; }
+            if (per < best) { best = per; bbx = sh[0]; bby = sh[1]; bband = band; bpair = pair; bflat = flat; }
         }
         SF_HIP(hipFree(dp));
     }
@@ -582,11 +594,12 @@ int sf_batch_autotune(sf_batch* b, int reps, char* report, int report_len) {
     B.v.bx = bbx;
     B.v.pair = bpair;
     B.v.by = bby;
+    B.v.flat = bflat;
     B.banded = bband;
     B.planned = false;
     char line[112];
-    std::snprintf(line, sizeof(line), "chosen %dx%d %s%s\n", bbx, bby, bband ? "xcd-banded" : "linear",
-                  bpair ? " 2 cells/lane" : "");
+    std::snprintf(line, sizeof(line), "chosen %dx%d %s%s%s\n", bbx, bby, bband ? "xcd-banded" : "linear",
+                  bpair ? " 2 cells/lane" : "", bflat ? " flat" : "");
     rep += line;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), report_len - 1);

@@ -344,6 +344,7 @@ struct BatchVariant {
     int bx = 64, by = 4;    // collide-stream workgroup shape
     int pair = 1;           // two cells per lane for the members where it applies
     int adr = 0;            // some member carries a scalar: the collide-stream runs its lagged step
+    int flat = 0;           // collide-stream workgroups take runs of the row-major cell index
 };
 
 template <typename P>
@@ -364,7 +365,7 @@ void batch_items_mdf_correction(Ctx& c, int i, int ncorr, BatchItem* it);   // B
 void batch_items_mdf_finish(Ctx& c, int ncorr, BatchItem& it);              // BK_MDF_FINISH
 void batch_items_none(int kind, BatchItem& it);            // a slot the member takes no part in
 size_t batch_pack_size(int kind);
-void batch_items_lbm(Ctx& c, int bx, int by, int pair, int shift, BatchItem* it);   // BK_LBM
+void batch_items_lbm(Ctx& c, int bx, int by, int pair, int flat, int shift, BatchItem* it);   // BK_LBM
 void batch_items_interp(Ctx& c, BatchItem* it);            // BK_INTERP
 // launch kind over nblocks workgroups of the map
 void batch_launch_membrane(int kind, const BatchVariant& v, const void* packs, const int2* map,

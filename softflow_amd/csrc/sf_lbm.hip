@@ -258,8 +258,10 @@ struct LbmArgs {
     // 8 * persist workgroups; the persist workgroups of an XCD walk its band
     // with stride persist.
     int nbx, nblk, per_xcd, remap, persist;
-    // batched collide-stream: this member takes two cells per lane (k_lbm_pair's body)
-    int pair;
+    // batched collide-stream: this member takes two cells per lane (k_lbm_pair's body);
+    // flat: its workgroups take consecutive runs of the row-major cell index
+    // (across row ends) instead of rectangles of bx x by cells
+    int pair, flat;
     // moving walls (lbm/moving_wall.cpp): x velocities of the solid rows y = 0 and y = ny-1
     int moving;
     double wall_bot, wall_top;
@@ -1133,14 +1135,32 @@ __global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* 
     if (w.x < 0) return;
     const LbmArgs a = packs[w.x];
     int capped;
+    // flat: lane l of workgroup w.y takes cell (pair) w.y * threads + l of the
+    // row-major index; lanes past the lattice get x = nx and do nothing
+    const int lane = threadIdx.y * blockDim.x + threadIdx.x;
+    const int threads = blockDim.x * blockDim.y;
     if (FP32 && a.pair) {
         if (w.y >= a.nblk) return;
-        const int x = 2 * ((w.y % a.nbx) * blockDim.x + threadIdx.x);
-        const int y = (w.y / a.nbx) * blockDim.y + threadIdx.y;
+        int x, y;
+        if (a.flat) {
+            const int n = 2 * (w.y * threads + lane);      // nx even: a pair stays in its row
+            x = n < a.N ? n % a.nx : a.nx;
+            y = n < a.N ? n / a.nx : 0;
+        } else {
+            x = 2 * ((w.y % a.nbx) * blockDim.x + threadIdx.x);
+            y = (w.y / a.nbx) * blockDim.y + threadIdx.y;
+        }
         capped = lbm_pair<OP, ADRF, false>(a, x, y);
     } else {
         int x, y;
-        if (!block_origin(a, w.y, x, y)) return;
+        if (a.flat) {
+            if (w.y >= a.nblk) return;
+            const int n = w.y * threads + lane;
+            x = n < a.N ? n % a.nx : a.nx;
+            y = n < a.N ? n / a.nx : 0;
+        } else if (!block_origin(a, w.y, x, y)) {
+            return;
+        }
         capped = lbm_cell<OP, FP32, false, ADRF>(a, x, y);
     }
     for (int off = 32; off > 0; off >>= 1) capped += __shfl_down(capped, off);
@@ -1152,12 +1172,17 @@ __global__ __launch_bounds__(256) void kb_lbm(const LbmArgs* packs, const int2* 
 // lagged scalar step of the previous step fused in, as one_step fuses it) with
 // a linear grid of bx x by workgroups; swaps the population (and scalar)
 // buffers as launch_lbm does.
-void batch_items_lbm(Ctx& c, int bx, int by, int pair, int shift, BatchItem* it) {
+void batch_items_lbm(Ctx& c, int bx, int by, int pair, int flat, int shift, BatchItem* it) {
     const bool fuse = c.nsp > 0;
     LbmArgs a = lbm_args(c, shift, fuse);
     dim3 g, b;
     a.pair = pair && lbm_pair_ok(c) && !c.generic_flags ? 1 : 0;
     set_geometry(c, a, bx, by, 0, 0, g, b, a.pair ? c.nx / 2 : 0);
+    a.flat = flat ? 1 : 0;
+    if (a.flat) {
+        const int per_block = bx * by * (a.pair ? 2 : 1);
+        a.nblk = (c.N + per_block - 1) / per_block;
+    }
     batch_put(it[0], a.nblk, a);
     c.fcur = 1 - c.fcur;
     if (fuse) {

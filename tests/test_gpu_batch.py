@@ -75,6 +75,16 @@ def test_batch_members_match_single_runs(precision):
         t.run(14)
     for i, (s, t) in enumerate(zip(sims, twins)):
         assert_same(s, t, f"member {i} after the second batch run")
+    # runs of the row-major cell index across row ends (rows narrower than
+    # a workgroup, odd widths, the members' last partial workgroups)
+    for shape in ((256, 1), (64, 1)):
+        ens.setLbmLaunch(*shape, True)
+        ens.setLbmFlat(True)
+        ens.run(9)
+        for t in twins:
+            t.run(9)
+        for i, (s, t) in enumerate(zip(sims, twins)):
+            assert_same(s, t, f"member {i} after flat runs {shape}")
 
 
 def test_batch_member_matches_oracle_fp64():
