@@ -160,6 +160,8 @@ static void harvest_timing(Ctx& c) {
 // One step in simulation.cpp:470-787 order.
 static void step_forces(Ctx& c) {
     if (c.ncaps == 0) return;
+    // the membrane kernel writes the arc lengths of the current positions
+    c.arc_fresh = c.arc != nullptr;
     if (contact_uses_cells(c)) {
         // the cell sort needs only node positions: it shares a launch with the
         // membrane forces
@@ -509,13 +511,16 @@ int build_ibb_links(Ctx& c) {
 }
 
 static void free_caps(Ctx& c) {
-    void* ptrs[] = {c.off, c.owner, c.desc, c.has_prev, c.x, c.y, c.vx, c.vy, c.fx, c.fy,
+    void* ptrs[] = {c.off, c.owner, c.ring, c.arc, c.desc, c.has_prev, c.x, c.y, c.vx, c.vy, c.fx, c.fy,
                     c.L0, c.th0, c.ds0, c.Lprev, c.spread, c.mdf_total, c.mdf_u0, c.mdf_hist,
                     c.mdf_feat, c.mdf_pred, c.mdf_f1, c.mdf_dmin, c.mdf_head};
     for (void* p : ptrs) if (p) hipFree(p);
     c.mdf_total = nullptr; c.mdf_u0 = nullptr; c.mdf_hist = nullptr; c.mdf_feat = nullptr;
     c.mdf_pred = nullptr; c.mdf_f1 = nullptr; c.mdf_dmin = nullptr; c.mdf_head = nullptr;
     c.off = c.owner = c.has_prev = nullptr;
+    c.ring = nullptr;
+    c.arc = nullptr;
+    c.arc_fresh = false;
     c.desc = nullptr;
     c.x = c.y = c.vx = c.vy = c.fx = c.fy = c.L0 = c.th0 = c.ds0 = c.Lprev = nullptr;
     c.spread = nullptr;
@@ -906,7 +911,14 @@ int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_d
     const int K = off[ncaps];
     c.ncaps = ncaps; c.K = K; c.max_nodes = maxn;
     std::vector<int> owner(K);
-    for (int i = 0; i < ncaps; ++i) for (int k = off[i]; k < off[i + 1]; ++k) owner[k] = i;
+    std::vector<int2> ring(K);
+    for (int i = 0; i < ncaps; ++i) {
+        const int o = off[i], n = off[i + 1] - o;
+        for (int k = o; k < o + n; ++k) {
+            owner[k] = i;
+            ring[k] = make_int2(o + (k - o - 1 + n) % n, o + (k - o + 1) % n);
+        }
+    }
     std::vector<double> zeros(K, 0.0);
     int rc = SF_OK;
     auto up = [&](double** d, const double* src) {
@@ -923,6 +935,9 @@ int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_d
     SF_HIP(hipMemcpy(c.off, off, (ncaps + 1) * sizeof(int), hipMemcpyHostToDevice));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.owner), K * sizeof(int)));
     SF_HIP(hipMemcpy(c.owner, owner.data(), K * sizeof(int), hipMemcpyHostToDevice));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.ring), K * sizeof(int2)));
+    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.arc), K * sizeof(double)));
+    SF_HIP(hipMemcpy(c.ring, ring.data(), K * sizeof(int2), hipMemcpyHostToDevice));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.desc), ncaps * sizeof(sf_capsule_desc)));
     SF_HIP(hipMemcpy(c.desc, desc, ncaps * sizeof(sf_capsule_desc), hipMemcpyHostToDevice));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.has_prev), ncaps * sizeof(int)));
@@ -988,6 +1003,7 @@ int sf_set_nodes(sf_ctx* h, const double* x, const double* y, const double* vx, 
     SF_HIP(hipStreamSynchronize(c.stream));
     const size_t b = static_cast<size_t>(c.K) * 8;
     if (b == 0) return SF_OK;
+    c.arc_fresh = false;
     if (x) SF_HIP(hipMemcpy(c.x, x, b, hipMemcpyHostToDevice));
     if (y) SF_HIP(hipMemcpy(c.y, y, b, hipMemcpyHostToDevice));
     if (vx) SF_HIP(hipMemcpy(c.vx, vx, b, hipMemcpyHostToDevice));
@@ -1377,6 +1393,39 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
     SF_HIP(hipEventCreate(&e0));
     SF_HIP(hipEventCreate(&e1));
     std::string rep;
+    if (c.K > 0 && c.tile_cnt) {
+        // The spread's persistent grid: node binning and the gather of the
+        // current node forces into the spare force buffer, which the next
+        // step overwrites (its tile flags stay exact); the records and tile
+        // lists are rebuilt by the next step's binning.
+        static const int grids[] = {512, 1024, 2048, 4096, 8192, 0};
+        const int keep_F = c.Fcur, keep_grid = c.spread_grid;
+        const bool keep_arc = c.arc_fresh;
+        c.arc_fresh = false;   // the positions may have moved since the arc lengths were taken
+        c.Fcur = 3 - c.Fcur - c.Fprev;
+        double sbest = 1e30;
+        int sgrid = keep_grid;
+        for (const int gsz : grids) {
+            c.spread_grid = gsz;
+            launch_spread(c, 0, true);   // warm
+            SF_HIP(hipEventRecord(e0, c.stream));
+            for (int r = 0; r < reps; ++r) launch_spread(c, 0, true);
+            SF_HIP(hipEventRecord(e1, c.stream));
+            SF_HIP(hipEventSynchronize(e1));
+            SF_HIP(hipGetLastError());
+            float ms = 0.f;
+            SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const double per = ms / reps;
+            char line[96];
+            std::snprintf(line, sizeof(line), "spread grid %d %.4f ms\n", gsz > 0 ? std::min(gsz, c.ntiles) : c.ntiles,
+                          per);
+            rep += line;
+            if (per < sbest) { sbest = per; sgrid = gsz; }
+        }
+        c.Fcur = keep_F;
+        c.arc_fresh = keep_arc;
+        c.spread_grid = sgrid;
+    }
     double best = 1e30;
     int bbx = c.lbm_bx, bby = c.lbm_by, brm = c.lbm_remap, bps = c.lbm_persist, bpr = c.lbm_pair;
     const int pair_max = lbm_pair_ok(c) ? 1 : 0;
@@ -1414,12 +1463,17 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
     launch_fill_int(c, c.counters, 1, 0);
     c.lbm_bx = bbx; c.lbm_by = bby; c.lbm_remap = brm; c.lbm_persist = bps; c.lbm_pair = bpr;
     invalidate_graph(c);
-    char line[112];
+    char line[160];
     const char* cl = (bpr && pair_max) ? " 2 cells/lane" : "";
+    char sg[48] = "";
+    if (c.K > 0 && c.tile_cnt)
+        std::snprintf(sg, sizeof(sg), ", spread grid %d",
+                      c.spread_grid > 0 ? std::min(c.spread_grid, c.ntiles) : c.ntiles);
     if (bps)
-        std::snprintf(line, sizeof(line), "chosen %dx%d persistent %d/xcd%s\n", bbx, bby, bps, cl);
+        std::snprintf(line, sizeof(line), "chosen %dx%d persistent %d/xcd%s%s\n", bbx, bby, bps, cl, sg);
     else
-        std::snprintf(line, sizeof(line), "chosen %dx%d %s%s\n", bbx, bby, brm ? "xcd-banded" : "linear", cl);
+        std::snprintf(line, sizeof(line), "chosen %dx%d %s%s%s\n", bbx, bby, brm ? "xcd-banded" : "linear", cl,
+                      sg);
     rep += line;
     if (report && report_len > 0) {
         std::strncpy(report, rep.c_str(), report_len - 1);

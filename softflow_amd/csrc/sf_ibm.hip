@@ -26,13 +26,110 @@ __device__ __forceinline__ double phi_lut(const double* t, double r) {
     return t[i] * (1.0 - frac) + t[i + 1] * frac;
 }
 
-__device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, double by, double Lx) {
-    double dx = ax - bx, dy = ay - by;   // immersed_boundary.cpp:16-23
-    if (Lx > 0.0) {
-        if (dx > 0.5 * Lx) dx -= Lx;
-        if (dx < -0.5 * Lx) dx += Lx;
+// Node binning, the first stage of a spread: every node's record, and its
+// reference in each lattice tile its 4x4 stencil touches.
+struct BinArgs {
+    NodeSpread* rec;        // per node
+    int* tile_cnt;          // per tile: nodes binned
+    int* tile_list;         // per tile: TILE_MAXL node indices
+    NodeSpread* tile_rec;   // per tile: copies of its first TILE_RECS records
+    int* counters;          // [1]: tile list overflow; [3]: touched-tile count, cleared here
+    int nx, ny, ntx;
+};
+
+// Arc length element of node k: half the distances to its ring neighbours
+// ring = (previous, next) node index (immersed_boundary.cpp:44-60), minimum
+// image over nx.
+__device__ __forceinline__ double node_arc(const double* x, const double* y, int2 ring, double px, double py,
+                                           int nx) {
+    const double Lx = static_cast<double>(nx);
+    const double dp = min_image_ds(px, py, x[ring.x], y[ring.x], Lx);
+    const double dn = min_image_ds(x[ring.y], y[ring.y], px, py, Lx);
+    return 0.5 * (dp + dn);
+}
+
+__device__ __forceinline__ bool finite_position(double px, double py) {
+    return !(isnan(px) || isnan(py) || isinf(px) || isinf(py));
+}
+
+// Node k's spread record: position, force, arc length, stencil origin; valid
+// when the position and force are finite and the stencil rows can reach the
+// lattice.
+__device__ __forceinline__ NodeSpread node_record(double px, double py, double fx, double fy, double ds, int k,
+                                                  int ny) {
+    NodeSpread s;
+    s.px = px;
+    s.py = py;
+    s.fx = fx;
+    s.fy = fy;
+    s.ds = ds;
+    const bool pos_ok = finite_position(px, py);
+    int i0 = 0, j0 = 0;
+    bool ok = pos_ok && !(isnan(fx) || isnan(fy));
+    if (pos_ok) {
+        i0 = static_cast<int>(floor(px));
+        j0 = static_cast<int>(floor(py));
+        if (j0 < -2 || j0 >= ny + 2) ok = false;
     }
-    return sqrt(dx * dx + dy * dy);
+    s.i0 = i0;
+    s.j0 = j0;
+    s.valid = ok ? 1 : 0;
+    s.pad = k;
+    return s;
+}
+
+// The tiles a record's stencil touches, as up to 8 candidates: candidate c
+// is the (c & 3)-th distinct tile column of the stencil columns i0-1 .. i0+2
+// (each wrapped once, in column order) in the (c >> 2)-th tile row of its rows
+// j0-1 .. j0+2 within the lattice, or -1.  A record with a finite position is
+// binned even when its force is not finite (it then adds nothing).  Registers
+// only: the distinct columns are kept in four scalars.
+__device__ __forceinline__ void node_tiles(const NodeSpread& s, int nx, int ny, int ntx, int (&t)[8]) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) t[c] = -1;
+    if (!finite_position(s.px, s.py)) return;
+    const int i0 = s.i0, j0 = s.j0;
+    if (j0 < -2 || j0 >= ny + 2) return;
+    const int ylo = j0 - 1 < 0 ? 0 : j0 - 1;
+    const int yhi = j0 + 2 > ny - 1 ? ny - 1 : j0 + 2;
+    if (ylo > yhi) return;
+    const int ty0 = ylo / TILE_Y, ty1 = yhi / TILE_Y;
+    int u0 = -1, u1 = -1, u2 = -1, u3 = -1, nt = 0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        int jx = i0 - 1 + d;
+        if (jx < 0) jx += nx;
+        if (jx >= nx) jx -= nx;
+        if (jx < 0 || jx >= nx) continue;
+        const int tx = jx / TILE_X;
+        if (tx == u0 || tx == u1 || tx == u2 || tx == u3) continue;
+        if (nt == 0) u0 = tx;
+        else if (nt == 1) u1 = tx;
+        else if (nt == 2) u2 = tx;
+        else u3 = tx;
+        ++nt;
+    }
+    const int u[4] = {u0, u1, u2, u3};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int ty = ty0 + (c >> 2);
+        if ((c & 3) < nt && ty <= ty1) t[c] = ty * ntx + u[c & 3];
+    }
+}
+
+// Appends node s.pad to each candidate tile's list (and its record to the
+// tile's copies): every slot is claimed before any store waits on one.
+__device__ __forceinline__ void bin_node(const BinArgs& b, const int (&t)[8], const NodeSpread& s) {
+    int slot[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) slot[c] = t[c] >= 0 ? atomicAdd(&b.tile_cnt[t[c]], 1) : 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (t[c] < 0) continue;
+        if (slot[c] < TILE_MAXL) b.tile_list[static_cast<size_t>(t[c]) * TILE_MAXL + slot[c]] = s.pad;
+        else atomicOr(&b.counters[1], 1);
+        if (slot[c] < TILE_RECS) b.tile_rec[static_cast<size_t>(t[c]) * TILE_RECS + slot[c]] = s;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -43,8 +140,10 @@ __device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, 
 // step -- except at iteration force_it (0: none), which is always applied;
 // otherwise the correction is counted into the node's step total, and into
 // its head total while it <= MDF_HEAD.
+// arc: the arc length elements at the current positions, or null (computed
+// here from the ring neighbours).
 __device__ __forceinline__ void node_prep_body(const double* x, const double* y, const double* fx, const double* fy,
-                            const int* off, const int* owner, NodeSpread* rec, int K, int nx, int ny,
+                            const int2* ring, const double* arc, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, NodeSpread* tile_rec,
                             int* counters, int mode, MdfDev* md, double tol, int it, int force_it,
                             double* total, double* head, int bx) {
@@ -68,71 +167,26 @@ __device__ __forceinline__ void node_prep_body(const double* x, const double* y,
         }
     }
     const double px = x[k], py = y[k];
-    NodeSpread s;
-    s.px = px;
-    s.py = py;
-    if (mode == 0) {
-        s.fx = fx[k];
-        s.fy = fy[k];
-        const int cap = owner[k];
-        const int o = off[cap], n = off[cap + 1] - o, i = k - o;
-        const int kp = o + (i - 1 + n) % n, kn = o + (i + 1) % n;
-        const double Lx = static_cast<double>(nx);
-        const double dp = min_image_ds(px, py, x[kp], y[kp], Lx);
-        const double dn = min_image_ds(x[kn], y[kn], px, py, Lx);
-        s.ds = 0.5 * (dp + dn);
-    } else {
-        s.fx = rec[k].fx;
-        s.fy = rec[k].fy;
-        s.ds = rec[k].ds;
+    if (mode != 0) {
+        // keep the force and arc length; re-validate
+        const NodeSpread r = rec[k];
+        rec[k] = node_record(px, py, r.fx, r.fy, r.ds, k, ny);
+        return;
     }
-    const bool pos_ok = !(isnan(px) || isnan(py) || isinf(px) || isinf(py));
-    int i0 = 0, j0 = 0;
-    bool ok = pos_ok && !(isnan(s.fx) || isnan(s.fy));
-    if (pos_ok) {
-        i0 = static_cast<int>(floor(px));
-        j0 = static_cast<int>(floor(py));
-        if (j0 < -2 || j0 >= ny + 2) ok = false;
-    }
-    s.i0 = i0;
-    s.j0 = j0;
-    s.valid = ok ? 1 : 0;
-    s.pad = k;
+    const double ds = arc ? arc[k] : node_arc(x, y, ring[k], px, py, nx);
+    const NodeSpread s = node_record(px, py, fx[k], fy[k], ds, k, ny);
     rec[k] = s;
-    if (mode != 0 || !pos_ok || j0 < -2 || j0 >= ny + 2) return;
-    // tiles touched by the stencil columns i0-1..i0+2 (single wrap) and rows j0-1..j0+2
-    int ylo = j0 - 1, yhi = j0 + 2;
-    if (ylo < 0) ylo = 0;
-    if (yhi > ny - 1) yhi = ny - 1;
-    if (ylo > yhi) return;
-    int txs[4], nt = 0;
-    for (int ix = i0 - 1; ix <= i0 + 2; ++ix) {
-        int jx = ix;
-        if (jx < 0) jx += nx;
-        if (jx >= nx) jx -= nx;
-        if (jx < 0 || jx >= nx) continue;
-        const int tx = jx / TILE_X;
-        bool seen = false;
-        for (int t = 0; t < nt; ++t) seen |= (txs[t] == tx);
-        if (!seen) txs[nt++] = tx;
-    }
-    for (int ty = ylo / TILE_Y; ty <= yhi / TILE_Y; ++ty)
-        for (int t = 0; t < nt; ++t) {
-            const int tile = ty * ntx + txs[t];
-            const int slot = atomicAdd(&tile_cnt[tile], 1);
-            if (slot < TILE_MAXL) tile_list[static_cast<size_t>(tile) * TILE_MAXL + slot] = k;
-            else atomicOr(&counters[1], 1);
-            if (slot < TILE_RECS) tile_rec[static_cast<size_t>(tile) * TILE_RECS + slot] = s;
-
-        }
+    int t[8];
+    node_tiles(s, nx, ny, ntx, t);
+    bin_node(BinArgs{rec, tile_cnt, tile_list, tile_rec, counters, nx, ny, ntx}, t, s);
 }
 
 __global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,
-                            const int* off, const int* owner, NodeSpread* rec, int K, int nx, int ny,
+                            const int2* ring, const double* arc, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, NodeSpread* tile_rec,
                             int* counters, int mode, MdfDev* md, double tol, int it, int force_it,
                             double* total, double* head) {
-    node_prep_body(x, y, fx, fy, off, owner, rec, K, nx, ny, ntx, nty, tile_cnt, tile_list, tile_rec, counters, mode, md, tol, it, force_it, total, head, blockIdx.x);
+    node_prep_body(x, y, fx, fy, ring, arc, rec, K, nx, ny, ntx, nty, tile_cnt, tile_list, tile_rec, counters, mode, md, tol, it, force_it, total, head, blockIdx.x);
 }
 
 // Gather: one 256-thread workgroup per TILE_X x TILE_Y tile.
@@ -142,7 +196,6 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
 constexpr int STAGE_MAX = 64;      // node records staged in LDS (lists up to this length)
 constexpr int CELL_MAXPC = 16;     // contributions one cell collects in LDS before falling back
 constexpr int NO_ROW = -0x40000000;
-constexpr int SPREAD_GRID = 1 << 30;  // workgroups walking the touched tiles (capped at the tile count)
 
 // Unwrapped stencil column (lo .. lo+3) whose single periodic wrap lands on cx,
 // or NO_ROW if none does.
@@ -287,7 +340,9 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
     // the delta table is read through the caches: staged in LDS it held the
     // spread to 7 workgroups per CU, and each tile is short, latency-bound work
     const double* tab = lut;
-    __shared__ double spx[STAGE_MAX], spy[STAGE_MAX], sfx[STAGE_MAX], sfy[STAGE_MAX], sds[STAGE_MAX];
+    __shared__ double sfx[STAGE_MAX], sfy[STAGE_MAX], sds[STAGE_MAX];
+    // each staged node's stencil weights: phi(px - (i0 - 1 + d)), phi(py - (j0 - 1 + d))
+    __shared__ double swx[STAGE_MAX][4], swy[STAGE_MAX][4];
     __shared__ int si0[STAGE_MAX], sj0[STAGE_MAX], skey[STAGE_MAX];
     __shared__ int ccnt[TILE_X * TILE_Y];
     __shared__ int cslot_raw[TILE_X * TILE_Y * CELL_MAXPC / 2];
@@ -295,6 +350,9 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
     int* list = cslot_raw;   // the long-list path keeps its sorted list here
     using T = typename std::conditional<FP32, float, double>::type;
     const int t = threadIdx.x;
+    // each list entry is read one tile ahead, the first one beside the count
+    // (the list holds ntiles entries; those past the count are never used)
+    int2 te_next = act[bx < ntiles ? bx : 0];
     const int nact = min(counters[3], ntiles);
     const bool skip = mode == 1 && md && md->conv;
     if (skip) {
@@ -303,11 +361,11 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
             for (int i = bx * blockDim.x + t; i < nact; i += gx * blockDim.x) tile_cnt[act[i].x] = 0;
         return;
     }
-    // workgroups past the touched-tile count have nothing to do (the grid is
-    // sized for every tile)
+    // workgroups past the touched-tile count have nothing to do
     if (bx >= nact) return;
     for (int ai = bx; ai < nact; ai += gx) {
-        const int2 te = act[ai];
+        const int2 te = te_next;
+        if (ai + gx < ntiles) te_next = act[ai + gx];
         const int tile = te.x;
         __syncthreads();          // the previous tile's LDS use is over
         int L = te.y;
@@ -323,21 +381,32 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
             ax = static_cast<const T*>(Fin)[2 * n];
             ay = static_cast<const T*>(Fin)[2 * n + 1];
         }
-        auto stage_rec = [&](int i, const NodeSpread& s) {
-            spx[i] = s.px; spy[i] = s.py; sfx[i] = s.fx; sfy[i] = s.fy; sds[i] = s.ds;
-            si0[i] = s.i0;
-            sj0[i] = s.valid ? s.j0 : NO_ROW;   // invalid records touch no row
-            skey[i] = s.pad;                    // the node index
+        // Staging: four threads per record; thread d of a record computes the
+        // record's stencil weights of column and row d (the table is read here
+        // only), thread 0 also stages the record.  A weight below is then the
+        // product phi(px - ix) * phi(py - iy) of the same two table values.
+        auto stage_rec = [&](int i, int d, const NodeSpread& s) {
+            const bool valid = s.valid != 0;
+            if (d == 0) {
+                sfx[i] = s.fx; sfy[i] = s.fy; sds[i] = s.ds;
+                si0[i] = s.i0;
+                sj0[i] = valid ? s.j0 : NO_ROW;   // invalid records touch no row
+                skey[i] = s.pad;                  // the node index
+            }
+            // a valid record's position is finite
+            swx[i][d] = valid ? phi_lut(tab, s.px - static_cast<double>(s.i0 - 1 + d)) : 0.0;
+            swy[i][d] = valid ? phi_lut(tab, s.py - static_cast<double>(s.j0 - 1 + d)) : 0.0;
         };
-        auto stage = [&](int i, int k) { stage_rec(i, rec[k]); };
+        auto stage = [&](int i, int d, int k) { stage_rec(i, d, rec[k]); };
         auto contribution = [&](int i, double& w, int& ok) {
             ok = 0;
             w = 0.0;
             const int dyc = cy - (sj0[i] - 1);
             if (dyc < 0 || dyc > 3) return;
-            const int ix = stencil_column(cx, si0[i] - 1, nx);
+            const int lo = si0[i] - 1;
+            const int ix = stencil_column(cx, lo, nx);
             if (ix == NO_ROW) return;
-            w = phi_lut(tab, spx[i] - static_cast<double>(ix)) * phi_lut(tab, spy[i] - static_cast<double>(cy));
+            w = swx[i][ix - lo] * swy[i][dyc];
             ok = w > 1e-15;
         };
 
@@ -349,9 +418,9 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
             // after binning, so mode 1 reads the records themselves
             if (mode == 0 && L <= TILE_RECS) {
                 const NodeSpread* tr = tile_rec + static_cast<size_t>(tile) * TILE_RECS;
-                for (int i = t; i < L; i += 256) stage_rec(i, tr[i]);
+                for (int i = t; i < 4 * L; i += 256) stage_rec(i >> 2, i & 3, tr[i >> 2]);
             } else {
-                for (int i = t; i < L; i += 256) stage(i, tl[i]);
+                for (int i = t; i < 4 * L; i += 256) stage(i >> 2, i & 3, tl[i >> 2]);
             }
             ccnt[t] = 0;
             __syncthreads();
@@ -373,8 +442,7 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
                 if (jx >= nx) jx -= nx;
                 const int lx = jx - tx0;
                 if (lx < 0 || lx >= TILE_X) continue;
-                const double w = phi_lut(tab, spx[p] - static_cast<double>(ix)) *
-                                 phi_lut(tab, spy[p] - static_cast<double>(iy));
+                const double w = swx[p][s & 3] * swy[p][s >> 2];
                 if (!(w > 1e-15)) continue;
                 const int cell = ly * TILE_X + lx;
                 const int slot = atomicAdd(&ccnt[cell], 1);
@@ -444,7 +512,7 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
             }
             for (int base = 0; base < L; base += STAGE_MAX) {
                 const int m = min(STAGE_MAX, L - base);
-                for (int i = t; i < m; i += 256) stage(i, list[base + i]);
+                for (int i = t; i < 4 * m; i += 256) stage(i >> 2, i & 3, list[base + (i >> 2)]);
                 __syncthreads();
                 if (inside) {
                     for (int i = 0; i < m; ++i) {
@@ -932,13 +1000,15 @@ void launch_spread(Ctx& c, int mode, bool last, int it, int last_it) {
         return;
     }
     if (mode == 0) {
+        // the membrane kernel left the arc lengths of the current positions
+        const double* arc = c.arc_fresh ? c.arc : nullptr;
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
-                           c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
+                           c.fx, c.fy, c.ring, arc, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
                            c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 0, nullptr, 0.0, 0, 0,
                            nullptr, nullptr);
     } else {
         hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
-                           c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
+                           c.fx, c.fy, c.ring, nullptr, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
                            c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 1, c.mdf, c.mdf_tol, it,
                            0, c.mdf_total, c.mdf_head);
     }
@@ -950,7 +1020,7 @@ void launch_spread(Ctx& c, int mode, bool last, int it, int last_it) {
     uint8_t* dout = c.tile_dirty + static_cast<size_t>(out) * c.ntiles;
     const MdfDev* md = mode == 1 ? c.mdf : nullptr;
     const dim3 rgrid((c.ntiles + rest_tiles_per_block(mode) - 1) / rest_tiles_per_block(mode));
-    const dim3 sgrid(std::min(c.ntiles, SPREAD_GRID));
+    const dim3 sgrid(c.spread_grid > 0 ? std::min(c.ntiles, c.spread_grid) : c.ntiles);
     if (c.fp32) {
         hipLaunchKernelGGL((k_spread_rest<true>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
                            din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md,
@@ -1033,6 +1103,7 @@ __global__ void k_recirculate_x(double* x, const int* off, int ncaps, double Lx)
 }
 
 void launch_interp_move(Ctx& c) {
+    c.arc_fresh = false;   // the nodes move
     if (c.K == 0) return;
     CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
     const int band = interp_band(c.K);
@@ -1062,7 +1133,7 @@ void launch_interp_move(Ctx& c) {
 // MDF correction; md null in mode 0)
 struct PrepPack {
     const double *x, *y, *fx, *fy;
-    const int *off, *owner;
+    const int2* ring;
     NodeSpread* rec;
     int K, nx, ny, ntx, nty;
     int* tile_cnt;
@@ -1145,7 +1216,7 @@ __global__ __launch_bounds__(256) void kb_node_prep(const PrepPack* packs, const
     const int2 w = map[blockIdx.x];
     if (w.x < 0) return;
     const PrepPack p = packs[w.x];
-    node_prep_body(p.x, p.y, p.fx, p.fy, p.off, p.owner, p.rec, p.K, p.nx, p.ny, p.ntx, p.nty,
+    node_prep_body(p.x, p.y, p.fx, p.fy, p.ring, nullptr, p.rec, p.K, p.nx, p.ny, p.ntx, p.nty,
                    p.tile_cnt, p.tile_list, p.tile_rec, p.counters, p.mode, p.md, p.tol, p.it, 0, p.total,
                    p.head, w.y);
 }
@@ -1207,7 +1278,7 @@ constexpr int BATCH_SPREAD_GRID = 64;
 // The arguments of launch_spread(c, 0, last) (no capsules: not batched).
 void batch_items_spread(Ctx& c, BatchItem* it, bool last) {
     batch_put(it[0], (c.K + 255) / 256,
-              PrepPack{c.x, c.y, c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
+              PrepPack{c.x, c.y, c.fx, c.fy, c.ring, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
                        c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 0, nullptr, 0.0, 0, nullptr, nullptr});
     const void* Fin = c.F[c.Fcur];
     void* Fout = c.F[c.Fcur];
@@ -1240,7 +1311,7 @@ void batch_items_mdf_interp(Ctx& c, int i, BatchItem& it) {
 void batch_items_mdf_correction(Ctx& c, int i, int ncorr, BatchItem* it) {
     batch_items_mdf_interp(c, i, it[0]);
     batch_put(it[1], (c.K + 255) / 256,
-              PrepPack{c.x, c.y, c.fx, c.fy, c.off, c.owner, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
+              PrepPack{c.x, c.y, c.fx, c.fy, c.ring, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
                        c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 1, c.mdf, c.mdf_tol, i,
                        c.mdf_total, c.mdf_head});
     const void* Fin = c.F[c.Fcur];

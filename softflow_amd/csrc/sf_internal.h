@@ -46,6 +46,16 @@ struct NodeSpread {
     int i0, j0, valid, pad;
 };
 
+// Distance between two points with the minimum image over Lx (> 0).
+__device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, double by, double Lx) {
+    double dx = ax - bx, dy = ay - by;   // immersed_boundary.cpp:16-23
+    if (Lx > 0.0) {
+        if (dx > 0.5 * Lx) dx -= Lx;
+        if (dx < -0.5 * Lx) dx += Lx;
+    }
+    return sqrt(dx * dx + dy * dy);
+}
+
 // Multi-direct forcing control.  Fixed mode (tol <= 0) is the reference's
 // ibm_iterations - 1 corrections; residual mode stops as soon as the no-slip
 // residual max_k |V_k - U(X_k)| is at most tol, after at most max_corr
@@ -146,6 +156,11 @@ struct Ctx {
     bool any_rigid = false;
     int* off = nullptr;
     int* owner = nullptr;
+    int2* ring = nullptr;                // per node: (previous, next) node of its ring
+    // per node: the spread's arc length element at the current positions,
+    // written by the membrane kernel (valid while arc_fresh)
+    double* arc = nullptr;
+    bool arc_fresh = false;
     sf_capsule_desc* desc = nullptr;
     int* has_prev = nullptr;
     double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr;
@@ -251,6 +266,7 @@ struct Ctx {
     // two cells per lane where the paired collide-stream applies (lbm_pair_ok):
     // 0 never, 1 channels, 2 also generic cell types
     int lbm_pair = 1;
+    int spread_grid = 0;                 // workgroups of the spread gather (0: one per tile; else persistent)
 };
 
 // Error reporting shared by the C-ABI translation units: sets the message

@@ -307,9 +307,7 @@ __device__ __forceinline__ void load_force(const void* F, int n, T& fx, T& fy) {
 // Force of the collision: raw spread + body force (FLUID cells) then the |F|
 // cap (simulation.cpp:541-609).  Returns 1 if the cap fired.
 template <typename T>
-__device__ __forceinline__ int collision_force(const LbmArgs& a, const void* F, int n, bool fluid,
-                                               T& fx, T& fy) {
-    load_force<T>(F, n, fx, fy);
+__device__ __forceinline__ int force_finish(const LbmArgs& a, bool fluid, T& fx, T& fy) {
     if (a.has_body && fluid) {
         fx += static_cast<T>(a.bfx);
         fy += static_cast<T>(a.bfy);
@@ -322,6 +320,13 @@ __device__ __forceinline__ int collision_force(const LbmArgs& a, const void* F, 
         return 1;
     }
     return 0;
+}
+
+template <typename T>
+__device__ __forceinline__ int collision_force(const LbmArgs& a, const void* F, int n, bool fluid,
+                                               T& fx, T& fy) {
+    load_force<T>(F, n, fx, fy);
+    return force_finish<T>(a, fluid, fx, fy);
 }
 
 // One cell of the advection-diffusion step (advection_diffusion.cpp:111-223):
@@ -582,16 +587,42 @@ __device__ __forceinline__ void st_pair(float* p, float v0, float v1) {
     __builtin_nontemporal_store(v, reinterpret_cast<pair_f2*>(p));
 }
 
-// lbm_cell's FP32 macroscopic state and collision of one cell
+// The raw forces of buffer F at cells n and n + 1 (n even: one 16-byte load).
+__device__ __forceinline__ float4 pair_force_raw(const void* F, int n) {
+    return *reinterpret_cast<const float4*>(static_cast<const float*>(F) + 2 * static_cast<size_t>(n));
+}
+
+// The pair's collision forces (collision_force of each cell; c0/c1: the cells
+// take the body force) and the forces of their collision velocities (the
+// shift: the previous step's collision force, or the raw partial force), each
+// buffer's pair in one 16-byte load.  Returns the number of capped collision
+// forces.
+__device__ __forceinline__ int pair_forces(const LbmArgs& a, int n, bool c0, bool c1, float (&F)[4],
+                                           float (&S)[4]) {
+    const float4 fc = pair_force_raw(a.Fcur, n);
+    F[0] = fc.x; F[1] = fc.y; F[2] = fc.z; F[3] = fc.w;
+    int capped = force_finish<float>(a, c0, F[0], F[1]);
+    capped += force_finish<float>(a, c1, F[2], F[3]);
+    S[0] = 0.f; S[1] = 0.f; S[2] = 0.f; S[3] = 0.f;
+    if (a.shift == SHIFT_PREV || a.shift == SHIFT_PARTIAL) {
+        const float4 fs = pair_force_raw(a.Fshift, n);
+        S[0] = fs.x; S[1] = fs.y; S[2] = fs.z; S[3] = fs.w;
+        if (a.shift == SHIFT_PREV) {
+            force_finish<float>(a, c0, S[0], S[1]);
+            force_finish<float>(a, c1, S[2], S[3]);
+        }
+    }
+    return capped;
+}
+
+// lbm_cell's FP32 macroscopic state and collision of one cell; (sx, sy) is
+// the force of its collision velocity (pair_forces)
 template <int OP>
 __device__ __forceinline__ void pair_collide(const LbmArgs& a, float (&f)[9], int n, bool collide, float Fx,
-                                             float Fy, float& ux, float& uy) {
+                                             float Fy, float sx, float sy, float& ux, float& uy) {
     float drho = ((f[0] + (f[1] + f[3])) + (f[2] + f[4])) + ((f[5] + f[7]) + (f[6] + f[8]));
     float jx = (f[1] - f[3]) + (f[5] - f[6]) + (f[8] - f[7]);
     float jy = (f[2] - f[4]) + (f[5] + f[6]) - (f[7] + f[8]);
-    float sx = 0.f, sy = 0.f;
-    if (a.shift == SHIFT_PREV) collision_force<float>(a, a.Fshift, n, collide, sx, sy);
-    else if (a.shift == SHIFT_PARTIAL) load_force<float>(a.Fshift, n, sx, sy);
     float rho = 1.0f + drho;
     if (a.shift == SHIFT_STORED) {
         rho = static_cast<float>(a.srho[n]);
@@ -689,9 +720,8 @@ __device__ __forceinline__ int lbm_pair_types(const LbmArgs& a, int x, int y, in
     const unsigned m0 = mm & 0xffffu, m1 = mm >> 16;
     const bool fl0 = mask_type(m0) != SF_SOLID, fl1 = mask_type(m1) != SF_SOLID;
     const bool co0 = mask_type(m0) == SF_FLUID, co1 = mask_type(m1) == SF_FLUID;
-    float Fx0, Fy0, Fx1, Fy1;
-    int capped = collision_force<float>(a, a.Fcur, n, co0, Fx0, Fy0);
-    capped += collision_force<float>(a, a.Fcur, n + 1, co1, Fx1, Fy1);
+    float F[4], S[4];
+    const int capped = pair_forces(a, n, co0, co1, F, S);
     if (!fl0 && !fl1) return capped;
     const float* src = static_cast<const float*>(a.fsrc);
     float* dst = static_cast<float*>(a.fdst);
@@ -704,8 +734,8 @@ __device__ __forceinline__ int lbm_pair_types(const LbmArgs& a, int x, int y, in
         f1[q] = v.y;
     }
     float ux0 = 0.f, uy0 = 0.f, ux1 = 0.f, uy1 = 0.f;
-    if (fl0) pair_collide<OP>(a, f0, n, co0, Fx0, Fy0, ux0, uy0);
-    if (fl1) pair_collide<OP>(a, f1, n + 1, co1, Fx1, Fy1, ux1, uy1);
+    if (fl0) pair_collide<OP>(a, f0, n, co0, F[0], F[1], S[0], S[1], ux0, uy0);
+    if (fl1) pair_collide<OP>(a, f1, n + 1, co1, F[2], F[3], S[2], S[3], ux1, uy1);
     const bool both = fl0 && fl1;
     if constexpr (ADRF) {
         for (int s = 0; s < a.nsp; ++s) {
@@ -771,23 +801,24 @@ __device__ __forceinline__ int lbm_pair(const LbmArgs& a, int x, int y) {
     const int n = y * a.nx + x;
     if constexpr (FLAGS) return lbm_pair_types<OP, ADRF>(a, x, y, n);
     const bool fluid = !is_solid(a, x, y, false);
-    float Fx0, Fy0, Fx1, Fy1;
-    int capped = collision_force<float>(a, a.Fcur, n, fluid, Fx0, Fy0);
-    capped += collision_force<float>(a, a.Fcur, n + 1, fluid, Fx1, Fy1);
-    if (!fluid) return capped;
     const float* src = static_cast<const float*>(a.fsrc);
     float* dst = static_cast<float*>(a.fdst);
     const size_t N = static_cast<size_t>(a.N);
     float f0[9], f1[9];
+    if (fluid) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        const pair_f2 v = ld_pair(src + q * N + n);
-        f0[q] = v.x;
-        f1[q] = v.y;
+        for (int q = 0; q < 9; ++q) {
+            const pair_f2 v = ld_pair(src + q * N + n);
+            f0[q] = v.x;
+            f1[q] = v.y;
+        }
     }
+    float F[4], S[4];
+    const int capped = pair_forces(a, n, fluid, fluid, F, S);
+    if (!fluid) return capped;
     float ux0, uy0, ux1, uy1;
-    pair_collide<OP>(a, f0, n, true, Fx0, Fy0, ux0, uy0);
-    pair_collide<OP>(a, f1, n + 1, true, Fx1, Fy1, ux1, uy1);
+    pair_collide<OP>(a, f0, n, true, F[0], F[1], S[0], S[1], ux0, uy0);
+    pair_collide<OP>(a, f1, n + 1, true, F[2], F[3], S[2], S[3], ux1, uy1);
     const int xp[2] = {x, x + 1};
     if constexpr (ADRF) {
         for (int s = 0; s < a.nsp; ++s) {

@@ -24,6 +24,10 @@ struct MemArgs {
     const sf_capsule_desc* desc;
     int periodic_nx;
     int* counters;    // counters[0] (capped cells of this step) is cleared here
+    // when set: each node's arc length element for the spread (half the
+    // distances to its ring neighbours, minimum image over ibm_nx)
+    double* arc;
+    int ibm_nx;
 };
 
 __device__ __forceinline__ void mimg(double ax, double ay, double bx, double by, double Lx,
@@ -36,10 +40,28 @@ __device__ __forceinline__ void mimg(double ax, double ay, double bx, double by,
     }
 }
 
+// Sequential FP64 sum init + v[0] + v[1] + ... + v[n-1] (one lane; the loads
+// of a chunk go out before its additions).
+__device__ __forceinline__ double serial_sum(const double* v, int n, double init) {
+    double acc = init;
+    int i = 0;
+    for (; i + 8 <= n; i += 8) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = v[i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += t[u];
+    }
+    for (; i < n; ++i) acc += v[i];
+    return acc;
+}
+
 template <int MAXN>
 __device__ __forceinline__ void membrane_body(MemArgs a, int bx) {
     __shared__ double X[MAXN], Y[MAXN], FX[MAXN], FY[MAXN];
     __shared__ double Ex[MAXN], Ey[MAXN], Gx[MAXN], Gy[MAXN], Cc[MAXN], Ln[MAXN];
+    // per-node inputs, all read from memory in the first pass
+    __shared__ double VX[MAXN], VY[MAXN], RL[MAXN], RTH[MAXN], RDS[MAXN], LP[MAXN];
     __shared__ int skip[MAXN];
     __shared__ double red[2];
     const int cap = bx;
@@ -51,10 +73,25 @@ __device__ __forceinline__ void membrane_body(MemArgs a, int bx) {
     for (int i = lane; i < n; i += 64) {
         X[i] = a.x[o + i];
         Y[i] = a.y[o + i];
+        VX[i] = a.vx[o + i];
+        VY[i] = a.vy[o + i];
+        RL[i] = a.L0[o + i];
+        RTH[i] = a.th0[o + i];
+        RDS[i] = a.ds0[o + i];
+        LP[i] = a.Lprev[o + i];
         FX[i] = 0.0;
         FY[i] = 0.0;
     }
     __syncthreads();
+    if (a.arc) {
+        const double Li = static_cast<double>(a.ibm_nx);
+        for (int i = lane; i < n; i += 64) {
+            const int ip = (i - 1 + n) % n, in = (i + 1) % n;
+            const double dp = min_image_ds(X[i], Y[i], X[ip], Y[ip], Li);
+            const double dn = min_image_ds(X[in], Y[in], X[i], Y[i], Li);
+            a.arc[o + i] = 0.5 * (dp + dn);
+        }
+    }
     if (d.is_rigid) {
         for (int i = lane; i < n; i += 64) { a.fx[o + i] = 0.0; a.fy[o + i] = 0.0; }
         return;
@@ -84,7 +121,7 @@ __device__ __forceinline__ void membrane_body(MemArgs a, int bx) {
         const double L = sqrt(dx * dx + dy * dy);
         skip[i] = L < 1e-15;
         if (skip[i]) continue;
-        const double L0 = a.L0[o + i];
+        const double L0 = RL[i];
         double Fm;
         if (d.model == 1) {
             double lam = L / L0;
@@ -122,8 +159,8 @@ __device__ __forceinline__ void membrane_body(MemArgs a, int bx) {
             const double lp2 = epx * epx + epy * epy, ln2 = enx * enx + eny * eny;
             skip[i] = (lp2 < 1e-30 || ln2 < 1e-30);
             if (skip[i]) continue;
-            const double dth = atan2(epx * eny - epy * enx, epx * enx + epy * eny) - a.th0[o + i];
-            Cc[i] = -d.k_bend * dth / a.ds0[o + i];
+            const double dth = atan2(epx * eny - epy * enx, epx * enx + epy * eny) - RTH[i];
+            Cc[i] = -d.k_bend * dth / RDS[i];
             Ex[i] = -eny / ln2;   // d theta / d x_{i+1}
             Ey[i] = enx / ln2;
             Gx[i] = -epy / lp2;   // d theta / d x_{i-1}
@@ -162,30 +199,46 @@ __device__ __forceinline__ void membrane_body(MemArgs a, int bx) {
     __syncthreads();
 
     // -- area penalty: capsule.cpp:547-586 (area() and the signed sum) ----
-    if (lane == 0) {
-        double A = 0.0;
-        if (Lx > 0.0 && n > 0) {
+    // Periodic x: the ring unwrapped from node 0 (capsule.cpp:177-186) by the
+    // sequential sum of the minimum-image steps, then the shoelace terms; one
+    // lane adds them in node order.  Ex/Ey hold the steps, Gx/Gy the unwrapped
+    // coordinates, Cc the terms.
+    if (Lx > 0.0) {
+        for (int i = lane; i < n; i += 64)
+            if (i >= 1) mimg(X[i], Y[i], X[i - 1], Y[i - 1], Lx, Ex[i], Ey[i]);
+        __syncthreads();
+        if (lane == 0 && n > 0) {
             double ux = X[0], uy = Y[0];
-            // unwrapped ring (capsule.cpp:177-186); Gx/Gy hold the coordinates
             Gx[0] = ux; Gy[0] = uy;
-            for (int i = 1; i < n; ++i) {
-                double dx, dy;
-                mimg(X[i], Y[i], X[i - 1], Y[i - 1], Lx, dx, dy);
-                ux = Gx[i - 1] + dx; uy = Gy[i - 1] + dy;
+            int i = 1;
+            for (; i + 8 <= n; i += 8) {
+                double tx[8], ty[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) { tx[u] = Ex[i + u]; ty[u] = Ey[i + u]; }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    ux = ux + tx[u]; uy = uy + ty[u];
+                    Gx[i + u] = ux; Gy[i + u] = uy;
+                }
+            }
+            for (; i < n; ++i) {
+                ux = ux + Ex[i]; uy = uy + Ey[i];
                 Gx[i] = ux; Gy[i] = uy;
             }
-            for (int i = 0; i < n; ++i) {
-                const int j = (i + 1) % n;
-                A += Gx[i] * Gy[j] - Gy[i] * Gx[j];
-            }
-        } else {
-            for (int i = 0; i < n; ++i) {
-                const int j = nxt(i);
-                A += X[i] * Y[j] - Y[i] * X[j];
-            }
         }
-        red[0] = A;
+        __syncthreads();
+        for (int i = lane; i < n; i += 64) {
+            const int j = (i + 1) % n;
+            Cc[i] = Gx[i] * Gy[j] - Gy[i] * Gx[j];
+        }
+    } else {
+        for (int i = lane; i < n; i += 64) {
+            const int j = nxt(i);
+            Cc[i] = X[i] * Y[j] - Y[i] * X[j];
+        }
     }
+    __syncthreads();
+    if (lane == 0) red[0] = serial_sum(Cc, n, 0.0);
     __syncthreads();
     {
         const double A0 = d.rest_area;
@@ -221,11 +274,7 @@ __device__ __forceinline__ void membrane_body(MemArgs a, int bx) {
         if (!skip[i]) { Ex[i] = dx / L; Ey[i] = dy / L; }
     }
     __syncthreads();
-    if (lane == 0) {
-        double P = 0.0;
-        for (int i = 0; i < n; ++i) P += Ln[i];
-        red[1] = P;
-    }
+    if (lane == 0) red[1] = serial_sum(Ln, n, 0.0);
     __syncthreads();
     {
         const double P0 = d.rest_perimeter;
@@ -241,8 +290,8 @@ __device__ __forceinline__ void membrane_body(MemArgs a, int bx) {
 
     // -- nodal damping: capsule.cpp:606-610 ------------------------------
     for (int i = lane; i < n; i += 64) {
-        FX[i] -= a.vx[o + i] * d.gamma_visc;
-        FY[i] -= a.vy[o + i] * d.gamma_visc;
+        FX[i] -= VX[i] * d.gamma_visc;
+        FY[i] -= VY[i] * d.gamma_visc;
     }
     __syncthreads();
 
@@ -257,7 +306,7 @@ __device__ __forceinline__ void membrane_body(MemArgs a, int bx) {
             if (first) { a.Lprev[o + i] = L; skip[i] = 1; continue; }
             skip[i] = L < 1e-15;
             if (skip[i]) continue;
-            const double sr = (L - a.Lprev[o + i]) / a.L0[o + i];
+            const double sr = (L - LP[i]) / RL[i];
             const double Fv = d.eta_membrane * sr;
             const double ux = dx / L, uy = dy / L;
             Ex[i] = ux * Fv;
@@ -287,6 +336,8 @@ static MemArgs mem_args(const Ctx& c) {
     a.off = c.off; a.desc = c.desc;
     a.periodic_nx = c.periodic_x ? c.nx : 0;
     a.counters = c.counters;
+    a.arc = c.arc;
+    a.ibm_nx = c.nx;
     return a;
 }
 
@@ -512,19 +563,89 @@ constexpr int REP_SLOTS = 6;       // non-zero pair forces a lane keeps
 // lane 0 screens them all itself.
 constexpr int REP_OBS_LDS = 256;
 
+// Lane 0 of a node: its membrane force plus the pair forces the lanes kept, in
+// the reference's order (or the serial loop after an overflow), then the
+// obstacles; stores the node's force and nearest distance.
+__device__ __forceinline__ void repulsion_lanes_sum(const RepArgs& a, int k, int g, int cap, double px, double py,
+                                                    double vx, double vy, bool obs_lds,
+                                                    double (*sfx)[11][REP_SLOTS], double (*sfy)[11][REP_SLOTS],
+                                                    int (*scnt)[16], float (*sdmin)[16],
+                                                    unsigned long long (*snear)[REP_OBS_LDS / 64], double fx,
+                                                    double fy) {
+    double Fx, Fy;
+    bool overflow = false;
+    for (int u = 0; u < 11; ++u) overflow |= scnt[g][u] > REP_SLOTS;
+    if (!overflow) {
+        for (int u = 0; u < 11; ++u)
+            for (int j = 0; j < scnt[g][u]; ++j) { fx += sfx[g][u][j]; fy += sfy[g][u][j]; }
+    } else {
+        // the serial loop of the reference (cells, then the wall images)
+        const int ci = cell_index(a, px, py);
+        const int cx0 = ci % a.ncx, cy0 = ci / a.ncx;
+        for (int dyc = -1; dyc <= 1; ++dyc) {
+            const int yy = cy0 + dyc;
+            if (yy < 0 || yy >= a.ncy) continue;
+            for (int dxc = -1; dxc <= 1; ++dxc) {
+                int xx = cx0 + dxc;
+                if (a.Lx > 0.0) { if (xx < 0) xx += a.ncx; if (xx >= a.ncx) xx -= a.ncx; }
+                else if (xx < 0 || xx >= a.ncx) continue;
+                const int cell = yy * a.ncx + xx;
+                int m = a.cnt[cell];
+                if (m > CELL_MAXC) m = CELL_MAXC;
+                const size_t base = static_cast<size_t>(cell) * CELL_MAXC;
+                for (int t = 0; t < m; ++t) {
+                    if (a.own[base + t] == cap) continue;
+                    const double4 b = a.pay[base + t];
+                    pair_force(a, px, py, vx, vy, b.x, b.y, b.z, b.w, Fx, Fy);
+                    fx += Fx; fy += Fy;
+                }
+            }
+        }
+        pair_force(a, px, py, vx, vy, px, a.ybot, 0.0, 0.0, Fx, Fy);
+        fx += Fx; fy += Fy;
+        pair_force(a, px, py, vx, vy, px, a.ytop, 0.0, 0.0, Fx, Fy);
+        fx += Fx; fy += Fy;
+    }
+    if (obs_lds && snear) {
+        for (int w = 0; w < REP_OBS_LDS / 64; ++w) {
+            unsigned long long bits = snear[g][w];
+            while (bits) {
+                const int o = w * 64 + __ffsll(static_cast<long long>(bits)) - 1;
+                bits &= bits - 1ull;
+                obstacle_contact(a.obs[o], px, py, vx, vy, a.eps, a.sig, a.pw, a.rcut, a.gn, a.mu, fx, fy);
+            }
+        }
+    } else {
+        obstacle_forces(a, px, py, vx, vy, fx, fy);
+    }
+    a.fx[k] = fx;
+    a.fy[k] = fy;
+    if (a.dmin) {
+        float dm = sdmin[g][0];
+        for (int u = 1; u < 9; ++u) dm = fminf(dm, sdmin[g][u]);
+        a.dmin[k] = dm;
+    }
+}
+
+// OBS: the lattice may hold obstacles (their holding discs are staged in LDS,
+// which the form without them leaves free for more resident workgroups).
+template <bool OBS>
 __device__ __forceinline__ void repulsion_lanes_body(RepArgs a, int bx) {
     __shared__ double sfx[REP_NODES][11][REP_SLOTS], sfy[REP_NODES][11][REP_SLOTS];
     __shared__ int scnt[REP_NODES][16];
     __shared__ float sdmin[REP_NODES][16];
-    __shared__ double sbc[REP_OBS_LDS][3];
-    __shared__ unsigned long long snear[REP_NODES][REP_OBS_LDS / 64];
+    __shared__ double sbc[OBS ? REP_OBS_LDS : 1][3];
+    __shared__ unsigned long long snear_lds[OBS ? REP_NODES : 1][REP_OBS_LDS / 64];
+    unsigned long long (*snear)[REP_OBS_LDS / 64] = OBS ? snear_lds : nullptr;
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     const int k = bx * REP_NODES + g;
     const bool live = k < a.K;
     int cap = 0;
     double px = 0.0, py = 0.0, vx = 0.0, vy = 0.0;
+    double fx0 = 0.0, fy0 = 0.0;   // the membrane force, read by the node's lane 0 up front
     if (live) { cap = a.owner[k]; px = a.x[k]; py = a.y[k]; vx = a.vx[k]; vy = a.vy[k]; }
-    const bool obs_lds = a.nobs > 0 && a.nobs <= REP_OBS_LDS;
+    if (live && l == 0) { fx0 = a.fx[k]; fy0 = a.fy[k]; }
+    const bool obs_lds = OBS && a.nobs > 0 && a.nobs <= REP_OBS_LDS;
     if (obs_lds) {
         for (int i = threadIdx.x; i < a.nobs; i += blockDim.x) {
             sbc[i][0] = a.obs[i].bc[0];
@@ -577,64 +698,13 @@ __device__ __forceinline__ void repulsion_lanes_body(RepArgs a, int bx) {
     scnt[g][l] = nz;
     sdmin[g][l] = static_cast<float>(sqrt(d2min));
     __syncthreads();
-    if (!live || l != 0) return;
-    double fx = a.fx[k], fy = a.fy[k];
-    bool overflow = false;
-    for (int u = 0; u < 11; ++u) overflow |= scnt[g][u] > REP_SLOTS;
-    if (!overflow) {
-        for (int u = 0; u < 11; ++u)
-            for (int j = 0; j < scnt[g][u]; ++j) { fx += sfx[g][u][j]; fy += sfy[g][u][j]; }
-    } else {
-        // the serial loop of the reference (cells, then the wall images)
-        const int ci = cell_index(a, px, py);
-        const int cx0 = ci % a.ncx, cy0 = ci / a.ncx;
-        for (int dyc = -1; dyc <= 1; ++dyc) {
-            const int yy = cy0 + dyc;
-            if (yy < 0 || yy >= a.ncy) continue;
-            for (int dxc = -1; dxc <= 1; ++dxc) {
-                int xx = cx0 + dxc;
-                if (a.Lx > 0.0) { if (xx < 0) xx += a.ncx; if (xx >= a.ncx) xx -= a.ncx; }
-                else if (xx < 0 || xx >= a.ncx) continue;
-                const int cell = yy * a.ncx + xx;
-                int m = a.cnt[cell];
-                if (m > CELL_MAXC) m = CELL_MAXC;
-                const size_t base = static_cast<size_t>(cell) * CELL_MAXC;
-                for (int t = 0; t < m; ++t) {
-                    if (a.own[base + t] == cap) continue;
-                    const double4 b = a.pay[base + t];
-                    pair_force(a, px, py, vx, vy, b.x, b.y, b.z, b.w, Fx, Fy);
-                    fx += Fx; fy += Fy;
-                }
-            }
-        }
-        pair_force(a, px, py, vx, vy, px, a.ybot, 0.0, 0.0, Fx, Fy);
-        fx += Fx; fy += Fy;
-        pair_force(a, px, py, vx, vy, px, a.ytop, 0.0, 0.0, Fx, Fy);
-        fx += Fx; fy += Fy;
-    }
-    if (obs_lds) {
-        for (int w = 0; w < REP_OBS_LDS / 64; ++w) {
-            unsigned long long bits = snear[g][w];
-            while (bits) {
-                const int o = w * 64 + __ffsll(static_cast<long long>(bits)) - 1;
-                bits &= bits - 1ull;
-                obstacle_contact(a.obs[o], px, py, vx, vy, a.eps, a.sig, a.pw, a.rcut, a.gn, a.mu, fx, fy);
-            }
-        }
-    } else {
-        obstacle_forces(a, px, py, vx, vy, fx, fy);
-    }
-    a.fx[k] = fx;
-    a.fy[k] = fy;
-    if (a.dmin) {
-        float dm = sdmin[g][0];
-        for (int u = 1; u < 9; ++u) dm = fminf(dm, sdmin[g][u]);
-        a.dmin[k] = dm;
-    }
+    if (live && l == 0)
+        repulsion_lanes_sum(a, k, g, cap, px, py, vx, vy, obs_lds, sfx, sfy, scnt, sdmin, snear, fx0, fy0);
 }
 
+template <bool OBS>
 __global__ __launch_bounds__(256) void k_repulsion_lanes(RepArgs a) {
-    repulsion_lanes_body(a, blockIdx.x);
+    repulsion_lanes_body<OBS>(a, blockIdx.x);
 }
 
 // x extent of the cell grid when x is not periodic (cell_list.h:50-66):
@@ -764,7 +834,8 @@ void launch_repulsion(Ctx& c) {
     a.cnt = cnt;
     a.pay = c.cell_pay;
     a.own = c.cell_own;
-    hipLaunchKernelGGL(k_repulsion_lanes, dim3((c.K + REP_NODES - 1) / REP_NODES), dim3(256), 0,
+    auto kern = c.nobs > 0 ? k_repulsion_lanes<true> : k_repulsion_lanes<false>;
+    hipLaunchKernelGGL(kern, dim3((c.K + REP_NODES - 1) / REP_NODES), dim3(256), 0,
                        c.stream, a);
 }
 
@@ -823,7 +894,7 @@ __global__ __launch_bounds__(256) void kb_contact(const ContactPack* packs, cons
     if (w.x < 0) return;
     const ContactPack p = packs[w.x];
     if (p.allpairs) repulsion_body<false>(p.a, w.y);
-    else repulsion_lanes_body(p.a, w.y);
+    else repulsion_lanes_body<true>(p.a, w.y);
 }
 
 void batch_items_membrane(Ctx& c, BatchItem* it) {
