@@ -250,6 +250,12 @@ int  sf_set_lbm_launch(sf_ctx* ctx, int bx, int by, int xcd_banded, int persist)
  * off.  sf_autotune times both forms and keeps the faster.  Results are
  * identical either way. */
 int  sf_set_lbm_pairs(sf_ctx* ctx, int on);
+/* Capsule-side launch choices (no counterpart in the reference): the spread
+ * gather's grid (0: one workgroup per lattice tile, else that many persistent
+ * workgroups walking the touched tiles) and the lanes per node of the
+ * interpolations (16, 8 or 4).  sf_autotune times both; results are
+ * identical for every choice. */
+int  sf_set_capsule_launch(sf_ctx* ctx, int spread_grid, int interp_lanes);
 
 /* ---- Multi-direct forcing control and the learned warm start ------------
  * (no counterpart in the reference; its MDF is a fixed iteration count,

@@ -29,7 +29,7 @@ EXPORTS = (
     "sf_set_scalar", "sf_get_scalar", "sf_step", "sf_set_graph_mode", "sf_synchronize", "sf_graph_captures",
     "sf_stage", "sf_last_capped", "sf_field_stats", "sf_field_stats_noalloc", "sf_global_stats", "sf_step_count", "sf_kernel_times",
     "sf_host_alloc", "sf_host_free", "sf_set_probes", "sf_snapshot_async", "sf_snapshot_wait",
-    "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_set_lbm_pairs", "sf_probe_math", "sf_copy_ceiling",
+    "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_set_lbm_pairs", "sf_set_capsule_launch", "sf_probe_math", "sf_copy_ceiling",
     "sf_mdf_set_control", "sf_mdf_stats", "sf_mdf_collect", "sf_mdf_samples", "sf_mdf_set_model",
     "sf_mlp_create", "sf_mlp_destroy", "sf_mlp_num_params", "sf_mlp_get_params", "sf_mlp_set_params",
     "sf_mlp_predict", "sf_mlp_loss_grad", "sf_mlp_train",
@@ -125,6 +125,7 @@ def load():
         "sf_autotune": [C.c_void_p, C.c_int, C.c_char_p, C.c_int],
         "sf_set_lbm_launch": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int],
         "sf_set_lbm_pairs": [C.c_void_p, C.c_int],
+        "sf_set_capsule_launch": [C.c_void_p, C.c_int, C.c_int],
         "sf_probe_math": [C.c_int, C.c_int, _DP, _DP, _DP],
         "sf_copy_ceiling": [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double)],
         "sf_mdf_set_control": [C.c_void_p, C.c_double, C.c_int, C.c_int],

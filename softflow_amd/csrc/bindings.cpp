@@ -527,6 +527,7 @@ PYBIND11_MODULE(softflow_core, m) {
         .def("graphCaptures", &Simulation::graphCaptures)
         .def("autotune", &Simulation::autotune, py::arg("reps") = 10)
         .def("setLbmPairs", &Simulation::setLbmPairs, py::arg("on"))
+        .def("setCapsuleLaunch", &Simulation::setCapsuleLaunch, py::arg("spread_grid"), py::arg("interp_lanes"))
         .def("setLbmLaunch", &Simulation::setLbmLaunch, py::arg("bx"), py::arg("by"), py::arg("xcd_banded") = true,
              py::arg("persist") = 0)
         .def("kernelTimes", &Simulation::kernelTimes)

@@ -953,6 +953,11 @@ void Simulation::setLbmLaunch(int bx, int by, bool xcd_banded, int persist) {
     check(sf_set_lbm_launch(ctx_, bx, by, xcd_banded ? 1 : 0, persist), "sf_set_lbm_launch");
 }
 
+void Simulation::setCapsuleLaunch(int spread_grid, int interp_lanes) {
+    if (!ctx_) throw std::runtime_error("Simulation: initialize() has not been called");
+    check(sf_set_capsule_launch(ctx_, spread_grid, interp_lanes), "sf_set_capsule_launch");
+}
+
 void Simulation::setLbmPairs(bool on) {
     if (!ctx_) throw std::runtime_error("Simulation: initialize() has not been called");
     check(sf_set_lbm_pairs(ctx_, on ? 1 : 0), "sf_set_lbm_pairs");

@@ -481,6 +481,8 @@ public:
     std::string autotune(int reps = 10);
     void setLbmLaunch(int bx, int by, bool xcd_banded, int persist = 0);
     void setLbmPairs(bool on);
+    // spread grid (0: one workgroup per tile) and interpolation lanes per node
+    void setCapsuleLaunch(int spread_grid, int interp_lanes);
 
     // multi-direct forcing control and the learned warm start (extensions)
     void setMdfControl(Real tol, int max_corrections, bool probe);

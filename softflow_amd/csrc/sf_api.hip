@@ -1425,6 +1425,29 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
         c.Fcur = keep_F;
         c.arc_fresh = keep_arc;
         c.spread_grid = sgrid;
+        // lanes per node of the interpolations, timed without the move and
+        // into scratch velocities
+        double* scratch_v = nullptr;
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&scratch_v), 2 * static_cast<size_t>(c.K) * sizeof(double)));
+        double ibest = 1e30;
+        int ilpn = c.interp_lpn;
+        for (const int lpn : {16, 8, 4}) {
+            launch_interp_probe(c, lpn, scratch_v, scratch_v + c.K);   // warm
+            SF_HIP(hipEventRecord(e0, c.stream));
+            for (int r = 0; r < reps; ++r) launch_interp_probe(c, lpn, scratch_v, scratch_v + c.K);
+            SF_HIP(hipEventRecord(e1, c.stream));
+            SF_HIP(hipEventSynchronize(e1));
+            SF_HIP(hipGetLastError());
+            float ms = 0.f;
+            SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const double per = ms / reps;
+            char line[96];
+            std::snprintf(line, sizeof(line), "interpolation %d lanes/node %.4f ms\n", lpn, per);
+            rep += line;
+            if (per < ibest) { ibest = per; ilpn = lpn; }
+        }
+        SF_HIP(hipFree(scratch_v));
+        c.interp_lpn = ilpn;
     }
     double best = 1e30;
     int bbx = c.lbm_bx, bby = c.lbm_by, brm = c.lbm_remap, bps = c.lbm_persist, bpr = c.lbm_pair;
@@ -1465,10 +1488,10 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
     invalidate_graph(c);
     char line[160];
     const char* cl = (bpr && pair_max) ? " 2 cells/lane" : "";
-    char sg[48] = "";
+    char sg[80] = "";
     if (c.K > 0 && c.tile_cnt)
-        std::snprintf(sg, sizeof(sg), ", spread grid %d",
-                      c.spread_grid > 0 ? std::min(c.spread_grid, c.ntiles) : c.ntiles);
+        std::snprintf(sg, sizeof(sg), ", spread grid %d, interpolation %d lanes/node",
+                      c.spread_grid > 0 ? std::min(c.spread_grid, c.ntiles) : c.ntiles, c.interp_lpn);
     if (bps)
         std::snprintf(line, sizeof(line), "chosen %dx%d persistent %d/xcd%s%s\n", bbx, bby, bps, cl, sg);
     else
@@ -1491,6 +1514,18 @@ int sf_set_lbm_launch(sf_ctx* h, int bx, int by, int xcd_banded, int persist) {
     Ctx& c = h->c;
     c.lbm_bx = bx; c.lbm_by = by; c.lbm_remap = (xcd_banded || persist) ? 1 : 0;
     c.lbm_persist = persist;
+    invalidate_graph(c);
+    return SF_OK;
+}
+
+int sf_set_capsule_launch(sf_ctx* h, int spread_grid, int interp_lanes) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    if (spread_grid < 0) return fail(SF_E_ARG, "spread grid must be >= 0");
+    if (interp_lanes != 16 && interp_lanes != 8 && interp_lanes != 4)
+        return fail(SF_E_ARG, "interpolation lanes per node must be 16, 8 or 4");
+    Ctx& c = h->c;
+    c.spread_grid = spread_grid;
+    c.interp_lpn = interp_lanes;
     invalidate_graph(c);
     return SF_OK;
 }

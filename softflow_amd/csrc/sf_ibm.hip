@@ -617,13 +617,15 @@ __device__ __forceinline__ void cell_velocity(const CellVelArgs& a, int x, int y
     }
 }
 
-// Interpolation at a point (immersed_boundary.cpp:191-229) with 16 lanes per
-// node: lane l handles stencil cell (iy, ix) = (l / 4, l % 4); the terms are
-// summed in the reference's loop order (iy outer, ix inner) by the group's
-// first lane, so the result is the reference's sequential sum.
-constexpr int INTERP_NODES = 16;   // nodes per 256-thread workgroup
+// Interpolation at a point (immersed_boundary.cpp:191-229) with LPN lanes
+// per node (16, 8 or 4): lane l handles the 16 / LPN stencil cells
+// l * 16 / LPN .., cell c being (iy, ix) = (c / 4, c % 4); the terms are summed
+// in the reference's loop order (iy outer, ix inner) by the group's first
+// lane, so the result is the reference's sequential sum whatever LPN is.
+constexpr int INTERP_NODES = 16;   // nodes per 256-thread workgroup at 16 lanes per node
+__host__ __device__ constexpr int interp_nodes(int lpn) { return 256 / lpn; }
 
-template <bool FP32>
+template <bool FP32, int LPN>
 __device__ __forceinline__ void interp_group(const CellVelArgs& a, const double* lut, double px,
                                              double py, int force_mode, int l, double* tx,
                                              double* ty, int* use, double& Ux, double& Uy) {
@@ -635,26 +637,31 @@ __device__ __forceinline__ void interp_group(const CellVelArgs& a, const double*
         iy0 = static_cast<int>(floor(py));
         if (iy0 < -2 || iy0 >= a.ny + 2) ok = false;
     }
-    int flag = 0;
-    double vx = 0.0, vy = 0.0;
-    if (ok) {
-        const int ix = ix0 - 1 + (l & 3), iy = iy0 - 1 + (l >> 2);
-        int jx = ix;
-        if (jx < 0) jx += a.nx;
-        if (jx >= a.nx) jx -= a.nx;
-        if (iy >= 0 && iy < a.ny) {
-            const double w = phi_lut(lut, px - static_cast<double>(ix)) *
-                             phi_lut(lut, py - static_cast<double>(iy));
-            if (w > 1e-15) {
-                double r, ux, uy;
-                cell_velocity<FP32>(a, jx, iy, force_mode, r, ux, uy);
-                vx = ux * w;
-                vy = uy * w;
-                flag = 1;
+    constexpr int CPL = 16 / LPN;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+        const int cell = l * CPL + j;
+        int flag = 0;
+        double vx = 0.0, vy = 0.0;
+        if (ok) {
+            const int ix = ix0 - 1 + (cell & 3), iy = iy0 - 1 + (cell >> 2);
+            int jx = ix;
+            if (jx < 0) jx += a.nx;
+            if (jx >= a.nx) jx -= a.nx;
+            if (iy >= 0 && iy < a.ny) {
+                const double w = phi_lut(lut, px - static_cast<double>(ix)) *
+                                 phi_lut(lut, py - static_cast<double>(iy));
+                if (w > 1e-15) {
+                    double r, ux, uy;
+                    cell_velocity<FP32>(a, jx, iy, force_mode, r, ux, uy);
+                    vx = ux * w;
+                    vy = uy * w;
+                    flag = 1;
+                }
             }
         }
+        tx[cell] = vx; ty[cell] = vy; use[cell] = flag;
     }
-    tx[l] = vx; ty[l] = vy; use[l] = flag;
     __syncthreads();
     Ux = 0.0; Uy = 0.0;
     if (l == 0) {
@@ -669,21 +676,22 @@ __device__ __forceinline__ void interp_group(const CellVelArgs& a, const double*
 // Also records the no-slip residual max_k |V_k - U(X_k)| of this iteration
 // (md->res[it], an order-independent max) and, when u0 is set, the slip
 // V - U per node.  After convergence (residual mode) it does nothing.
-template <bool FP32>
+template <bool FP32, int LPN = 16>
 __device__ __forceinline__ void mdf_interp_body(const CellVelArgs& a, const double* lut, const double* x,
                                                 const double* y, const double* vx, const double* vy,
                                                 NodeSpread* rec, int K, MdfDev* md, int it, double* u0,
                                                 int bx) {
-    __shared__ double tx[256], ty[256];
-    __shared__ int use[256];
-    __shared__ double sres[INTERP_NODES];
+    constexpr int NPW = interp_nodes(LPN);
+    __shared__ double tx[16 * NPW], ty[16 * NPW];
+    __shared__ int use[16 * NPW];
+    __shared__ double sres[NPW];
     if (md->conv) return;       // uniform: no kernel writes it while this one runs
-    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    const int k = bx * INTERP_NODES + g;
+    const int g = threadIdx.x / LPN, l = threadIdx.x % LPN;
+    const int k = bx * NPW + g;
     const bool live = k < K;
     const double px = live ? x[k] : 0.0, py = live ? y[k] : 0.0;
     double Ux, Uy;
-    interp_group<FP32>(a, lut, px, py, 0, l, tx + 16 * g, ty + 16 * g, use + 16 * g, Ux, Uy);
+    interp_group<FP32, LPN>(a, lut, px, py, 0, l, tx + 16 * g, ty + 16 * g, use + 16 * g, Ux, Uy);
     if (l == 0) {
         double res = 0.0;
         if (live) {
@@ -709,7 +717,7 @@ __device__ __forceinline__ void mdf_interp_body(const CellVelArgs& a, const doub
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = 0.0;
-        for (int i = 0; i < INTERP_NODES; ++i) m = (sres[i] > m || sres[i] != sres[i]) ? sres[i] : m;
+        for (int i = 0; i < NPW; ++i) m = (sres[i] > m || sres[i] != sres[i]) ? sres[i] : m;
         // non-negative doubles order like their bit patterns (NaN above all)
         atomicMax(&md->res[it], static_cast<unsigned long long>(__double_as_longlong(m)));
     }
@@ -725,14 +733,14 @@ __device__ __forceinline__ int band_block(int b, int band) {
     return band > 0 ? (b % 8) * band + b / 8 : b;
 }
 
-template <bool FP32>
+template <bool FP32, int LPN>
 __global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double* lut, const double* x,
                                                     const double* y, const double* vx, const double* vy,
                                                     NodeSpread* rec, int K, MdfDev* md, int it,
                                                     double* u0, int band) {
     const int bx = band_block(blockIdx.x, band);
-    if (bx * INTERP_NODES >= K) return;       // padding workgroup of the banded grid
-    mdf_interp_body<FP32>(a, lut, x, y, vx, vy, rec, K, md, it, u0, bx);
+    if (bx * interp_nodes(LPN) >= K) return;       // padding workgroup of the banded grid
+    mdf_interp_body<FP32, LPN>(a, lut, x, y, vx, vy, rec, K, md, it, u0, bx);
 }
 
 // Per-node inputs of the warm-start model, in FP32: slip V - U at the first
@@ -883,17 +891,18 @@ __device__ __forceinline__ void move_node(const MoveArgs& m, double& px, double&
     }
 }
 
-template <bool FP32>
+template <bool FP32, int LPN = 16>
 __device__ __forceinline__ void interp_body(CellVelArgs a, const double* lut, double* x,
                                                 double* y, double* vx, double* vy, int K, MoveArgs mv, int bx) {
-    __shared__ double tx[256], ty[256];
-    __shared__ int use[256];
-    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    const int k = bx * INTERP_NODES + g;
+    constexpr int NPW = interp_nodes(LPN);
+    __shared__ double tx[16 * NPW], ty[16 * NPW];
+    __shared__ int use[16 * NPW];
+    const int g = threadIdx.x / LPN, l = threadIdx.x % LPN;
+    const int k = bx * NPW + g;
     const bool live = k < K;
     double px = live ? x[k] : 0.0, py = live ? y[k] : 0.0;
     double Ux, Uy;
-    interp_group<FP32>(a, lut, px, py, 1, l, tx + 16 * g, ty + 16 * g, use + 16 * g, Ux, Uy);
+    interp_group<FP32, LPN>(a, lut, px, py, 1, l, tx + 16 * g, ty + 16 * g, use + 16 * g, Ux, Uy);
     if (live && l == 0) {
         vx[k] = Ux;
         vy[k] = Uy;
@@ -905,13 +914,13 @@ __device__ __forceinline__ void interp_body(CellVelArgs a, const double* lut, do
     }
 }
 
-template <bool FP32>
+template <bool FP32, int LPN>
 __global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut, double* x,
                                                 double* y, double* vx, double* vy, int K, MoveArgs mv,
                                                 int band) {
     const int bx = band_block(blockIdx.x, band);
-    if (bx * INTERP_NODES >= K) return;       // padding workgroup of the banded grid
-    interp_body<FP32>(a, lut, x, y, vx, vy, K, mv, bx);
+    if (bx * interp_nodes(LPN) >= K) return;       // padding workgroup of the banded grid
+    interp_body<FP32, LPN>(a, lut, x, y, vx, vy, K, mv, bx);
 }
 
 // moveAllNodes + y-clamp (capsule.cpp:267-297, simulation.cpp:665-675); one
@@ -985,10 +994,30 @@ static CellVelArgs vel_args(Ctx& c, const void* f, const void* F) {
     return a;
 }
 
-// workgroups per XCD band of the node-parallel interpolations (K nodes)
-static int interp_band(int K) {
-    const int blocks = (K + INTERP_NODES - 1) / INTERP_NODES;
+// workgroups per XCD band of the node-parallel interpolations (K nodes, lpn
+// lanes per node)
+static int interp_band(int K, int lpn) {
+    const int blocks = (K + interp_nodes(lpn) - 1) / interp_nodes(lpn);
     return (blocks + 7) / 8;
+}
+
+// k_interp at lpn lanes per node (an interpolation with mv.move = 0 only
+// writes the velocities)
+static void launch_interp_lpn(Ctx& c, int lpn, const CellVelArgs& a, double* vx, double* vy, const MoveArgs& mv) {
+    const int band = interp_band(c.K, lpn);
+    const dim3 grid(8 * band);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, vx, vy, c.K, mv, band);
+    };
+    if (c.fp32) {
+        if (lpn == 4) go(k_interp<true, 4>);
+        else if (lpn == 8) go(k_interp<true, 8>);
+        else go(k_interp<true, 16>);
+    } else {
+        if (lpn == 4) go(k_interp<false, 4>);
+        else if (lpn == 8) go(k_interp<false, 8>);
+        else go(k_interp<false, 16>);
+    }
 }
 
 // mode 0: bin nodes and spread node forces into F[Fcur] (overwrite).
@@ -1047,15 +1076,23 @@ void launch_spread(Ctx& c, int mode, bool last, int it, int last_it) {
 void launch_mdf_interp(Ctx& c, int it) {
     if (c.K == 0) return;
     CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
-    const int band = interp_band(c.K);
+    const int lpn = c.interp_lpn;
+    const int band = interp_band(c.K, lpn);
     const dim3 grid(8 * band);
     double* u0 = it == 1 && c.mdf_hist_on ? c.mdf_u0 : nullptr;
-    if (c.fp32)
-        hipLaunchKernelGGL((k_mdf_interp<true>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y,
-                           c.vx, c.vy, c.spread, c.K, c.mdf, it, u0, band);
-    else
-        hipLaunchKernelGGL((k_mdf_interp<false>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y,
-                           c.vx, c.vy, c.spread, c.K, c.mdf, it, u0, band);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, c.vx, c.vy, c.spread, c.K,
+                           c.mdf, it, u0, band);
+    };
+    if (c.fp32) {
+        if (lpn == 4) go(k_mdf_interp<true, 4>);
+        else if (lpn == 8) go(k_mdf_interp<true, 8>);
+        else go(k_mdf_interp<true, 16>);
+    } else {
+        if (lpn == 4) go(k_mdf_interp<false, 4>);
+        else if (lpn == 8) go(k_mdf_interp<false, 8>);
+        else go(k_mdf_interp<false, 16>);
+    }
 }
 
 void launch_mdf_begin(Ctx& c) {
@@ -1102,20 +1139,19 @@ __global__ void k_recirculate_x(double* x, const int* off, int ncaps, double Lx)
         for (int i = 0; i < n; ++i) x[o + i] += shift;
 }
 
+void launch_interp_probe(Ctx& c, int lpn, double* vx, double* vy) {
+    if (c.K == 0) return;
+    const MoveArgs mv{0, c.p.dt, 0, 0, 0.0, 0.0};
+    launch_interp_lpn(c, lpn, vel_args(c, c.f[c.fcur], c.F[c.Fcur]), vx, vy, mv);
+}
+
 void launch_interp_move(Ctx& c) {
     c.arc_fresh = false;   // the nodes move
     if (c.K == 0) return;
-    CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
-    const int band = interp_band(c.K);
-    const dim3 grid(8 * band);
+    const CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
     const MoveArgs mv{c.any_rigid ? 0 : 1, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
                       1.5, static_cast<double>(c.ny) - 1.5};
-    if (c.fp32)
-        hipLaunchKernelGGL((k_interp<true>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, c.vx,
-                           c.vy, c.K, mv, band);
-    else
-        hipLaunchKernelGGL((k_interp<false>), grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, c.vx,
-                           c.vy, c.K, mv, band);
+    launch_interp_lpn(c, c.interp_lpn, a, c.vx, c.vy, mv);
     if (c.any_rigid)
         hipLaunchKernelGGL(k_move, dim3(c.ncaps), dim3(64), 0, c.stream, c.x, c.y, c.vx, c.vy, c.off,
                            c.desc, c.ncaps, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,

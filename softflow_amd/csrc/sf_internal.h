@@ -267,6 +267,7 @@ struct Ctx {
     // 0 never, 1 channels, 2 also generic cell types
     int lbm_pair = 1;
     int spread_grid = 0;                 // workgroups of the spread gather (0: one per tile; else persistent)
+    int interp_lpn = 16;                 // lanes per node of the interpolations (16, 8 or 4)
 };
 
 // Error reporting shared by the C-ABI translation units: sets the message
@@ -315,6 +316,9 @@ void launch_mdf_features(Ctx& c);
 void launch_mdf_record(Ctx& c, int ncorr);
 void launch_mlp_infer(Ctx& c);
 void launch_interp_move(Ctx& c);
+// The step's interpolation at lpn lanes per node without the move, writing
+// the velocities into vx/vy (the autotuner's timing run).
+void launch_interp_probe(Ctx& c, int lpn, double* vx, double* vy);
 void launch_fill_int(Ctx& c, int* p, int n, int v);
 void launch_fill_zero(Ctx& c, void* p, size_t bytes);
 void probe_math(int op, int n, const double* a, const double* b, double* out);

@@ -60,6 +60,7 @@ def test_argument_validation_before_device():
         assert not h.value
     assert L.sf_create(None, 0, None) == capi.SF_E_ARG
     assert L.sf_set_obstacles(None, 0, None, None, None) == capi.SF_E_ARG
+    assert L.sf_set_capsule_launch(None, 0, 16) == capi.SF_E_ARG
     g = C.c_double()
     for nbytes, reps, out, msg in ((1 << 30, 10, None, "null"), (1000, 10, C.byref(g), "multiple of 16"),
                                    ((1 << 20) + 8, 10, C.byref(g), "multiple of 16"),
