@@ -11,7 +11,11 @@ held to the oracle at 1e-12 by the parity tests):
   the capsule centroid trajectories and the Taylor deformation;
 - the batched ensemble of 256 channels drawn with seed 7: every member finite,
   and members sampled across the batch (smallest, largest, and spread over
-  the index range) bit-identical to the same channel stepped on its own.
+  the index range) bit-identical to the same channel stepped on its own;
+- the 2048 x 512 multi-direct-forcing channel with 500 capsules in residual
+  mode: plain FP64 MDF and the learned FP32 warm start (trained here on rows
+  the plain solver collects) both hold the no-slip residual at or below the
+  tolerance, and the learned run applies fewer corrections per step.
 """
 import numpy as np
 import pytest
@@ -132,3 +136,39 @@ def test_ensemble_256_members_bitwise_to_solo():
                 pa, pb = ca[c].nodePosition(n), cb[c].nodePosition(n)
                 assert pa.x == pb.x and pa.y == pb.y, (i, c, n)
         print(f"member {i} ({specs[i]['nx']}x{specs[i]['ny']}, {len(ca)} capsules): bitwise", flush=True)
+
+
+def test_mdf_channel_learned_against_plain():
+    from softflow_amd import softflow_core as sc
+    from softflow_amd.workloads import build_mdf_channel
+    tol, max_corr, develop, collect, timed = 1e-5, 16, 1500, 60, 100
+    print(f"mdf channel: plain FP64, {develop} steps of 5-iteration MDF", flush=True)
+    plain, placed = build_mdf_channel("fp64", 5)
+    assert placed == 500
+    plain.run(develop)
+    plain.setMdfControl(tol, max_corr, True)
+    plain.run(6)
+    plain.mdfStats()
+    plain.run(timed)
+    sp = plain.mdfStats()
+    print(f"mdf channel: learned FP32, develop, collect {collect} steps, train", flush=True)
+    learned, _ = build_mdf_channel("fp32", 5)
+    learned.run(develop)
+    learned.setMdfControl(1e-300, 4, True)
+    learned.collectMdfSamples(collect * learned.capsules().totalNodes())
+    learned.run(collect)
+    model = sc.MdfModel(0, 7)
+    loss = model.train(learned, epochs=4, batch=4096, lr=2e-3, seed=11)
+    learned.collectMdfSamples(0)
+    learned.setMdfModel(model, True)
+    learned.setMdfControl(tol, max_corr, True)
+    learned.run(6)
+    learned.mdfStats()
+    learned.run(timed)
+    sl = learned.mdfStats()
+    print(f"mdf channel over {timed} steps at tolerance {tol:g}: plain FP64 {sp['mean_corrections']:.3f} "
+          f"corrections/step, residual max {sp['max_residual']:.3e}; learned FP32 {sl['mean_corrections']:.3f}, "
+          f"residual max {sl['max_residual']:.3e}; training loss {loss[0]:.3f} -> {loss[-1]:.3f}", flush=True)
+    assert sp["max_residual"] <= tol and sl["max_residual"] <= tol
+    assert sl["mean_corrections"] < sp["mean_corrections"]
+    assert loss[-1] < loss[0]
