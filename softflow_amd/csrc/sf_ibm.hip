@@ -118,11 +118,15 @@ __device__ __forceinline__ void node_tiles(const NodeSpread& s, int nx, int ny, 
 }
 
 // Appends node s.pad to each candidate tile's list (and its record to the
-// tile's copies): every slot is claimed before any store waits on one.
+// tile's copies).  Every lane of the wavefront calls it (t all -1 for lanes
+// without a node); the slots are claimed per wavefront (wave_claim), every
+// candidate's atomics issued before any store waits on one.
 __device__ __forceinline__ void bin_node(const BinArgs& b, const int (&t)[8], const NodeSpread& s) {
-    int slot[8];
+    int lead[8], base[8], slot[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) slot[c] = t[c] >= 0 ? atomicAdd(&b.tile_cnt[t[c]], 1) : 0;
+    for (int c = 0; c < 8; ++c) wave_claim(b.tile_cnt, t[c], lead[c], base[c]);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) slot[c] = wave_slot(lead[c], base[c]);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         if (t[c] < 0) continue;
@@ -148,7 +152,24 @@ __device__ __forceinline__ void node_prep_body(const double* x, const double* y,
                             int* counters, int mode, MdfDev* md, double tol, int it, int force_it,
                             double* total, double* head, int bx) {
     const int k = bx * blockDim.x + threadIdx.x;
-    if (mode == 0 && k == 0) counters[3] = 0;     // the touched-tile list is rebuilt after binning
+    if (mode == 0) {
+        // binning: every lane of the wavefront takes part in the slot claims
+        if (k == 0) counters[3] = 0;     // the touched-tile list is rebuilt after binning
+        int t[8];
+        NodeSpread s{};
+        if (k < K) {
+            const double px = x[k], py = y[k];
+            const double ds = arc ? arc[k] : node_arc(x, y, ring[k], px, py, nx);
+            s = node_record(px, py, fx[k], fy[k], ds, k, ny);
+            rec[k] = s;
+            node_tiles(s, nx, ny, ntx, t);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) t[c] = -1;
+        }
+        bin_node(BinArgs{rec, tile_cnt, tile_list, tile_rec, counters, nx, ny, ntx}, t, s);
+        return;
+    }
     if (k >= K) return;
     if (mode == 1 && md) {
         if (md->conv) return;
@@ -166,19 +187,10 @@ __device__ __forceinline__ void node_prep_body(const double* x, const double* y,
             head[2 * k + 1] += rec[k].fy;
         }
     }
+    // mode 1: keep the force and arc length; re-validate
     const double px = x[k], py = y[k];
-    if (mode != 0) {
-        // keep the force and arc length; re-validate
-        const NodeSpread r = rec[k];
-        rec[k] = node_record(px, py, r.fx, r.fy, r.ds, k, ny);
-        return;
-    }
-    const double ds = arc ? arc[k] : node_arc(x, y, ring[k], px, py, nx);
-    const NodeSpread s = node_record(px, py, fx[k], fy[k], ds, k, ny);
-    rec[k] = s;
-    int t[8];
-    node_tiles(s, nx, ny, ntx, t);
-    bin_node(BinArgs{rec, tile_cnt, tile_list, tile_rec, counters, nx, ny, ntx}, t, s);
+    const NodeSpread r = rec[k];
+    rec[k] = node_record(px, py, r.fx, r.fy, r.ds, k, ny);
 }
 
 __global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,

@@ -46,6 +46,32 @@ struct NodeSpread {
     int i0, j0, valid, pad;
 };
 
+// Slot claims aggregated over a wavefront (1-D workgroups of whole
+// wavefronts; every lane of the wavefront takes part).  Each lane with a key
+// >= 0 claims one slot of the counter cnt[key].  A run of consecutive lanes
+// with the same key shares one atomic: its first lane adds the run's length,
+// and the run's lanes take consecutive slots in lane order.  Consecutive nodes
+// of a membrane mostly fall into the same bins, so a wavefront issues a few
+// atomics instead of one per lane.  leader/base: the lane's run leader and
+// the value its atomic returned (0 for lanes without a key); the slot is
+// base + lane - leader, read with wave_slot once the atomics are issued.
+__device__ __forceinline__ void wave_claim(int* cnt, int key, int& leader, int& base) {
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1);
+    const bool start = lane == 0 || key != prev;
+    const unsigned long long starts = __ballot(start);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);   // lanes 0 .. lane
+    leader = 63 - __clzll(static_cast<long long>(starts & upto));
+    const unsigned long long after = starts & ~upto;
+    const int next = after ? __ffsll(static_cast<long long>(after)) - 1 : 64;
+    base = 0;
+    if (start && key >= 0) base = atomicAdd(&cnt[key], next - lane);
+}
+
+__device__ __forceinline__ int wave_slot(int leader, int base) {
+    return __shfl(base, leader) + (static_cast<int>(threadIdx.x & 63) - leader);
+}
+
 // Distance between two points with the minimum image over Lx (> 0).
 __device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, double by, double Lx) {
     double dx = ax - bx, dy = ay - by;   // immersed_boundary.cpp:16-23

@@ -445,11 +445,15 @@ __device__ __forceinline__ int cell_index(const RepArgs& a, double px0, double p
     return iy * a.ncx + ix;
 }
 
+// Every lane of the wavefront takes part in the slot claims (wave_claim):
+// consecutive nodes mostly share a cell.
 __device__ __forceinline__ void cells_fill_body(RepArgs a, int* cnt, int* list, int* counters, int bx) {
     const int k = bx * blockDim.x + threadIdx.x;
-    if (k >= a.K) return;
-    const int ci = cell_index(a, a.x[k], a.y[k]);
-    const int slot = atomicAdd(&cnt[ci], 1);
+    const int ci = k < a.K ? cell_index(a, a.x[k], a.y[k]) : -1;
+    int lead, base;
+    wave_claim(cnt, ci, lead, base);
+    const int slot = wave_slot(lead, base);
+    if (ci < 0) return;
     if (slot < CELL_MAXC) list[static_cast<size_t>(ci) * CELL_MAXC + slot] = k;
     else atomicOr(&counters[2], 1);
 }
