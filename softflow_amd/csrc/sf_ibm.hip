@@ -174,7 +174,7 @@ __device__ __forceinline__ void node_prep_body(const double* x, const double* y,
     if (mode == 1 && md) {
         if (md->conv) return;
         if (tol > 0.0 && it != force_it &&
-            __longlong_as_double(static_cast<long long>(md->res[it])) <= tol) {
+            mdf_res(md, it) <= tol) {
             md->conv = 1;          // every thread reaches the same decision
             md->conv_it = it;
             return;
@@ -731,7 +731,7 @@ __device__ __forceinline__ void mdf_interp_body(const CellVelArgs& a, const doub
         double m = 0.0;
         for (int i = 0; i < NPW; ++i) m = (sres[i] > m || sres[i] != sres[i]) ? sres[i] : m;
         // non-negative doubles order like their bit patterns (NaN above all)
-        atomicMax(&md->res[it], static_cast<unsigned long long>(__double_as_longlong(m)));
+        atomicMax(&md->res[bx % MDF_RES_SUB][it], static_cast<unsigned long long>(__double_as_longlong(m)));
     }
 }
 
@@ -803,7 +803,7 @@ __global__ void k_mdf_features(const double* x, const double* y, const double* f
 __device__ __forceinline__ void mdf_begin_body(MdfDev* md, double* total, double* head, int K, int bx) {
     const int i = bx * blockDim.x + threadIdx.x;
     if (i == 0) { md->conv = 0; md->applied = 0; md->conv_it = 0; }
-    if (i < MDF_MAX_CORR + 2) md->res[i] = 0ull;
+    if (i < MDF_RES_SUB * (MDF_MAX_CORR + 2)) md->res[i / (MDF_MAX_CORR + 2)][i % (MDF_MAX_CORR + 2)] = 0ull;
     if (i < 2 * K) { total[i] = 0.0; head[i] = 0.0; }
 }
 
@@ -862,7 +862,7 @@ __global__ void k_mdf_record(MdfDev* md, const double* total, const double* head
 __device__ __forceinline__ void mdf_finish_body(MdfDev* md, int ncorr, int probe, int K, int collect,
                                                 int64_t ds_cap) {
     const int it_final = md->conv ? md->conv_it : (probe ? ncorr + 1 : ncorr);
-    const double r = __longlong_as_double(static_cast<long long>(md->res[it_final]));
+    const double r = mdf_res(md, it_final);
     md->sum_applied += md->applied;
     md->sum_final += r;
     md->max_final = (r > md->max_final || r != r) ? r : md->max_final;
@@ -1108,7 +1108,7 @@ void launch_mdf_interp(Ctx& c, int it) {
 }
 
 void launch_mdf_begin(Ctx& c) {
-    const int n = std::max(2 * c.K, MDF_MAX_CORR + 2);
+    const int n = std::max(2 * c.K, MDF_RES_SUB * (MDF_MAX_CORR + 2));
     hipLaunchKernelGGL(k_mdf_begin, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.mdf, c.mdf_total,
                        c.mdf_head, c.K);
 }
@@ -1343,7 +1343,7 @@ void batch_items_spread(Ctx& c, BatchItem* it, bool last) {
 
 // launch_mdf_begin's arguments
 void batch_items_mdf_begin(Ctx& c, BatchItem& it) {
-    const int n = std::max(2 * c.K, MDF_MAX_CORR + 2);
+    const int n = std::max(2 * c.K, MDF_RES_SUB * (MDF_MAX_CORR + 2));
     batch_put(it, (n + 255) / 256, MdfBeginPack{c.mdf, c.mdf_total, c.mdf_head, c.K});
 }
 

@@ -95,17 +95,33 @@ constexpr int MLP_HIST = 5;
 constexpr int MDF_HEAD = 2;           // plain corrections the model's output stands for
 constexpr double MDF_GAIN_MAX = 8.0;  // |gain| bound of the targets; predictions are held to [0, 4] x [-2, 2]
 
+// The residual maximum of an iteration is kept in MDF_RES_SUB sub-slots on
+// their own cache lines (workgroup b updates sub-slot b mod MDF_RES_SUB), so
+// the interpolation's workgroups do not all queue on one word; mdf_res folds
+// them.
+constexpr int MDF_RES_SUB = 8;
+constexpr int MDF_RES_LD = 48;   // iterations per sub-slot row (>= MDF_MAX_CORR + 2, 128-byte rows)
 struct MdfDev {
     int conv;                    // the residual reached tol this step
     int applied;                 // corrections spread this step
     int conv_it;                 // iteration at which it converged (0: not yet)
     int pad;
-    unsigned long long res[MDF_MAX_CORR + 2];   // max residual (double bits) per iteration
+    // max residual (double bits) per iteration and sub-slot
+    alignas(128) unsigned long long res[MDF_RES_SUB][MDF_RES_LD];
     double sum_applied;          // accumulated over steps since the last read
     double sum_final, max_final;
     double steps;
     int64_t ds_rows;             // rows written to the sample set
 };
+
+// The residual maximum of iteration it (non-negative doubles order like their
+// bit patterns, NaN above all).
+__device__ __forceinline__ double mdf_res(const MdfDev* md, int it) {
+    unsigned long long m = 0ull;
+#pragma unroll
+    for (int s = 0; s < MDF_RES_SUB; ++s) m = md->res[s][it] > m ? md->res[s][it] : m;
+    return __longlong_as_double(static_cast<long long>(m));
+}
 
 constexpr int MLP_W = 64;           // hidden width (two hidden layers)
 constexpr int MLP_P = MLP_W * MLP_IN + MLP_W + MLP_W * MLP_W + MLP_W + MLP_OUT * MLP_W + MLP_OUT;

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(MLP_WAVES * 64) void k_mlp_infer(const float* __res
                                                               const float* __restrict__ norm,
                                                               const float* feat, double* gain, int K,
                                                               const MdfDev* md, double tol) {
-    if (tol > 0.0 && __longlong_as_double(static_cast<long long>(md->res[1])) <= tol) return;
+    if (tol > 0.0 && mdf_res(md, 1) <= tol) return;
     __shared__ MlpStage s;
     __shared__ MlpWave waves[MLP_WAVES];
     mlp_stage(s, P, norm);
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(MLP_WAVES * 64) void k_mlp_predict(const float* __r
 // fluid.
 __global__ void k_mdf_smooth(const double* gain, NodeSpread* rec, const int* off, const int* owner,
                              int K, const MdfDev* md, double tol) {
-    if (tol > 0.0 && __longlong_as_double(static_cast<long long>(md->res[1])) <= tol) return;
+    if (tol > 0.0 && mdf_res(md, 1) <= tol) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     const int cap = owner[k];
