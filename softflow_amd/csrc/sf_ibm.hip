@@ -273,10 +273,21 @@ __device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void
         need = copy || dirty_out[tile];
     }
     if (mode == 0) {
+        // one list atomic per workgroup (at most 256 threads): the wavefronts'
+        // touched counts meet in LDS (mode 0 has no early exit)
+        __shared__ int s_cnt[4], s_base;
+        const int wv = static_cast<int>(threadIdx.x >> 6);
         const unsigned long long touched = __ballot(cnt > 0);
-        int base = 0;
-        if (lane == 0 && touched) base = atomicAdd(&counters[3], __popcll(touched));
-        base = __shfl(base, 0);
+        if (lane == 0) s_cnt[wv] = __popcll(touched);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int total = 0;
+            for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) total += s_cnt[w];
+            s_base = total ? atomicAdd(&counters[3], total) : 0;
+        }
+        __syncthreads();
+        int base = s_base;
+        for (int w = 0; w < wv; ++w) base += s_cnt[w];
         if (cnt > 0) {
             const int pos = base + __popcll(touched & ((1ull << lane) - 1ull));
             if (pos < ntiles) act[pos] = make_int2(tile, cnt);
