@@ -174,11 +174,11 @@ def _residual_curve(sim, kmax=6, steps=60):
     return out
 
 
-def mdf_learned_bench(steps, train_steps, epochs, curve=True):
+def mdf_learned_bench(steps, train_steps, epochs, curve=True, develop=None):
     """The learned MDF warm start against plain MDF on the MDF channel
     (MDF_CHANNEL), both held to the same no-slip residual MDF_TOL.
 
-    Protocol, the same for both schemes: MDF_DEVELOP steps of the reference's
+    Protocol, the same for both schemes: ``develop`` (default MDF_DEVELOP) steps of the reference's
     5-iteration MDF from rest, then ``steps`` timed steps.
     - plain, FP64: plain corrections until the residual r = max_k |V_k - U(X_k)|
       is <= MDF_TOL (at most MDF_MAX_CORR per step); the reference's fixed
@@ -192,16 +192,17 @@ def mdf_learned_bench(steps, train_steps, epochs, curve=True):
     steps, and the residual after k corrections for both schemes."""
     from softflow_amd import softflow_core as sc
 
+    develop = MDF_DEVELOP if develop is None else develop
     ch = MDF_CHANNEL
     out = {"workload": f"{ch['nx']}x{ch['ny']} regularized tau={ch['tau']} fx={ch['bfx']:g}, {ch['count']} Skalak "
                        f"(G_s=0.06, C={ch['mem']['C_skalak']:g}) r={ch['radius']:g} ({ch['nodes']} nodes) seeded "
                        "over the channel (2.4% area fraction)",
-           "residual_tolerance": MDF_TOL, "max_corrections": MDF_MAX_CORR, "develop_steps": MDF_DEVELOP,
+           "residual_tolerance": MDF_TOL, "max_corrections": MDF_MAX_CORR, "develop_steps": develop,
            "timed_steps": steps}
     # plain FP64: develop, the reference's fixed scheme, then residual control
     plain, placed = build_mdf_channel("fp64", 5, USE_GRAPH)
     out["capsules"] = placed
-    plain.run(MDF_DEVELOP)
+    plain.run(develop)
     plain.setMdfControl(1e-300, 4, True)        # 4 corrections, residual measured
     plain.run(6)
     plain.mdfStats()
@@ -217,7 +218,7 @@ def mdf_learned_bench(steps, train_steps, epochs, curve=True):
     # learned FP32: develop, collect rows from the plain solver, train
     t0 = time.perf_counter()
     learned, _ = build_mdf_channel("fp32", 5, USE_GRAPH)
-    learned.run(MDF_DEVELOP)
+    learned.run(develop)
     learned.setMdfControl(1e-300, 4, True)
     K = learned.capsules().totalNodes()
     learned.collectMdfSamples(train_steps * K)
@@ -506,6 +507,10 @@ def main():
                     help="skip the learned-vs-plain MDF comparison")
     ap.add_argument("--mdf-steps", type=int, default=20000)
     ap.add_argument("--mdf-train-steps", type=int, default=160)
+    ap.add_argument("--mdf-develop", type=int, default=MDF_DEVELOP,
+                    help="steps of the reference's 5-iteration MDF before the timed segments")
+    ap.add_argument("--no-mdf-curve", action="store_true",
+                    help="skip the residual-after-k-corrections curves of the MDF comparison")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=2000)
@@ -537,7 +542,8 @@ def main():
         if args.only == "ensemble":
             part = ensemble_bench(args.ensemble_steps, seq_steps=args.ensemble_sequential_steps)
         elif args.only == "mdf":
-            part = mdf_learned_bench(args.mdf_steps, args.mdf_train_steps, 6)
+            part = mdf_learned_bench(args.mdf_steps, args.mdf_train_steps, 6, not args.no_mdf_curve,
+                                     args.mdf_develop)
         elif args.only == "mdf-stability":
             part = mdf_stability_bench()
         elif args.only == "geometry":
@@ -635,7 +641,8 @@ def main():
     mdf = None
     if not args.no_mdf:
         try:
-            mdf = mdf_learned_bench(args.mdf_steps, args.mdf_train_steps, 6)
+            mdf = mdf_learned_bench(args.mdf_steps, args.mdf_train_steps, 6, not args.no_mdf_curve,
+                                    args.mdf_develop)
         except RuntimeError as e:   # reported, so the main line is still printed
             mdf = {"error": str(e)}
         print("# learned vs plain MDF: " + json.dumps(mdf), file=sys.stderr, flush=True)
