@@ -184,8 +184,8 @@ static int step_ibm(Ctx& c) {
     int shift;
     const int ncorr = mdf_corrections(c);
     const bool mdf = ncorr > 0 && c.K > 0;
-    if (mdf) launch_mdf_begin(c);
-    { Scope s(c, "spread"); launch_spread(c, 0, !mdf); }
+    launch_mdf_fold(c);   // a step staged without its closing interpolation
+    { Scope s(c, "spread"); launch_spread(c, !mdf, mdf); }
     if (ncorr > 0) {
         if (c.K > 0) {
             const bool learned = c.mdf_warm && c.model;
@@ -195,7 +195,7 @@ static int step_ibm(Ctx& c) {
                     { Scope s(c, "mdf_features"); launch_mdf_features(c); }
                     if (learned) { Scope s(c, "mlp_infer"); launch_mlp_infer(c); }
                 }
-                { Scope s(c, "mdf_spread"); launch_spread(c, 1, it == ncorr, it, ncorr); }
+                { Scope s(c, "mdf_apply"); launch_mdf_apply(c, it, ncorr); }
             }
             if (c.mdf_probe) { Scope s(c, "mdf_probe"); launch_mdf_interp(c, ncorr + 1); }
             launch_mdf_record(c, ncorr);
@@ -277,6 +277,7 @@ constexpr size_t MAX_GRAPHS = 24;
 // length, which leaves the roles rotated as the same steps run eagerly would.
 static int run_graph(Ctx& c, int nsteps) {
     const int period = 6;
+    launch_mdf_fold(c);   // never inside a captured step
     while (nsteps > 0) {
         if (c.shift_next != SHIFT_PREV ||
             (c.nsp > 0 && (c.c_from_store || (mdf_corrections(c) <= 0 && !c.adr_pending)))) {
@@ -1407,9 +1408,9 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
         int sgrid = keep_grid;
         for (const int gsz : grids) {
             c.spread_grid = gsz;
-            launch_spread(c, 0, true);   // warm
+            launch_spread(c, true);   // warm
             SF_HIP(hipEventRecord(e0, c.stream));
-            for (int r = 0; r < reps; ++r) launch_spread(c, 0, true);
+            for (int r = 0; r < reps; ++r) launch_spread(c, true);
             SF_HIP(hipEventRecord(e1, c.stream));
             SF_HIP(hipEventSynchronize(e1));
             SF_HIP(hipGetLastError());
@@ -1555,6 +1556,7 @@ int sf_mdf_stats(sf_ctx* h, double* out4) {
     if (!h || !out4) return fail(SF_E_ARG, "null pointer");
     Ctx& c = h->c;
     SF_HIP(hipSetDevice(c.device));
+    launch_mdf_fold(c);   // a staged step whose closing interpolation has not run
     SF_HIP(hipStreamSynchronize(c.stream));
     MdfDev md{};
     SF_HIP(hipMemcpy(&md, c.mdf, sizeof(MdfDev), hipMemcpyDeviceToHost));
@@ -1572,6 +1574,7 @@ int sf_mdf_collect(sf_ctx* h, int64_t capacity) {
     if (!h || capacity < 0) return fail(SF_E_ARG, "bad arguments");
     Ctx& c = h->c;
     SF_HIP(hipSetDevice(c.device));
+    launch_mdf_fold(c);
     SF_HIP(hipStreamSynchronize(c.stream));
     if (c.ds_buf) hipFree(c.ds_buf);
     c.ds_buf = nullptr;
@@ -1594,6 +1597,7 @@ int sf_mdf_samples(sf_ctx* h, int64_t* rows, float* rows_out) {
     if (!h || !rows) return fail(SF_E_ARG, "null pointer");
     Ctx& c = h->c;
     SF_HIP(hipSetDevice(c.device));
+    launch_mdf_fold(c);
     SF_HIP(hipStreamSynchronize(c.stream));
     MdfDev md{};
     SF_HIP(hipMemcpy(&md, c.mdf, sizeof(MdfDev), hipMemcpyDeviceToHost));

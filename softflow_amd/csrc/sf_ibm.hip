@@ -48,36 +48,6 @@ __device__ __forceinline__ double node_arc(const double* x, const double* y, int
     return 0.5 * (dp + dn);
 }
 
-__device__ __forceinline__ bool finite_position(double px, double py) {
-    return !(isnan(px) || isnan(py) || isinf(px) || isinf(py));
-}
-
-// Node k's spread record: position, force, arc length, stencil origin; valid
-// when the position and force are finite and the stencil rows can reach the
-// lattice.
-__device__ __forceinline__ NodeSpread node_record(double px, double py, double fx, double fy, double ds, int k,
-                                                  int ny) {
-    NodeSpread s;
-    s.px = px;
-    s.py = py;
-    s.fx = fx;
-    s.fy = fy;
-    s.ds = ds;
-    const bool pos_ok = finite_position(px, py);
-    int i0 = 0, j0 = 0;
-    bool ok = pos_ok && !(isnan(fx) || isnan(fy));
-    if (pos_ok) {
-        i0 = static_cast<int>(floor(px));
-        j0 = static_cast<int>(floor(py));
-        if (j0 < -2 || j0 >= ny + 2) ok = false;
-    }
-    s.i0 = i0;
-    s.j0 = j0;
-    s.valid = ok ? 1 : 0;
-    s.pad = k;
-    return s;
-}
-
 // The tiles a record's stencil touches, as up to 8 candidates: candidate c
 // is the (c & 3)-th distinct tile column of the stencil columns i0-1 .. i0+2
 // (each wrapped once, in column order) in the (c >> 2)-th tile row of its rows
@@ -137,22 +107,47 @@ __device__ __forceinline__ void bin_node(const BinArgs& b, const int (&t)[8], co
 }
 
 // ---------------------------------------------------------------------------
-// Node records + tile binning.  mode 0: the node's own force; mode 1: keep the
-// force already written by the MDF interpolation (only re-validate).
-// In mode 1 the MDF control decides first: once the residual of this
-// iteration is within tolerance (residual mode) nothing more is spread this
-// step -- except at iteration force_it (0: none), which is always applied;
-// otherwise the correction is counted into the node's step total, and into
-// its head total while it <= MDF_HEAD.
+// The node part of MDF correction it, one thread per node k: records the
+// iteration's decision (mdf_decide; node 0) and, when the correction is
+// applied, counts it (node 0) and adds it to the node's step total, and to its
+// head total while it <= MDF_HEAD.
+__device__ __forceinline__ void mdf_node_apply(MdfDev* md, double* total, double* head, const NodeSpread* rec,
+                                               int K, int it, double tol, int k) {
+    if (k >= K) return;
+    const int dec = mdf_decide(md, it, tol);
+    if (k == 0) md->cby[it] = dec;
+    if (dec) return;
+    if (k == 0) md->applied += 1;
+    total[2 * k] += rec[k].fx;
+    total[2 * k + 1] += rec[k].fy;
+    if (it <= MDF_HEAD) {
+        head[2 * k] += rec[k].fx;
+        head[2 * k + 1] += rec[k].fy;
+    }
+}
+
+// Node records + tile binning (mode 0: the node's own force; with md set it
+// also clears the step's MDF state -- decisions, residuals, totals -- ahead of
+// the first correction).  mode 1 is MDF correction it's node part
+// (mdf_node_apply): the MDF interpolation has already written the node's
+// record with the correction.
 // arc: the arc length elements at the current positions, or null (computed
 // here from the ring neighbours).
 __device__ __forceinline__ void node_prep_body(const double* x, const double* y, const double* fx, const double* fy,
                             const int2* ring, const double* arc, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, NodeSpread* tile_rec,
-                            int* counters, int mode, MdfDev* md, double tol, int it, int force_it,
+                            int* counters, int mode, MdfDev* md, double tol, int it,
                             double* total, double* head, int bx) {
     const int k = bx * blockDim.x + threadIdx.x;
     if (mode == 0) {
+        if (md) {
+            const int nthr = gridDim.x * blockDim.x;
+            if (k == 0) { md->conv = 0; md->applied = 0; md->conv_it = 0; }
+            for (int i = k; i < MDF_RES_LD; i += nthr) md->cby[i] = 0;
+            for (int i = k; i < MDF_RES_SUB * (MDF_MAX_CORR + 2); i += nthr)
+                md->res[i / (MDF_MAX_CORR + 2)][i % (MDF_MAX_CORR + 2)] = 0ull;
+            for (int i = k; i < 2 * K; i += nthr) { total[i] = 0.0; head[i] = 0.0; }
+        }
         // binning: every lane of the wavefront takes part in the slot claims
         if (k == 0) counters[3] = 0;     // the touched-tile list is rebuilt after binning
         int t[8];
@@ -170,35 +165,15 @@ __device__ __forceinline__ void node_prep_body(const double* x, const double* y,
         bin_node(BinArgs{rec, tile_cnt, tile_list, tile_rec, counters, nx, ny, ntx}, t, s);
         return;
     }
-    if (k >= K) return;
-    if (mode == 1 && md) {
-        if (md->conv) return;
-        if (tol > 0.0 && it != force_it &&
-            mdf_res(md, it) <= tol) {
-            md->conv = 1;          // every thread reaches the same decision
-            md->conv_it = it;
-            return;
-        }
-        if (k == 0) md->applied += 1;
-        total[2 * k] += rec[k].fx;
-        total[2 * k + 1] += rec[k].fy;
-        if (it <= MDF_HEAD) {
-            head[2 * k] += rec[k].fx;
-            head[2 * k + 1] += rec[k].fy;
-        }
-    }
-    // mode 1: keep the force and arc length; re-validate
-    const double px = x[k], py = y[k];
-    const NodeSpread r = rec[k];
-    rec[k] = node_record(px, py, r.fx, r.fy, r.ds, k, ny);
+    if (mode == 1) mdf_node_apply(md, total, head, rec, K, it, tol, k);
 }
 
 __global__ void k_node_prep(const double* x, const double* y, const double* fx, const double* fy,
                             const int2* ring, const double* arc, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, NodeSpread* tile_rec,
-                            int* counters, int mode, MdfDev* md, double tol, int it, int force_it,
+                            int* counters, int mode, MdfDev* md, double tol, int it,
                             double* total, double* head) {
-    node_prep_body(x, y, fx, fy, ring, arc, rec, K, nx, ny, ntx, nty, tile_cnt, tile_list, tile_rec, counters, mode, md, tol, it, force_it, total, head, blockIdx.x);
+    node_prep_body(x, y, fx, fy, ring, arc, rec, K, nx, ny, ntx, nty, tile_cnt, tile_list, tile_rec, counters, mode, md, tol, it, total, head, blockIdx.x);
 }
 
 // Gather: one 256-thread workgroup per TILE_X x TILE_Y tile.
@@ -247,12 +222,14 @@ __device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void
                                                      const uint8_t* dirty_in, uint8_t* dirty_out,
                                                      int2* act, int* counters, int nx, int ny,
                                                      int ntx, int ntiles, int mode,
-                                                     const MdfDev* md, int bx, int it = 0, int last_it = 0) {
+                                                     const MdfDev* md, int bx, int it = 0, int last_it = 0,
+                                                     double tol = 0.0) {
     using T = typename std::conditional<FP32, float, double>::type;
-    const bool skip = mode == 1 && md && md->conv;
+    const int dec = mode == 1 && md ? mdf_decide(md, it, tol) : 0;
+    const bool skip = dec != 0;
     if (skip && last_it > 0) {
         if (it < last_it) return;
-        if (((last_it - md->conv_it) & 1) != 0) {
+        if (((last_it - dec) & 1) != 0) {
             // the force is in this iteration's output buffer: copy it back
             const void* t = Fin;
             Fin = Fout;
@@ -344,22 +321,24 @@ __global__ __launch_bounds__(256) void k_spread_rest(const int* tile_cnt, const 
                                                      const uint8_t* dirty_in, uint8_t* dirty_out,
                                                      int2* act, int* counters, int nx, int ny,
                                                      int ntx, int ntiles, int mode,
-                                                     const MdfDev* md, int it, int last_it) {
+                                                     const MdfDev* md, int it, int last_it, double tol) {
     spread_rest_body<FP32>(tile_cnt, Fin, Fout, dirty_in, dirty_out, act, counters, nx, ny, ntx, ntiles, mode, md,
-                           blockIdx.x, it, last_it);
+                           blockIdx.x, it, last_it, tol);
 }
 
 // Touched tiles: a persistent grid walks the list of tiles the node binning
 // touched (act[0 .. counters[3])), one 256-thread workgroup per tile and one
 // thread per cell of its TILE_X x TILE_Y cells.
-// mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction).
+// mode 0: F = sum;  mode 1: F_out = F_in + sum  (MDF correction it, unless
+// the loop has stopped, mdf_decide).  last: the tile counts are cleared.
 template <bool FP32>
 __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpread* tile_rec,
                                                 int* tile_cnt, const int* tile_list, const int2* act,
                                                 const int* counters, const double* lut,
                                                 const void* Fin, void* Fout, uint8_t* dirty_out,
                                                 int nx, int ny, int ntx, int ntiles, int mode,
-                                                int last, const MdfDev* md, int bx, int gx) {
+                                                int last, const MdfDev* md, int bx, int gx, int it = 0,
+                                                double tol = 0.0) {
     // the delta table is read through the caches: staged in LDS it held the
     // spread to 7 workgroups per CU, and each tile is short, latency-bound work
     const double* tab = lut;
@@ -377,7 +356,7 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
     // (the list holds ntiles entries; those past the count are never used)
     int2 te_next = act[bx < ntiles ? bx : 0];
     const int nact = min(counters[3], ntiles);
-    const bool skip = mode == 1 && md && md->conv;
+    const bool skip = mode == 1 && md && mdf_decide(md, it, tol) != 0;
     if (skip) {
         // converged MDF: nothing to add (the rest kernel copied F_in)
         if (last)
@@ -567,6 +546,30 @@ __global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const Nod
     spread_body<FP32>(rec, tile_rec, tile_cnt, tile_list, act, counters, lut, Fin, Fout, dirty_out, nx, ny, ntx, ntiles, mode, last, md, blockIdx.x, gridDim.x);
 }
 
+// MDF correction it in one launch: workgroups [0, nrest) the untouched tiles
+// (spread_rest_body), [nrest, nrest + nspread) the touched ones (spread_body),
+// the rest the node part (mdf_node_apply, one node per thread).  The three
+// parts write disjoint data and each takes the iteration's decision itself.
+// The tile counts stay (the rest part reads them): the statistics fold clears
+// them.
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_mdf_apply(const NodeSpread* rec, int* tile_cnt, const int* tile_list,
+                                                   int2* act, int* counters, const double* lut, const void* Fin,
+                                                   void* Fout, const uint8_t* dirty_in, uint8_t* dirty_out,
+                                                   int nx, int ny, int ntx, int ntiles, MdfDev* md, double tol,
+                                                   int it, int last_it, double* total, double* head, int K,
+                                                   int nrest, int nspread) {
+    const int b = blockIdx.x;
+    if (b < nrest)
+        spread_rest_body<FP32>(tile_cnt, Fin, Fout, dirty_in, dirty_out, act, counters, nx, ny, ntx, ntiles, 1, md,
+                               b, it, last_it, tol);
+    else if (b < nrest + nspread)
+        spread_body<FP32>(rec, nullptr, tile_cnt, tile_list, act, counters, lut, Fin, Fout, dirty_out, nx, ny, ntx,
+                          ntiles, 1, 0, md, b - nrest, nspread, it, tol);
+    else
+        mdf_node_apply(md, total, head, rec, K, it, tol, (b - nrest - nspread) * blockDim.x + threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Velocity at a stencil cell from the populations and a force field.
 // force_mode 0: raw force (the MDF's computeMacroscopic on F so far);
@@ -708,7 +711,7 @@ __device__ __forceinline__ void mdf_interp_body(const CellVelArgs& a, const doub
     __shared__ double tx[16 * NPW], ty[16 * NPW];
     __shared__ int use[16 * NPW];
     __shared__ double sres[NPW];
-    if (md->conv) return;       // uniform: no kernel writes it while this one runs
+    if (it > 1 && md->cby[it - 1]) return;   // the loop stopped at an earlier iteration
     const int g = threadIdx.x / LPN, l = threadIdx.x % LPN;
     const int k = bx * NPW + g;
     const bool live = k < K;
@@ -730,8 +733,8 @@ __device__ __forceinline__ void mdf_interp_body(const CellVelArgs& a, const doub
                 rl = r;
             }
             const double two = 2.0 * rl;
-            rec[k].fx = dux * two;
-            rec[k].fy = duy * two;
+            // the whole record, valid where the correction is finite
+            rec[k] = node_record(px, py, dux * two, duy * two, rec[k].ds, k, a.ny);
             if (u0) { u0[2 * k] = dux; u0[2 * k + 1] = duy; }
             res = sqrt(dux * dux + duy * duy);
         }
@@ -810,17 +813,16 @@ __global__ void k_mdf_features(const double* x, const double* y, const double* f
     }
 }
 
-// Clears the per-step MDF state (graph-safe: a kernel, not a memset).
+// Clears the per-step MDF state of a batch member (graph-safe: a kernel, not a
+// memset).
 __device__ __forceinline__ void mdf_begin_body(MdfDev* md, double* total, double* head, int K, int bx) {
     const int i = bx * blockDim.x + threadIdx.x;
     if (i == 0) { md->conv = 0; md->applied = 0; md->conv_it = 0; }
+    if (i < MDF_RES_LD) md->cby[i] = 0;
     if (i < MDF_RES_SUB * (MDF_MAX_CORR + 2)) md->res[i / (MDF_MAX_CORR + 2)][i % (MDF_MAX_CORR + 2)] = 0ull;
     if (i < 2 * K) { total[i] = 0.0; head[i] = 0.0; }
 }
 
-__global__ void k_mdf_begin(MdfDev* md, double* total, double* head, int K) {
-    mdf_begin_body(md, total, head, K, blockIdx.x);
-}
 
 // The model's targets: the sum T of the step's first MDF_HEAD plain
 // corrections in the frame of the first one, dF1: T = g_par dF1 +
@@ -872,7 +874,10 @@ __global__ void k_mdf_record(MdfDev* md, const double* total, const double* head
 
 __device__ __forceinline__ void mdf_finish_body(MdfDev* md, int ncorr, int probe, int K, int collect,
                                                 int64_t ds_cap) {
-    const int it_final = md->conv ? md->conv_it : (probe ? ncorr + 1 : ncorr);
+    const int dec = md->cby[ncorr];
+    md->conv = dec != 0;
+    md->conv_it = dec;
+    const int it_final = dec ? dec : (probe ? ncorr + 1 : ncorr);
     const double r = mdf_res(md, it_final);
     md->sum_applied += md->applied;
     md->sum_final += r;
@@ -881,8 +886,13 @@ __device__ __forceinline__ void mdf_finish_body(MdfDev* md, int ncorr, int probe
     if (collect && md->ds_rows + K <= ds_cap) md->ds_rows += K;
 }
 
-__global__ void k_mdf_finish(MdfDev* md, int ncorr, int probe, int K, int collect, int64_t ds_cap) {
-    mdf_finish_body(md, ncorr, probe, K, collect, ds_cap);
+// The statistics fold (one thread) and the tile counts of the step cleared
+// (every thread), which the MDF corrections leave set.
+__global__ void k_mdf_finish(MdfDev* md, int ncorr, int probe, int K, int collect, int64_t ds_cap, int* tile_cnt,
+                             int ntiles) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) mdf_finish_body(md, ncorr, probe, K, collect, ds_cap);
+    if (t < ntiles) tile_cnt[t] = 0;
 }
 
 // interpolateVelocity (immersed_boundary.cpp:174-189) with the post-step
@@ -937,10 +947,28 @@ __device__ __forceinline__ void interp_body(CellVelArgs a, const double* lut, do
     }
 }
 
+// The step's MDF statistics fold (mdf_finish_body) and the tile counts the
+// corrections leave set, cleared; done by the interpolation that ends a step
+// with MDF (md null: nothing).
+struct MdfFold {
+    MdfDev* md;
+    int ncorr, probe, K, collect;
+    int64_t ds_cap;
+    int* tile_cnt;
+    int ntiles;
+};
+
+__device__ __forceinline__ void mdf_fold(const MdfFold& f) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
+    for (int t = gid; t < f.ntiles; t += nthr) f.tile_cnt[t] = 0;
+    if (gid == 0) mdf_finish_body(f.md, f.ncorr, f.probe, f.K, f.collect, f.ds_cap);
+}
+
 template <bool FP32, int LPN>
 __global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut, double* x,
                                                 double* y, double* vx, double* vy, int K, MoveArgs mv,
-                                                int band) {
+                                                int band, MdfFold fold) {
+    if (fold.md) mdf_fold(fold);
     const int bx = band_block(blockIdx.x, band);
     if (bx * interp_nodes(LPN) >= K) return;       // padding workgroup of the banded grid
     interp_body<FP32, LPN>(a, lut, x, y, vx, vy, K, mv, bx);
@@ -1026,11 +1054,12 @@ static int interp_band(int K, int lpn) {
 
 // k_interp at lpn lanes per node (an interpolation with mv.move = 0 only
 // writes the velocities)
-static void launch_interp_lpn(Ctx& c, int lpn, const CellVelArgs& a, double* vx, double* vy, const MoveArgs& mv) {
+static void launch_interp_lpn(Ctx& c, int lpn, const CellVelArgs& a, double* vx, double* vy, const MoveArgs& mv,
+                              const MdfFold& fold) {
     const int band = interp_band(c.K, lpn);
     const dim3 grid(8 * band);
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, vx, vy, c.K, mv, band);
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, vx, vy, c.K, mv, band, fold);
     };
     if (c.fp32) {
         if (lpn == 4) go(k_interp<true, 4>);
@@ -1043,55 +1072,62 @@ static void launch_interp_lpn(Ctx& c, int lpn, const CellVelArgs& a, double* vx,
     }
 }
 
-// mode 0: bin nodes and spread node forces into F[Fcur] (overwrite).
-// mode 1: spread the MDF corrections on top of F[Fcur] into the spare buffer
-//         and make it current; the previous contents become F[Fpart].
-void launch_spread(Ctx& c, int mode, bool last, int it, int last_it) {
+// Bin the nodes and spread their forces into F[Fcur] (overwrite); last: the
+// tile counts are cleared after (a step without MDF corrections); mdf: the
+// binning also clears the step's MDF state.
+void launch_spread(Ctx& c, bool last, bool mdf) {
     if (c.K == 0) {
-        if (mode == 0) launch_fill_zero(c, c.F[c.Fcur], (c.fp32 ? 8 : 16) * static_cast<size_t>(c.N));
+        launch_fill_zero(c, c.F[c.Fcur], (c.fp32 ? 8 : 16) * static_cast<size_t>(c.N));
         return;
     }
-    if (mode == 0) {
-        // the membrane kernel left the arc lengths of the current positions
-        const double* arc = c.arc_fresh ? c.arc : nullptr;
-        hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
-                           c.fx, c.fy, c.ring, arc, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                           c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 0, nullptr, 0.0, 0, 0,
-                           nullptr, nullptr);
+    // the membrane kernel left the arc lengths of the current positions
+    const double* arc = c.arc_fresh ? c.arc : nullptr;
+    hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
+                       c.fx, c.fy, c.ring, arc, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
+                       c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 0, mdf ? c.mdf : nullptr, 0.0, 0,
+                       mdf ? c.mdf_total : nullptr, mdf ? c.mdf_head : nullptr);
+    void* F = c.F[c.Fcur];
+    uint8_t* d = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
+    const dim3 rgrid((c.ntiles + rest_tiles_per_block(0) - 1) / rest_tiles_per_block(0));
+    const dim3 sgrid(c.spread_grid > 0 ? std::min(c.ntiles, c.spread_grid) : c.ntiles);
+    if (c.fp32) {
+        hipLaunchKernelGGL((k_spread_rest<true>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, F, F, d, d,
+                           c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, 0, nullptr, 0, 0, 0.0);
+        hipLaunchKernelGGL((k_spread<true>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
+                           c.tile_list, c.tile_act, c.counters, c.phi, F, F, d, c.nx, c.ny, c.ntx, c.ntiles, 0,
+                           last ? 1 : 0, nullptr);
     } else {
-        hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
-                           c.fx, c.fy, c.ring, nullptr, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
-                           c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 1, c.mdf, c.mdf_tol, it,
-                           0, c.mdf_total, c.mdf_head);
+        hipLaunchKernelGGL((k_spread_rest<false>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, F, F, d, d,
+                           c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, 0, nullptr, 0, 0, 0.0);
+        hipLaunchKernelGGL((k_spread<false>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
+                           c.tile_list, c.tile_act, c.counters, c.phi, F, F, d, c.nx, c.ny, c.ntx, c.ntiles, 0,
+                           last ? 1 : 0, nullptr);
     }
+}
+
+// MDF correction it of last_it: the correction on top of F[Fcur] into the
+// buffer that is neither current nor previous, which becomes current; the
+// previous contents become F[Fpart].
+void launch_mdf_apply(Ctx& c, int it, int last_it) {
     const void* Fin = c.F[c.Fcur];
-    // mode 1 writes into the buffer that is neither current nor previous
-    const int out = mode == 1 ? 3 - c.Fcur - c.Fprev : c.Fcur;
+    const int out = 3 - c.Fcur - c.Fprev;
     void* Fout = c.F[out];
     const uint8_t* din = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
     uint8_t* dout = c.tile_dirty + static_cast<size_t>(out) * c.ntiles;
-    const MdfDev* md = mode == 1 ? c.mdf : nullptr;
-    const dim3 rgrid((c.ntiles + rest_tiles_per_block(mode) - 1) / rest_tiles_per_block(mode));
-    const dim3 sgrid(c.spread_grid > 0 ? std::min(c.ntiles, c.spread_grid) : c.ntiles);
-    if (c.fp32) {
-        hipLaunchKernelGGL((k_spread_rest<true>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
-                           din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md,
-                           it, last_it);
-        hipLaunchKernelGGL((k_spread<true>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
-                           c.tile_list, c.tile_act, c.counters, c.phi, Fin, Fout, dout, c.nx, c.ny,
-                           c.ntx, c.ntiles, mode, last ? 1 : 0, md);
-    } else {
-        hipLaunchKernelGGL((k_spread_rest<false>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, Fin, Fout,
-                           din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, mode, md,
-                           it, last_it);
-        hipLaunchKernelGGL((k_spread<false>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
-                           c.tile_list, c.tile_act, c.counters, c.phi, Fin, Fout, dout, c.nx, c.ny,
-                           c.ntx, c.ntiles, mode, last ? 1 : 0, md);
-    }
-    if (mode == 1) {
-        c.Fpart = c.Fcur;
-        c.Fcur = out;
-    }
+    const int nrest = (c.ntiles + rest_tiles_per_block(1) - 1) / rest_tiles_per_block(1);
+    const int nspread = c.spread_grid > 0 ? std::min(c.ntiles, c.spread_grid) : c.ntiles;
+    const int nnode = (c.K + 255) / 256;
+    const dim3 g(nrest + nspread + nnode);
+    if (c.fp32)
+        hipLaunchKernelGGL((k_mdf_apply<true>), g, dim3(256), 0, c.stream, c.spread, c.tile_cnt, c.tile_list,
+                           c.tile_act, c.counters, c.phi, Fin, Fout, din, dout, c.nx, c.ny, c.ntx, c.ntiles, c.mdf,
+                           c.mdf_tol, it, last_it, c.mdf_total, c.mdf_head, c.K, nrest, nspread);
+    else
+        hipLaunchKernelGGL((k_mdf_apply<false>), g, dim3(256), 0, c.stream, c.spread, c.tile_cnt, c.tile_list,
+                           c.tile_act, c.counters, c.phi, Fin, Fout, din, dout, c.nx, c.ny, c.ntx, c.ntiles, c.mdf,
+                           c.mdf_tol, it, last_it, c.mdf_total, c.mdf_head, c.K, nrest, nspread);
+    c.Fpart = c.Fcur;
+    c.Fcur = out;
 }
 
 // MDF iteration it (1-based; MDF_MAX_CORR + 1 at most, the last being the
@@ -1118,11 +1154,6 @@ void launch_mdf_interp(Ctx& c, int it) {
     }
 }
 
-void launch_mdf_begin(Ctx& c) {
-    const int n = std::max(2 * c.K, MDF_RES_SUB * (MDF_MAX_CORR + 2));
-    hipLaunchKernelGGL(k_mdf_begin, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.mdf, c.mdf_total,
-                       c.mdf_head, c.K);
-}
 
 void launch_mdf_features(Ctx& c) {
     hipLaunchKernelGGL(k_mdf_features, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
@@ -1139,8 +1170,18 @@ void launch_mdf_record(Ctx& c, int ncorr) {
                            c.mdf_total, c.mdf_head, c.mdf_u0, c.mdf_f1, c.mdf_hist, c.mdf_feat, c.ds_buf,
                            c.ds_cap, c.K,
                            c.mdf_hist_on);
-    hipLaunchKernelGGL(k_mdf_finish, dim3(1), dim3(1), 0, c.stream, c.mdf, ncorr, c.mdf_probe, c.K,
-                       collect ? 1 : 0, c.ds_cap);
+    // the statistics fold and the tile counts ride on the step's closing
+    // interpolation (launch_interp_move), or launch_mdf_fold when none follows
+    c.mdf_fold_pending = true;
+    c.mdf_fold_ncorr = ncorr;
+}
+
+void launch_mdf_fold(Ctx& c) {
+    if (!c.mdf_fold_pending) return;
+    hipLaunchKernelGGL(k_mdf_finish, dim3((c.ntiles + 255) / 256), dim3(256), 0, c.stream, c.mdf,
+                       c.mdf_fold_ncorr, c.mdf_probe, c.K, c.ds_buf != nullptr ? 1 : 0, c.ds_cap, c.tile_cnt,
+                       c.ntiles);
+    c.mdf_fold_pending = false;
 }
 
 // Capsule recirculation in x over a non-periodic fluid (Simulation::step 10a,
@@ -1165,7 +1206,7 @@ __global__ void k_recirculate_x(double* x, const int* off, int ncaps, double Lx)
 void launch_interp_probe(Ctx& c, int lpn, double* vx, double* vy) {
     if (c.K == 0) return;
     const MoveArgs mv{0, c.p.dt, 0, 0, 0.0, 0.0};
-    launch_interp_lpn(c, lpn, vel_args(c, c.f[c.fcur], c.F[c.Fcur]), vx, vy, mv);
+    launch_interp_lpn(c, lpn, vel_args(c, c.f[c.fcur], c.F[c.Fcur]), vx, vy, mv, MdfFold{});
 }
 
 void launch_interp_move(Ctx& c) {
@@ -1174,7 +1215,13 @@ void launch_interp_move(Ctx& c) {
     const CellVelArgs a = vel_args(c, c.f[c.fcur], c.F[c.Fcur]);
     const MoveArgs mv{c.any_rigid ? 0 : 1, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
                       1.5, static_cast<double>(c.ny) - 1.5};
-    launch_interp_lpn(c, c.interp_lpn, a, c.vx, c.vy, mv);
+    MdfFold fold{};
+    if (c.mdf_fold_pending) {
+        fold = MdfFold{c.mdf, c.mdf_fold_ncorr, c.mdf_probe, c.K, c.ds_buf != nullptr ? 1 : 0, c.ds_cap,
+                       c.tile_cnt, c.ntiles};
+        c.mdf_fold_pending = false;
+    }
+    launch_interp_lpn(c, c.interp_lpn, a, c.vx, c.vy, mv, fold);
     if (c.any_rigid)
         hipLaunchKernelGGL(k_move, dim3(c.ncaps), dim3(64), 0, c.stream, c.x, c.y, c.vx, c.vy, c.off,
                            c.desc, c.ncaps, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
@@ -1217,6 +1264,7 @@ struct RestPack {
     int mode;
     const MdfDev* md;
     int it, last_it;
+    double tol;
 };
 struct SpreadPack {
     const NodeSpread* rec;
@@ -1232,6 +1280,8 @@ struct SpreadPack {
     int nx, ny, ntx, ntiles, gx;
     int mode, last;
     const MdfDev* md;
+    int it;
+    double tol;
 };
 struct MdfBeginPack {
     MdfDev* md;
@@ -1250,6 +1300,8 @@ struct MdfInterpPack {
 struct MdfFinishPack {
     MdfDev* md;
     int ncorr, probe, K;
+    int* tile_cnt;
+    int ntiles;
 };
 
 size_t batch_pack_size(int kind) {
@@ -1276,7 +1328,7 @@ __global__ __launch_bounds__(256) void kb_node_prep(const PrepPack* packs, const
     if (w.x < 0) return;
     const PrepPack p = packs[w.x];
     node_prep_body(p.x, p.y, p.fx, p.fy, p.ring, nullptr, p.rec, p.K, p.nx, p.ny, p.ntx, p.nty,
-                   p.tile_cnt, p.tile_list, p.tile_rec, p.counters, p.mode, p.md, p.tol, p.it, 0, p.total,
+                   p.tile_cnt, p.tile_list, p.tile_rec, p.counters, p.mode, p.md, p.tol, p.it, p.total,
                    p.head, w.y);
 }
 
@@ -1286,7 +1338,7 @@ __global__ __launch_bounds__(256) void kb_spread_rest(const RestPack* packs, con
     if (w.x < 0) return;
     const RestPack p = packs[w.x];
     spread_rest_body<FP32>(p.tile_cnt, p.Fin, p.Fout, p.din, p.dout, p.act, p.counters, p.nx, p.ny,
-                           p.ntx, p.ntiles, p.mode, p.md, w.y, p.it, p.last_it);
+                           p.ntx, p.ntiles, p.mode, p.md, w.y, p.it, p.last_it, p.tol);
 }
 
 template <bool FP32>
@@ -1295,7 +1347,7 @@ __global__ __launch_bounds__(256) void kb_spread(const SpreadPack* packs, const 
     if (w.x < 0) return;
     const SpreadPack p = packs[w.x];
     spread_body<FP32>(p.rec, p.tile_rec, p.tile_cnt, p.tile_list, p.act, p.counters, p.lut, p.Fin,
-                      p.Fout, p.dout, p.nx, p.ny, p.ntx, p.ntiles, p.mode, p.last, p.md, w.y, p.gx);
+                      p.Fout, p.dout, p.nx, p.ny, p.ntx, p.ntiles, p.mode, p.last, p.md, w.y, p.gx, p.it, p.tol);
 }
 
 __global__ __launch_bounds__(256) void kb_mdf_begin(const MdfBeginPack* packs, const int2* map) {
@@ -1314,11 +1366,13 @@ __global__ __launch_bounds__(256) void kb_mdf_interp(const MdfInterpPack* packs,
 }
 
 // one workgroup per member; its first thread folds the step's MDF statistics
+// and all its threads clear the member's tile counts
 __global__ __launch_bounds__(256) void kb_mdf_finish(const MdfFinishPack* packs, const int2* map) {
     const int2 w = map[blockIdx.x];
-    if (w.x < 0 || threadIdx.x != 0) return;
+    if (w.x < 0) return;
     const MdfFinishPack p = packs[w.x];
-    mdf_finish_body(p.md, p.ncorr, p.probe, p.K, 0, 0);
+    if (threadIdx.x == 0) mdf_finish_body(p.md, p.ncorr, p.probe, p.K, 0, 0);
+    for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) p.tile_cnt[t] = 0;
 }
 
 template <bool FP32>
@@ -1345,14 +1399,15 @@ void batch_items_spread(Ctx& c, BatchItem* it, bool last) {
     uint8_t* dout = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
     batch_put(it[1], (c.ntiles + rest_tiles_per_block(0) - 1) / rest_tiles_per_block(0),
               RestPack{c.tile_cnt, Fin, Fout, din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx,
-                       c.ntiles, 0, nullptr, 0, 0});
+                       c.ntiles, 0, nullptr, 0, 0, 0.0});
     const int gx = std::min(c.ntiles, BATCH_SPREAD_GRID);
     batch_put(it[2], gx,
               SpreadPack{c.spread, c.tile_rec, c.tile_cnt, c.tile_list, c.tile_act, c.counters, c.phi,
-                         Fin, Fout, dout, c.nx, c.ny, c.ntx, c.ntiles, gx, 0, last ? 1 : 0, nullptr});
+                         Fin, Fout, dout, c.nx, c.ny, c.ntx, c.ntiles, gx, 0, last ? 1 : 0, nullptr, 0, 0.0});
 }
 
-// launch_mdf_begin's arguments
+// the clear of a member's MDF state at the start of its step (a single
+// context clears it in its binning pass)
 void batch_items_mdf_begin(Ctx& c, BatchItem& it) {
     const int n = std::max(2 * c.K, MDF_RES_SUB * (MDF_MAX_CORR + 2));
     batch_put(it, (n + 255) / 256, MdfBeginPack{c.mdf, c.mdf_total, c.mdf_head, c.K});
@@ -1365,8 +1420,9 @@ void batch_items_mdf_interp(Ctx& c, int i, BatchItem& it) {
                             c.K, c.mdf, i});
 }
 
-// MDF correction i of ncorr: launch_mdf_interp(c, i) then launch_spread(c, 1,
-// i == ncorr, i, ncorr); rotates the force roles as launch_spread does
+// MDF correction i of ncorr as four launches: the interpolation, the node part
+// (node_prep mode 1), the untouched tiles and the touched tiles -- the parts
+// launch_mdf_apply runs as one; rotates the force roles as it does
 void batch_items_mdf_correction(Ctx& c, int i, int ncorr, BatchItem* it) {
     batch_items_mdf_interp(c, i, it[0]);
     batch_put(it[1], (c.K + 255) / 256,
@@ -1380,18 +1436,18 @@ void batch_items_mdf_correction(Ctx& c, int i, int ncorr, BatchItem* it) {
     uint8_t* dout = c.tile_dirty + static_cast<size_t>(out) * c.ntiles;
     batch_put(it[2], (c.ntiles + rest_tiles_per_block(1) - 1) / rest_tiles_per_block(1),
               RestPack{c.tile_cnt, Fin, Fout, din, dout, c.tile_act, c.counters, c.nx, c.ny, c.ntx,
-                       c.ntiles, 1, c.mdf, i, ncorr});
+                       c.ntiles, 1, c.mdf, i, ncorr, c.mdf_tol});
     const int gx = std::min(c.ntiles, BATCH_SPREAD_GRID);
     batch_put(it[3], gx,
               SpreadPack{c.spread, c.tile_rec, c.tile_cnt, c.tile_list, c.tile_act, c.counters, c.phi,
-                         Fin, Fout, dout, c.nx, c.ny, c.ntx, c.ntiles, gx, 1, i == ncorr ? 1 : 0, c.mdf});
+                         Fin, Fout, dout, c.nx, c.ny, c.ntx, c.ntiles, gx, 1, 0, c.mdf, i, c.mdf_tol});
     c.Fpart = c.Fcur;
     c.Fcur = out;
 }
 
 // launch_mdf_record's statistics (no model, no sample collection)
 void batch_items_mdf_finish(Ctx& c, int ncorr, BatchItem& it) {
-    batch_put(it, 1, MdfFinishPack{c.mdf, ncorr, c.mdf_probe, c.K});
+    batch_put(it, 1, MdfFinishPack{c.mdf, ncorr, c.mdf_probe, c.K, c.tile_cnt, c.ntiles});
 }
 
 // a slot this member takes no part in: no workgroups, a pack of the slot's size

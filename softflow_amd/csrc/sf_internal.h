@@ -46,6 +46,36 @@ struct NodeSpread {
     int i0, j0, valid, pad;
 };
 
+__device__ __forceinline__ bool finite_position(double px, double py) {
+    return !(isnan(px) || isnan(py) || isinf(px) || isinf(py));
+}
+
+// Node k's spread record: position, force, arc length, stencil origin; valid
+// when the position and force are finite and the stencil rows can reach the
+// lattice.
+__device__ __forceinline__ NodeSpread node_record(double px, double py, double fx, double fy, double ds, int k,
+                                                  int ny) {
+    NodeSpread s;
+    s.px = px;
+    s.py = py;
+    s.fx = fx;
+    s.fy = fy;
+    s.ds = ds;
+    const bool pos_ok = finite_position(px, py);
+    int i0 = 0, j0 = 0;
+    bool ok = pos_ok && !(isnan(fx) || isnan(fy));
+    if (pos_ok) {
+        i0 = static_cast<int>(floor(px));
+        j0 = static_cast<int>(floor(py));
+        if (j0 < -2 || j0 >= ny + 2) ok = false;
+    }
+    s.i0 = i0;
+    s.j0 = j0;
+    s.valid = ok ? 1 : 0;
+    s.pad = k;
+    return s;
+}
+
 // Slot claims aggregated over a wavefront (1-D workgroups of whole
 // wavefronts; every lane of the wavefront takes part).  Each lane with a key
 // >= 0 claims one slot of the counter cnt[key].  A run of consecutive lanes
@@ -102,10 +132,15 @@ constexpr double MDF_GAIN_MAX = 8.0;  // |gain| bound of the targets; prediction
 constexpr int MDF_RES_SUB = 8;
 constexpr int MDF_RES_LD = 48;   // iterations per sub-slot row (>= MDF_MAX_CORR + 2, 128-byte rows)
 struct MdfDev {
-    int conv;                    // the residual reached tol this step
+    int conv;                    // the residual reached tol this step (set by the statistics fold)
     int applied;                 // corrections spread this step
-    int conv_it;                 // iteration at which it converged (0: not yet)
+    int conv_it;                 // iteration at which it converged (0: not)
     int pad;
+    // the decision taken after iteration it (mdf_decide): the iteration at
+    // which the loop stopped, 0 while it runs on.  Slot it is written while
+    // iteration it is decided and read by the launches of later iterations,
+    // so no launch reads a slot it writes.
+    int cby[MDF_RES_LD];
     // max residual (double bits) per iteration and sub-slot
     alignas(128) unsigned long long res[MDF_RES_SUB][MDF_RES_LD];
     double sum_applied;          // accumulated over steps since the last read
@@ -121,6 +156,19 @@ __device__ __forceinline__ double mdf_res(const MdfDev* md, int it) {
 #pragma unroll
     for (int s = 0; s < MDF_RES_SUB; ++s) m = md->res[s][it] > m ? md->res[s][it] : m;
     return __longlong_as_double(static_cast<long long>(m));
+}
+
+// The decision after iteration it (1-based), as every launch of that
+// iteration takes it: the iteration at which the loop stopped -- carried
+// forward from it - 1, or it itself when its residual is within tolerance
+// (residual mode) -- or 0 when correction it is applied.  Each launch computes
+// it from slot it - 1 and the residuals of iteration it, which earlier
+// launches completed, so launches of one iteration agree without waiting on
+// each other.
+__device__ __forceinline__ int mdf_decide(const MdfDev* md, int it, double tol) {
+    const int prev = md->cby[it - 1];
+    if (prev) return prev;
+    return (tol > 0.0 && mdf_res(md, it) <= tol) ? it : 0;
 }
 
 constexpr int MLP_W = 64;           // hidden width (two hidden layers)
@@ -269,6 +317,10 @@ struct Ctx {
     float* mdf_hist = nullptr;           // MLP_HIST x 2K: slips of the previous steps
     float* mdf_feat = nullptr;           // K x MLP_IN
     int mdf_hist_on = 0;                 // history is being recorded
+    // the step's MDF statistics fold and tile-count clear wait for its
+    // closing interpolation
+    bool mdf_fold_pending = false;
+    int mdf_fold_ncorr = 0;
     float* ds_buf = nullptr;             // sample rows (MLP_IN + MLP_OUT floats)
     int64_t ds_cap = 0;
     Mlp* model = nullptr;                // borrowed from the caller
@@ -350,12 +402,19 @@ void launch_repulsion(Ctx& c);
 bool contact_uses_cells(const Ctx& c);
 void launch_contact_fill(Ctx& c);
 void launch_membrane_sort(Ctx& c);
-// mode 0: node forces into F[Fcur]; mode 1: MDF correction it of last_it
-void launch_spread(Ctx& c, int mode, bool last, int it = 0, int last_it = 0);
+// node binning and the spread of the node forces into F[Fcur] (last: no MDF
+// corrections follow; mdf: the binning also clears the step's MDF state)
+void launch_spread(Ctx& c, bool last, bool mdf = false);
 void launch_mdf_interp(Ctx& c, int it);
-void launch_mdf_begin(Ctx& c);
 void launch_mdf_features(Ctx& c);
+// history and sample rows; leaves the statistics fold pending for the step's
+// closing interpolation (launch_mdf_fold runs it on its own)
 void launch_mdf_record(Ctx& c, int ncorr);
+void launch_mdf_fold(Ctx& c);
+// MDF correction it of last_it after its interpolation: the decision, the
+// correction spread (touched tiles) and the untouched tiles' copies in one
+// launch; rotates the force roles as launch_spread mode 1 does
+void launch_mdf_apply(Ctx& c, int it, int last_it);
 void launch_mlp_infer(Ctx& c);
 void launch_interp_move(Ctx& c);
 // The step's interpolation at lpn lanes per node without the move, writing

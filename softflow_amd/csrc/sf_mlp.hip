@@ -240,7 +240,7 @@ __global__ __launch_bounds__(MLP_WAVES * 64) void k_mlp_predict(const float* __r
 // interpolated velocity, so the residual cannot see it, yet it pushes the
 // fluid.
 __global__ void k_mdf_smooth(const double* gain, NodeSpread* rec, const int* off, const int* owner,
-                             int K, const MdfDev* md, double tol) {
+                             const double* x, const double* y, int K, int ny, const MdfDev* md, double tol) {
     if (tol > 0.0 && mdf_res(md, 1) <= tol) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
@@ -256,9 +256,9 @@ __global__ void k_mdf_smooth(const double* gain, NodeSpread* rec, const int* off
         gp *= 0.0625;
         gq *= 0.0625;
     }
-    const double ax = rec[k].fx, ay = rec[k].fy;    // dF1
-    rec[k].fx = gp * ax - gq * ay;
-    rec[k].fy = gp * ay + gq * ax;
+    const NodeSpread r = rec[k];    // dF1
+    // the whole record, valid where the correction is finite
+    rec[k] = node_record(x[k], y[k], gp * r.fx - gq * r.fy, gp * r.fy + gq * r.fx, r.ds, k, ny);
 }
 
 // Forward + backward of GB samples; the workgroup's gradient of
@@ -433,7 +433,7 @@ void launch_mlp_infer(Ctx& c) {
                        c.stream, c.model->params, c.model->norm, c.mdf_feat, c.mdf_pred, c.K, c.mdf,
                        c.mdf_tol);
     hipLaunchKernelGGL(k_mdf_smooth, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.mdf_pred,
-                       c.spread, c.off, c.owner, c.K, c.mdf, c.mdf_tol);
+                       c.spread, c.off, c.owner, c.x, c.y, c.K, c.ny, c.mdf, c.mdf_tol);
 }
 
 static int ensure_partial(Mlp& m, int nblk) {
@@ -661,6 +661,7 @@ int sf_mlp_train(sf_mlp* h, sf_ctx* ctx, int epochs, int batch, float lr, uint64
     if (c.device != m.device) return fail(SF_E_ARG, "network and context are on different devices");
     if (!c.ds_buf) return fail(SF_E_STATE, "no sample set (call sf_mdf_collect and step first)");
     SF_HIP(hipSetDevice(m.device));
+    launch_mdf_fold(c);
     SF_HIP(hipStreamSynchronize(c.stream));
     MdfDev md{};
     SF_HIP(hipMemcpy(&md, c.mdf, sizeof(MdfDev), hipMemcpyDeviceToHost));
