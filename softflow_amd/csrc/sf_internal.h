@@ -174,6 +174,7 @@ __device__ __forceinline__ int mdf_decide(const MdfDev* md, int it, double tol) 
 constexpr int MLP_W = 64;           // hidden width (two hidden layers)
 constexpr int MLP_P = MLP_W * MLP_IN + MLP_W + MLP_W * MLP_W + MLP_W + MLP_OUT * MLP_W + MLP_OUT;
 constexpr int MLP_NORM = 2 * MLP_IN + 2 * MLP_OUT;   // in_mean, in_scale, out_mean, out_scale
+constexpr int MLP_PACKED = 160 * 64;                 // lane-major copy of the weights for inference
 
 // The warm-start network: x -> GELU(W1 x^ + b1) -> GELU(W2 . + b2) -> W3 . + b3,
 // with x^ = (x - in_mean) / in_scale and y = out_mean + out_scale * out.
@@ -183,6 +184,7 @@ struct Mlp {
     hipStream_t stream = nullptr;
     float* params = nullptr;             // MLP_P
     float* norm = nullptr;               // MLP_NORM
+    float* packed = nullptr;             // MLP_PACKED: params + norm in inference order, kept current
     float* adam_m = nullptr;             // MLP_P
     float* adam_v = nullptr;             // MLP_P
     int64_t adam_t = 0;

@@ -39,143 +39,136 @@ __device__ __forceinline__ float gelu_d(float z) {
            z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-// The network on the matrix cores.  A wavefront evaluates 32 nodes at once:
-// each layer is a [32 x K] x [K x 64] product done as two 32x32 output tiles
-// of v_mfma_f32_32x32x2f32 (K in steps of 2), with the weights staged once
-// per workgroup in LDS and the activations passed between layers through
-// LDS.  Operand layout of that instruction (wave64): lane l supplies
-// A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator r of lane l holds
-// D[r % 4 + 8 (r / 4) + 4 (l / 32)][l % 32].  Rows of odd length keep the
-// LDS accesses of a 32-lane half free of bank conflicts.  The sums run in the
-// matrix cores' order; the training forward (k_mlp_grad) runs them in index
-// order, so the two agree to FP32 rounding.
-typedef float f32x16 __attribute__((ext_vector_type(16)));
+// The network on the matrix cores, one wavefront per 16 nodes.  Lane
+// l = 16 q + j works on node j of the wave.  Each layer is a product
+// [64 units x K] x [K x 16 nodes] done as four 16x16 output tiles of
+// v_mfma_f32_16x16x4f32 (K in steps of 4).  Operand layout of that
+// instruction (wave64): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator r of lane l holds D[4 (l / 16) + r][l % 16].
+// After a layer, lane (q, j) therefore holds units 16 t + 4 q + r (tile t,
+// accumulator r) of node j, which is the B operand the next layer needs at
+// K step 4 t + r when that layer's K order is permuted the same way: the
+// activations never leave the registers.  The weights are read in that
+// permuted order from a lane-major copy (k_mlp_pack), one 16-byte load per
+// four values, so a wavefront needs neither LDS nor a barrier, and a
+// launch has one workgroup per 16 nodes to spread over the chip.  The sums
+// run in the matrix cores' order; the training forward (k_mlp_grad) runs
+// them in index order, so the two agree to FP32 rounding.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int NODES_PER_WAVE = 32;
-constexpr int X_LD = 19;                 // 17 normalised inputs + zero padding
-constexpr int W1_LD = 19;
-constexpr int H_LD = MLP_W + 1;
-constexpr int MLP_WAVES = 4;             // wavefronts per workgroup
-constexpr int MLP_NODES_WG = MLP_WAVES * NODES_PER_WAVE;
+constexpr int MLP_NODES = 16;            // nodes per wavefront (and workgroup)
+constexpr int K1_STEPS = 5;              // layer-1 K steps: 17 inputs padded to 20
+// per-lane slots of the packed copy
+constexpr int SL_W1 = 0;                              // [tile 4][step 5]
+constexpr int SL_W2 = SL_W1 + 4 * K1_STEPS;           // [out tile 4][in tile 4][acc 4]
+constexpr int SL_W3 = SL_W2 + 64;                     // [output 2][tile 4][acc 4]
+constexpr int SL_B1 = SL_W3 + 32;                     // [tile 4][acc 4]
+constexpr int SL_B2 = SL_B1 + 16;
+constexpr int SL_MEAN = SL_B2 + 16;                   // input means, [step 5]
+constexpr int SL_SCALE = SL_MEAN + K1_STEPS;          // input scales, [step 5]
+constexpr int MLP_SLOTS = (SL_SCALE + K1_STEPS + 3) / 4 * 4;
+static_assert(MLP_SLOTS == MLP_PACKED / 64, "packed copy size");
 
-struct MlpStage {
-    float w1[MLP_W * W1_LD], b1[MLP_W], w2[MLP_W * H_LD], b2[MLP_W];
-    float w3[MLP_OUT * MLP_W], b3[MLP_OUT], norm[MLP_NORM];
-};
-struct MlpWave {
-    float x[NODES_PER_WAVE * X_LD], h1[NODES_PER_WAVE * H_LD], h2[NODES_PER_WAVE * H_LD];
-    int bad[NODES_PER_WAVE];
-};
-
-// Every thread issues all its loads before its LDS stores (fixed trip counts),
-// so a workgroup waits for the weights about once, not once per row.
-constexpr int STAGE_THREADS = MLP_WAVES * 64;
-constexpr int W1_PER = (MLP_W * MLP_IN + STAGE_THREADS - 1) / STAGE_THREADS;
-constexpr int W2_PER = (MLP_W * MLP_W + STAGE_THREADS - 1) / STAGE_THREADS;
-
-__device__ void mlp_stage(MlpStage& s, const float* P, const float* norm) {
-    const int t = threadIdx.x;
-    float r1[W1_PER], r2[W2_PER];
-#pragma unroll
-    for (int u = 0; u < W1_PER; ++u) {
-        const int i = t + u * STAGE_THREADS;
-        r1[u] = i < MLP_W * MLP_IN ? P[W1_OFF + i] : 0.f;
+// Slot v of lane l: the weight, bias or input normalisation that lane feeds
+// to the matrix cores or its own arithmetic (zero-weight, unit-scale padding
+// past input 17).
+__device__ float mlp_slot(const float* P, const float* norm, int v, int l) {
+    const int q = l >> 4, j = l & 15;
+    if (v < SL_W2) {
+        const int t = v / K1_STEPS, f = 4 * (v % K1_STEPS) + q;
+        return f < MLP_IN ? P[W1_OFF + (16 * t + j) * MLP_IN + f] : 0.f;
     }
-#pragma unroll
-    for (int u = 0; u < W2_PER; ++u) {
-        const int i = t + u * STAGE_THREADS;
-        r2[u] = i < MLP_W * MLP_W ? P[W2_OFF + i] : 0.f;
+    if (v < SL_W3) {
+        const int u = v - SL_W2, to = u >> 4, ti = (u >> 2) & 3, r = u & 3;
+        return P[W2_OFF + (16 * to + j) * MLP_W + 16 * ti + 4 * q + r];
     }
-    float rb1 = 0.f, rb2 = 0.f, rw3 = 0.f, rb3 = 0.f, rn = 0.f;
-    if (t < MLP_W) { rb1 = P[B1_OFF + t]; rb2 = P[B2_OFF + t]; }
-    if (t < MLP_OUT * MLP_W) rw3 = P[W3_OFF + t];
-    if (t < MLP_OUT) rb3 = P[B3_OFF + t];
-    if (t < MLP_NORM) rn = norm[t];
-#pragma unroll
-    for (int u = 0; u < W1_PER; ++u) {
-        const int i = t + u * STAGE_THREADS;
-        if (i < MLP_W * MLP_IN) s.w1[(i / MLP_IN) * W1_LD + i % MLP_IN] = r1[u];
+    if (v < SL_B1) {
+        const int u = v - SL_W3, o = u >> 4, t = (u >> 2) & 3, r = u & 3;
+        return P[W3_OFF + o * MLP_W + 16 * t + 4 * q + r];
     }
-    if (t < MLP_W)
-        for (int k = MLP_IN; k < W1_LD; ++k) s.w1[t * W1_LD + k] = 0.f;   // the padding the inputs meet
-#pragma unroll
-    for (int u = 0; u < W2_PER; ++u) {
-        const int i = t + u * STAGE_THREADS;
-        if (i < MLP_W * MLP_W) s.w2[(i / MLP_W) * H_LD + i % MLP_W] = r2[u];
+    if (v < SL_B2) return P[B1_OFF + 16 * ((v - SL_B1) >> 2) + 4 * q + ((v - SL_B1) & 3)];
+    if (v < SL_MEAN) return P[B2_OFF + 16 * ((v - SL_B2) >> 2) + 4 * q + ((v - SL_B2) & 3)];
+    if (v < SL_SCALE) {
+        const int f = 4 * (v - SL_MEAN) + q;
+        return f < MLP_IN ? norm[f] : 0.f;
     }
-    if (t < MLP_W) { s.b1[t] = rb1; s.b2[t] = rb2; }
-    if (t < MLP_OUT * MLP_W) s.w3[t] = rw3;
-    if (t < MLP_OUT) s.b3[t] = rb3;
-    if (t < MLP_NORM) s.norm[t] = rn;
+    if (v < SL_SCALE + K1_STEPS) {
+        const int f = 4 * (v - SL_SCALE) + q;
+        return f < MLP_IN ? norm[MLP_IN + f] : 1.f;
+    }
+    return 0.f;
 }
 
-// Inputs of nodes k0 .. k0+31 (rows of x, clipped at n) normalised into the
-// wave's LDS; bad[node] is set where an input is not finite.  The caller
-// synchronises the workgroup before the values are read.
-constexpr int LOAD_PER = (NODES_PER_WAVE * MLP_IN + 63) / 64;
+// The lane-major copy: packed[(v / 4) * 256 + 4 l + v % 4] = slot v of lane l.
+__global__ void k_mlp_pack(const float* __restrict__ P, const float* __restrict__ norm, float* packed) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < MLP_PACKED; i += gridDim.x * blockDim.x)
+        packed[i] = mlp_slot(P, norm, (i >> 8) * 4 + (i & 3), (i >> 2) & 63);
+}
 
-__device__ void mlp_load(const MlpStage& s, MlpWave& w, const float* x, int k0, int n) {
-    const int l = threadIdx.x & 63;
-    if (l < NODES_PER_WAVE) {
-        w.bad[l] = 0;
+struct MlpOut {
+    float y0, y1;      // the two outputs of node j before denormalisation
+    bool ok;           // all inputs of node j finite
+};
+
+// The network on nodes k0 .. k0+15 of x (rows of MLP_IN, clipped at n).
+// Every lane returns the result of its node j = l % 16.
+__device__ MlpOut mlp_wave(const float* __restrict__ packed, const float* __restrict__ P, const float* x,
+                           int k0, int n) {
+    const int l = threadIdx.x & 63, q = l >> 4, j = l & 15;
+    const int node = k0 + j;
+    float w[MLP_SLOTS];
+    const f32x4* pk = reinterpret_cast<const f32x4*>(packed);
 #pragma unroll
-        for (int k = MLP_IN; k < X_LD; ++k) w.x[l * X_LD + k] = 0.f;
+    for (int u = 0; u < MLP_SLOTS / 4; ++u) {
+        const f32x4 v = pk[u * 64 + l];
+        w[4 * u] = v[0]; w[4 * u + 1] = v[1]; w[4 * u + 2] = v[2]; w[4 * u + 3] = v[3];
     }
-    const int avail = max(0, min(NODES_PER_WAVE, n - k0)) * MLP_IN;   // contiguous input values
-    const float* src = x + static_cast<size_t>(max(k0, 0)) * MLP_IN;
-    float v[LOAD_PER];
+    float xh[K1_STEPS];
+    int bad = 0;
 #pragma unroll
-    for (int u = 0; u < LOAD_PER; ++u) {
-        const int idx = l + 64 * u;
-        v[u] = idx < avail ? src[idx] : 0.f;
+    for (int s = 0; s < K1_STEPS; ++s) {
+        const int f = 4 * s + q;
+        const float v = node < n && f < MLP_IN ? x[static_cast<size_t>(node) * MLP_IN + f] : 0.f;
+        bad |= !isfinite(v);
+        xh[s] = (v - w[SL_MEAN + s]) / w[SL_SCALE + s];
     }
+    bad |= __shfl_xor(bad, 16);
+    bad |= __shfl_xor(bad, 32);
+    f32x4 acc[4];
 #pragma unroll
-    for (int u = 0; u < LOAD_PER; ++u) {
-        const int idx = l + 64 * u;
-        if (idx < NODES_PER_WAVE * MLP_IN) {
-            const int node = idx / MLP_IN, f = idx % MLP_IN;
-            w.x[node * X_LD + f] = (v[u] - s.norm[f]) / s.norm[MLP_IN + f];
-            if (!isfinite(v[u])) w.bad[node] = 1;
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < K1_STEPS; ++s)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[SL_W1 + t * K1_STEPS + s], xh[s], acc[t], 0, 0, 0);
+    float h[16];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) h[4 * t + r] = gelu(acc[t][r] + w[SL_B1 + 4 * t + r]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[SL_W2 + 16 * t + s], h[s], acc[t], 0, 0, 0);
+    // the 64 -> 2 layer: this lane's 16 units, then the sum over the four q
+    float o0 = 0.f, o1 = 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float g = gelu(acc[t][r] + w[SL_B2 + 4 * t + r]);
+            o0 += w[SL_W3 + 4 * t + r] * g;
+            o1 += w[SL_W3 + 16 + 4 * t + r] * g;
         }
-    }
-}
-
-// The network on the wave's 32 loaded nodes.  Returns, on lane l, output l / 32
-// of node l % 32 (before denormalisation).  Every wave of the workgroup calls it.
-__device__ float mlp_wave(const MlpStage& s, MlpWave& w) {
-    const int l = threadIdx.x & 63, lo = l & 31, kh = l >> 5;
-    f32x16 c0 = {}, c1 = {};
-    for (int kk = 0; kk < (MLP_IN + 1) / 2; ++kk) {
-        const int k = 2 * kk + kh;
-        const float a = w.x[lo * X_LD + k];
-        c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, s.w1[lo * W1_LD + k], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, s.w1[(32 + lo) * W1_LD + k], c1, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int i = (r & 3) + 8 * (r >> 2) + 4 * kh;
-        w.h1[i * H_LD + lo] = gelu(c0[r] + s.b1[lo]);
-        w.h1[i * H_LD + 32 + lo] = gelu(c1[r] + s.b1[32 + lo]);
-    }
-    __syncthreads();
-    c0 = f32x16{};
-    c1 = f32x16{};
-    for (int kk = 0; kk < MLP_W / 2; ++kk) {
-        const int k = 2 * kk + kh;
-        const float a = w.h1[lo * H_LD + k];
-        c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, s.w2[lo * H_LD + k], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, s.w2[(32 + lo) * H_LD + k], c1, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int i = (r & 3) + 8 * (r >> 2) + 4 * kh;
-        w.h2[i * H_LD + lo] = gelu(c0[r] + s.b2[lo]);
-        w.h2[i * H_LD + 32 + lo] = gelu(c1[r] + s.b2[32 + lo]);
-    }
-    __syncthreads();
-    float out = s.b3[kh];
-    for (int k = 0; k < MLP_W; ++k) out += s.w3[kh * MLP_W + k] * w.h2[lo * H_LD + k];
-    return out;
+    o0 += __shfl_xor(o0, 16);
+    o1 += __shfl_xor(o1, 16);
+    o0 += __shfl_xor(o0, 32);
+    o1 += __shfl_xor(o1, 32);
+    return MlpOut{P[B3_OFF] + o0, P[B3_OFF + 1] + o1, bad == 0};
 }
 
 // Inference inside the step: the network's two outputs are the gains of the
@@ -184,28 +177,18 @@ __device__ float mlp_wave(const MlpStage& s, MlpWave& w) {
 // g_par in [0, 4] and g_perp in [-2, 2], and default to (1, 0), the plain
 // correction, where an input or an output is not finite.  Nothing
 // happens when the residual is already within tolerance.
-__global__ __launch_bounds__(MLP_WAVES * 64) void k_mlp_infer(const float* __restrict__ P,
-                                                              const float* __restrict__ norm,
-                                                              const float* feat, double* gain, int K,
-                                                              const MdfDev* md, double tol) {
+__global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ packed, const float* __restrict__ P,
+                                                  const float* __restrict__ norm, const float* feat, double* gain,
+                                                  int K, const MdfDev* md, double tol) {
     if (tol > 0.0 && mdf_res(md, 1) <= tol) return;
-    __shared__ MlpStage s;
-    __shared__ MlpWave waves[MLP_WAVES];
-    mlp_stage(s, P, norm);
-    __syncthreads();
-    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int k0 = blockIdx.x * MLP_NODES_WG + wv * NODES_PER_WAVE;
-    mlp_load(s, waves[wv], feat, k0, K);
-    __syncthreads();
-    const bool ok = l < NODES_PER_WAVE && waves[wv].bad[l] == 0;
-    const float out = mlp_wave(s, waves[wv]);
-    const float out1 = __shfl(out, (l & 31) + 32);
-    const int k = k0 + l;
-    if (l < NODES_PER_WAVE && k < K) {
+    const int k0 = blockIdx.x * MLP_NODES;
+    const MlpOut r = mlp_wave(packed, P, feat, k0, K);
+    const int l = threadIdx.x, k = k0 + (l & 15);
+    if (l < MLP_NODES && k < K) {
         double g0 = 1.0, g1 = 0.0;
-        if (ok) {
-            const float gp = s.norm[2 * MLP_IN] + s.norm[2 * MLP_IN + MLP_OUT] * out;
-            const float gq = s.norm[2 * MLP_IN + 1] + s.norm[2 * MLP_IN + MLP_OUT + 1] * out1;
+        if (r.ok) {
+            const float gp = norm[2 * MLP_IN] + norm[2 * MLP_IN + MLP_OUT] * r.y0;
+            const float gq = norm[2 * MLP_IN + 1] + norm[2 * MLP_IN + MLP_OUT + 1] * r.y1;
             if (isfinite(gp) && isfinite(gq)) {
                 g0 = fminf(fmaxf(gp, 0.f), 4.f);
                 g1 = fminf(fmaxf(gq, -2.f), 2.f);
@@ -217,20 +200,15 @@ __global__ __launch_bounds__(MLP_WAVES * 64) void k_mlp_infer(const float* __res
 }
 
 // The network alone on n inputs, on the matrix cores: y = out_mean + out_scale * net(x^).
-__global__ __launch_bounds__(MLP_WAVES * 64) void k_mlp_predict(const float* __restrict__ P,
-                                                                const float* __restrict__ norm,
-                                                                const float* x, float* y, int n) {
-    __shared__ MlpStage s;
-    __shared__ MlpWave waves[MLP_WAVES];
-    mlp_stage(s, P, norm);
-    __syncthreads();
-    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int k0 = blockIdx.x * MLP_NODES_WG + wv * NODES_PER_WAVE;
-    mlp_load(s, waves[wv], x, k0, n);
-    __syncthreads();
-    const float out = mlp_wave(s, waves[wv]);
-    const int k = k0 + (l & 31), o = l >> 5;
-    if (k < n) y[static_cast<size_t>(k) * MLP_OUT + o] = s.norm[2 * MLP_IN + o] + s.norm[2 * MLP_IN + MLP_OUT + o] * out;
+__global__ __launch_bounds__(64) void k_mlp_predict(const float* __restrict__ packed, const float* __restrict__ P,
+                                                    const float* __restrict__ norm, const float* x, float* y, int n) {
+    const int k0 = blockIdx.x * MLP_NODES;
+    const MlpOut r = mlp_wave(packed, P, x, k0, n);
+    const int l = threadIdx.x, k = k0 + (l & 15);
+    if (l < MLP_NODES && k < n) {
+        y[static_cast<size_t>(k) * MLP_OUT] = norm[2 * MLP_IN] + norm[2 * MLP_IN + MLP_OUT] * r.y0;
+        y[static_cast<size_t>(k) * MLP_OUT + 1] = norm[2 * MLP_IN + 1] + norm[2 * MLP_IN + MLP_OUT + 1] * r.y1;
+    }
 }
 
 // The gains, low-passed along each membrane with the binomial weights
@@ -429,8 +407,8 @@ __global__ void k_colstats(const float* rows, int64_t nrows, int64_t chunk, doub
 
 void launch_mlp_infer(Ctx& c) {
     if (!c.model || c.K == 0) return;
-    hipLaunchKernelGGL(k_mlp_infer, dim3((c.K + MLP_NODES_WG - 1) / MLP_NODES_WG), dim3(MLP_WAVES * 64), 0,
-                       c.stream, c.model->params, c.model->norm, c.mdf_feat, c.mdf_pred, c.K, c.mdf,
+    hipLaunchKernelGGL(k_mlp_infer, dim3((c.K + MLP_NODES - 1) / MLP_NODES), dim3(64), 0, c.stream,
+                       c.model->packed, c.model->params, c.model->norm, c.mdf_feat, c.mdf_pred, c.K, c.mdf,
                        c.mdf_tol);
     hipLaunchKernelGGL(k_mdf_smooth, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.mdf_pred,
                        c.spread, c.off, c.owner, c.x, c.y, c.K, c.ny, c.mdf, c.mdf_tol);
@@ -448,6 +426,15 @@ static int ensure_partial(Mlp& m, int nblk) {
 
 static double* partial_loss(Mlp& m) {
     return reinterpret_cast<double*>(m.partial + static_cast<size_t>(m.partial_blocks) * MLP_P);
+}
+
+// Refresh the lane-major copy inference reads; every change of the weights or
+// the normalisation ends here, before the device is used on another stream.
+static int repack(Mlp& m) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(MLP_PACKED / 256), dim3(256), 0, m.stream, m.params, m.norm, m.packed);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipStreamSynchronize(m.stream));
+    return SF_OK;
 }
 
 // Gradient (and optionally an Adam step) over count rows starting at first.
@@ -504,7 +491,7 @@ static int fit_norm(Mlp& m, const float* rows, int64_t nrows) {
     }
     SF_HIP(hipMemcpy(m.norm, nrm, sizeof(nrm), hipMemcpyHostToDevice));
     m.norm_set = true;
-    return SF_OK;
+    return repack(m);
 }
 
 }  // namespace sf
@@ -526,6 +513,7 @@ int sf_mlp_create(sf_mlp** out, int device, uint64_t seed) {
     bool ok = hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.params), MLP_P * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.norm), MLP_NORM * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(reinterpret_cast<void**>(&m.packed), MLP_PACKED * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.adam_m), MLP_P * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.adam_v), MLP_P * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m.acc), sizeof(double)) == hipSuccess;
@@ -548,7 +536,7 @@ int sf_mlp_create(sf_mlp** out, int device, uint64_t seed) {
          hipMemcpy(m.norm, nrm, sizeof(nrm), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(m.adam_m, zeros.data(), MLP_P * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(m.adam_v, zeros.data(), MLP_P * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
-    if (!ok) { sf_mlp_destroy(h); return fail(SF_E_HIP, "network upload failed"); }
+    if (!ok || repack(m) != SF_OK) { sf_mlp_destroy(h); return fail(SF_E_HIP, "network upload failed"); }
     *out = h;
     return SF_OK;
 }
@@ -558,7 +546,7 @@ int sf_mlp_destroy(sf_mlp* h) {
     Mlp& m = h->m;
     hipSetDevice(m.device);
     if (m.stream) hipStreamSynchronize(m.stream);
-    void* ptrs[] = {m.params, m.norm, m.adam_m, m.adam_v, m.partial, m.acc};
+    void* ptrs[] = {m.params, m.norm, m.packed, m.adam_m, m.adam_v, m.partial, m.acc};
     for (void* p : ptrs) if (p) hipFree(p);
     if (m.stream) hipStreamDestroy(m.stream);
     delete h;
@@ -601,7 +589,7 @@ int sf_mlp_set_params(sf_mlp* h, const float* params, const float* norm) {
         SF_HIP(hipMemcpy(m.adam_v, zeros.data(), MLP_P * sizeof(float), hipMemcpyHostToDevice));
         m.adam_t = 0;
     }
-    return SF_OK;
+    return repack(m);
 }
 
 int sf_mlp_predict(sf_mlp* h, int n, const float* x, float* y) {
@@ -614,8 +602,8 @@ int sf_mlp_predict(sf_mlp* h, int n, const float* x, float* y) {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&dy), static_cast<size_t>(n) * MLP_OUT * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(dx, x, static_cast<size_t>(n) * MLP_IN * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_mlp_predict, dim3((n + MLP_NODES_WG - 1) / MLP_NODES_WG), dim3(MLP_WAVES * 64),
-                           0, m.stream, m.params, m.norm, dx, dy, n);
+        hipLaunchKernelGGL(k_mlp_predict, dim3((n + MLP_NODES - 1) / MLP_NODES), dim3(64), 0, m.stream,
+                           m.packed, m.params, m.norm, dx, dy, n);
         e = hipStreamSynchronize(m.stream);
     }
     if (e == hipSuccess) e = hipMemcpy(y, dy, static_cast<size_t>(n) * MLP_OUT * sizeof(float), hipMemcpyDeviceToHost);
@@ -698,7 +686,8 @@ int sf_mlp_train(sf_mlp* h, sf_ctx* ctx, int epochs, int batch, float lr, uint64
         if (epoch_loss) epoch_loss[ep] = 0.5 * L / static_cast<double>(nrows);
     }
     hipFree(didx);
-    return rc;
+    const int rp = repack(m);
+    return rc != SF_OK ? rc : rp;
 }
 
 }  // extern "C"
