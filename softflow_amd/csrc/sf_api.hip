@@ -185,7 +185,7 @@ static int step_ibm(Ctx& c) {
     const int ncorr = mdf_corrections(c);
     const bool mdf = ncorr > 0 && c.K > 0;
     launch_mdf_fold(c);   // a step staged without its closing interpolation
-    { Scope s(c, "spread"); launch_spread(c, !mdf, mdf); }
+    { Scope s(c, "spread"); launch_spread(c, mdf); }
     if (ncorr > 0) {
         if (c.K > 0) {
             const bool learned = c.mdf_warm && c.model;
@@ -278,6 +278,7 @@ constexpr size_t MAX_GRAPHS = 24;
 static int run_graph(Ctx& c, int nsteps) {
     const int period = 6;
     launch_mdf_fold(c);   // never inside a captured step
+    launch_bins_close(c);
     while (nsteps > 0) {
         if (c.shift_next != SHIFT_PREV ||
             (c.nsp > 0 && (c.c_from_store || (mdf_corrections(c) <= 0 && !c.adr_pending)))) {
@@ -287,15 +288,15 @@ static int run_graph(Ctx& c, int nsteps) {
         }
         const int len = std::min(nsteps, period);
         const int f0 = c.fcur, g0 = c.gcur, a0 = c.Fcur, b0 = c.Fprev;
-        const int key[5] = {f0, g0, a0, b0, c.cell_cur};
+        const int key[6] = {f0, g0, a0, b0, c.cell_cur, c.cells_prefilled ? 1 : 0};
         // a graph bakes in the buffer roles it was captured with: replay the
         // one captured from these roles, or capture it
         Ctx::GraphEntry* hit = nullptr;
         for (Ctx::GraphEntry& g : c.graphs)
-            if (g.len == len && std::equal(key, key + 5, g.key)) hit = &g;
+            if (g.len == len && std::equal(key, key + 6, g.key)) hit = &g;
         if (!hit) {
             Ctx::GraphEntry g{};
-            std::copy(key, key + 5, g.key);
+            std::copy(key, key + 6, g.key);
             g.len = len;
             SF_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
             for (int s = 0; s < len; ++s) one_step(c);
@@ -306,8 +307,8 @@ static int run_graph(Ctx& c, int nsteps) {
                 hipGraphDestroy(g.graph);
                 return fail(SF_E_STATE, "step buffer rotation did not close over the graph period");
             }
-            const int out[6] = {c.fcur, c.gcur, c.Fcur, c.Fprev, c.Fpart, c.cell_cur};
-            std::copy(out, out + 6, g.out);
+            const int out[7] = {c.fcur, c.gcur, c.Fcur, c.Fprev, c.Fpart, c.cell_cur, c.cells_prefilled ? 1 : 0};
+            std::copy(out, out + 7, g.out);
             e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
             if (e != hipSuccess) {
                 hipGraphDestroy(g.graph);
@@ -324,6 +325,7 @@ static int run_graph(Ctx& c, int nsteps) {
         SF_HIP(hipGraphLaunch(hit->exec, c.stream));
         c.fcur = hit->out[0]; c.gcur = hit->out[1]; c.Fcur = hit->out[2];
         c.Fprev = hit->out[3]; c.Fpart = hit->out[4]; c.cell_cur = hit->out[5];
+        c.cells_prefilled = hit->out[6] != 0;
         c.step_count += len;
         nsteps -= len;
         c.c_from_store = false;
@@ -512,6 +514,7 @@ int build_ibb_links(Ctx& c) {
 }
 
 static void free_caps(Ctx& c) {
+    drop_contact_prefill(c);
     void* ptrs[] = {c.off, c.owner, c.ring, c.arc, c.desc, c.has_prev, c.x, c.y, c.vx, c.vy, c.fx, c.fy,
                     c.L0, c.th0, c.ds0, c.Lprev, c.spread, c.mdf_total, c.mdf_u0, c.mdf_hist,
                     c.mdf_feat, c.mdf_pred, c.mdf_f1, c.mdf_dmin, c.mdf_head};
@@ -1005,6 +1008,7 @@ int sf_set_nodes(sf_ctx* h, const double* x, const double* y, const double* vx, 
     const size_t b = static_cast<size_t>(c.K) * 8;
     if (b == 0) return SF_OK;
     c.arc_fresh = false;
+    if (x || y) drop_contact_prefill(c);
     if (x) SF_HIP(hipMemcpy(c.x, x, b, hipMemcpyHostToDevice));
     if (y) SF_HIP(hipMemcpy(c.y, y, b, hipMemcpyHostToDevice));
     if (vx) SF_HIP(hipMemcpy(c.vx, vx, b, hipMemcpyHostToDevice));
@@ -1408,9 +1412,9 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
         int sgrid = keep_grid;
         for (const int gsz : grids) {
             c.spread_grid = gsz;
-            launch_spread(c, true);   // warm
+            launch_spread(c);   // warm
             SF_HIP(hipEventRecord(e0, c.stream));
-            for (int r = 0; r < reps; ++r) launch_spread(c, true);
+            for (int r = 0; r < reps; ++r) launch_spread(c);
             SF_HIP(hipEventRecord(e1, c.stream));
             SF_HIP(hipEventSynchronize(e1));
             SF_HIP(hipGetLastError());
@@ -1423,6 +1427,7 @@ int sf_autotune(sf_ctx* h, int reps, char* report, int report_len) {
             rep += line;
             if (per < sbest) { sbest = per; sgrid = gsz; }
         }
+        launch_bins_close(c);
         c.Fcur = keep_F;
         c.arc_fresh = keep_arc;
         c.spread_grid = sgrid;

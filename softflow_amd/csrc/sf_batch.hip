@@ -346,14 +346,23 @@ static bool plan_current(const sf_batch& B) {
 
 // Work queued on the members' own streams is done before the batch starts.
 static int wait_members(sf_batch& B) {
-    for (Ctx* c : B.mem) SF_HIP(hipStreamSynchronize(c->stream));
+    for (Ctx* c : B.mem) {
+        launch_bins_close(*c);      // the batched binning starts from empty tiles
+        drop_contact_prefill(*c);   // and the batched cell fill from empty cells
+        SF_HIP(hipStreamSynchronize(c->stream));
+    }
     return SF_OK;
 }
 
-// Later work on the members' own streams waits for the batch.
+// Later work on the members' own streams waits for the batch.  The batched
+// steps leave their touched-tile counts set: a member's own next spread
+// clears them first.
 static int release_members(sf_batch& B) {
     SF_HIP(hipEventRecord(B.ev, B.stream));
-    for (Ctx* c : B.mem) SF_HIP(hipStreamWaitEvent(c->stream, B.ev, 0));
+    for (Ctx* c : B.mem) {
+        SF_HIP(hipStreamWaitEvent(c->stream, B.ev, 0));
+        if (c->tile_cnt) c->bins_open = true;
+    }
     return SF_OK;
 }
 

@@ -33,8 +33,10 @@ struct BinArgs {
     int* tile_cnt;          // per tile: nodes binned
     int* tile_list;         // per tile: TILE_MAXL node indices
     NodeSpread* tile_rec;   // per tile: copies of its first TILE_RECS records
-    int* counters;          // [1]: tile list overflow; [3]: touched-tile count, cleared here
+    int* counters;          // [1]: tile list overflow; [3]: touched-tile count
     int nx, ny, ntx;
+    int2* act;              // touched-tile list (tile, -1) built here, or null (built after binning)
+    int ntiles;
 };
 
 // Arc length element of node k: half the distances to its ring neighbours
@@ -104,6 +106,39 @@ __device__ __forceinline__ void bin_node(const BinArgs& b, const int (&t)[8], co
         else atomicOr(&b.counters[1], 1);
         if (slot[c] < TILE_RECS) b.tile_rec[static_cast<size_t>(t[c]) * TILE_RECS + slot[c]] = s;
     }
+    if (b.act) {
+        // The claim that opened a tile (slot 0) lists it as touched, one list
+        // atomic per workgroup (at most 256 threads, all of which get here);
+        // the gather reads the tile's count once the binning is over (entry
+        // .y = -1).
+        __shared__ int s_cnt[4], s_base;
+        const int lane = threadIdx.x & 63, wv = static_cast<int>(threadIdx.x >> 6);
+        int fresh = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) fresh += t[c] >= 0 && slot[c] == 0;
+        int incl = fresh;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d);
+            if (lane >= d) incl += v;
+        }
+        if (lane == 63) s_cnt[wv] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int total = 0;
+            for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) total += s_cnt[w];
+            s_base = total ? atomicAdd(&b.counters[3], total) : 0;
+        }
+        __syncthreads();
+        int pos = s_base + incl - fresh;
+        for (int w = 0; w < wv; ++w) pos += s_cnt[w];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (t[c] < 0 || slot[c] != 0) continue;
+            if (pos < b.ntiles) b.act[pos] = make_int2(t[c], -1);
+            ++pos;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -137,7 +172,7 @@ __device__ __forceinline__ void node_prep_body(const double* x, const double* y,
                             const int2* ring, const double* arc, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, NodeSpread* tile_rec,
                             int* counters, int mode, MdfDev* md, double tol, int it,
-                            double* total, double* head, int bx) {
+                            double* total, double* head, int2* act, int bx) {
     const int k = bx * blockDim.x + threadIdx.x;
     if (mode == 0) {
         if (md) {
@@ -148,8 +183,11 @@ __device__ __forceinline__ void node_prep_body(const double* x, const double* y,
                 md->res[i / (MDF_MAX_CORR + 2)][i % (MDF_MAX_CORR + 2)] = 0ull;
             for (int i = k; i < 2 * K; i += nthr) { total[i] = 0.0; head[i] = 0.0; }
         }
-        // binning: every lane of the wavefront takes part in the slot claims
-        if (k == 0) counters[3] = 0;     // the touched-tile list is rebuilt after binning
+        // binning: every lane of the wavefront takes part in the slot claims.
+        // Without act the touched-tile list is built after binning, from zero;
+        // with it, the binning appends to a list the previous step's close
+        // (or launch_bins_close) emptied.
+        if (k == 0 && !act) counters[3] = 0;
         int t[8];
         NodeSpread s{};
         if (k < K) {
@@ -162,7 +200,7 @@ __device__ __forceinline__ void node_prep_body(const double* x, const double* y,
 #pragma unroll
             for (int c = 0; c < 8; ++c) t[c] = -1;
         }
-        bin_node(BinArgs{rec, tile_cnt, tile_list, tile_rec, counters, nx, ny, ntx}, t, s);
+        bin_node(BinArgs{rec, tile_cnt, tile_list, tile_rec, counters, nx, ny, ntx, act, ntx * nty}, t, s);
         return;
     }
     if (mode == 1) mdf_node_apply(md, total, head, rec, K, it, tol, k);
@@ -172,8 +210,8 @@ __global__ void k_node_prep(const double* x, const double* y, const double* fx, 
                             const int2* ring, const double* arc, NodeSpread* rec, int K, int nx, int ny,
                             int ntx, int nty, int* tile_cnt, int* tile_list, NodeSpread* tile_rec,
                             int* counters, int mode, MdfDev* md, double tol, int it,
-                            double* total, double* head) {
-    node_prep_body(x, y, fx, fy, ring, arc, rec, K, nx, ny, ntx, nty, tile_cnt, tile_list, tile_rec, counters, mode, md, tol, it, total, head, blockIdx.x);
+                            double* total, double* head, int2* act) {
+    node_prep_body(x, y, fx, fy, ring, arc, rec, K, nx, ny, ntx, nty, tile_cnt, tile_list, tile_rec, counters, mode, md, tol, it, total, head, act, blockIdx.x);
 }
 
 // Gather: one 256-thread workgroup per TILE_X x TILE_Y tile.
@@ -249,9 +287,10 @@ __device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void
         copy = mode == 1 && dirty_in[tile];
         need = copy || dirty_out[tile];
     }
-    if (mode == 0) {
-        // one list atomic per workgroup (at most 256 threads): the wavefronts'
-        // touched counts meet in LDS (mode 0 has no early exit)
+    if (mode == 0 && act) {
+        // the touched-tile list, when the binning did not make it: one list
+        // atomic per workgroup (at most 256 threads); the wavefronts' touched
+        // counts meet in LDS (mode 0 has no early exit)
         __shared__ int s_cnt[4], s_base;
         const int wv = static_cast<int>(threadIdx.x >> 6);
         const unsigned long long touched = __ballot(cnt > 0);
@@ -316,16 +355,6 @@ __device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void
     }
 }
 
-template <bool FP32>
-__global__ __launch_bounds__(256) void k_spread_rest(const int* tile_cnt, const void* Fin, void* Fout,
-                                                     const uint8_t* dirty_in, uint8_t* dirty_out,
-                                                     int2* act, int* counters, int nx, int ny,
-                                                     int ntx, int ntiles, int mode,
-                                                     const MdfDev* md, int it, int last_it, double tol) {
-    spread_rest_body<FP32>(tile_cnt, Fin, Fout, dirty_in, dirty_out, act, counters, nx, ny, ntx, ntiles, mode, md,
-                           blockIdx.x, it, last_it, tol);
-}
-
 // Touched tiles: a persistent grid walks the list of tiles the node binning
 // touched (act[0 .. counters[3])), one 256-thread workgroup per tile and one
 // thread per cell of its TILE_X x TILE_Y cells.
@@ -370,7 +399,7 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
         if (ai + gx < ntiles) te_next = act[ai + gx];
         const int tile = te.x;
         __syncthreads();          // the previous tile's LDS use is over
-        int L = te.y;
+        int L = te.y >= 0 ? te.y : tile_cnt[tile];   // -1: listed by the binning
         if (L > TILE_MAXL) L = TILE_MAXL;
         const int tx0 = (tile % ntx) * TILE_X, ty0 = (tile / ntx) * TILE_Y;
         const int cx = tx0 + (t % TILE_X);
@@ -536,14 +565,24 @@ __device__ __forceinline__ void spread_body(const NodeSpread* rec, const NodeSpr
     }
 }
 
+// The spread of a step's node forces in one launch: workgroups [0, nrest)
+// clear the untouched tiles that still hold force (spread_rest_body; the
+// binning made the touched-tile list), the rest gather the touched tiles
+// (spread_body).  The two parts write disjoint tiles.  The tile counts stay
+// set through the launch (the untouched part reads them): the step's closing
+// interpolation clears them, or launch_bins_close.
 template <bool FP32>
-__global__ __launch_bounds__(256) void k_spread(const NodeSpread* rec, const NodeSpread* tile_rec,
-                                                int* tile_cnt, const int* tile_list, const int2* act,
-                                                const int* counters, const double* lut,
-                                                const void* Fin, void* Fout, uint8_t* dirty_out,
-                                                int nx, int ny, int ntx, int ntiles, int mode,
-                                                int last, const MdfDev* md) {
-    spread_body<FP32>(rec, tile_rec, tile_cnt, tile_list, act, counters, lut, Fin, Fout, dirty_out, nx, ny, ntx, ntiles, mode, last, md, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(256) void k_spread_step(const NodeSpread* rec, const NodeSpread* tile_rec,
+                                                     int* tile_cnt, const int* tile_list, const int2* act,
+                                                     int* counters, const double* lut, void* F, uint8_t* dirty,
+                                                     int nx, int ny, int ntx, int ntiles, int nrest) {
+    const int b = blockIdx.x;
+    if (b < nrest) {
+        spread_rest_body<FP32>(tile_cnt, F, F, dirty, dirty, nullptr, counters, nx, ny, ntx, ntiles, 0, nullptr, b);
+        return;
+    }
+    spread_body<FP32>(rec, tile_rec, tile_cnt, tile_list, act, counters, lut, F, F, dirty, nx, ny, ntx, ntiles, 0,
+                      0, nullptr, b - nrest, static_cast<int>(gridDim.x) - nrest);
 }
 
 // MDF correction it in one launch: workgroups [0, nrest) the untouched tiles
@@ -889,9 +928,12 @@ __device__ __forceinline__ void mdf_finish_body(MdfDev* md, int ncorr, int probe
 // The statistics fold (one thread) and the tile counts of the step cleared
 // (every thread), which the MDF corrections leave set.
 __global__ void k_mdf_finish(MdfDev* md, int ncorr, int probe, int K, int collect, int64_t ds_cap, int* tile_cnt,
-                             int ntiles) {
+                             int ntiles, int* act_cnt) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) mdf_finish_body(md, ncorr, probe, K, collect, ds_cap);
+    if (t == 0) {
+        mdf_finish_body(md, ncorr, probe, K, collect, ds_cap);
+        *act_cnt = 0;
+    }
     if (t < ntiles) tile_cnt[t] = 0;
 }
 
@@ -926,7 +968,8 @@ __device__ __forceinline__ void move_node(const MoveArgs& m, double& px, double&
 
 template <bool FP32, int LPN = 16>
 __device__ __forceinline__ void interp_body(CellVelArgs a, const double* lut, double* x,
-                                                double* y, double* vx, double* vy, int K, MoveArgs mv, int bx) {
+                                                double* y, double* vx, double* vy, int K, MoveArgs mv, int bx,
+                                                const CellFill& cf = CellFill{}) {
     constexpr int NPW = interp_nodes(LPN);
     __shared__ double tx[16 * NPW], ty[16 * NPW];
     __shared__ int use[16 * NPW];
@@ -945,33 +988,44 @@ __device__ __forceinline__ void interp_body(CellVelArgs a, const double* lut, do
             y[k] = py;
         }
     }
+    if (cf.cnt) {
+        // the next step's contact cell list, from the moved position
+        const int ci = live && l == 0
+                           ? contact_cell(px, py, cf.xorg, cf.Lx, cf.cdx, cf.cdy, cf.ybot, cf.ncx, cf.ncy)
+                           : -1;
+        contact_cell_claim(cf.cnt, cf.list, cf.counters, ci, k);
+    }
 }
 
-// The step's MDF statistics fold (mdf_finish_body) and the tile counts the
-// corrections leave set, cleared; done by the interpolation that ends a step
-// with MDF (md null: nothing).
+// What the interpolation that ends a step also does: clear the bins the
+// step's spread left open (tile counts, touched-tile count; tile_cnt null:
+// none) and fold the step's MDF statistics (mdf_finish_body; md null: none).
 struct MdfFold {
     MdfDev* md;
     int ncorr, probe, K, collect;
     int64_t ds_cap;
     int* tile_cnt;
     int ntiles;
+    int* act_cnt;
 };
 
 __device__ __forceinline__ void mdf_fold(const MdfFold& f) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
-    for (int t = gid; t < f.ntiles; t += nthr) f.tile_cnt[t] = 0;
-    if (gid == 0) mdf_finish_body(f.md, f.ncorr, f.probe, f.K, f.collect, f.ds_cap);
+    if (f.tile_cnt) {
+        for (int t = gid; t < f.ntiles; t += nthr) f.tile_cnt[t] = 0;
+        if (gid == 0) *f.act_cnt = 0;
+    }
+    if (gid == 0 && f.md) mdf_finish_body(f.md, f.ncorr, f.probe, f.K, f.collect, f.ds_cap);
 }
 
 template <bool FP32, int LPN>
 __global__ __launch_bounds__(256) void k_interp(CellVelArgs a, const double* lut, double* x,
                                                 double* y, double* vx, double* vy, int K, MoveArgs mv,
-                                                int band, MdfFold fold) {
-    if (fold.md) mdf_fold(fold);
+                                                int band, MdfFold fold, CellFill cf) {
+    if (fold.md || fold.tile_cnt) mdf_fold(fold);
     const int bx = band_block(blockIdx.x, band);
     if (bx * interp_nodes(LPN) >= K) return;       // padding workgroup of the banded grid
-    interp_body<FP32, LPN>(a, lut, x, y, vx, vy, K, mv, bx);
+    interp_body<FP32, LPN>(a, lut, x, y, vx, vy, K, mv, bx, cf);
 }
 
 // moveAllNodes + y-clamp (capsule.cpp:267-297, simulation.cpp:665-675); one
@@ -1055,11 +1109,12 @@ static int interp_band(int K, int lpn) {
 // k_interp at lpn lanes per node (an interpolation with mv.move = 0 only
 // writes the velocities)
 static void launch_interp_lpn(Ctx& c, int lpn, const CellVelArgs& a, double* vx, double* vy, const MoveArgs& mv,
-                              const MdfFold& fold) {
+                              const MdfFold& fold, const CellFill& cf = CellFill{}) {
     const int band = interp_band(c.K, lpn);
     const dim3 grid(8 * band);
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, vx, vy, c.K, mv, band, fold);
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, c.stream, a, c.phi, c.x, c.y, vx, vy, c.K, mv, band, fold,
+                           cf);
     };
     if (c.fp32) {
         if (lpn == 4) go(k_interp<true, 4>);
@@ -1072,37 +1127,42 @@ static void launch_interp_lpn(Ctx& c, int lpn, const CellVelArgs& a, double* vx,
     }
 }
 
-// Bin the nodes and spread their forces into F[Fcur] (overwrite); last: the
-// tile counts are cleared after (a step without MDF corrections); mdf: the
-// binning also clears the step's MDF state.
-void launch_spread(Ctx& c, bool last, bool mdf) {
+// Bin the nodes and spread their forces into F[Fcur] (overwrite); mdf: the
+// binning also clears the step's MDF state.  The bins stay open (tile counts,
+// touched-tile list) for the step's MDF corrections until the step's closing
+// interpolation or launch_bins_close clears them.
+void launch_spread(Ctx& c, bool mdf) {
     if (c.K == 0) {
         launch_fill_zero(c, c.F[c.Fcur], (c.fp32 ? 8 : 16) * static_cast<size_t>(c.N));
         return;
     }
+    launch_bins_close(c);   // a binning no interpolation closed (the autotuner's repeats)
     // the membrane kernel left the arc lengths of the current positions
     const double* arc = c.arc_fresh ? c.arc : nullptr;
     hipLaunchKernelGGL(k_node_prep, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
                        c.fx, c.fy, c.ring, arc, c.spread, c.K, c.nx, c.ny, c.ntx, c.nty,
                        c.tile_cnt, c.tile_list, c.tile_rec, c.counters, 0, mdf ? c.mdf : nullptr, 0.0, 0,
-                       mdf ? c.mdf_total : nullptr, mdf ? c.mdf_head : nullptr);
+                       mdf ? c.mdf_total : nullptr, mdf ? c.mdf_head : nullptr, c.tile_act);
+    c.bins_open = true;
     void* F = c.F[c.Fcur];
     uint8_t* d = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
-    const dim3 rgrid((c.ntiles + rest_tiles_per_block(0) - 1) / rest_tiles_per_block(0));
-    const dim3 sgrid(c.spread_grid > 0 ? std::min(c.ntiles, c.spread_grid) : c.ntiles);
-    if (c.fp32) {
-        hipLaunchKernelGGL((k_spread_rest<true>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, F, F, d, d,
-                           c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, 0, nullptr, 0, 0, 0.0);
-        hipLaunchKernelGGL((k_spread<true>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
-                           c.tile_list, c.tile_act, c.counters, c.phi, F, F, d, c.nx, c.ny, c.ntx, c.ntiles, 0,
-                           last ? 1 : 0, nullptr);
-    } else {
-        hipLaunchKernelGGL((k_spread_rest<false>), rgrid, dim3(256), 0, c.stream, c.tile_cnt, F, F, d, d,
-                           c.tile_act, c.counters, c.nx, c.ny, c.ntx, c.ntiles, 0, nullptr, 0, 0, 0.0);
-        hipLaunchKernelGGL((k_spread<false>), sgrid, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
-                           c.tile_list, c.tile_act, c.counters, c.phi, F, F, d, c.nx, c.ny, c.ntx, c.ntiles, 0,
-                           last ? 1 : 0, nullptr);
-    }
+    const int nrest = (c.ntiles + rest_tiles_per_block(0) - 1) / rest_tiles_per_block(0);
+    const int nspread = c.spread_grid > 0 ? std::min(c.ntiles, c.spread_grid) : c.ntiles;
+    const dim3 g(nrest + nspread);
+    if (c.fp32)
+        hipLaunchKernelGGL((k_spread_step<true>), g, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
+                           c.tile_list, c.tile_act, c.counters, c.phi, F, d, c.nx, c.ny, c.ntx, c.ntiles, nrest);
+    else
+        hipLaunchKernelGGL((k_spread_step<false>), g, dim3(256), 0, c.stream, c.spread, c.tile_rec, c.tile_cnt,
+                           c.tile_list, c.tile_act, c.counters, c.phi, F, d, c.nx, c.ny, c.ntx, c.ntiles, nrest);
+}
+
+// Clear bins a spread left open when no closing interpolation follows.
+void launch_bins_close(Ctx& c) {
+    if (!c.bins_open) return;
+    if (c.tile_cnt) launch_fill_int(c, c.tile_cnt, c.ntiles, 0);
+    launch_fill_int(c, c.counters + 3, 1, 0);
+    c.bins_open = false;
 }
 
 // MDF correction it of last_it: the correction on top of F[Fcur] into the
@@ -1180,8 +1240,9 @@ void launch_mdf_fold(Ctx& c) {
     if (!c.mdf_fold_pending) return;
     hipLaunchKernelGGL(k_mdf_finish, dim3((c.ntiles + 255) / 256), dim3(256), 0, c.stream, c.mdf,
                        c.mdf_fold_ncorr, c.mdf_probe, c.K, c.ds_buf != nullptr ? 1 : 0, c.ds_cap, c.tile_cnt,
-                       c.ntiles);
+                       c.ntiles, c.counters + 3);
     c.mdf_fold_pending = false;
+    c.bins_open = false;   // the tile counts and the touched-tile count are cleared
 }
 
 // Capsule recirculation in x over a non-periodic fluid (Simulation::step 10a,
@@ -1217,11 +1278,16 @@ void launch_interp_move(Ctx& c) {
                       1.5, static_cast<double>(c.ny) - 1.5};
     MdfFold fold{};
     if (c.mdf_fold_pending) {
-        fold = MdfFold{c.mdf, c.mdf_fold_ncorr, c.mdf_probe, c.K, c.ds_buf != nullptr ? 1 : 0, c.ds_cap,
-                       c.tile_cnt, c.ntiles};
+        fold = MdfFold{c.mdf, c.mdf_fold_ncorr, c.mdf_probe, c.K, c.ds_buf != nullptr ? 1 : 0, c.ds_cap};
         c.mdf_fold_pending = false;
     }
-    launch_interp_lpn(c, c.interp_lpn, a, c.vx, c.vy, mv, fold);
+    if (c.bins_open) {
+        fold.tile_cnt = c.tile_cnt;
+        fold.ntiles = c.ntiles;
+        fold.act_cnt = c.counters + 3;
+        c.bins_open = false;
+    }
+    launch_interp_lpn(c, c.interp_lpn, a, c.vx, c.vy, mv, fold, contact_fill_ahead(c, mv.move != 0));
     if (c.any_rigid)
         hipLaunchKernelGGL(k_move, dim3(c.ncaps), dim3(64), 0, c.stream, c.x, c.y, c.vx, c.vy, c.off,
                            c.desc, c.ncaps, c.p.dt, c.periodic_x ? c.nx : 0, c.p.periodic_y ? 0 : 1,
@@ -1329,7 +1395,7 @@ __global__ __launch_bounds__(256) void kb_node_prep(const PrepPack* packs, const
     const PrepPack p = packs[w.x];
     node_prep_body(p.x, p.y, p.fx, p.fy, p.ring, nullptr, p.rec, p.K, p.nx, p.ny, p.ntx, p.nty,
                    p.tile_cnt, p.tile_list, p.tile_rec, p.counters, p.mode, p.md, p.tol, p.it, p.total,
-                   p.head, w.y);
+                   p.head, nullptr, w.y);
 }
 
 template <bool FP32>

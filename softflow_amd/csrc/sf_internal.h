@@ -102,6 +102,48 @@ __device__ __forceinline__ int wave_slot(int leader, int base) {
     return __shfl(base, leader) + (static_cast<int>(threadIdx.x & 63) - leader);
 }
 
+// The contact cell of a position: cells of cdx x cdy from (xorg, ybot),
+// x wrapped over Lx (> 0) and both indices clamped into the grid; a
+// non-finite position goes to cell 0.
+__device__ __forceinline__ int contact_cell(double px0, double py, double xorg, double Lx, double cdx,
+                                            double cdy, double ybot, int ncx, int ncy) {
+    if (isnan(px0) || isnan(py) || isinf(px0) || isinf(py)) return 0;
+    double px = px0 - xorg;
+    if (Lx > 0.0) {
+        px = fmod(px, Lx);
+        if (px < 0.0) px += Lx;
+    }
+    int ix = static_cast<int>(floor(px / cdx));
+    int iy = static_cast<int>(floor((py - ybot) / cdy));
+    if (ix < 0) ix = 0;
+    if (ix >= ncx) ix = ncx - 1;
+    if (iy < 0) iy = 0;
+    if (iy >= ncy) iy = ncy - 1;
+    return iy * ncx + ix;
+}
+
+// Listing nodes in the contact cells: node k into cell ci (< 0: none).  Every
+// lane of the wavefront calls it (wave_claim); a cell's list is in claim order
+// until the cell sort puts it in node order.
+__device__ __forceinline__ void contact_cell_claim(int* cnt, int* list, int* counters, int ci, int k) {
+    int lead, base;
+    wave_claim(cnt, ci, lead, base);
+    const int slot = wave_slot(lead, base);
+    if (ci < 0) return;
+    if (slot < CELL_MAXC) list[static_cast<size_t>(ci) * CELL_MAXC + slot] = k;
+    else atomicOr(&counters[2], 1);
+}
+
+// The contact cell list's fill done by the step's closing interpolation, at
+// the positions it moves the nodes to (cnt null: none).
+struct CellFill {
+    int* cnt;
+    int* list;
+    int* counters;
+    double xorg, Lx, cdx, cdy, ybot;
+    int ncx, ncy;
+};
+
 // Distance between two points with the minimum image over Lx (> 0).
 __device__ __forceinline__ double min_image_ds(double ax, double ay, double bx, double by, double Lx) {
     double dx = ax - bx, dy = ay - by;   // immersed_boundary.cpp:16-23
@@ -322,6 +364,8 @@ struct Ctx {
     // the step's MDF statistics fold and tile-count clear wait for its
     // closing interpolation
     bool mdf_fold_pending = false;
+    bool bins_open = false;              // a spread's tile counts / touched-tile list await clearing
+    bool cells_prefilled = false;        // the contact cell counts of cell_cur hold the current positions
     int mdf_fold_ncorr = 0;
     float* ds_buf = nullptr;             // sample rows (MLP_IN + MLP_OUT floats)
     int64_t ds_cap = 0;
@@ -340,9 +384,9 @@ struct Ctx {
     // replays only from the roles it was captured with, and the rotation
     // cycles through a few such sets
     struct GraphEntry {
-        int key[5];
+        int key[6];                      // fcur, gcur, Fcur, Fprev, cell_cur, cells_prefilled
         int len;                         // steps in the graph (1..6)
-        int out[6];                      // fcur, gcur, Fcur, Fprev, Fpart, cell_cur after them
+        int out[7];                      // fcur, gcur, Fcur, Fprev, Fpart, cell_cur, cells_prefilled after them
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
@@ -403,10 +447,13 @@ void launch_repulsion(Ctx& c);
 // (membrane forces and the cell sort in one launch), then launch_repulsion.
 bool contact_uses_cells(const Ctx& c);
 void launch_contact_fill(Ctx& c);
+CellFill contact_fill_ahead(Ctx& c, bool move);
+void drop_contact_prefill(Ctx& c);
 void launch_membrane_sort(Ctx& c);
 // node binning and the spread of the node forces into F[Fcur] (last: no MDF
 // corrections follow; mdf: the binning also clears the step's MDF state)
-void launch_spread(Ctx& c, bool last, bool mdf = false);
+void launch_spread(Ctx& c, bool mdf = false);
+void launch_bins_close(Ctx& c);
 void launch_mdf_interp(Ctx& c, int it);
 void launch_mdf_features(Ctx& c);
 // history and sample rows; leaves the statistics fold pending for the step's

@@ -430,19 +430,7 @@ __device__ __forceinline__ void pair_force(const RepArgs& a, double pax, double 
 }
 
 __device__ __forceinline__ int cell_index(const RepArgs& a, double px0, double py) {
-    if (isnan(px0) || isnan(py) || isinf(px0) || isinf(py)) return 0;
-    double px = px0 - a.xorg;
-    if (a.Lx > 0.0) {
-        px = fmod(px, a.Lx);
-        if (px < 0.0) px += a.Lx;
-    }
-    int ix = static_cast<int>(floor(px / a.cdx));
-    int iy = static_cast<int>(floor((py - a.ybot) / a.cdy));
-    if (ix < 0) ix = 0;
-    if (ix >= a.ncx) ix = a.ncx - 1;
-    if (iy < 0) iy = 0;
-    if (iy >= a.ncy) iy = a.ncy - 1;
-    return iy * a.ncx + ix;
+    return contact_cell(px0, py, a.xorg, a.Lx, a.cdx, a.cdy, a.ybot, a.ncx, a.ncy);
 }
 
 // Every lane of the wavefront takes part in the slot claims (wave_claim):
@@ -450,12 +438,7 @@ __device__ __forceinline__ int cell_index(const RepArgs& a, double px0, double p
 __device__ __forceinline__ void cells_fill_body(RepArgs a, int* cnt, int* list, int* counters, int bx) {
     const int k = bx * blockDim.x + threadIdx.x;
     const int ci = k < a.K ? cell_index(a, a.x[k], a.y[k]) : -1;
-    int lead, base;
-    wave_claim(cnt, ci, lead, base);
-    const int slot = wave_slot(lead, base);
-    if (ci < 0) return;
-    if (slot < CELL_MAXC) list[static_cast<size_t>(ci) * CELL_MAXC + slot] = k;
-    else atomicOr(&counters[2], 1);
+    contact_cell_claim(cnt, list, counters, ci, k);
 }
 
 __global__ void k_cells_fill(RepArgs a, int* cnt, int* list, int* counters) {
@@ -760,6 +743,8 @@ bool contact_uses_cells(const Ctx& c) {
     return c.ncaps > 0 && c.K > REP_ALLPAIRS_MAX && c.periodic_x;
 }
 
+// The contact cell list of the current positions: filled here, or already by
+// the previous step's closing interpolation (cells_prefilled).
 void launch_contact_fill(Ctx& c) {
     RepArgs a = rep_args(c);
     a.ncy = c.ncy; a.cdy = c.cdy;
@@ -767,10 +752,33 @@ void launch_contact_fill(Ctx& c) {
     const int nmax = c.ncx * c.ncy;
     int* cnt = c.cell_cnt + static_cast<size_t>(c.cell_cur) * nmax;
     const int bs = 128, nb = (c.K + bs - 1) / bs;
-    hipLaunchKernelGGL(k_cells_fill, dim3(nb), dim3(bs), 0, c.stream, a, cnt, c.cell_list, c.counters);
+    if (!c.cells_prefilled)
+        hipLaunchKernelGGL(k_cells_fill, dim3(nb), dim3(bs), 0, c.stream, a, cnt, c.cell_list, c.counters);
+    c.cells_prefilled = false;
     c.cell_built = cnt;
     c.cell_other = c.cell_cnt + static_cast<size_t>(1 - c.cell_cur) * nmax;
     c.cell_cur = 1 - c.cell_cur;
+}
+
+// The fill the step's closing interpolation does for the next step: the
+// count array launch_contact_fill would fill (cleared by this step's sort), at
+// the moved positions.  None without the cell list, or where the positions
+// still change after the interpolation (rigid capsules move in their own
+// kernel).
+CellFill contact_fill_ahead(Ctx& c, bool move) {
+    if (!move || !contact_uses_cells(c) || !c.cell_cnt) return CellFill{};
+    const int nmax = c.ncx * c.ncy;
+    c.cells_prefilled = true;
+    return CellFill{c.cell_cnt + static_cast<size_t>(c.cell_cur) * nmax, c.cell_list, c.counters, 0.0,
+                    static_cast<double>(c.nx), c.cdx, c.cdy, 0.5, c.ncx, c.ncy};
+}
+
+// Positions changed from the host: forget a fill done ahead.
+void drop_contact_prefill(Ctx& c) {
+    if (!c.cells_prefilled) return;
+    if (c.cell_cnt) launch_fill_int(c, c.cell_cnt + static_cast<size_t>(c.cell_cur) * c.ncx * c.ncy,
+                                    c.ncx * c.ncy, 0);
+    c.cells_prefilled = false;
 }
 
 // Membrane forces (workgroups [0, ncaps)) and the contact cell sort (the rest,
