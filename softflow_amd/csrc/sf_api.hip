@@ -192,8 +192,9 @@ static int step_ibm(Ctx& c) {
             for (int it = 1; it <= ncorr; ++it) {
                 { Scope s(c, "mdf_interp"); launch_mdf_interp(c, it); }
                 if (it == 1 && c.mdf_hist_on) {
-                    { Scope s(c, "mdf_features"); launch_mdf_features(c); }
-                    if (learned) { Scope s(c, "mlp_infer"); launch_mlp_infer(c); }
+                    Scope s(c, "mdf_warm_start");
+                    if (learned) launch_mdf_warm(c);
+                    else launch_mdf_features(c);
                 }
                 { Scope s(c, "mdf_apply"); launch_mdf_apply(c, it, ncorr); }
             }
@@ -517,10 +518,12 @@ static void free_caps(Ctx& c) {
     drop_contact_prefill(c);
     void* ptrs[] = {c.off, c.owner, c.ring, c.arc, c.desc, c.has_prev, c.x, c.y, c.vx, c.vy, c.fx, c.fy,
                     c.L0, c.th0, c.ds0, c.Lprev, c.spread, c.mdf_total, c.mdf_u0, c.mdf_hist,
-                    c.mdf_feat, c.mdf_pred, c.mdf_f1, c.mdf_dmin, c.mdf_head};
+                    c.mdf_feat, c.mdf_f1, c.mdf_dmin, c.mdf_head, c.warm_grp};
     for (void* p : ptrs) if (p) hipFree(p);
     c.mdf_total = nullptr; c.mdf_u0 = nullptr; c.mdf_hist = nullptr; c.mdf_feat = nullptr;
-    c.mdf_pred = nullptr; c.mdf_f1 = nullptr; c.mdf_dmin = nullptr; c.mdf_head = nullptr;
+    c.mdf_f1 = nullptr; c.mdf_dmin = nullptr; c.mdf_head = nullptr;
+    c.warm_grp = nullptr;
+    c.warm_ngroups = 0;
     c.off = c.owner = c.has_prev = nullptr;
     c.ring = nullptr;
     c.arc = nullptr;
@@ -949,7 +952,12 @@ int sf_set_capsules(sf_ctx* h, int ncaps, const int32_t* off, const sf_capsule_d
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.spread), K * sizeof(NodeSpread)));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_total), 2 * static_cast<size_t>(K) * sizeof(double)));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_u0), 2 * static_cast<size_t>(K) * sizeof(double)));
-    SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_pred), 2 * static_cast<size_t>(K) * sizeof(double)));
+    {
+        const std::vector<int> grp = warm_groups(off, ncaps);
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.warm_grp), grp.size() * sizeof(int)));
+        SF_HIP(hipMemcpy(c.warm_grp, grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice));
+        c.warm_ngroups = static_cast<int>(grp.size()) - 1;
+    }
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_f1), 2 * static_cast<size_t>(K) * sizeof(double)));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_dmin), static_cast<size_t>(K) * sizeof(float)));
     SF_HIP(hipMalloc(reinterpret_cast<void**>(&c.mdf_head), 2 * static_cast<size_t>(K) * sizeof(double)));

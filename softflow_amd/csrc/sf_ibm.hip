@@ -808,48 +808,14 @@ __global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double*
     mdf_interp_body<FP32, LPN>(a, lut, x, y, vx, vy, rec, K, md, it, u0, bx);
 }
 
-// Per-node inputs of the warm-start model, in FP32: slip V - U at the first
-// correction, the node's membrane + contact force, the signed discrete
-// curvature (turning angle / arc length), the distances to the nearest node of
-// another capsule and to the nearest wall plane (both capped at 2 r_cut: the
-// Peskin stencils of close neighbours overlap and a uniform correction then
-// moves the node's velocity more), and the slips of the last MLP_HIST steps.
-// The history holds measured slips, never the model's own outputs, so a
-// prediction cannot feed back into later inputs.
-// It also keeps the plain first correction dF1 (the reference frame of the
-// model's output).
-__global__ void k_mdf_features(const double* x, const double* y, const double* fx, const double* fy,
-                               const int* off, const int* owner, const double* u0,
-                               const float* hist, float* feat, const NodeSpread* rec, double* f1,
-                               const float* dmin, int K, double Lx, double ybot, double ytop,
-                               double dcap) {
+// The warm-start model's inputs of every node (mdf_node_features), and the
+// plain first correction dF1 (the reference frame of the model's output).
+__global__ void k_mdf_features(FeatArgs a, float* feat, const NodeSpread* rec, double* f1) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
+    if (k >= a.K) return;
     f1[2 * k] = rec[k].fx;
     f1[2 * k + 1] = rec[k].fy;
-    const int cap = owner[k];
-    const int o = off[cap], n = off[cap + 1] - o, i = k - o;
-    const int kp = o + (i - 1 + n) % n, kn = o + (i + 1) % n;
-    double ex0 = x[k] - x[kp], ey0 = y[k] - y[kp];
-    double ex1 = x[kn] - x[k], ey1 = y[kn] - y[k];
-    if (Lx > 0.0) {
-        ex0 -= Lx * rint(ex0 / Lx);
-        ex1 -= Lx * rint(ex1 / Lx);
-    }
-    const double th = atan2(ex0 * ey1 - ey0 * ex1, ex0 * ex1 + ey0 * ey1);
-    const double ds = 0.5 * (sqrt(ex0 * ex0 + ey0 * ey0) + sqrt(ex1 * ex1 + ey1 * ey1));
-    float* f = feat + static_cast<size_t>(k) * MLP_IN;
-    f[0] = static_cast<float>(u0[2 * k]);
-    f[1] = static_cast<float>(u0[2 * k + 1]);
-    f[2] = static_cast<float>(fx[k]);
-    f[3] = static_cast<float>(fy[k]);
-    f[4] = static_cast<float>(ds > 0.0 ? th / ds : 0.0);
-    f[5] = fminf(dmin[k], static_cast<float>(dcap));
-    f[6] = static_cast<float>(fmin(fmax(fmin(y[k] - ybot, ytop - y[k]), 0.0), dcap));
-    for (int h = 0; h < MLP_HIST; ++h) {
-        f[7 + 2 * h] = hist[(static_cast<size_t>(h) * K + k) * 2];
-        f[8 + 2 * h] = hist[(static_cast<size_t>(h) * K + k) * 2 + 1];
-    }
+    mdf_node_features(a, k, feat + static_cast<size_t>(k) * MLP_IN);
 }
 
 // Clears the per-step MDF state of a batch member (graph-safe: a kernel, not a
@@ -1215,12 +1181,15 @@ void launch_mdf_interp(Ctx& c, int it) {
 }
 
 
+FeatArgs feat_args(const Ctx& c) {
+    return FeatArgs{c.x, c.y, c.fx, c.fy, c.off, c.owner, c.mdf_u0, c.mdf_hist, c.mdf_dmin, c.K,
+                    c.periodic_x ? static_cast<double>(c.nx) : 0.0, c.p.periodic_y ? -1e30 : 0.5,
+                    c.p.periodic_y ? 1e30 : static_cast<double>(c.ny) - 1.5, 2.0 * c.p.rep_r_cut};
+}
+
 void launch_mdf_features(Ctx& c) {
-    hipLaunchKernelGGL(k_mdf_features, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.x, c.y,
-                       c.fx, c.fy, c.off, c.owner, c.mdf_u0, c.mdf_hist, c.mdf_feat, c.spread,
-                       c.mdf_f1, c.mdf_dmin, c.K, c.periodic_x ? static_cast<double>(c.nx) : 0.0,
-                       c.p.periodic_y ? -1e30 : 0.5,
-                       c.p.periodic_y ? 1e30 : static_cast<double>(c.ny) - 1.5, 2.0 * c.p.rep_r_cut);
+    hipLaunchKernelGGL(k_mdf_features, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, feat_args(c), c.mdf_feat,
+                       c.spread, c.mdf_f1);
 }
 
 void launch_mdf_record(Ctx& c, int ncorr) {

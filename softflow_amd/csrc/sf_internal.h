@@ -167,6 +167,50 @@ constexpr int MLP_HIST = 5;
 constexpr int MDF_HEAD = 2;           // plain corrections the model's output stands for
 constexpr double MDF_GAIN_MAX = 8.0;  // |gain| bound of the targets; predictions are held to [0, 4] x [-2, 2]
 
+// Per-node inputs of the warm-start model, in FP32: slip V - U at the first
+// correction, the node's membrane + contact force, the signed discrete
+// curvature (turning angle / arc length), the distances to the nearest node of
+// another capsule and to the nearest wall plane (both capped at 2 r_cut: the
+// Peskin stencils of close neighbours overlap and a uniform correction then
+// moves the node's velocity more), and the slips of the last MLP_HIST steps.
+// The history holds measured slips, never the model's own outputs, so a
+// prediction cannot feed back into later inputs.
+struct FeatArgs {
+    const double *x, *y, *fx, *fy;
+    const int *off, *owner;
+    const double* u0;       // slip at the first correction, 2 per node
+    const float* hist;      // MLP_HIST x K x 2 slips of the previous steps
+    const float* dmin;      // distance to the nearest node of another capsule
+    int K;
+    double Lx;              // > 0: x periodic over Lx
+    double ybot, ytop, dcap;
+};
+
+__device__ __forceinline__ void mdf_node_features(const FeatArgs& a, int k, float* f) {
+    const int cap = a.owner[k];
+    const int o = a.off[cap], n = a.off[cap + 1] - o, i = k - o;
+    const int kp = o + (i - 1 + n) % n, kn = o + (i + 1) % n;
+    double ex0 = a.x[k] - a.x[kp], ey0 = a.y[k] - a.y[kp];
+    double ex1 = a.x[kn] - a.x[k], ey1 = a.y[kn] - a.y[k];
+    if (a.Lx > 0.0) {
+        ex0 -= a.Lx * rint(ex0 / a.Lx);
+        ex1 -= a.Lx * rint(ex1 / a.Lx);
+    }
+    const double th = atan2(ex0 * ey1 - ey0 * ex1, ex0 * ex1 + ey0 * ey1);
+    const double ds = 0.5 * (sqrt(ex0 * ex0 + ey0 * ey0) + sqrt(ex1 * ex1 + ey1 * ey1));
+    f[0] = static_cast<float>(a.u0[2 * k]);
+    f[1] = static_cast<float>(a.u0[2 * k + 1]);
+    f[2] = static_cast<float>(a.fx[k]);
+    f[3] = static_cast<float>(a.fy[k]);
+    f[4] = static_cast<float>(ds > 0.0 ? th / ds : 0.0);
+    f[5] = fminf(a.dmin[k], static_cast<float>(a.dcap));
+    f[6] = static_cast<float>(fmin(fmax(fmin(a.y[k] - a.ybot, a.ytop - a.y[k]), 0.0), a.dcap));
+    for (int h = 0; h < MLP_HIST; ++h) {
+        f[7 + 2 * h] = a.hist[(static_cast<size_t>(h) * a.K + k) * 2];
+        f[8 + 2 * h] = a.hist[(static_cast<size_t>(h) * a.K + k) * 2 + 1];
+    }
+}
+
 // The residual maximum of an iteration is kept in MDF_RES_SUB sub-slots on
 // their own cache lines (workgroup b updates sub-slot b mod MDF_RES_SUB), so
 // the interpolation's workgroups do not all queue on one word; mdf_res folds
@@ -355,7 +399,8 @@ struct Ctx {
     double* mdf_total = nullptr;         // 2K: corrections spread this step, per node
     double* mdf_head = nullptr;          // 2K: sum of the first MDF_HEAD corrections this step
     double* mdf_u0 = nullptr;            // 2K: slip V - U at the first correction
-    double* mdf_pred = nullptr;          // 2K: the model's gains before filtering
+    int* warm_grp = nullptr;             // capsule groups of the fused warm start (warm_ngroups + 1)
+    int warm_ngroups = 0;
     double* mdf_f1 = nullptr;            // 2K: the plain first correction dF1
     float* mdf_dmin = nullptr;           // K: distance to the nearest other capsule's node
     float* mdf_hist = nullptr;           // MLP_HIST x 2K: slips of the previous steps
@@ -455,6 +500,7 @@ void launch_membrane_sort(Ctx& c);
 void launch_spread(Ctx& c, bool mdf = false);
 void launch_bins_close(Ctx& c);
 void launch_mdf_interp(Ctx& c, int it);
+FeatArgs feat_args(const Ctx& c);
 void launch_mdf_features(Ctx& c);
 // history and sample rows; leaves the statistics fold pending for the step's
 // closing interpolation (launch_mdf_fold runs it on its own)
@@ -464,7 +510,8 @@ void launch_mdf_fold(Ctx& c);
 // correction spread (touched tiles) and the untouched tiles' copies in one
 // launch; rotates the force roles as launch_spread mode 1 does
 void launch_mdf_apply(Ctx& c, int it, int last_it);
-void launch_mlp_infer(Ctx& c);
+void launch_mdf_warm(Ctx& c);
+std::vector<int> warm_groups(const int32_t* off, int ncaps);
 void launch_interp_move(Ctx& c);
 // The step's interpolation at lpn lanes per node without the move, writing
 // the velocities into vx/vy (the autotuner's timing run).

@@ -110,19 +110,24 @@ struct MlpOut {
     bool ok;           // all inputs of node j finite
 };
 
-// The network on nodes k0 .. k0+15 of x (rows of MLP_IN, clipped at n).
-// Every lane returns the result of its node j = l % 16.
-__device__ MlpOut mlp_wave(const float* __restrict__ packed, const float* __restrict__ P, const float* x,
-                           int k0, int n) {
-    const int l = threadIdx.x & 63, q = l >> 4, j = l & 15;
-    const int node = k0 + j;
-    float w[MLP_SLOTS];
+// This lane's slots of the packed copy, into registers.
+__device__ __forceinline__ void mlp_weights(const float* __restrict__ packed, float (&w)[MLP_SLOTS]) {
+    const int l = threadIdx.x & 63;
     const f32x4* pk = reinterpret_cast<const f32x4*>(packed);
 #pragma unroll
     for (int u = 0; u < MLP_SLOTS / 4; ++u) {
         const f32x4 v = pk[u * 64 + l];
         w[4 * u] = v[0]; w[4 * u + 1] = v[1]; w[4 * u + 2] = v[2]; w[4 * u + 3] = v[3];
     }
+}
+
+// The network on nodes k0 .. k0+15 of x (rows of MLP_IN, clipped at n; x may
+// point into LDS), with this lane's weight slots w.  Every lane returns the
+// result of its node j = l % 16.
+__device__ __forceinline__ MlpOut mlp_tile(const float (&w)[MLP_SLOTS], const float* __restrict__ P,
+                                           const float* x, int k0, int n) {
+    const int l = threadIdx.x & 63, q = l >> 4, j = l & 15;
+    const int node = k0 + j;
     float xh[K1_STEPS];
     int bad = 0;
 #pragma unroll
@@ -171,31 +176,26 @@ __device__ MlpOut mlp_wave(const float* __restrict__ packed, const float* __rest
     return MlpOut{P[B3_OFF] + o0, P[B3_OFF + 1] + o1, bad == 0};
 }
 
-// Inference inside the step: the network's two outputs are the gains of the
-// node's total correction in the frame of its first plain correction dF1,
-// T = g_par dF1 + g_perp rot90(dF1).  They go to gain[2k], held to
-// g_par in [0, 4] and g_perp in [-2, 2], and default to (1, 0), the plain
-// correction, where an input or an output is not finite.  Nothing
-// happens when the residual is already within tolerance.
-__global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ packed, const float* __restrict__ P,
-                                                  const float* __restrict__ norm, const float* feat, double* gain,
-                                                  int K, const MdfDev* md, double tol) {
-    if (tol > 0.0 && mdf_res(md, 1) <= tol) return;
-    const int k0 = blockIdx.x * MLP_NODES;
-    const MlpOut r = mlp_wave(packed, P, feat, k0, K);
-    const int l = threadIdx.x, k = k0 + (l & 15);
-    if (l < MLP_NODES && k < K) {
-        double g0 = 1.0, g1 = 0.0;
-        if (r.ok) {
-            const float gp = norm[2 * MLP_IN] + norm[2 * MLP_IN + MLP_OUT] * r.y0;
-            const float gq = norm[2 * MLP_IN + 1] + norm[2 * MLP_IN + MLP_OUT + 1] * r.y1;
-            if (isfinite(gp) && isfinite(gq)) {
-                g0 = fminf(fmaxf(gp, 0.f), 4.f);
-                g1 = fminf(fmaxf(gq, -2.f), 2.f);
-            }
+__device__ MlpOut mlp_wave(const float* __restrict__ packed, const float* __restrict__ P, const float* x,
+                           int k0, int n) {
+    float w[MLP_SLOTS];
+    mlp_weights(packed, w);
+    return mlp_tile(w, P, x, k0, n);
+}
+
+// The gains of one node from the network's outputs (the in-step inference): held to
+// g_par in [0, 4] and g_perp in [-2, 2], (1, 0) where an input or an output is
+// not finite.
+__device__ __forceinline__ void mlp_gains(const MlpOut& r, const float* __restrict__ norm, double& g0, double& g1) {
+    g0 = 1.0;
+    g1 = 0.0;
+    if (r.ok) {
+        const float gp = norm[2 * MLP_IN] + norm[2 * MLP_IN + MLP_OUT] * r.y0;
+        const float gq = norm[2 * MLP_IN + 1] + norm[2 * MLP_IN + MLP_OUT + 1] * r.y1;
+        if (isfinite(gp) && isfinite(gq)) {
+            g0 = fminf(fmaxf(gp, 0.f), 4.f);
+            g1 = fminf(fmaxf(gq, -2.f), 2.f);
         }
-        gain[2 * k] = g0;
-        gain[2 * k + 1] = g1;
     }
 }
 
@@ -217,26 +217,84 @@ __global__ __launch_bounds__(64) void k_mlp_predict(const float* __restrict__ pa
 // would otherwise alternate between neighbours: that zig-zag barely moves the
 // interpolated velocity, so the residual cannot see it, yet it pushes the
 // fluid.
-__global__ void k_mdf_smooth(const double* gain, NodeSpread* rec, const int* off, const int* owner,
-                             const double* x, const double* y, int K, int ny, const MdfDev* md, double tol) {
-    if (tol > 0.0 && mdf_res(md, 1) <= tol) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
+// Node k's smoothed gains scale its record's dF1; gain[2 m] holds node m's
+// gains for m in [base, base + count) (the node's capsule lies inside).
+__device__ __forceinline__ void mdf_smooth_node(const double* gain, int base, NodeSpread* rec, const int* off,
+                                                const int* owner, const double* x, const double* y, int ny, int k) {
     const int cap = owner[k];
     const int o = off[cap], n = off[cap + 1] - o, i = k - o;
-    double gp = gain[2 * k], gq = gain[2 * k + 1];
+    double gp = gain[2 * (k - base)], gq = gain[2 * (k - base) + 1];
     if (n >= 5) {
         const int j[5] = {o + (i - 2 + n) % n, o + (i - 1 + n) % n, k, o + (i + 1) % n, o + (i + 2) % n};
         const double w[5] = {1.0, 4.0, 6.0, 4.0, 1.0};
         gp = 0.0;
         gq = 0.0;
-        for (int t = 0; t < 5; ++t) { gp += w[t] * gain[2 * j[t]]; gq += w[t] * gain[2 * j[t] + 1]; }
+        for (int t = 0; t < 5; ++t) {
+            gp += w[t] * gain[2 * (j[t] - base)];
+            gq += w[t] * gain[2 * (j[t] - base) + 1];
+        }
         gp *= 0.0625;
         gq *= 0.0625;
     }
     const NodeSpread r = rec[k];    // dF1
     // the whole record, valid where the correction is finite
     rec[k] = node_record(x[k], y[k], gp * r.fx - gq * r.fy, gp * r.fy + gq * r.fx, r.ds, k, ny);
+}
+
+// The learned warm start in one launch, one workgroup per group of whole
+// capsules (at most WARM_NODES nodes, or one larger capsule): every node's
+// inputs (mdf_node_features, also kept for the sample rows) and dF1 into
+// f1; then, unless the residual is already within tolerance, the network on
+// 16-node tiles of the group (one wavefront per tile, the weights loaded into
+// registers once per wavefront), the gains held in LDS, and the smoothed
+// correction into each node's record.  Everything is the three-kernel path's
+// arithmetic as the network alone (k_mlp_predict): each node's network output
+// depends on its own inputs only, whichever tile holds it.
+constexpr int WARM_NODES = 64;           // nodes a group of several capsules may hold
+constexpr int WARM_MAX = 256;            // nodes of the largest capsule
+struct WarmArgs {
+    FeatArgs fa;
+    const int* grp;                      // first capsule of each group, ngroups + 1 entries
+    float* feat;
+    double* f1;
+    NodeSpread* rec;
+    const float *packed, *P, *norm;
+    const MdfDev* md;
+    double tol;
+    int ny;
+};
+
+__global__ __launch_bounds__(256) void k_mdf_warm(WarmArgs a) {
+    __shared__ float sx[WARM_MAX * MLP_IN];
+    __shared__ double sg[2 * WARM_MAX];
+    const int g = blockIdx.x;
+    const int base = a.fa.off[a.grp[g]], n = a.fa.off[a.grp[g + 1]] - base;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int k = base + t;
+        a.f1[2 * k] = a.rec[k].fx;
+        a.f1[2 * k + 1] = a.rec[k].fy;
+        float f[MLP_IN];
+        mdf_node_features(a.fa, k, f);
+#pragma unroll
+        for (int j = 0; j < MLP_IN; ++j) {
+            sx[t * MLP_IN + j] = f[j];
+            a.feat[static_cast<size_t>(k) * MLP_IN + j] = f[j];
+        }
+    }
+    if (a.tol > 0.0 && mdf_res(a.md, 1) <= a.tol) return;    // the same for the whole grid
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (wv * MLP_NODES < n) {
+        float w[MLP_SLOTS];
+        mlp_weights(a.packed, w);
+        for (int t0 = wv * MLP_NODES; t0 < n; t0 += 4 * MLP_NODES) {
+            const MlpOut r = mlp_tile(w, a.P, sx, t0, n);
+            if (l < MLP_NODES && t0 + l < n) mlp_gains(r, a.norm, sg[2 * (t0 + l)], sg[2 * (t0 + l) + 1]);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x)
+        mdf_smooth_node(sg, base, a.rec, a.fa.off, a.fa.owner, a.fa.x, a.fa.y, a.ny, base + t);
 }
 
 // Forward + backward of GB samples; the workgroup's gradient of
@@ -405,13 +463,26 @@ __global__ void k_colstats(const float* rows, int64_t nrows, int64_t chunk, doub
     }
 }
 
-void launch_mlp_infer(Ctx& c) {
+// The first correction's warm start (k_mdf_warm), in place of launch_mdf_features.
+void launch_mdf_warm(Ctx& c) {
     if (!c.model || c.K == 0) return;
-    hipLaunchKernelGGL(k_mlp_infer, dim3((c.K + MLP_NODES - 1) / MLP_NODES), dim3(64), 0, c.stream,
-                       c.model->packed, c.model->params, c.model->norm, c.mdf_feat, c.mdf_pred, c.K, c.mdf,
-                       c.mdf_tol);
-    hipLaunchKernelGGL(k_mdf_smooth, dim3((c.K + 255) / 256), dim3(256), 0, c.stream, c.mdf_pred,
-                       c.spread, c.off, c.owner, c.x, c.y, c.K, c.ny, c.mdf, c.mdf_tol);
+    const WarmArgs a{feat_args(c), c.warm_grp, c.mdf_feat, c.mdf_f1, c.spread, c.model->packed, c.model->params,
+                     c.model->norm, c.mdf, c.mdf_tol, c.ny};
+    hipLaunchKernelGGL(k_mdf_warm, dim3(c.warm_ngroups), dim3(256), 0, c.stream, a);
+}
+
+// Capsule groups of the warm start: consecutive capsules while their nodes
+// fit WARM_NODES (a larger capsule alone).
+std::vector<int> warm_groups(const int32_t* off, int ncaps) {
+    std::vector<int> grp{0};
+    int first = 0;
+    for (int i = 1; i <= ncaps; ++i) {
+        if (i == ncaps || off[i + 1] - off[first] > WARM_NODES) {
+            grp.push_back(i);
+            first = i;
+        }
+    }
+    return grp;
 }
 
 static int ensure_partial(Mlp& m, int nblk) {
