@@ -265,6 +265,32 @@ def len_prime(steps):
     return min(steps, GRAPH_PERIOD) + (GRAPH_PERIOD if r and steps > GRAPH_PERIOD else (GRAPH_PERIOD - r if r else 0))
 
 
+def mdf_iteration_costs(steps=1200, develop=300):
+    """The parts of an MDF step on the MDF channel, for the cost model
+    T = B + n A + (max - n) I: A, one applied correction (interpolation +
+    correction launch), from an unreachable tolerance with 4 against 1
+    corrections; I, one iteration after convergence (both launches exit at
+    once), from a tolerance every step meets at once with 16 against 1; and
+    the step with one applied correction.  FP64 and FP32, each developed
+    ``develop`` steps with the reference's 5-iteration MDF, graph replay."""
+    out = {}
+    for prec in ("fp64", "fp32"):
+        sim, _ = build_mdf_channel(prec, 5, USE_GRAPH)
+        sim.run(develop)
+
+        def per_step(tol, mx):
+            sim.setMdfControl(tol, mx, True)
+            sim.run(6)
+            dt, _ = timed_run(sim, steps)
+            return dt / steps * 1e6
+        t1, t16 = per_step(1.0, 1), per_step(1.0, 16)
+        a1, a4 = per_step(1e-300, 1), per_step(1e-300, 4)
+        out[prec] = {"applied_correction_us": round((a4 - a1) / 3, 2),
+                     "iteration_after_convergence_us": round((t16 - t1) / 15, 2),
+                     "step_with_one_correction_us": round(a1, 1)}
+    return out
+
+
 def mdf_stability_bench(steps=4000, chunk=250):
     """The reference's plain 5-iteration MDF (FP64, the oracle's arithmetic) on
     the MDF channel at the original specification's body force 3e-5, with
@@ -523,7 +549,8 @@ def main():
     ap.add_argument("--output-steps", type=int, default=500)
     ap.add_argument("--no-geometry", action="store_true",
                     help="skip the pillar-array measurement")
-    ap.add_argument("--only", choices=["ensemble", "mdf", "mdf-stability", "output", "geometry"], default=None,
+    ap.add_argument("--only", choices=["ensemble", "mdf", "mdf-costs", "mdf-stability", "output", "geometry"],
+                    default=None,
                     help="run only the batched-ensemble or the learned-vs-plain MDF measurement "
                          "and print its object as the JSON line (for profiling)")
     args = ap.parse_args()
@@ -544,6 +571,8 @@ def main():
         elif args.only == "mdf":
             part = mdf_learned_bench(args.mdf_steps, args.mdf_train_steps, 6, not args.no_mdf_curve,
                                      args.mdf_develop)
+        elif args.only == "mdf-costs":
+            part = mdf_iteration_costs()
         elif args.only == "mdf-stability":
             part = mdf_stability_bench()
         elif args.only == "geometry":

@@ -1233,10 +1233,10 @@ __global__ void k_recirculate_x(double* x, const int* off, int ncaps, double Lx)
         for (int i = 0; i < n; ++i) x[o + i] += shift;
 }
 
-void launch_interp_probe(Ctx& c, int lpn, double* vx, double* vy) {
+void launch_interp_probe(Ctx& c, int lpn, double* vx, double* vy, const CellFill& cf) {
     if (c.K == 0) return;
     const MoveArgs mv{0, c.p.dt, 0, 0, 0.0, 0.0};
-    launch_interp_lpn(c, lpn, vel_args(c, c.f[c.fcur], c.F[c.Fcur]), vx, vy, mv, MdfFold{});
+    launch_interp_lpn(c, lpn, vel_args(c, c.f[c.fcur], c.F[c.Fcur]), vx, vy, mv, MdfFold{}, cf);
 }
 
 void launch_interp_move(Ctx& c) {

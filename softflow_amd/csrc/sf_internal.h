@@ -515,7 +515,7 @@ std::vector<int> warm_groups(const int32_t* off, int ncaps);
 void launch_interp_move(Ctx& c);
 // The step's interpolation at lpn lanes per node without the move, writing
 // the velocities into vx/vy (the autotuner's timing run).
-void launch_interp_probe(Ctx& c, int lpn, double* vx, double* vy);
+void launch_interp_probe(Ctx& c, int lpn, double* vx, double* vy, const CellFill& cf = CellFill{});
 void launch_fill_int(Ctx& c, int* p, int n, int v);
 void launch_fill_zero(Ctx& c, void* p, size_t bytes);
 void probe_math(int op, int n, const double* a, const double* b, double* out);
