@@ -955,11 +955,19 @@ __device__ __forceinline__ void interp_body(CellVelArgs a, const double* lut, do
         }
     }
     if (cf.cnt) {
-        // the next step's contact cell list, from the moved position
-        const int ci = live && l == 0
-                           ? contact_cell(px, py, cf.xorg, cf.Lx, cf.cdx, cf.cdy, cf.ybot, cf.ncx, cf.ncy)
-                           : -1;
-        contact_cell_claim(cf.cnt, cf.list, cf.counters, ci, k);
+        // the next step's contact cell list, from the moved positions: the
+        // workgroup's nodes claim their slots together from its first
+        // wavefront, in node order, so a run of consecutive nodes in one cell
+        // shares an atomic across the whole workgroup
+        static_assert(NPW <= 64, "one wavefront claims for the workgroup");
+        __shared__ int s_cell[NPW];
+        if (l == 0)
+            s_cell[g] = live ? contact_cell(px, py, cf.xorg, cf.Lx, cf.cdx, cf.cdy, cf.ybot, cf.ncx, cf.ncy) : -1;
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int i = static_cast<int>(threadIdx.x);
+            contact_cell_claim(cf.cnt, cf.list, cf.counters, i < NPW ? s_cell[i] : -1, bx * NPW + i);
+        }
     }
 }
 
