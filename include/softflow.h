@@ -268,6 +268,13 @@ int  sf_set_capsule_launch(sf_ctx* ctx, int spread_grid, int interp_lanes);
  * final correction (for statistics).  Residual mode runs MDF even when
  * ibm_iterations is 1. */
 int  sf_mdf_set_control(sf_ctx* ctx, double tol, int max_corrections, int probe);
+/* Residual mode: MDF iterations first .. max_corrections run in one launch
+ * (default first = 6; 0: every iteration is its own launch pair).  That
+ * launch leaves at once when the loop has stopped before first, so a cap well
+ * above the corrections a step needs costs one launch instead of two per
+ * unused iteration; the iterations it runs give the same results bit for bit.
+ * No counterpart in the reference (an extension of sf_mdf_set_control). */
+int  sf_mdf_set_tail(sf_ctx* ctx, int first);
 /* out4: mean corrections per step, mean and max final residual, steps; since
  * the last call (which resets them). */
 int  sf_mdf_stats(sf_ctx* ctx, double* out4);

@@ -30,7 +30,7 @@ EXPORTS = (
     "sf_stage", "sf_last_capped", "sf_field_stats", "sf_field_stats_noalloc", "sf_global_stats", "sf_step_count", "sf_kernel_times",
     "sf_host_alloc", "sf_host_free", "sf_set_probes", "sf_snapshot_async", "sf_snapshot_wait",
     "sf_set_timing", "sf_autotune", "sf_set_lbm_launch", "sf_set_lbm_pairs", "sf_set_capsule_launch", "sf_probe_math", "sf_copy_ceiling",
-    "sf_mdf_set_control", "sf_mdf_stats", "sf_mdf_collect", "sf_mdf_samples", "sf_mdf_set_model",
+    "sf_mdf_set_control", "sf_mdf_set_tail", "sf_mdf_stats", "sf_mdf_collect", "sf_mdf_samples", "sf_mdf_set_model",
     "sf_mlp_create", "sf_mlp_destroy", "sf_mlp_num_params", "sf_mlp_get_params", "sf_mlp_set_params",
     "sf_mlp_predict", "sf_mlp_loss_grad", "sf_mlp_train",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_step", "sf_batch_synchronize",
@@ -129,6 +129,7 @@ def load():
         "sf_probe_math": [C.c_int, C.c_int, _DP, _DP, _DP],
         "sf_copy_ceiling": [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_double)],
         "sf_mdf_set_control": [C.c_void_p, C.c_double, C.c_int, C.c_int],
+        "sf_mdf_set_tail": [C.c_void_p, C.c_int],
         "sf_mdf_stats": [C.c_void_p, _DP],
         "sf_mdf_collect": [C.c_void_p, C.c_int64],
         "sf_mdf_samples": [C.c_void_p, C.POINTER(C.c_int64), _FP],

@@ -533,6 +533,7 @@ PYBIND11_MODULE(softflow_core, m) {
         .def("kernelTimes", &Simulation::kernelTimes)
         .def("setMdfControl", &Simulation::setMdfControl, py::arg("tolerance"),
              py::arg("max_corrections") = 8, py::arg("probe") = false)
+        .def("setMdfTail", &Simulation::setMdfTail, py::arg("first"))
         .def("mdfStats", [](Simulation& s) {
             auto v = s.mdfStats();
             py::dict d;

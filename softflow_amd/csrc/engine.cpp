@@ -678,6 +678,7 @@ void Simulation::initialize() {
     step_count_ = 0;
     pushDirty();
     check(sf_mdf_set_control(ctx_, mdf_tol_, mdf_max_, mdf_probe_ ? 1 : 0), "sf_mdf_set_control");
+    check(sf_mdf_set_tail(ctx_, mdf_tail_), "sf_mdf_set_tail");
     if (mdf_collect_ > 0) check(sf_mdf_collect(ctx_, mdf_collect_), "sf_mdf_collect");
     if (mdf_model_) check(sf_mdf_set_model(ctx_, mdf_model_->handle(), mdf_warm_ ? 1 : 0), "sf_mdf_set_model");
     std::cout << "Simulation initialized: " << nx << "x" << ny << " lattice, " << caps_.caps_.size()
@@ -968,6 +969,11 @@ void Simulation::setMdfControl(Real tol, int max_corrections, bool probe) {
     mdf_max_ = max_corrections;
     mdf_probe_ = probe;
     if (ctx_) check(sf_mdf_set_control(ctx_, tol, max_corrections, probe ? 1 : 0), "sf_mdf_set_control");
+}
+
+void Simulation::setMdfTail(int first) {
+    mdf_tail_ = first;
+    if (ctx_) check(sf_mdf_set_tail(ctx_, first), "sf_mdf_set_tail");
 }
 
 std::vector<double> Simulation::mdfStats() {

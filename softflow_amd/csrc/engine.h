@@ -486,6 +486,8 @@ public:
 
     // multi-direct forcing control and the learned warm start (extensions)
     void setMdfControl(Real tol, int max_corrections, bool probe);
+    // residual mode: iterations from `first` on in one launch (0: one launch pair each)
+    void setMdfTail(int first);
     std::vector<double> mdfStats();
     void collectMdfSamples(int64_t capacity);
     int64_t mdfSampleCount();
@@ -526,6 +528,7 @@ private:
     Real mdf_tol_ = 0.0;
     int mdf_max_ = 0;
     bool mdf_probe_ = false;
+    int mdf_tail_ = 6;
     int64_t mdf_collect_ = 0;
     MdfModel* mdf_model_ = nullptr;
     bool mdf_warm_ = false;

@@ -189,7 +189,18 @@ static int step_ibm(Ctx& c) {
     if (ncorr > 0) {
         if (c.K > 0) {
             const bool learned = c.mdf_warm && c.model;
+            // residual mode: the iterations from mdf_tail_from on run in one
+            // launch, which leaves at once when the loop has already stopped
+            const int tail = c.mdf_tol > 0.0 && c.mdf_tail_from >= 2 && ncorr - c.mdf_tail_from >= 1 &&
+                                     mdf_tail_ready(c)
+                                 ? c.mdf_tail_from
+                                 : 0;
             for (int it = 1; it <= ncorr; ++it) {
+                if (it == tail) {
+                    Scope s(c, "mdf_tail");
+                    launch_mdf_tail(c, it, ncorr);
+                    break;
+                }
                 { Scope s(c, "mdf_interp"); launch_mdf_interp(c, it); }
                 if (it == 1 && c.mdf_hist_on) {
                     Scope s(c, "mdf_warm_start");
@@ -338,6 +349,7 @@ static int counters_verdict(Ctx& c) {
     if (c.host_counters[1] & 1) return fail(SF_E_CAPACITY, "IBM tile node list overflowed (more than 1024 nodes near one 32x8 tile)");
     if (c.host_counters[2] & 1) return fail(SF_E_CAPACITY, "contact cell list overflowed (more than 32 nodes in one cell)");
     if (c.host_counters[2] & 2) return fail(SF_E_CAPACITY, "contact cell grid exceeded its capacity");
+    if (c.host_counters[2] & 8) return fail(SF_E_HIP, "MDF correction loop: a grid barrier timed out");
     return SF_OK;
 }
 
@@ -592,6 +604,7 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     chk(dmalloc(&c.counters, 64 * sizeof(int)));
     chk(dmalloc(&c.phi, 1025 * 8));
     chk(dmalloc(&c.mdf, sizeof(MdfDev)));
+    chk(dmalloc(&c.mdf_bar, 64 * sizeof(unsigned)));
     if (rc != SF_OK) { sf_destroy(h); return rc; }
     SF_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.host_counters), 64 * sizeof(int)));
     {   // the allocation-free entry point's scratch address on this device
@@ -628,6 +641,7 @@ int sf_create(sf_ctx** out, int device, const sf_params* p) {
     for (int b = 0; b < 3; ++b) SF_HIP(hipMemset(c.F[b], 0, 2 * N * es));
     SF_HIP(hipMemset(c.counters, 0, 64 * sizeof(int)));
     SF_HIP(hipMemset(c.mdf, 0, sizeof(MdfDev)));
+    SF_HIP(hipMemset(c.mdf_bar, 0, 64 * sizeof(unsigned)));
     c.Fcur = 0; c.Fprev = 1; c.Fpart = 2;
     double tab[1025];
     for (int i = 0; i <= 1024; ++i) tab[i] = phi_exact_host(2.0 * static_cast<double>(i) / 1024);
@@ -677,7 +691,7 @@ int sf_destroy(sf_ctx* h) {
     free_caps(c);
     void* ptrs[] = {c.flags, c.f[0], c.f[1], c.F[0], c.F[1], c.F[2], c.stored_rho, c.stored_u,
                     c.counters, c.phi, c.tile_cnt, c.tile_list, c.tile_act, c.tile_rec, c.tile_dirty, c.cell_cnt, c.cell_list,
-                    c.cell_pay, c.cell_own, c.mdf, c.ds_buf, c.obs, c.obs_v, c.ibb_cell, c.ibb_off,
+                    c.cell_pay, c.cell_own, c.mdf, c.mdf_bar, c.ds_buf, c.obs, c.obs_v, c.ibb_cell, c.ibb_off,
                     c.ibb_q, c.ibb_up, c.ibb_qf, c.snap_dev, c.probe_cells, c.nbm};
     delete c.snap_sync;     // the stream has drained: no host function is pending
     c.snap_sync = nullptr;
@@ -1586,6 +1600,17 @@ int sf_mdf_set_control(sf_ctx* h, double tol, int max_corrections, int probe) {
     c.mdf_tol = tol > 0.0 ? tol : 0.0;
     c.mdf_max_corr = tol > 0.0 ? max_corrections : 0;
     c.mdf_probe = probe ? 1 : 0;
+    SF_HIP(hipSetDevice(c.device));
+    mdf_tail_size(c);
+    invalidate_graph(c);
+    return SF_OK;
+}
+
+int sf_mdf_set_tail(sf_ctx* h, int first) {
+    if (!h) return fail(SF_E_ARG, "null pointer");
+    if (first != 0 && (first < 2 || first > MDF_MAX_CORR)) return fail(SF_E_ARG, "first must be 0 or in 2..32");
+    Ctx& c = h->c;
+    c.mdf_tail_from = first;
     invalidate_graph(c);
     return SF_OK;
 }

@@ -261,9 +261,11 @@ __device__ __forceinline__ void spread_rest_body(const int* tile_cnt, const void
                                                      int2* act, int* counters, int nx, int ny,
                                                      int ntx, int ntiles, int mode,
                                                      const MdfDev* md, int bx, int it = 0, int last_it = 0,
-                                                     double tol = 0.0) {
+                                                     double tol = 0.0, int dec_given = -1) {
     using T = typename std::conditional<FP32, float, double>::type;
-    const int dec = mode == 1 && md ? mdf_decide(md, it, tol) : 0;
+    // dec_given >= 0: the decision, known to the caller (the correction loop's
+    // tail, which skips the iterations after the one that stopped it)
+    const int dec = dec_given >= 0 ? dec_given : (mode == 1 && md ? mdf_decide(md, it, tol) : 0);
     const bool skip = dec != 0;
     if (skip && last_it > 0) {
         if (it < last_it) return;
@@ -808,6 +810,122 @@ __global__ __launch_bounds__(256) void k_mdf_interp(CellVelArgs a, const double*
     mdf_interp_body<FP32, LPN>(a, lut, x, y, vx, vy, rec, K, md, it, u0, bx);
 }
 
+// ---------------------------------------------------------------------------
+// The correction loop's tail: iterations first .. last of a residual-controlled
+// MDF step in one launch.  Once the loop has stopped, the separate launches of
+// the remaining iterations would each cost their dispatch while doing nothing
+// but carry the decision forward; here the launch finds the decision at its
+// start and leaves.  Each iteration that still runs is the separate launches'
+// work -- the interpolation (mdf_interp_body) over node blocks, then the
+// correction (spread_rest_body, spread_body, mdf_node_apply) -- with the
+// same arguments, so results are the same bit for bit; a grid barrier stands
+// where the kernel boundary stood.  The grid holds only as many workgroups as
+// are resident at once (the launcher sizes it), which the barrier needs.
+//
+// The barrier: the last workgroup to arrive resets the count and advances the
+// generation; the others wait for the generation to move, for at most `spin`
+// polls (then bit 3 of *flag is raised and the workgroup leaves: the results
+// are void and the host reports the failure).  The agent-scope release before
+// arriving and acquire after leaving make each workgroup's global writes
+// visible to the others across the XCDs' L2 caches.  bar[0] (count) and
+// bar[32] (generation) sit on their own cache lines; the count is 0 between
+// barriers.
+__device__ bool grid_barrier(unsigned* bar, int* flag, unsigned spin) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned* count = bar;
+        unsigned* gen = bar + 32;
+        const unsigned g0 = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        int ok = 1;
+        if (arrived == gridDim.x) {
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned n = 0;
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++n > spin) { ok = 0; break; }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (!ok) atomicOr(flag, 8);
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+struct TailArgs {
+    CellVelArgs a;                       // a.F: set per iteration
+    const double *lut, *x, *y, *vx, *vy;
+    NodeSpread* rec;
+    int K, nblk_interp;                  // nodes; node blocks of the interpolation
+    int* tile_cnt;
+    const int* tile_list;
+    int2* act;
+    int* counters;
+    void *FP, *FQ;                       // the correction ping-pong: first reads FP, writes FQ
+    uint8_t *dP, *dQ;                    // their tile flags
+    int nx, ny, ntx, ntiles, nrest;
+    MdfDev* md;
+    double tol;
+    int first, last;
+    double *total, *head;
+    unsigned* bar;
+    unsigned spin;
+};
+
+template <bool FP32, int LPN>
+__global__ __launch_bounds__(256) void k_mdf_tail(TailArgs t) {
+    const int G = static_cast<int>(gridDim.x);
+    const int nnode = (t.K + 255) / 256;
+    const int napply = t.nrest + G + nnode;
+    for (int it = t.first; it <= t.last; ++it) {
+        const bool odd = ((it - t.first) & 1) != 0;
+        void* Fin = odd ? t.FQ : t.FP;
+        void* Fout = odd ? t.FP : t.FQ;
+        uint8_t* din = odd ? t.dQ : t.dP;
+        uint8_t* dout = odd ? t.dP : t.dQ;
+        // written before this launch, or before the last barrier
+        const int dec = __hip_atomic_load(&t.md->cby[it - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (dec) {
+            // stopped: the decision holds for the remaining iterations, and
+            // the last iteration's copy keeps both buffers of the ping-pong
+            // equal (spread_rest_body at last, as its own launch would run it)
+            if (blockIdx.x == 0 && threadIdx.x == 0)
+                for (int j = it; j <= t.last; ++j) t.md->cby[j] = dec;
+            const bool lodd = ((t.last - t.first) & 1) != 0;
+            for (int b = blockIdx.x; b < t.nrest; b += G)
+                spread_rest_body<FP32>(t.tile_cnt, lodd ? t.FQ : t.FP, lodd ? t.FP : t.FQ, lodd ? t.dQ : t.dP,
+                                       lodd ? t.dP : t.dQ, t.act, t.counters, t.nx, t.ny, t.ntx, t.ntiles, 1, t.md,
+                                       b, t.last, t.last, t.tol, dec);
+            return;
+        }
+        CellVelArgs a = t.a;
+        a.F = Fin;
+        for (int b = blockIdx.x; b < t.nblk_interp; b += G) {
+            mdf_interp_body<FP32, LPN>(a, t.lut, t.x, t.y, t.vx, t.vy, t.rec, t.K, t.md, it, nullptr, b);
+            __syncthreads();   // the body's LDS is reused by the next block
+        }
+        if (!grid_barrier(t.bar, t.counters + 2, t.spin)) return;
+        for (int b = blockIdx.x; b < napply; b += G) {
+            if (b < t.nrest)
+                spread_rest_body<FP32>(t.tile_cnt, Fin, Fout, din, dout, t.act, t.counters, t.nx, t.ny, t.ntx,
+                                       t.ntiles, 1, t.md, b, it, t.last, t.tol);
+            else if (b < t.nrest + G)
+                spread_body<FP32>(t.rec, nullptr, t.tile_cnt, t.tile_list, t.act, t.counters, t.lut, Fin, Fout, dout,
+                                  t.nx, t.ny, t.ntx, t.ntiles, 1, 0, t.md, b - t.nrest, G, it, t.tol);
+            else
+                mdf_node_apply(t.md, t.total, t.head, t.rec, t.K, it, t.tol,
+                               (b - t.nrest - G) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x));
+        }
+        if (!grid_barrier(t.bar, t.counters + 2, t.spin)) return;
+    }
+}
+
 // The warm-start model's inputs of every node (mdf_node_features), and the
 // plain first correction dF1 (the reference frame of the model's output).
 __global__ void k_mdf_features(FeatArgs a, float* feat, const NodeSpread* rec, double* f1) {
@@ -1162,6 +1280,71 @@ void launch_mdf_apply(Ctx& c, int it, int last_it) {
                            c.mdf_tol, it, last_it, c.mdf_total, c.mdf_head, c.K, nrest, nspread);
     c.Fpart = c.Fcur;
     c.Fcur = out;
+}
+
+// Workgroups of the correction loop's tail: one per CU, as long as one fits
+// next to nothing else (the launch follows the step's other kernels in
+// stream order, so the whole grid is resident and the grid barrier can
+// complete).  0: the tail is not available (the loop stays one launch pair per
+// iteration).  Sized outside any stream capture (sf_mdf_set_control).
+int mdf_tail_size(Ctx& c) {
+    if (c.mdf_tail_grid >= 0) return c.mdf_tail_grid;
+    int cus = 0, per = 0;
+    c.mdf_tail_grid = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess) return 0;
+    auto occ = [&](auto kern) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0); };
+    hipError_t e;
+    if (c.fp32) e = c.interp_lpn == 4 ? occ(k_mdf_tail<true, 4>) : c.interp_lpn == 8 ? occ(k_mdf_tail<true, 8>)
+                                                                                     : occ(k_mdf_tail<true, 16>);
+    else e = c.interp_lpn == 4 ? occ(k_mdf_tail<false, 4>) : c.interp_lpn == 8 ? occ(k_mdf_tail<false, 8>)
+                                                                               : occ(k_mdf_tail<false, 16>);
+    if (e == hipSuccess && per >= 1) c.mdf_tail_grid = cus;
+    return c.mdf_tail_grid;
+}
+
+bool mdf_tail_ready(const Ctx& c) { return c.mdf_bar && c.mdf_tail_grid > 0; }
+
+// MDF iterations first .. last in one launch (k_mdf_tail); rotates the force
+// roles as that many launch_mdf_apply calls do.
+void launch_mdf_tail(Ctx& c, int first, int last) {
+    const int out = 3 - c.Fcur - c.Fprev;
+    TailArgs t{};
+    t.a = vel_args(c, c.f[c.fcur], nullptr);
+    t.lut = c.phi; t.x = c.x; t.y = c.y; t.vx = c.vx; t.vy = c.vy;
+    t.rec = c.spread;
+    t.K = c.K;
+    t.nblk_interp = (c.K + interp_nodes(c.interp_lpn) - 1) / interp_nodes(c.interp_lpn);
+    t.tile_cnt = c.tile_cnt; t.tile_list = c.tile_list; t.act = c.tile_act; t.counters = c.counters;
+    t.FP = c.F[c.Fcur];
+    t.FQ = c.F[out];
+    t.dP = c.tile_dirty + static_cast<size_t>(c.Fcur) * c.ntiles;
+    t.dQ = c.tile_dirty + static_cast<size_t>(out) * c.ntiles;
+    t.nx = c.nx; t.ny = c.ny; t.ntx = c.ntx; t.ntiles = c.ntiles;
+    t.nrest = (c.ntiles + rest_tiles_per_block(1) - 1) / rest_tiles_per_block(1);
+    t.md = c.mdf;
+    t.tol = c.mdf_tol;
+    t.first = first;
+    t.last = last;
+    t.total = c.mdf_total;
+    t.head = c.mdf_head;
+    t.bar = c.mdf_bar;
+    t.spin = 1u << 20;
+    const dim3 g(c.mdf_tail_grid);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, dim3(256), 0, c.stream, t); };
+    if (c.fp32) {
+        if (c.interp_lpn == 4) go(k_mdf_tail<true, 4>);
+        else if (c.interp_lpn == 8) go(k_mdf_tail<true, 8>);
+        else go(k_mdf_tail<true, 16>);
+    } else {
+        if (c.interp_lpn == 4) go(k_mdf_tail<false, 4>);
+        else if (c.interp_lpn == 8) go(k_mdf_tail<false, 8>);
+        else go(k_mdf_tail<false, 16>);
+    }
+    for (int it = first; it <= last; ++it) {
+        const int o = 3 - c.Fcur - c.Fprev;
+        c.Fpart = c.Fcur;
+        c.Fcur = o;
+    }
 }
 
 // MDF iteration it (1-based; MDF_MAX_CORR + 1 at most, the last being the

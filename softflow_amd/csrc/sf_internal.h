@@ -394,6 +394,9 @@ struct Ctx {
     double mdf_tol = 0.0;
     int mdf_max_corr = 0;
     int mdf_probe = 0;
+    int mdf_tail_from = 6;               // residual mode: iterations from here on run in one launch (0: never)
+    int mdf_tail_grid = -1;              // its workgroups (-1: not sized yet, 0: unavailable)
+    unsigned* mdf_bar = nullptr;         // its grid barrier (count, generation)
     bool mdf_warm = false;
     MdfDev* mdf = nullptr;               // device
     double* mdf_total = nullptr;         // 2K: corrections spread this step, per node
@@ -510,6 +513,11 @@ void launch_mdf_fold(Ctx& c);
 // correction spread (touched tiles) and the untouched tiles' copies in one
 // launch; rotates the force roles as launch_spread mode 1 does
 void launch_mdf_apply(Ctx& c, int it, int last_it);
+// MDF iterations first .. last (residual mode) in one launch: the tail of the
+// correction loop, which leaves at once when the loop has stopped
+void launch_mdf_tail(Ctx& c, int first, int last);
+int mdf_tail_size(Ctx& c);
+bool mdf_tail_ready(const Ctx& c);
 void launch_mdf_warm(Ctx& c);
 std::vector<int> warm_groups(const int32_t* off, int ncaps);
 void launch_interp_move(Ctx& c);

@@ -22,10 +22,12 @@ CASE = dict(nx=80, ny=40, tau=0.8, bfx=3e-5, coll=2, mdf=1,
                            mem=SKALAK)])
 
 
-def test_residual_mode_matches_oracle_fp64():
-    print("residual-mode MDF parity", flush=True)
+@pytest.mark.parametrize("tail", [0, 3])
+def test_residual_mode_matches_oracle_fp64(tail):
+    print(f"residual-mode MDF parity, tail from {tail}", flush=True)
     o = oracle_sim(CASE)
     p = product_sim(CASE)
+    p.setMdfTail(tail)
     o.step(50)
     p.run(50)
     o.set_mdf_control(1e-300, 1, probe=True)
@@ -47,6 +49,44 @@ def test_residual_mode_matches_oracle_fp64():
     ref, got = oracle_state(o), product_state(p)
     for k in ("rho", "ux", "uy", "x", "y", "vx", "vy"):
         assert rel_l2(got[k], ref[k]) <= 1e-12, k
+
+
+def _tail_pair(precision, tol_of_r1, max_corr, steps):
+    case = dict(CASE, precision=precision)
+    sims = [product_sim(case), product_sim(case)]
+    for s, tail in zip(sims, (3, 0)):
+        s.setMdfTail(tail)
+        s.run(50)
+        s.setMdfControl(1e-300, 1, True)
+        s.run(1)
+    r1 = sims[0].mdfStats()["mean_residual"]
+    sims[1].mdfStats()
+    tol = 1e-300 if tol_of_r1 is None else tol_of_r1 * r1
+    for s in sims:
+        s.setMdfControl(tol, max_corr, True)
+        s.run(steps)
+    return sims
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_mdf_tail_matches_separate_launches(precision):
+    """The correction loop's tail (iterations 3.. in one launch with grid
+    barriers) against one launch pair per iteration, bit for bit: a tolerance
+    some steps reach after more than two corrections and others not within
+    ten, and an unreachable one (every iteration of the tail applied)."""
+    for tol_of_r1, max_corr in ((0.4, 10), (None, 7)):
+        print(f"mdf tail {precision}: tol {tol_of_r1} x r1, {max_corr} corrections", flush=True)
+        a, b = _tail_pair(precision, tol_of_r1, max_corr, 120)
+        sa, sb = a.mdfStats(), b.mdfStats()
+        print("tail", sa, "separate", sb, flush=True)
+        assert sa == sb
+        if tol_of_r1 is not None:
+            assert 2.0 < sa["mean_corrections"] < max_corr   # the tail both stops early and runs
+        else:
+            assert sa["mean_corrections"] == max_corr
+        ga, gb = product_state(a), product_state(b)
+        for k in ("rho", "ux", "uy", "fx", "fy", "x", "y", "vx", "vy"):
+            assert np.array_equal(np.asarray(ga[k]), np.asarray(gb[k])), (precision, tol_of_r1, k)
 
 
 def _random_net(seed):
