@@ -470,12 +470,15 @@ def geometry_bench(steps, cpu_steps, threads, with_cpu=True):
     same workload on the host cores (where oracle/_ref exists)."""
     case = pillar_case()
     sim = build_product(case, "fp32", graph=USE_GRAPH)
+    sim.run(10)
+    tune = sim.autotune(10).strip().splitlines()[-1].strip()   # as the 4096x1024 channel is tuned
     sim.run(20)
     dt, _ = timed_run(sim, steps)
     out = {"workload": "4096x1024 MRT tau=0.7 fx=1e-5 through 200 staggered circular pillars (r 24) "
                        "with interpolated bounce-back, 1000 Skalak+Helfrich capsules, ADR D=0.05; FP32",
            "pillars": len(case["obstacles"]), "capsules": sum(c["count"] for c in case["capsules"]),
-           "ms_per_step": round(dt * 1e3 / steps, 5), "mlups": round(NX * NY * steps / dt / 1e6, 1)}
+           "ms_per_step": round(dt * 1e3 / steps, 5), "mlups": round(NX * NY * steps / dt / 1e6, 1),
+           "lbm_launch": tune}
     del sim
     if with_cpu:
         try:

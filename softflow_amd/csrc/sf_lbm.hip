@@ -737,6 +737,32 @@ __device__ __forceinline__ int lbm_pair_types(const LbmArgs& a, int x, int y, in
     if (fl0) pair_collide<OP>(a, f0, n, co0, F[0], F[1], S[0], S[1], ux0, uy0);
     if (fl1) pair_collide<OP>(a, f1, n + 1, co1, F[2], F[3], S[2], S[3], ux1, uy1);
     const bool both = fl0 && fl1;
+    // the populations leave before the scalar's are loaded, so the two sets are
+    // never held at once
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        int yd = y + dcy(q);
+        bool oob = false;
+        if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
+        else oob = (yd < 0 || yd >= a.ny);
+        if (!oob) {
+            const bool b0 = mask_solid_nb(m0, q), b1 = mask_solid_nb(m1, q);
+            if (both && b0 == b1) {
+                pair_store_both(dst, N, n, x, a.nx, q, yd, b0, f0[q], f1[q]);
+            } else {
+                if (fl0) pair_store_one(dst, N, n, x, a.nx, q, yd, b0, f0[q]);
+                if (fl1) pair_store_one(dst, N, n + 1, x + 1, a.nx, q, yd, b1, f1[q]);
+            }
+        }
+        if (q != 0) {
+            const int ys = y - dcy(q);
+            if (!a.periodic_y && (ys < 0 || ys >= a.ny)) {
+                if (both) st_pair(dst + q * N + n, f0[q], f1[q]);
+                else if (fl0) st_nt(dst + q * N + n, f0[q]);
+                else st_nt(dst + q * N + n + 1, f1[q]);
+            }
+        }
+    }
     if constexpr (ADRF) {
         for (int s = 0; s < a.nsp; ++s) {
             const float* gs = static_cast<const float*>(a.gsrc[s]);
@@ -768,30 +794,6 @@ __device__ __forceinline__ int lbm_pair_types(const LbmArgs& a, int x, int y, in
             }
         }
     }
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        int yd = y + dcy(q);
-        bool oob = false;
-        if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
-        else oob = (yd < 0 || yd >= a.ny);
-        if (!oob) {
-            const bool b0 = mask_solid_nb(m0, q), b1 = mask_solid_nb(m1, q);
-            if (both && b0 == b1) {
-                pair_store_both(dst, N, n, x, a.nx, q, yd, b0, f0[q], f1[q]);
-            } else {
-                if (fl0) pair_store_one(dst, N, n, x, a.nx, q, yd, b0, f0[q]);
-                if (fl1) pair_store_one(dst, N, n + 1, x + 1, a.nx, q, yd, b1, f1[q]);
-            }
-        }
-        if (q != 0) {
-            const int ys = y - dcy(q);
-            if (!a.periodic_y && (ys < 0 || ys >= a.ny)) {
-                if (both) st_pair(dst + q * N + n, f0[q], f1[q]);
-                else if (fl0) st_nt(dst + q * N + n, f0[q]);
-                else st_nt(dst + q * N + n + 1, f1[q]);
-            }
-        }
-    }
     return capped;
 }
 
@@ -820,6 +822,36 @@ __device__ __forceinline__ int lbm_pair(const LbmArgs& a, int x, int y) {
     pair_collide<OP>(a, f0, n, true, F[0], F[1], S[0], S[1], ux0, uy0);
     pair_collide<OP>(a, f1, n + 1, true, F[2], F[3], S[2], S[3], ux1, uy1);
     const int xp[2] = {x, x + 1};
+    // push with halfway bounce-back (lbm_solver.cpp:212-235, bounce_back.cpp:12-40); the
+    // populations leave before the scalar's are loaded, so the two sets are
+    // never held at once
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        int yd = y + dcy(q);
+        bool oob = false;
+        if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
+        else oob = (yd < 0 || yd >= a.ny);
+        if (!oob) {
+            if (is_solid(a, 0, yd, false)) {
+                st_pair(dst + dopp(q) * N + n, f0[q], f1[q]);
+            } else if (dcx(q) == 0) {
+                st_pair(dst + q * N + (static_cast<size_t>(yd) * a.nx + x), f0[q], f1[q]);
+            } else {
+                const size_t row = static_cast<size_t>(yd) * a.nx;
+                const int xa = xp[0] + dcx(q);
+                if (xa >= 0 && xa + 1 < a.nx) {
+                    st_pair_u(dst + q * N + row + xa, f0[q], f1[q]);
+                } else {
+                    st_nt(dst + q * N + row + wrap_x(xp[0] + dcx(q), a.nx), f0[q]);
+                    st_nt(dst + q * N + row + wrap_x(xp[1] + dcx(q), a.nx), f1[q]);
+                }
+            }
+        }
+        if (q != 0) {
+            const int ys = y - dcy(q);
+            if (!a.periodic_y && (ys < 0 || ys >= a.ny)) st_pair(dst + q * N + n, f0[q], f1[q]);
+        }
+    }
     if constexpr (ADRF) {
         for (int s = 0; s < a.nsp; ++s) {
             const float* gs = static_cast<const float*>(a.gsrc[s]);
@@ -860,34 +892,6 @@ __device__ __forceinline__ int lbm_pair(const LbmArgs& a, int x, int y) {
             }
         }
     }
-    // push with halfway bounce-back (lbm_solver.cpp:212-235, bounce_back.cpp:12-40)
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        int yd = y + dcy(q);
-        bool oob = false;
-        if (a.periodic_y) yd = yd < 0 ? yd + a.ny : (yd >= a.ny ? yd - a.ny : yd);
-        else oob = (yd < 0 || yd >= a.ny);
-        if (!oob) {
-            if (is_solid(a, 0, yd, false)) {
-                st_pair(dst + dopp(q) * N + n, f0[q], f1[q]);
-            } else if (dcx(q) == 0) {
-                st_pair(dst + q * N + (static_cast<size_t>(yd) * a.nx + x), f0[q], f1[q]);
-            } else {
-                const size_t row = static_cast<size_t>(yd) * a.nx;
-                const int xa = xp[0] + dcx(q);
-                if (xa >= 0 && xa + 1 < a.nx) {
-                    st_pair_u(dst + q * N + row + xa, f0[q], f1[q]);
-                } else {
-                    st_nt(dst + q * N + row + wrap_x(xp[0] + dcx(q), a.nx), f0[q]);
-                    st_nt(dst + q * N + row + wrap_x(xp[1] + dcx(q), a.nx), f1[q]);
-                }
-            }
-        }
-        if (q != 0) {
-            const int ys = y - dcy(q);
-            if (!a.periodic_y && (ys < 0 || ys >= a.ny)) st_pair(dst + q * N + n, f0[q], f1[q]);
-        }
-    }
     return capped;
 }
 
@@ -915,17 +919,17 @@ __device__ __forceinline__ void lbm_pair_grid(const LbmArgs& a) {
         atomicAdd(&a.counters[0], capped);
 }
 
-// Channels: four waves per SIMD.  The MRT form with the scalar would otherwise
-// take 137 VGPRs (three waves); at 128 it spills 32 bytes per lane and still
-// runs faster, the extra wave keeping more bytes in flight.
+// Channels: at least four waves per SIMD.  With the populations stored before
+// the scalar's are loaded, the MRT form with the scalar holds 88 VGPRs (five
+// waves, no spill); held to six waves it spills and runs slower.
 template <int OP, bool ADRF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_lbm_pair(LbmArgs a) {
     lbm_pair_grid<OP, ADRF, false>(a);
 }
 
 // Generic cell types: not held to four waves.  With that bound the MRT form
-// with the scalar would spill 200 bytes per lane, so the allocator keeps its
-// own count (177 VGPRs, two waves per SIMD).
+// with the scalar spills 96 bytes per lane and runs slower, so the allocator
+// keeps its own count (154 VGPRs, three waves per SIMD).
 template <int OP, bool ADRF>
 __global__ __launch_bounds__(256) void k_lbm_pair_types(LbmArgs a) {
     lbm_pair_grid<OP, ADRF, true>(a);
