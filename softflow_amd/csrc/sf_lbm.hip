@@ -419,6 +419,7 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
 #pragma unroll
             for (int q = 0; q < 9; ++q) f[q] = ld_nt(src + q * N + n);
             double rho_wall = 0.0;    // the density a moving wall reflects with
+            double adr_ux = 0.0, adr_uy = 0.0;   // the scalar step's velocity (after the push)
             if constexpr (!FP32) {
                 double r = 0.0, vx = 0.0, vy = 0.0;
 #pragma unroll
@@ -444,7 +445,8 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                     uy = 0.0;
                 }
                 rho_wall = r;
-                if constexpr (ADRF) adr_cell<false, FLAGS>(a, x, y, n, ux, uy, m);
+                adr_ux = ux;
+                adr_uy = uy;
                 if (!collide) {
                 } else if (OP == SF_REGULARIZED) col_reg(f, a.om, r, ux, uy, Fx, Fy);
                 else if (OP == SF_MRT) col_mrt(f, a.om, a.se, a.seps, a.sq, r, ux, uy, Fx, Fy);
@@ -468,7 +470,8 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                     uy = (jy + 0.5f * sy) * inv;
                 }
                 rho_wall = static_cast<double>(rho);
-                if constexpr (ADRF) adr_cell<true, FLAGS>(a, x, y, n, static_cast<double>(ux), static_cast<double>(uy), m);
+                adr_ux = static_cast<double>(ux);
+                adr_uy = static_cast<double>(uy);
                 const float om = static_cast<float>(a.om);
                 if (!collide) {
                 } else if (OP == SF_REGULARIZED) dcol_reg(f, om, drho, rho, ux, uy, Fx, Fy);
@@ -511,6 +514,9 @@ __device__ __forceinline__ int lbm_cell(const LbmArgs& a, int x, int y) {
                     if (soob) st_nt(dst + q * N + n, f[q]);
                 }
             }
+            // the scalar step after the populations have left, so the two sets
+            // are never held at once
+            if constexpr (ADRF) adr_cell<FP32, FLAGS>(a, x, y, n, adr_ux, adr_uy, m);
         }
     }
     return capped;
