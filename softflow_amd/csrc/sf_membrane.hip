@@ -534,17 +534,18 @@ __global__ void k_repulsion(RepArgs a) {
     repulsion_body<CELLS>(a, blockIdx.x);
 }
 
-// Cell-list contact with 16 lanes per node: lane u < 9 scans neighbour cell u
+// Cell-list contact with REP_LANES lanes per node: lane u < 9 scans neighbour cell u
 // (the reference's dy-outer, dx-inner order), lanes 9 and 10 take the two
 // wall images, all in parallel; every non-zero pair force is kept in LDS with
 // its place in that order, and the node's lane 0 adds them to the membrane
 // force in order -- the sequence of additions of the serial loop, minus its
 // exact zeros.  A lane with more than REP_SLOTS non-zero pairs makes lane 0
 // redo the node serially.
-constexpr int REP_NODES = 16;      // nodes per 256-thread workgroup
+constexpr int REP_LANES = 12;      // lanes per node: 9 cells, 2 wall images, one spare
+constexpr int REP_NODES = 256 / REP_LANES;   // nodes per 256-thread workgroup (21; 4 threads idle)
 constexpr int REP_SLOTS = 6;       // non-zero pair forces a lane keeps
 
-// Obstacles: the 16 lanes of a node screen the obstacles' holding discs (staged
+// Obstacles: the lanes of a node screen the obstacles' holding discs (staged
 // in LDS) in parallel and mark the ones within reach; the node's lane 0 then
 // adds their terms in list order.  Up to REP_OBS_LDS obstacles; beyond that
 // lane 0 screens them all itself.
@@ -556,7 +557,7 @@ constexpr int REP_OBS_LDS = 256;
 __device__ __forceinline__ void repulsion_lanes_sum(const RepArgs& a, int k, int g, int cap, double px, double py,
                                                     double vx, double vy, bool obs_lds,
                                                     double (*sfx)[11][REP_SLOTS], double (*sfy)[11][REP_SLOTS],
-                                                    int (*scnt)[16], float (*sdmin)[16],
+                                                    int (*scnt)[REP_LANES], float (*sdmin)[REP_LANES],
                                                     unsigned long long (*snear)[REP_OBS_LDS / 64], double fx,
                                                     double fy) {
     double Fx, Fy;
@@ -619,14 +620,15 @@ __device__ __forceinline__ void repulsion_lanes_sum(const RepArgs& a, int k, int
 template <bool OBS>
 __device__ __forceinline__ void repulsion_lanes_body(RepArgs a, int bx) {
     __shared__ double sfx[REP_NODES][11][REP_SLOTS], sfy[REP_NODES][11][REP_SLOTS];
-    __shared__ int scnt[REP_NODES][16];
-    __shared__ float sdmin[REP_NODES][16];
+    __shared__ int scnt[REP_NODES][REP_LANES];
+    __shared__ float sdmin[REP_NODES][REP_LANES];
     __shared__ double sbc[OBS ? REP_OBS_LDS : 1][3];
     __shared__ unsigned long long snear_lds[OBS ? REP_NODES : 1][REP_OBS_LDS / 64];
     unsigned long long (*snear)[REP_OBS_LDS / 64] = OBS ? snear_lds : nullptr;
-    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int g = static_cast<int>(threadIdx.x) / REP_LANES, l = static_cast<int>(threadIdx.x) % REP_LANES;
+    const bool grp = g < REP_NODES;          // the workgroup's last threads hold no node
     const int k = bx * REP_NODES + g;
-    const bool live = k < a.K;
+    const bool live = grp && k < a.K;
     int cap = 0;
     double px = 0.0, py = 0.0, vx = 0.0, vy = 0.0;
     double fx0 = 0.0, fy0 = 0.0;   // the membrane force, read by the node's lane 0 up front
@@ -639,10 +641,10 @@ __device__ __forceinline__ void repulsion_lanes_body(RepArgs a, int bx) {
             sbc[i][1] = a.obs[i].bc[1];
             sbc[i][2] = a.obs[i].bc[2];
         }
-        if (l < REP_OBS_LDS / 64) snear[g][l] = 0ull;
+        if (grp && l < REP_OBS_LDS / 64) snear[g][l] = 0ull;
         __syncthreads();
         if (live)
-            for (int o = l; o < a.nobs; o += 16) {
+            for (int o = l; o < a.nobs; o += REP_LANES) {
                 const double dx = px - sbc[o][0], dy = py - sbc[o][1];
                 const double lim = sbc[o][2] + a.rcut;
                 if (!(dx * dx + dy * dy > lim * lim * (1.0 + 1e-9)))   // obstacle_far, negated
@@ -682,8 +684,10 @@ __device__ __forceinline__ void repulsion_lanes_body(RepArgs a, int bx) {
         pair_force(a, px, py, vx, vy, px, l == 9 ? a.ybot : a.ytop, 0.0, 0.0, Fx, Fy);
         keep(Fx, Fy);
     }
-    scnt[g][l] = nz;
-    sdmin[g][l] = static_cast<float>(sqrt(d2min));
+    if (grp) {
+        scnt[g][l] = nz;
+        sdmin[g][l] = static_cast<float>(sqrt(d2min));
+    }
     __syncthreads();
     if (live && l == 0)
         repulsion_lanes_sum(a, k, g, cap, px, py, vx, vy, obs_lds, sfx, sfy, scnt, sdmin, snear, fx0, fy0);
