@@ -894,14 +894,45 @@ __global__ __launch_bounds__(256) void k_mdf_tail(TailArgs t) {
         if (dec) {
             // stopped: the decision holds for the remaining iterations, and
             // the last iteration's copy keeps both buffers of the ping-pong
-            // equal (spread_rest_body at last, as its own launch would run it)
+            // equal
             if (blockIdx.x == 0 && threadIdx.x == 0)
                 for (int j = it; j <= t.last; ++j) t.md->cby[j] = dec;
             const bool lodd = ((t.last - t.first) & 1) != 0;
-            for (int b = blockIdx.x; b < t.nrest; b += G)
-                spread_rest_body<FP32>(t.tile_cnt, lodd ? t.FQ : t.FP, lodd ? t.FP : t.FQ, lodd ? t.dQ : t.dP,
-                                       lodd ? t.dP : t.dQ, t.act, t.counters, t.nx, t.ny, t.ntx, t.ntiles, 1, t.md,
-                                       b, t.last, t.last, t.tol, dec);
+            void* Lin = lodd ? t.FQ : t.FP;
+            void* Lout = lodd ? t.FP : t.FQ;
+            uint8_t* Ldin = lodd ? t.dQ : t.dP;
+            uint8_t* Ldout = lodd ? t.dP : t.dQ;
+            if (dec >= 2) {
+                // A correction ran this step, so both buffers hold force in the
+                // touched tiles only (each correction copied or cleared every
+                // other tile): copying those tiles, one workgroup each, is the
+                // whole of the last iteration's copy.  The force is in the last
+                // iteration's input buffer when (last - dec) is even, else in
+                // its output buffer.
+                using T2 = typename std::conditional<FP32, float2, double2>::type;
+                const bool back = ((t.last - dec) & 1) != 0;
+                const T2* src = static_cast<const T2*>(back ? Lout : Lin);
+                T2* dst = static_cast<T2*>(back ? Lin : Lout);
+                uint8_t* ddst = back ? Ldin : Ldout;
+                const int nact = min(t.counters[3], t.ntiles);
+                for (int i = blockIdx.x; i < nact; i += G) {
+                    const int tile = t.act[i].x;
+                    const int cx = (tile % t.ntx) * TILE_X + static_cast<int>(threadIdx.x) % TILE_X;
+                    const int cy = (tile / t.ntx) * TILE_Y + static_cast<int>(threadIdx.x) / TILE_X;
+                    if (threadIdx.x < TILE_X * TILE_Y && cx < t.nx && cy < t.ny) {
+                        const size_t n = static_cast<size_t>(cy) * t.nx + cx;
+                        dst[n] = src[n];
+                    }
+                    if (threadIdx.x == 0) ddst[tile] = 1;
+                }
+            } else {
+                // no correction ran: the other buffer may hold an older step's
+                // force anywhere (spread_rest_body at last, as its own launch
+                // would run it)
+                for (int b = blockIdx.x; b < t.nrest; b += G)
+                    spread_rest_body<FP32>(t.tile_cnt, Lin, Lout, Ldin, Ldout, t.act, t.counters, t.nx, t.ny,
+                                           t.ntx, t.ntiles, 1, t.md, b, t.last, t.last, t.tol, dec);
+            }
             return;
         }
         CellVelArgs a = t.a;
